@@ -1,0 +1,131 @@
+"""Benchmark: particle-steps/s of the shower transport on one MI355X.
+
+One step = one ptg_run (collide() over every primary of one beam, all
+generations of secondaries) with the beam already resident in HBM. The
+workload is the focused-laser LCFA cascade: FastFocusedLaser (a0 = 10,
+waist 4 um, 30 fs FWHM, 0.8 um, linear) against 1e7 electrons of 16.5 GeV
+(5 um rms transverse, 20 um rms longitudinal), emission and Breit-Wheeler
+pair creation on, secondaries tracked, FP64.
+
+Prints one JSON line. `roofline` describes the dominant kernel (the
+electron/positron transport kernel): algorithmic FP64 flops per particle-step
+(counted per operation in the device code, see FLOPS_PER_LEPTON_STEP) times its
+steps, over its average launch time measured with HIP events on the stream it
+runs on. `cpu_baseline` times the CPU oracle (a scalar restatement of the
+reference loop, one xoshiro256** stream per thread as per MPI rank) on a
+bounded subsample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+# Algorithmic FP64 operations per particle-step of the FastFocusedLaser path,
+# counted from the device code with add/sub/mul/div/sqrt = 1, fma = 2,
+# exp/log = 20 and the carrier sin/cos pair = 50 (its polynomial). Derivation
+# in DESIGN.md ("Roofline accounting").
+FLOPS_PER_LEPTON_STEP = 1070.0
+FLOPS_PER_PHOTON_STEP = 560.0
+PEAK_FP64_TFLOPS = 78.6
+
+
+def cpu_baseline(n_primaries, threads):
+    """The oracle on `n_primaries` primaries of the same workload, `threads`
+    emulated ranks. Returns (steps_per_s, sample description)."""
+    from oracle import pyoracle as orc
+    from ptarmigan_amd import workloads
+    laser, beam, opts = workloads.focused_lcfa_cascade(n_primaries)
+    ld, od = laser.desc(), opts.desc()
+    z0 = orc.lib().orc_laser_ideal_initial_z(ld) + 3.0 * beam.length
+    prim = orc.build_beam(beam.desc(), z0, 0)
+    _, _, cnt, secs = orc.collide(ld, od, prim, 0, threads)
+    steps = cnt["lepton_steps"] + cnt["photon_steps"]
+    return steps / secs, f"{n_primaries} primaries of the same beam ({steps} particle-steps, {secs:.1f} s)"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--primaries", type=int, default=10_000_000)
+    ap.add_argument("--cpu-primaries", type=int, default=2000)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the transport shards by primaries, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from ptarmigan_amd import Engine, workloads
+    laser, beam, opts = workloads.focused_lcfa_cascade(args.primaries)
+    eng = Engine(0)
+    eng.set_laser(laser)
+    eng.build_beam(beam, seed=0)
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:
+        sync = lambda: None  # noqa: E731
+
+    for _ in range(args.warmup):
+        eng.run(opts, seed=0)
+    sync()
+    t0 = time.perf_counter()
+    totals = None
+    for k in range(args.steps):
+        cnt = eng.run(opts, seed=k + 1)
+        if totals is None:
+            totals = {key: 0 for key in cnt}
+        for key, v in cnt.items():
+            totals[key] += v
+    sync()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    steps = totals["lepton_steps"] + totals["photon_steps"]
+    value = steps / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+
+    lep_launch_ms = totals["lepton_ms"] / max(1, totals["lepton_launches"])
+    lep_steps_per_launch = totals["lepton_steps"] / max(1, totals["lepton_launches"])
+    achieved = FLOPS_PER_LEPTON_STEP * lep_steps_per_launch / (lep_launch_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "pipe": "fp64 vector (VALU); MI355X FP64 vector and matrix peaks are both 78.6",
+                "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                "kernel": "k_leptons_lcfa<FastFocusedLaser, linear>",
+                "kernel_avg_ms": round(lep_launch_ms, 3), "flops_per_step": FLOPS_PER_LEPTON_STEP}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        v, sample = cpu_baseline(args.cpu_primaries, threads)
+        cpu = {"value": v, "unit": "particle-steps/s", "cores": threads, "kind": "port", "sample": sample}
+
+    line = {
+        "metric": "particle-steps/s on 1 MI355X: surrogate+FP32 vs plain FP64; photon yield err",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "focused LCFA cascade: FastFocusedLaser a0=10 w0=4um 30fs 0.8um LP, "
+                               "16.5 GeV e- beam 5um x 20um, emission + Breit-Wheeler, secondaries tracked, FP64",
+                   "primaries": args.primaries, "precision": "fp64"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "detail": {"lepton_steps": totals["lepton_steps"], "photon_steps": totals["photon_steps"],
+                   "emissions": totals["emissions"], "pair_events": totals["pair_events"],
+                   "final_state_per_step": totals["n_secondaries"] / args.steps,
+                   "generations_per_step": totals["generations"] / args.steps,
+                   "kernel_ms_per_step": totals["kernel_ms"] / args.steps,
+                   "error_flags": totals["error_flags"]},
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

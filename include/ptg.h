@@ -1,0 +1,248 @@
+/*
+ * ptg.h -- C ABI of the MI355X particle-shower engine (libptg.so).
+ *
+ * Drop-in boundary for Ptarmigan's per-particle Monte Carlo loop
+ * (tgblackburn/ptarmigan v1.5.0). The reference has no FFI: the path sits behind
+ *   fn collide<F: Field, R: Rng>(field: &F, incident: Particle, rng: &mut R,
+ *                                current_id: &mut u64, options: CollideOptions) -> Shower
+ * (src/main.rs:134) and the `Field` trait (src/field/mod.rs:126-192), driven by
+ * the chunked map in src/main.rs:1118-1135. Each entry point below names the
+ * reference interface it replaces.
+ *
+ * Conventions
+ *  - Plain C types only. Host buffers are owned by the caller; device buffers,
+ *    tables and results are owned by the context until the next ptg_run/ptg_collide.
+ *  - Four-vectors are n x 4 row-major doubles, as the reference's FourVector
+ *    arrays ((ct, x, y, z) in metres; (gamma, u_x, u_y, u_z) normalised momentum;
+ *    Stokes (I, Q, U, V)).
+ *  - Every call returns PTG_OK (0) or a negative PTG_E_* code and never aborts;
+ *    ptg_last_error() gives the message. Device-side consistency failures (the
+ *    reference's assert!/panic! sites) are accumulated in ptg_counters.error_flags.
+ *  - One context per host thread. Work runs on the context's HIP stream; calls
+ *    return when their results are complete.
+ */
+#ifndef PTG_H
+#define PTG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PTG_OK = 0,
+    PTG_E_ARG = -1,      /* invalid argument (null pointer, bad enum, inconsistent options) */
+    PTG_E_TABLES = -2,   /* table blob missing or malformed */
+    PTG_E_HIP = -3,      /* HIP runtime error */
+    PTG_E_OOM = -4,      /* device allocation failed */
+    PTG_E_DOMAIN = -5,   /* parameters outside the tabulated domain (reference: prints and returns None) */
+    PTG_E_UNSUPPORTED = -6, /* feature not built into this library (stated in the message) */
+    PTG_E_SANITY = -7,   /* a reference assert! would have fired (see error_flags) */
+    PTG_E_STATE = -8     /* call out of order (e.g. ptg_run before ptg_set_laser) */
+};
+
+/* Device error-flag bits (ptg_counters.error_flags) */
+enum {
+    PTG_FLAG_REJECT_CAP = 1u << 0,   /* a rejection loop hit its trial cap and used the fallback */
+    PTG_FLAG_TABLE_RANGE = 1u << 1,  /* a table lookup fell outside its domain (probability set to 0) */
+    PTG_FLAG_SANITY = 1u << 2,       /* photon/pair construction failed the reference's sanity check */
+    PTG_FLAG_STEP_CAP = 1u << 3      /* a particle hit the per-particle step cap */
+};
+
+typedef enum { PTG_ELECTRON = 0, PTG_POSITRON = 1, PTG_PHOTON = 2 } ptg_species;   /* particle/mod.rs:35-40 */
+
+/* src/field/mod.rs:117-123 (enum Laser) */
+typedef enum { PTG_PLANE_WAVE = 0, PTG_FOCUSED_LASER = 1, PTG_FAST_PLANE_WAVE = 2, PTG_FAST_FOCUSED_LASER = 3 } ptg_laser_kind;
+typedef enum { PTG_COS_SQUARED = 0, PTG_FLATTOP = 1, PTG_GAUSSIAN = 2 } ptg_envelope;       /* field/mod.rs:43-48 */
+typedef enum { PTG_LINEAR = 0, PTG_CIRCULAR = 1 } ptg_polarization;                        /* field/mod.rs:24-28 */
+
+/* Mirrors the laser construction in src/main.rs:1042-1060. n_cycles is the
+ * reference's n_cycles (for gaussian envelopes: c * fwhm_duration / wavelength). */
+typedef struct {
+    int32_t kind;            /* ptg_laser_kind */
+    int32_t envelope;        /* ptg_envelope */
+    int32_t polarization;    /* ptg_polarization */
+    int32_t finite_bandwidth;/* control:bandwidth_correction (LMA fields only) */
+    double a0;
+    double wavelength;       /* m */
+    double waist;            /* m (focused kinds only) */
+    double n_cycles;
+    double pol_angle;        /* rad, linear polarisation only */
+    double chirp_b;          /* plane waves only */
+} ptg_laser_desc;
+
+/* Precision of the per-step arithmetic (mixed mode keeps phase accumulation,
+ * momentum update and cumulative probability in FP64). */
+typedef enum { PTG_PREC_FP64 = 0, PTG_PREC_MIXED = 1 } ptg_precision;
+
+/* Field-for-field mirror of CollideOptions (src/main.rs:72-105). */
+typedef struct {
+    double dt_multiplier;
+    double rate_increase;
+    double t_stop;           /* s; +inf for none */
+    double min_energy;       /* MeV */
+    double max_angle;        /* rad */
+    int64_t multiplicity;    /* -1 = None */
+    int32_t discard_bg_e;
+    int32_t discard_bg_ph;
+    int32_t rr;
+    int32_t tracking_photons;
+    int32_t tracking_secondaries;
+    int32_t keep_decayed_photons;
+    int32_t pol_resolved;
+    int32_t rotate_stokes_pars;
+    int32_t classical;
+    int32_t gaunt_factor;
+    int32_t precision;       /* ptg_precision */
+    int32_t reserved;
+} ptg_collide_options;
+
+/* Particle set, structure of arrays (particle/mod.rs:66-78). Any pointer except
+ * species, r and u may be NULL on input (defaults: unpolarised, weight 1, ids 0..n-1). */
+typedef struct {
+    uint8_t* species;          /* n */
+    double* r;                 /* n x 4 current position */
+    double* u;                 /* n x 4 current normalised momentum */
+    double* r0;                /* n x 4 position at creation */
+    double* u0;                /* n x 4 momentum at creation */
+    double* pol;               /* n x 4 Stokes parameters */
+    double* payload;           /* n  a_eff at creation */
+    double* parent_chi;        /* n */
+    double* interaction_count; /* n */
+    double* work;              /* n  absorbed energy / mc^2 */
+    double* weight;            /* n */
+    uint64_t* id;              /* n */
+    uint64_t* parent_id;       /* n */
+} ptg_particles;
+
+/* Counters for one ptg_run. One particle-step is one iteration of collide()'s
+ * electron/positron loop (src/main.rs:169) or photon loop (src/main.rs:214). */
+typedef struct {
+    uint64_t lepton_steps;
+    uint64_t photon_steps;
+    uint64_t emissions;
+    uint64_t pair_events;
+    uint64_t n_secondaries;     /* final-state particles held by the context */
+    uint64_t n_intermediates;   /* decayed photons kept (keep_decayed_photons) */
+    uint64_t generations;
+    uint64_t rejection_trials;
+    uint32_t error_flags;
+    uint32_t reserved;
+    double kernel_ms;           /* device time of all transport kernels (HIP events on the context stream) */
+    double lepton_ms;           /* ... of the electron/positron kernels */
+    double photon_ms;           /* ... of the photon kernels */
+    uint64_t lepton_launches;
+    uint64_t photon_launches;
+} ptg_counters;
+
+/* Beam description, mirroring BeamBuilder (src/particle/builder.rs:173-219) for
+ * the normal-gamma case (particle/dstr.rs:161-176). */
+typedef struct {
+    uint64_t n;
+    int32_t species;
+    int32_t radial;            /* 0 = normal, 1 = truncated normal, 2 = uniform (dstr.rs:26-51) */
+    double gamma, sigma;       /* mean Lorentz factor, standard deviation */
+    double energy_chirp;       /* rho (sign as in builder.rs:117-120) */
+    double radius, max_radius; /* sigma_xy (normal) or r_max (uniform); truncation radius */
+    double length;             /* sigma_z, m */
+    double rms_div;            /* rad */
+    double angle;              /* collision angle, rad */
+    double collision_plane_angle;
+    double offset[3];
+    double pol[4];             /* Stokes */
+    double weight;
+} ptg_beam_desc;
+
+/* Quantities a histogram can bin: the accessors of src/output/functions.rs:10-263
+ * (same names and definitions; momenta in MeV, lengths in m, angles in rad). */
+typedef enum {
+    PTG_Q_ANGLE_X = 0, PTG_Q_ANGLE_Y, PTG_Q_THETA, PTG_Q_POLAR_ANGLE, PTG_Q_PX, PTG_Q_PY, PTG_Q_PZ,
+    PTG_Q_P_PERP, PTG_Q_R_PERP, PTG_Q_R_X, PTG_Q_R_Y, PTG_Q_P_MINUS, PTG_Q_P_PLUS, PTG_Q_GAMMA,
+    PTG_Q_ENERGY, PTG_Q_UNIT, PTG_Q_X, PTG_Q_Y, PTG_Q_Z, PTG_Q_RADIUS, PTG_Q_BIRTH_A, PTG_Q_PARENT_CHI,
+    PTG_Q_N_INTER, PTG_Q_ABSORPTION, PTG_Q_S1, PTG_Q_S2, PTG_Q_S3,
+    PTG_Q_COUNT
+} ptg_quantity;
+
+/* Histogram weights (functions.rs:232-263). */
+typedef enum { PTG_W_NUMBER = 0, PTG_W_ENERGY = 1, PTG_W_HELICITY = 2 } ptg_weight_fn;
+
+/* 1-D weighted histogram of the resident final-state particles of one species,
+ * as Histogram::generate_1d (src/output/hgram.rs:209-323): bins span the global
+ * min..max of the (optionally log-scaled) quantity, the particle at exactly max
+ * is not binned, the total counts every weight. */
+typedef struct {
+    int32_t species;         /* ptg_species */
+    int32_t quantity;        /* ptg_quantity */
+    int32_t weight_fn;       /* ptg_weight_fn */
+    int32_t log_scale;       /* BinSpec::LogScaled: bin ln(quantity) */
+    uint32_t nbins;          /* BinSpec::FixedNumber */
+    int32_t fixed_range;     /* use min/max below instead of the data range */
+    double min, max;         /* in/out: range used (of ln(q) when log_scale) */
+} ptg_hist_spec;
+
+typedef struct ptg_ctx ptg_ctx;
+
+/* Creates a context on HIP device `device`, loading tables from the blob at
+ * `table_path` (NULL = the blob installed next to libptg.so). */
+int ptg_create(ptg_ctx** out, const char* table_path, int device);
+void ptg_destroy(ptg_ctx* ctx);
+const char* ptg_last_error(const ptg_ctx* ctx);
+
+/* Selects the field: replaces the laser construction at src/main.rs:1042-1060. */
+int ptg_set_laser(ptg_ctx* ctx, const ptg_laser_desc* laser);
+/* Field::max_timestep / ideal_initial_z (src/field/mod.rs:130,138). */
+int ptg_field_info(ptg_ctx* ctx, double* max_timestep, double* ideal_initial_z);
+
+/* Generates `beam->n` primaries on the device: replaces BeamBuilder::build
+ * (src/particle/builder.rs:173-219). ids are 0..n-1. */
+int ptg_build_beam(ptg_ctx* ctx, const ptg_beam_desc* beam, double initial_z, uint64_t seed);
+/* Uploads host primaries to the device (replaces the Vec<Particle> handed to collide). */
+int ptg_load_primaries(ptg_ctx* ctx, const ptg_particles* primaries, size_t n);
+
+/* Runs collide() over every resident primary: replaces the chunked map at
+ * src/main.rs:1118-1135 and collide() itself (src/main.rs:134-289). Primaries
+ * must already be resident (ptg_load_primaries / ptg_build_beam). Results stay on
+ * the device until ptg_fetch. `seed` is control:rng_seed. */
+int ptg_run(ptg_ctx* ctx, const ptg_collide_options* opt, uint64_t seed, ptg_counters* counters);
+/* ptg_load_primaries + ptg_run. */
+int ptg_collide(ptg_ctx* ctx, const ptg_particles* primaries, size_t n, const ptg_collide_options* opt,
+                uint64_t seed, ptg_counters* counters);
+/* Copies results to caller buffers sized from counters (n_secondaries / n_intermediates).
+ * Either output may be NULL. Replaces the Shower/ParticleBunch fields (particle/shower.rs:9-46). */
+int ptg_fetch(ptg_ctx* ctx, ptg_particles* secondaries, ptg_particles* intermediates);
+
+/* Field-only push of n particles for up to n_steps steps each (stops when a particle
+ * leaves the field): repeated Field::push (src/field/mod.rs:147-152 and the LMA
+ * overrides, e.g. plane_wave.rs:196-249). r, u are n x 4, updated in place (host). */
+int ptg_push(ptg_ctx* ctx, const uint8_t* species, double* r, double* u, size_t n, uint32_t n_steps,
+             const ptg_collide_options* opt, uint32_t* steps_taken);
+
+/* Weighted histogram of the resident final-state particles (hgram.rs:209-323).
+ * bins_out has nbins entries (min/max written back when computed). */
+int ptg_histogram(ptg_ctx* ctx, ptg_hist_spec* spec, double* bins_out, double* total_weight);
+
+/* Batched evaluation of one device function for parity testing. fn ids are
+ * listed in ptg_eval_fn; in is n x in_w, out is n x out_w (host buffers). */
+typedef enum {
+    PTG_FN_LCFA_RATE = 0,        /* (chi, gamma) -> rate      lcfa/photon_emission/mod.rs:13 */
+    PTG_FN_LCFA_SAMPLE = 1,      /* (chi, gamma, r1, r2, r3) -> (omega, theta_ok, theta, phi)  :80 */
+    PTG_FN_AUX_T = 2,            /* chi -> (T_par, T_perp)   lcfa/pair_creation/mod.rs:13 */
+    PTG_FN_AIRY = 3,             /* x -> (Ai, Ai', K1/3, K2/3) special_functions/airy.rs:41-136 */
+    PTG_FN_FIELDS = 4,           /* (ct, x, y, z) -> (E[3], B[3], a) current laser */
+    PTG_FN_GAUNT = 5,            /* chi -> g(chi)  lcfa/photon_emission/mod.rs:330 */
+    PTG_FN_PAIR_SPECTRUM = 6,    /* (s, chi, sv1) -> spectrum  lcfa/pair_creation/mod.rs:80 */
+    PTG_FN_LCFA_STOKES = 7,      /* (k[4], chi, gamma, v[3], w[3]) -> Stokes[4]  mod.rs:239 */
+    PTG_FN_LP_RATE = 8,          /* (a, eta) -> rate  nonlinear_compton/lp/mod.rs:430 */
+    PTG_FN_LP_HARMONIC = 9,      /* (a, eta, r) -> n  nonlinear_compton/lp/mod.rs:478 */
+    PTG_FN_DOUBLE_BESSEL = 10,   /* (n, x, y) -> J(n,x,y)  special_functions/double_bessel.rs */
+    PTG_FN_LP_SAMPLE = 11,       /* (a, eta, seed) -> (n, s, cphi, stokes...) nonlinear_compton/lp/mod.rs:586 */
+    PTG_FN_COUNT = 12
+} ptg_eval_fn;
+int ptg_eval(ptg_ctx* ctx, int fn, const double* in, size_t n, size_t in_w, double* out, size_t out_w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PTG_H */

@@ -1,0 +1,763 @@
+// Host side of libptg: context, device memory, the generation loop that
+// replaces the per-rank collide() map (src/main.rs:1118-1135), and the C ABI
+// declared in include/ptg.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <dlfcn.h>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "kern_misc.h"
+
+using namespace ptg;
+
+namespace {
+
+constexpr double PI_H = 3.14159265358979323846;
+
+// Device particle buffer with capacity management.
+struct DevBuf {
+    void* base = nullptr;
+    size_t cap = 0;
+    PBuf v{};
+};
+
+hipError_t alloc_buf(DevBuf& b, size_t cap) {
+    if (cap < 64) cap = 64;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, cap * PBUF_BYTES_PER);
+    if (e != hipSuccess) return e;
+    char* c = static_cast<char*>(p);
+    PBuf v;
+    for (int k = 0; k < P_NDOUBLE; ++k) { v.d[k] = reinterpret_cast<double*>(c); c += cap * 8; }
+    v.id = reinterpret_cast<uint64_t*>(c); c += cap * 8;
+    v.parent_id = reinterpret_cast<uint64_t*>(c); c += cap * 8;
+    v.stream = reinterpret_cast<uint64_t*>(c); c += cap * 8;
+    v.shower = reinterpret_cast<uint32_t*>(c); c += cap * 4;
+    v.meta = reinterpret_cast<uint32_t*>(c); c += cap * 4;
+    v.nev = reinterpret_cast<uint32_t*>(c);
+    if (b.base) (void)hipFree(b.base);
+    b.base = p;
+    b.cap = cap;
+    b.v = v;
+    return hipSuccess;
+}
+
+// Grow keeping the first `keep` records.
+hipError_t grow_buf(DevBuf& b, size_t cap, size_t keep, hipStream_t s) {
+    if (cap <= b.cap) return hipSuccess;
+    DevBuf nb;
+    hipError_t e = alloc_buf(nb, cap);
+    if (e != hipSuccess) return e;
+    if (b.base && keep) {
+        for (int k = 0; k < P_NDOUBLE; ++k) (void)hipMemcpyAsync(nb.v.d[k], b.v.d[k], keep * 8, hipMemcpyDeviceToDevice, s);
+        (void)hipMemcpyAsync(nb.v.id, b.v.id, keep * 8, hipMemcpyDeviceToDevice, s);
+        (void)hipMemcpyAsync(nb.v.parent_id, b.v.parent_id, keep * 8, hipMemcpyDeviceToDevice, s);
+        (void)hipMemcpyAsync(nb.v.stream, b.v.stream, keep * 8, hipMemcpyDeviceToDevice, s);
+        (void)hipMemcpyAsync(nb.v.shower, b.v.shower, keep * 4, hipMemcpyDeviceToDevice, s);
+        (void)hipMemcpyAsync(nb.v.meta, b.v.meta, keep * 4, hipMemcpyDeviceToDevice, s);
+        (void)hipMemcpyAsync(nb.v.nev, b.v.nev, keep * 4, hipMemcpyDeviceToDevice, s);
+        (void)hipStreamSynchronize(s);
+    }
+    if (b.base) (void)hipFree(b.base);
+    b = nb;
+    return hipSuccess;
+}
+
+void free_buf(DevBuf& b) {
+    if (b.base) (void)hipFree(b.base);
+    b = DevBuf{};
+}
+
+// Counter block layout (unsigned long long)
+enum { C_WORK = 0, C_NEXT, C_OUT, C_INTER, C_ID, C_FLAGS, C_STATS, C_N = C_STATS + STAT_N };
+
+std::string default_table_path() {
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void*>(&default_table_path), &info) && info.dli_fname) {
+        std::string p(info.dli_fname);
+        size_t k = p.rfind('/');
+        std::string dir = k == std::string::npos ? "." : p.substr(0, k);
+        return dir + "/tables/ptg_tables.bin";
+    }
+    return "tables/ptg_tables.bin";
+}
+
+}  // namespace
+
+struct ptg_ctx {
+    int device = 0;
+    int n_cu = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::map<std::string, std::vector<double>> tables;
+    double* d_lcfa = nullptr;
+    LcfaScalars sc{};
+    LaserDev laser{};
+    bool have_laser = false;
+    DevBuf prim, q[2], out, inter, tmp;
+    double* d_n0 = nullptr;
+    size_t n0_cap = 0;
+    size_t n_prim = 0;
+    uint32_t* d_shower_count = nullptr;
+    size_t shower_cap = 0;
+    unsigned long long* d_ctr = nullptr;
+    TransportArgs* d_args = nullptr;
+    uint64_t n_out = 0, n_inter = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+int fail(ptg_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+#define HIPCHK(ctx, expr)                                                                         \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(ctx, e_ == hipErrorOutOfMemory ? PTG_E_OOM : PTG_E_HIP,                   \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                       \
+    } while (0)
+
+bool load_blob(const std::string& path, std::map<std::string, std::vector<double>>& out, std::string& err) {
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) { err = "cannot open table blob " + path; return false; }
+    std::fseek(f, 0, SEEK_END);
+    long len = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    std::vector<char> raw(len > 0 ? len : 0);
+    bool ok = len > 16 && std::fread(raw.data(), 1, len, f) == size_t(len);
+    std::fclose(f);
+    if (!ok || std::memcmp(raw.data(), "PTGTBL01", 8) != 0) { err = "malformed table blob " + path; return false; }
+    uint32_t n;
+    std::memcpy(&n, raw.data() + 8, 4);
+    const size_t rec = 96 + 8 + 8 + 4 + 16 + 4;
+    size_t base = 16 + size_t(n) * rec;
+    for (uint32_t i = 0; i < n; ++i) {
+        const char* r = raw.data() + 16 + i * rec;
+        char name[97] = {0};
+        std::memcpy(name, r, 96);
+        uint64_t off, cnt;
+        std::memcpy(&off, r + 96, 8);
+        std::memcpy(&cnt, r + 104, 8);
+        if (base + (off + cnt) * 8 > size_t(len)) { err = "table blob truncated"; return false; }
+        std::vector<double> v(cnt);
+        std::memcpy(v.data(), raw.data() + base + off * 8, cnt * 8);
+        out[name] = std::move(v);
+    }
+    return true;
+}
+
+bool build_lcfa_pack(ptg_ctx* c, std::vector<double>& pack) {
+    using namespace lcfa_pack;
+    pack.assign(SIZE, 0.0);
+    auto put = [&](int off, const char* name, size_t expect) -> bool {
+        auto it = c->tables.find(name);
+        if (it == c->tables.end() || it->second.size() != expect) {
+            c->err = std::string("table ") + name + " missing or wrong size";
+            return false;
+        }
+        std::copy(it->second.begin(), it->second.end(), pack.begin() + off);
+        return true;
+    };
+    const char* PE = "lcfa/photon_emission/tables:";
+    std::string pe(PE);
+    bool ok = put(LN_H_CHI, (pe + "LN_H_CHI_TABLE").c_str(), 82) && put(LN_T_CHI, "lcfa/pair_creation/tables:LN_T_CHI_TABLE", 243) &&
+              put(QUANTUM_CDF, (pe + "QUANTUM_CDF").c_str(), 41 * 64) && put(Y_CDF, (pe + "Y_CDF").c_str(), 31 * 64) &&
+              put(Y_INF, (pe + "Y_INFINITE_DELTA_CDF").c_str(), 64) && put(HIGH_CHI, (pe + "HIGH_CHI_CDF").c_str(), 64) &&
+              put(HIGH_CHI_PEAK, (pe + "HIGH_CHI_PEAK_CDF").c_str(), 64) &&
+              put(CLASSICAL, (pe + "CLASSICAL_SPECTRUM_TABLE").c_str(), 82) &&
+              put(GAUNT, "lcfa/photon_emission/mod:GAUNT_FACTOR_TABLE", 50) && put(GL_NODES, "quadrature/tables:GL_NODES", 32) &&
+              put(GL_WEIGHTS, "quadrature/tables:GL_WEIGHTS", 32) &&
+              put(AIRY_SERIES, "special_functions/airy:AIRY_SMALL_X", 40) &&
+              put(AIRYP_SERIES, "special_functions/airy:AIRY_PRIME_SMALL_X", 40) &&
+              put(AIRY_N40, "special_functions/airy:GL_m1_6_NODES_40", 40) && put(AIRY_W40, "special_functions/airy:GL_m1_6_WEIGHTS_40", 40) &&
+              put(AIRY_N16, "special_functions/airy:GL_m1_6_NODES_16", 16) && put(AIRY_W16, "special_functions/airy:GL_m1_6_WEIGHTS_16", 16) &&
+              put(AIRY_N4, "special_functions/airy:GL_m1_6_NODES_4", 4) && put(AIRY_W4, "special_functions/airy:GL_m1_6_WEIGHTS_4", 4) &&
+              put(AIRYP_N32, "special_functions/airy:GL_p1_6_NODES_32", 32) && put(AIRYP_W32, "special_functions/airy:GL_p1_6_WEIGHTS_32", 32) &&
+              put(AIRYP_N16, "special_functions/airy:GL_p1_6_NODES_16", 16) && put(AIRYP_W16, "special_functions/airy:GL_p1_6_WEIGHTS_16", 16) &&
+              put(AIRYP_N8, "special_functions/airy:GL_p1_6_NODES_8", 8) && put(AIRYP_W8, "special_functions/airy:GL_p1_6_WEIGHTS_8", 8) &&
+              put(AIRYP_N4, "special_functions/airy:GL_p1_6_NODES_4", 4) && put(AIRYP_W4, "special_functions/airy:GL_p1_6_WEIGHTS_4", 4);
+    if (!ok) return false;
+    // node-only factors of the pair spectrum quadrature (lcfa/pair_creation/mod.rs:80-89)
+    for (int k = 0; k < 32; ++k) {
+        double t = pack[GL_NODES + k], w = pack[GL_WEIGHTS + k];
+        double* q = pack.data() + PAIR_QN + 5 * k;
+        q[0] = t;
+        q[1] = w;
+        q[2] = std::cosh(t);
+        q[3] = std::cosh(2.0 * t / 3.0);
+        q[4] = std::cosh(t / 3.0) / std::cosh(t);
+    }
+    auto sc1 = [&](const char* name) { return c->tables[name][0]; };
+    c->sc.delta_ln_chi_h = sc1("lcfa/photon_emission/tables:DELTA_LN_CHI");
+    c->sc.ln_chi_min = sc1("lcfa/photon_emission/tables:LN_CHI_MIN");
+    c->sc.ln_chi_step = sc1("lcfa/photon_emission/tables:LN_CHI_STEP");
+    c->sc.ln_delta_min = sc1("lcfa/photon_emission/tables:LN_DELTA_MIN");
+    c->sc.ln_delta_step = sc1("lcfa/photon_emission/tables:LN_DELTA_STEP");
+    c->sc.delta_ln_chi_t = sc1("lcfa/pair_creation/tables:DELTA_LN_CHI");
+    return true;
+}
+
+// Laser constants (fast_focused_laser.rs:24-38,47-53,129-169; fast_plane_wave.rs:23-108).
+int make_laser(ptg_ctx* c, const ptg_laser_desc* d, LaserDev& L) {
+    if (d->kind < 0 || d->kind > 3) return fail(c, PTG_E_ARG, "laser kind out of range");
+    if (d->envelope < 0 || d->envelope > 2) return fail(c, PTG_E_ARG, "envelope out of range");
+    if (!(d->a0 > 0.0) || !(d->wavelength > 0.0) || !(d->n_cycles > 0.0)) return fail(c, PTG_E_ARG, "a0, wavelength and n_cycles must be positive");
+    if ((d->kind == PTG_FOCUSED_LASER || d->kind == PTG_FAST_FOCUSED_LASER) && !(d->waist > 0.0))
+        return fail(c, PTG_E_ARG, "focused lasers need a positive waist");
+    if (d->kind == PTG_PLANE_WAVE || d->kind == PTG_FOCUSED_LASER)
+        return fail(c, PTG_E_UNSUPPORTED, "LMA fields (PlaneWave, FocusedLaser) are not built into this library yet");
+    L = LaserDev{};
+    L.kind = d->kind;
+    L.envelope = d->envelope;
+    L.circular = d->polarization == PTG_CIRCULAR;
+    L.finite_bandwidth = d->finite_bandwidth;
+    L.a0 = d->a0;
+    L.k0 = 2.0 * PI_H / d->wavelength;
+    L.omega = C_LIGHT * L.k0;
+    double pol_angle = L.circular ? 0.0 : d->pol_angle;
+    L.sin_pa = std::sin(pol_angle);
+    L.cos_pa = std::cos(pol_angle);
+    L.chirp_b = d->chirp_b;
+    L.a_norm = E_CHARGE / (M_ELECTRON * C_LIGHT * L.omega);
+    double multiplier = std::fmin(std::sqrt(3.0) / (5.0 * ALPHA * d->a0), 0.1);
+    if (d->kind == PTG_FAST_FOCUSED_LASER) {
+        double duration = d->n_cycles * d->wavelength / C_LIGHT;
+        L.n_cycles = C_LIGHT * duration * L.k0 / (2.0 * PI_H);
+        double zr = 0.5 * L.k0 * d->waist * d->waist;
+        L.inv_waist = 1.0 / d->waist;
+        L.inv_zr = 1.0 / zr;
+        L.eps = d->waist / zr;
+        L.E0 = M_ELECTRON * C_LIGHT * L.omega * d->a0 / E_CHARGE;
+        L.B0 = L.E0 / C_LIGHT;
+        double tau = L.omega * duration;
+        L.gauss_c = 2.0 * LN2_D / (tau * tau);
+        L.dt = multiplier / L.omega;
+    } else {
+        L.n_cycles = d->n_cycles;
+        L.amplitude = (M_ELECTRON * C_LIGHT_SQD * L.k0 * d->a0) / E_CHARGE;
+        double chirp = 1.0 + 2.0 * d->chirp_b * PI_H * d->n_cycles;
+        L.dt = multiplier / (C_LIGHT * L.k0 * chirp);
+    }
+    double nc = L.n_cycles;
+    L.max_phase = d->envelope == PTG_COS_SQUARED ? PI_H * nc : (d->envelope == PTG_FLATTOP ? PI_H * (nc + 1.0) : 6.0 * PI_H * nc);
+    return PTG_OK;
+}
+
+double ideal_initial_z(const ptg_laser_desc* d, const LaserDev& L) {
+    double wl = 2.0 * PI_H / L.k0;
+    double nc = L.n_cycles;
+    return d->envelope == PTG_COS_SQUARED ? 0.5 * wl * nc : (d->envelope == PTG_FLATTOP ? 0.5 * wl * (nc + 1.0) : 2.0 * wl * nc);
+}
+
+int ensure_n0(ptg_ctx* c, size_t n) {
+    if (n <= c->n0_cap) return PTG_OK;
+    if (c->d_n0) (void)hipFree(c->d_n0);
+    c->d_n0 = nullptr;
+    HIPCHK(c, hipMalloc(&c->d_n0, 3 * n * sizeof(double)));
+    c->n0_cap = n;
+    return PTG_OK;
+}
+
+uint64_t key_from_seed(uint64_t seed) { return mix64(0x8658b90036b165ebULL + seed * 0x32f55cddaebae910ULL); }
+
+}  // namespace
+
+extern "C" {
+
+int ptg_create(ptg_ctx** out, const char* table_path, int device) {
+    if (!out) return PTG_E_ARG;
+    *out = nullptr;
+    ptg_ctx* c = new ptg_ctx;
+    c->device = device;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
+        c->err = "no HIP device " + std::to_string(device);
+        *out = c;
+        return PTG_E_HIP;
+    }
+    HIPCHK(c, hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCHK(c, hipGetDeviceProperties(&prop, device));
+    c->n_cu = prop.multiProcessorCount;
+    std::string path = table_path ? table_path : default_table_path();
+    if (!load_blob(path, c->tables, c->err)) { *out = c; return PTG_E_TABLES; }
+    std::vector<double> pack;
+    if (!build_lcfa_pack(c, pack)) { *out = c; return PTG_E_TABLES; }
+    HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(c, hipMalloc(&c->d_lcfa, pack.size() * sizeof(double)));
+    HIPCHK(c, hipMemcpy(c->d_lcfa, pack.data(), pack.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMalloc(&c->d_ctr, C_N * sizeof(unsigned long long)));
+    HIPCHK(c, hipMalloc(&c->d_args, sizeof(TransportArgs)));
+    HIPCHK(c, hipEventCreate(&c->ev0));
+    HIPCHK(c, hipEventCreate(&c->ev1));
+    *out = c;
+    return PTG_OK;
+}
+
+void ptg_destroy(ptg_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_buf(c->prim); free_buf(c->q[0]); free_buf(c->q[1]); free_buf(c->out); free_buf(c->inter); free_buf(c->tmp);
+    if (c->d_lcfa) (void)hipFree(c->d_lcfa);
+    if (c->d_n0) (void)hipFree(c->d_n0);
+    if (c->d_shower_count) (void)hipFree(c->d_shower_count);
+    if (c->d_ctr) (void)hipFree(c->d_ctr);
+    if (c->d_args) (void)hipFree(c->d_args);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* ptg_last_error(const ptg_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int ptg_set_laser(ptg_ctx* c, const ptg_laser_desc* d) {
+    if (!c || !d) return fail(c, PTG_E_ARG, "null argument");
+    LaserDev L;
+    int rc = make_laser(c, d, L);
+    if (rc) return rc;
+    c->laser = L;
+    c->have_laser = true;
+    return PTG_OK;
+}
+
+int ptg_field_info(ptg_ctx* c, double* max_timestep, double* z0) {
+    if (!c) return PTG_E_ARG;
+    if (!c->have_laser) return fail(c, PTG_E_STATE, "no laser set");
+    if (max_timestep) *max_timestep = c->laser.dt;
+    if (z0) {
+        ptg_laser_desc d{};
+        d.envelope = c->laser.envelope;
+        *z0 = ideal_initial_z(&d, c->laser);
+    }
+    return PTG_OK;
+}
+
+int ptg_build_beam(ptg_ctx* c, const ptg_beam_desc* b, double initial_z, uint64_t seed) {
+    if (!c || !b) return fail(c, PTG_E_ARG, "null argument");
+    if (b->n == 0 || b->n > 0xffffffffull) return fail(c, PTG_E_ARG, "beam size must be in [1, 2^32)");
+    if (b->species < 0 || b->species > 2 || b->radial < 0 || b->radial > 2) return fail(c, PTG_E_ARG, "bad species or radial distribution");
+    if (!(std::fabs(b->energy_chirp) <= 1.0)) return fail(c, PTG_E_ARG, "|energy_chirp| must be <= 1");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->prim.cap < b->n) HIPCHK(c, alloc_buf(c->prim, b->n));
+    int rc = ensure_n0(c, b->n);
+    if (rc) return rc;
+    BeamDev bd{};
+    bd.n = uint32_t(b->n);
+    bd.species = b->species;
+    bd.radial = b->radial;
+    bd.gamma = b->gamma;
+    bd.sigma = b->sigma;
+    bd.rho = -b->energy_chirp;
+    bd.radius = b->radius;
+    bd.max_radius = b->max_radius;
+    bd.length = b->length;
+    bd.rms_div = b->rms_div;
+    bd.angle = b->angle;
+    bd.plane_angle = b->collision_plane_angle;
+    bd.off_x = b->offset[0]; bd.off_y = b->offset[1]; bd.off_z = b->offset[2];
+    for (int k = 0; k < 4; ++k) bd.pol[k] = b->pol[k];
+    bd.weight = b->weight;
+    bd.initial_z = initial_z;
+    bd.key = mix64(key_from_seed(seed) ^ 0x6265616d6265616dULL);
+    launch_build_beam(bd, c->prim.v, c->d_n0, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->n_prim = b->n;
+    return PTG_OK;
+}
+
+static void soa_from_host(std::vector<double>* cols, std::vector<uint64_t>* u64, std::vector<uint32_t>* u32,
+                          const ptg_particles* p, size_t n, bool primaries) {
+    for (int k = 0; k < P_NDOUBLE; ++k) cols[k].assign(n, 0.0);
+    for (int k = 0; k < 3; ++k) u64[k].assign(n, 0);
+    for (int k = 0; k < 3; ++k) u32[k].assign(n, 0);
+    for (size_t i = 0; i < n; ++i) {
+        for (int c4 = 0; c4 < 4; ++c4) {
+            cols[P_RT + c4][i] = p->r[4 * i + c4];
+            cols[P_UT + c4][i] = p->u[4 * i + c4];
+            cols[P_R0T + c4][i] = p->r0 ? p->r0[4 * i + c4] : p->r[4 * i + c4];
+            double u0def = (c4 == 0 && p->species[i] != 2) ? 1.0 : 0.0;
+            cols[P_U0T + c4][i] = p->u0 ? p->u0[4 * i + c4] : u0def;
+            cols[P_SI + c4][i] = p->pol ? p->pol[4 * i + c4] : (c4 == 0 ? 1.0 : 0.0);
+        }
+        cols[P_PAYLOAD][i] = p->payload ? p->payload[i] : 0.0;
+        cols[P_PCHI][i] = p->parent_chi ? p->parent_chi[i] : 0.0;
+        cols[P_ICOUNT][i] = p->interaction_count ? p->interaction_count[i] : 0.0;
+        cols[P_WORK][i] = p->work ? p->work[i] : 0.0;
+        cols[P_WEIGHT][i] = p->weight ? p->weight[i] : 1.0;
+        u64[0][i] = p->id ? p->id[i] : i;
+        u64[1][i] = p->parent_id ? p->parent_id[i] : u64[0][i];
+        u64[2][i] = i;  // RNG stream of primary i
+        u32[0][i] = uint32_t(i);
+        u32[1][i] = uint32_t(p->species[i]) | (primaries ? META_PRIMARY : 0u);
+        u32[2][i] = 0;
+    }
+}
+
+int ptg_load_primaries(ptg_ctx* c, const ptg_particles* p, size_t n) {
+    if (!c || !p) return fail(c, PTG_E_ARG, "null argument");
+    if (n == 0 || n > 0xffffffffull) return fail(c, PTG_E_ARG, "number of primaries must be in [1, 2^32)");
+    if (!p->species || !p->r || !p->u) return fail(c, PTG_E_ARG, "species, r and u are required");
+    for (size_t i = 0; i < n; ++i)
+        if (p->species[i] > 2) return fail(c, PTG_E_ARG, "species out of range");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->prim.cap < n) HIPCHK(c, alloc_buf(c->prim, n));
+    int rc = ensure_n0(c, n);
+    if (rc) return rc;
+    std::vector<double> cols[P_NDOUBLE];
+    std::vector<uint64_t> u64[3];
+    std::vector<uint32_t> u32[3];
+    soa_from_host(cols, u64, u32, p, n, true);
+    for (int k = 0; k < P_NDOUBLE; ++k) HIPCHK(c, hipMemcpy(c->prim.v.d[k], cols[k].data(), n * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->prim.v.id, u64[0].data(), n * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->prim.v.parent_id, u64[1].data(), n * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->prim.v.stream, u64[2].data(), n * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->prim.v.shower, u32[0].data(), n * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->prim.v.meta, u32[1].data(), n * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->prim.v.nev, u32[2].data(), n * 4, hipMemcpyHostToDevice));
+    launch_primary_dirs(c->prim.v, uint32_t(n), c->d_n0, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->n_prim = n;
+    return PTG_OK;
+}
+
+static int validate_options(ptg_ctx* c, const ptg_collide_options* o) {
+    if (!(o->dt_multiplier > 0.0)) return fail(c, PTG_E_ARG, "dt_multiplier must be positive");
+    if (!(o->rate_increase >= 1.0)) return fail(c, PTG_E_ARG, "increase in pair creation rate must be >= 1.0");
+    if (o->rate_increase > 1.0 && o->multiplicity >= 0)
+        return fail(c, PTG_E_ARG, "event biasing (pair rate increase > 1) is incompatible with multiplicity selection");
+    if (o->gaunt_factor && !o->classical) return fail(c, PTG_E_ARG, "gaunt_factor requires classical");
+    if (o->precision != PTG_PREC_FP64 && o->precision != PTG_PREC_MIXED) return fail(c, PTG_E_ARG, "unknown precision mode");
+    return PTG_OK;
+}
+
+// One transport launch over `n_in` particles, repeated with a larger secondary
+// queue if the first attempt overflowed it (the launch is deterministic, so the
+// repeat reproduces the same particles).
+static int transport(ptg_ctx* c, TransportArgs a, bool photons, DevBuf& in, size_t n_in, DevBuf& next, double* ms,
+                     uint64_t* launches) {
+    if (n_in == 0) return PTG_OK;
+    // room for one final-state record per input particle
+    HIPCHK(c, grow_buf(c->out, c->n_out + n_in + n_in / 4 + 1024, c->n_out, c->stream));
+    if (a.opt.keep_decayed_photons && photons) HIPCHK(c, grow_buf(c->inter, c->n_inter + n_in + 1024, c->n_inter, c->stream));
+    if (next.cap < 1024) HIPCHK(c, alloc_buf(next, std::max<size_t>(1024, n_in)));
+    unsigned long long snap[C_N];
+    HIPCHK(c, hipMemcpy(snap, c->d_ctr, sizeof(snap), hipMemcpyDeviceToHost));
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        snap[C_WORK] = 0;
+        snap[C_NEXT] = 0;
+        HIPCHK(c, hipMemcpyAsync(c->d_ctr, snap, sizeof(snap), hipMemcpyHostToDevice, c->stream));
+        a.in = in.v;
+        a.n_in = uint32_t(n_in);
+        a.out = c->out.v;
+        a.inter = c->inter.v;
+        a.next = next.v;
+        a.next_cap = next.cap;
+        int blocks_per_cu = 2;
+        int grid = int(std::min<size_t>(size_t(c->n_cu) * blocks_per_cu, (n_in + 255) / 256));
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_args, &a, sizeof(a), hipMemcpyHostToDevice, c->stream));
+        if (photons) launch_photons_lcfa(c->d_args, a.laser.kind, a.laser.circular, grid, c->stream);
+        else launch_leptons_lcfa(c->d_args, a.laser.kind, a.laser.circular, grid, c->stream);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+        unsigned long long now[C_N];
+        HIPCHK(c, hipMemcpyAsync(now, c->d_ctr, sizeof(now), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        float t = 0.0f;
+        (void)hipEventElapsedTime(&t, c->ev0, c->ev1);
+        *ms += t;
+        ++*launches;
+        if (now[C_NEXT] <= next.cap) {
+            c->n_out = now[C_OUT];
+            c->n_inter = now[C_INTER];
+            return PTG_OK;
+        }
+        // overflow: grow the queue and repeat from the snapshot
+        HIPCHK(c, alloc_buf(next, size_t(now[C_NEXT] + now[C_NEXT] / 8 + 1024)));
+    }
+    return fail(c, PTG_E_OOM, "secondary queue kept overflowing");
+}
+
+int ptg_run(ptg_ctx* c, const ptg_collide_options* o, uint64_t seed, ptg_counters* cnt) {
+    if (!c || !o) return fail(c, PTG_E_ARG, "null argument");
+    if (!c->have_laser) return fail(c, PTG_E_STATE, "ptg_set_laser must be called first");
+    if (c->n_prim == 0) return fail(c, PTG_E_STATE, "no primaries resident (ptg_load_primaries / ptg_build_beam)");
+    int rc = validate_options(c, o);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t n = c->n_prim;
+
+    TransportArgs a{};
+    a.laser = c->laser;
+    a.sc = c->sc;
+    a.dt = c->laser.dt * o->dt_multiplier;
+    a.opt.rate_increase = o->rate_increase;
+    a.opt.t_stop = o->t_stop;
+    a.opt.min_energy = o->min_energy;
+    a.opt.cos_max_angle = std::cos(o->max_angle);
+    a.opt.tracking_photons = o->tracking_photons;
+    a.opt.tracking_secondaries = o->tracking_secondaries;
+    a.opt.keep_decayed_photons = o->keep_decayed_photons;
+    a.opt.pol_resolved = o->pol_resolved;
+    a.opt.rotate_stokes_pars = o->rotate_stokes_pars;
+    a.opt.discard_bg_e = o->discard_bg_e;
+    a.opt.discard_bg_ph = o->discard_bg_ph;
+    a.opt.multiplicity_on = o->multiplicity >= 0;
+    a.rr_mode = (o->classical && o->rr) ? (o->gaunt_factor ? 2 : 1) : 0;
+    a.classical_mode = o->classical && !o->gaunt_factor;
+    a.recoil = !o->classical && o->rr;
+    a.precision = o->precision;
+    a.key = key_from_seed(seed);
+    a.step_cap = 1u << 22;
+    a.lcfa_tables = c->d_lcfa;
+    a.work_ctr = reinterpret_cast<uint32_t*>(c->d_ctr + C_WORK);
+    a.out_ctr = c->d_ctr + C_OUT;
+    a.inter_ctr = c->d_ctr + C_INTER;
+    a.next_ctr = c->d_ctr + C_NEXT;
+    a.id_ctr = c->d_ctr + C_ID;
+    a.id_base = n;
+    a.prim_n0 = c->d_n0;
+    a.n_prim = uint32_t(n);
+    a.stats = c->d_ctr + C_STATS;
+    a.flags = reinterpret_cast<uint32_t*>(c->d_ctr + C_FLAGS);
+    if (a.opt.multiplicity_on) {
+        if (c->shower_cap < n) {
+            if (c->d_shower_count) (void)hipFree(c->d_shower_count);
+            c->d_shower_count = nullptr;
+            HIPCHK(c, hipMalloc(&c->d_shower_count, n * sizeof(uint32_t)));
+            c->shower_cap = n;
+        }
+        HIPCHK(c, hipMemsetAsync(c->d_shower_count, 0, n * sizeof(uint32_t), c->stream));
+    }
+    a.shower_count = c->d_shower_count;
+
+    HIPCHK(c, hipMemsetAsync(c->d_ctr, 0, C_N * sizeof(unsigned long long), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->n_out = 0;
+    c->n_inter = 0;
+
+    // Generation loop: primaries and pairs are electrons/positrons (or photons,
+    // for photon beams); every kernel consumes one queue and fills the other.
+    double ms[2] = {0.0, 0.0};  // lepton, photon kernels
+    uint64_t launches[2] = {0, 0};
+    uint64_t gens = 0;
+    DevBuf* cur = &c->prim;
+    size_t n_cur = n;
+    bool photon_stage = false;  // leptons first; each stage forwards the other kind
+    int qi = 0;
+    while (n_cur > 0) {
+        DevBuf& next = c->q[qi];
+        if (&next == cur) return fail(c, PTG_E_STATE, "internal queue aliasing");
+        rc = transport(c, a, photon_stage, *cur, n_cur, next, &ms[photon_stage], &launches[photon_stage]);
+        if (rc) return rc;
+        unsigned long long nn = 0;
+        HIPCHK(c, hipMemcpy(&nn, c->d_ctr + C_NEXT, sizeof(nn), hipMemcpyDeviceToHost));
+        cur = &next;
+        n_cur = size_t(nn);
+        qi ^= 1;
+        photon_stage = !photon_stage;
+        ++gens;
+    }
+
+    if (a.opt.multiplicity_on && c->n_out) {
+        HIPCHK(c, grow_buf(c->tmp, c->n_out, 0, c->stream));
+        unsigned long long zero = 0;
+        HIPCHK(c, hipMemcpy(c->d_ctr + C_WORK, &zero, sizeof(zero), hipMemcpyHostToDevice));
+        launch_select_multiplicity(c->out.v, c->n_out, c->tmp.v, c->d_ctr + C_WORK, c->d_shower_count,
+                                   uint32_t(o->multiplicity + 1), c->stream);
+        HIPCHK(c, hipGetLastError());
+        unsigned long long kept = 0;
+        HIPCHK(c, hipMemcpy(&kept, c->d_ctr + C_WORK, sizeof(kept), hipMemcpyDeviceToHost));
+        std::swap(c->out, c->tmp);
+        c->n_out = kept;
+    }
+
+    unsigned long long fin[C_N];
+    HIPCHK(c, hipMemcpy(fin, c->d_ctr, sizeof(fin), hipMemcpyDeviceToHost));
+    if (cnt) {
+        cnt->lepton_steps = fin[C_STATS + STAT_LEPTON_STEPS];
+        cnt->photon_steps = fin[C_STATS + STAT_PHOTON_STEPS];
+        cnt->emissions = fin[C_STATS + STAT_EMISSIONS];
+        cnt->pair_events = fin[C_STATS + STAT_PAIRS];
+        cnt->rejection_trials = fin[C_STATS + STAT_TRIALS];
+        cnt->n_secondaries = c->n_out;
+        cnt->n_intermediates = c->n_inter;
+        cnt->generations = gens;
+        cnt->error_flags = uint32_t(fin[C_FLAGS]);
+        cnt->kernel_ms = ms[0] + ms[1];
+        cnt->lepton_ms = ms[0];
+        cnt->photon_ms = ms[1];
+        cnt->lepton_launches = launches[0];
+        cnt->photon_launches = launches[1];
+    }
+    return PTG_OK;
+}
+
+int ptg_collide(ptg_ctx* c, const ptg_particles* p, size_t n, const ptg_collide_options* o, uint64_t seed, ptg_counters* cnt) {
+    int rc = ptg_load_primaries(c, p, n);
+    if (rc) return rc;
+    return ptg_run(c, o, seed, cnt);
+}
+
+static int fetch_buf(ptg_ctx* c, const DevBuf& b, size_t n, ptg_particles* p) {
+    if (!p || n == 0) return PTG_OK;
+    std::vector<double> col(n);
+    auto get4 = [&](int first, double* dst) -> int {
+        if (!dst) return PTG_OK;
+        for (int k = 0; k < 4; ++k) {
+            HIPCHK(c, hipMemcpy(col.data(), b.v.d[first + k], n * 8, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < n; ++i) dst[4 * i + k] = col[i];
+        }
+        return PTG_OK;
+    };
+    auto get1 = [&](int k, double* dst) -> int {
+        if (dst) HIPCHK(c, hipMemcpy(dst, b.v.d[k], n * 8, hipMemcpyDeviceToHost));
+        return PTG_OK;
+    };
+    int rc = 0;
+    if ((rc = get4(P_RT, p->r)) || (rc = get4(P_UT, p->u)) || (rc = get4(P_R0T, p->r0)) || (rc = get4(P_U0T, p->u0)) ||
+        (rc = get4(P_SI, p->pol)) || (rc = get1(P_PAYLOAD, p->payload)) || (rc = get1(P_PCHI, p->parent_chi)) ||
+        (rc = get1(P_ICOUNT, p->interaction_count)) || (rc = get1(P_WORK, p->work)) || (rc = get1(P_WEIGHT, p->weight)))
+        return rc;
+    if (p->id) HIPCHK(c, hipMemcpy(p->id, b.v.id, n * 8, hipMemcpyDeviceToHost));
+    if (p->parent_id) HIPCHK(c, hipMemcpy(p->parent_id, b.v.parent_id, n * 8, hipMemcpyDeviceToHost));
+    if (p->species) {
+        std::vector<uint32_t> m(n);
+        HIPCHK(c, hipMemcpy(m.data(), b.v.meta, n * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) p->species[i] = uint8_t(m[i] & 3u);
+    }
+    return PTG_OK;
+}
+
+int ptg_fetch(ptg_ctx* c, ptg_particles* sec, ptg_particles* inter) {
+    if (!c) return PTG_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = fetch_buf(c, c->out, c->n_out, sec);
+    if (rc) return rc;
+    return fetch_buf(c, c->inter, c->n_inter, inter);
+}
+
+int ptg_push(ptg_ctx* c, const uint8_t* species, double* r, double* u, size_t n, uint32_t n_steps, const ptg_collide_options* o,
+             uint32_t* steps_taken) {
+    if (!c || !species || !r || !u || !o) return fail(c, PTG_E_ARG, "null argument");
+    if (!c->have_laser) return fail(c, PTG_E_STATE, "ptg_set_laser must be called first");
+    if (n == 0) return PTG_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    uint8_t* ds = nullptr;
+    double *dr = nullptr, *du = nullptr;
+    uint32_t* dk = nullptr;
+    HIPCHK(c, hipMalloc(&ds, n));
+    HIPCHK(c, hipMalloc(&dr, n * 32));
+    HIPCHK(c, hipMalloc(&du, n * 32));
+    HIPCHK(c, hipMalloc(&dk, n * 4));
+    HIPCHK(c, hipMemcpy(ds, species, n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dr, r, n * 32, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(du, u, n * 32, hipMemcpyHostToDevice));
+    int rr_mode = (o->classical && o->rr) ? (o->gaunt_factor ? 2 : 1) : 0;
+    launch_push_lcfa(c->laser, c->laser.dt * o->dt_multiplier, rr_mode, ds, dr, du, uint32_t(n), n_steps, dk, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(r, dr, n * 32, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(u, du, n * 32, hipMemcpyDeviceToHost));
+    if (steps_taken) HIPCHK(c, hipMemcpy(steps_taken, dk, n * 4, hipMemcpyDeviceToHost));
+    (void)hipFree(ds); (void)hipFree(dr); (void)hipFree(du); (void)hipFree(dk);
+    return PTG_OK;
+}
+
+int ptg_histogram(ptg_ctx* c, ptg_hist_spec* s, double* bins_out, double* total_weight) {
+    if (!c || !s || !bins_out) return fail(c, PTG_E_ARG, "null argument");
+    if (s->nbins == 0 || s->nbins > 8192) return fail(c, PTG_E_ARG, "nbins must be in [1, 8192]");
+    if (s->quantity < 0 || s->quantity >= PTG_Q_COUNT) return fail(c, PTG_E_ARG, "unknown quantity");
+    HIPCHK(c, hipSetDevice(c->device));
+    HistArgs h{};
+    h.p = c->out.v;
+    h.n = c->n_out;
+    h.species = s->species;
+    h.q = s->quantity;
+    h.wfn = s->weight_fn;
+    h.log_scale = s->log_scale;
+    h.nbins = s->nbins;
+    int grid = std::max(1, std::min(c->n_cu * 4, int((h.n + 255) / 256)));
+    double gmin = s->min, gmax = s->max;
+    if (!s->fixed_range) {
+        unsigned long long mm[2] = {~0ull, 0ull};
+        HIPCHK(c, hipMemcpy(c->d_ctr + C_WORK, mm, sizeof(mm), hipMemcpyHostToDevice));
+        if (h.n) launch_hist_minmax(h, c->d_ctr + C_WORK, grid, c->stream);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipMemcpy(mm, c->d_ctr + C_WORK, sizeof(mm), hipMemcpyDeviceToHost));
+        if (mm[0] == ~0ull) {  // nothing to bin (reference returns None)
+            for (uint32_t b = 0; b < s->nbins; ++b) bins_out[b] = 0.0;
+            if (total_weight) *total_weight = 0.0;
+            return fail(c, PTG_E_DOMAIN, "no particles of this species to histogram");
+        }
+        gmin = host_order_key_inv(mm[0]);
+        gmax = host_order_key_inv(mm[1]);
+    }
+    uint32_t nbins = gmin == gmax ? 1u : s->nbins;  // hgram.rs:175-177
+    if (nbins != s->nbins) for (uint32_t b = 0; b < s->nbins; ++b) bins_out[b] = 0.0;
+    h.nbins = nbins;
+    h.gmin = gmin;
+    h.bin_vol = gmin == gmax ? 1.0 : (gmax - gmin) / double(nbins);
+    double* d_bins = nullptr;
+    HIPCHK(c, hipMalloc(&d_bins, (nbins + 1) * sizeof(double)));
+    HIPCHK(c, hipMemsetAsync(d_bins, 0, (nbins + 1) * sizeof(double), c->stream));
+    if (h.n) launch_hist_bin(h, d_bins, d_bins + nbins, grid, c->stream);
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> hb(nbins + 1);
+    HIPCHK(c, hipMemcpyAsync(hb.data(), d_bins, (nbins + 1) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(d_bins);
+    for (uint32_t b = 0; b < nbins; ++b) bins_out[b] = hb[b];
+    if (total_weight) *total_weight = hb[nbins];
+    s->min = gmin;
+    s->max = gmax;
+    return PTG_OK;
+}
+
+int ptg_eval(ptg_ctx* c, int fn, const double* in, size_t n, size_t in_w, double* out, size_t out_w) {
+    if (!c || !in || !out) return fail(c, PTG_E_ARG, "null argument");
+    if (fn < 0 || fn >= PTG_FN_COUNT) return fail(c, PTG_E_ARG, "unknown function id");
+    static const int need_in[] = {2, 5, 1, 1, 4, 1, 3, 12, 2, 3, 3, 3};
+    static const int need_out[] = {1, 4, 2, 4, 7, 1, 1, 4, 1, 1, 1, 8};
+    if (int(in_w) < need_in[fn] || int(out_w) < need_out[fn]) return fail(c, PTG_E_ARG, "row width too small for this function");
+    if (fn == PTG_FN_FIELDS && !c->have_laser) return fail(c, PTG_E_STATE, "no laser set");
+    if (fn >= PTG_FN_LP_RATE) return fail(c, PTG_E_UNSUPPORTED, "LMA probes are not built into this library yet");
+    if (n == 0) return PTG_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    double *din = nullptr, *dout = nullptr;
+    HIPCHK(c, hipMalloc(&din, n * in_w * 8));
+    HIPCHK(c, hipMalloc(&dout, n * out_w * 8));
+    HIPCHK(c, hipMemcpy(din, in, n * in_w * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(dout, 0, n * out_w * 8));
+    EvalArgs e{};
+    e.fn = fn;
+    e.in = din;
+    e.out = dout;
+    e.n = uint32_t(n);
+    e.in_w = uint32_t(in_w);
+    e.out_w = uint32_t(out_w);
+    e.lcfa = c->d_lcfa;
+    e.sc = c->sc;
+    e.laser = c->laser;
+    launch_eval(e, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, dout, n * out_w * 8, hipMemcpyDeviceToHost));
+    (void)hipFree(din);
+    (void)hipFree(dout);
+    return PTG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,47 @@
+// Arguments and launchers of the beam, histogram, selection and probe kernels.
+#pragma once
+#include "kern_common.h"
+
+namespace ptg {
+
+struct BeamDev {
+    uint32_t n;
+    int species, radial;
+    double gamma, sigma, rho, radius, max_radius, length, rms_div, angle, plane_angle;
+    double off_x, off_y, off_z;
+    double pol[4];
+    double weight;
+    double initial_z;
+    uint64_t key;
+};
+
+struct HistArgs {
+    PBuf p;
+    uint64_t n;
+    int species, q, wfn, log_scale;
+    uint32_t nbins;
+    double gmin, bin_vol;
+};
+
+struct EvalArgs {
+    int fn;
+    const double* in;
+    double* out;
+    uint32_t n, in_w, out_w;
+    const double* lcfa;
+    const double* lma;
+    LcfaScalars sc;
+    LaserDev laser;
+    uint64_t key;
+};
+
+void launch_build_beam(const BeamDev& b, const PBuf& p, double* n0, hipStream_t s);
+void launch_primary_dirs(const PBuf& p, uint32_t n, double* n0, hipStream_t s);
+void launch_hist_minmax(const HistArgs& h, unsigned long long* mm, int grid, hipStream_t s);
+void launch_hist_bin(const HistArgs& h, double* bins, double* total, int grid, hipStream_t s);
+double host_order_key_inv(unsigned long long k);
+void launch_select_multiplicity(const PBuf& src, uint64_t n, const PBuf& dst, unsigned long long* ctr, const uint32_t* count,
+                                uint32_t want, hipStream_t s);
+void launch_eval(const EvalArgs& e, hipStream_t s);
+
+}  // namespace ptg

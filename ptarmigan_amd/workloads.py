@@ -1,0 +1,38 @@
+"""The benchmark workloads, stated with their sizes.
+
+Each function returns (laser, beam, options) for one collision setup, built
+with the reference's own parameter semantics (src/main.rs:339-880).
+"""
+import math
+
+from .engine import Beam, CollideOptions, Envelope, Laser, LaserKind, Polarization, Species
+
+ELECTRON_MASS_MEV = 0.510999
+SPEED_OF_LIGHT = 2.997925e8
+
+
+def focused_lcfa_cascade(n=10_000_000):
+    """Focused Gaussian laser (waist 4 um, a0 = 10, 30 fs FWHM, 0.8 um, linear
+    polarisation) against a 16.5 GeV electron beam (n primaries, 5 um rms
+    transverse, 20 um rms longitudinal Gaussian); LCFA emission and Breit-Wheeler
+    pair creation with all secondaries tracked; FP64 state and tables."""
+    wavelength = 0.8e-6
+    n_cycles = SPEED_OF_LIGHT * 30e-15 / wavelength  # laser:fwhm_duration (src/main.rs:455-460)
+    laser = Laser(LaserKind.FAST_FOCUSED_LASER, a0=10.0, wavelength=wavelength, n_cycles=n_cycles, waist=4e-6,
+                  envelope=Envelope.GAUSSIAN, polarization=Polarization.LINEAR)
+    beam = Beam(n=n, gamma=16500.0 / ELECTRON_MASS_MEV, sigma=0.0, species=Species.ELECTRON, radius=5e-6, radial=0,
+                length=20e-6)
+    options = CollideOptions(tracking_photons=True, tracking_secondaries=True)
+    return laser, beam, options
+
+
+def plane_wave_lma(n=100_000):
+    """Linearly polarised plane-wave pulse, 0.8 um, a0 = 2, 16-cycle cos^2 envelope,
+    head-on with a 10 GeV electron beam (n primaries, 1% Gaussian energy spread,
+    1 urad rms divergence); LMA nonlinear Compton only (pair creation off)."""
+    gamma = 10000.0 / ELECTRON_MASS_MEV
+    laser = Laser(LaserKind.PLANE_WAVE, a0=2.0, wavelength=0.8e-6, n_cycles=16.0, envelope=Envelope.COS_SQUARED,
+                  polarization=Polarization.LINEAR)
+    beam = Beam(n=n, gamma=gamma, sigma=0.01 * gamma, species=Species.ELECTRON, rms_div=1e-6)
+    options = CollideOptions(tracking_photons=False)
+    return laser, beam, options

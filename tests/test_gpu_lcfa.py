@@ -1,0 +1,268 @@
+"""GPU parity of the locally-constant-field path against the CPU oracle.
+
+Deterministic pieces (fields, field-only trajectories, rate tables, the
+inverse-CDF sampler at given uniforms) are compared value by value; the full
+stochastic shower is compared statistically at matched sample size (the RNG
+streams differ: xoshiro256** per rank on the CPU, Philox per particle on the
+GPU). Tolerances: trajectories 1e-12 relative (north-star bound for FP64
+field-only pushes), rates and sampled values 1e-6 relative (the PWMCI inverse
+stops at 1e-6), spectra by two-sample chi-squared / KS at p > 1e-3.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+C_LIGHT = 2.997925e8
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def lasers():
+    from ptarmigan_amd import Envelope, Laser, LaserKind, Polarization
+    nc = C_LIGHT * 30e-15 / 0.8e-6
+    return {
+        "ffl_lp_gauss": Laser(LaserKind.FAST_FOCUSED_LASER, 10.0, 0.8e-6, nc, 4e-6, Envelope.GAUSSIAN, Polarization.LINEAR),
+        "ffl_cp_cos2": Laser(LaserKind.FAST_FOCUSED_LASER, 5.0, 0.8e-6, 8.0, 3e-6, Envelope.COS_SQUARED, Polarization.CIRCULAR),
+        "ffl_lp_flat_angle": Laser(LaserKind.FAST_FOCUSED_LASER, 3.0, 1.0e-6, 6.0, 5e-6, Envelope.FLATTOP,
+                                   Polarization.LINEAR, pol_angle=0.3),
+        "fpw_lp_cos2": Laser(LaserKind.FAST_PLANE_WAVE, 2.0, 0.8e-6, 16.0, 0.0, Envelope.COS_SQUARED, Polarization.LINEAR),
+        "fpw_cp_gauss_chirp": Laser(LaserKind.FAST_PLANE_WAVE, 4.0, 0.8e-6, 5.0, 0.0, Envelope.GAUSSIAN,
+                                    Polarization.CIRCULAR, chirp_b=1e-3),
+    }
+
+
+@pytest.mark.parametrize("name", list(lasers()))
+def test_fields_match_oracle(engine, orc, name):
+    from ptarmigan_amd import Fn
+    laser = lasers()[name]
+    say("fields", name)
+    engine.set_laser(laser)
+    rng = np.random.default_rng(1)
+    n = 2000
+    lam = laser.wavelength
+    pts = np.stack([rng.uniform(-20, 20, n) * lam, rng.normal(0, 3e-6, n), rng.normal(0, 3e-6, n),
+                    rng.uniform(-20, 20, n) * lam], axis=1)
+    got = engine.eval(Fn.FIELDS, pts, 7)
+    ref = orc.fields(laser.desc(), pts)
+    scale = np.abs(ref[:, :3]).max()
+    bscale = np.abs(ref[:, 3:6]).max()
+    assert np.abs(got[:, :3] - ref[:, :3]).max() / scale < 1e-13
+    assert np.abs(got[:, 3:6] - ref[:, 3:6]).max() / bscale < 1e-13
+    assert np.allclose(got[:, 6], ref[:, 6], rtol=1e-12, atol=1e-15 * laser.a0)
+
+
+@pytest.mark.parametrize("name", list(lasers()))
+def test_field_only_trajectories(engine, orc, name):
+    """Field::push repeated to the edge of the field: final momentum within 1e-12
+    of the oracle (relative to |u|), positions within 1e-12 of the distance travelled."""
+    laser = lasers()[name]
+    say("trajectories", name)
+    engine.set_laser(laser)
+    dt, z0 = engine.field_info()
+    assert math.isclose(dt, orc.lib().orc_laser_max_timestep(laser.desc()), rel_tol=1e-15)
+    assert math.isclose(z0, orc.lib().orc_laser_ideal_initial_z(laser.desc()), rel_tol=1e-15)
+    rng = np.random.default_rng(2)
+    n = 256
+    gamma = rng.uniform(1e3, 3e4, n)
+    ux = rng.normal(0, 5e-6, n) * gamma
+    uy = rng.normal(0, 5e-6, n) * gamma
+    uz = -np.sqrt(gamma ** 2 - 1 - ux ** 2 - uy ** 2)
+    u = np.stack([gamma, ux, uy, uz], axis=1)
+    x = rng.normal(0, 2e-6, n)
+    y = rng.normal(0, 2e-6, n)
+    r = np.stack([-z0 * np.ones(n), x, y, z0 * np.ones(n)], axis=1)
+    species = np.where(rng.uniform(size=n) < 0.5, 0, 1).astype(np.uint8)
+    r_g, u_g, k_g = engine.push(species, r, u, 200000)
+    r_c, u_c, k_c = orc.push(laser.desc(), species, r, u, 200000)
+    assert np.array_equal(k_g, k_c)
+    du = np.abs(u_g - u_c).max(axis=1) / np.linalg.norm(u_c[:, 1:], axis=1)
+    travelled = np.abs(r_c - r).max(axis=1) + 1e-30
+    dr = np.abs(r_g - r_c).max(axis=1) / travelled
+    say("  max rel du", du.max(), "dr", dr.max(), "steps", k_c.max())
+    assert du.max() < 1e-12
+    assert dr.max() < 1e-12
+
+
+def test_lcfa_rate_and_aux_t(engine, orc):
+    from ptarmigan_amd import Fn
+    chi = np.exp(np.linspace(np.log(1e-4), np.log(500.0), 3001))
+    gamma = np.full_like(chi, 2e4)
+    got = engine.eval(Fn.LCFA_RATE, np.stack([chi, gamma], 1), 1)[:, 0]
+    ref = np.array([orc.lib().orc_lcfa_rate(c, g) for c, g in zip(chi, gamma)])
+    assert np.max(np.abs(got - ref) / ref) < 1e-6
+    t = engine.eval(Fn.AUX_T, chi[:, None], 2)
+    tr = np.array([orc.aux_t(c) for c in chi])
+    mask = tr[:, 0] > 0
+    assert np.array_equal(t[:, 0] > 0, mask)
+    assert np.max(np.abs(t[mask] - tr[mask]) / tr[mask]) < 1e-6
+    g = engine.eval(Fn.GAUNT, chi[:, None], 1)[:, 0]
+    gr = np.array([orc.lib().orc_gaunt(c) for c in chi])
+    assert np.max(np.abs(g - gr) / gr) < 1e-6
+
+
+def test_airy_and_bessel_k(engine, orc):
+    from ptarmigan_amd import Fn
+    x = np.concatenate([np.linspace(0.0, 0.999, 200), np.exp(np.linspace(0, np.log(120.0), 400))])
+    got = engine.eval(Fn.AIRY, x[:, None], 4)
+    ref = np.array([orc.airy(v)[0] for v in x])
+    for col in (0, 1):
+        m = ref[:, col] != 0
+        assert np.max(np.abs(got[m, col] - ref[m, col]) / np.abs(ref[m, col])) < 1e-12
+    m = x > 0
+    for col in (2, 3):
+        mm = m & (ref[:, col] != 0)
+        assert np.max(np.abs(got[mm, col] - ref[mm, col]) / np.abs(ref[mm, col])) < 1e-12
+
+
+def test_lcfa_sampler_at_given_uniforms(engine, orc):
+    """lcfa::photon_emission::sample is deterministic in (chi, gamma, r1, r2, r3)."""
+    from ptarmigan_amd import Fn
+    rng = np.random.default_rng(3)
+    n = 4000
+    chi = np.exp(rng.uniform(np.log(2e-3), np.log(300.0), n))
+    gamma = rng.uniform(200.0, 5e4, n)
+    r = rng.uniform(1e-12, 1.0, (n, 3))
+    x = np.column_stack([chi, gamma, r])
+    got = engine.eval(Fn.LCFA_SAMPLE, x, 4)
+    ref = np.array([orc.lcfa_sample(*row) for row in x])
+    assert np.array_equal(got[:, 1], ref[:, 1])
+    ok = ref[:, 1] == 1.0
+    assert np.max(np.abs(got[:, 0] - ref[:, 0]) / ref[:, 0]) < 1e-6
+    dth = np.abs(got[ok, 2] - ref[ok, 2]) / np.maximum(ref[ok, 2], 1e-300)
+    assert np.quantile(dth, 0.999) < 1e-6
+    assert np.allclose(got[:, 3], ref[:, 3], rtol=1e-15)
+
+
+def test_lcfa_stokes(engine, orc):
+    from ptarmigan_amd import Fn
+    rng = np.random.default_rng(4)
+    rows = []
+    for _ in range(500):
+        gamma = rng.uniform(1e3, 2e4)
+        chi = rng.uniform(0.05, 5.0)
+        v = np.array([0.0, 0.0, -1.0]) + rng.normal(0, 1e-4, 3)
+        w = np.cross(v, rng.normal(size=3))
+        om = gamma * rng.uniform(0.01, 0.9)
+        th = rng.uniform(0, 3.0 / gamma)
+        ph = rng.uniform(0, 2 * np.pi)
+        vn = v / np.linalg.norm(v)
+        e1 = np.cross(vn, [1.0, 0, 0])
+        e1 /= np.linalg.norm(e1)
+        e2 = np.cross(vn, e1)
+        k3 = om * (np.cos(th) * vn + np.sin(th) * (np.cos(ph) * e1 + np.sin(ph) * e2))
+        k = np.array([np.linalg.norm(k3), *k3])
+        rows.append(np.concatenate([k, [chi, gamma], v, w]))
+    x = np.array(rows)
+    got = engine.eval(Fn.LCFA_STOKES, x, 4)
+    ref = np.array([orc.lcfa_stokes(r[:4], r[4], r[5], r[6:9], r[9:12]) for r in x])
+    assert np.max(np.abs(got - ref)) < 1e-10
+
+
+def _run_both(engine, orc, laser, beam, opts, n_threads=8, seed=5):
+    engine.set_laser(laser)
+    dt, z0 = engine.field_info()
+    prim = orc.build_beam(beam.desc(), z0 + 3.0 * beam.length, seed)
+    sec_g, inter_g, cnt_g = engine.collide(prim, opts, seed=seed)
+    sec_c, inter_c, cnt_c, _ = orc.collide(laser.desc(), opts.desc(), prim, seed, n_threads)
+    return prim, (sec_g, inter_g, cnt_g), (sec_c, inter_c, cnt_c)
+
+
+def test_collide_recoil_free_trajectories_exact(engine, orc):
+    """With radiation_reaction off the electron never recoils, so collide()'s
+    final electron states are deterministic: compare them one by one."""
+    from ptarmigan_amd import workloads
+    laser, beam, opts = workloads.focused_lcfa_cascade(400)
+    opts.rr = False
+    opts.tracking_photons = False
+    say("collide rr=off exact")
+    prim, (sg, _, cg), (sc, _, cc) = _run_both(engine, orc, laser, beam, opts)
+    assert cg["lepton_steps"] == cc["lepton_steps"]
+    eg = sg["species"] == 0
+    ec = sc["species"] == 0
+    assert eg.sum() == ec.sum() == beam.n
+    og = np.argsort(sg["id"][eg])
+    oc = np.argsort(sc["id"][ec])
+    ug, uc = sg["u"][eg][og], sc["u"][ec][oc]
+    rg, rc = sg["r"][eg][og], sc["r"][ec][oc]
+    du = np.abs(ug - uc).max(axis=1) / np.linalg.norm(uc[:, 1:], axis=1)
+    dr = np.abs(rg - rc).max(axis=1) / np.abs(rc - prim["r"][np.sort(sc["id"][ec]).astype(int)]).max(axis=1)
+    say("  du", du.max(), "dr", dr.max())
+    assert du.max() < 1e-12 and dr.max() < 1e-12
+    # the work done by the field accumulates the same way
+    wg, wc = sg["work"][eg][og], sc["work"][ec][oc]
+    assert np.max(np.abs(wg - wc)) <= 1e-9 * np.max(np.abs(wc)) + 1e-12
+
+
+def _chi2_two_sample(a, b, bins):
+    ha, _ = np.histogram(a, bins)
+    hb, _ = np.histogram(b, bins)
+    na, nb = ha.sum(), hb.sum()
+    m = (ha + hb) > 0
+    k1, k2 = math.sqrt(nb / na), math.sqrt(na / nb)
+    chi2 = np.sum((k1 * ha[m] - k2 * hb[m]) ** 2 / (ha[m] + hb[m]))
+    dof = m.sum() - 1
+    from scipy.stats import chi2 as c2
+    return c2.sf(chi2, dof)
+
+
+def test_shower_statistics_match_oracle(engine, orc):
+    """Photon and pair yields, photon spectrum and final electron spectrum agree
+    statistically with the oracle at matched sample size."""
+    from scipy.stats import ks_2samp
+    from ptarmigan_amd import workloads
+    laser, beam, opts = workloads.focused_lcfa_cascade(3000)
+    beam.radius = 1.5e-6  # near the axis: strong field, many events
+    say("shower statistics")
+    prim, (sg, _, cg), (sc, _, cc) = _run_both(engine, orc, laser, beam, opts, n_threads=16)
+    say("  gpu", cg, "\n  cpu", cc)
+    for key in ("emissions", "pair_events"):
+        a, b = cg[key], cc[key]
+        assert abs(a - b) < 5.0 * math.sqrt(a + b + 1.0), (key, a, b)
+    for key in ("lepton_steps", "photon_steps"):
+        assert abs(cg[key] - cc[key]) < 0.02 * cc[key] + 1000, key
+    pg = sg["u"][sg["species"] == 2, 0]
+    pc = sc["u"][sc["species"] == 2, 0]
+    bins = np.exp(np.linspace(np.log(1e-3), np.log(4e4), 41))
+    assert _chi2_two_sample(pg, pc, bins) > 1e-3
+    eg = sg["u"][sg["species"] == 0, 0]
+    ec = sc["u"][sc["species"] == 0, 0]
+    assert ks_2samp(eg, ec).pvalue > 1e-3
+    # photon angle spectrum
+    ag = np.arctan2(sg["u"][sg["species"] == 2, 1], -sg["u"][sg["species"] == 2, 3])
+    ac = np.arctan2(sc["u"][sc["species"] == 2, 1], -sc["u"][sc["species"] == 2, 3])
+    assert ks_2samp(ag, ac).pvalue > 1e-3
+    # photon Stokes parameters (means over the spectrum)
+    for col in (1, 2, 3):
+        sgm = sg["pol"][sg["species"] == 2, col]
+        scm = sc["pol"][sc["species"] == 2, col]
+        se = math.sqrt(sgm.var() / len(sgm) + scm.var() / len(scm)) + 1e-12
+        assert abs(sgm.mean() - scm.mean()) < 5 * se, col
+
+
+def test_histogram_matches_numpy(engine):
+    """ptg_histogram follows Histogram::generate_1d: global range, max value
+    dropped, total counts every weight."""
+    from ptarmigan_amd import Species, workloads
+    from ptarmigan_amd.engine import _abi  # noqa: F401
+    laser, beam, opts = workloads.focused_lcfa_cascade(2000)
+    beam.radius = 1.5e-6
+    engine.set_laser(laser)
+    engine.build_beam(beam, seed=9)
+    cnt = engine.run(opts, seed=9)
+    sec, _ = engine.fetch(cnt)
+    ph = sec["species"] == 2
+    e = 0.510999 * sec["u"][ph, 0]
+    counts, lo, hi, total = engine.histogram(Species.PHOTON, 14, 200)  # PTG_Q_ENERGY
+    assert lo == e.min() and hi == e.max()
+    width = (hi - lo) / 200
+    idx = np.floor((e - lo) / width).astype(np.int64)
+    ref = np.bincount(idx[idx < 200], minlength=200).astype(float)
+    assert np.array_equal(counts, ref)
+    assert total == pytest.approx(sec["weight"][ph].sum())

@@ -336,18 +336,21 @@ void orc_run_free(void* h) { delete static_cast<OrcRun*>(h); }
 
 /* Field-only push, stopping when a particle leaves the field (Field::push). */
 void orc_push(const ptg_laser_desc* ld, const uint8_t* species, double* r, double* u, size_t n, uint32_t n_steps,
-              double dt_multiplier, int eqn, uint32_t* steps_taken) {
+              double dt_multiplier, int eqn, uint32_t* steps_taken, double* work) {
     Laser L = to_laser(ld);
     double dt = L.max_timestep() * dt_multiplier;
     for (size_t i = 0; i < n; ++i) {
         V4 ri = get4(r, i), ui = get4(u, i);
         double q = species[i] == 0 ? ELECTRON_CHARGE / ELECTRON_MASS : (species[i] == 1 ? -ELECTRON_CHARGE / ELECTRON_MASS : 0.0);
         uint32_t k = 0;
+        double w = 0.0;
         for (; k < n_steps && L.contains(ri); ++k) {
             PushOut po = field_push(L, ri, ui, q, dt, Eqn(eqn));
             ri = po.r;
             ui = po.u;
+            w += po.work;
         }
+        if (work) work[i] = w;
         put4(r, i, ri);
         put4(u, i, ui);
         if (steps_taken) steps_taken[i] = k;

@@ -70,7 +70,8 @@ def lib():
         L.orc_run_counts.argtypes = [C.c_void_p, P(C.c_uint64)]
         L.orc_run_fetch.argtypes = [C.c_void_p, P(Particles), P(Particles)]
         L.orc_run_free.argtypes = [C.c_void_p]
-        L.orc_push.argtypes = [P(LaserDesc), P(C.c_uint8), P(d), P(d), C.c_size_t, C.c_uint32, d, C.c_int, P(C.c_uint32)]
+        L.orc_push.argtypes = [P(LaserDesc), P(C.c_uint8), P(d), P(d), C.c_size_t, C.c_uint32, d, C.c_int, P(C.c_uint32),
+                               P(d)]
         L.orc_build_beam.argtypes = [P(BeamDesc), d, C.c_uint64, P(Particles)]
         for f in ("orc_laser_max_timestep", "orc_laser_ideal_initial_z"):
             getattr(L, f).restype = d
@@ -143,14 +144,16 @@ def build_beam(beam, initial_z, seed=0):
     return arrs
 
 
-def push(laser, species, r, u, n_steps, dt_multiplier=1.0, eqn=0):
-    r = np.ascontiguousarray(r, dtype=np.float64).copy()
-    u = np.ascontiguousarray(u, dtype=np.float64).copy()
+def push(laser, species, r, u, n_steps, dt_multiplier=1.0, eqn=0, with_work=False):
+    """Field::push repeated up to n_steps times per particle, stopping at the field edge."""
+    r = np.ascontiguousarray(r, dtype=np.float64).reshape(-1, 4).copy()
+    u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1, 4).copy()
     sp = np.ascontiguousarray(species, dtype=np.uint8)
     steps = np.zeros(len(sp), dtype=np.uint32)
+    work = np.zeros(len(sp))
     lib().orc_push(C.byref(laser), sp.ctypes.data_as(C.POINTER(C.c_uint8)), dptr(r), dptr(u), len(sp), n_steps,
-                   dt_multiplier, eqn, steps.ctypes.data_as(C.POINTER(C.c_uint32)))
-    return r, u, steps
+                   dt_multiplier, eqn, steps.ctypes.data_as(C.POINTER(C.c_uint32)), dptr(work))
+    return (r, u, steps, work) if with_work else (r, u, steps)
 
 
 def fields(laser, r):

@@ -215,9 +215,10 @@ int ptg_fetch(ptg_ctx* ctx, ptg_particles* secondaries, ptg_particles* intermedi
 
 /* Field-only push of n particles for up to n_steps steps each (stops when a particle
  * leaves the field): repeated Field::push (src/field/mod.rs:147-152 and the LMA
- * overrides, e.g. plane_wave.rs:196-249). r, u are n x 4, updated in place (host). */
+ * override plane_wave.rs:196-249). r, u are n x 4, updated in place (host);
+ * work (may be NULL) receives the summed work returned by each push. */
 int ptg_push(ptg_ctx* ctx, const uint8_t* species, double* r, double* u, size_t n, uint32_t n_steps,
-             const ptg_collide_options* opt, uint32_t* steps_taken);
+             const ptg_collide_options* opt, uint32_t* steps_taken, double* work);
 
 /* Weighted histogram of the resident final-state particles (hgram.rs:209-323).
  * bins_out has nbins entries (min/max written back when computed). */
@@ -234,10 +235,10 @@ typedef enum {
     PTG_FN_GAUNT = 5,            /* chi -> g(chi)  lcfa/photon_emission/mod.rs:330 */
     PTG_FN_PAIR_SPECTRUM = 6,    /* (s, chi, sv1) -> spectrum  lcfa/pair_creation/mod.rs:80 */
     PTG_FN_LCFA_STOKES = 7,      /* (k[4], chi, gamma, v[3], w[3]) -> Stokes[4]  mod.rs:239 */
-    PTG_FN_LP_RATE = 8,          /* (a, eta) -> rate  nonlinear_compton/lp/mod.rs:430 */
-    PTG_FN_LP_HARMONIC = 9,      /* (a, eta, r) -> n  nonlinear_compton/lp/mod.rs:478 */
-    PTG_FN_DOUBLE_BESSEL = 10,   /* (n, x, y) -> J(n,x,y)  special_functions/double_bessel.rs */
-    PTG_FN_LP_SAMPLE = 11,       /* (a, eta, seed) -> (n, s, cphi, stokes...) nonlinear_compton/lp/mod.rs:586 */
+    PTG_FN_LP_RATE = 8,          /* (a, eta) -> (f, in_table)  nonlinear_compton/lp/mod.rs:430 */
+    PTG_FN_LP_HARMONIC = 9,      /* (a, eta, frac) -> n  nonlinear_compton/lp/mod.rs:478 */
+    PTG_FN_DOUBLE_BESSEL = 10,   /* (n, x, y, recurrence_only) -> J(n-2..n+2, x, y)  special_functions/double_bessel.rs */
+    PTG_FN_LP_SAMPLE = 11,       /* (a, eta, stream, fixed_n) -> (n, s, theta, S1, S2, trials)  lp/mod.rs:586 */
     PTG_FN_COUNT = 12
 } ptg_eval_fn;
 int ptg_eval(ptg_ctx* ctx, int fn, const double* in, size_t n, size_t in_w, double* out, size_t out_w);

@@ -93,7 +93,7 @@ def load():
     L.ptg_run.argtypes = [vp, P(CollideOptions), C.c_uint64, P(Counters)]
     L.ptg_collide.argtypes = [vp, P(Particles), C.c_size_t, P(CollideOptions), C.c_uint64, P(Counters)]
     L.ptg_fetch.argtypes = [vp, P(Particles), P(Particles)]
-    L.ptg_push.argtypes = [vp, P(C.c_uint8), P(d), P(d), C.c_size_t, C.c_uint32, P(CollideOptions), P(C.c_uint32)]
+    L.ptg_push.argtypes = [vp, P(C.c_uint8), P(d), P(d), C.c_size_t, C.c_uint32, P(CollideOptions), P(C.c_uint32), P(d)]
     L.ptg_histogram.argtypes = [vp, P(HistSpec), P(d), P(d)]
     L.ptg_eval.argtypes = [vp, C.c_int, P(d), C.c_size_t, C.c_size_t, P(d), C.c_size_t]
     for f in ("ptg_set_laser", "ptg_field_info", "ptg_build_beam", "ptg_load_primaries", "ptg_run", "ptg_collide",

@@ -1,0 +1,591 @@
+// Locally-monochromatic (LMA) plane wave and linearly polarised nonlinear
+// Compton scattering, device side.
+//
+// Step path (every step of every electron): cycle-averaged ponderomotive push
+// in proper time and the total emission rate from the LP rate table
+// (src/field/plane_wave.rs:68-249, src/nonlinear_compton/lp/mod.rs:430-463).
+// Event path (one thread per emission, in the event kernel): harmonic from the
+// CDF table, (s, theta) by rejection on the double-differential rate built
+// from the generalised Bessel function J(n, x, y), photon momentum by a boost
+// from the zero-momentum frame, Stokes parameters
+// (src/nonlinear_compton/mod.rs:58-138, lp/mod.rs:21-676,
+// special_functions/double_bessel.rs).
+#pragma once
+#include "dev_common.h"
+#include "dev_fields.h"
+
+namespace ptg {
+
+// ---------------------------------------------------------------- table layout
+// The LMA pack (doubles): LP rate table (68 x 61) then the LP CDF table
+// (4148 x 16 x 2), with their log-grid origins and steps as scalars.
+namespace lma_pack {
+constexpr int RATE = 0;                       // nonlinear_compton/lp/rate_table.rs:5
+constexpr int RATE_COLS = 61, RATE_ROWS = 68;
+constexpr int CDF = RATE + RATE_COLS * RATE_ROWS;  // nonlinear_compton/lp/cdf_table.rs:4
+constexpr int CDF_COLS = 61, CDF_N = 4148;
+constexpr int SIZE = CDF + CDF_N * 32;
+}  // namespace lma_pack
+
+struct LmaScalars {
+    double rate_min[2], rate_step[2];  // ln a, ln eta (rate_table.rs:3-4)
+    double cdf_min[2], cdf_step[2];    // (cdf_table.rs:2-3)
+};
+
+// ---------------------------------------------------------------- complex functions
+__device__ __forceinline__ double c_arg(cplx a) { return atan2(a.im, a.re); }
+__device__ __forceinline__ double c_abs(cplx a) { return hypot(a.re, a.im); }
+__device__ __forceinline__ cplx c_polar(double r, double th) { double s, c; sincos(th, &s, &c); return {r * c, r * s}; }
+__device__ __forceinline__ cplx c_log(cplx a) { return {log(c_abs(a)), c_arg(a)}; }
+__device__ __forceinline__ cplx c_sqrt(cplx a) {
+    if (a.im == 0.0) {
+        if (!signbit(a.re)) return {sqrt(a.re), a.im};
+        double im = sqrt(-a.re);
+        return signbit(a.im) ? cplx{0.0, -im} : cplx{0.0, im};
+    }
+    if (a.re == 0.0) {
+        double x = sqrt(fabs(a.im) / 2.0);
+        return signbit(a.im) ? cplx{x, -x} : cplx{x, x};
+    }
+    return c_polar(sqrt(c_abs(a)), c_arg(a) / 2.0);
+}
+__device__ __forceinline__ cplx c_acos(cplx z) {  // -i ln(i sqrt(1 - z^2) + z)
+    cplx w = mul_i(c_sqrt(cplx{1.0 - (z.re * z.re - z.im * z.im), -2.0 * z.re * z.im})) + z;
+    return mul_mi(c_log(w));
+}
+__device__ __forceinline__ cplx c_sin(cplx a) {
+    double s, c;
+    sincos(a.re, &s, &c);
+    return {s * cosh(a.im), c * sinh(a.im)};
+}
+__device__ __forceinline__ cplx c_exp(cplx a) { return c_polar(exp(a.re), a.im); }
+
+// ---------------------------------------------------------------- plane wave (plane_wave.rs)
+// <a^2> and d<a^2>/dphase for the LMA envelope (plane_wave.rs:68-150).
+__device__ __forceinline__ double pw_a_sqd(const LaserDev& L, double phase) {
+    double norm = L.circular ? 1.0 : 0.5;
+    double nc = L.n_cycles;
+    double a02 = L.a0 * L.a0;
+    if (L.envelope == ENV_COS2) {
+        if (fabs(phase) < PI_D * nc) {
+            double s, c;
+            sincos_phase(phase / (2.0 * nc), &s, &c);
+            double c2 = c * c;
+            return norm * a02 * c2 * c2;
+        }
+        return 0.0;
+    }
+    if (L.envelope == ENV_FLATTOP) {
+        double ap = fabs(phase);
+        if (ap > PI_D * (nc + 1.0)) return 0.0;
+        if (ap > PI_D * (nc - 1.0)) {
+            double s, c;
+            sincos_phase(0.25 * (ap - (nc - 1.0) * PI_D), &s, &c);
+            double c2 = c * c;
+            return norm * a02 * c2 * c2;
+        }
+        return norm * a02;
+    }
+    double q = phase / (PI_D * nc);
+    return norm * a02 * exp2(-q * q);
+}
+// d<a^2>/d(phase) * k0 (the magnitude of the time and z components of the four-gradient)
+__device__ __forceinline__ double pw_grad(const LaserDev& L, double phase) {
+    double norm = L.circular ? 1.0 : 0.5;
+    double nc = L.n_cycles;
+    double a02 = L.a0 * L.a0;
+    if (L.envelope == ENV_COS2) {
+        if (fabs(phase) < PI_D * nc) {
+            double s, c, s2, c2;
+            sincos_phase(phase / nc, &s2, &c2);
+            sincos_phase(phase / (2.0 * nc), &s, &c);
+            return norm * L.k0 * a02 * s2 * c * c / nc;
+        }
+        return 0.0;
+    }
+    if (L.envelope == ENV_FLATTOP) {
+        double ap = fabs(phase);
+        if (ap > PI_D * (nc + 1.0) || ap < PI_D * (nc - 1.0)) return 0.0;
+        double s, c;
+        sincos_phase(0.25 * (ap - (nc - 1.0) * PI_D), &s, &c);
+        return norm * L.k0 * a02 * copysign(1.0, phase) * s * c * c * c;
+    }
+    double q = phase / (PI_D * nc);
+    return norm * L.k0 * a02 * 2.0 * LN2_D * phase * exp2(-q * q) / ((PI_D * nc) * (PI_D * nc));
+}
+
+// Leapfrog in proper time with the ponderomotive force, optional cycle-averaged
+// Landau-Lifshitz reaction, then the mass-shell fix u^2 = 1 + <a^2>
+// (plane_wave.rs:196-249). r = (ct, x, y, z), u = (u0, u). Returns the work.
+__device__ __forceinline__ double pw_push(const LaserDev& L, d4& r, d4& u, double rqm, double dt, int ll) {
+    double dtau = dt / u.t;
+    double sc = rqm / RQM_ELECTRON;
+    double scale = sc * sc;
+    const double kc = C_LIGHT * T_COMPTON;
+    d4 u_mid{0.0, 0.0, 0.0, 0.0};
+    if (ll) {
+        double ph = L.k0 * (r.t - r.z);
+        double f = -0.5 * C_LIGHT * scale * pw_grad(L, ph);  // both t and z components
+        double eta = kc * L.k0 * (u.t - u.z);
+        double gfac = -2.0 * ALPHA * pw_a_sqd(L, ph) * eta * eta / (3.0 * T_COMPTON) * scale * scale;
+        u_mid = {u.t + 0.5 * (f + gfac * u.t) * dtau, u.x + 0.5 * (gfac * u.x) * dtau, u.y + 0.5 * (gfac * u.y) * dtau,
+                 u.z + 0.5 * (f + gfac * u.z) * dtau};
+    }
+    double h = 0.5 * C_LIGHT * dtau;
+    r = {r.t + h * u.t, r.x + h * u.x, r.y + h * u.y, r.z + h * u.z};
+    double ph = L.k0 * (r.t - r.z);
+    double f = -0.5 * C_LIGHT * scale * pw_grad(L, ph);
+    double dwork;
+    if (ll) {
+        double eta = kc * L.k0 * (u_mid.t - u_mid.z);
+        double asq = pw_a_sqd(L, ph);
+        double gfac = -2.0 * ALPHA * asq * eta * eta / (3.0 * T_COMPTON) * scale * scale;
+        u = {u.t + (f + gfac * u_mid.t) * dtau, u.x + gfac * u_mid.x * dtau, u.y + gfac * u_mid.y * dtau,
+             u.z + (f + gfac * u_mid.z) * dtau};
+        double omega = C_LIGHT * L.k0;
+        double delta = L.circular ? 1.0 : 0.75;
+        dwork = (f + 2.0 * ALPHA * omega * eta * delta * asq * asq / 3.0) * dtau;
+    } else {
+        u = {u.t + f * dtau, u.x, u.y, u.z + f * dtau};
+        dwork = f * dtau;
+    }
+    r = {r.t + h * u.t, r.x + h * u.x, r.y + h * u.y, r.z + h * u.z};
+    double asq = pw_a_sqd(L, L.k0 * (r.t - r.z));
+    u.t = sqrt(1.0 + asq + u.x * u.x + u.y * u.y + u.z * u.z);
+    return dwork;
+}
+
+// ---------------------------------------------------------------- LP rate (lp/mod.rs:430-463, lp/linear.rs:6-45)
+__device__ __forceinline__ double lp_linear_rate(double a, double eta) {
+    double eta2 = eta * eta;
+    if (eta > 1.0e-3)
+        return (a * a / 8.0) * (0.5 + 4.0 / eta - 0.5 / ((1.0 + 2.0 * eta) * (1.0 + 2.0 * eta)) +
+                                (1.0 - 2.0 / eta - 2.0 / eta2) * log1p(2.0 * eta));
+    return (a * a / 8.0) * (8.0 * eta / 3.0 - 16.0 * eta2 / 3.0 + 208.0 * eta2 * eta / 15.0);
+}
+// f(a, eta) such that dP/dphase = alpha f / eta; returns false when (a, eta)
+// is outside the table (the reference then reports and uses probability 0).
+__device__ __forceinline__ bool lp_rate(const double* rt, const LmaScalars& s, double a, double eta, double& out) {
+    double x = log(a), y = log(eta);
+    if (x < s.rate_min[0]) { out = lp_linear_rate(a, eta); return true; }
+    double fx = (x - s.rate_min[0]) / s.rate_step[0];
+    int ix = int(fx);
+    double dx = fx - double(ix);
+    if (y < s.rate_min[1]) {
+        if (ix >= lma_pack::RATE_COLS - 1) return false;
+        double f = (1.0 - dx) * rt[ix] + dx * rt[ix + 1];
+        out = exp(f - s.rate_min[1]) * eta;
+        return true;
+    }
+    double fy = (y - s.rate_min[1]) / s.rate_step[1];
+    int iy = int(fy);
+    if (!(ix < lma_pack::RATE_COLS - 1 && iy < lma_pack::RATE_ROWS - 1)) return false;
+    double dy = fy - double(iy);
+    const double* t0 = rt + iy * lma_pack::RATE_COLS + ix;
+    const double* t1 = t0 + lma_pack::RATE_COLS;
+    double f = (1.0 - dx) * (1.0 - dy) * t0[0] + dx * (1.0 - dy) * t0[1] + (1.0 - dx) * dy * t1[0] + dx * dy * t1[1];
+    out = exp(f);
+    return true;
+}
+
+// ---------------------------------------------------------------- harmonic index (lp/mod.rs:465-553)
+__device__ inline double harmonic_from_table(const double* t16, double a, double frac) {
+    if (frac <= t16[1]) return 0.9;
+    double n = 0.0;
+    if (a < 5.0) {
+        pwmci_invert(t16, 16, frac, false, n);
+        return n;
+    }
+    // rescaled table: ln n against ln(-ln(1 - F)), first 15 points
+    double rs[30];
+    for (int i = 0; i < 15; ++i) {
+        rs[2 * i] = log(t16[2 * i]);
+        rs[2 * i + 1] = log(-log(1.0 - t16[2 * i + 1]));
+    }
+    pwmci_invert(rs, 15, log(-log(1.0 - frac)), true, n);
+    return exp(n);
+}
+__device__ inline int lp_harmonic(const double* cdf, const LmaScalars& sc, double a, double eta, double frac) {
+    double la = log(a), le = log(eta);
+    if (la <= sc.cdf_min[0]) return 1;
+    double fx = (la - sc.cdf_min[0]) / sc.cdf_step[0];
+    int ix = int(fx);
+    double dx = fx - double(ix);
+    double n;
+    if (le <= sc.cdf_min[1]) {
+        n = harmonic_from_table(cdf + 32 * ix, a, frac) * (1.0 - dx) + harmonic_from_table(cdf + 32 * (ix + 1), a, frac) * dx;
+    } else {
+        double fy = (le - sc.cdf_min[1]) / sc.cdf_step[1];
+        int iy = int(fy);
+        double dy = fy - double(iy);
+        const int C = lma_pack::CDF_COLS;
+        n = harmonic_from_table(cdf + 32 * (C * iy + ix), a, frac) * ((1.0 - dx) * (1.0 - dy)) +
+            harmonic_from_table(cdf + 32 * (C * iy + ix + 1), a, frac) * (dx * (1.0 - dy)) +
+            harmonic_from_table(cdf + 32 * (C * (iy + 1) + ix), a, frac) * ((1.0 - dx) * dy) +
+            harmonic_from_table(cdf + 32 * (C * (iy + 1) + ix + 1), a, frac) * (dx * dy);
+    }
+    return int(ceil(n));
+}
+
+// ---------------------------------------------------------------- generalised Bessel function J(n, x, y)
+// J(n, x, y) = sum_k J_{n+2k}(x) J_k(y) for the five orders n-2..n+2.
+//
+// Recurrence method (double_bessel.rs:142-246): a forward pass builds the
+// continued-fraction coefficients xi (3) and lambda (2) for every order k in
+// [k_min, k_min + len); a backward pass runs the two three-/two-term
+// recurrences f and g, which are matched at the turning point and normalised
+// by sum_k J = 1 (sign) and sum_k J^2 = 1 (magnitude). Only the forward
+// coefficients are stored, in per-lane scratch `ws` with stride `st` between
+// consecutive doubles of one lane (so a wave's accesses are contiguous); the
+// backward pass keeps running sums instead of the reference's stored f, g, h.
+struct DBessel {
+    int n, k_min, len;
+    double* ws;   // 5 * len doubles, element (i, c) at ws[(5 * i + c) * st]
+    size_t st;
+};
+__device__ __forceinline__ DBessel dbessel_at(int n, double* ws, size_t st) {
+    double xm = double(n) * 1.4142135623730951, ym = double(n) * 0.5;
+    int n_minus_bound = int(floor(-2.0 * ym - xm));
+    int n_plus_bound = xm > 4.0 * (2.0 - 1.4142135623730951) * ym ? int(ceil(xm)) : int(ceil(2.0 * ym + xm * xm / (16.0 * ym)));
+    return DBessel{n, n_minus_bound - 100, n_plus_bound - n_minus_bound + 200, ws, st};
+}
+__device__ __forceinline__ int dbessel_len(int n) {
+    double xm = double(n) * 1.4142135623730951, ym = double(n) * 0.5;
+    int lo = int(floor(-2.0 * ym - xm));
+    int hi = xm > 4.0 * (2.0 - 1.4142135623730951) * ym ? int(ceil(xm)) : int(ceil(2.0 * ym + xm * xm / (16.0 * ym)));
+    return hi - lo + 200;
+}
+
+__device__ inline void dbessel_recurrence(const DBessel& D, double x, double y, double j[5]) {
+    const int len = D.len;
+    const size_t st = D.st;
+    double* ws = D.ws;
+    auto W = [&](int i, int c) -> double& { return ws[(size_t(5) * i + c) * st]; };
+    // forward pass (element 0 holds the initial value 1 for all coefficients)
+    double x0 = 1.0, x1 = 1.0, x2 = 1.0, l0 = 1.0, l1 = 1.0;
+    W(0, 0) = 1.0; W(0, 1) = 1.0; W(0, 2) = 1.0; W(0, 3) = 1.0; W(0, 4) = 1.0;
+    for (int i = 1; i < len; ++i) {
+        int k = i + D.k_min;
+        double n0 = -x - 4.0 * y * y / x2;
+        double n1 = 2.0 * double(k - 1) - 2.0 * y * x0 / x2;
+        double n2 = -x - 2.0 * y * x1 / x2;
+        double m0 = n0 - 2.0 * y * n2 / l1;
+        double m1 = n1 - l0 * n2 / l1;
+        x0 = n0; x1 = n1; x2 = n2; l0 = m0; l1 = m1;
+        W(i, 0) = n0; W(i, 1) = n1; W(i, 2) = n2; W(i, 3) = m0; W(i, 4) = m1;
+    }
+    int n_minus = int(floor(-2.0 * y - x));
+    int n_plus = 8.0 * y > x ? int(ceil(2.0 * y + x * x / (16.0 * y))) : int(ceil(-2.0 * y + x));
+    int i_match = (n_plus + n_minus) / 2 - D.k_min;
+    int i_target = D.n - D.k_min;
+    double initial = D.n < 2000 ? 1.0e-100 : 1.0e-300;
+    // backward pass; f[i] from f[i+1..i+3], g[i] from g[i+1..i+2]
+    double f3 = 0.0, f2 = 1.0 * initial, f1 = 2.0 * initial;  // f[len-1], f[len-2], f[len-3]
+    double g2 = 0.0, g1 = 1.0 / initial;                       // g[len-1], g[len-2]
+    double g0 = -(2.0 * y * g2 + W(len - 2, 3) * g1) / W(len - 2, 4);  // g[len-3]
+    // tail values (i >= i_match) keep f; head values (i < i_match) keep g.
+    // Sums over the tail are accumulated in f units and rescaled with f.
+    double ft_sum = 0.0, ft_rss = 0.0;                  // sum f, sqrt(sum f^2) over i >= i_match
+    double gh_sum = 0.0, gh_rss = 0.0;                  // sum g, sqrt(sum g^2) over i < i_match
+    double tgt[5] = {0, 0, 0, 0, 0};
+    bool tgt_is_f[5] = {false, false, false, false, false};
+    double f_match = 0.0, g_match = 0.0;
+    bool have_match = false;
+    // the reference zeroes f[i_match] and g[i_match] before the pass; they are
+    // then recomputed in order, so the sentinels only matter for the rescaling test
+    auto visit_tail = [&](int i, double fv) {
+        ft_sum += fv;
+        ft_rss = hypot(ft_rss, fv);
+        int q = i - (i_target - 2);
+        if (q >= 0 && q < 5) { tgt[q] = fv; tgt_is_f[q] = true; }
+    };
+    // elements len-1, len-2, len-3 (f known), all >= i_match
+    visit_tail(len - 1, f3);
+    visit_tail(len - 2, f2);
+    visit_tail(len - 3, f1);
+    // g values at len-1.. are only used for i < i_match, never here
+    double gp2 = g1, gp1 = g0;  // g[i+2], g[i+1] as i descends from len-4
+    for (int i = len - 4; i >= 0; --i) {
+        double fi = -(2.0 * y * f3 + W(i + 2, 0) * f2 + W(i + 2, 1) * f1) / W(i + 2, 2);
+        double gi = -(2.0 * y * gp2 + W(i + 1, 3) * gp1) / W(i + 1, 4);
+        f3 = f2; f2 = f1; f1 = fi;
+        gp2 = gp1; gp1 = gi;
+        if (i >= i_match) {
+            if (i == i_match) { f_match = fi; g_match = gi; have_match = true; }
+            visit_tail(i, fi);
+        } else {
+            gh_sum += gi;
+            gh_rss = hypot(gh_rss, gi);
+            int q = i - (i_target - 2);
+            if (q >= 0 && q < 5) { tgt[q] = gi; tgt_is_f[q] = false; }
+        }
+        // rescaling (double_bessel.rs:199-204): f of elements >= i by 1e-100, g by 1e100
+        double gm = have_match ? g_match : 0.0;
+        if (fi > 1.0 && fabs(gm) < 1.0e200) {
+            f3 *= 1.0e-100; f2 *= 1.0e-100; f1 *= 1.0e-100;
+            gp2 *= 1.0e100; gp1 *= 1.0e100;
+            ft_sum *= 1.0e-100;
+            ft_rss *= 1.0e-100;
+            if (have_match) { f_match *= 1.0e-100; g_match *= 1.0e100; }
+            for (int q = 0; q < 5; ++q)
+                if (tgt_is_f[q]) tgt[q] *= 1.0e-100;
+            if (i < i_match) {  // head g values computed so far are also scaled in the reference
+                gh_sum *= 1.0e100;
+                gh_rss *= 1.0e100;
+                for (int q = 0; q < 5; ++q)
+                    if (!tgt_is_f[q] && (q + i_target - 2) >= i) tgt[q] *= 1.0e100;
+            }
+        }
+    }
+    // h = g for i < i_match, g_match f / f_match for i >= i_match
+    double ratio = g_match / f_match;
+    double sum_linear = gh_sum + ratio * ft_sum;
+    double rss = hypot(gh_rss, fabs(ratio) * ft_rss);
+    for (int q = 0; q < 5; ++q) {
+        double h = tgt_is_f[q] ? ratio * tgt[q] : tgt[q];
+        double val = copysign(1.0, h / sum_linear) * fabs(h) / rss;
+        j[q] = isfinite(val) ? val : 0.0;
+    }
+}
+
+// Saddle-point approximation of J(n, x, y) for large n (double_bessel.rs:272-355).
+__device__ inline double dbessel_spa(int ni, double x, double y) {
+    double n = double(ni);
+    if (n <= -2.0 * y + x) {
+        double cos_theta = y > 0.0 ? x / (8.0 * y) - sqrt(x * x / (64.0 * y * y) + 0.5 - n / (4.0 * y)) : n / x;
+        double theta = acos(cos_theta);
+        double st, ct;
+        sincos(theta, &st, &ct);
+        double s2 = 2.0 * st * ct;
+        double f = x * st - y * s2;
+        double d2f = -x * st + 4.0 * y * s2;
+        double phi = (PI_D / 4.0) * copysign(1.0, d2f);
+        return cos(f - n * theta + phi) * sqrt(2.0 / (PI_D * fabs(d2f)));
+    } else if (8.0 * y < x || n > 2.0 * y + x * x / (16.0 * y)) {
+        double disc = x * x / (64.0 * y * y) + 0.5 - n / (4.0 * y);
+        cplx cos_theta;
+        if (isnan(disc)) cos_theta = {n / x, 0.0};
+        else if (disc < 0.0) cos_theta = {x / (8.0 * y), sqrt(-disc)};
+        else cos_theta = {x / (8.0 * y) - sqrt(disc), 0.0};
+        cplx theta = c_acos(cos_theta);
+        cplx s1 = c_sin(theta), s2 = c_sin(2.0 * theta);
+        cplx f = mul_i(cplx{-n * theta.re + x * s1.re - y * s2.re, -n * theta.im + x * s1.im - y * s2.im});
+        cplx d2f = mul_i(cplx{-x * s1.re + 4.0 * y * s2.re, -x * s1.im + 4.0 * y * s2.im});
+        cplx phase = f + cplx{0.0, (PI_D - c_arg(d2f)) / 2.0};
+        double amp = sqrt(2.0 / (PI_D * c_abs(d2f)));
+        double re;
+        if (f.im == 0.0 && d2f.im == 0.0) re = 0.5 * exp(-f.re) * amp;
+        else re = c_exp(phase).re * amp;
+        return disc != 0.0 ? re : 0.0;
+    } else {
+        double r = sqrt(x * x / (64.0 * y * y) + 0.5 - n / (4.0 * y));
+        double out = 0.0;
+        for (int k = 0; k < 2; ++k) {
+            double theta = acos(x / (8.0 * y) + (k ? r : -r));
+            double st, ct;
+            sincos(theta, &st, &ct);
+            double s2 = 2.0 * st * ct;
+            double f = x * st - y * s2;
+            double d2f = -x * st + 4.0 * y * s2;
+            double phi = (PI_D / 4.0) * copysign(1.0, d2f);
+            out += cos(f - n * theta + phi) * sqrt(2.0 / (PI_D * fabs(d2f)));
+        }
+        return out;
+    }
+}
+__device__ inline bool dbessel_use_spa(int n, double x, double y) {
+    if (n <= 10) return false;
+    double ne = double(n);
+    ne = ne * (1.0 - sqrt(10.0 / ne));
+    double x_crit = y > ne / 6.0 ? 4.0 * sqrt(y * (ne - 2.0 * y)) : ne + 2.0 * y;
+    return x < x_crit;
+}
+__device__ inline void dbessel_eval(const DBessel& D, double x, double y, double j[5]) {
+    if (dbessel_use_spa(D.n, x, y))
+        for (int q = 0; q < 5; ++q) j[q] = dbessel_spa(D.n - 2 + q, x, y);
+    else
+        dbessel_recurrence(D, x, y, j);
+}
+
+// ---------------------------------------------------------------- LP double-differential rate (lp/mod.rs:21-140)
+__device__ inline double lp_ddpr(const DBessel& D, double a, double eta, double s, double theta) {
+    int n = D.n;
+    double sn = 2.0 * double(n) * eta / (1.0 + 0.5 * a * a);
+    double wn = fmin(s / (sn * (1.0 - s)), 1.0);
+    double alpha = 2.0 * double(n) * a * cos(theta) * sqrt(wn * (1.0 - wn)) / sqrt(1.0 + 0.5 * a * a);
+    double beta = a * a * s / (8.0 * eta * (1.0 - s));
+    double j[5];
+    dbessel_eval(D, alpha, beta, j);
+    double g0, g1, g2;
+    if (n % 2 == 0) { g0 = j[2]; g1 = -0.5 * (j[1] + j[3]); g2 = 0.25 * (j[0] + 2.0 * j[2] + j[4]); }
+    else { g0 = -j[2]; g1 = 0.5 * (j[1] + j[3]); g2 = -0.25 * (j[0] + 2.0 * j[2] + j[4]); }
+    return (-g0 * g0 - a * a * (1.0 + 0.5 * s * s / (1.0 - s)) * (g0 * g2 - g1 * g1)) / (2.0 * PI_D);
+}
+__device__ inline int lp_sum_limit(double a, double eta) {
+    double m = 3.0 - 0.4 * pow(eta, 0.25);
+    return int(ceil(6.0 * (1.0 + a * a) + 2.0 * pow(a, m)));
+}
+__device__ inline double lp_ceiling(const DBessel& D, double a, double eta) {
+    int n = D.n;
+    double sn = 2.0 * double(n) * eta / (1.0 + 0.5 * a * a);
+    double smax = sn / (1.0 + sn);
+    double mx = -INFINITY;
+    if (n == 1) {
+        mx = fmax(lp_ddpr(D, a, eta, 0.01 * smax, PI_D / 2.0), lp_ddpr(D, a, eta, 0.99 * smax, PI_D / 2.0));
+    } else if (a <= 2.0) {
+        for (int i = 1; i <= 32; ++i) mx = fmax(mx, lp_ddpr(D, a, eta, smax * (0.4 + 0.6 * double(i) / 32.0), 0.0));
+    } else if (a <= 5.0) {
+        int n_switch = int(ceil(1.3 * pow(a, 1.2)));
+        if (n < n_switch) {
+            double theta = (PI_D / 2.0) * (1.0 - log(double(n)) / log(double(n_switch)));
+            for (int i = 0; i <= 32; ++i) mx = fmax(mx, lp_ddpr(D, a, eta, smax * double(i) / 32.0, theta));
+        } else {
+            double x_min = fmax(4.0 * double(n) * eta / (3.0 * a * a + 4.0 * double(n) * eta), 0.4);
+            int m = n < lp_sum_limit(a, eta) / 2 ? 32 : 64;
+            for (int i = 1; i <= m; ++i) mx = fmax(mx, lp_ddpr(D, a, eta, smax * (x_min + (1.0 - x_min) * double(i) / double(m)), 0.0));
+        }
+    } else {
+        int n_switch = int(ceil(a * a));
+        if (n < n_switch) {
+            double ns = double(n_switch);
+            double theta = (PI_D / 2.0) * (pow(double(n), -0.7) - pow(ns, -0.7)) / (1.0 - pow(ns, -0.7));
+            for (int i = 1; i <= 64; ++i) mx = fmax(mx, lp_ddpr(D, a, eta, smax * (0.3 + 0.7 * double(i) / 64.0), theta));
+        } else {
+            double x_min = fmax(4.0 * double(n) * eta / (3.0 * a * a + 4.0 * double(n) * eta), 0.3);
+            double x_peak = (1.0 + sn) / (2.0 + sn);
+            int len = n > 10 * n_switch ? 64 : (n > 3 * n_switch ? 42 : 32);
+            for (int i = 1; i <= len; ++i)
+                mx = fmax(mx, lp_ddpr(D, a, eta, smax * (x_min + (x_peak - x_min) * double(i) / double(len)), 0.0));
+        }
+    }
+    return 1.15 * mx;
+}
+
+// Largest theta contributing at each s (lp/mod.rs:143-301).
+struct ThetaBound { double s[16], f[16]; };
+__device__ inline double ln_dbessel_x_zero(int ni, double y) {
+    double n = double(ni);
+    double z = sqrt(n / (4.0 * y) - 0.5);
+    cplx theta{PI_D / 2.0, log(sqrt(1.0 + z * z) - z)};
+    cplx s2 = c_sin(2.0 * theta);
+    cplx f = mul_i(cplx{-n * theta.re - y * s2.re, -n * theta.im - y * s2.im});
+    cplx f2 = mul_i(4.0 * y * s2);
+    double e = 0.5 * (PI_D - c_arg(f2));
+    cplx phase{f.re, f.im + e};
+    return log(sqrt(2.0 / (PI_D * c_abs(f2)))) + phase.re + log(fabs(cos(phase.im)));
+}
+__device__ inline ThetaBound theta_bound(int n, double a, double eta) {
+    ThetaBound tb;
+    double sn = 2.0 * double(n) * eta / (1.0 + 0.5 * a * a);
+    double ln_j_crit = log(0.443) - log(double(n)) / 3.0;
+    for (int i = 0; i < 16; ++i) {
+        double z = cos(PI_D * double(i) / 15.0);
+        double s = 0.5 * (z + 1.0) * sn / (1.0 + sn);
+        tb.s[i] = s;
+        double wn = fmin(s / (sn * (1.0 - s)), 1.0);
+        double x = 2.0 * double(n) * a * sqrt(wn * (1.0 - wn)) / sqrt(1.0 + 0.5 * a * a);
+        double y = a * a * s / (8.0 * eta * (1.0 - s));
+        double x_crit = y > double(n) / 6.0 ? 4.0 * sqrt(y) * sqrt(double(n) - 2.0 * y) : double(n) + 2.0 * y;
+        double ln_j_bdy = fmin(ln_dbessel_x_zero(n % 2 == 0 ? n : n + 1, y), ln_j_crit);
+        double cos_theta = fmax(1.0 - x_crit * log(0.01) / (x * (ln_j_bdy - ln_j_crit)), -1.0);
+        double f = acos(cos_theta);
+        tb.f[i] = isnan(f) ? PI_D : f;
+    }
+    double f_min = tb.f[0];
+    for (int i = 1; i < 16; ++i) {
+        if (tb.f[i] < f_min) f_min = tb.f[i];
+        else tb.f[i] = f_min;
+    }
+    return tb;
+}
+__device__ __forceinline__ double theta_bound_at(const ThetaBound& tb, double arg) {
+    double val = tb.f[15];
+    for (int i = 1; i < 16; ++i) {
+        if (arg > tb.s[i]) {
+            double w = (arg - tb.s[i - 1]) / (tb.s[i] - tb.s[i - 1]);
+            val = w * tb.f[i] + (1.0 - w) * tb.f[i - 1];
+            break;
+        }
+    }
+    return fmin(val, PI_D / 2.0);
+}
+
+// (n, s, theta, Stokes) of one LP emission (lp/mod.rs:586-676). `trial_cap`
+// bounds the rejection loop; hitting it sets `capped` and keeps the last trial.
+__device__ inline void lp_sample(const double* cdf, const LmaScalars& sc, double a, double eta, Rng& rng, double* ws, size_t st,
+                                 int& n_out, double& s_out, double& theta_out, sv4& pol, uint32_t& trials, bool& capped,
+                                 int fixed_n = 0) {
+    int n = fixed_n > 0 ? fixed_n : lp_harmonic(cdf, sc, a, eta, rng_uniform(rng));
+    DBessel D = dbessel_at(n, ws, st);
+    ThetaBound tb = theta_bound(n, a, eta);
+    double sn = 2.0 * double(n) * eta / (1.0 + 0.5 * a * a);
+    double smax = sn / (1.0 + sn);
+    double mx = lp_ceiling(D, a, eta);
+    double s = 0.5 * smax, theta = 0.0;
+    const uint32_t cap = 1u << 20;
+    for (uint32_t t = 0;; ++t) {
+        double u1, u2;
+        rng_uniform2(rng, u1, u2);
+        ++trials;
+        s = smax * u1;
+        theta = (PI_D / 2.0) * u2;
+        if (t >= cap) { capped = true; break; }
+        if (theta > theta_bound_at(tb, s)) continue;
+        double z = mx * rng_uniform(rng);
+        if (z < lp_ddpr(D, a, eta, s, theta)) break;
+    }
+    double uq = rng_uniform(rng);
+    int quadrant = int(uq * 4.0);
+    if (quadrant == 1) theta = PI_D - theta;
+    else if (quadrant == 2) theta = PI_D + theta;
+    else if (quadrant == 3) theta = 2.0 * PI_D - theta;
+    double wn = fmin(s / (sn * (1.0 - s)), 1.0);
+    double x = -2.0 * double(n) * a * cos(theta) * sqrt(wn * (1.0 - wn)) / sqrt(1.0 + 0.5 * a * a);
+    double y = a * a * s / (8.0 * eta * (1.0 - s));
+    double j[5];
+    dbessel_eval(D, fabs(x), y, j);
+    double A0, A1, A2;
+    if (x > 0.0) { A0 = j[2]; A1 = 0.5 * (j[1] + j[3]); A2 = 0.25 * (j[0] + 2.0 * j[2] + j[4]); }
+    else if (n % 2 == 0) { A0 = j[2]; A1 = -0.5 * (j[1] + j[3]); A2 = 0.25 * (j[0] + 2.0 * j[2] + j[4]); }
+    else { A0 = -j[2]; A1 = 0.5 * (j[1] + j[3]); A2 = -0.25 * (j[0] + 2.0 * j[2] + j[4]); }
+    double u = sqrt(2.0 * double(n) * eta * (1.0 - s) / s - (1.0 + 0.5 * a * a));
+    double st_, ct_;
+    sincos(theta, &st_, &ct_);
+    double A0a = A0 / a;
+    double xi0 = (1.0 - s + 1.0 / (1.0 - s)) * (A1 * A1 - A0 * A2) - 2.0 * A0a * A0a;
+    double xi1 = 2.0 * (A1 * A1 - A0 * A2) - 2.0 * (1.0 + 2.0 * u * u * st_ * st_) * A0a * A0a;
+    double xi2 = -(2.0 * (u * A0a) * (u * A0a) * (2.0 * st_ * ct_) + 4.0 * u * A0 * A1 * st_ / a);
+    n_out = n;
+    s_out = s;
+    theta_out = theta;
+    pol = {1.0, xi1 / xi0, xi2 / xi0, 0.0};
+}
+
+// ---------------------------------------------------------------- photon from the ZMF (nonlinear_compton/mod.rs:58-138)
+__device__ __forceinline__ d4 boost4(d4 s, d4 u) {  // four_vector.rs:68-80
+    double gamma = u.t;
+    double beta = sqrt(1.0 - 1.0 / (gamma * gamma));
+    d3 n = unit(sp(u));
+    d3 z = sp(s);
+    double nz = dot(n, z);
+    double c1 = (gamma - 1.0) * nz - gamma * beta * s.t;
+    return {gamma * (s.t - beta * nz), z.x + c1 * n.x, z.y + c1 * n.y, z.z + c1 * n.z};
+}
+
+// Stokes parameters from the LMA basis to the global basis (polarization.rs:104-157).
+__device__ __forceinline__ sv4 stokes_from_lma(sv4 s, d4 k) {
+    double k_minus = k.t - k.z;
+    double cos_t = (k.z * k_minus - k.x * k.x) / (k_minus * hypot(k.x, k.z));
+    double sign = signbit(k.x * k.y) ? -1.0 : 1.0;
+    cos_t = fmin(fmax(cos_t, -1.0), 1.0);
+    double sin_t = -(sign * sqrt(1.0 - cos_t * cos_t));
+    double c2 = cos_t * cos_t - sin_t * sin_t, s2 = 2.0 * cos_t * sin_t;
+    return {s.i, c2 * s.q - s2 * s.u, s2 * s.q + c2 * s.u, s.v};
+}
+__device__ __forceinline__ sv4 stokes_rotate(sv4 s, double theta) {  // polarization.rs:44-53
+    double s2, c2;
+    sincos(2.0 * theta, &s2, &c2);
+    return {s.i, c2 * s.q + s2 * s.u, -s2 * s.q + c2 * s.u, s.v};
+}
+
+}  // namespace ptg

@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "kern_lma.h"
 #include "kern_misc.h"
 
 using namespace ptg;
@@ -74,7 +75,7 @@ void free_buf(DevBuf& b) {
 }
 
 // Counter block layout (unsigned long long)
-enum { C_WORK = 0, C_NEXT, C_OUT, C_INTER, C_ID, C_FLAGS, C_STATS, C_N = C_STATS + STAT_N };
+enum { C_WORK = 0, C_NEXT, C_OUT, C_INTER, C_ID, C_FLAGS, C_EV, C_LNEXT, C_PH, C_STATS, C_N = C_STATS + STAT_N };
 
 std::string default_table_path() {
     Dl_info info;
@@ -97,6 +98,14 @@ struct ptg_ctx {
     std::map<std::string, std::vector<double>> tables;
     double* d_lcfa = nullptr;
     LcfaScalars sc{};
+    double* d_lma = nullptr;
+    LmaScalars lsc{};
+    double lma_bandwidth = 0.0, pol_angle = 0.0;
+    DevBuf evq, lq[2], phq;
+    double* d_ev_extra = nullptr;  // ev_k0, ev_a
+    size_t ev_extra_cap = 0;
+    double* d_scratch = nullptr;
+    size_t scratch_bytes = 0;
     LaserDev laser{};
     bool have_laser = false;
     DevBuf prim, q[2], out, inter, tmp;
@@ -205,15 +214,44 @@ bool build_lcfa_pack(ptg_ctx* c, std::vector<double>& pack) {
     return true;
 }
 
-// Laser constants (fast_focused_laser.rs:24-38,47-53,129-169; fast_plane_wave.rs:23-108).
+bool build_lma_pack(ptg_ctx* c, std::vector<double>& pack) {
+    pack.assign(lma_pack::SIZE, 0.0);
+    auto need = [&](const char* name, size_t n) -> const std::vector<double>* {
+        auto it = c->tables.find(name);
+        if (it == c->tables.end() || it->second.size() != n) {
+            c->err = std::string("table ") + name + " missing or wrong size";
+            return nullptr;
+        }
+        return &it->second;
+    };
+    const auto* rate = need("nonlinear_compton/lp/rate_table:TABLE", lma_pack::RATE_COLS * lma_pack::RATE_ROWS);
+    const auto* cdf = need("nonlinear_compton/lp/cdf_table:TABLE", size_t(lma_pack::CDF_N) * 32);
+    const auto* rmin = need("nonlinear_compton/lp/rate_table:MIN", 2);
+    const auto* rstep = need("nonlinear_compton/lp/rate_table:STEP", 2);
+    const auto* cmin = need("nonlinear_compton/lp/cdf_table:MIN", 2);
+    const auto* cstep = need("nonlinear_compton/lp/cdf_table:STEP", 2);
+    if (!rate || !cdf || !rmin || !rstep || !cmin || !cstep) return false;
+    std::copy(rate->begin(), rate->end(), pack.begin() + lma_pack::RATE);
+    std::copy(cdf->begin(), cdf->end(), pack.begin() + lma_pack::CDF);
+    for (int k = 0; k < 2; ++k) {
+        c->lsc.rate_min[k] = (*rmin)[k];
+        c->lsc.rate_step[k] = (*rstep)[k];
+        c->lsc.cdf_min[k] = (*cmin)[k];
+        c->lsc.cdf_step[k] = (*cstep)[k];
+    }
+    return true;
+}
+
+// Laser constants (fast_focused_laser.rs:24-38,47-53,129-169; fast_plane_wave.rs:23-108;
+// plane_wave.rs:27-63,176-192).
 int make_laser(ptg_ctx* c, const ptg_laser_desc* d, LaserDev& L) {
     if (d->kind < 0 || d->kind > 3) return fail(c, PTG_E_ARG, "laser kind out of range");
     if (d->envelope < 0 || d->envelope > 2) return fail(c, PTG_E_ARG, "envelope out of range");
     if (!(d->a0 > 0.0) || !(d->wavelength > 0.0) || !(d->n_cycles > 0.0)) return fail(c, PTG_E_ARG, "a0, wavelength and n_cycles must be positive");
     if ((d->kind == PTG_FOCUSED_LASER || d->kind == PTG_FAST_FOCUSED_LASER) && !(d->waist > 0.0))
         return fail(c, PTG_E_ARG, "focused lasers need a positive waist");
-    if (d->kind == PTG_PLANE_WAVE || d->kind == PTG_FOCUSED_LASER)
-        return fail(c, PTG_E_UNSUPPORTED, "LMA fields (PlaneWave, FocusedLaser) are not built into this library yet");
+    if (d->kind == PTG_FOCUSED_LASER)
+        return fail(c, PTG_E_UNSUPPORTED, "FocusedLaser (LMA, focused) is not built into this library yet");
     L = LaserDev{};
     L.kind = d->kind;
     L.envelope = d->envelope;
@@ -240,11 +278,14 @@ int make_laser(ptg_ctx* c, const ptg_laser_desc* d, LaserDev& L) {
         double tau = L.omega * duration;
         L.gauss_c = 2.0 * LN2_D / (tau * tau);
         L.dt = multiplier / L.omega;
-    } else {
+    } else if (d->kind == PTG_FAST_PLANE_WAVE) {
         L.n_cycles = d->n_cycles;
         L.amplitude = (M_ELECTRON * C_LIGHT_SQD * L.k0 * d->a0) / E_CHARGE;
         double chirp = 1.0 + 2.0 * d->chirp_b * PI_H * d->n_cycles;
         L.dt = multiplier / (C_LIGHT * L.k0 * chirp);
+    } else {  // PlaneWave (LMA)
+        L.n_cycles = d->n_cycles;
+        L.dt = (d->envelope == PTG_FLATTOP ? 0.2 : 1.0) / (C_LIGHT * L.k0);
     }
     double nc = L.n_cycles;
     L.max_phase = d->envelope == PTG_COS_SQUARED ? PI_H * nc : (d->envelope == PTG_FLATTOP ? PI_H * (nc + 1.0) : 6.0 * PI_H * nc);
@@ -291,7 +332,11 @@ int ptg_create(ptg_ctx** out, const char* table_path, int device) {
     if (!load_blob(path, c->tables, c->err)) { *out = c; return PTG_E_TABLES; }
     std::vector<double> pack;
     if (!build_lcfa_pack(c, pack)) { *out = c; return PTG_E_TABLES; }
+    std::vector<double> lpack;
+    if (!build_lma_pack(c, lpack)) { *out = c; return PTG_E_TABLES; }
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(c, hipMalloc(&c->d_lma, lpack.size() * sizeof(double)));
+    HIPCHK(c, hipMemcpy(c->d_lma, lpack.data(), lpack.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMalloc(&c->d_lcfa, pack.size() * sizeof(double)));
     HIPCHK(c, hipMemcpy(c->d_lcfa, pack.data(), pack.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMalloc(&c->d_ctr, C_N * sizeof(unsigned long long)));
@@ -308,6 +353,10 @@ void ptg_destroy(ptg_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_buf(c->prim); free_buf(c->q[0]); free_buf(c->q[1]); free_buf(c->out); free_buf(c->inter); free_buf(c->tmp);
     if (c->d_lcfa) (void)hipFree(c->d_lcfa);
+    if (c->d_lma) (void)hipFree(c->d_lma);
+    if (c->d_ev_extra) (void)hipFree(c->d_ev_extra);
+    if (c->d_scratch) (void)hipFree(c->d_scratch);
+    free_buf(c->evq); free_buf(c->lq[0]); free_buf(c->lq[1]); free_buf(c->phq);
     if (c->d_n0) (void)hipFree(c->d_n0);
     if (c->d_shower_count) (void)hipFree(c->d_shower_count);
     if (c->d_ctr) (void)hipFree(c->d_ctr);
@@ -327,6 +376,12 @@ int ptg_set_laser(ptg_ctx* c, const ptg_laser_desc* d) {
     if (rc) return rc;
     c->laser = L;
     c->have_laser = true;
+    c->pol_angle = d->polarization == PTG_CIRCULAR ? 0.0 : d->pol_angle;
+    c->lma_bandwidth = 0.0;
+    if (d->finite_bandwidth) {  // plane_wave.rs:49-63
+        double n_fwhm = d->envelope == PTG_COS_SQUARED ? 0.36405666377387671305 * d->n_cycles : d->n_cycles;
+        c->lma_bandwidth = std::sqrt(0.5 * LN2_D) / (PI_H * n_fwhm);
+    }
     return PTG_OK;
 }
 
@@ -490,12 +545,148 @@ static int transport(ptg_ctx* c, TransportArgs a, bool photons, DevBuf& in, size
     return fail(c, PTG_E_OOM, "secondary queue kept overflowing");
 }
 
+// Largest harmonic the LP CDF table can return at peak amplitude a0, with margin
+// (lp/mod.rs:414-419 at eta -> 0, where the harmonic sum is longest).
+static int lma_harmonic_cap(double a0) {
+    return int(std::ceil(1.5 * (6.0 * (1.0 + a0 * a0) + 2.0 * a0 * a0 * a0))) + 16;
+}
+
+static int ensure_ev_extra(ptg_ctx* c, size_t n) {
+    if (n <= c->ev_extra_cap) return PTG_OK;
+    if (c->d_ev_extra) (void)hipFree(c->d_ev_extra);
+    c->d_ev_extra = nullptr;
+    HIPCHK(c, hipMalloc(&c->d_ev_extra, 2 * n * sizeof(double)));
+    c->ev_extra_cap = n;
+    return PTG_OK;
+}
+
+// The LMA loop: step kernel over the live electrons, then the event kernel over
+// the ones that emitted, until every electron has left the field. Photons of the
+// primary beam are handed to the photon pass-through at the end.
+static int run_lma(ptg_ctx* c, TransportArgs a, double* ms, uint64_t* launches, uint64_t* gens) {
+    const size_t n = c->n_prim;
+    int len_cap = 0;
+    {
+        // workspace per lane for J(n, x, y), 5 doubles per order
+        int ncap = lma_harmonic_cap(c->laser.a0);
+        double xm = ncap * 1.4142135623730951, ym = ncap * 0.5;
+        len_cap = int(std::ceil(xm)) - int(std::floor(-2.0 * ym - xm)) + 200;
+    }
+    const size_t budget = size_t(4) << 30;
+    size_t max_lanes = std::max<size_t>(64, std::min<size_t>(65536, budget / (size_t(len_cap) * 40)));
+    max_lanes = (max_lanes / 64) * 64;
+    size_t need = max_lanes * size_t(len_cap) * 5 * sizeof(double);
+    if (need > c->scratch_bytes) {
+        if (c->d_scratch) (void)hipFree(c->d_scratch);
+        c->d_scratch = nullptr;
+        HIPCHK(c, hipMalloc(&c->d_scratch, need));
+        c->scratch_bytes = need;
+    }
+    LmaArgs m{};
+    m.tables = c->d_lma;
+    m.sc = c->lsc;
+    m.bandwidth = c->lma_bandwidth;
+    m.pol_angle = c->pol_angle;
+    m.sin_pa = std::sin(c->pol_angle);
+    m.cos_pa = std::cos(c->pol_angle);
+    m.ev_ctr = c->d_ctr + C_EV;
+    m.next_ctr = c->d_ctr + C_LNEXT;
+    m.ph_ctr = c->d_ctr + C_PH;
+    m.photons_to_out = 1;
+    m.scratch = c->d_scratch;
+    m.scratch_len = len_cap;
+
+    if (c->phq.cap < n) HIPCHK(c, alloc_buf(c->phq, n));  // photons of the primary beam
+    DevBuf* cur = &c->prim;
+    size_t n_cur = n;
+    int li = 0;
+    const unsigned long long zero = 0;
+    HIPCHK(c, hipMemcpy(c->d_ctr + C_NEXT, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    while (n_cur > 0) {
+        // capacities: each live electron yields at most one final state or one event,
+        // each event at most one photon and one continuing electron
+        HIPCHK(c, grow_buf(c->out, c->n_out + 2 * n_cur + 1024, c->n_out, c->stream));
+        if (c->evq.cap < n_cur) HIPCHK(c, alloc_buf(c->evq, n_cur + n_cur / 4));
+        int rc = ensure_ev_extra(c, c->evq.cap);
+        if (rc) return rc;
+        DevBuf& nxt = c->lq[li];
+        if (&nxt == cur) return fail(c, PTG_E_STATE, "internal queue aliasing");
+        if (nxt.cap < n_cur) HIPCHK(c, alloc_buf(nxt, n_cur + n_cur / 4));
+        HIPCHK(c, hipMemcpyAsync(c->d_ctr + C_WORK, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_ctr + C_EV, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_ctr + C_LNEXT, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
+        a.in = cur->v;
+        a.n_in = uint32_t(n_cur);
+        a.out = c->out.v;
+        a.inter = c->inter.v;
+        a.next = c->phq.v;  // photons forwarded by the step kernel
+        a.next_cap = c->phq.cap;
+        m.ev = c->evq.v;
+        m.ev_cap = c->evq.cap;
+        m.ev_k0 = c->d_ev_extra;
+        m.ev_a = c->d_ev_extra + c->ev_extra_cap;
+        HIPCHK(c, hipMemcpyAsync(c->d_args, &a, sizeof(a), hipMemcpyHostToDevice, c->stream));
+        int grid = int(std::min<size_t>(size_t(c->n_cu) * 4, (n_cur + 255) / 256));
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+        launch_step_lma(c->d_args, m, grid, c->stream);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+        unsigned long long n_ev = 0;
+        HIPCHK(c, hipMemcpyAsync(&n_ev, c->d_ctr + C_EV, sizeof(n_ev), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        float t = 0.0f;
+        (void)hipEventElapsedTime(&t, c->ev0, c->ev1);
+        ms[0] += t;
+        ++launches[0];
+        if (n_ev > 0) {
+            m.ev_in = c->evq.v;
+            m.n_ev = n_ev;
+            m.next = nxt.v;
+            size_t lanes = std::min<size_t>(max_lanes, ((n_ev + 63) / 64) * 64);
+            HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+            launch_event_lma(c->d_args, m, int(lanes), c->stream);
+            HIPCHK(c, hipGetLastError());
+            HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            (void)hipEventElapsedTime(&t, c->ev0, c->ev1);
+            ms[1] += t;
+            ++launches[1];
+        }
+        unsigned long long now[C_N];
+        HIPCHK(c, hipMemcpy(now, c->d_ctr, sizeof(now), hipMemcpyDeviceToHost));
+        c->n_out = now[C_OUT];
+        cur = &nxt;
+        n_cur = size_t(now[C_LNEXT]);
+        li ^= 1;
+        ++*gens;
+    }
+    // photons of the primary beam: not tracked in the LMA path, final cut only
+    unsigned long long n_ph = 0;
+    HIPCHK(c, hipMemcpy(&n_ph, c->d_ctr + C_NEXT, sizeof(n_ph), hipMemcpyDeviceToHost));
+    if (n_ph > 0) {
+        TransportArgs b = a;
+        b.laser.kind = K_FAST_PLANE_WAVE;  // the pass-through never evaluates fields
+        b.opt.tracking_photons = 0;
+        double pms = 0.0;
+        uint64_t pl = 0;
+        int rc = transport(c, b, true, c->phq, size_t(n_ph), c->q[0], &pms, &pl);
+        if (rc) return rc;
+    }
+    return PTG_OK;
+}
+
 int ptg_run(ptg_ctx* c, const ptg_collide_options* o, uint64_t seed, ptg_counters* cnt) {
     if (!c || !o) return fail(c, PTG_E_ARG, "null argument");
     if (!c->have_laser) return fail(c, PTG_E_STATE, "ptg_set_laser must be called first");
     if (c->n_prim == 0) return fail(c, PTG_E_STATE, "no primaries resident (ptg_load_primaries / ptg_build_beam)");
     int rc = validate_options(c, o);
     if (rc) return rc;
+    if (c->laser.kind == K_PLANE_WAVE) {
+        if (o->tracking_photons)
+            return fail(c, PTG_E_UNSUPPORTED, "photon tracking (LMA Breit-Wheeler pair creation) in a PlaneWave is not built into this library yet");
+        if (o->classical) return fail(c, PTG_E_UNSUPPORTED, "classical LMA emission is not built into this library");
+        if (c->laser.circular) return fail(c, PTG_E_UNSUPPORTED, "circularly polarised LMA emission is not built into this library yet");
+    }
     HIPCHK(c, hipSetDevice(c->device));
     const size_t n = c->n_prim;
 
@@ -548,27 +739,32 @@ int ptg_run(ptg_ctx* c, const ptg_collide_options* o, uint64_t seed, ptg_counter
     c->n_out = 0;
     c->n_inter = 0;
 
-    // Generation loop: primaries and pairs are electrons/positrons (or photons,
-    // for photon beams); every kernel consumes one queue and fills the other.
     double ms[2] = {0.0, 0.0};  // lepton, photon kernels
     uint64_t launches[2] = {0, 0};
     uint64_t gens = 0;
-    DevBuf* cur = &c->prim;
-    size_t n_cur = n;
-    bool photon_stage = false;  // leptons first; each stage forwards the other kind
-    int qi = 0;
-    while (n_cur > 0) {
-        DevBuf& next = c->q[qi];
-        if (&next == cur) return fail(c, PTG_E_STATE, "internal queue aliasing");
-        rc = transport(c, a, photon_stage, *cur, n_cur, next, &ms[photon_stage], &launches[photon_stage]);
+    if (c->laser.kind == K_PLANE_WAVE) {
+        rc = run_lma(c, a, ms, launches, &gens);
         if (rc) return rc;
-        unsigned long long nn = 0;
-        HIPCHK(c, hipMemcpy(&nn, c->d_ctr + C_NEXT, sizeof(nn), hipMemcpyDeviceToHost));
-        cur = &next;
-        n_cur = size_t(nn);
-        qi ^= 1;
-        photon_stage = !photon_stage;
-        ++gens;
+    } else {
+        // Generation loop: primaries and pairs are electrons/positrons (or photons,
+        // for photon beams); every kernel consumes one queue and fills the other.
+        DevBuf* cur = &c->prim;
+        size_t n_cur = n;
+        bool photon_stage = false;  // leptons first; each stage forwards the other kind
+        int qi = 0;
+        while (n_cur > 0) {
+            DevBuf& next = c->q[qi];
+            if (&next == cur) return fail(c, PTG_E_STATE, "internal queue aliasing");
+            rc = transport(c, a, photon_stage, *cur, n_cur, next, &ms[photon_stage], &launches[photon_stage]);
+            if (rc) return rc;
+            unsigned long long nn = 0;
+            HIPCHK(c, hipMemcpy(&nn, c->d_ctr + C_NEXT, sizeof(nn), hipMemcpyDeviceToHost));
+            cur = &next;
+            n_cur = size_t(nn);
+            qi ^= 1;
+            photon_stage = !photon_stage;
+            ++gens;
+        }
     }
 
     if (a.opt.multiplicity_on && c->n_out) {
@@ -650,29 +846,38 @@ int ptg_fetch(ptg_ctx* c, ptg_particles* sec, ptg_particles* inter) {
 }
 
 int ptg_push(ptg_ctx* c, const uint8_t* species, double* r, double* u, size_t n, uint32_t n_steps, const ptg_collide_options* o,
-             uint32_t* steps_taken) {
+             uint32_t* steps_taken, double* work) {
     if (!c || !species || !r || !u || !o) return fail(c, PTG_E_ARG, "null argument");
     if (!c->have_laser) return fail(c, PTG_E_STATE, "ptg_set_laser must be called first");
     if (n == 0) return PTG_OK;
     HIPCHK(c, hipSetDevice(c->device));
     uint8_t* ds = nullptr;
-    double *dr = nullptr, *du = nullptr;
+    double *dr = nullptr, *du = nullptr, *dw = nullptr;
     uint32_t* dk = nullptr;
     HIPCHK(c, hipMalloc(&ds, n));
     HIPCHK(c, hipMalloc(&dr, n * 32));
     HIPCHK(c, hipMalloc(&du, n * 32));
     HIPCHK(c, hipMalloc(&dk, n * 4));
+    HIPCHK(c, hipMalloc(&dw, n * 8));
     HIPCHK(c, hipMemcpy(ds, species, n, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(dr, r, n * 32, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(du, u, n * 32, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(dw, 0, n * 8));
     int rr_mode = (o->classical && o->rr) ? (o->gaunt_factor ? 2 : 1) : 0;
-    launch_push_lcfa(c->laser, c->laser.dt * o->dt_multiplier, rr_mode, ds, dr, du, uint32_t(n), n_steps, dk, c->stream);
+    double dt = c->laser.dt * o->dt_multiplier;
+    if (c->laser.kind == K_PLANE_WAVE) {
+        if (rr_mode == 2) return fail(c, PTG_E_ARG, "Gaunt factor correction is unavailable in LMA mode");
+        launch_push_lma(c->laser, dt, rr_mode, ds, dr, du, uint32_t(n), n_steps, dk, dw, c->stream);
+    } else {
+        launch_push_lcfa(c->laser, dt, rr_mode, ds, dr, du, uint32_t(n), n_steps, dk, dw, c->stream);
+    }
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(r, dr, n * 32, hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(u, du, n * 32, hipMemcpyDeviceToHost));
     if (steps_taken) HIPCHK(c, hipMemcpy(steps_taken, dk, n * 4, hipMemcpyDeviceToHost));
-    (void)hipFree(ds); (void)hipFree(dr); (void)hipFree(du); (void)hipFree(dk);
+    if (work) HIPCHK(c, hipMemcpy(work, dw, n * 8, hipMemcpyDeviceToHost));
+    (void)hipFree(ds); (void)hipFree(dr); (void)hipFree(du); (void)hipFree(dk); (void)hipFree(dw);
     return PTG_OK;
 }
 
@@ -729,12 +934,38 @@ int ptg_histogram(ptg_ctx* c, ptg_hist_spec* s, double* bins_out, double* total_
 int ptg_eval(ptg_ctx* c, int fn, const double* in, size_t n, size_t in_w, double* out, size_t out_w) {
     if (!c || !in || !out) return fail(c, PTG_E_ARG, "null argument");
     if (fn < 0 || fn >= PTG_FN_COUNT) return fail(c, PTG_E_ARG, "unknown function id");
-    static const int need_in[] = {2, 5, 1, 1, 4, 1, 3, 12, 2, 3, 3, 3};
-    static const int need_out[] = {1, 4, 2, 4, 7, 1, 1, 4, 1, 1, 1, 8};
+    static const int need_in[] = {2, 5, 1, 1, 4, 1, 3, 12, 2, 3, 4, 4};
+    static const int need_out[] = {1, 4, 2, 4, 7, 1, 1, 4, 2, 1, 5, 6};
     if (int(in_w) < need_in[fn] || int(out_w) < need_out[fn]) return fail(c, PTG_E_ARG, "row width too small for this function");
     if (fn == PTG_FN_FIELDS && !c->have_laser) return fail(c, PTG_E_STATE, "no laser set");
-    if (fn >= PTG_FN_LP_RATE) return fail(c, PTG_E_UNSUPPORTED, "LMA probes are not built into this library yet");
     if (n == 0) return PTG_OK;
+    if (fn >= PTG_FN_LP_RATE) {
+        HIPCHK(c, hipSetDevice(c->device));
+        int nmax = 16;
+        if (fn == PTG_FN_DOUBLE_BESSEL || fn == PTG_FN_LP_SAMPLE) {
+            for (size_t i = 0; i < n; ++i) {
+                const double* x = in + i * in_w;
+                int nn = fn == PTG_FN_DOUBLE_BESSEL ? int(x[0]) : (x[3] > 0 ? int(x[3]) : lma_harmonic_cap(x[0] / 1.4142135623730951));
+                if (fn == PTG_FN_DOUBLE_BESSEL && (nn < 3 || nn > 200000)) return fail(c, PTG_E_ARG, "J(n, x, y) probe needs 3 <= n <= 2e5");
+                nmax = std::max(nmax, nn);
+            }
+        }
+        double xm = nmax * 1.4142135623730951, ym = nmax * 0.5;
+        size_t len = size_t(int(std::ceil(xm)) - int(std::floor(-2.0 * ym - xm)) + 200);
+        double *din = nullptr, *dout = nullptr, *dscr = nullptr;
+        HIPCHK(c, hipMalloc(&din, n * in_w * 8));
+        HIPCHK(c, hipMalloc(&dout, n * out_w * 8));
+        HIPCHK(c, hipMalloc(&dscr, n * len * 5 * 8));
+        HIPCHK(c, hipMemcpy(din, in, n * in_w * 8, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemset(dout, 0, n * out_w * 8));
+        launch_eval_lma(fn, din, dout, uint32_t(n), uint32_t(in_w), uint32_t(out_w), c->d_lma, c->lsc, dscr,
+                        key_from_seed(0), c->stream);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpy(out, dout, n * out_w * 8, hipMemcpyDeviceToHost));
+        (void)hipFree(din); (void)hipFree(dout); (void)hipFree(dscr);
+        return PTG_OK;
+    }
     HIPCHK(c, hipSetDevice(c->device));
     double *din = nullptr, *dout = nullptr;
     HIPCHK(c, hipMalloc(&din, n * in_w * 8));

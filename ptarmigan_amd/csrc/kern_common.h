@@ -187,6 +187,6 @@ __device__ __forceinline__ void forward_record(const TransportArgs& a, uint32_t 
 void launch_leptons_lcfa(const TransportArgs* a, int kind, bool cp, int grid, hipStream_t s);
 void launch_photons_lcfa(const TransportArgs* a, int kind, bool cp, int grid, hipStream_t s);
 void launch_push_lcfa(const LaserDev& L, double dt, int rr_mode, const uint8_t* sp_, double* r4, double* u4, uint32_t n,
-                      uint32_t n_steps, uint32_t* taken, hipStream_t s);
+                      uint32_t n_steps, uint32_t* taken, double* work, hipStream_t s);
 
 }  // namespace ptg

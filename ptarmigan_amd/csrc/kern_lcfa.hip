@@ -326,7 +326,7 @@ void launch_photons_lcfa(const TransportArgs* a, int kind, bool cp, int grid, hi
 // Field-only push (test support for Field::push): n particles, at most n_steps each.
 template <int KIND, bool CP>
 __global__ __launch_bounds__(256) void k_push_lcfa(LaserDev L, double dt, int rr_mode, const uint8_t* species, double* r4, double* u4, uint32_t n,
-                            uint32_t n_steps, uint32_t* taken) {
+                            uint32_t n_steps, uint32_t* taken, double* work) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     d4 r{r4[4 * i], r4[4 * i + 1], r4[4 * i + 2], r4[4 * i + 3]};
@@ -335,11 +335,12 @@ __global__ __launch_bounds__(256) void k_push_lcfa(LaserDev L, double dt, int rr
     double rqm = species[i] == 1 ? -RQM_ELECTRON : (species[i] == 0 ? RQM_ELECTRON : 0.0);
     const double half_cdt = 0.5 * C_LIGHT * dt;
     uint32_t k = 0;
+    double w = 0.0;
     for (; k < n_steps && field_contains(L, r); ++k) {
         d4 rh{r.t + half_cdt, r.x + half_cdt * u.x / ut, r.y + half_cdt * u.y / ut, r.z + half_cdt * u.z / ut};
         d3 E, B;
         lcfa_fields<KIND, CP>(L, rh, E, B);
-        vay_push(u, E, B, rqm, dt, rr_mode, [](double) { return 1.0; });
+        w += vay_push(u, E, B, rqm, dt, rr_mode, [](double) { return 1.0; });
         ut = sqrt(1.0 + nsq(u));
         double hg = half_cdt / ut;
         r = {rh.t + half_cdt, rh.x + hg * u.x, rh.y + hg * u.y, rh.z + hg * u.z};
@@ -347,16 +348,17 @@ __global__ __launch_bounds__(256) void k_push_lcfa(LaserDev L, double dt, int rr
     r4[4 * i] = r.t; r4[4 * i + 1] = r.x; r4[4 * i + 2] = r.y; r4[4 * i + 3] = r.z;
     u4[4 * i] = ut; u4[4 * i + 1] = u.x; u4[4 * i + 2] = u.y; u4[4 * i + 3] = u.z;
     taken[i] = k;
+    if (work) work[i] = w;
 }
 
 void launch_push_lcfa(const LaserDev& L, double dt, int rr_mode, const uint8_t* sp_, double* r4, double* u4, uint32_t n,
-                      uint32_t n_steps, uint32_t* taken, hipStream_t s) {
+                      uint32_t n_steps, uint32_t* taken, double* work, hipStream_t s) {
     int grid = int((n + 255) / 256);
     if (L.kind == K_FAST_FOCUSED_LASER) {
-        if (L.circular) k_push_lcfa<K_FAST_FOCUSED_LASER, true><<<grid, 256, 0, s>>>(L, dt, rr_mode, sp_, r4, u4, n, n_steps, taken);
-        else k_push_lcfa<K_FAST_FOCUSED_LASER, false><<<grid, 256, 0, s>>>(L, dt, rr_mode, sp_, r4, u4, n, n_steps, taken);
+        if (L.circular) k_push_lcfa<K_FAST_FOCUSED_LASER, true><<<grid, 256, 0, s>>>(L, dt, rr_mode, sp_, r4, u4, n, n_steps, taken, work);
+        else k_push_lcfa<K_FAST_FOCUSED_LASER, false><<<grid, 256, 0, s>>>(L, dt, rr_mode, sp_, r4, u4, n, n_steps, taken, work);
     } else {
-        k_push_lcfa<K_FAST_PLANE_WAVE, false><<<grid, 256, 0, s>>>(L, dt, rr_mode, sp_, r4, u4, n, n_steps, taken);
+        k_push_lcfa<K_FAST_PLANE_WAVE, false><<<grid, 256, 0, s>>>(L, dt, rr_mode, sp_, r4, u4, n, n_steps, taken, work);
     }
 }
 

@@ -1,0 +1,41 @@
+// Arguments and launchers of the LMA (PlaneWave) kernels.
+#pragma once
+#include "dev_lma.h"
+#include "kern_common.h"
+
+namespace ptg {
+
+struct LmaArgs {
+    const double* tables;        // LMA pack (dev_lma.h)
+    LmaScalars sc;
+    double bandwidth;            // fractional bandwidth (plane_wave.rs:49-63); 0 = monochromatic
+    double pol_angle, sin_pa, cos_pa;
+    // step kernel -> event queue
+    PBuf ev;
+    double* ev_k0;               // kappa^0 at the event (kappa = k0 (1, 0, 0, 1))
+    double* ev_a;                // sqrt(<a^2>) at the event position
+    unsigned long long* ev_ctr;
+    unsigned long long ev_cap;
+    // event kernel input
+    PBuf ev_in;
+    const double* ev_in_k0_unused;
+    uint64_t n_ev;
+    // event kernel outputs
+    PBuf next;                   // electrons continuing after their event
+    unsigned long long* next_ctr;
+    PBuf ph;                     // emitted photons when photons are tracked
+    unsigned long long* ph_ctr;
+    unsigned long long ph_cap;
+    int photons_to_out;          // photons are final state (photon tracking off)
+    double* scratch;             // J(n, x, y) workspace, 5 * scratch_len doubles per lane
+    int scratch_len;
+};
+
+void launch_step_lma(const TransportArgs* a, const LmaArgs& m, int grid, hipStream_t s);
+void launch_event_lma(const TransportArgs* a, const LmaArgs& m, int lanes, hipStream_t s);
+void launch_push_lma(const LaserDev& L, double dt, int ll, const uint8_t* sp_, double* r4, double* u4, uint32_t n,
+                     uint32_t n_steps, uint32_t* taken, double* work, hipStream_t s);
+void launch_eval_lma(int fn, const double* in, double* out, uint32_t n, uint32_t in_w, uint32_t out_w, const double* tables,
+                     const LmaScalars& sc, double* scratch, uint64_t key, hipStream_t s);
+
+}  // namespace ptg

@@ -1,0 +1,284 @@
+// Transport kernels for the LMA plane wave (PlaneWave): collide()'s
+// electron/positron loop (src/main.rs:169-210) with the step and the emission
+// event split into two kernels.
+//
+//  * k_step_lma: persistent lanes advance electrons with the ponderomotive
+//    push and accumulate the survival S *= 1 - P_k (P from the LP rate table).
+//    When S < U the lane writes the particle, the local wavevector kappa and
+//    <a^2> at the event into the event queue and pulls the next particle:
+//    heavy, divergent event sampling never stalls a wave of stepping lanes.
+//  * k_event_lma: one lane per queued event samples the harmonic, (s, theta)
+//    by rejection on J(n, x, y) with per-lane scratch, builds the photon in the
+//    zero-momentum frame, writes it out and re-queues the recoiled electron.
+// The host alternates the two until no electron is left in the field.
+#include "kern_lma.h"
+
+namespace ptg {
+
+// RNG counter blocks: stepping of segment k (after k events) uses counter
+// (c0, c1 = k); sampling event k uses (c0, c1 = k | EVENT_BLOCK).
+constexpr uint32_t EVENT_BLOCK = 0x80000000u;
+
+__global__ __launch_bounds__(256) void k_step_lma(const TransportArgs* __restrict__ A, LmaArgs M) {
+    const LaserDev& L = A->laser;
+    const double dt = A->dt;
+    const double kc = C_LIGHT * T_COMPTON;
+    __shared__ double rt[lma_pack::RATE_COLS * lma_pack::RATE_ROWS];  // LP rate table (32 KB)
+    for (int i = threadIdx.x; i < lma_pack::RATE_COLS * lma_pack::RATE_ROWS; i += blockDim.x) rt[i] = M.tables[lma_pack::RATE + i];
+    __syncthreads();
+
+    uint64_t n_steps = 0;
+    uint32_t flags = 0, psteps = 0;
+    uint32_t idx = 0, meta = 0, nev = 0;
+    d4 r, u;
+    double work = 0.0, S = 1.0, U = 0.0, rqm = RQM_ELECTRON;
+    Rng rng{};
+    bool skip = false;
+
+    auto fetch = [&]() -> bool {
+        for (;;) {
+            uint32_t i = atomicAdd(A->work_ctr, 1u);
+            if (i >= A->n_in) return false;
+            const PBuf& in = A->in;
+            meta = in.meta[i];
+            if ((meta & 3u) == 2u) { forward_record(*A, i); continue; }
+            idx = i;
+            r = {in.d[P_RT][i], in.d[P_RX][i], in.d[P_RY][i], in.d[P_RZ][i]};
+            u = {in.d[P_UT][i], in.d[P_UX][i], in.d[P_UY][i], in.d[P_UZ][i]};
+            work = in.d[P_WORK][i];
+            nev = in.nev[i];
+            rqm = (meta & 3u) == 1u ? -RQM_ELECTRON : RQM_ELECTRON;
+            rng = rng_make(A->key, in.stream[i]);
+            rng.c1 = nev;
+            U = rng_uniform(rng);
+            S = 1.0;
+            skip = !A->opt.tracking_secondaries && !(meta & META_PRIMARY);
+            psteps = 0;
+            return true;
+        }
+    };
+    bool live = fetch();
+    while (live) {
+        bool inside = !skip && field_contains(L, r) && r.t / C_LIGHT < A->opt.t_stop;
+        if (inside && psteps >= A->step_cap) { flags |= PTG_FLAG_STEP_CAP; inside = false; }
+        if (!inside) {
+            double icount = A->in.d[P_ICOUNT][idx];
+            finalize_lepton(*A, idx, r, u.t, sp(u), work, icount, meta);
+            live = fetch();
+            continue;
+        }
+        ++n_steps;
+        ++psteps;
+        work += pw_push(L, r, u, rqm, dt, A->rr_mode == 1);
+        // PlaneWave::radiate (plane_wave.rs:251-282)
+        double phase = L.k0 * (r.t - r.z);
+        double chirp = 1.0 + 2.0 * L.chirp_b * phase;
+        double width = 1.0;
+        if (M.bandwidth != 0.0) {
+            double u1, u2;
+            rng_uniform2(rng, u1, u2);
+            double sn, cs;
+            sincos(2.0 * PI_D * u2, &sn, &cs);
+            width = 1.0 + M.bandwidth * sqrt(-2.0 * log(1.0 - u1)) * cs;
+        }
+        double k0 = kc * L.k0 * chirp * width;  // kappa = k0 (1, 0, 0, 1)
+        double a_sqd_q = u.t * u.t - u.x * u.x - u.y * u.y - u.z * u.z - 1.0;
+        double a_rms = a_sqd_q >= 0.0 ? sqrt(a_sqd_q) : 0.0;
+        double eta = k0 * (u.t - u.z);
+        double dphi = dt * eta / (T_COMPTON * u.t);
+        double f;
+        double prob = 0.0;
+        if (lp_rate(rt, M.sc, a_rms * 1.4142135623730951, eta, f)) prob = ALPHA * f * dphi / eta;
+        else flags |= PTG_FLAG_TABLE_RANGE;
+        S *= (1.0 - prob);
+        if (S < U) {
+            // suspend: queue the particle with the event parameters
+            unsigned long long j = atomicAdd(M.ev_ctr, 1ull);
+            if (j < M.ev_cap) {
+                const PBuf& q = M.ev;
+                copy_record(q, j, A->in, idx);
+                q.d[P_RT][j] = r.t; q.d[P_RX][j] = r.x; q.d[P_RY][j] = r.y; q.d[P_RZ][j] = r.z;
+                q.d[P_UT][j] = u.t; q.d[P_UX][j] = u.x; q.d[P_UY][j] = u.y; q.d[P_UZ][j] = u.z;
+                q.d[P_WORK][j] = work;
+                q.d[P_ICOUNT][j] = A->in.d[P_ICOUNT][idx];
+                q.nev[j] = nev;
+                M.ev_k0[j] = k0;
+                M.ev_a[j] = sqrt(pw_a_sqd(L, phase));
+            }
+            live = fetch();
+            continue;
+        }
+    }
+    wave_add(A->stats + STAT_LEPTON_STEPS, n_steps);
+    if (flags) atomicOr(A->flags, flags);
+}
+
+// One emission per lane (nonlinear_compton::generate, mod.rs:58-138, with
+// lp::sample, lp/mod.rs:586-676).
+__global__ __launch_bounds__(64) void k_event_lma(const TransportArgs* __restrict__ A, LmaArgs M) {
+    const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n_lanes = gridDim.x * blockDim.x;
+    double* ws = M.scratch + lane;  // this lane's J(n, x, y) workspace, stride n_lanes
+    const double* cdf = M.tables + lma_pack::CDF;
+    uint64_t n_emit = 0, trials = 0;
+    uint32_t flags = 0;
+    for (uint64_t e = lane; e < M.n_ev; e += n_lanes) {
+        const PBuf& q = M.ev_in;
+        uint32_t i = uint32_t(e);
+        d4 r{q.d[P_RT][i], q.d[P_RX][i], q.d[P_RY][i], q.d[P_RZ][i]};
+        d4 u{q.d[P_UT][i], q.d[P_UX][i], q.d[P_UY][i], q.d[P_UZ][i]};
+        double k0 = M.ev_k0[i], a_field = M.ev_a[i];
+        uint32_t nev = q.nev[i];
+        Rng rng = rng_make(A->key, q.stream[i]);
+        rng.c1 = nev | EVENT_BLOCK;
+        double a = sqrt(u.t * u.t - u.x * u.x - u.y * u.y - u.z * u.z - 1.0);  // rms a
+        double eta = k0 * (u.t - u.z);
+        double alp = a * 1.4142135623730951;
+        int n;
+        double s, cphi;
+        sv4 sv;
+        uint32_t tr = 0;
+        bool capped = false;
+        int n_cap_ok = 1;
+        {
+            // harmonic first, to check the workspace is large enough
+            int nh = lp_harmonic(cdf, M.sc, alp, eta, rng_uniform(rng));
+            if (dbessel_len(nh) > M.scratch_len) { flags |= PTG_FLAG_TABLE_RANGE; n_cap_ok = 0; }
+            else lp_sample(cdf, M.sc, alp, eta, rng, ws, n_lanes, n, s, cphi, sv, tr, capped, nh);
+        }
+        trials += tr;
+        if (capped) flags |= PTG_FLAG_REJECT_CAP;
+        d4 u_new = u;
+        double work = q.d[P_WORK][i], icount = q.d[P_ICOUNT][i];
+        if (n_cap_ok) {
+            // photon momentum in the zero-momentum frame, boosted back (mod.rs:73-116)
+            double ell = double(n);
+            double p_zmf = ell * eta / sqrt(1.0 + a * a + 2.0 * ell * eta);
+            double cos_theta_zmf = 1.0 - s * (1.0 + a * a + 2.0 * ell * eta) / (ell * eta);
+            d4 qk{u.t + ell * k0, u.x, u.y, u.z + ell * k0};
+            double inv = 1.0 / sqrt(mdot(qk, qk));
+            d4 u_zmf{qk.t * inv, qk.x * inv, qk.y * inv, qk.z * inv};
+            d3 along = -unit(sp(boost4(u, u_zmf)));
+            d3 eps = unit(sp(boost4(d4{0.0, M.cos_pa, M.sin_pa, 0.0}, u_zmf)));
+            eps = unit(cross(along, cross(eps, along)));
+            double sp_, cp_;
+            sincos(cphi, &sp_, &cp_);
+            d3 perp = rotate(eps, along, sp_, cp_);
+            d3 kp = p_zmf * (cos_theta_zmf * along + sqrt(1.0 - cos_theta_zmf * cos_theta_zmf) * perp);
+            d4 kz{sqrt(nsq(kp)), kp.x, kp.y, kp.z};
+            d4 k = boost4(kz, d4{u_zmf.t, -u_zmf.x, -u_zmf.y, -u_zmf.z});
+            sv = stokes_from_lma(stokes_rotate(sv, -M.pol_angle), k);
+            // sanity check of the construction (mod.rs:118-135)
+            double s_new = (k0 * (k.t - k.z)) / eta;
+            if (!(fabs(s - s_new) / s < 1.0e-2)) flags |= PTG_FLAG_SANITY;
+            double chi = a_field * eta;
+            // photon (src/main.rs:172-189)
+            const uint32_t shower = q.shower[i];
+            uint64_t id = A->id_base + atomicAdd(A->id_ctr, 1ull);
+            uint64_t pstream = rng_child(q.stream[i], nev);
+            if (M.photons_to_out) {
+                if (passes_cut(*A, shower, M_E_MEV * k.t, sp(k))) {
+                    uint64_t j = atomicAdd(A->out_ctr, 1ull);
+                    write_new(A->out, j, 2u, r, k, d4{0.0, 0.0, 0.0, 0.0}, sv, a_field, chi, q.d[P_WEIGHT][i], id, q.id[i],
+                              pstream, shower);
+                    if (A->opt.multiplicity_on) atomicAdd(A->shower_count + shower, 1u);
+                }
+            } else {
+                uint64_t j = atomicAdd(M.ph_ctr, 1ull);
+                if (j < M.ph_cap)
+                    write_new(M.ph, j, 2u, r, k, d4{0.0, 0.0, 0.0, 0.0}, sv, a_field, chi, q.d[P_WEIGHT][i], id, q.id[i],
+                              pstream, shower);
+            }
+            ++n_emit;
+            if (A->recoil) u_new = {u.t + ell * k0 - k.t, u.x - k.x, u.y - k.y, u.z + ell * k0 - k.z};
+            icount += 1.0;
+            work += ell * k0;  // absorption (src/main.rs:198-200)
+        }
+        // the electron continues with its next RNG segment
+        unsigned long long j = atomicAdd(M.next_ctr, 1ull);
+        const PBuf& o = M.next;
+        copy_record(o, j, q, i);
+        o.d[P_RT][j] = r.t; o.d[P_RX][j] = r.x; o.d[P_RY][j] = r.y; o.d[P_RZ][j] = r.z;
+        o.d[P_UT][j] = u_new.t; o.d[P_UX][j] = u_new.x; o.d[P_UY][j] = u_new.y; o.d[P_UZ][j] = u_new.z;
+        o.d[P_WORK][j] = work;
+        o.d[P_ICOUNT][j] = icount;
+        o.nev[j] = nev + 1;
+    }
+    wave_add(A->stats + STAT_EMISSIONS, n_emit);
+    wave_add(A->stats + STAT_TRIALS, trials);
+    if (flags) atomicOr(A->flags, flags);
+}
+
+void launch_step_lma(const TransportArgs* a, const LmaArgs& m, int grid, hipStream_t s) {
+    k_step_lma<<<grid, 256, 0, s>>>(a, m);
+}
+void launch_event_lma(const TransportArgs* a, const LmaArgs& m, int lanes, hipStream_t s) {
+    k_event_lma<<<lanes / 64, 64, 0, s>>>(a, m);
+}
+
+// Field-only push in the plane wave (test support for PlaneWave::push).
+__global__ void k_push_lma(LaserDev L, double dt, int ll, const uint8_t* species, double* r4, double* u4, uint32_t n,
+                           uint32_t n_steps, uint32_t* taken, double* work) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    d4 r{r4[4 * i], r4[4 * i + 1], r4[4 * i + 2], r4[4 * i + 3]};
+    d4 u{u4[4 * i], u4[4 * i + 1], u4[4 * i + 2], u4[4 * i + 3]};
+    double rqm = species[i] == 1 ? -RQM_ELECTRON : RQM_ELECTRON;
+    double w = 0.0;
+    uint32_t k = 0;
+    for (; k < n_steps && field_contains(L, r); ++k) w += pw_push(L, r, u, rqm, dt, ll);
+    r4[4 * i] = r.t; r4[4 * i + 1] = r.x; r4[4 * i + 2] = r.y; r4[4 * i + 3] = r.z;
+    u4[4 * i] = u.t; u4[4 * i + 1] = u.x; u4[4 * i + 2] = u.y; u4[4 * i + 3] = u.z;
+    taken[i] = k;
+    if (work) work[i] = w;
+}
+void launch_push_lma(const LaserDev& L, double dt, int ll, const uint8_t* sp_, double* r4, double* u4, uint32_t n,
+                     uint32_t n_steps, uint32_t* taken, double* work, hipStream_t s) {
+    k_push_lma<<<(n + 255) / 256, 256, 0, s>>>(L, dt, ll, sp_, r4, u4, n, n_steps, taken, work);
+}
+
+// Probes of the LMA device functions (parity tests).
+__global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, uint32_t in_w, uint32_t out_w,
+                           const double* tables, LmaScalars sc, double* scratch, uint64_t key) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* x = in + uint64_t(i) * in_w;
+    double* y = out + uint64_t(i) * out_w;
+    switch (fn) {
+    case PTG_FN_LP_RATE: {
+        double f = 0.0;
+        y[1] = lp_rate(tables + lma_pack::RATE, sc, x[0], x[1], f) ? 1.0 : 0.0;
+        y[0] = f;
+        break;
+    }
+    case PTG_FN_LP_HARMONIC: y[0] = lp_harmonic(tables + lma_pack::CDF, sc, x[0], x[1], x[2]); break;
+    case PTG_FN_DOUBLE_BESSEL: {
+        int nn = int(x[0]);
+        DBessel D = dbessel_at(nn, scratch + i, n);
+        double j[5];
+        if (x[3] != 0.0) dbessel_recurrence(D, x[1], x[2], j);
+        else dbessel_eval(D, x[1], x[2], j);
+        for (int q = 0; q < 5; ++q) y[q] = j[q];
+        break;
+    }
+    case PTG_FN_LP_SAMPLE: {
+        // (a, eta, stream, fixed_n) -> (n, s, theta, S1, S2, trials)
+        Rng rng = rng_make(key, uint64_t(x[2]));
+        int nn;
+        double s, th;
+        sv4 pol;
+        uint32_t tr = 0;
+        bool capped = false;
+        lp_sample(tables + lma_pack::CDF, sc, x[0], x[1], rng, scratch + i, n, nn, s, th, pol, tr, capped, int(x[3]));
+        y[0] = nn; y[1] = s; y[2] = th; y[3] = pol.q; y[4] = pol.u; y[5] = tr;
+        break;
+    }
+    default: break;
+    }
+}
+void launch_eval_lma(int fn, const double* in, double* out, uint32_t n, uint32_t in_w, uint32_t out_w, const double* tables,
+                     const LmaScalars& sc, double* scratch, uint64_t key, hipStream_t s) {
+    if (n) k_eval_lma<<<(n + 63) / 64, 64, 0, s>>>(fn, in, out, n, in_w, out_w, tables, sc, scratch, key);
+}
+
+}  // namespace ptg

@@ -253,17 +253,18 @@ class Engine:
         sec, inter = self.fetch(cnt, intermediates=options.keep_decayed_photons)
         return sec, inter, cnt
 
-    def push(self, species, r, u, n_steps, options: CollideOptions = None):
+    def push(self, species, r, u, n_steps, options: CollideOptions = None, with_work=False):
         """Field-only Field::push of each particle for up to n_steps steps (stops at the field edge)."""
         options = options or CollideOptions()
         sp = np.ascontiguousarray(species, dtype=np.uint8)
-        r = np.ascontiguousarray(r, dtype=np.float64).copy()
-        u = np.ascontiguousarray(u, dtype=np.float64).copy()
+        r = np.ascontiguousarray(r, dtype=np.float64).reshape(-1, 4).copy()
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1, 4).copy()
         taken = np.zeros(len(sp), dtype=np.uint32)
+        work = np.zeros(len(sp))
         self._check(self._lib.ptg_push(self._ctx, sp.ctypes.data_as(C.POINTER(C.c_uint8)), _dptr(r), _dptr(u),
                                        len(sp), n_steps, C.byref(options.desc()),
-                                       taken.ctypes.data_as(C.POINTER(C.c_uint32))))
-        return r, u, taken
+                                       taken.ctypes.data_as(C.POINTER(C.c_uint32)), _dptr(work)))
+        return (r, u, taken, work) if with_work else (r, u, taken)
 
     def histogram(self, species, quantity, nbins, weight_fn=0, log_scale=False, range_=None):
         """Histogram::generate_1d over the resident final-state particles; returns (counts, min, max, total)."""

@@ -53,7 +53,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--primaries", type=int, default=10_000_000)
-    ap.add_argument("--cpu-primaries", type=int, default=2000)
+    ap.add_argument("--cpu-primaries", type=int, default=40_000)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()

@@ -377,6 +377,46 @@ struct ThetaBound {
     }
 };
 
+// lp/mod.rs:304-410: Gauss-Legendre integral of ddpr over 0 < s < s_max, 0 < theta < 2 pi,
+// and the largest integrand value met. Used only to check the rate and ceiling tables.
+static double partial_rate(int n, double a, double eta, double* max_out) {
+    Table G16N = table("quadrature/tables:GAUSS_16_NODES"), G16W = table("quadrature/tables:GAUSS_16_WEIGHTS");
+    Table G32N = table("quadrature/tables:GAUSS_32_NODES"), G32W = table("quadrature/tables:GAUSS_32_WEIGHTS");
+    double sn = 2.0 * double(n) * eta / (1.0 + 0.5 * a * a);
+    double smax = sn / (1.0 + sn);
+    double s_peak = sn / (2.0 + sn);
+    DoubleBessel dj(n, double(n) * SQRT_2, double(n) * 0.5);
+    ThetaBound tmax = ThetaBound::for_harmonic(n, a, eta);
+    double mx = 0.0;
+    auto single = [&](double s) {  // lp/mod.rs:306-319
+        double th_max = tmax.at(s), sum = 0.0;
+        for (int i = 0; i < 32; ++i) {
+            double th = 0.5 * (G32N[i] + 1.0) * th_max;
+            double r = ddpr(a, eta, s, th, dj);
+            sum += 4.0 * (0.5 * th_max) * G32W[i] * r;
+            mx = std::fmax(mx, r);
+        }
+        return sum;
+    };
+    auto segment = [&](double s0, double s1, const Table& N, const Table& W, int m) {
+        double sum = 0.0;
+        for (int i = 0; i < m; ++i) sum += W[i] * 0.5 * (s1 - s0) * single(s0 + 0.5 * (s1 - s0) * (N[i] + 1.0));
+        return sum;
+    };
+    double total;
+    if (sn < 1.0) {
+        total = segment(0.0, s_peak, G32N, G32W, 32);
+        total += segment(s_peak, smax, G32N, G32W, 32);
+    } else {
+        double s1 = smax - 2.0 * (smax - s_peak);
+        total = segment(0.0, s1, G16N, G16W, 16);
+        total += segment(s1, s_peak, G32N, G32W, 32);
+        total += segment(s_peak, smax, G32N, G32W, 32);
+    }
+    if (max_out) *max_out = mx;
+    return total;
+}
+
 // lp/mod.rs:586-676
 static void lp_sample(double a, double eta, Xoshiro& rng, int fixed_n, int* n_out, double* s_out, double* theta_out, SV* pol_out,
                       uint64_t* trials) {
@@ -612,6 +652,10 @@ double orc_lp_ddpr(int n, double a, double eta, double s, double theta) {
     DoubleBessel dj(n, double(n) * SQRT_2, double(n) * 0.5);
     return ddpr(a, eta, s, theta, dj);
 }
+double orc_lp_partial_rate(int n, double a, double eta, double* max_out) { return partial_rate(n, a, eta, max_out); }
+int orc_lp_sum_limit(double a, double eta) { return sum_limit(a, eta); }
+double orc_lp_linear_rate(double a, double eta) { return linear_rate(a, eta); }
+double orc_lp_theta_bound(int n, double a, double eta, double s) { return ThetaBound::for_harmonic(n, a, eta).at(s); }
 double orc_lp_ceiling(int n, double a, double eta) {
     DoubleBessel dj(n, double(n) * SQRT_2, double(n) * 0.5);
     return ceiling_ddpr(a, eta, dj);

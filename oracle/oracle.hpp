@@ -145,7 +145,7 @@ inline C operator/(C a, double s) { return {a.re / s, a.im / s}; }
 inline C operator+(C a, double s) { return {a.re + s, a.im}; }
 inline C operator+(double s, C a) { return {s + a.re, a.im}; }
 inline C operator-(C a, double s) { return {a.re - s, a.im}; }
-inline C operator-(double s, C a) { return {s - a.re, -a.im}; }
+inline C operator-(double s, C a) { return {s - a.re, 0.0 - a.im}; }  // as Complex - Complex: +0 imag for real a
 inline C operator-(C a) { return {-a.re, -a.im}; }
 inline C cexp(C a) { double e = std::exp(a.re); return {e * std::cos(a.im), e * std::sin(a.im)}; }
 inline C cpowf(C a, double p) {

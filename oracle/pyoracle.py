@@ -95,6 +95,26 @@ def lib():
         L.orc_pwmci.restype = C.c_int
         L.orc_pwmci.argtypes = [P(d), C.c_int, d, C.c_int, C.c_int, P(d)]
         L.orc_xoshiro_stream.argtypes = [C.c_uint64, C.c_int, C.c_size_t, P(C.c_uint64)]
+        # LMA (linearly polarised plane wave)
+        L.orc_lp_rate.restype = C.c_int
+        L.orc_lp_rate.argtypes = [d, d, P(d)]
+        L.orc_lp_harmonic.restype = C.c_int
+        L.orc_lp_harmonic.argtypes = [d, d, d]
+        L.orc_double_bessel.argtypes = [C.c_int, d, d, P(d)]
+        L.orc_double_bessel_recurrence.argtypes = [C.c_int, d, d, P(d)]
+        L.orc_lp_ddpr.restype = d
+        L.orc_lp_ddpr.argtypes = [C.c_int, d, d, d, d]
+        L.orc_lp_partial_rate.restype = d
+        L.orc_lp_partial_rate.argtypes = [C.c_int, d, d, P(d)]
+        L.orc_lp_sum_limit.restype = C.c_int
+        L.orc_lp_sum_limit.argtypes = [d, d]
+        L.orc_lp_linear_rate.restype = d
+        L.orc_lp_linear_rate.argtypes = [d, d]
+        L.orc_lp_theta_bound.restype = d
+        L.orc_lp_theta_bound.argtypes = [C.c_int, d, d, d]
+        L.orc_lp_ceiling.restype = d
+        L.orc_lp_ceiling.argtypes = [C.c_int, d, d]
+        L.orc_lp_sample.argtypes = [d, d, C.c_uint64, C.c_int, C.c_size_t, P(d)]
         rc = L.orc_init(os.path.abspath(TABLES).encode())
         if rc != 0:
             raise RuntimeError(f"oracle: cannot load tables from {TABLES}")
@@ -205,4 +225,36 @@ def pair_probability(ell, sv, chi, a_perp, dt):
 def xoshiro(seed, n, jumps=0):
     out = np.zeros(n, dtype=np.uint64)
     lib().orc_xoshiro_stream(seed, jumps, n, out.ctypes.data_as(C.POINTER(C.c_uint64)))
+    return out
+
+
+# ---- LMA probes
+def lp_rate(a, eta):
+    out = np.zeros(1)
+    ok = lib().orc_lp_rate(a, eta, dptr(out))
+    return out[0] if ok else None
+
+
+def lp_harmonic(a, eta, frac):
+    return lib().orc_lp_harmonic(a, eta, frac)
+
+
+def double_bessel(n, x, y, recurrence=False):
+    """(J(n-2..n+2, x, y)); evaluate() picks the saddle point where the reference does."""
+    out = np.zeros(5)
+    f = lib().orc_double_bessel_recurrence if recurrence else lib().orc_double_bessel
+    f(n, x, y, dptr(out))
+    return out
+
+
+def lp_partial_rate(n, a, eta):
+    mx = np.zeros(1)
+    v = lib().orc_lp_partial_rate(n, a, eta, dptr(mx))
+    return v, mx[0]
+
+
+def lp_sample(a, eta, count, seed=0, fixed_n=0):
+    """count samples of (n, s, theta, S1, S2)."""
+    out = np.zeros((count, 5))
+    lib().orc_lp_sample(a, eta, seed, fixed_n, count, dptr(out))
     return out

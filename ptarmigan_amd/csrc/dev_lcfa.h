@@ -69,7 +69,7 @@ __device__ inline double bessel_k13(TP tb, double x) {
 __device__ inline double bessel_k23(TP tb, double x) {
     if (!(x > 0.0)) return 0.0;
     double z = pow(1.5 * x, 2.0 / 3.0);
-    return -PI_D * 0.83268799532620223 * pow(2.0 / x, 2.0 / 3.0) * airy_ai_prime(tb, z);  // 3^{-1/6}
+    return -PI_D * 0.8326831776556043 * pow(2.0 / x, 2.0 / 3.0) * airy_ai_prime(tb, z);  // 3^{-1/6}
 }
 
 // ---------------------------------------------------------------- emission rate

@@ -49,9 +49,15 @@ __device__ __forceinline__ cplx c_sqrt(cplx a) {
     }
     return c_polar(sqrt(c_abs(a)), c_arg(a) / 2.0);
 }
+// Products with i and -i written out as full complex products, (0, 1) * a and
+// (-0, -1) * a, so that the signs of zero parts match num-complex: they pick the
+// branch of c_arg (and so of c_acos) on the real axis.
+__device__ __forceinline__ cplx rmul_i(cplx a) { return {0.0 * a.re - a.im, 0.0 * a.im + a.re}; }
+__device__ __forceinline__ cplx rmul_mi(cplx a) { return {-0.0 * a.re + a.im, -0.0 * a.im - a.re}; }
 __device__ __forceinline__ cplx c_acos(cplx z) {  // -i ln(i sqrt(1 - z^2) + z)
-    cplx w = mul_i(c_sqrt(cplx{1.0 - (z.re * z.re - z.im * z.im), -2.0 * z.re * z.im})) + z;
-    return mul_mi(c_log(w));
+    cplx zz{z.re * z.re - z.im * z.im, z.re * z.im + z.im * z.re};
+    cplx w = rmul_i(c_sqrt(cplx{1.0 - zz.re, 0.0 - zz.im})) + z;
+    return rmul_mi(c_log(w));
 }
 __device__ __forceinline__ cplx c_sin(cplx a) {
     double s, c;
@@ -169,17 +175,17 @@ __device__ __forceinline__ bool lp_rate(const double* rt, const LmaScalars& s, d
     double x = log(a), y = log(eta);
     if (x < s.rate_min[0]) { out = lp_linear_rate(a, eta); return true; }
     double fx = (x - s.rate_min[0]) / s.rate_step[0];
+    if (!(fx >= 0.0 && fx < double(lma_pack::RATE_COLS - 1))) return false;  // also NaN
     int ix = int(fx);
     double dx = fx - double(ix);
     if (y < s.rate_min[1]) {
-        if (ix >= lma_pack::RATE_COLS - 1) return false;
         double f = (1.0 - dx) * rt[ix] + dx * rt[ix + 1];
         out = exp(f - s.rate_min[1]) * eta;
         return true;
     }
     double fy = (y - s.rate_min[1]) / s.rate_step[1];
+    if (!(fy >= 0.0 && fy < double(lma_pack::RATE_ROWS - 1))) return false;
     int iy = int(fy);
-    if (!(ix < lma_pack::RATE_COLS - 1 && iy < lma_pack::RATE_ROWS - 1)) return false;
     double dy = fy - double(iy);
     const double* t0 = rt + iy * lma_pack::RATE_COLS + ix;
     const double* t1 = t0 + lma_pack::RATE_COLS;
@@ -205,10 +211,15 @@ __device__ inline double harmonic_from_table(const double* t16, double a, double
     pwmci_invert(rs, 15, log(-log(1.0 - frac)), true, n);
     return exp(n);
 }
+// Returns 0 when (a, eta) is outside the table, where the reference's lookup
+// would panic; a valid index is at least 1.
 __device__ inline int lp_harmonic(const double* cdf, const LmaScalars& sc, double a, double eta, double frac) {
     double la = log(a), le = log(eta);
     if (la <= sc.cdf_min[0]) return 1;
     double fx = (la - sc.cdf_min[0]) / sc.cdf_step[0];
+    if (!(fx >= 0.0 && fx < double(lma_pack::CDF_COLS - 1))) return 0;
+    if (!(le <= sc.cdf_min[1]) && !((le - sc.cdf_min[1]) / sc.cdf_step[1] < double(lma_pack::CDF_N / lma_pack::CDF_COLS - 1)))
+        return 0;
     int ix = int(fx);
     double dx = fx - double(ix);
     double n;
@@ -224,7 +235,8 @@ __device__ inline int lp_harmonic(const double* cdf, const LmaScalars& sc, doubl
             harmonic_from_table(cdf + 32 * (C * (iy + 1) + ix), a, frac) * ((1.0 - dx) * dy) +
             harmonic_from_table(cdf + 32 * (C * (iy + 1) + ix + 1), a, frac) * (dx * dy);
     }
-    return int(ceil(n));
+    n = ceil(n);
+    return (n >= 1.0 && n <= 1.0e7) ? int(n) : 0;
 }
 
 // ---------------------------------------------------------------- generalised Bessel function J(n, x, y)
@@ -369,8 +381,8 @@ __device__ inline double dbessel_spa(int ni, double x, double y) {
         else cos_theta = {x / (8.0 * y) - sqrt(disc), 0.0};
         cplx theta = c_acos(cos_theta);
         cplx s1 = c_sin(theta), s2 = c_sin(2.0 * theta);
-        cplx f = mul_i(cplx{-n * theta.re + x * s1.re - y * s2.re, -n * theta.im + x * s1.im - y * s2.im});
-        cplx d2f = mul_i(cplx{-x * s1.re + 4.0 * y * s2.re, -x * s1.im + 4.0 * y * s2.im});
+        cplx f = rmul_i(cplx{-n * theta.re + x * s1.re - y * s2.re, -n * theta.im + x * s1.im - y * s2.im});
+        cplx d2f = rmul_i(cplx{-x * s1.re + 4.0 * y * s2.re, -x * s1.im + 4.0 * y * s2.im});
         cplx phase = f + cplx{0.0, (PI_D - c_arg(d2f)) / 2.0};
         double amp = sqrt(2.0 / (PI_D * c_abs(d2f)));
         double re;
@@ -468,8 +480,8 @@ __device__ inline double ln_dbessel_x_zero(int ni, double y) {
     double z = sqrt(n / (4.0 * y) - 0.5);
     cplx theta{PI_D / 2.0, log(sqrt(1.0 + z * z) - z)};
     cplx s2 = c_sin(2.0 * theta);
-    cplx f = mul_i(cplx{-n * theta.re - y * s2.re, -n * theta.im - y * s2.im});
-    cplx f2 = mul_i(4.0 * y * s2);
+    cplx f = rmul_i(cplx{-n * theta.re - y * s2.re, -n * theta.im - y * s2.im});
+    cplx f2 = rmul_i(4.0 * y * s2);
     double e = 0.5 * (PI_D - c_arg(f2));
     cplx phase{f.re, f.im + e};
     return log(sqrt(2.0 / (PI_D * c_abs(f2)))) + phase.re + log(fabs(cos(phase.im)));

@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <dlfcn.h>
 #include <map>
@@ -75,7 +76,7 @@ void free_buf(DevBuf& b) {
 }
 
 // Counter block layout (unsigned long long)
-enum { C_WORK = 0, C_NEXT, C_OUT, C_INTER, C_ID, C_FLAGS, C_EV, C_LNEXT, C_PH, C_STATS, C_N = C_STATS + STAT_N };
+enum { C_WORK = 0, C_NEXT, C_OUT, C_INTER, C_ID, C_FLAGS, C_EV, C_LNEXT, C_PH, C_OVF, C_OVFLEN, C_STATS, C_N = C_STATS + STAT_N };
 
 std::string default_table_path() {
     Dl_info info;
@@ -106,6 +107,10 @@ struct ptg_ctx {
     size_t ev_extra_cap = 0;
     double* d_scratch = nullptr;
     size_t scratch_bytes = 0;
+    uint32_t* d_ovf = nullptr;      // LMA events deferred for a larger J(n, x, y) workspace
+    size_t ovf_cap = 0;
+    double* d_scratch2 = nullptr;   // that larger workspace
+    size_t scratch2_bytes = 0;
     LaserDev laser{};
     bool have_laser = false;
     DevBuf prim, q[2], out, inter, tmp;
@@ -133,6 +138,17 @@ int fail(ptg_ctx* c, int code, const std::string& msg) {
             return fail(ctx, e_ == hipErrorOutOfMemory ? PTG_E_OOM : PTG_E_HIP,                   \
                         std::string(#expr) + ": " + hipGetErrorString(e_));                       \
     } while (0)
+
+// Every copy and fill goes through the context's stream: the stream is created
+// non-blocking, so the legacy-stream hipMemcpy/hipMemset would not be ordered
+// after kernels already queued on it. Copies are synchronous for the caller.
+static hipError_t scopy(ptg_ctx* c, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, c->stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(c->stream);
+}
+static hipError_t sfill(ptg_ctx* c, void* dst, int value, size_t bytes) {
+    return hipMemsetAsync(dst, value, bytes, c->stream);
+}
 
 bool load_blob(const std::string& path, std::map<std::string, std::vector<double>>& out, std::string& err) {
     FILE* f = std::fopen(path.c_str(), "rb");
@@ -336,9 +352,9 @@ int ptg_create(ptg_ctx** out, const char* table_path, int device) {
     if (!build_lma_pack(c, lpack)) { *out = c; return PTG_E_TABLES; }
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(c, hipMalloc(&c->d_lma, lpack.size() * sizeof(double)));
-    HIPCHK(c, hipMemcpy(c->d_lma, lpack.data(), lpack.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, scopy(c, c->d_lma, lpack.data(), lpack.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMalloc(&c->d_lcfa, pack.size() * sizeof(double)));
-    HIPCHK(c, hipMemcpy(c->d_lcfa, pack.data(), pack.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, scopy(c, c->d_lcfa, pack.data(), pack.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMalloc(&c->d_ctr, C_N * sizeof(unsigned long long)));
     HIPCHK(c, hipMalloc(&c->d_args, sizeof(TransportArgs)));
     HIPCHK(c, hipEventCreate(&c->ev0));
@@ -356,6 +372,8 @@ void ptg_destroy(ptg_ctx* c) {
     if (c->d_lma) (void)hipFree(c->d_lma);
     if (c->d_ev_extra) (void)hipFree(c->d_ev_extra);
     if (c->d_scratch) (void)hipFree(c->d_scratch);
+    if (c->d_ovf) (void)hipFree(c->d_ovf);
+    if (c->d_scratch2) (void)hipFree(c->d_scratch2);
     free_buf(c->evq); free_buf(c->lq[0]); free_buf(c->lq[1]); free_buf(c->phq);
     if (c->d_n0) (void)hipFree(c->d_n0);
     if (c->d_shower_count) (void)hipFree(c->d_shower_count);
@@ -473,13 +491,13 @@ int ptg_load_primaries(ptg_ctx* c, const ptg_particles* p, size_t n) {
     std::vector<uint64_t> u64[3];
     std::vector<uint32_t> u32[3];
     soa_from_host(cols, u64, u32, p, n, true);
-    for (int k = 0; k < P_NDOUBLE; ++k) HIPCHK(c, hipMemcpy(c->prim.v.d[k], cols[k].data(), n * 8, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->prim.v.id, u64[0].data(), n * 8, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->prim.v.parent_id, u64[1].data(), n * 8, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->prim.v.stream, u64[2].data(), n * 8, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->prim.v.shower, u32[0].data(), n * 4, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->prim.v.meta, u32[1].data(), n * 4, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->prim.v.nev, u32[2].data(), n * 4, hipMemcpyHostToDevice));
+    for (int k = 0; k < P_NDOUBLE; ++k) HIPCHK(c, scopy(c, c->prim.v.d[k], cols[k].data(), n * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, scopy(c, c->prim.v.id, u64[0].data(), n * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, scopy(c, c->prim.v.parent_id, u64[1].data(), n * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, scopy(c, c->prim.v.stream, u64[2].data(), n * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, scopy(c, c->prim.v.shower, u32[0].data(), n * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, scopy(c, c->prim.v.meta, u32[1].data(), n * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, scopy(c, c->prim.v.nev, u32[2].data(), n * 4, hipMemcpyHostToDevice));
     launch_primary_dirs(c->prim.v, uint32_t(n), c->d_n0, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -508,7 +526,7 @@ static int transport(ptg_ctx* c, TransportArgs a, bool photons, DevBuf& in, size
     if (a.opt.keep_decayed_photons && photons) HIPCHK(c, grow_buf(c->inter, c->n_inter + n_in + 1024, c->n_inter, c->stream));
     if (next.cap < 1024) HIPCHK(c, alloc_buf(next, std::max<size_t>(1024, n_in)));
     unsigned long long snap[C_N];
-    HIPCHK(c, hipMemcpy(snap, c->d_ctr, sizeof(snap), hipMemcpyDeviceToHost));
+    HIPCHK(c, scopy(c, snap, c->d_ctr, sizeof(snap), hipMemcpyDeviceToHost));
     for (int attempt = 0; attempt < 4; ++attempt) {
         snap[C_WORK] = 0;
         snap[C_NEXT] = 0;
@@ -569,6 +587,12 @@ static int run_lma(ptg_ctx* c, TransportArgs a, double* ms, uint64_t* launches, 
     {
         // workspace per lane for J(n, x, y), 5 doubles per order
         int ncap = lma_harmonic_cap(c->laser.a0);
+        // diagnostic: a smaller workspace sends most events through the deferred pass,
+        // which must reproduce the same shower
+        if (const char* e = std::getenv("PTG_LMA_HARMONIC_CAP")) {
+            int v = std::atoi(e);
+            if (v >= 3) ncap = v;
+        }
         double xm = ncap * 1.4142135623730951, ym = ncap * 0.5;
         len_cap = int(std::ceil(xm)) - int(std::floor(-2.0 * ym - xm)) + 200;
     }
@@ -601,7 +625,7 @@ static int run_lma(ptg_ctx* c, TransportArgs a, double* ms, uint64_t* launches, 
     size_t n_cur = n;
     int li = 0;
     const unsigned long long zero = 0;
-    HIPCHK(c, hipMemcpy(c->d_ctr + C_NEXT, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    HIPCHK(c, scopy(c, c->d_ctr + C_NEXT, &zero, sizeof(zero), hipMemcpyHostToDevice));
     while (n_cur > 0) {
         // capacities: each live electron yields at most one final state or one event,
         // each event at most one photon and one continuing electron
@@ -639,21 +663,56 @@ static int run_lma(ptg_ctx* c, TransportArgs a, double* ms, uint64_t* launches, 
         ms[0] += t;
         ++launches[0];
         if (n_ev > 0) {
+            if (c->ovf_cap < c->evq.cap) {
+                if (c->d_ovf) (void)hipFree(c->d_ovf);
+                c->d_ovf = nullptr;
+                HIPCHK(c, hipMalloc(&c->d_ovf, c->evq.cap * sizeof(uint32_t)));
+                c->ovf_cap = c->evq.cap;
+            }
+            HIPCHK(c, hipMemsetAsync(c->d_ctr + C_OVF, 0, 2 * sizeof(unsigned long long), c->stream));
             m.ev_in = c->evq.v;
+            m.ev_list = nullptr;
             m.n_ev = n_ev;
             m.next = nxt.v;
+            m.ovf_list = c->d_ovf;
+            m.ovf_ctr = c->d_ctr + C_OVF;
+            m.ovf_len = c->d_ctr + C_OVFLEN;
             size_t lanes = std::min<size_t>(max_lanes, ((n_ev + 63) / 64) * 64);
             HIPCHK(c, hipEventRecord(c->ev0, c->stream));
             launch_event_lma(c->d_args, m, int(lanes), c->stream);
             HIPCHK(c, hipGetLastError());
             HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+            unsigned long long ovf[2] = {0, 0};
+            HIPCHK(c, hipMemcpyAsync(ovf, c->d_ctr + C_OVF, sizeof(ovf), hipMemcpyDeviceToHost, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
             (void)hipEventElapsedTime(&t, c->ev0, c->ev1);
             ms[1] += t;
             ++launches[1];
+            if (ovf[0] > 0) {
+                // rare high harmonics: one 64-lane pass with a workspace for the largest
+                const size_t olanes = 64;
+                size_t bytes = olanes * size_t(ovf[1]) * 5 * sizeof(double);
+                if (bytes > (size_t(16) << 30)) return fail(c, PTG_E_DOMAIN, "LMA harmonic index beyond the supported range");
+                if (bytes > c->scratch2_bytes) {
+                    if (c->d_scratch2) (void)hipFree(c->d_scratch2);
+                    c->d_scratch2 = nullptr;
+                    HIPCHK(c, hipMalloc(&c->d_scratch2, bytes));
+                    c->scratch2_bytes = bytes;
+                }
+                LmaArgs mo = m;
+                mo.ev_list = c->d_ovf;
+                mo.n_ev = ovf[0];
+                mo.ovf_list = nullptr;
+                mo.scratch = c->d_scratch2;
+                mo.scratch_len = int(ovf[1]);
+                launch_event_lma(c->d_args, mo, int(olanes), c->stream);
+                HIPCHK(c, hipGetLastError());
+                HIPCHK(c, hipStreamSynchronize(c->stream));
+                ++launches[1];
+            }
         }
         unsigned long long now[C_N];
-        HIPCHK(c, hipMemcpy(now, c->d_ctr, sizeof(now), hipMemcpyDeviceToHost));
+        HIPCHK(c, scopy(c, now, c->d_ctr, sizeof(now), hipMemcpyDeviceToHost));
         c->n_out = now[C_OUT];
         cur = &nxt;
         n_cur = size_t(now[C_LNEXT]);
@@ -662,7 +721,7 @@ static int run_lma(ptg_ctx* c, TransportArgs a, double* ms, uint64_t* launches, 
     }
     // photons of the primary beam: not tracked in the LMA path, final cut only
     unsigned long long n_ph = 0;
-    HIPCHK(c, hipMemcpy(&n_ph, c->d_ctr + C_NEXT, sizeof(n_ph), hipMemcpyDeviceToHost));
+    HIPCHK(c, scopy(c, &n_ph, c->d_ctr + C_NEXT, sizeof(n_ph), hipMemcpyDeviceToHost));
     if (n_ph > 0) {
         TransportArgs b = a;
         b.laser.kind = K_FAST_PLANE_WAVE;  // the pass-through never evaluates fields
@@ -758,7 +817,7 @@ int ptg_run(ptg_ctx* c, const ptg_collide_options* o, uint64_t seed, ptg_counter
             rc = transport(c, a, photon_stage, *cur, n_cur, next, &ms[photon_stage], &launches[photon_stage]);
             if (rc) return rc;
             unsigned long long nn = 0;
-            HIPCHK(c, hipMemcpy(&nn, c->d_ctr + C_NEXT, sizeof(nn), hipMemcpyDeviceToHost));
+            HIPCHK(c, scopy(c, &nn, c->d_ctr + C_NEXT, sizeof(nn), hipMemcpyDeviceToHost));
             cur = &next;
             n_cur = size_t(nn);
             qi ^= 1;
@@ -770,18 +829,18 @@ int ptg_run(ptg_ctx* c, const ptg_collide_options* o, uint64_t seed, ptg_counter
     if (a.opt.multiplicity_on && c->n_out) {
         HIPCHK(c, grow_buf(c->tmp, c->n_out, 0, c->stream));
         unsigned long long zero = 0;
-        HIPCHK(c, hipMemcpy(c->d_ctr + C_WORK, &zero, sizeof(zero), hipMemcpyHostToDevice));
+        HIPCHK(c, scopy(c, c->d_ctr + C_WORK, &zero, sizeof(zero), hipMemcpyHostToDevice));
         launch_select_multiplicity(c->out.v, c->n_out, c->tmp.v, c->d_ctr + C_WORK, c->d_shower_count,
                                    uint32_t(o->multiplicity + 1), c->stream);
         HIPCHK(c, hipGetLastError());
         unsigned long long kept = 0;
-        HIPCHK(c, hipMemcpy(&kept, c->d_ctr + C_WORK, sizeof(kept), hipMemcpyDeviceToHost));
+        HIPCHK(c, scopy(c, &kept, c->d_ctr + C_WORK, sizeof(kept), hipMemcpyDeviceToHost));
         std::swap(c->out, c->tmp);
         c->n_out = kept;
     }
 
     unsigned long long fin[C_N];
-    HIPCHK(c, hipMemcpy(fin, c->d_ctr, sizeof(fin), hipMemcpyDeviceToHost));
+    HIPCHK(c, scopy(c, fin, c->d_ctr, sizeof(fin), hipMemcpyDeviceToHost));
     if (cnt) {
         cnt->lepton_steps = fin[C_STATS + STAT_LEPTON_STEPS];
         cnt->photon_steps = fin[C_STATS + STAT_PHOTON_STEPS];
@@ -813,13 +872,13 @@ static int fetch_buf(ptg_ctx* c, const DevBuf& b, size_t n, ptg_particles* p) {
     auto get4 = [&](int first, double* dst) -> int {
         if (!dst) return PTG_OK;
         for (int k = 0; k < 4; ++k) {
-            HIPCHK(c, hipMemcpy(col.data(), b.v.d[first + k], n * 8, hipMemcpyDeviceToHost));
+            HIPCHK(c, scopy(c, col.data(), b.v.d[first + k], n * 8, hipMemcpyDeviceToHost));
             for (size_t i = 0; i < n; ++i) dst[4 * i + k] = col[i];
         }
         return PTG_OK;
     };
     auto get1 = [&](int k, double* dst) -> int {
-        if (dst) HIPCHK(c, hipMemcpy(dst, b.v.d[k], n * 8, hipMemcpyDeviceToHost));
+        if (dst) HIPCHK(c, scopy(c, dst, b.v.d[k], n * 8, hipMemcpyDeviceToHost));
         return PTG_OK;
     };
     int rc = 0;
@@ -827,11 +886,11 @@ static int fetch_buf(ptg_ctx* c, const DevBuf& b, size_t n, ptg_particles* p) {
         (rc = get4(P_SI, p->pol)) || (rc = get1(P_PAYLOAD, p->payload)) || (rc = get1(P_PCHI, p->parent_chi)) ||
         (rc = get1(P_ICOUNT, p->interaction_count)) || (rc = get1(P_WORK, p->work)) || (rc = get1(P_WEIGHT, p->weight)))
         return rc;
-    if (p->id) HIPCHK(c, hipMemcpy(p->id, b.v.id, n * 8, hipMemcpyDeviceToHost));
-    if (p->parent_id) HIPCHK(c, hipMemcpy(p->parent_id, b.v.parent_id, n * 8, hipMemcpyDeviceToHost));
+    if (p->id) HIPCHK(c, scopy(c, p->id, b.v.id, n * 8, hipMemcpyDeviceToHost));
+    if (p->parent_id) HIPCHK(c, scopy(c, p->parent_id, b.v.parent_id, n * 8, hipMemcpyDeviceToHost));
     if (p->species) {
         std::vector<uint32_t> m(n);
-        HIPCHK(c, hipMemcpy(m.data(), b.v.meta, n * 4, hipMemcpyDeviceToHost));
+        HIPCHK(c, scopy(c, m.data(), b.v.meta, n * 4, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < n; ++i) p->species[i] = uint8_t(m[i] & 3u);
     }
     return PTG_OK;
@@ -859,10 +918,10 @@ int ptg_push(ptg_ctx* c, const uint8_t* species, double* r, double* u, size_t n,
     HIPCHK(c, hipMalloc(&du, n * 32));
     HIPCHK(c, hipMalloc(&dk, n * 4));
     HIPCHK(c, hipMalloc(&dw, n * 8));
-    HIPCHK(c, hipMemcpy(ds, species, n, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(dr, r, n * 32, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(du, u, n * 32, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemset(dw, 0, n * 8));
+    HIPCHK(c, scopy(c, ds, species, n, hipMemcpyHostToDevice));
+    HIPCHK(c, scopy(c, dr, r, n * 32, hipMemcpyHostToDevice));
+    HIPCHK(c, scopy(c, du, u, n * 32, hipMemcpyHostToDevice));
+    HIPCHK(c, sfill(c, dw, 0, n * 8));
     int rr_mode = (o->classical && o->rr) ? (o->gaunt_factor ? 2 : 1) : 0;
     double dt = c->laser.dt * o->dt_multiplier;
     if (c->laser.kind == K_PLANE_WAVE) {
@@ -873,10 +932,10 @@ int ptg_push(ptg_ctx* c, const uint8_t* species, double* r, double* u, size_t n,
     }
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(r, dr, n * 32, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(u, du, n * 32, hipMemcpyDeviceToHost));
-    if (steps_taken) HIPCHK(c, hipMemcpy(steps_taken, dk, n * 4, hipMemcpyDeviceToHost));
-    if (work) HIPCHK(c, hipMemcpy(work, dw, n * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, scopy(c, r, dr, n * 32, hipMemcpyDeviceToHost));
+    HIPCHK(c, scopy(c, u, du, n * 32, hipMemcpyDeviceToHost));
+    if (steps_taken) HIPCHK(c, scopy(c, steps_taken, dk, n * 4, hipMemcpyDeviceToHost));
+    if (work) HIPCHK(c, scopy(c, work, dw, n * 8, hipMemcpyDeviceToHost));
     (void)hipFree(ds); (void)hipFree(dr); (void)hipFree(du); (void)hipFree(dk); (void)hipFree(dw);
     return PTG_OK;
 }
@@ -898,10 +957,10 @@ int ptg_histogram(ptg_ctx* c, ptg_hist_spec* s, double* bins_out, double* total_
     double gmin = s->min, gmax = s->max;
     if (!s->fixed_range) {
         unsigned long long mm[2] = {~0ull, 0ull};
-        HIPCHK(c, hipMemcpy(c->d_ctr + C_WORK, mm, sizeof(mm), hipMemcpyHostToDevice));
+        HIPCHK(c, scopy(c, c->d_ctr + C_WORK, mm, sizeof(mm), hipMemcpyHostToDevice));
         if (h.n) launch_hist_minmax(h, c->d_ctr + C_WORK, grid, c->stream);
         HIPCHK(c, hipGetLastError());
-        HIPCHK(c, hipMemcpy(mm, c->d_ctr + C_WORK, sizeof(mm), hipMemcpyDeviceToHost));
+        HIPCHK(c, scopy(c, mm, c->d_ctr + C_WORK, sizeof(mm), hipMemcpyDeviceToHost));
         if (mm[0] == ~0ull) {  // nothing to bin (reference returns None)
             for (uint32_t b = 0; b < s->nbins; ++b) bins_out[b] = 0.0;
             if (total_weight) *total_weight = 0.0;
@@ -945,33 +1004,64 @@ int ptg_eval(ptg_ctx* c, int fn, const double* in, size_t n, size_t in_w, double
         if (fn == PTG_FN_DOUBLE_BESSEL || fn == PTG_FN_LP_SAMPLE) {
             for (size_t i = 0; i < n; ++i) {
                 const double* x = in + i * in_w;
-                int nn = fn == PTG_FN_DOUBLE_BESSEL ? int(x[0]) : (x[3] > 0 ? int(x[3]) : lma_harmonic_cap(x[0] / 1.4142135623730951));
-                if (fn == PTG_FN_DOUBLE_BESSEL && (nn < 3 || nn > 200000)) return fail(c, PTG_E_ARG, "J(n, x, y) probe needs 3 <= n <= 2e5");
+                int nn;
+                if (fn == PTG_FN_DOUBLE_BESSEL) {
+                    // the workspace is sized for x <= n sqrt 2, y <= n / 2 (double_bessel.rs:39-76)
+                    nn = int(x[0]);
+                    if (!(x[0] >= 3.0 && x[0] <= 2.0e5) || double(nn) != x[0])
+                        return fail(c, PTG_E_ARG, "J(n, x, y) probe needs an integer 3 <= n <= 2e5");
+                    if (!(x[1] >= 0.0 && x[1] <= nn * 1.4142135623730951 && x[2] >= 0.0 && x[2] <= 0.5 * nn))
+                        return fail(c, PTG_E_ARG, "J(n, x, y) probe needs 0 <= x <= n sqrt 2 and 0 <= y <= n / 2");
+                } else {
+                    if (!(x[3] >= 0.0 && x[3] <= 2.0e5)) return fail(c, PTG_E_ARG, "LP sample probe needs 0 <= fixed n <= 2e5");
+                    nn = x[3] > 0 ? int(x[3]) : lma_harmonic_cap(x[0]);
+                }
                 nmax = std::max(nmax, nn);
             }
         }
-        double xm = nmax * 1.4142135623730951, ym = nmax * 0.5;
-        size_t len = size_t(int(std::ceil(xm)) - int(std::floor(-2.0 * ym - xm)) + 200);
-        double *din = nullptr, *dout = nullptr, *dscr = nullptr;
-        HIPCHK(c, hipMalloc(&din, n * in_w * 8));
-        HIPCHK(c, hipMalloc(&dout, n * out_w * 8));
-        HIPCHK(c, hipMalloc(&dscr, n * len * 5 * 8));
-        HIPCHK(c, hipMemcpy(din, in, n * in_w * 8, hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemset(dout, 0, n * out_w * 8));
-        launch_eval_lma(fn, din, dout, uint32_t(n), uint32_t(in_w), uint32_t(out_w), c->d_lma, c->lsc, dscr,
-                        key_from_seed(0), c->stream);
-        HIPCHK(c, hipGetLastError());
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        HIPCHK(c, hipMemcpy(out, dout, n * out_w * 8, hipMemcpyDeviceToHost));
-        (void)hipFree(din); (void)hipFree(dout); (void)hipFree(dscr);
+        // LP samples whose harmonic outgrows the workspace come back with trials = -1;
+        // they are run again, alone, with a workspace for the largest such harmonic
+        std::vector<size_t> rows(n);
+        for (size_t i = 0; i < n; ++i) rows[i] = i;
+        for (int pass = 0; pass < 2 && !rows.empty(); ++pass) {
+            const size_t m = rows.size();
+            std::vector<double> hin(m * in_w), hout(m * out_w);
+            for (size_t k = 0; k < m; ++k) std::memcpy(&hin[k * in_w], in + rows[k] * in_w, in_w * 8);
+            double xm = nmax * 1.4142135623730951, ym = nmax * 0.5;
+            size_t len = size_t(int(std::ceil(xm)) - int(std::floor(-2.0 * ym - xm)) + 200);
+            if (m * len * 5 * 8 > (size_t(16) << 30)) return fail(c, PTG_E_ARG, "probe batch too large for its J(n, x, y) workspace");
+            double *din = nullptr, *dout = nullptr, *dscr = nullptr;
+            HIPCHK(c, hipMalloc(&din, m * in_w * 8));
+            HIPCHK(c, hipMalloc(&dout, m * out_w * 8));
+            HIPCHK(c, hipMalloc(&dscr, m * len * 5 * 8));
+            HIPCHK(c, scopy(c, din, hin.data(), m * in_w * 8, hipMemcpyHostToDevice));
+            HIPCHK(c, sfill(c, dout, 0, m * out_w * 8));
+            launch_eval_lma(fn, din, dout, uint32_t(m), uint32_t(in_w), uint32_t(out_w), c->d_lma, c->lsc, dscr, int(len),
+                            key_from_seed(0), c->stream);
+            HIPCHK(c, hipGetLastError());
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            HIPCHK(c, scopy(c, hout.data(), dout, m * out_w * 8, hipMemcpyDeviceToHost));
+            (void)hipFree(din); (void)hipFree(dout); (void)hipFree(dscr);
+            std::vector<size_t> again;
+            nmax = 16;
+            for (size_t k = 0; k < m; ++k) {
+                std::memcpy(out + rows[k] * out_w, &hout[k * out_w], out_w * 8);
+                if (fn == PTG_FN_LP_SAMPLE && hout[k * out_w + 5] < 0.0) {
+                    again.push_back(rows[k]);
+                    nmax = std::max(nmax, int(hout[k * out_w]));
+                }
+            }
+            rows.swap(again);
+        }
+        if (!rows.empty()) return fail(c, PTG_E_STATE, "LP sample probe: workspace re-run did not converge");
         return PTG_OK;
     }
     HIPCHK(c, hipSetDevice(c->device));
     double *din = nullptr, *dout = nullptr;
     HIPCHK(c, hipMalloc(&din, n * in_w * 8));
     HIPCHK(c, hipMalloc(&dout, n * out_w * 8));
-    HIPCHK(c, hipMemcpy(din, in, n * in_w * 8, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemset(dout, 0, n * out_w * 8));
+    HIPCHK(c, scopy(c, din, in, n * in_w * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, sfill(c, dout, 0, n * out_w * 8));
     EvalArgs e{};
     e.fn = fn;
     e.in = din;
@@ -985,7 +1075,7 @@ int ptg_eval(ptg_ctx* c, int fn, const double* in, size_t n, size_t in_w, double
     launch_eval(e, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(out, dout, n * out_w * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, scopy(c, out, dout, n * out_w * 8, hipMemcpyDeviceToHost));
     (void)hipFree(din);
     (void)hipFree(dout);
     return PTG_OK;

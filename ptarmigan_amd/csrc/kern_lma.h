@@ -16,10 +16,16 @@ struct LmaArgs {
     double* ev_a;                // sqrt(<a^2>) at the event position
     unsigned long long* ev_ctr;
     unsigned long long ev_cap;
-    // event kernel input
+    // event kernel input: events ev_list[0..n_ev) of ev_in, or 0..n_ev when ev_list is null
     PBuf ev_in;
-    const double* ev_in_k0_unused;
+    const uint32_t* ev_list;
     uint64_t n_ev;
+    // events whose harmonic needs a larger J(n, x, y) workspace than scratch_len are
+    // deferred here (when ovf_list is set) and re-run by the host with a workspace
+    // of ovf_len; their RNG counters make the re-run draw the same numbers
+    uint32_t* ovf_list;
+    unsigned long long* ovf_ctr;
+    unsigned long long* ovf_len;
     // event kernel outputs
     PBuf next;                   // electrons continuing after their event
     unsigned long long* next_ctr;
@@ -36,6 +42,6 @@ void launch_event_lma(const TransportArgs* a, const LmaArgs& m, int lanes, hipSt
 void launch_push_lma(const LaserDev& L, double dt, int ll, const uint8_t* sp_, double* r4, double* u4, uint32_t n,
                      uint32_t n_steps, uint32_t* taken, double* work, hipStream_t s);
 void launch_eval_lma(int fn, const double* in, double* out, uint32_t n, uint32_t in_w, uint32_t out_w, const double* tables,
-                     const LmaScalars& sc, double* scratch, uint64_t key, hipStream_t s);
+                     const LmaScalars& sc, double* scratch, int scratch_len, uint64_t key, hipStream_t s);
 
 }  // namespace ptg

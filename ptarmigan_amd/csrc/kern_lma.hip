@@ -124,7 +124,7 @@ __global__ __launch_bounds__(64) void k_event_lma(const TransportArgs* __restric
     uint32_t flags = 0;
     for (uint64_t e = lane; e < M.n_ev; e += n_lanes) {
         const PBuf& q = M.ev_in;
-        uint32_t i = uint32_t(e);
+        uint32_t i = M.ev_list ? M.ev_list[e] : uint32_t(e);
         d4 r{q.d[P_RT][i], q.d[P_RX][i], q.d[P_RY][i], q.d[P_RZ][i]};
         d4 u{q.d[P_UT][i], q.d[P_UX][i], q.d[P_UY][i], q.d[P_UZ][i]};
         double k0 = M.ev_k0[i], a_field = M.ev_a[i];
@@ -143,8 +143,21 @@ __global__ __launch_bounds__(64) void k_event_lma(const TransportArgs* __restric
         {
             // harmonic first, to check the workspace is large enough
             int nh = lp_harmonic(cdf, M.sc, alp, eta, rng_uniform(rng));
-            if (dbessel_len(nh) > M.scratch_len) { flags |= PTG_FLAG_TABLE_RANGE; n_cap_ok = 0; }
-            else lp_sample(cdf, M.sc, alp, eta, rng, ws, n_lanes, n, s, cphi, sv, tr, capped, nh);
+            int need = dbessel_len(nh);
+            if (nh < 1) {
+                flags |= PTG_FLAG_TABLE_RANGE;
+                n_cap_ok = 0;
+            } else if (need > M.scratch_len) {
+                if (M.ovf_list) {  // defer: no output now, the host re-runs this event
+                    M.ovf_list[atomicAdd(M.ovf_ctr, 1ull)] = i;
+                    atomicMax(M.ovf_len, (unsigned long long)need);
+                    continue;
+                }
+                flags |= PTG_FLAG_TABLE_RANGE;
+                n_cap_ok = 0;
+            } else {
+                lp_sample(cdf, M.sc, alp, eta, rng, ws, n_lanes, n, s, cphi, sv, tr, capped, nh);
+            }
         }
         trials += tr;
         if (capped) flags |= PTG_FLAG_REJECT_CAP;
@@ -239,7 +252,7 @@ void launch_push_lma(const LaserDev& L, double dt, int ll, const uint8_t* sp_, d
 
 // Probes of the LMA device functions (parity tests).
 __global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, uint32_t in_w, uint32_t out_w,
-                           const double* tables, LmaScalars sc, double* scratch, uint64_t key) {
+                           const double* tables, LmaScalars sc, double* scratch, int scratch_len, uint64_t key) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double* x = in + uint64_t(i) * in_w;
@@ -253,7 +266,7 @@ __global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, ui
     }
     case PTG_FN_LP_HARMONIC: y[0] = lp_harmonic(tables + lma_pack::CDF, sc, x[0], x[1], x[2]); break;
     case PTG_FN_DOUBLE_BESSEL: {
-        int nn = int(x[0]);
+        int nn = int(x[0]);  // the host checks 3 <= n, 0 <= x <= n sqrt 2, 0 <= y <= n / 2, len(n) <= scratch_len
         DBessel D = dbessel_at(nn, scratch + i, n);
         double j[5];
         if (x[3] != 0.0) dbessel_recurrence(D, x[1], x[2], j);
@@ -262,14 +275,18 @@ __global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, ui
         break;
     }
     case PTG_FN_LP_SAMPLE: {
-        // (a, eta, stream, fixed_n) -> (n, s, theta, S1, S2, trials)
+        // (a, eta, stream, fixed_n) -> (n, s, theta, S1, S2, trials); trials = -1 marks a
+        // harmonic n too large for the workspace (the host re-runs such rows)
         Rng rng = rng_make(key, uint64_t(x[2]));
+        int nh = x[3] > 0.0 ? int(x[3]) : lp_harmonic(tables + lma_pack::CDF, sc, x[0], x[1], rng_uniform(rng));
+        if (nh < 1) { y[0] = 0.0; y[5] = -2.0; break; }  // outside the table
+        if (dbessel_len(nh) > scratch_len) { y[0] = nh; y[5] = -1.0; break; }
         int nn;
         double s, th;
         sv4 pol;
         uint32_t tr = 0;
         bool capped = false;
-        lp_sample(tables + lma_pack::CDF, sc, x[0], x[1], rng, scratch + i, n, nn, s, th, pol, tr, capped, int(x[3]));
+        lp_sample(tables + lma_pack::CDF, sc, x[0], x[1], rng, scratch + i, n, nn, s, th, pol, tr, capped, nh);
         y[0] = nn; y[1] = s; y[2] = th; y[3] = pol.q; y[4] = pol.u; y[5] = tr;
         break;
     }
@@ -277,8 +294,8 @@ __global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, ui
     }
 }
 void launch_eval_lma(int fn, const double* in, double* out, uint32_t n, uint32_t in_w, uint32_t out_w, const double* tables,
-                     const LmaScalars& sc, double* scratch, uint64_t key, hipStream_t s) {
-    if (n) k_eval_lma<<<(n + 63) / 64, 64, 0, s>>>(fn, in, out, n, in_w, out_w, tables, sc, scratch, key);
+                     const LmaScalars& sc, double* scratch, int scratch_len, uint64_t key, hipStream_t s) {
+    if (n) k_eval_lma<<<(n + 63) / 64, 64, 0, s>>>(fn, in, out, n, in_w, out_w, tables, sc, scratch, scratch_len, key);
 }
 
 }  // namespace ptg

@@ -24,3 +24,10 @@ def engine():
     e = Engine(0)
     yield e
     e.close()
+
+
+@pytest.fixture(scope="session")
+def golden():
+    import numpy as np
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    return lambda name: np.load(os.path.join(here, name + ".npz"))

@@ -11,6 +11,11 @@
             [(1/(s(1-s)) -+ 1 - 2) K_{2/3}(xi) + int_xi^inf K_{1/3}],  xi = 2/(3 chi s(1-s))
   (the first reproduces the reference's known-answer rates, e.g. 1.58485e16 /s at
   chi = 3.5, gamma = 1000, to 2e-6.)
+* lma.npz:     the generalised Bessel function
+    J(n, x, y) = (1/pi) int_0^pi cos(x sin t - y sin 2t - n t) dt
+  (the integral form consistent with the two-variable Bessel series
+  sum_m J_{n+2m}(x) J_m(y)), at 40 significant digits (mpmath), on points inside
+  the region the LP emission sampler reaches: 0 <= x <= n sqrt 2, 0 <= y <= n/2.
 
 Run: python tests/golden/gen_golden.py
 """
@@ -82,7 +87,27 @@ def lcfa_vectors():
                 cdf_u=cdf_u, cdf=cdf)
 
 
+def double_bessel(n, x, y):
+    f = lambda t: mp.cos(x * mp.sin(t) - y * mp.sin(2 * t) - n * t)
+    pts = mp.linspace(0, mp.pi, 4 * (n + int(x + 2 * y)) + 2)  # resolve every oscillation
+    return float(mp.quad(f, pts) / mp.pi)
+
+
+def lma_vectors():
+    mp.mp.dps = 40
+    rng = np.random.default_rng(2024)
+    rows = []
+    for n in (1, 2, 3, 4, 7, 12, 25, 60):
+        for _ in range(12):
+            x = n * np.sqrt(2.0) * rng.uniform()
+            y = 0.5 * n * rng.uniform()
+            rows.append((n, x, y, double_bessel(n, x, y)))
+    a = np.array(rows)
+    return dict(n=a[:, 0].astype(np.int64), x=a[:, 1], y=a[:, 2], j=a[:, 3])
+
+
 if __name__ == "__main__":
     np.savez(os.path.join(HERE, "special.npz"), **special_vectors())
     np.savez(os.path.join(HERE, "lcfa.npz"), **lcfa_vectors())
-    print("wrote special.npz, lcfa.npz")
+    np.savez(os.path.join(HERE, "lma.npz"), **lma_vectors())
+    print("wrote special.npz, lcfa.npz, lma.npz")

@@ -30,15 +30,31 @@ sys.path.insert(0, HERE)
 # in DESIGN.md ("Roofline accounting").
 FLOPS_PER_LEPTON_STEP = 1070.0
 FLOPS_PER_PHOTON_STEP = 560.0
+# LMA plane-wave step (cos^2 envelope, quantum): ponderomotive leapfrog with two
+# envelope evaluations (<a^2> and its gradient: three sin/cos pairs) and the LP
+# rate lookup (two logs, one exp).
+FLOPS_PER_LMA_STEP = 300.0
 PEAK_FP64_TFLOPS = 78.6
 
 
-def cpu_baseline(n_primaries, threads):
+WORKLOADS = {
+    "lcfa": ("focused LCFA cascade: FastFocusedLaser a0=10 w0=4um 30fs 0.8um LP, "
+             "16.5 GeV e- beam 5um x 20um, emission + Breit-Wheeler, secondaries tracked, FP64"),
+    "lma": ("LMA plane wave: PlaneWave a0=2 0.8um LP 16-cycle cos^2, 10 GeV e- beam 1% spread 1urad, "
+            "nonlinear Compton only, FP64"),
+}
+
+
+def make_workload(name, n):
+    from ptarmigan_amd import workloads
+    return workloads.focused_lcfa_cascade(n) if name == "lcfa" else workloads.plane_wave_lma(n)
+
+
+def cpu_baseline(name, n_primaries, threads):
     """The oracle on `n_primaries` primaries of the same workload, `threads`
     emulated ranks. Returns (steps_per_s, sample description)."""
     from oracle import pyoracle as orc
-    from ptarmigan_amd import workloads
-    laser, beam, opts = workloads.focused_lcfa_cascade(n_primaries)
+    laser, beam, opts = make_workload(name, n_primaries)
     ld, od = laser.desc(), opts.desc()
     z0 = orc.lib().orc_laser_ideal_initial_z(ld) + 3.0 * beam.length
     prim = orc.build_beam(beam.desc(), z0, 0)
@@ -56,6 +72,8 @@ def main():
     ap.add_argument("--cpu-primaries", type=int, default=40_000)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="lcfa",
+                    help="lcfa: the focused-laser cascade (default, the headline workload); lma: the plane-wave LMA case")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -63,8 +81,10 @@ def main():
               "(the transport shards by primaries, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
-    from ptarmigan_amd import Engine, workloads
-    laser, beam, opts = workloads.focused_lcfa_cascade(args.primaries)
+    from ptarmigan_amd import Engine
+    if args.workload == "lma" and args.primaries == 10_000_000:
+        args.primaries = 100_000  # the plane-wave case is quoted on 1e5 primaries
+    laser, beam, opts = make_workload(args.workload, args.primaries)
     eng = Engine(0)
     eng.set_laser(laser)
     eng.build_beam(beam, seed=0)
@@ -94,17 +114,18 @@ def main():
 
     lep_launch_ms = totals["lepton_ms"] / max(1, totals["lepton_launches"])
     lep_steps_per_launch = totals["lepton_steps"] / max(1, totals["lepton_launches"])
-    achieved = FLOPS_PER_LEPTON_STEP * lep_steps_per_launch / (lep_launch_ms * 1e-3) / 1e12
+    flops = FLOPS_PER_LEPTON_STEP if args.workload == "lcfa" else FLOPS_PER_LMA_STEP
+    kname = "k_leptons_lcfa<FastFocusedLaser, linear>" if args.workload == "lcfa" else "k_step_lma"
+    achieved = flops * lep_steps_per_launch / (lep_launch_ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "pipe": "fp64 vector (VALU); MI355X FP64 vector and matrix peaks are both 78.6",
                 "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
-                "kernel": "k_leptons_lcfa<FastFocusedLaser, linear>",
-                "kernel_avg_ms": round(lep_launch_ms, 3), "flops_per_step": FLOPS_PER_LEPTON_STEP}
+                "kernel": kname, "kernel_avg_ms": round(lep_launch_ms, 3), "flops_per_step": flops}
 
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        v, sample = cpu_baseline(args.cpu_primaries, threads)
+        v, sample = cpu_baseline(args.workload, args.cpu_primaries, threads)
         cpu = {"value": v, "unit": "particle-steps/s", "cores": threads, "kind": "port", "sample": sample}
 
     line = {
@@ -112,9 +133,7 @@ def main():
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "focused LCFA cascade: FastFocusedLaser a0=10 w0=4um 30fs 0.8um LP, "
-                               "16.5 GeV e- beam 5um x 20um, emission + Breit-Wheeler, secondaries tracked, FP64",
-                   "primaries": args.primaries, "precision": "fp64"},
+        "config": {"workload": WORKLOADS[args.workload], "primaries": args.primaries, "precision": "fp64"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "detail": {"lepton_steps": totals["lepton_steps"], "photon_steps": totals["photon_steps"],
@@ -122,6 +141,9 @@ def main():
                    "final_state_per_step": totals["n_secondaries"] / args.steps,
                    "generations_per_step": totals["generations"] / args.steps,
                    "kernel_ms_per_step": totals["kernel_ms"] / args.steps,
+                   "lepton_kernel_ms_per_step": totals["lepton_ms"] / args.steps,
+                   "second_kernel_ms_per_step": totals["photon_ms"] / args.steps,
+                   "rejection_trials": totals["rejection_trials"],
                    "error_flags": totals["error_flags"]},
     }
     print(json.dumps(line))

@@ -200,6 +200,9 @@ static bool lp_rate(double a, double eta, double* out) {
     if (x < MIN[0]) { *out = linear_rate(a, eta); return true; }
     if (y < MIN[1]) {
         size_t ix = size_t((x - MIN[0]) / STEP[0]);
+        // beyond the last column the reference's TABLE[0][ix + 1] panics (index out
+        // of bounds); report it as outside the table
+        if (ix >= size_t(N_COLS - 1)) return false;
         double dx = (x - MIN[0]) / STEP[0] - double(ix);
         double f = (1.0 - dx) * T[ix] + dx * T[ix + 1];
         *out = std::exp(f - MIN[1]) * eta;

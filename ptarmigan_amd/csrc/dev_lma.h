@@ -349,12 +349,13 @@ __device__ inline void dbessel_recurrence(const DBessel& D, double x, double y, 
             }
         }
     }
-    // h = g for i < i_match, g_match f / f_match for i >= i_match
-    double ratio = g_match / f_match;
-    double sum_linear = gh_sum + ratio * ft_sum;
-    double rss = hypot(gh_rss, fabs(ratio) * ft_rss);
+    // h = g for i < i_match, (g_match f) / f_match for i >= i_match, in that order:
+    // g_match / f_match alone can overflow (e.g. 1e207 / 1e-191) where the
+    // products stay finite
+    double sum_linear = gh_sum + (g_match * ft_sum) / f_match;
+    double rss = hypot(gh_rss, fabs(g_match * ft_rss) / fabs(f_match));
     for (int q = 0; q < 5; ++q) {
-        double h = tgt_is_f[q] ? ratio * tgt[q] : tgt[q];
+        double h = tgt_is_f[q] ? (g_match * tgt[q]) / f_match : tgt[q];
         double val = copysign(1.0, h / sum_linear) * fabs(h) / rss;
         j[q] = isfinite(val) ? val : 0.0;
     }

@@ -153,7 +153,9 @@ def test_plane_wave_trajectories(engine, orc, name, rr):
     assert np.array_equal(kg, kc)
     assert np.abs(ug - uc).max() / np.abs(uc).max() < 1e-12
     assert np.abs(rg - rc).max() / np.abs(rc).max() < 1e-12
-    assert np.allclose(wg, wc, rtol=1e-10, atol=1e-12 * np.abs(wc).max())
+    # work (units of m c^2) to the same 1e-12 of gamma as the momenta: without
+    # radiation reaction it cancels to ~1e-9 over the pulse, pure rounding
+    assert np.abs(wg - wc).max() <= 1e-12 * np.abs(uc[:, 0]).max()
 
 
 def test_plane_wave_ll_depletion_on_gpu(engine):

@@ -144,6 +144,7 @@ def main():
                    "lepton_kernel_ms_per_step": totals["lepton_ms"] / args.steps,
                    "second_kernel_ms_per_step": totals["photon_ms"] / args.steps,
                    "rejection_trials": totals["rejection_trials"],
+                   "photon_steps_closed_form_not_counted": totals["photon_steps_closed_form"],
                    "error_flags": totals["error_flags"]},
     }
     print(json.dumps(line))

@@ -135,6 +135,8 @@ typedef struct {
     double photon_ms;           /* ... of the photon kernels */
     uint64_t lepton_launches;
     uint64_t photon_launches;
+    uint64_t photon_steps_closed_form; /* photon steps with provably zero pair probability taken in closed
+                                          form (not included in photon_steps) */
 } ptg_counters;
 
 /* Beam description, mirroring BeamBuilder (src/particle/builder.rs:173-219) for

@@ -41,6 +41,8 @@ bool load_tables(const char* path, std::string* err) {
     return true;
 }
 
+bool has_table(const char* name) { return g_tables.count(name) != 0; }
+
 Table table(const char* name) {
     auto it = g_tables.find(name);
     if (it == g_tables.end()) {

@@ -3,6 +3,7 @@
 // polarised nonlinear Compton rates and sampler (src/nonlinear_compton/mod.rs,
 // lp/mod.rs, lp/linear.rs) and the generalised Bessel function J(n, x, y)
 // (src/special_functions/double_bessel.rs).
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -631,9 +632,665 @@ bool lma_radiate(const Laser& L, V4 r, V4 u, double dt, Xoshiro& rng, bool class
     return true;
 }
 
-double lma_pair_create(const Laser&, V4, V4, SV, double, Xoshiro&, double, SV*, bool*, PairEvent*) {
-    std::fprintf(stderr, "oracle: LMA pair creation not restated\n");
-    std::abort();
+// ================================================================ LP nonlinear Breit-Wheeler (pair_creation/lp)
+static const char* BT = "pair_creation/lp/tables/total:";
+static const char* BC = "pair_creation/lp/tables/cdf:";
+static const char* MT = "pair_creation/lp/tables/mid_range/total:";
+static const char* MC = "pair_creation/lp/tables/mid_range/cdf:TABLE";
+
+// pair_creation/lp/theta_bound.rs:10-100
+struct BwThetaBound {
+    double s[16], f[16];
+    static BwThetaBound for_harmonic(int n, double a, double eta) {
+        BwThetaBound t;
+        double s_min = 0.5;
+        double s_max = 0.5 + std::sqrt(0.25 - (1.0 + 0.5 * a * a) / (2.0 * double(n) * eta));
+        for (int i = 0; i < 16; ++i) {
+            double z = std::cos(PI * double(i) / 15.0);
+            t.s[i] = s_min + 0.5 * (z + 1.0) * (s_max - s_min);
+            double r_n = std::sqrt(2.0 * double(n) * eta * t.s[i] * (1.0 - t.s[i]) - (1.0 + 0.5 * a * a));
+            double x = a * r_n / (eta * t.s[i] * (1.0 - t.s[i]));
+            double y = a * a / (8.0 * eta * t.s[i] * (1.0 - t.s[i]));
+            double x_crit = y > double(n) / 6.0 ? 4.0 * std::sqrt(y) * std::sqrt(double(n) - 2.0 * y) : double(n) + 2.0 * y;
+            double ln_j_crit = std::log(0.443) - std::log(double(n)) / 3.0;
+            double ln_j_bdy = n % 2 == 0 ? ThetaBound::ln_double_bessel_x_zero(n, y) : ThetaBound::ln_double_bessel_x_zero(n + 1, y);
+            ln_j_bdy = std::fmin(ln_j_bdy, ln_j_crit);
+            double cos_theta = 1.0 - x_crit * std::log(0.01) / (x * (ln_j_bdy - ln_j_crit));
+            cos_theta = std::fmax(cos_theta, -1.0);
+            t.f[i] = std::acos(cos_theta);
+            if (std::isnan(t.f[i])) t.f[i] = PI;
+        }
+        double f_min = t.f[0];
+        for (int i = 1; i < 16; ++i) {
+            if (t.f[i] < f_min) f_min = t.f[i];
+            else t.f[i] = f_min;
+        }
+        return t;
+    }
+    double at(double sv) const {
+        double val = f[15];
+        for (int i = 1; i < 16; ++i) {
+            if (sv > s[i]) {
+                double w = (sv - s[i - 1]) / (s[i] - s[i - 1]);
+                val = w * f[i] + (1.0 - w) * f[i - 1];
+                break;
+            }
+        }
+        return std::fmin(val, FRAC_PI_2);
+    }
+};
+
+// quadrature/adaptive.rs: 2-D adaptive Clenshaw-Curtis integration of a complex
+// function over a rectangle, splitting the region with the largest error.
+struct CcRegion { C integral, error; double x[2], y[2]; C cache[3][3]; };
+static const double CC5[5][3] = {{0.0, 1.0 / 30.0, 1.0 / 6.0}, {0.1464466094067262, 4.0 / 15.0, 0.0}, {0.5, 2.0 / 5.0, 2.0 / 3.0},
+                                 {0.8535533905932738, 4.0 / 15.0, 0.0}, {1.0, 1.0 / 30.0, 1.0 / 6.0}};
+static const double CC5X[9] = {0.0, 0.1464466094067262, 0.5, 0.8535533905932738, 1.0, 1.1464466094067262, 1.5, 1.8535533905932738, 2.0};
+static const double CC9[9][3] = {{0.0000000000000000, 0.0079365079365079, 1.0 / 30.0}, {0.0380602337443566, 0.0731093246080091, 0.0},
+                                 {0.1464466094067262, 0.1396825396825397, 4.0 / 15.0}, {0.3086582838174551, 0.1808589293602449, 0.0},
+                                 {0.5000000000000000, 0.1968253968253968, 2.0 / 5.0}, {0.6913417161825449, 0.1808589293602449, 0.0},
+                                 {0.8535533905932738, 0.1396825396825397, 4.0 / 15.0}, {0.9619397662556434, 0.0731093246080091, 0.0},
+                                 {1.0000000000000000, 0.0079365079365079, 1.0 / 30.0}};
+static const double CC9X[17] = {0.0000000000000000, 0.0380602337443566, 0.1464466094067262, 0.3086582838174551, 0.5000000000000000,
+                                0.6913417161825449, 0.8535533905932738, 0.9619397662556434, 1.0000000000000000, 1.0380602337443566,
+                                1.1464466094067262, 1.3086582838174551, 1.5000000000000000, 1.6913417161825449, 1.8535533905932738,
+                                1.9619397662556434, 2.0000000000000000};
+
+template <class F, class R>
+static CcRegion cc_new(F& f, R& region, double x0, double x1, double y0, double y1) {  // adaptive.rs:276-313
+    CcRegion g{};
+    C result{0.0, 0.0}, error{0.0, 0.0};
+    for (int i = 0; i < 5; ++i) {
+        double y = y0 + CC5[i][0] * (y1 - y0);
+        for (int j = 0; j < 5; ++j) {
+            double x = x0 + CC5[j][0] * (x1 - x0);
+            C z = region(x, y) ? f(x, y) : C{0.0, 0.0};
+            result = result + (CC5[i][1] * CC5[j][1] * (y1 - y0) * (x1 - x0)) * z;
+            error = error + (CC5[i][2] * CC5[j][2] * (y1 - y0) * (x1 - x0)) * z;
+            if (i % 2 == 0 && j % 2 == 0) g.cache[i / 2][j / 2] = z;
+        }
+    }
+    g.integral = result;
+    g.error = result - error;
+    g.x[0] = x0; g.x[1] = x1; g.y[0] = y0; g.y[1] = y1;
+    return g;
+}
+
+template <class F, class R>
+static void cc_custom_partition(const CcRegion& g, F& f, R& region, const double* xs, const double* ys, CcRegion out[4]) {
+    // adaptive.rs:32-179
+    double x0 = g.x[0], x1 = g.x[1], y0 = g.y[0], y1 = g.y[1];
+    double x_mid = xs ? *xs : 0.5 * (x0 + x1);
+    double y_mid = ys ? *ys : 0.5 * (y0 + y1);
+    C z[9][9];
+    for (int i = 0; i < 9; ++i) {
+        double ty = CC5X[i];
+        double y = i < 4 ? y0 + ty * (y_mid - y0) : y_mid + (ty - 1.0) * (y1 - y_mid);
+        for (int j = 0; j < 9; ++j) {
+            double tx = CC5X[j];
+            double x = j < 4 ? x0 + tx * (x_mid - x0) : x_mid + (tx - 1.0) * (x1 - x_mid);
+            if (i % 8 == 0 && j % 8 == 0) z[i][j] = g.cache[i / 4][j / 4];
+            else if (i % 4 == 0 && j % 4 == 0) {
+                if (!xs && !ys) z[i][j] = g.cache[i / 4][j / 4];
+                else z[i][j] = region(x, y) ? f(x, y) : C{0.0, 0.0};
+            } else z[i][j] = region(x, y) ? f(x, y) : C{0.0, 0.0};
+        }
+    }
+    C result[4] = {}, error[4] = {};
+    for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 2; ++c)
+            for (int i = 0; i < 5; ++i) {
+                double wy = CC5[i][1], ey = CC5[i][2];
+                for (int j = 0; j < 5; ++j) {
+                    double wx = CC5[j][1], ex = CC5[j][2];
+                    C val = z[i + 4 * r][j + 4 * c];
+                    double dy = r < 1 ? y_mid - y0 : y1 - y_mid;
+                    double dx = c < 1 ? x_mid - x0 : x1 - x_mid;
+                    result[2 * r + c] = result[2 * r + c] + (wx * wy * dx * dy) * val;
+                    error[2 * r + c] = error[2 * r + c] + (ex * ey * dx * dy) * val;
+                }
+            }
+    const double xr[2][2] = {{x0, x_mid}, {x_mid, x1}}, yr[2][2] = {{y0, y_mid}, {y_mid, y1}};
+    for (int q = 0; q < 4; ++q) {
+        int r = q / 2, c = q % 2;
+        CcRegion& o = out[q];
+        o.integral = result[q];
+        o.error = result[q] - error[q];
+        o.x[0] = xr[c][0]; o.x[1] = xr[c][1]; o.y[0] = yr[r][0]; o.y[1] = yr[r][1];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) o.cache[a][b] = z[4 * r + 2 * a][4 * c + 2 * b];
+    }
+}
+
+template <class F, class R>
+static void cc_partition(const CcRegion& g, F& f, R& region, CcRegion out[4]) {  // adaptive.rs:181-274
+    double x0 = g.x[0], x1 = g.x[1], y0 = g.y[0], y1 = g.y[1];
+    double x_mid = 0.5 * (x0 + x1), y_mid = 0.5 * (y0 + y1);
+    static thread_local C z[17][17];
+    for (int i = 0; i < 17; ++i) {
+        double y = y0 + 0.5 * CC9X[i] * (y1 - y0);
+        for (int j = 0; j < 17; ++j) {
+            double x = x0 + 0.5 * CC9X[j] * (x1 - x0);
+            if (i % 8 == 0 && j % 8 == 0) z[i][j] = g.cache[i / 8][j / 8];
+            else z[i][j] = region(x, y) ? f(x, y) : C{0.0, 0.0};
+        }
+    }
+    C result[4] = {}, error[4] = {};
+    for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 2; ++c)
+            for (int i = 0; i < 9; ++i) {
+                double wy = CC9[i][1], ey = CC9[i][2];
+                for (int j = 0; j < 9; ++j) {
+                    double wx = CC9[j][1], ex = CC9[j][2];
+                    C val = z[i + 8 * r][j + 8 * c];
+                    double dy = 0.5 * (y1 - y0);
+                    double dx = 0.5 * (x1 - x0);
+                    result[2 * r + c] = result[2 * r + c] + (wx * wy * dx * dy) * val;
+                    error[2 * r + c] = error[2 * r + c] + (ex * ey * dx * dy) * val;
+                }
+            }
+    const double xr[2][2] = {{x0, x_mid}, {x_mid, x1}}, yr[2][2] = {{y0, y_mid}, {y_mid, y1}};
+    for (int q = 0; q < 4; ++q) {
+        int r = q / 2, c = q % 2;
+        CcRegion& o = out[q];
+        o.integral = result[q];
+        o.error = result[q] - error[q];
+        o.x[0] = xr[c][0]; o.x[1] = xr[c][1]; o.y[0] = yr[r][0]; o.y[1] = yr[r][1];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) o.cache[a][b] = z[8 * r + 4 * a][8 * c + 4 * b];
+    }
+}
+
+template <class F, class R>
+static C cc_integrate_2d(F f, double x0, const double* x_div, double x1, double y0, const double* y_div, double y1, R region,
+                         double tol, int max_recursion) {  // adaptive.rs:316-380
+    std::vector<CcRegion> regions;
+    regions.reserve(16 + 4 * size_t(max_recursion));
+    CcRegion first = cc_new(f, region, x0, x1, y0, y1);
+    if (!x_div && !y_div) regions.push_back(first);
+    else {
+        CcRegion q[4];
+        cc_custom_partition(first, f, region, x_div, y_div, q);
+        for (auto& r : q) regions.push_back(r);
+    }
+    C integral{0.0, 0.0};
+    for (int it = 0; it < max_recursion; ++it) {
+        C err{0.0, 0.0};
+        integral = C{0.0, 0.0};
+        for (auto& r : regions) { integral = integral + r.integral; err = err + r.error; }
+        if (std::fabs(err.re) <= tol * std::fabs(integral.re) && std::fabs(err.im) <= tol * std::fabs(integral.im)) break;
+        CcRegion g = regions.back();
+        regions.pop_back();
+        CcRegion q[4];
+        cc_partition(g, f, region, q);
+        for (auto& r : q) regions.push_back(r);
+        std::stable_sort(regions.begin(), regions.end(), [](const CcRegion& a, const CcRegion& b) {
+            return (a.error.re * a.error.re + a.error.im * a.error.im) < (b.error.re * b.error.re + b.error.im * b.error.im);
+        });
+    }
+    return integral;
+}
+
+// pair_creation/lp/mod.rs:31-302
+struct BwDdpr {
+    int n;
+    double a, eta;
+    DoubleBessel dj;
+    BwThetaBound tb;
+    BwDdpr(int n_, double a_, double eta_)
+        : n(n_), a(a_), eta(eta_), dj(n_, double(n_) * SQRT_2, double(n_) * 0.5), tb(BwThetaBound::for_harmonic(n_, a_, eta_)) {}
+    void s_bounds(double* s_min, double* s_max) const {  // :50-55
+        *s_min = 0.5;
+        *s_max = std::fmax(0.5 + std::sqrt(0.25 - (1.0 + 0.5 * a * a) / (2.0 * double(n) * eta)), 0.5);
+    }
+    double max_theta(double s) const { return tb.at(s); }
+    void elements(double s, double theta, double* unpol, double* d1, double* d2) {  // :63-95
+        double r_n_sqd = 2.0 * double(n) * eta * s * (1.0 - s) - (1.0 + 0.5 * a * a);
+        if (!(r_n_sqd > 0.0)) { *unpol = *d1 = *d2 = 0.0; return; }
+        double x = a * std::sqrt(r_n_sqd) * std::cos(theta) / (eta * s * (1.0 - s));
+        double y = a * a / (8.0 * eta * s * (1.0 - s));
+        double j[5];
+        dj.evaluate(x, y, j);
+        double g0 = j[2], g1 = 0.5 * (j[1] + j[3]), g2 = 0.25 * (j[0] + 2.0 * j[2] + j[4]);
+        double h_s = 0.5 / (s * (1.0 - s)) - 1.0;
+        *unpol = powi(g0, 2) + a * a * h_s * (powi(g1, 2) - g0 * g2);
+        *d1 = powi(g0 * std::sqrt(r_n_sqd) * std::cos(theta) - g1 * a, 2) - powi(g0 * std::sin(theta), 2) * r_n_sqd;
+        *d2 = 2.0 * g0 * std::sqrt(r_n_sqd) * std::sin(theta) * (g0 * std::sqrt(r_n_sqd) * std::cos(theta) - g1 * a);
+    }
+    C at(double s, double theta) {  // :97-106
+        double u, d1, d2;
+        elements(s, theta, &u, &d1, &d2);
+        return {(u - d1) / (2.0 * PI), (u + d1) / (2.0 * PI)};
+    }
+    double fully_resolved(double s, double theta, double sv1, double sv2) {  // :109-112
+        double u, d1, d2;
+        elements(s, theta, &u, &d1, &d2);
+        return (u - sv1 * d1 - sv2 * d2) / (2.0 * PI);
+    }
+    C integrate_direct() {  // :125-185
+        Table N16 = table("quadrature/tables:GAUSS_16_NODES"), W16 = table("quadrature/tables:GAUSS_16_WEIGHTS");
+        Table N32 = table("quadrature/tables:GAUSS_32_NODES"), W32 = table("quadrature/tables:GAUSS_32_WEIGHTS");
+        double s_min, s_max;
+        s_bounds(&s_min, &s_max);
+        double s_peak = std::fmin(0.5 + std::sqrt(0.25 - (1.0 + a * a) / (2.0 * double(n) * eta)), 0.5);
+        auto seg = [&](double s0, double s1, const Table& Nn, const Table& Ww, int m) {
+            C acc{0.0, 0.0};
+            for (int i = 0; i < m; ++i) {
+                double sv = s0 + 0.5 * (Nn[i] + 1.0) * (s1 - s0);
+                C sd = theta_integral(sv, max_theta(sv));
+                acc = acc + (0.5 * Ww[i] * (s1 - s0)) * sd;
+            }
+            return acc;
+        };
+        C integral;
+        if (s_peak < s_min + 2.0 * (s_max - s_min) / 3.0) {
+            s_peak = std::fmin(s_peak, s_min + (s_max - s_min) / 3.0);
+            integral = seg(s_min, s_peak, N32, W32, 32) + seg(s_peak, s_max, N32, W32, 32);
+        } else {
+            double delta = s_max - s_peak;
+            integral = seg(s_min, s_max - 2.0 * delta, N16, W16, 16) + seg(s_max - 2.0 * delta, s_peak, N32, W32, 32) +
+                       seg(s_peak, s_max, N32, W32, 32);
+        }
+        return (2.0 * 4.0) * integral;
+    }
+    C theta_integral(double sv, double max_th) {  // integrate_over_theta, :114-123
+        Table N = table("quadrature/tables:GAUSS_32_NODES"), W = table("quadrature/tables:GAUSS_32_WEIGHTS");
+        C sum{0.0, 0.0};
+        for (int i = 0; i < 32; ++i) {
+            double th = 0.5 * (N[i] + 1.0) * max_th;
+            sum = sum + (0.5 * W[i] * max_th) * at(sv, th);
+        }
+        return sum;
+    }
+    C integrate_adaptive() {  // :187-217
+        double s_min, s_max;
+        s_bounds(&s_min, &s_max);
+        double s_peak = 0.5 + std::sqrt(0.25 - (1.0 + a * a) / (2.0 * double(n) * eta));
+        bool have_peak = s_peak > s_min + 2.0 * (s_max - s_min) / 3.0;
+        double theta_split = tb.at(have_peak ? s_peak : 0.5);
+        auto f = [&](double sv, double th) { return at(sv, th); };
+        auto region = [&](double sv, double th) { return th < tb.at(sv); };
+        C integral = cc_integrate_2d(f, s_min, have_peak ? &s_peak : nullptr, s_max, 0.0, &theta_split, FRAC_PI_2, region, 1.0e-3, 16);
+        return (4.0 * 2.0) * integral;
+    }
+    C integrate() {  // :219-228
+        if (max_theta(0.5) < 0.75 * FRAC_PI_2) return integrate_adaptive();
+        return integrate_direct();
+    }
+    double ceiling(double sv1, double sv2) {  // :230-258
+        double s_min, s_max;
+        s_bounds(&s_min, &s_max);
+        double centre = 0.5 + std::sqrt(0.25 - (1.0 + a * a) / (2.0 * double(n) * eta));
+        double lower = s_min, upper = s_max;
+        if (std::isfinite(centre)) {
+            lower = std::fmax(centre - 0.2, s_min);
+            upper = 0.5 * (centre + s_max);
+        }
+        double mx = -INFINITY;
+        for (int i = 0; i < 50; ++i) mx = std::fmax(mx, fully_resolved(lower + (upper - lower) * double(i) / 50.0, 0.0, sv1, sv2));
+        return 1.2 * mx;
+    }
+    void sample(double sv1, double sv2, Xoshiro& rng, double* s_out, double* th_out, uint64_t* trials) {  // :261-301
+        double s_min, s_max;
+        s_bounds(&s_min, &s_max);
+        double mx = ceiling(sv1, sv2);
+        double s, theta;
+        for (;;) {
+            s = s_min + (s_max - s_min) * rng.gen_f64();
+            theta = FRAC_PI_2 * rng.gen_f64();
+            if (theta > max_theta(s)) continue;
+            double z = mx * rng.gen_f64();
+            double f = fully_resolved(s, theta, sv1, sv2);
+            if (trials) ++*trials;
+            if (z < f) break;
+        }
+        // rand 0.7 gen_range(0, 2) and gen_range(0, 4): widening multiply of one 32-bit draw
+        uint32_t r1 = uint32_t(rng.next_u64() >> 32);
+        if (int((uint64_t(r1) * 2u) >> 32) == 0) s = 1.0 - s;
+        uint32_t r2 = uint32_t(rng.next_u64() >> 32);
+        switch (int((uint64_t(r2) * 4u) >> 32)) {
+        case 1: theta = PI - theta; break;
+        case 2: theta = PI + theta; break;
+        case 3: theta = 2.0 * PI - theta; break;
+        default: break;
+        }
+        *s_out = s;
+        *th_out = theta;
+    }
+};
+
+// pair_creation/lp/mod.rs:345-362
+static void bw_sum_limits(double a, double eta, int* n_lo, int* n_hi) {
+    double n_min = std::ceil(2.0 * (1.0 + 0.5 * a * a) / eta);
+    double range = a < 1.0 ? 2.0 + (2.0 + 20.0 * a * a) * std::exp(-std::sqrt(0.5 * eta))
+                           : 3.0 + 2.8 * std::pow(a, 8.0 / 3.0) / std::cbrt(eta) + 0.25 * powi(a, 3) * std::sqrt(eta);
+    double test = 0.25 - (1.0 + 0.5 * a * a) / (2.0 * n_min * eta);
+    if (test <= 2.220446049250313e-16) { *n_lo = int(n_min) + 1; *n_hi = int(n_min + 1.0 + range); }
+    else { *n_lo = int(n_min); *n_hi = int(n_min + range); }
+}
+static bool bw_is_negligible(double a, double eta) {  // :371-374
+    return std::log10(eta) < -1.0 - powi(std::log10(a) + 2.0, 2) / 4.5;
+}
+
+// tables/mod.rs:18-60 (high range)
+static bool bw_hi_contains(double a, double eta) {
+    return std::log(a) >= tb(BT, "LN_MIN_A")[0] && a < 25.0 && std::log(eta) >= tb(BT, "LN_MIN_ETA")[0] && eta < 2.0;
+}
+static void bw_hi_interpolate(double a, double eta, double out[2]) {
+    const int N_COLS = 25, N_ROWS = 61;
+    double LMA_ = tb(BT, "LN_MIN_A")[0], LME = tb(BT, "LN_MIN_ETA")[0], LAS = tb(BT, "LN_A_STEP")[0], LES = tb(BT, "LN_ETA_STEP")[0];
+    Table T = tb(BT, "TABLE");
+    size_t ia = size_t((std::log(a) - LMA_) / LAS), ie = size_t((std::log(eta) - LME) / LES);
+    if (!(ia < size_t(N_COLS - 1) && ie < size_t(N_ROWS - 1))) { std::fprintf(stderr, "oracle: NBW table index\n"); std::abort(); }
+    auto at = [&](size_t r, size_t c, int k) { return T[(r * N_COLS + c) * 2 + k]; };
+    for (int k = 0; k < 2; ++k) {
+        double f;
+        if (a * eta > 2.0) {
+            double da = (std::log(a) - LMA_) / LAS - double(ia), de = (std::log(eta) - LME) / LES - double(ie);
+            f = (1.0 - da) * (1.0 - de) * at(ie, ia, k) + da * (1.0 - de) * at(ie, ia + 1, k) + (1.0 - da) * de * at(ie + 1, ia, k) +
+                da * de * at(ie + 1, ia + 1, k);
+        } else {
+            double a_min = std::exp(LMA_ + double(ia) * LAS), a_max = std::exp(LMA_ + double(ia + 1) * LAS);
+            double e_min = std::exp(LME + double(ie) * LES), e_max = std::exp(LME + double(ie + 1) * LES);
+            double da = ((a_min * a_max / a) - a_min) / (a_max - a_min);
+            double de = ((e_min * e_max / eta) - e_min) / (e_max - e_min);
+            f = at(ie, ia, k) * da * de + at(ie, ia + 1, k) * (1.0 - da) * de + at(ie + 1, ia, k) * da * (1.0 - de) +
+                at(ie + 1, ia + 1, k) * (1.0 - da) * (1.0 - de);
+        }
+        out[k] = std::exp(f);
+    }
+}
+static double poldep_interp(double par, double perp, double sv1) { return 0.5 * ((par + perp) + sv1 * (par - perp)); }
+
+// tables/mod.rs:62-120
+static int bw_hi_invert(double a, double eta, double sv1, double frac) {
+    const int N_COLS = 25;
+    Table MIN = tb(BC, "MIN"), STEP = tb(BC, "STEP"), T = tb(BC, "TABLE");
+    double iaf = (std::log(a) - MIN[0]) / STEP[0], da = fract(iaf);
+    size_t ia = size_t(iaf);
+    double ief = (std::log(eta) - MIN[1]) / STEP[1], de = fract(ief);
+    size_t ie = size_t(ief);
+    size_t idx[4] = {N_COLS * ie + ia, N_COLS * ie + ia + 1, N_COLS * (ie + 1) + ia, N_COLS * (ie + 1) + ia + 1};
+    double w[4] = {(1.0 - da) * (1.0 - de), da * (1.0 - de), (1.0 - da) * de, da * de};
+    double u = 0.0;
+    for (int k = 0; k < 4; ++k) {
+        const double* src = T.p + idx[k] * 48;
+        double n_min = src[0];
+        double f_max = poldep_interp(src[15 * 3 + 1], src[15 * 3 + 2], sv1);
+        double tbl[30];
+        for (int i = 0; i < 15; ++i) {
+            tbl[2 * i] = std::log(src[3 * i] - n_min + 1.0);
+            double f = poldep_interp(src[3 * i + 1], src[3 * i + 2], sv1);
+            tbl[2 * i + 1] = std::log(-1.0 * std::log(1.0 - f / f_max));
+        }
+        double fr = std::log(-1.0 * std::log(1.0 - frac));
+        double uk;
+        if (!Interp{tbl, 15, true}.invert(fr, &uk)) { std::fprintf(stderr, "oracle: NBW invert failed\n"); std::abort(); }
+        u += uk * w[k];
+    }
+    int n_lo, n_hi;
+    bw_sum_limits(a, eta, &n_lo, &n_hi);
+    double n = double(n_lo) + std::expm1(u);
+    return int(n) + 1;
+}
+
+// tables/mid_range/mod.rs:9-16
+static bool bw_mid_contains(double a, double eta) {
+    double ln_a = std::log(a), ln_ep = std::log(eta / (1.0 + 0.5 * a * a));
+    return ln_a > tb(MT, "LN_MIN_A")[0] && a < 2.0 && ln_ep > tb(MT, "LN_MIN_ETA_PRIME")[0] && ln_ep < tb(MT, "LN_MAX_ETA_PRIME")[0];
+}
+// mid_range/mod.rs:18-82: three-point power-law fit in eta' along one a column
+static void bw_mid_interp_eta_prime(double ep, size_t ia, double out[2]) {
+    const int N_COLS = 36;
+    double D = tb(MT, "ETA_PRIME_DENSITY")[0], LMX = tb(MT, "LN_MAX_ETA_PRIME")[0];
+    Table T = tb(MT, "TABLE");
+    size_t ie = size_t(D * (std::exp(LMX) / ep - 1.0));
+    if (ie % size_t(D) == size_t(D) - 1) ie = ie - 1;
+    auto row = [&](size_t r, int k) { return T[(r * N_COLS + ia) * 2 + k]; };
+    double lo0 = 2.0 / (1.0 + double(ie + 2) / D), mid0 = 2.0 / (1.0 + double(ie + 1) / D), up0 = 2.0 / (1.0 + double(ie) / D);
+    for (int k = 0; k < 2; ++k) {
+        double lo1 = row(ie + 2, k), mid1 = row(ie + 1, k), up1 = row(ie, k);
+        double m = std::log((up1 - lo1) / (mid1 - lo1)) / std::log((up0 - lo0) / (mid0 - lo0));
+        double alpha = (mid1 - lo1) / std::pow(mid0 - lo0, m);
+        double v = alpha * std::pow(ep - lo0, m) + lo1;
+        if (!std::isfinite(v)) {
+            if (ep < mid0) {
+                double w = (std::log(ep) - std::log(lo0)) / (std::log(mid0) - std::log(lo0));
+                v = (1.0 - w) * lo1 + w * mid1;
+            } else {
+                double w = (std::log(ep) - std::log(mid0)) / (std::log(up0) - std::log(mid0));
+                v = (1.0 - w) * mid1 + w * up1;
+            }
+        }
+        out[k] = v;
+    }
+}
+static void bw_mid_interpolate(double a, double eta, double out[2]) {  // mid_range/mod.rs:84-100
+    double LMA_ = tb(MT, "LN_MIN_A")[0], LAS = tb(MT, "LN_A_STEP")[0];
+    size_t ia = size_t((std::log(a) - LMA_) / LAS);
+    double ep = eta / (1.0 + 0.5 * a * a);
+    double f0[2], f1[2];
+    bw_mid_interp_eta_prime(ep, ia, f0);
+    bw_mid_interp_eta_prime(ep, ia + 1, f1);
+    for (int k = 0; k < 2; ++k) {
+        double f;
+        if (a < 0.5) {
+            double da = (std::log(a) - LMA_) / LAS - double(ia);
+            f = (1.0 - da) * f0[k] + da * f1[k];
+        } else {
+            double a_min = std::exp(LMA_ + double(ia) * LAS), a_max = std::exp(LMA_ + double(ia + 1) * LAS);
+            double da = (a - a_min) / (a_max - a_min);
+            f = ((1.0 - da) * a_min * f0[k] + da * a_max * f1[k]) / a;
+        }
+        out[k] = std::exp(f);
+    }
+}
+static double bw_mid_project(double a, double eta, double sv1) {
+    double f[2];
+    bw_mid_interpolate(a, eta, f);
+    return poldep_interp(f[0], f[1], sv1);
+}
+// mid_range/mod.rs:109-162
+static void bw_mid_cdf_setup(double a, double eta, size_t* idx, double* w, bool* next_to_bdy) {
+    const int N_COLS = 36;
+    double LMA_ = tb(MT, "LN_MIN_A")[0], LAS = tb(MT, "LN_A_STEP")[0], LMX = tb(MT, "LN_MAX_ETA_PRIME")[0];
+    double D = tb(MT, "ETA_PRIME_DENSITY")[0];
+    size_t ia = size_t((std::log(a) - LMA_) / LAS);
+    double da = (std::log(a) - LMA_) / LAS - double(ia);
+    double ep = eta / (1.0 + 0.5 * a * a);
+    size_t ie = size_t(D * (std::exp(LMX) / ep - 1.0));
+    if (next_to_bdy) {
+        *next_to_bdy = ie % size_t(D) == size_t(D) - 1;
+        if (*next_to_bdy) ie = ie - 1;
+    }
+    double lower = std::exp(LMX) / (1.0 + double(ie + 1) / D), upper = std::exp(LMX) / (1.0 + double(ie) / D);
+    double de = (std::log(ep) - std::log(lower)) / (std::log(upper) - std::log(lower));
+    idx[0] = N_COLS * ie + ia; idx[1] = N_COLS * ie + ia + 1; idx[2] = N_COLS * (ie + 1) + ia; idx[3] = N_COLS * (ie + 1) + ia + 1;
+    w[0] = (1.0 - da) * de; w[1] = da * de; w[2] = (1.0 - da) * (1.0 - de); w[3] = da * (1.0 - de);
+}
+static void bw_mid_lowest(double a, double eta, double sv1, int* n_min_out, double* rate_out) {
+    size_t idx[4];
+    double w[4];
+    bw_mid_cdf_setup(a, eta, idx, w, nullptr);
+    Table T = table(MC);
+    int n_lo, n_hi;
+    bw_sum_limits(a, eta, &n_lo, &n_hi);
+    double total = bw_mid_project(a, eta, sv1);
+    double acc = 0.0;
+    for (int k = 0; k < 4; ++k) {
+        const double* t = T.p + idx[k] * 48;
+        double frac = 0.0;
+        if (!(t[0] > double(n_lo))) frac = poldep_interp(t[1], t[2], sv1) / poldep_interp(t[45 + 1], t[45 + 2], sv1);
+        acc += w[k] * powi(frac * total, 3);
+    }
+    *n_min_out = n_lo;
+    *rate_out = std::cbrt(acc);
+}
+static int bw_mid_invert(double a, double eta, double sv1, double frac) {  // mid_range/mod.rs:164-272
+    size_t idx[4];
+    double w[4];
+    bool bdy = false;
+    bw_mid_cdf_setup(a, eta, idx, w, &bdy);
+    Table T = table(MC);
+    double n = 0.0;
+    if (bdy) {
+        int n_min;
+        double rate_lowest;
+        bw_mid_lowest(a, eta, sv1, &n_min, &rate_lowest);
+        double p_lowest = rate_lowest / bw_mid_project(a, eta, sv1);
+        if (frac <= p_lowest) return n_min;
+        for (int k = 0; k < 4; ++k) {
+            const double* src = T.p + idx[k] * 48;
+            double f_max = poldep_interp(src[46], src[47], sv1);
+            double tbl[32];
+            for (int i = 0; i < 16; ++i) { tbl[2 * i] = src[3 * i]; tbl[2 * i + 1] = poldep_interp(src[3 * i + 1], src[3 * i + 2], sv1) / f_max; }
+            if (tbl[0] == double(n_min)) {
+                double t01 = tbl[1];
+                for (int i = 0; i < 16; ++i) tbl[2 * i + 1] = (t01 - tbl[2 * i + 1] + p_lowest * (tbl[2 * i + 1] - 1.0)) / (t01 - 1.0);
+            }
+            double nk;
+            if (!Interp{tbl, 16}.invert(frac, &nk)) { std::fprintf(stderr, "oracle: NBW mid invert failed\n"); std::abort(); }
+            n += nk * w[k];
+        }
+    } else {
+        for (int k = 0; k < 4; ++k) {
+            const double* src = T.p + idx[k] * 48;
+            double f_max = poldep_interp(src[46], src[47], sv1);
+            double tbl[32];
+            for (int i = 0; i < 16; ++i) { tbl[2 * i] = src[3 * i]; tbl[2 * i + 1] = poldep_interp(src[3 * i + 1], src[3 * i + 2], sv1) / f_max; }
+            double nk;
+            if (frac <= tbl[1]) nk = tbl[0] - 0.1;
+            else if (!Interp{tbl, 16}.invert(frac, &nk)) { std::fprintf(stderr, "oracle: NBW mid invert failed\n"); std::abort(); }
+            n += nk * w[k];
+        }
+    }
+    return int(std::ceil(n));
+}
+
+// pair_creation/lp/mod.rs:445-512 (TotalRate::by_summation): partial rates of every
+// harmonic in sum_limits, cumulated, sampled at 16 log-spaced harmonics; out48 is
+// the [n, par, perp] x 16 row of a CDF table. Used to check the regenerated table.
+static void bw_by_summation(double a, double eta, double out48[48]) {
+    int n_min, n_max;
+    bw_sum_limits(a, eta, &n_min, &n_max);
+    std::vector<C> cum;
+    C total{0.0, 0.0};
+    for (int n = n_min; n <= n_max; ++n) {
+        BwDdpr d(n, a, eta);
+        total = total + d.integrate();
+        cum.push_back(total);
+    }
+    double delta = double(n_max - n_min);
+    double u_max = std::log(1.0 + delta);
+    double nn[16];
+    nn[0] = double(n_min);
+    double u = 0.0, du = u_max / 15.0;
+    for (int i = 1; i <= 15; ++i) {
+        u = u + du;
+        double n = double(n_min) + std::expm1(u);
+        if (std::round(n) <= nn[i - 1]) {
+            n = nn[i - 1] + 1.0;
+            u = std::log(n - double(n_min) + 1.0);
+            du = (u_max - u) / (15.0 - double(i));
+            nn[i] = n;
+        } else nn[i] = std::round(n);
+    }
+    for (int i = 0; i < 16; ++i) {
+        size_t index = size_t(nn[i]) - size_t(n_min);
+        C c = index < cum.size() ? cum[index] : cum.back();
+        out48[3 * i] = nn[i];
+        out48[3 * i + 1] = c.re;
+        out48[3 * i + 2] = c.im;
+    }
+}
+
+// pair_creation/lp/mod.rs:385-422
+static double bw_probability(double a, double eta, SV sv, double dphi, SV* sv_out) {
+    double f0 = 0.0, f1 = 0.0;
+    if (bw_is_negligible(a, eta)) {}
+    else if (bw_mid_contains(a, eta)) { double f[2]; bw_mid_interpolate(a, eta, f); f0 = f[0]; f1 = f[1]; }
+    else if (bw_hi_contains(a, eta)) { double f[2]; bw_hi_interpolate(a, eta, f); f0 = f[0]; f1 = f[1]; }
+    double prob = ALPHA_FINE * (0.5 * ((f0 + f1) + sv[1] * (f0 - f1))) * dphi / eta;
+    double prob_avg = ALPHA_FINE * (0.5 * (f0 + f1)) * dphi / eta;
+    double delta = ALPHA_FINE * (0.5 * (f0 - f1)) * dphi / eta;
+    double sv1 = (sv[1] * (1.0 - prob_avg) - delta) / (1.0 - prob);
+    double sv2 = sv[2] * (1.0 - prob_avg) / (1.0 - prob);
+    double sv3 = sv[3] * (1.0 - prob_avg) / (1.0 - prob);
+    *sv_out = SV{sv[0], sv1, sv2, sv3};
+    return prob;
+}
+// pair_creation/lp/mod.rs:424-442
+static void bw_sample(double a, double eta, double sv1, double sv2, Xoshiro& rng, int* n_out, double* s_out, double* th_out,
+                      uint64_t* trials) {
+    double frac = rng.gen_f64();
+    int n;
+    if (bw_mid_contains(a, eta)) n = bw_mid_invert(a, eta, sv1, frac);
+    else if (bw_hi_contains(a, eta)) n = bw_hi_invert(a, eta, sv1, frac);
+    else { std::fprintf(stderr, "oracle: NBW [LP] rate lookup out of bounds\n"); std::abort(); }
+    BwDdpr sp(n, a, eta);
+    sp.sample(sv1, sv2, rng, s_out, th_out, trials);
+    *n_out = n;
+}
+
+// pair_creation/mod.rs:21-41 (linear polarisation)
+static double nbw_probability(V4 ell, SV sv, V4 k, double a, double dt, double pol_angle, SV* sv_out) {
+    double eta = k * ell;
+    SV s = in_lma_basis(sv, ell);
+    s = rotate_by(s, pol_angle);
+    double dphi = eta * dt / (COMPTON_TIME * ell[0]);
+    SV s2;
+    double prob = bw_probability(a * SQRT_2, eta, s, dphi, &s2);
+    s2 = rotate_by(s2, -pol_angle);
+    *sv_out = from_lma_basis(s2, ell);
+    return prob;
+}
+// pair_creation/mod.rs:47-107 (linear polarisation)
+static V4 nbw_generate(V4 ell, SV sv, V4 k, double a, double pol_angle, Xoshiro& rng, int* n_out) {
+    double eta = k * ell;
+    SV s = rotate_by(in_lma_basis(sv, ell), pol_angle);
+    int n;
+    double sfrac, cphi;
+    bw_sample(a * SQRT_2, eta, s[1], s[2], rng, &n, &sfrac, &cphi, nullptr);
+    double j = double(n);
+    double e_zmf = std::sqrt(0.5 * j * eta);
+    double p_zmf = std::sqrt(0.5 * j * eta - 1.0 - a * a);
+    double cos_theta_zmf = (1.0 - 2.0 * sfrac) * e_zmf / p_zmf;
+    V4 ljk = ell + j * k;
+    V4 u_zmf = ljk / std::sqrt(ljk * ljk);
+    V3 along = -normalize(spatial(boost_by(j * k, u_zmf)));
+    V4 eps4{0.0, std::cos(pol_angle), std::sin(pol_angle), 0.0};
+    V3 eps = normalize(spatial(boost_by(eps4, u_zmf)));
+    eps = normalize(cross(along, cross(eps, along)));
+    V3 perp = rotate_around(eps, along, cphi);
+    V3 q3 = p_zmf * (cos_theta_zmf * along + std::sqrt(1.0 - powi(cos_theta_zmf, 2)) * perp);
+    V4 q = with_sqr(lightlike(q3.x, q3.y, q3.z), 1.0 + a * a);
+    q = boost_by(q, V4{u_zmf.t, -u_zmf.x, -u_zmf.y, -u_zmf.z});
+    double s_new = 1.0 - (k * q) / (k * ell);
+    double error = std::fabs(sfrac - s_new) / sfrac;
+    if (!(error < 1.0e-3)) { std::fprintf(stderr, "oracle: pair_creation::generate sanity check failed\n"); std::abort(); }
+    *n_out = n;
+    return q;
+}
+
+// plane_wave.rs:284-316
+double lma_pair_create(const Laser& L, V4 r, V4 ell, SV pol, double dt, Xoshiro& rng, double rate_increase, SV* pol_new,
+                       bool* happened, PairEvent* ev) {
+    if (L.kind != Kind::PlaneWave) { std::fprintf(stderr, "oracle: FocusedLaser not restated\n"); std::abort(); }
+    if (L.pol != Pol::Linear) { std::fprintf(stderr, "oracle: circular LMA pair creation not restated\n"); std::abort(); }
+    double a = std::sqrt(pw_a_sqd(L, r));
+    double phase = L.wavevector * r;
+    double chirp = 1.0 + 2.0 * L.chirp_b * phase;
+    V4 kappa = SPEED_OF_LIGHT * COMPTON_TIME * L.wavevector * chirp;
+    double prob = nbw_probability(ell, pol, kappa, a, dt, L.pol_angle, pol_new);
+    double ri = prob * rate_increase > 0.1 ? 0.1 / prob : rate_increase;
+    *happened = false;
+    if (rng.gen_f64() < prob * ri) {
+        int n;
+        V4 q_p = nbw_generate(ell, pol, kappa, a, L.pol_angle, rng, &n);
+        *ev = PairEvent{ell + double(n) * kappa - q_p, q_p, 1.0 / ri, a, a * (ell * kappa), double(n) * kappa[0]};
+        *happened = true;
+    }
+    return prob;
 }
 
 }  // namespace orc
@@ -662,6 +1319,52 @@ double orc_lp_theta_bound(int n, double a, double eta, double s) { return ThetaB
 double orc_lp_ceiling(int n, double a, double eta) {
     DoubleBessel dj(n, double(n) * SQRT_2, double(n) * 0.5);
     return ceiling_ddpr(a, eta, dj);
+}
+void orc_bw_partial_rate(int n, double a, double eta, int method, double* out2) {  // 0 integrate, 1 direct, 2 adaptive
+    BwDdpr d(n, a, eta);
+    C v = method == 1 ? d.integrate_direct() : (method == 2 ? d.integrate_adaptive() : d.integrate());
+    out2[0] = v.re; out2[1] = v.im;
+}
+double orc_bw_ddpr(int n, double a, double eta, double s, double theta, double sv1, double sv2) {
+    BwDdpr d(n, a, eta);
+    return d.fully_resolved(s, theta, sv1, sv2);
+}
+double orc_bw_ceiling(int n, double a, double eta, double sv1, double sv2) { BwDdpr d(n, a, eta); return d.ceiling(sv1, sv2); }
+double orc_bw_theta_bound(int n, double a, double eta, double s) { return BwThetaBound::for_harmonic(n, a, eta).at(s); }
+int orc_bw_sum_limits(double a, double eta, int* hi) { int lo; bw_sum_limits(a, eta, &lo, hi); return lo; }
+/* total rate (par, perp) -> out2; returns 0 negligible, 1 mid-range table, 2 high-range table, -1 outside */
+int orc_bw_rate(double a, double eta, double* out2) {
+    out2[0] = out2[1] = 0.0;
+    if (bw_is_negligible(a, eta)) return 0;
+    if (bw_mid_contains(a, eta)) { bw_mid_interpolate(a, eta, out2); return 1; }
+    if (bw_hi_contains(a, eta)) { bw_hi_interpolate(a, eta, out2); return 2; }
+    return -1;
+}
+double orc_bw_probability(double a, double eta, const double* sv4, double dphi, double* sv_out4) {
+    SV o;
+    double p = bw_probability(a, eta, SV{sv4[0], sv4[1], sv4[2], sv4[3]}, dphi, &o);
+    sv_out4[0] = o.i; sv_out4[1] = o.q; sv_out4[2] = o.u; sv_out4[3] = o.v;
+    return p;
+}
+void orc_bw_by_summation(double a, double eta, double* out48) { bw_by_summation(a, eta, out48); }
+int orc_bw_harmonic(double a, double eta, double sv1, double frac) {
+    if (bw_mid_contains(a, eta)) return has_table(MC) ? bw_mid_invert(a, eta, sv1, frac) : -2;
+    if (bw_hi_contains(a, eta)) return bw_hi_invert(a, eta, sv1, frac);
+    return -1;
+}
+/* count samples of (n, s, theta) */
+void orc_bw_sample(double a, double eta, double sv1, double sv2, uint64_t seed, int fixed_n, size_t count, double* out3) {
+    Xoshiro rng(seed);
+    for (size_t i = 0; i < count; ++i) {
+        int n;
+        double sf, th;
+        if (fixed_n > 0) {
+            BwDdpr d(fixed_n, a, eta);
+            d.sample(sv1, sv2, rng, &sf, &th, nullptr);
+            n = fixed_n;
+        } else bw_sample(a, eta, sv1, sv2, rng, &n, &sf, &th, nullptr);
+        out3[3 * i] = n; out3[3 * i + 1] = sf; out3[3 * i + 2] = th;
+    }
 }
 /* n samples of (n, s, theta, S1, S2) at (a, eta); fixed_n > 0 fixes the harmonic. */
 void orc_lp_sample(double a, double eta, uint64_t seed, int fixed_n, size_t count, double* out5) {

@@ -162,6 +162,7 @@ struct Table {
 };
 bool load_tables(const char* path, std::string* err);
 Table table(const char* name);  // aborts if absent (oracle misuse is a bug)
+bool has_table(const char* name);
 
 // ---------------------------------------------------------------- RNG
 // rand_xoshiro 0.4.0 Xoshiro256StarStar; seed_from_u64 via SplitMix64; jump();

@@ -46,7 +46,8 @@ class Counters(C.Structure):
                 ("pair_events", C.c_uint64), ("n_secondaries", C.c_uint64), ("n_intermediates", C.c_uint64),
                 ("generations", C.c_uint64), ("rejection_trials", C.c_uint64), ("error_flags", C.c_uint32),
                 ("reserved", C.c_uint32), ("kernel_ms", C.c_double), ("lepton_ms", C.c_double),
-                ("photon_ms", C.c_double), ("lepton_launches", C.c_uint64), ("photon_launches", C.c_uint64)]
+                ("photon_ms", C.c_double), ("lepton_launches", C.c_uint64), ("photon_launches", C.c_uint64),
+                ("photon_steps_closed_form", C.c_uint64)]
 
 
 class BeamDesc(C.Structure):

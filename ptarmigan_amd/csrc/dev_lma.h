@@ -166,7 +166,7 @@ __device__ __forceinline__ double lp_linear_rate(double a, double eta) {
     double eta2 = eta * eta;
     if (eta > 1.0e-3)
         return (a * a / 8.0) * (0.5 + 4.0 / eta - 0.5 / ((1.0 + 2.0 * eta) * (1.0 + 2.0 * eta)) +
-                                (1.0 - 2.0 / eta - 2.0 / eta2) * log1p(2.0 * eta));
+                                (1.0 - 2.0 / eta - 2.0 / eta2) * log(1.0 + 2.0 * eta));
     return (a * a / 8.0) * (8.0 * eta / 3.0 - 16.0 * eta2 / 3.0 + 208.0 * eta2 * eta / 15.0);
 }
 // f(a, eta) such that dP/dphase = alpha f / eta; returns false when (a, eta)

@@ -537,7 +537,11 @@ static int transport(ptg_ctx* c, TransportArgs a, bool photons, DevBuf& in, size
         a.inter = c->inter.v;
         a.next = next.v;
         a.next_cap = next.cap;
-        int blocks_per_cu = 2;
+        int blocks_per_cu = lcfa_blocks_per_cu(a.laser.kind, a.laser.circular, photons);
+        if (const char* e = std::getenv("PTG_BLOCKS_PER_CU")) {  // diagnostic override for grid sweeps
+            int v = std::atoi(e);
+            if (v >= 1 && v <= 16) blocks_per_cu = v;
+        }
         int grid = int(std::min<size_t>(size_t(c->n_cu) * blocks_per_cu, (n_in + 255) / 256));
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->d_args, &a, sizeof(a), hipMemcpyHostToDevice, c->stream));
@@ -771,6 +775,13 @@ int ptg_run(ptg_ctx* c, const ptg_collide_options* o, uint64_t seed, ptg_counter
     a.precision = o->precision;
     a.key = key_from_seed(seed);
     a.step_cap = 1u << 22;
+    {
+        // peak |E| + c|B| of the pulse, with a factor 1000 of margin for the
+        // focusing corrections: below ff_kt_max a photon's chi stays under 0.01
+        double e_peak = c->laser.kind == K_FAST_FOCUSED_LASER ? 2.0 * c->laser.E0 : 2.0 * c->laser.amplitude;
+        a.ff_kt_max = e_peak > 0.0 ? 0.01 * E_CRIT / (1000.0 * e_peak) : 0.0;
+        a.ff_after = 4096;
+    }
     a.lcfa_tables = c->d_lcfa;
     a.work_ctr = reinterpret_cast<uint32_t*>(c->d_ctr + C_WORK);
     a.out_ctr = c->d_ctr + C_OUT;
@@ -856,6 +867,7 @@ int ptg_run(ptg_ctx* c, const ptg_collide_options* o, uint64_t seed, ptg_counter
         cnt->photon_ms = ms[1];
         cnt->lepton_launches = launches[0];
         cnt->photon_launches = launches[1];
+        cnt->photon_steps_closed_form = fin[C_STATS + STAT_PHOTON_STEPS_CF];
     }
     return PTG_OK;
 }

@@ -10,7 +10,7 @@
 
 namespace ptg {
 
-enum { STAT_LEPTON_STEPS = 0, STAT_PHOTON_STEPS, STAT_EMISSIONS, STAT_PAIRS, STAT_TRIALS, STAT_N };
+enum { STAT_LEPTON_STEPS = 0, STAT_PHOTON_STEPS, STAT_EMISSIONS, STAT_PAIRS, STAT_TRIALS, STAT_PHOTON_STEPS_CF, STAT_N };
 
 // CollideOptions as the kernels need them (src/main.rs:72-105).
 struct DevOptions {
@@ -29,6 +29,11 @@ struct TransportArgs {
     int recoil;           // electron_recoils (src/main.rs:159)
     int precision;        // ptg_precision
     uint32_t step_cap;    // steps after which a particle is stopped and PTG_FLAG_STEP_CAP raised
+    // Photons with energy below ff_kt_max (units of m c^2) cannot reach chi = 0.01 anywhere
+    // in the field, where the pair rate is exactly zero (lcfa/pair_creation/mod.rs:13-18);
+    // after ff_after steps their remaining straight-line steps are taken in closed form.
+    double ff_kt_max;
+    uint32_t ff_after;
     uint64_t key;         // RNG key derived from control:rng_seed
     const double* lcfa_tables;
     const double* lma_tables;
@@ -186,6 +191,7 @@ __device__ __forceinline__ void forward_record(const TransportArgs& a, uint32_t 
 // `a` is a device pointer to the launch arguments.
 void launch_leptons_lcfa(const TransportArgs* a, int kind, bool cp, int grid, hipStream_t s);
 void launch_photons_lcfa(const TransportArgs* a, int kind, bool cp, int grid, hipStream_t s);
+int lcfa_blocks_per_cu(int kind, bool cp, bool photons);
 void launch_push_lcfa(const LaserDev& L, double dt, int rr_mode, const uint8_t* sp_, double* r4, double* u4, uint32_t n,
                       uint32_t n_steps, uint32_t* taken, double* work, hipStream_t s);
 

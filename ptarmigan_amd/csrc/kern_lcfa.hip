@@ -19,7 +19,7 @@
 // Occupancy target of the transport kernels (waves per SIMD); bounds the
 // registers of the step loop and of the out-of-line event samplers it calls.
 #ifndef PTG_WAVES_PER_EU
-#define PTG_WAVES_PER_EU 4
+#define PTG_WAVES_PER_EU 3
 #endif
 
 namespace ptg {
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PTG_WAVES_P
     const double dt = A->dt;
     const double cdt = C_LIGHT * dt;
 
-    uint64_t n_steps = 0, n_pairs = 0, trials = 0;
+    uint64_t n_steps = 0, n_pairs = 0, trials = 0, ff_steps = 0;
     uint32_t flags = 0, psteps = 0;
 
     uint32_t idx = 0;
@@ -262,9 +262,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PTG_WAVES_P
             live = fetch();
             continue;
         }
+        double ik = cdt / k.t;
+        if (psteps == A->ff_after && k.t < A->ff_kt_max) {
+            // zero pair probability for the rest of the path: advance to two steps
+            // before the exit (phase or t_stop) and let the loop take the last ones
+            double dphi = L.k0 * (cdt - ik * k.z);
+            double n_ph = dphi > 0.0 ? ceil((L.max_phase - L.k0 * (r.t - r.z)) / dphi) : 1.0e300;
+            double n_t = isfinite(A->opt.t_stop) ? ceil((A->opt.t_stop * C_LIGHT - r.t) / cdt) : 1.0e300;
+            double n_go = fmin(fmin(n_ph, n_t), double(A->step_cap - psteps)) - 2.0;
+            if (n_go > 1.0) {
+                r = {r.t + n_go * cdt, r.x + n_go * (ik * k.x), r.y + n_go * (ik * k.y), r.z + n_go * (ik * k.z)};
+                psteps += uint32_t(n_go);
+                ff_steps += uint64_t(n_go);
+                continue;
+            }
+        }
         ++n_steps;
         ++psteps;
-        double ik = cdt / k.t;
         r = {r.t + cdt, r.x + ik * k.x, r.y + ik * k.y, r.z + ik * k.z};
         sv4 pin = A->opt.pol_resolved ? pol : sv4{1.0, 0.0, 0.0, 0.0};
         d3 E, B;
@@ -301,6 +315,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PTG_WAVES_P
         icount += prob;
     }
     wave_add(A->stats + STAT_PHOTON_STEPS, n_steps);
+    wave_add(A->stats + STAT_PHOTON_STEPS_CF, ff_steps);
     wave_add(A->stats + STAT_PAIRS, n_pairs);
     wave_add(A->stats + STAT_TRIALS, trials);
     if (flags) atomicOr(A->flags, flags);
@@ -314,6 +329,23 @@ void launch_leptons_lcfa(const TransportArgs* a, int kind, bool cp, int grid, hi
         k_leptons_lcfa<K_FAST_PLANE_WAVE, false><<<grid, 256, 0, s>>>(a);
     }
 }
+// Resident 256-lane workgroups per CU for the transport kernel that
+// launch_leptons_lcfa / launch_photons_lcfa would run (the persistent grid size).
+int lcfa_blocks_per_cu(int kind, bool cp, bool photons) {
+    int nb = 0;
+    hipError_t e;
+    if (kind == K_FAST_FOCUSED_LASER && cp)
+        e = photons ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_photons_lcfa<K_FAST_FOCUSED_LASER, true>, 256, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_leptons_lcfa<K_FAST_FOCUSED_LASER, true>, 256, 0);
+    else if (kind == K_FAST_FOCUSED_LASER)
+        e = photons ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_photons_lcfa<K_FAST_FOCUSED_LASER, false>, 256, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_leptons_lcfa<K_FAST_FOCUSED_LASER, false>, 256, 0);
+    else
+        e = photons ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_photons_lcfa<K_FAST_PLANE_WAVE, false>, 256, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_leptons_lcfa<K_FAST_PLANE_WAVE, false>, 256, 0);
+    return (e == hipSuccess && nb > 0) ? nb : 1;
+}
+
 void launch_photons_lcfa(const TransportArgs* a, int kind, bool cp, int grid, hipStream_t s) {
     if (kind == K_FAST_FOCUSED_LASER) {
         if (cp) k_photons_lcfa<K_FAST_FOCUSED_LASER, true><<<grid, 256, 0, s>>>(a);

@@ -267,3 +267,45 @@ def test_histogram_matches_numpy(engine):
     ref = np.bincount(idx[idx < 200], minlength=200).astype(float)
     assert np.array_equal(counts, ref)
     assert total == pytest.approx(sec["weight"][ph].sum())
+
+
+def test_zero_probability_photons_closed_form(engine, orc):
+    """Soft photons moving almost with the laser take ~1e5 steps with exactly zero
+    pair probability (chi < 0.01, lcfa/pair_creation/mod.rs:13-18). The device
+    takes those steps in closed form; final states and the step count agree with
+    the oracle, which steps them one by one."""
+    from ptarmigan_amd import CollideOptions, workloads
+    laser, _, _ = workloads.focused_lcfa_cascade(1)
+    engine.set_laser(laser)
+    dt, z0 = engine.field_info()
+    n = 64
+    rng = np.random.default_rng(5)
+    prim = {k: np.zeros((n, 4) if k in ("r", "u", "r0", "u0", "pol") else n,
+                        dtype=np.uint8 if k == "species" else (np.uint64 if k in ("id", "parent_id") else np.float64))
+            for k in ("species", "r", "u", "r0", "u0", "pol", "payload", "parent_chi", "interaction_count", "work",
+                      "weight", "id", "parent_id")}
+    prim["species"][:] = 2
+    kt = 10.0 ** rng.uniform(-11, -6, n)
+    th = rng.uniform(0.05, 0.2, n)               # 50-200 mrad off +z: 1e5-2e6 steps each
+    ph = rng.uniform(0, 2 * np.pi, n)
+    prim["u"] = np.stack([kt, kt * np.sin(th) * np.cos(ph), kt * np.sin(th) * np.sin(ph), kt * np.cos(th)], 1)
+    prim["r"] = np.stack([-z0 * np.ones(n), rng.normal(0, 2e-6, n), rng.normal(0, 2e-6, n), z0 * np.ones(n)], 1)
+    prim["r0"] = prim["r"].copy()
+    prim["u0"] = prim["u"].copy()
+    prim["pol"][:, 0] = 1.0
+    prim["weight"][:] = 1.0
+    prim["id"] = np.arange(n, dtype=np.uint64)
+    opts = CollideOptions()
+    say("closed-form photons")
+    sg, _, cg = engine.collide(prim, opts, seed=1)
+    sc, _, cc, _ = orc.collide(laser.desc(), opts.desc(), prim, 1, 4)
+    say("  gpu", cg, "\n  cpu", cc)
+    assert cg["photon_steps_closed_form"] > 0
+    total = cg["photon_steps"] + cg["photon_steps_closed_form"]
+    assert abs(total - cc["photon_steps"]) <= 2 * n
+    og, oc = np.argsort(sg["id"]), np.argsort(sc["id"])
+    assert np.array_equal(sg["id"][og], sc["id"][oc])
+    assert np.allclose(sg["u"][og], sc["u"][oc], rtol=1e-14, atol=0)
+    # final positions: the same straight line, ending within one step of each other
+    step = 2.997925e8 * dt
+    assert np.abs(sg["r"][og] - sc["r"][oc]).max() <= 1.01 * step

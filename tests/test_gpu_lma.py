@@ -48,7 +48,9 @@ def test_lp_rate_matches_oracle(engine, orc):
             assert got[i, 1] == 0.0
         else:
             assert got[i, 1] == 1.0
-            assert abs(got[i, 0] - ref) <= 1e-12 * abs(ref)
+            # 1e-9: below the table the closed form of lp/linear.rs cancels terms of
+            # order 4/eta, so one ulp of log() between libraries grows to ~1e-11
+            assert abs(got[i, 0] - ref) <= 1e-9 * abs(ref)
 
 
 def test_lp_harmonic_matches_oracle(engine, orc):

@@ -241,7 +241,11 @@ typedef enum {
     PTG_FN_LP_HARMONIC = 9,      /* (a, eta, frac) -> n  nonlinear_compton/lp/mod.rs:478 */
     PTG_FN_DOUBLE_BESSEL = 10,   /* (n, x, y, recurrence_only) -> J(n-2..n+2, x, y)  special_functions/double_bessel.rs */
     PTG_FN_LP_SAMPLE = 11,       /* (a, eta, stream, fixed_n) -> (n, s, theta, S1, S2, trials)  lp/mod.rs:586 */
-    PTG_FN_COUNT = 12
+    PTG_FN_BW_RATE = 12,         /* (a, eta) -> (par, perp, table: 0 negligible, 1 mid, 2 high, -1 outside)
+                                    pair_creation/lp/mod.rs:385-401 */
+    PTG_FN_BW_HARMONIC = 13,     /* (a, eta, S1, frac) -> n  pair_creation/lp/mod.rs:424-436 */
+    PTG_FN_BW_SAMPLE = 14,       /* (a, eta, S1, S2, stream, fixed_n) -> (n, s, theta, trials)  lp/mod.rs:261-301 */
+    PTG_FN_COUNT = 15
 } ptg_eval_fn;
 int ptg_eval(ptg_ctx* ctx, int fn, const double* in, size_t n, size_t in_w, double* out, size_t out_w);
 

@@ -115,6 +115,24 @@ def lib():
         L.orc_lp_ceiling.restype = d
         L.orc_lp_ceiling.argtypes = [C.c_int, d, d]
         L.orc_lp_sample.argtypes = [d, d, C.c_uint64, C.c_int, C.c_size_t, P(d)]
+        # LP Breit-Wheeler
+        L.orc_bw_partial_rate.argtypes = [C.c_int, d, d, C.c_int, P(d)]
+        L.orc_bw_ddpr.restype = d
+        L.orc_bw_ddpr.argtypes = [C.c_int, d, d, d, d, d, d]
+        L.orc_bw_ceiling.restype = d
+        L.orc_bw_ceiling.argtypes = [C.c_int, d, d, d, d]
+        L.orc_bw_theta_bound.restype = d
+        L.orc_bw_theta_bound.argtypes = [C.c_int, d, d, d]
+        L.orc_bw_sum_limits.restype = C.c_int
+        L.orc_bw_sum_limits.argtypes = [d, d, P(C.c_int)]
+        L.orc_bw_rate.restype = C.c_int
+        L.orc_bw_rate.argtypes = [d, d, P(d)]
+        L.orc_bw_probability.restype = d
+        L.orc_bw_probability.argtypes = [d, d, P(d), d, P(d)]
+        L.orc_bw_by_summation.argtypes = [d, d, P(d)]
+        L.orc_bw_harmonic.restype = C.c_int
+        L.orc_bw_harmonic.argtypes = [d, d, d, d]
+        L.orc_bw_sample.argtypes = [d, d, d, d, C.c_uint64, C.c_int, C.c_size_t, P(d)]
         rc = L.orc_init(os.path.abspath(TABLES).encode())
         if rc != 0:
             raise RuntimeError(f"oracle: cannot load tables from {TABLES}")
@@ -257,4 +275,37 @@ def lp_sample(a, eta, count, seed=0, fixed_n=0):
     """count samples of (n, s, theta, S1, S2)."""
     out = np.zeros((count, 5))
     lib().orc_lp_sample(a, eta, seed, fixed_n, count, dptr(out))
+    return out
+
+
+# ---- LP Breit-Wheeler probes
+def bw_partial_rate(n, a, eta, method=0):
+    """(par, perp) partial rate of harmonic n; method 0 = integrate(), 1 direct, 2 adaptive."""
+    out = np.zeros(2)
+    lib().orc_bw_partial_rate(n, a, eta, method, dptr(out))
+    return out
+
+
+def bw_rate(a, eta):
+    """(status, [par, perp]): status 0 negligible, 1 mid-range table, 2 high-range table, -1 outside."""
+    out = np.zeros(2)
+    k = lib().orc_bw_rate(a, eta, dptr(out))
+    return k, out
+
+
+def bw_sum_limits(a, eta):
+    hi = C.c_int()
+    lo = lib().orc_bw_sum_limits(a, eta, C.byref(hi))
+    return lo, hi.value
+
+
+def bw_by_summation(a, eta):
+    out = np.zeros(48)
+    lib().orc_bw_by_summation(a, eta, dptr(out))
+    return out.reshape(16, 3)
+
+
+def bw_sample(a, eta, sv1, sv2, count, seed=0, fixed_n=0):
+    out = np.zeros((count, 3))
+    lib().orc_bw_sample(a, eta, sv1, sv2, seed, fixed_n, count, dptr(out))
     return out

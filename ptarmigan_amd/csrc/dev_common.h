@@ -9,6 +9,9 @@
 
 namespace ptg {
 
+// Pure-math helpers that the host-side table generator shares with the kernels.
+#define PTG_HD __host__ __device__
+
 // Physical constants, SI, 7 significant digits as in the reference (src/constants.rs:4-29).
 constexpr double C_LIGHT = 2.997925e8;
 constexpr double C_LIGHT_SQD = 89875517873681764.0;
@@ -89,13 +92,13 @@ __device__ __forceinline__ double mdot(d4 a, d4 b) { return a.t * b.t - a.x * b.
 
 // ---------------------------------------------------------------- complex
 struct cplx { double re, im; };
-__device__ __forceinline__ cplx operator+(cplx a, cplx b) { return {a.re + b.re, a.im + b.im}; }
-__device__ __forceinline__ cplx operator-(cplx a, cplx b) { return {a.re - b.re, a.im - b.im}; }
-__device__ __forceinline__ cplx operator*(cplx a, cplx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
-__device__ __forceinline__ cplx operator*(double s, cplx a) { return {s * a.re, s * a.im}; }
-__device__ __forceinline__ cplx operator+(double s, cplx a) { return {s + a.re, a.im}; }
-__device__ __forceinline__ cplx mul_i(cplx a) { return {-a.im, a.re}; }      // i * a
-__device__ __forceinline__ cplx mul_mi(cplx a) { return {a.im, -a.re}; }     // -i * a
+PTG_HD __forceinline__ cplx operator+(cplx a, cplx b) { return {a.re + b.re, a.im + b.im}; }
+PTG_HD __forceinline__ cplx operator-(cplx a, cplx b) { return {a.re - b.re, a.im - b.im}; }
+PTG_HD __forceinline__ cplx operator*(cplx a, cplx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+PTG_HD __forceinline__ cplx operator*(double s, cplx a) { return {s * a.re, s * a.im}; }
+PTG_HD __forceinline__ cplx operator+(double s, cplx a) { return {s + a.re, a.im}; }
+PTG_HD __forceinline__ cplx mul_i(cplx a) { return {-a.im, a.re}; }      // i * a
+PTG_HD __forceinline__ cplx mul_mi(cplx a) { return {a.im, -a.re}; }     // -i * a
 
 // ---------------------------------------------------------------- Stokes vectors
 struct sv4 { double i, q, u, v; };

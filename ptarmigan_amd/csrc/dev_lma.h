@@ -33,38 +33,38 @@ struct LmaScalars {
 };
 
 // ---------------------------------------------------------------- complex functions
-__device__ __forceinline__ double c_arg(cplx a) { return atan2(a.im, a.re); }
-__device__ __forceinline__ double c_abs(cplx a) { return hypot(a.re, a.im); }
-__device__ __forceinline__ cplx c_polar(double r, double th) { double s, c; sincos(th, &s, &c); return {r * c, r * s}; }
-__device__ __forceinline__ cplx c_log(cplx a) { return {log(c_abs(a)), c_arg(a)}; }
-__device__ __forceinline__ cplx c_sqrt(cplx a) {
+PTG_HD __forceinline__ double c_arg(cplx a) { return atan2(a.im, a.re); }
+PTG_HD __forceinline__ double c_abs(cplx a) { return hypot(a.re, a.im); }
+PTG_HD __forceinline__ cplx c_polar(double r, double th) { double s, c; sincos(th, &s, &c); return {r * c, r * s}; }
+PTG_HD __forceinline__ cplx c_log(cplx a) { return {log(c_abs(a)), c_arg(a)}; }
+PTG_HD __forceinline__ cplx c_sqrt(cplx a) {
     if (a.im == 0.0) {
-        if (!signbit(a.re)) return {sqrt(a.re), a.im};
+        if (!__builtin_signbit(a.re)) return {sqrt(a.re), a.im};
         double im = sqrt(-a.re);
-        return signbit(a.im) ? cplx{0.0, -im} : cplx{0.0, im};
+        return __builtin_signbit(a.im) ? cplx{0.0, -im} : cplx{0.0, im};
     }
     if (a.re == 0.0) {
         double x = sqrt(fabs(a.im) / 2.0);
-        return signbit(a.im) ? cplx{x, -x} : cplx{x, x};
+        return __builtin_signbit(a.im) ? cplx{x, -x} : cplx{x, x};
     }
     return c_polar(sqrt(c_abs(a)), c_arg(a) / 2.0);
 }
 // Products with i and -i written out as full complex products, (0, 1) * a and
 // (-0, -1) * a, so that the signs of zero parts match num-complex: they pick the
 // branch of c_arg (and so of c_acos) on the real axis.
-__device__ __forceinline__ cplx rmul_i(cplx a) { return {0.0 * a.re - a.im, 0.0 * a.im + a.re}; }
-__device__ __forceinline__ cplx rmul_mi(cplx a) { return {-0.0 * a.re + a.im, -0.0 * a.im - a.re}; }
-__device__ __forceinline__ cplx c_acos(cplx z) {  // -i ln(i sqrt(1 - z^2) + z)
+PTG_HD __forceinline__ cplx rmul_i(cplx a) { return {0.0 * a.re - a.im, 0.0 * a.im + a.re}; }
+PTG_HD __forceinline__ cplx rmul_mi(cplx a) { return {-0.0 * a.re + a.im, -0.0 * a.im - a.re}; }
+PTG_HD __forceinline__ cplx c_acos(cplx z) {  // -i ln(i sqrt(1 - z^2) + z)
     cplx zz{z.re * z.re - z.im * z.im, z.re * z.im + z.im * z.re};
     cplx w = rmul_i(c_sqrt(cplx{1.0 - zz.re, 0.0 - zz.im})) + z;
     return rmul_mi(c_log(w));
 }
-__device__ __forceinline__ cplx c_sin(cplx a) {
+PTG_HD __forceinline__ cplx c_sin(cplx a) {
     double s, c;
     sincos(a.re, &s, &c);
     return {s * cosh(a.im), c * sinh(a.im)};
 }
-__device__ __forceinline__ cplx c_exp(cplx a) { return c_polar(exp(a.re), a.im); }
+PTG_HD __forceinline__ cplx c_exp(cplx a) { return c_polar(exp(a.re), a.im); }
 
 // ---------------------------------------------------------------- plane wave (plane_wave.rs)
 // <a^2> and d<a^2>/dphase for the LMA envelope (plane_wave.rs:68-150).
@@ -255,20 +255,20 @@ struct DBessel {
     double* ws;   // 5 * len doubles, element (i, c) at ws[(5 * i + c) * st]
     size_t st;
 };
-__device__ __forceinline__ DBessel dbessel_at(int n, double* ws, size_t st) {
+PTG_HD __forceinline__ DBessel dbessel_at(int n, double* ws, size_t st) {
     double xm = double(n) * 1.4142135623730951, ym = double(n) * 0.5;
     int n_minus_bound = int(floor(-2.0 * ym - xm));
     int n_plus_bound = xm > 4.0 * (2.0 - 1.4142135623730951) * ym ? int(ceil(xm)) : int(ceil(2.0 * ym + xm * xm / (16.0 * ym)));
     return DBessel{n, n_minus_bound - 100, n_plus_bound - n_minus_bound + 200, ws, st};
 }
-__device__ __forceinline__ int dbessel_len(int n) {
+PTG_HD __forceinline__ int dbessel_len(int n) {
     double xm = double(n) * 1.4142135623730951, ym = double(n) * 0.5;
     int lo = int(floor(-2.0 * ym - xm));
     int hi = xm > 4.0 * (2.0 - 1.4142135623730951) * ym ? int(ceil(xm)) : int(ceil(2.0 * ym + xm * xm / (16.0 * ym)));
     return hi - lo + 200;
 }
 
-__device__ inline void dbessel_recurrence(const DBessel& D, double x, double y, double j[5]) {
+PTG_HD inline void dbessel_recurrence(const DBessel& D, double x, double y, double j[5]) {
     const int len = D.len;
     const size_t st = D.st;
     double* ws = D.ws;
@@ -357,12 +357,12 @@ __device__ inline void dbessel_recurrence(const DBessel& D, double x, double y, 
     for (int q = 0; q < 5; ++q) {
         double h = tgt_is_f[q] ? (g_match * tgt[q]) / f_match : tgt[q];
         double val = copysign(1.0, h / sum_linear) * fabs(h) / rss;
-        j[q] = isfinite(val) ? val : 0.0;
+        j[q] = __builtin_isfinite(val) ? val : 0.0;
     }
 }
 
 // Saddle-point approximation of J(n, x, y) for large n (double_bessel.rs:272-355).
-__device__ inline double dbessel_spa(int ni, double x, double y) {
+PTG_HD inline double dbessel_spa(int ni, double x, double y) {
     double n = double(ni);
     if (n <= -2.0 * y + x) {
         double cos_theta = y > 0.0 ? x / (8.0 * y) - sqrt(x * x / (64.0 * y * y) + 0.5 - n / (4.0 * y)) : n / x;
@@ -377,7 +377,7 @@ __device__ inline double dbessel_spa(int ni, double x, double y) {
     } else if (8.0 * y < x || n > 2.0 * y + x * x / (16.0 * y)) {
         double disc = x * x / (64.0 * y * y) + 0.5 - n / (4.0 * y);
         cplx cos_theta;
-        if (isnan(disc)) cos_theta = {n / x, 0.0};
+        if (__builtin_isnan(disc)) cos_theta = {n / x, 0.0};
         else if (disc < 0.0) cos_theta = {x / (8.0 * y), sqrt(-disc)};
         else cos_theta = {x / (8.0 * y) - sqrt(disc), 0.0};
         cplx theta = c_acos(cos_theta);
@@ -406,14 +406,14 @@ __device__ inline double dbessel_spa(int ni, double x, double y) {
         return out;
     }
 }
-__device__ inline bool dbessel_use_spa(int n, double x, double y) {
+PTG_HD inline bool dbessel_use_spa(int n, double x, double y) {
     if (n <= 10) return false;
     double ne = double(n);
     ne = ne * (1.0 - sqrt(10.0 / ne));
     double x_crit = y > ne / 6.0 ? 4.0 * sqrt(y * (ne - 2.0 * y)) : ne + 2.0 * y;
     return x < x_crit;
 }
-__device__ inline void dbessel_eval(const DBessel& D, double x, double y, double j[5]) {
+PTG_HD inline void dbessel_eval(const DBessel& D, double x, double y, double j[5]) {
     if (dbessel_use_spa(D.n, x, y))
         for (int q = 0; q < 5; ++q) j[q] = dbessel_spa(D.n - 2 + q, x, y);
     else
@@ -476,7 +476,7 @@ __device__ inline double lp_ceiling(const DBessel& D, double a, double eta) {
 
 // Largest theta contributing at each s (lp/mod.rs:143-301).
 struct ThetaBound { double s[16], f[16]; };
-__device__ inline double ln_dbessel_x_zero(int ni, double y) {
+PTG_HD inline double ln_dbessel_x_zero(int ni, double y) {
     double n = double(ni);
     double z = sqrt(n / (4.0 * y) - 0.5);
     cplx theta{PI_D / 2.0, log(sqrt(1.0 + z * z) - z)};
@@ -592,6 +592,16 @@ __device__ __forceinline__ sv4 stokes_from_lma(sv4 s, d4 k) {
     double sign = signbit(k.x * k.y) ? -1.0 : 1.0;
     cos_t = fmin(fmax(cos_t, -1.0), 1.0);
     double sin_t = -(sign * sqrt(1.0 - cos_t * cos_t));
+    double c2 = cos_t * cos_t - sin_t * sin_t, s2 = 2.0 * cos_t * sin_t;
+    return {s.i, c2 * s.q - s2 * s.u, s2 * s.q + c2 * s.u, s.v};
+}
+// polarization.rs:116-131 (in_lma_basis): the inverse rotation of stokes_from_lma
+__device__ __forceinline__ sv4 stokes_to_lma(sv4 s, d4 k) {
+    double k_minus = k.t - k.z;
+    double cos_t = (k.z * k_minus - k.x * k.x) / (k_minus * hypot(k.x, k.z));
+    double sign = signbit(k.x * k.y) ? -1.0 : 1.0;
+    cos_t = fmin(fmax(cos_t, -1.0), 1.0);
+    double sin_t = sign * sqrt(1.0 - cos_t * cos_t);
     double c2 = cos_t * cos_t - sin_t * sin_t, s2 = 2.0 * cos_t * sin_t;
     return {s.i, c2 * s.q - s2 * s.u, s2 * s.q + c2 * s.u, s.v};
 }

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "kern_lma.h"
+#include "dev_bw.h"
 #include "kern_misc.h"
 
 using namespace ptg;
@@ -76,7 +77,7 @@ void free_buf(DevBuf& b) {
 }
 
 // Counter block layout (unsigned long long)
-enum { C_WORK = 0, C_NEXT, C_OUT, C_INTER, C_ID, C_FLAGS, C_EV, C_LNEXT, C_PH, C_OVF, C_OVFLEN, C_STATS, C_N = C_STATS + STAT_N };
+enum { C_WORK = 0, C_NEXT, C_OUT, C_INTER, C_ID, C_FLAGS, C_EV, C_LNEXT, C_PH, C_OVF, C_OVFLEN, C_PAIRS, C_STATS, C_N = C_STATS + STAT_N };
 
 std::string default_table_path() {
     Dl_info info;
@@ -101,8 +102,10 @@ struct ptg_ctx {
     LcfaScalars sc{};
     double* d_lma = nullptr;
     LmaScalars lsc{};
+    double* d_bw = nullptr;         // LP Breit-Wheeler pack (dev_bw.h); null if its tables are absent
+    BwScalars bsc{};
     double lma_bandwidth = 0.0, pol_angle = 0.0;
-    DevBuf evq, lq[2], phq;
+    DevBuf evq, lq[2], phq, phq2, pairq;
     double* d_ev_extra = nullptr;  // ev_k0, ev_a
     size_t ev_extra_cap = 0;
     double* d_scratch = nullptr;
@@ -258,6 +261,41 @@ bool build_lma_pack(ptg_ctx* c, std::vector<double>& pack) {
     return true;
 }
 
+// LP nonlinear Breit-Wheeler tables (dev_bw.h). The mid-range CDF is regenerated
+// (tables/gen_bw_tables.py); without it, pair creation in a PlaneWave is unavailable.
+bool build_bw_pack(ptg_ctx* c, std::vector<double>& pack) {
+    using namespace bw_pack;
+    pack.assign(SIZE, 0.0);
+    auto get = [&](const char* name, size_t n) -> const std::vector<double>* {
+        auto it = c->tables.find(name);
+        return (it == c->tables.end() || it->second.size() != n) ? nullptr : &it->second;
+    };
+    const auto* hi = get("pair_creation/lp/tables/total:TABLE", size_t(HI_ROWS) * HI_COLS * 2);
+    const auto* hic = get("pair_creation/lp/tables/cdf:TABLE", size_t(HI_CDF_N) * 48);
+    const auto* mid = get("pair_creation/lp/tables/mid_range/total:TABLE", size_t(MID_ROWS) * MID_COLS * 2);
+    const auto* midc = get("pair_creation/lp/tables/mid_range/cdf:TABLE", size_t(MID_CDF_N) * 48);
+    auto one = [&](const char* name) { const auto* v = get(name, 1); return v ? (*v)[0] : NAN; };
+    const auto* cmin = get("pair_creation/lp/tables/cdf:MIN", 2);
+    const auto* cstep = get("pair_creation/lp/tables/cdf:STEP", 2);
+    if (!hi || !hic || !mid || !midc || !cmin || !cstep) return false;
+    std::copy(hi->begin(), hi->end(), pack.begin() + HI_TOTAL);
+    std::copy(hic->begin(), hic->end(), pack.begin() + HI_CDF);
+    std::copy(mid->begin(), mid->end(), pack.begin() + MID_TOTAL);
+    std::copy(midc->begin(), midc->end(), pack.begin() + MID_CDF);
+    BwScalars& b = c->bsc;
+    b.hi_ln_min_a = one("pair_creation/lp/tables/total:LN_MIN_A");
+    b.hi_ln_min_eta = one("pair_creation/lp/tables/total:LN_MIN_ETA");
+    b.hi_ln_a_step = one("pair_creation/lp/tables/total:LN_A_STEP");
+    b.hi_ln_eta_step = one("pair_creation/lp/tables/total:LN_ETA_STEP");
+    for (int k = 0; k < 2; ++k) { b.hi_cdf_min[k] = (*cmin)[k]; b.hi_cdf_step[k] = (*cstep)[k]; }
+    b.mid_ln_min_a = one("pair_creation/lp/tables/mid_range/total:LN_MIN_A");
+    b.mid_ln_max_eta_prime = one("pair_creation/lp/tables/mid_range/total:LN_MAX_ETA_PRIME");
+    b.mid_ln_min_eta_prime = one("pair_creation/lp/tables/mid_range/total:LN_MIN_ETA_PRIME");
+    b.mid_ln_a_step = one("pair_creation/lp/tables/mid_range/total:LN_A_STEP");
+    b.mid_eta_prime_density = one("pair_creation/lp/tables/mid_range/total:ETA_PRIME_DENSITY");
+    return std::isfinite(b.hi_ln_min_a) && std::isfinite(b.mid_eta_prime_density);
+}
+
 // Laser constants (fast_focused_laser.rs:24-38,47-53,129-169; fast_plane_wave.rs:23-108;
 // plane_wave.rs:27-63,176-192).
 int make_laser(ptg_ctx* c, const ptg_laser_desc* d, LaserDev& L) {
@@ -351,6 +389,11 @@ int ptg_create(ptg_ctx** out, const char* table_path, int device) {
     std::vector<double> lpack;
     if (!build_lma_pack(c, lpack)) { *out = c; return PTG_E_TABLES; }
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    std::vector<double> bpack;
+    if (build_bw_pack(c, bpack)) {
+        HIPCHK(c, hipMalloc(&c->d_bw, bpack.size() * sizeof(double)));
+        HIPCHK(c, scopy(c, c->d_bw, bpack.data(), bpack.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     HIPCHK(c, hipMalloc(&c->d_lma, lpack.size() * sizeof(double)));
     HIPCHK(c, scopy(c, c->d_lma, lpack.data(), lpack.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMalloc(&c->d_lcfa, pack.size() * sizeof(double)));
@@ -370,11 +413,12 @@ void ptg_destroy(ptg_ctx* c) {
     free_buf(c->prim); free_buf(c->q[0]); free_buf(c->q[1]); free_buf(c->out); free_buf(c->inter); free_buf(c->tmp);
     if (c->d_lcfa) (void)hipFree(c->d_lcfa);
     if (c->d_lma) (void)hipFree(c->d_lma);
+    if (c->d_bw) (void)hipFree(c->d_bw);
     if (c->d_ev_extra) (void)hipFree(c->d_ev_extra);
     if (c->d_scratch) (void)hipFree(c->d_scratch);
     if (c->d_ovf) (void)hipFree(c->d_ovf);
     if (c->d_scratch2) (void)hipFree(c->d_scratch2);
-    free_buf(c->evq); free_buf(c->lq[0]); free_buf(c->lq[1]); free_buf(c->phq);
+    free_buf(c->evq); free_buf(c->lq[0]); free_buf(c->lq[1]); free_buf(c->phq); free_buf(c->phq2); free_buf(c->pairq);
     if (c->d_n0) (void)hipFree(c->d_n0);
     if (c->d_shower_count) (void)hipFree(c->d_shower_count);
     if (c->d_ctr) (void)hipFree(c->d_ctr);
@@ -582,11 +626,81 @@ static int ensure_ev_extra(ptg_ctx* c, size_t n) {
     return PTG_OK;
 }
 
-// The LMA loop: step kernel over the live electrons, then the event kernel over
-// the ones that emitted, until every electron has left the field. Photons of the
-// primary beam are handed to the photon pass-through at the end.
+// Runs the LMA event kernel (emissions, or pairs when `pairs`) over the n_ev
+// events in evq. Events whose harmonic needs a J(n, x, y) workspace larger than
+// m.scratch_len are deferred and run again with one sized for the largest; their
+// RNG counters make the second pass draw the same numbers.
+static int lma_event_pass(ptg_ctx* c, LmaArgs m, bool pairs, size_t n_ev, size_t max_lanes, double* ms, uint64_t* launches) {
+    if (c->ovf_cap < c->evq.cap) {
+        if (c->d_ovf) (void)hipFree(c->d_ovf);
+        c->d_ovf = nullptr;
+        HIPCHK(c, hipMalloc(&c->d_ovf, c->evq.cap * sizeof(uint32_t)));
+        c->ovf_cap = c->evq.cap;
+    }
+    HIPCHK(c, hipMemsetAsync(c->d_ctr + C_OVF, 0, 2 * sizeof(unsigned long long), c->stream));
+    m.ev_in = c->evq.v;
+    m.ev_list = nullptr;
+    m.n_ev = n_ev;
+    m.ovf_list = c->d_ovf;
+    m.ovf_ctr = c->d_ctr + C_OVF;
+    m.ovf_len = c->d_ctr + C_OVFLEN;
+    size_t lanes = std::min<size_t>(max_lanes, ((n_ev + 63) / 64) * 64);
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    if (pairs) launch_pair_event_lma(c->d_args, m, int(lanes), c->stream);
+    else launch_event_lma(c->d_args, m, int(lanes), c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    unsigned long long ovf[2] = {0, 0};
+    HIPCHK(c, hipMemcpyAsync(ovf, c->d_ctr + C_OVF, sizeof(ovf), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float t = 0.0f;
+    (void)hipEventElapsedTime(&t, c->ev0, c->ev1);
+    *ms += t;
+    ++*launches;
+    if (ovf[0] == 0) return PTG_OK;
+    const size_t per_lane = size_t(ovf[1]) * 5 * sizeof(double);
+    size_t olanes = std::min<size_t>(((ovf[0] + 63) / 64) * 64, ((size_t(8) << 30) / per_lane / 64) * 64);
+    olanes = std::max<size_t>(olanes, 64);
+    size_t bytes = olanes * per_lane;
+    if (bytes > (size_t(16) << 30)) return fail(c, PTG_E_DOMAIN, "LMA harmonic index beyond the supported range");
+    if (bytes > c->scratch2_bytes) {
+        if (c->d_scratch2) (void)hipFree(c->d_scratch2);
+        c->d_scratch2 = nullptr;
+        HIPCHK(c, hipMalloc(&c->d_scratch2, bytes));
+        c->scratch2_bytes = bytes;
+    }
+    m.ev_list = c->d_ovf;
+    m.n_ev = ovf[0];
+    m.ovf_list = nullptr;
+    m.scratch = c->d_scratch2;
+    m.scratch_len = int(ovf[1]);
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    if (pairs) launch_pair_event_lma(c->d_args, m, int(olanes), c->stream);
+    else launch_event_lma(c->d_args, m, int(olanes), c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipEventElapsedTime(&t, c->ev0, c->ev1);
+    *ms += t;
+    ++*launches;
+    return PTG_OK;
+}
+
+static unsigned long long read_ctr(ptg_ctx* c, int which) {
+    unsigned long long v = 0;
+    if (scopy(c, &v, c->d_ctr + which, sizeof(v), hipMemcpyDeviceToHost) != hipSuccess) return ~0ull;
+    return v;
+}
+
+// The LMA loop. Lepton phase: the step kernel over the live electrons and
+// positrons, then the emission-event kernel over the ones that emitted, until
+// none is left in the field. Photon phase (photons tracked): the photon step
+// kernel, then the pair-event kernel, until no photon is left; the pairs start
+// the next lepton phase. Without photon tracking, photons are final state and
+// those of the primary beam only pass the final cuts.
 static int run_lma(ptg_ctx* c, TransportArgs a, double* ms, uint64_t* launches, uint64_t* gens) {
     const size_t n = c->n_prim;
+    const bool track = a.opt.tracking_photons != 0;
     int len_cap = 0;
     {
         // workspace per lane for J(n, x, y), 5 doubles per order
@@ -613,127 +727,160 @@ static int run_lma(ptg_ctx* c, TransportArgs a, double* ms, uint64_t* launches, 
     LmaArgs m{};
     m.tables = c->d_lma;
     m.sc = c->lsc;
+    m.bw = c->d_bw;
+    m.bsc = c->bsc;
     m.bandwidth = c->lma_bandwidth;
     m.pol_angle = c->pol_angle;
     m.sin_pa = std::sin(c->pol_angle);
     m.cos_pa = std::cos(c->pol_angle);
     m.ev_ctr = c->d_ctr + C_EV;
-    m.next_ctr = c->d_ctr + C_LNEXT;
     m.ph_ctr = c->d_ctr + C_PH;
-    m.photons_to_out = 1;
+    m.pairs_ctr = c->d_ctr + C_PAIRS;
+    m.photons_to_out = track ? 0 : 1;
     m.scratch = c->d_scratch;
     m.scratch_len = len_cap;
 
-    if (c->phq.cap < n) HIPCHK(c, alloc_buf(c->phq, n));  // photons of the primary beam
+    const unsigned long long zero = 0;
+    auto zero_ctr = [&](int which) { return hipMemcpyAsync(c->d_ctr + which, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream); };
+    HIPCHK(c, zero_ctr(C_NEXT));
+    HIPCHK(c, zero_ctr(C_PH));
     DevBuf* cur = &c->prim;
     size_t n_cur = n;
     int li = 0;
-    const unsigned long long zero = 0;
-    HIPCHK(c, scopy(c, c->d_ctr + C_NEXT, &zero, sizeof(zero), hipMemcpyHostToDevice));
-    while (n_cur > 0) {
-        // capacities: each live electron yields at most one final state or one event,
-        // each event at most one photon and one continuing electron
-        HIPCHK(c, grow_buf(c->out, c->n_out + 2 * n_cur + 1024, c->n_out, c->stream));
-        if (c->evq.cap < n_cur) HIPCHK(c, alloc_buf(c->evq, n_cur + n_cur / 4));
-        int rc = ensure_ev_extra(c, c->evq.cap);
-        if (rc) return rc;
-        DevBuf& nxt = c->lq[li];
-        if (&nxt == cur) return fail(c, PTG_E_STATE, "internal queue aliasing");
-        if (nxt.cap < n_cur) HIPCHK(c, alloc_buf(nxt, n_cur + n_cur / 4));
-        HIPCHK(c, hipMemcpyAsync(c->d_ctr + C_WORK, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->d_ctr + C_EV, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->d_ctr + C_LNEXT, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
-        a.in = cur->v;
-        a.n_in = uint32_t(n_cur);
-        a.out = c->out.v;
-        a.inter = c->inter.v;
-        a.next = c->phq.v;  // photons forwarded by the step kernel
-        a.next_cap = c->phq.cap;
-        m.ev = c->evq.v;
-        m.ev_cap = c->evq.cap;
-        m.ev_k0 = c->d_ev_extra;
-        m.ev_a = c->d_ev_extra + c->ev_extra_cap;
-        HIPCHK(c, hipMemcpyAsync(c->d_args, &a, sizeof(a), hipMemcpyHostToDevice, c->stream));
-        int grid = int(std::min<size_t>(size_t(c->n_cu) * 4, (n_cur + 255) / 256));
-        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-        launch_step_lma(c->d_args, m, grid, c->stream);
-        HIPCHK(c, hipGetLastError());
-        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-        unsigned long long n_ev = 0;
-        HIPCHK(c, hipMemcpyAsync(&n_ev, c->d_ctr + C_EV, sizeof(n_ev), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        float t = 0.0f;
-        (void)hipEventElapsedTime(&t, c->ev0, c->ev1);
-        ms[0] += t;
-        ++launches[0];
-        if (n_ev > 0) {
-            if (c->ovf_cap < c->evq.cap) {
-                if (c->d_ovf) (void)hipFree(c->d_ovf);
-                c->d_ovf = nullptr;
-                HIPCHK(c, hipMalloc(&c->d_ovf, c->evq.cap * sizeof(uint32_t)));
-                c->ovf_cap = c->evq.cap;
-            }
-            HIPCHK(c, hipMemsetAsync(c->d_ctr + C_OVF, 0, 2 * sizeof(unsigned long long), c->stream));
-            m.ev_in = c->evq.v;
-            m.ev_list = nullptr;
-            m.n_ev = n_ev;
-            m.next = nxt.v;
-            m.ovf_list = c->d_ovf;
-            m.ovf_ctr = c->d_ctr + C_OVF;
-            m.ovf_len = c->d_ctr + C_OVFLEN;
-            size_t lanes = std::min<size_t>(max_lanes, ((n_ev + 63) / 64) * 64);
+    for (;;) {
+        // ---------------- lepton phase
+        while (n_cur > 0) {
+            // each live lepton yields at most one final state or one event; each event
+            // at most one photon and one continuing lepton
+            HIPCHK(c, grow_buf(c->out, c->n_out + 2 * n_cur + 1024, c->n_out, c->stream));
+            if (c->evq.cap < n_cur) HIPCHK(c, alloc_buf(c->evq, n_cur + n_cur / 4));
+            int rc = ensure_ev_extra(c, c->evq.cap);
+            if (rc) return rc;
+            DevBuf& nxt = c->lq[li];
+            if (&nxt == cur) return fail(c, PTG_E_STATE, "internal queue aliasing");
+            if (nxt.cap < n_cur) HIPCHK(c, alloc_buf(nxt, n_cur + n_cur / 4));
+            unsigned long long n_ph = read_ctr(c, track ? C_PH : C_NEXT);
+            HIPCHK(c, grow_buf(c->phq, size_t(n_ph) + 2 * n_cur + 1024, size_t(n_ph), c->stream));
+            HIPCHK(c, zero_ctr(C_WORK));
+            HIPCHK(c, zero_ctr(C_EV));
+            HIPCHK(c, zero_ctr(C_LNEXT));
+            a.in = cur->v;
+            a.n_in = uint32_t(n_cur);
+            a.out = c->out.v;
+            a.inter = c->inter.v;
+            a.next = c->phq.v;  // photons among the inputs are handed on here
+            a.next_ctr = c->d_ctr + (track ? C_PH : C_NEXT);
+            a.next_cap = c->phq.cap;
+            m.ev = c->evq.v;
+            m.ev_cap = c->evq.cap;
+            m.ev_k0 = c->d_ev_extra;
+            m.ev_a = c->d_ev_extra + c->ev_extra_cap;
+            m.ph = c->phq.v;
+            m.ph_cap = c->phq.cap;
+            HIPCHK(c, hipMemcpyAsync(c->d_args, &a, sizeof(a), hipMemcpyHostToDevice, c->stream));
+            int grid = int(std::min<size_t>(size_t(c->n_cu) * 4, (n_cur + 255) / 256));
             HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-            launch_event_lma(c->d_args, m, int(lanes), c->stream);
+            launch_step_lma(c->d_args, m, grid, c->stream);
             HIPCHK(c, hipGetLastError());
             HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-            unsigned long long ovf[2] = {0, 0};
-            HIPCHK(c, hipMemcpyAsync(ovf, c->d_ctr + C_OVF, sizeof(ovf), hipMemcpyDeviceToHost, c->stream));
+            unsigned long long n_ev = 0;
+            HIPCHK(c, hipMemcpyAsync(&n_ev, c->d_ctr + C_EV, sizeof(n_ev), hipMemcpyDeviceToHost, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
+            float t = 0.0f;
+            (void)hipEventElapsedTime(&t, c->ev0, c->ev1);
+            ms[0] += t;
+            ++launches[0];
+            if (n_ev > 0) {
+                m.next = nxt.v;
+                m.next_ctr = c->d_ctr + C_LNEXT;
+                rc = lma_event_pass(c, m, false, size_t(n_ev), max_lanes, &ms[1], &launches[1]);
+                if (rc) return rc;
+            }
+            unsigned long long now[C_N];
+            HIPCHK(c, scopy(c, now, c->d_ctr, sizeof(now), hipMemcpyDeviceToHost));
+            c->n_out = now[C_OUT];
+            cur = &nxt;
+            n_cur = size_t(now[C_LNEXT]);
+            li ^= 1;
+            ++*gens;
+        }
+        if (!track) break;
+        // ---------------- photon phase
+        size_t n_ph = size_t(read_ctr(c, C_PH));
+        if (n_ph == 0) break;
+        HIPCHK(c, zero_ctr(C_PAIRS));
+        DevBuf* pcur = &c->phq;
+        while (n_ph > 0) {
+            HIPCHK(c, grow_buf(c->out, c->n_out + n_ph + 1024, c->n_out, c->stream));
+            if (a.opt.keep_decayed_photons) HIPCHK(c, grow_buf(c->inter, c->n_inter + n_ph + 1024, c->n_inter, c->stream));
+            if (c->evq.cap < n_ph) HIPCHK(c, alloc_buf(c->evq, n_ph + n_ph / 4));
+            int rc = ensure_ev_extra(c, c->evq.cap);
+            if (rc) return rc;
+            DevBuf& pnxt = pcur == &c->phq ? c->phq2 : c->phq;
+            if (pnxt.cap < n_ph) HIPCHK(c, alloc_buf(pnxt, n_ph + n_ph / 4));
+            unsigned long long n_pairs = read_ctr(c, C_PAIRS);
+            HIPCHK(c, grow_buf(c->pairq, size_t(n_pairs) + 2 * n_ph + 1024, size_t(n_pairs), c->stream));
+            HIPCHK(c, zero_ctr(C_WORK));
+            HIPCHK(c, zero_ctr(C_EV));
+            HIPCHK(c, zero_ctr(C_LNEXT));
+            a.in = pcur->v;
+            a.n_in = uint32_t(n_ph);
+            a.out = c->out.v;
+            a.inter = c->inter.v;
+            a.next = c->pairq.v;  // (no leptons are queued with photons)
+            a.next_ctr = c->d_ctr + C_PAIRS;
+            a.next_cap = c->pairq.cap;
+            m.ev = c->evq.v;
+            m.ev_cap = c->evq.cap;
+            m.ev_k0 = c->d_ev_extra;
+            m.ev_a = c->d_ev_extra + c->ev_extra_cap;
+            HIPCHK(c, hipMemcpyAsync(c->d_args, &a, sizeof(a), hipMemcpyHostToDevice, c->stream));
+            int grid = int(std::min<size_t>(size_t(c->n_cu) * 4, (n_ph + 255) / 256));
+            HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+            launch_photon_step_lma(c->d_args, m, grid, c->stream);
+            HIPCHK(c, hipGetLastError());
+            HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+            unsigned long long n_ev = 0;
+            HIPCHK(c, hipMemcpyAsync(&n_ev, c->d_ctr + C_EV, sizeof(n_ev), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            float t = 0.0f;
             (void)hipEventElapsedTime(&t, c->ev0, c->ev1);
             ms[1] += t;
             ++launches[1];
-            if (ovf[0] > 0) {
-                // rare high harmonics: one 64-lane pass with a workspace for the largest
-                const size_t olanes = 64;
-                size_t bytes = olanes * size_t(ovf[1]) * 5 * sizeof(double);
-                if (bytes > (size_t(16) << 30)) return fail(c, PTG_E_DOMAIN, "LMA harmonic index beyond the supported range");
-                if (bytes > c->scratch2_bytes) {
-                    if (c->d_scratch2) (void)hipFree(c->d_scratch2);
-                    c->d_scratch2 = nullptr;
-                    HIPCHK(c, hipMalloc(&c->d_scratch2, bytes));
-                    c->scratch2_bytes = bytes;
-                }
-                LmaArgs mo = m;
-                mo.ev_list = c->d_ovf;
-                mo.n_ev = ovf[0];
-                mo.ovf_list = nullptr;
-                mo.scratch = c->d_scratch2;
-                mo.scratch_len = int(ovf[1]);
-                launch_event_lma(c->d_args, mo, int(olanes), c->stream);
-                HIPCHK(c, hipGetLastError());
-                HIPCHK(c, hipStreamSynchronize(c->stream));
-                ++launches[1];
+            if (n_ev > 0) {
+                m.next = pnxt.v;
+                m.next_ctr = c->d_ctr + C_LNEXT;
+                m.pairs = c->pairq.v;
+                m.pairs_cap = c->pairq.cap;
+                rc = lma_event_pass(c, m, true, size_t(n_ev), max_lanes, &ms[1], &launches[1]);
+                if (rc) return rc;
             }
+            unsigned long long now[C_N];
+            HIPCHK(c, scopy(c, now, c->d_ctr, sizeof(now), hipMemcpyDeviceToHost));
+            c->n_out = now[C_OUT];
+            c->n_inter = now[C_INTER];
+            pcur = &pnxt;
+            n_ph = size_t(now[C_LNEXT]);
+            ++*gens;
         }
-        unsigned long long now[C_N];
-        HIPCHK(c, scopy(c, now, c->d_ctr, sizeof(now), hipMemcpyDeviceToHost));
-        c->n_out = now[C_OUT];
-        cur = &nxt;
-        n_cur = size_t(now[C_LNEXT]);
-        li ^= 1;
-        ++*gens;
+        // the pairs start the next lepton phase
+        n_cur = size_t(read_ctr(c, C_PAIRS));
+        if (n_cur == 0) break;
+        cur = &c->pairq;
+        HIPCHK(c, zero_ctr(C_PH));
     }
-    // photons of the primary beam: not tracked in the LMA path, final cut only
-    unsigned long long n_ph = 0;
-    HIPCHK(c, scopy(c, &n_ph, c->d_ctr + C_NEXT, sizeof(n_ph), hipMemcpyDeviceToHost));
-    if (n_ph > 0) {
-        TransportArgs b = a;
-        b.laser.kind = K_FAST_PLANE_WAVE;  // the pass-through never evaluates fields
-        b.opt.tracking_photons = 0;
-        double pms = 0.0;
-        uint64_t pl = 0;
-        int rc = transport(c, b, true, c->phq, size_t(n_ph), c->q[0], &pms, &pl);
-        if (rc) return rc;
+    if (!track) {
+        // photons of the primary beam: not tracked, final cuts only
+        unsigned long long n_ph = read_ctr(c, C_NEXT);
+        if (n_ph > 0) {
+            TransportArgs b = a;
+            b.laser.kind = K_FAST_PLANE_WAVE;  // the pass-through never evaluates fields
+            b.opt.tracking_photons = 0;
+            double pms = 0.0;
+            uint64_t pl = 0;
+            int rc = transport(c, b, true, c->phq, size_t(n_ph), c->q[0], &pms, &pl);
+            if (rc) return rc;
+        }
     }
     return PTG_OK;
 }
@@ -745,8 +892,9 @@ int ptg_run(ptg_ctx* c, const ptg_collide_options* o, uint64_t seed, ptg_counter
     int rc = validate_options(c, o);
     if (rc) return rc;
     if (c->laser.kind == K_PLANE_WAVE) {
-        if (o->tracking_photons)
-            return fail(c, PTG_E_UNSUPPORTED, "photon tracking (LMA Breit-Wheeler pair creation) in a PlaneWave is not built into this library yet");
+        if (o->tracking_photons && !c->d_bw)
+            return fail(c, PTG_E_UNSUPPORTED, "pair creation in a PlaneWave needs the regenerated Breit-Wheeler mid-range table "
+                                              "(tables/gen_bw_tables.py, then tables/extract.py)");
         if (o->classical) return fail(c, PTG_E_UNSUPPORTED, "classical LMA emission is not built into this library");
         if (c->laser.circular) return fail(c, PTG_E_UNSUPPORTED, "circularly polarised LMA emission is not built into this library yet");
     }
@@ -1005,14 +1153,25 @@ int ptg_histogram(ptg_ctx* c, ptg_hist_spec* s, double* bins_out, double* total_
 int ptg_eval(ptg_ctx* c, int fn, const double* in, size_t n, size_t in_w, double* out, size_t out_w) {
     if (!c || !in || !out) return fail(c, PTG_E_ARG, "null argument");
     if (fn < 0 || fn >= PTG_FN_COUNT) return fail(c, PTG_E_ARG, "unknown function id");
-    static const int need_in[] = {2, 5, 1, 1, 4, 1, 3, 12, 2, 3, 4, 4};
-    static const int need_out[] = {1, 4, 2, 4, 7, 1, 1, 4, 2, 1, 5, 6};
+    static const int need_in[] = {2, 5, 1, 1, 4, 1, 3, 12, 2, 3, 4, 4, 2, 4, 6};
+    static const int need_out[] = {1, 4, 2, 4, 7, 1, 1, 4, 2, 1, 5, 6, 3, 1, 4};
     if (int(in_w) < need_in[fn] || int(out_w) < need_out[fn]) return fail(c, PTG_E_ARG, "row width too small for this function");
     if (fn == PTG_FN_FIELDS && !c->have_laser) return fail(c, PTG_E_STATE, "no laser set");
     if (n == 0) return PTG_OK;
+    if (fn >= PTG_FN_BW_RATE && !c->d_bw) return fail(c, PTG_E_UNSUPPORTED, "Breit-Wheeler tables not loaded");
     if (fn >= PTG_FN_LP_RATE) {
         HIPCHK(c, hipSetDevice(c->device));
         int nmax = 16;
+        if (fn == PTG_FN_BW_SAMPLE) {
+            for (size_t i = 0; i < n; ++i) {
+                const double* x = in + i * in_w;
+                if (!(x[5] >= 0.0 && x[5] <= 2.0e5)) return fail(c, PTG_E_ARG, "BW sample probe needs 0 <= fixed n <= 2e5");
+                int lo = 0, hi = 0;
+                if (x[5] > 0) hi = int(x[5]);
+                else if (x[0] > 0.0 && x[1] > 0.0) bw_sum_limits(x[0], x[1], lo, hi);
+                nmax = std::max(nmax, std::min(hi, 200000));
+            }
+        }
         if (fn == PTG_FN_DOUBLE_BESSEL || fn == PTG_FN_LP_SAMPLE) {
             for (size_t i = 0; i < n; ++i) {
                 const double* x = in + i * in_w;
@@ -1048,8 +1207,8 @@ int ptg_eval(ptg_ctx* c, int fn, const double* in, size_t n, size_t in_w, double
             HIPCHK(c, hipMalloc(&dscr, m * len * 5 * 8));
             HIPCHK(c, scopy(c, din, hin.data(), m * in_w * 8, hipMemcpyHostToDevice));
             HIPCHK(c, sfill(c, dout, 0, m * out_w * 8));
-            launch_eval_lma(fn, din, dout, uint32_t(m), uint32_t(in_w), uint32_t(out_w), c->d_lma, c->lsc, dscr, int(len),
-                            key_from_seed(0), c->stream);
+            launch_eval_lma(fn, din, dout, uint32_t(m), uint32_t(in_w), uint32_t(out_w), c->d_lma, c->lsc, c->d_bw, c->bsc, dscr,
+                            int(len), key_from_seed(0), c->stream);
             HIPCHK(c, hipGetLastError());
             HIPCHK(c, hipStreamSynchronize(c->stream));
             HIPCHK(c, scopy(c, hout.data(), dout, m * out_w * 8, hipMemcpyDeviceToHost));
@@ -1058,7 +1217,7 @@ int ptg_eval(ptg_ctx* c, int fn, const double* in, size_t n, size_t in_w, double
             nmax = 16;
             for (size_t k = 0; k < m; ++k) {
                 std::memcpy(out + rows[k] * out_w, &hout[k * out_w], out_w * 8);
-                if (fn == PTG_FN_LP_SAMPLE && hout[k * out_w + 5] < 0.0) {
+                if ((fn == PTG_FN_LP_SAMPLE && hout[k * out_w + 5] == -1.0) || (fn == PTG_FN_BW_SAMPLE && hout[k * out_w + 3] == -1.0)) {
                     again.push_back(rows[k]);
                     nmax = std::max(nmax, int(hout[k * out_w]));
                 }

@@ -96,12 +96,12 @@ __device__ __forceinline__ void finalize_lepton(const TransportArgs& a, uint32_t
     if (a.opt.multiplicity_on) atomicAdd(a.shower_count + shower, 1u);
 }
 
-__device__ __forceinline__ void finalize_photon(const TransportArgs& a, uint32_t i, d4 r, d4 k, sv4 pol, double weight,
-                                                double icount, uint32_t meta, bool decayed) {
+__device__ __forceinline__ void finalize_photon_from(const TransportArgs& a, const PBuf& src, uint32_t i, d4 r, d4 k, sv4 pol,
+                                                     double weight, double icount, uint32_t meta, bool decayed) {
     bool primary = meta & META_PRIMARY;
     const PBuf* o = nullptr;
     unsigned long long* ctr = nullptr;
-    uint32_t shower = a.in.shower[i];
+    uint32_t shower = src.shower[i];
     if (!decayed) {
         if (primary && a.opt.discard_bg_ph) return;
         if (!passes_cut(a, shower, M_E_MEV * k.t, sp(k))) return;
@@ -114,13 +114,17 @@ __device__ __forceinline__ void finalize_photon(const TransportArgs& a, uint32_t
         ctr = a.inter_ctr;
     }
     uint64_t j = atomicAdd(ctr, 1ull);
-    copy_record(*o, j, a.in, i);
+    copy_record(*o, j, src, i);
     o->d[P_RT][j] = r.t; o->d[P_RX][j] = r.x; o->d[P_RY][j] = r.y; o->d[P_RZ][j] = r.z;
     o->d[P_UT][j] = k.t; o->d[P_UX][j] = k.x; o->d[P_UY][j] = k.y; o->d[P_UZ][j] = k.z;
     o->d[P_SI][j] = pol.i; o->d[P_SQ][j] = pol.q; o->d[P_SU][j] = pol.u; o->d[P_SV][j] = pol.v;
     o->d[P_WEIGHT][j] = weight;
     o->d[P_ICOUNT][j] = icount;
-    o->d[P_WORK][j] = a.in.d[P_WORK][i];
+    o->d[P_WORK][j] = src.d[P_WORK][i];
+}
+__device__ __forceinline__ void finalize_photon(const TransportArgs& a, uint32_t i, d4 r, d4 k, sv4 pol, double weight,
+                                                double icount, uint32_t meta, bool decayed) {
+    finalize_photon_from(a, a.in, i, r, k, pol, weight, icount, meta, decayed);
 }
 
 // Writes one new particle into the secondary queue at slot j.

@@ -1,5 +1,6 @@
 // Arguments and launchers of the LMA (PlaneWave) kernels.
 #pragma once
+#include "dev_bw.h"
 #include "dev_lma.h"
 #include "kern_common.h"
 
@@ -35,13 +36,22 @@ struct LmaArgs {
     int photons_to_out;          // photons are final state (photon tracking off)
     double* scratch;             // J(n, x, y) workspace, 5 * scratch_len doubles per lane
     int scratch_len;
+    // LP Breit-Wheeler (photon phase)
+    const double* bw;            // BW pack (dev_bw.h)
+    BwScalars bsc;
+    PBuf pairs;                  // electrons and positrons created by photons
+    unsigned long long* pairs_ctr;
+    unsigned long long pairs_cap;
 };
 
 void launch_step_lma(const TransportArgs* a, const LmaArgs& m, int grid, hipStream_t s);
 void launch_event_lma(const TransportArgs* a, const LmaArgs& m, int lanes, hipStream_t s);
+void launch_photon_step_lma(const TransportArgs* a, const LmaArgs& m, int grid, hipStream_t s);
+void launch_pair_event_lma(const TransportArgs* a, const LmaArgs& m, int lanes, hipStream_t s);
 void launch_push_lma(const LaserDev& L, double dt, int ll, const uint8_t* sp_, double* r4, double* u4, uint32_t n,
                      uint32_t n_steps, uint32_t* taken, double* work, hipStream_t s);
 void launch_eval_lma(int fn, const double* in, double* out, uint32_t n, uint32_t in_w, uint32_t out_w, const double* tables,
-                     const LmaScalars& sc, double* scratch, int scratch_len, uint64_t key, hipStream_t s);
+                     const LmaScalars& sc, const double* bw, const BwScalars& bsc, double* scratch, int scratch_len, uint64_t key,
+                     hipStream_t s);
 
 }  // namespace ptg

@@ -222,6 +222,229 @@ __global__ __launch_bounds__(64) void k_event_lma(const TransportArgs* __restric
     if (flags) atomicOr(A->flags, flags);
 }
 
+// ================================================================ photons: LP Breit-Wheeler
+// Photon loop of collide() in the plane wave (src/main.rs:212-264 with
+// plane_wave.rs:284-316 and pair_creation/mod.rs:21-41): straight-line steps,
+// the decay probability and Stokes evolution of the photon, and the survival
+// test. A photon that decays is suspended to the event queue with its state at
+// the start of the step's update (position after the move, Stokes parameters
+// before it); the pair-event kernel repeats the step's probability from that
+// record and completes the step.
+__device__ __forceinline__ double lma_photon_prob(const LmaArgs& M, const LaserDev& L, double dt, d4 r, d4 k, sv4 pin,
+                                                  double& a, double& k0, double& eta, sv4& pnew, bool& in_table) {
+    const double kc = C_LIGHT * T_COMPTON;
+    double phase = L.k0 * (r.t - r.z);
+    a = sqrt(pw_a_sqd(L, phase));
+    k0 = kc * L.k0 * (1.0 + 2.0 * L.chirp_b * phase);   // kappa = k0 (1, 0, 0, 1)
+    eta = k0 * (k.t - k.z);
+    double dphi = eta * dt / (T_COMPTON * k.t);
+    sv4 s = stokes_rotate(stokes_to_lma(pin, k), M.pol_angle);
+    double prob = bw_probability(M.bw, M.bsc, a * 1.4142135623730951, eta, s, dphi, in_table);
+    pnew = stokes_from_lma(stokes_rotate(s, -M.pol_angle), k);
+    return prob;
+}
+
+__global__ __launch_bounds__(256) void k_photon_step_lma(const TransportArgs* __restrict__ A, LmaArgs M) {
+    const LaserDev& L = A->laser;
+    const double dt = A->dt, cdt = C_LIGHT * dt;
+    uint64_t n_steps = 0;
+    uint32_t flags = 0, psteps = 0;
+    uint32_t idx = 0, meta = 0, nev = 0;
+    d4 r, k;
+    sv4 pol;
+    double icount = 0.0, weight = 1.0, S = 1.0, U = 0.0;
+    Rng rng{};
+    bool skip = false;
+
+    auto fetch = [&]() -> bool {
+        for (;;) {
+            uint32_t i = atomicAdd(A->work_ctr, 1u);
+            if (i >= A->n_in) return false;
+            const PBuf& in = A->in;
+            meta = in.meta[i];
+            if ((meta & 3u) != 2u) { forward_record(*A, i); continue; }
+            idx = i;
+            r = {in.d[P_RT][i], in.d[P_RX][i], in.d[P_RY][i], in.d[P_RZ][i]};
+            k = {in.d[P_UT][i], in.d[P_UX][i], in.d[P_UY][i], in.d[P_UZ][i]};
+            pol = {in.d[P_SI][i], in.d[P_SQ][i], in.d[P_SU][i], in.d[P_SV][i]};
+            icount = in.d[P_ICOUNT][i];
+            weight = in.d[P_WEIGHT][i];
+            nev = in.nev[i];
+            rng = rng_make(A->key, in.stream[i]);
+            rng.c1 = nev;
+            U = rng_uniform(rng);
+            S = 1.0;
+            skip = !A->opt.tracking_photons || (!A->opt.tracking_secondaries && !(meta & META_PRIMARY));
+            psteps = 0;
+            return true;
+        }
+    };
+    bool live = fetch();
+    while (live) {
+        bool inside = !skip && field_contains(L, r) && r.t / C_LIGHT < A->opt.t_stop;
+        if (inside && psteps >= A->step_cap) { flags |= PTG_FLAG_STEP_CAP; inside = false; }
+        if (!inside) {
+            finalize_photon(*A, idx, r, k, pol, weight, icount, meta, false);
+            live = fetch();
+            continue;
+        }
+        ++n_steps;
+        ++psteps;
+        double ik = cdt / k.t;
+        r = {r.t + cdt, r.x + ik * k.x, r.y + ik * k.y, r.z + ik * k.z};
+        sv4 pin = A->opt.pol_resolved ? pol : sv4{1.0, 0.0, 0.0, 0.0};
+        double a, k0, eta;
+        sv4 pnew;
+        bool in_table;
+        double prob = lma_photon_prob(M, L, dt, r, k, pin, a, k0, eta, pnew, in_table);
+        if (!in_table) flags |= PTG_FLAG_TABLE_RANGE;
+        double ri = A->opt.rate_increase;
+        if (prob * ri > 0.1) ri = 0.1 / prob;
+        S *= (1.0 - prob * ri);
+        if (S < U) {
+            unsigned long long j = atomicAdd(M.ev_ctr, 1ull);
+            if (j < M.ev_cap) {
+                const PBuf& q = M.ev;
+                copy_record(q, j, A->in, idx);
+                q.d[P_RT][j] = r.t; q.d[P_RX][j] = r.x; q.d[P_RY][j] = r.y; q.d[P_RZ][j] = r.z;
+                q.d[P_UT][j] = k.t; q.d[P_UX][j] = k.x; q.d[P_UY][j] = k.y; q.d[P_UZ][j] = k.z;
+                q.d[P_SI][j] = pol.i; q.d[P_SQ][j] = pol.q; q.d[P_SU][j] = pol.u; q.d[P_SV][j] = pol.v;
+                q.d[P_WORK][j] = A->in.d[P_WORK][idx];
+                q.d[P_ICOUNT][j] = icount;
+                q.d[P_WEIGHT][j] = weight;
+                q.nev[j] = nev;
+                M.ev_k0[j] = k0;
+                M.ev_a[j] = a;
+            }
+            live = fetch();
+            continue;
+        }
+        if (A->opt.rotate_stokes_pars && A->opt.pol_resolved) pol = pnew;
+        icount += prob;
+    }
+    wave_add(A->stats + STAT_PHOTON_STEPS, n_steps);
+    if (flags) atomicOr(A->flags, flags);
+}
+
+// One pair per lane (pair_creation/mod.rs:47-107 with lp/mod.rs:424-442 and
+// plane_wave.rs:305-316): harmonic from the CDF tables, (s, theta) by rejection
+// on the double-differential rate, the positron built in the zero-momentum
+// frame; the electron and positron go to the pair queue, the photon is kept
+// (decayed) or re-queued with its reduced weight.
+__global__ __launch_bounds__(64) void k_pair_event_lma(const TransportArgs* __restrict__ A, LmaArgs M) {
+    const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n_lanes = gridDim.x * blockDim.x;
+    double* ws = M.scratch + lane;
+    const LaserDev& L = A->laser;
+    uint64_t n_pairs = 0, trials = 0;
+    uint32_t flags = 0;
+    for (uint64_t e = lane; e < M.n_ev; e += n_lanes) {
+        const PBuf& q = M.ev_in;
+        uint32_t i = M.ev_list ? M.ev_list[e] : uint32_t(e);
+        d4 r{q.d[P_RT][i], q.d[P_RX][i], q.d[P_RY][i], q.d[P_RZ][i]};
+        d4 k{q.d[P_UT][i], q.d[P_UX][i], q.d[P_UY][i], q.d[P_UZ][i]};
+        sv4 pol{q.d[P_SI][i], q.d[P_SQ][i], q.d[P_SU][i], q.d[P_SV][i]};
+        double icount = q.d[P_ICOUNT][i], weight = q.d[P_WEIGHT][i];
+        uint32_t nev = q.nev[i], meta = q.meta[i];
+        sv4 pin = A->opt.pol_resolved ? pol : sv4{1.0, 0.0, 0.0, 0.0};
+        double a, k0, eta;
+        sv4 pnew;
+        bool in_table;
+        double prob = lma_photon_prob(M, L, A->dt, r, k, pin, a, k0, eta, pnew, in_table);
+        double ri = A->opt.rate_increase;
+        if (prob * ri > 0.1) ri = 0.1 / prob;
+        double frac = 1.0 / ri;
+        Rng rng = rng_make(A->key, q.stream[i]);
+        rng.c1 = nev | EVENT_BLOCK;
+        double alp = a * 1.4142135623730951;
+        sv4 sl = stokes_rotate(stokes_to_lma(pin, k), M.pol_angle);
+        int n = bw_harmonic(M.bw, M.bsc, alp, eta, sl.q, rng_uniform(rng));
+        bool made = false;
+        if (n < 1) {
+            flags |= PTG_FLAG_TABLE_RANGE;
+        } else if (dbessel_len(n) > M.scratch_len) {
+            if (M.ovf_list) {
+                M.ovf_list[atomicAdd(M.ovf_ctr, 1ull)] = i;
+                atomicMax(M.ovf_len, (unsigned long long)dbessel_len(n));
+                continue;
+            }
+            flags |= PTG_FLAG_TABLE_RANGE;
+        } else {
+            DBessel D = dbessel_at(n, ws, n_lanes);
+            double s, cphi;
+            uint32_t tr = 0;
+            bool capped = false;
+            bw_sample_at(D, alp, eta, sl.q, sl.u, rng, s, cphi, tr, capped);
+            trials += tr;
+            if (capped) flags |= PTG_FLAG_REJECT_CAP;
+            // positron in the zero-momentum frame (pair_creation/mod.rs:63-104)
+            double jn = double(n);
+            double e_zmf = sqrt(0.5 * jn * eta);
+            double p_zmf = sqrt(0.5 * jn * eta - 1.0 - a * a);
+            double cos_theta_zmf = (1.0 - 2.0 * s) * e_zmf / p_zmf;
+            d4 lk{k.t + jn * k0, k.x, k.y, k.z + jn * k0};
+            double inv = 1.0 / sqrt(mdot(lk, lk));
+            d4 u_zmf{lk.t * inv, lk.x * inv, lk.y * inv, lk.z * inv};
+            d3 along = -unit(sp(boost4(d4{jn * k0, 0.0, 0.0, jn * k0}, u_zmf)));
+            d3 eps = unit(sp(boost4(d4{0.0, M.cos_pa, M.sin_pa, 0.0}, u_zmf)));
+            eps = unit(cross(along, cross(eps, along)));
+            double sp_, cp_;
+            sincos(cphi, &sp_, &cp_);
+            d3 perp = rotate(eps, along, sp_, cp_);
+            d3 q3 = p_zmf * (cos_theta_zmf * along + sqrt(1.0 - cos_theta_zmf * cos_theta_zmf) * perp);
+            d4 qp{sqrt(1.0 + a * a + nsq(q3)), q3.x, q3.y, q3.z};
+            qp = boost4(qp, d4{u_zmf.t, -u_zmf.x, -u_zmf.y, -u_zmf.z});
+            double s_new = 1.0 - (k0 * (qp.t - qp.z)) / eta;
+            if (!(fabs(s - s_new) / s < 1.0e-3)) flags |= PTG_FLAG_SANITY;
+            d4 qe{k.t + jn * k0 - qp.t, k.x - qp.x, k.y - qp.y, k.z + jn * k0 - qp.z};
+            // electron and positron (src/main.rs:220-243): weight frac * w, half the absorbed energy each
+            unsigned long long j = atomicAdd(M.pairs_ctr, 2ull);
+            uint64_t id = A->id_base + atomicAdd(A->id_ctr, 2ull);
+            if (j + 1 < M.pairs_cap) {
+                sv4 unpol{1.0, 0.0, 0.0, 0.0};
+                d4 u0{1.0, 0.0, 0.0, 0.0};
+                uint64_t base = rng_child(q.stream[i], nev);
+                write_new(M.pairs, j, 0u, r, qe, u0, unpol, a, a * eta, frac * weight, id, q.id[i], rng_child(base, 0), q.shower[i]);
+                write_new(M.pairs, j + 1, 1u, r, qp, u0, unpol, a, a * eta, frac * weight, id + 1, q.id[i], rng_child(base, 1),
+                          q.shower[i]);
+                M.pairs.d[P_WORK][j] = 0.5 * jn * k0;
+                M.pairs.d[P_WORK][j + 1] = 0.5 * jn * k0;
+            }
+            ++n_pairs;
+            made = true;
+        }
+        // complete the photon's step (src/main.rs:244-257)
+        bool decayed = made && frac == 1.0;
+        if (made && !decayed) weight = weight * (1.0 - frac);
+        if (A->opt.rotate_stokes_pars && A->opt.pol_resolved) pol = pnew;
+        icount += prob;
+        if (decayed) {
+            finalize_photon_from(*A, q, i, r, k, pol, weight, icount, meta, true);
+        } else {
+            unsigned long long j = atomicAdd(M.next_ctr, 1ull);
+            const PBuf& o = M.next;
+            copy_record(o, j, q, i);
+            o.d[P_RT][j] = r.t; o.d[P_RX][j] = r.x; o.d[P_RY][j] = r.y; o.d[P_RZ][j] = r.z;
+            o.d[P_UT][j] = k.t; o.d[P_UX][j] = k.x; o.d[P_UY][j] = k.y; o.d[P_UZ][j] = k.z;
+            o.d[P_SI][j] = pol.i; o.d[P_SQ][j] = pol.q; o.d[P_SU][j] = pol.u; o.d[P_SV][j] = pol.v;
+            o.d[P_WORK][j] = q.d[P_WORK][i];
+            o.d[P_ICOUNT][j] = icount;
+            o.d[P_WEIGHT][j] = weight;
+            o.nev[j] = nev + 1;
+        }
+    }
+    wave_add(A->stats + STAT_PAIRS, n_pairs);
+    wave_add(A->stats + STAT_TRIALS, trials);
+    if (flags) atomicOr(A->flags, flags);
+}
+
+void launch_photon_step_lma(const TransportArgs* a, const LmaArgs& m, int grid, hipStream_t s) {
+    k_photon_step_lma<<<grid, 256, 0, s>>>(a, m);
+}
+void launch_pair_event_lma(const TransportArgs* a, const LmaArgs& m, int lanes, hipStream_t s) {
+    k_pair_event_lma<<<lanes / 64, 64, 0, s>>>(a, m);
+}
+
 void launch_step_lma(const TransportArgs* a, const LmaArgs& m, int grid, hipStream_t s) {
     k_step_lma<<<grid, 256, 0, s>>>(a, m);
 }
@@ -252,7 +475,8 @@ void launch_push_lma(const LaserDev& L, double dt, int ll, const uint8_t* sp_, d
 
 // Probes of the LMA device functions (parity tests).
 __global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, uint32_t in_w, uint32_t out_w,
-                           const double* tables, LmaScalars sc, double* scratch, int scratch_len, uint64_t key) {
+                           const double* tables, LmaScalars sc, const double* bw, BwScalars bsc, double* scratch,
+                           int scratch_len, uint64_t key) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double* x = in + uint64_t(i) * in_w;
@@ -290,12 +514,38 @@ __global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, ui
         y[0] = nn; y[1] = s; y[2] = th; y[3] = pol.q; y[4] = pol.u; y[5] = tr;
         break;
     }
+    case PTG_FN_BW_RATE: {
+        double f[2] = {0.0, 0.0};
+        double a = x[0], eta = x[1];
+        if (bw_is_negligible(a, eta)) y[2] = 0.0;
+        else if (bw_mid_contains(bsc, a, eta)) { bw_mid_interpolate(bw + bw_pack::MID_TOTAL, bsc, a, eta, f); y[2] = 1.0; }
+        else if (bw_hi_contains(bsc, a, eta)) y[2] = bw_hi_interpolate(bw + bw_pack::HI_TOTAL, bsc, a, eta, f) ? 2.0 : -1.0;
+        else y[2] = -1.0;
+        y[0] = f[0]; y[1] = f[1];
+        break;
+    }
+    case PTG_FN_BW_HARMONIC: y[0] = bw_harmonic(bw, bsc, x[0], x[1], x[2], x[3]); break;
+    case PTG_FN_BW_SAMPLE: {
+        // trials = -1: harmonic too large for the workspace (the host re-runs the row); -2: outside the tables
+        Rng rng = rng_make(key, uint64_t(x[4]));
+        int nh = x[5] > 0.0 ? int(x[5]) : bw_harmonic(bw, bsc, x[0], x[1], x[2], rng_uniform(rng));
+        if (nh < 1) { y[0] = 0.0; y[3] = -2.0; break; }
+        if (dbessel_len(nh) > scratch_len) { y[0] = nh; y[3] = -1.0; break; }
+        DBessel D = dbessel_at(nh, scratch + i, n);
+        double s, th;
+        uint32_t tr = 0;
+        bool capped = false;
+        bw_sample_at(D, x[0], x[1], x[2], x[3], rng, s, th, tr, capped);
+        y[0] = nh; y[1] = s; y[2] = th; y[3] = tr;
+        break;
+    }
     default: break;
     }
 }
 void launch_eval_lma(int fn, const double* in, double* out, uint32_t n, uint32_t in_w, uint32_t out_w, const double* tables,
-                     const LmaScalars& sc, double* scratch, int scratch_len, uint64_t key, hipStream_t s) {
-    if (n) k_eval_lma<<<(n + 63) / 64, 64, 0, s>>>(fn, in, out, n, in_w, out_w, tables, sc, scratch, scratch_len, key);
+                     const LmaScalars& sc, const double* bw, const BwScalars& bsc, double* scratch, int scratch_len, uint64_t key,
+                     hipStream_t s) {
+    if (n) k_eval_lma<<<(n + 63) / 64, 64, 0, s>>>(fn, in, out, n, in_w, out_w, tables, sc, bw, bsc, scratch, scratch_len, key);
 }
 
 }  // namespace ptg

@@ -296,3 +296,6 @@ class Fn(enum.IntEnum):
     LP_HARMONIC = 9
     DOUBLE_BESSEL = 10
     LP_SAMPLE = 11
+    BW_RATE = 12
+    BW_HARMONIC = 13
+    BW_SAMPLE = 14

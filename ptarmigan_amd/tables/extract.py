@@ -129,6 +129,33 @@ def extract_file(ref_root, rel, names):
     return out
 
 
+# Tables the reference generates with code of its own but does not ship; made by
+# the generator beside this file and kept under generated/ (name, file, shape, source).
+GENERATED = [
+    ("pair_creation/lp/tables/mid_range/cdf:TABLE", "nbw_lp_mid_cdf.f64", [5652, 16, 3],
+     "regenerated by tables/gen_bw_tables.py (reference generator: src/pair_creation/lp/mod.rs:1065-1163)"),
+]
+
+
+def generated_entries(outdir):
+    out = []
+    for key, fname, shape, src in GENERATED:
+        path = os.path.join(outdir, "generated", fname)
+        if not os.path.exists(path):
+            print(f"note: {path} not present; {key} left out", file=sys.stderr)
+            continue
+        with open(path, "rb") as f:
+            raw = f.read()
+        n = len(raw) // 8
+        want = 1
+        for d in shape:
+            want *= d
+        if n != want:
+            raise ValueError(f"{path}: {n} values, expected {want}")
+        out.append((key, src, shape, list(struct.unpack(f"<{n}d", raw))))
+    return out
+
+
 def main():
     ref_root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     outdir = sys.argv[2] if len(sys.argv) > 2 else os.path.dirname(os.path.abspath(__file__))
@@ -140,6 +167,7 @@ def main():
             if missing:
                 raise ValueError(f"{rel}: missing {sorted(missing)}")
         entries.extend(got)
+    entries.extend(generated_entries(outdir))
 
     header = bytearray(b"PTGTBL01")
     header += struct.pack("<II", len(entries), 0)

@@ -31,3 +31,10 @@ def golden():
     import numpy as np
     here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
     return lambda name: np.load(os.path.join(here, name + ".npz"))
+
+
+@pytest.fixture(scope="session")
+def golden_tables():
+    """The packed table blob the library and the oracle load (ptarmigan_amd/tables)."""
+    from ptarmigan_amd import tables
+    return tables.load()

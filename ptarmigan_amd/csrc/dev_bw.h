@@ -258,8 +258,10 @@ __device__ inline void bw_mid_interpolate(const double* T, const BwScalars& c, d
         f[k] = exp(v);
     }
 }
-// bilinear CDF cell in (ln a, ln eta') and whether eta' sits next to a harmonic boundary
-__device__ inline void bw_mid_cell(const BwScalars& c, double a, double eta, int idx[4], double w[4], bool& bdy) {
+// bilinear CDF cell in (ln a, ln eta') and whether eta' sits next to a harmonic
+// boundary; with `shift` the cell moves one row down there (invert does, the
+// lowest-harmonic estimate does not: mid_range/mod.rs:109-131 against :164-179)
+__device__ inline void bw_mid_cell(const BwScalars& c, double a, double eta, bool shift, int idx[4], double w[4], bool& bdy) {
     using namespace bw_pack;
     const double D = c.mid_eta_prime_density;
     double fa = (log(a) - c.mid_ln_min_a) / c.mid_ln_a_step;
@@ -269,7 +271,7 @@ __device__ inline void bw_mid_cell(const BwScalars& c, double a, double eta, int
     double ep = eta / (1.0 + 0.5 * a * a);
     int ie = int(D * (exp(c.mid_ln_max_eta_prime) / ep - 1.0));
     bdy = ie % int(D) == int(D) - 1;
-    if (bdy) ie = ie - 1;
+    if (bdy && shift) ie = ie - 1;
     ie = ie < 0 ? 0 : (ie > MID_ROWS - 2 ? MID_ROWS - 2 : ie);
     double lower = exp(c.mid_ln_max_eta_prime) / (1.0 + double(ie + 1) / D);
     double upper = exp(c.mid_ln_max_eta_prime) / (1.0 + double(ie) / D);
@@ -284,7 +286,7 @@ __device__ inline int bw_mid_invert(const double* T, const double* C, const BwSc
     int idx[4];
     double w[4];
     bool bdy;
-    bw_mid_cell(c, a, eta, idx, w, bdy);
+    bw_mid_cell(c, a, eta, true, idx, w, bdy);
     double tot[2];
     bw_mid_interpolate(T, c, a, eta, tot);
     double total = bw_interp(tot[0], tot[1], sv1);
@@ -292,13 +294,17 @@ __device__ inline int bw_mid_invert(const double* T, const double* C, const BwSc
     bw_sum_limits(a, eta, n_lo, n_hi);
     double n = 0.0;
     if (bdy) {
-        double acc = 0.0;  // rate of the lowest harmonic (mid_range/mod.rs:109-162)
+        int idl[4];  // rate of the lowest harmonic (mid_range/mod.rs:109-162)
+        double wl[4];
+        bool unused;
+        bw_mid_cell(c, a, eta, false, idl, wl, unused);
+        double acc = 0.0;
         for (int q = 0; q < 4; ++q) {
-            const double* t = C + 48 * idx[q];
+            const double* t = C + 48 * idl[q];
             double fr = 0.0;
             if (!(t[0] > double(n_lo))) fr = bw_interp(t[1], t[2], sv1) / bw_interp(t[46], t[47], sv1);
             double v = fr * total;
-            acc += w[q] * v * v * v;
+            acc += wl[q] * v * v * v;
         }
         double p_lowest = cbrt(acc) / total;
         if (frac <= p_lowest) return n_lo;

@@ -296,8 +296,8 @@ __global__ __launch_bounds__(256) void k_photon_step_lma(const TransportArgs* __
         double a, k0, eta;
         sv4 pnew;
         bool in_table;
+        // outside both rate tables the probability is 0 without a diagnostic (lp/mod.rs:388-397)
         double prob = lma_photon_prob(M, L, dt, r, k, pin, a, k0, eta, pnew, in_table);
-        if (!in_table) flags |= PTG_FLAG_TABLE_RANGE;
         double ri = A->opt.rate_increase;
         if (prob * ri > 0.1) ri = 0.1 / prob;
         S *= (1.0 - prob * ri);

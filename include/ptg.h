@@ -245,7 +245,17 @@ typedef enum {
                                     pair_creation/lp/mod.rs:385-401 */
     PTG_FN_BW_HARMONIC = 13,     /* (a, eta, S1, frac) -> n  pair_creation/lp/mod.rs:424-436 */
     PTG_FN_BW_SAMPLE = 14,       /* (a, eta, S1, S2, stream, fixed_n) -> (n, s, theta, trials)  lp/mod.rs:261-301 */
-    PTG_FN_COUNT = 15
+    PTG_FN_BESSEL_J = 15,        /* (n, x) -> (J_{n-1}, J_n, J_{n+1})(x)  special_functions/bessel.rs:26-33 */
+    PTG_FN_CP_RATE = 16,         /* (a, eta, classical) -> (f, in_table)  nonlinear_compton/cp/mod.rs:162-176, cp/classical.rs:22-44 */
+    PTG_FN_CP_HARMONIC = 17,     /* (a, eta, frac, classical) -> n  cp/tables/mod.rs:46-121, cp/classical.rs:83-123 */
+    PTG_FN_CP_PARTIAL_RATE = 18, /* (n, a, eta, classical) -> (partial rate, rejection ceiling)  cp/mod.rs:42-155,
+                                    cp/classical.rs:46-69 (classical: rate per unit eta) */
+    PTG_FN_CP_SAMPLE = 19,       /* (a, eta, stream, fixed_n, classical) -> (n, s, theta, S1, S2, S3, trials)  cp/mod.rs:181-234 */
+    PTG_FN_LP_CL_RATE = 20,      /* (a, eta) -> (f, in_table)  nonlinear_compton/lp/classical.rs:189-213 */
+    PTG_FN_LP_CL_HARMONIC = 21,  /* (a, frac) -> n  lp/classical.rs:215-252 */
+    PTG_FN_LP_CL_CEILING = 22,   /* (n, a) -> rejection ceiling  lp/classical.rs:43-110 */
+    PTG_FN_LP_CL_SAMPLE = 23,    /* (a, eta, stream, fixed_n) -> (n, s, theta, S1, S2, trials)  lp/classical.rs:254-333 */
+    PTG_FN_COUNT = 24
 } ptg_eval_fn;
 int ptg_eval(ptg_ctx* ctx, int fn, const double* in, size_t n, size_t in_w, double* out, size_t out_w);
 

@@ -368,6 +368,31 @@ struct ThetaBound {
         }
         return tb_;
     }
+    // lp/mod.rs:211-284: the same construction at fixed v = s / s_n in the classical limit
+    static ThetaBound for_harmonic_low_eta(int n, double a) {
+        ThetaBound tb_;
+        for (int i = 0; i < 16; ++i) {
+            double z = std::cos(PI * double(i) / 15.0);
+            double v = 0.5 * (z + 1.0);
+            tb_.s[i] = v;
+            double x = 2.0 * double(n) * a * std::sqrt(v * (1.0 - v) / (1.0 + 0.5 * a * a));
+            double y = 0.25 * double(n) * a * a * v / (1.0 + 0.5 * a * a);
+            double x_crit = y > double(n) / 6.0 ? 4.0 * std::sqrt(y) * std::sqrt(double(n) - 2.0 * y) : double(n) + 2.0 * y;
+            double ln_j_crit = std::log(0.443) - std::log(double(n)) / 3.0;
+            double ln_j_bdy = n % 2 == 0 ? ln_double_bessel_x_zero(n, y) : ln_double_bessel_x_zero(n + 1, y);
+            ln_j_bdy = std::fmin(ln_j_bdy, ln_j_crit);
+            double cos_theta = 1.0 - x_crit * std::log(0.01) / (x * (ln_j_bdy - ln_j_crit));
+            cos_theta = std::fmax(cos_theta, -1.0);
+            tb_.f[i] = std::acos(cos_theta);
+            if (std::isnan(tb_.f[i])) tb_.f[i] = PI;
+        }
+        double f_min = tb_.f[0];
+        for (int i = 1; i < 16; ++i) {
+            if (tb_.f[i] < f_min) f_min = tb_.f[i];
+            else tb_.f[i] = f_min;
+        }
+        return tb_;
+    }
     double at(double arg) const {
         double val = f[15];
         for (int i = 1; i < 16; ++i) {
@@ -468,33 +493,211 @@ static void lp_sample(double a, double eta, Xoshiro& rng, int fixed_n, int* n_ou
     *pol_out = {1.0, xi1 / xi0, xi2 / xi0, 0.0};
 }
 
-// nonlinear_compton/mod.rs:19-47 (linear polarisation, quantum)
-static bool nlc_probability(V4 k, V4 q, double dt, Pol pol, double* out) {
+// ---------------------------------------------------------------- LP nonlinear Compton, classical (lp/classical.rs)
+static const char* LCR = "nonlinear_compton/lp/classical/rate_table:";
+static const char* LCC = "nonlinear_compton/lp/classical/cdf_table:";
+
+// lp/classical.rs:17-40: rate differential in v = s / s_n and theta, 0 < theta < pi/2
+static double lp_cl_ddpr(double a, double v, double theta, DoubleBessel& dj) {
+    int n = dj.n;
+    double x = 2.0 * double(n) * a * std::cos(theta) * std::sqrt(v * (1.0 - v) / (1.0 + 0.5 * a * a));
+    double y = 0.25 * double(n) * a * a * v / (1.0 + 0.5 * a * a);
+    double j[5];
+    dj.evaluate(x, y, j);
+    double g0, g1, g2;
+    if (n % 2 == 0) { g0 = j[2]; g1 = -0.5 * (j[1] + j[3]); g2 = 0.25 * (j[0] + 2.0 * j[2] + j[4]); }
+    else { g0 = -j[2]; g1 = 0.5 * (j[1] + j[3]); g2 = -0.25 * (j[0] + 2.0 * j[2] + j[4]); }
+    return (-g0 * g0 - a * a * (g0 * g2 - g1 * g1)) / (2.0 * PI);
+}
+// lp/classical.rs:43-110
+static double lp_cl_ceiling(double a, DoubleBessel& dj) {
+    int n = dj.n;
+    double mx = 0.0;
+    if (n == 1) {
+        mx = lp_cl_ddpr(a, 0.01, FRAC_PI_2, dj);
+    } else if (a <= 1.5) {
+        double v_min = 0.1 + (9.0 / std::sqrt(double(n)) - 1.0) / 10.0;
+        double v_max = std::fmin(v_min + 0.4, 1.0);
+        for (int i = 1; i <= 16; ++i) {
+            double f = lp_cl_ddpr(a, v_min + (v_max - v_min) * double(i) / 16.0, 0.0, dj);
+            mx = i == 1 ? f : std::fmax(mx, f);
+        }
+    } else if (a < 5.0) {
+        int n_switch = int(3.0 * std::pow(a / 2.0, 4.0 / 3.0));
+        double theta = n > n_switch ? 0.0 : FRAC_PI_2 * (1.0 - std::log(double(n)) / std::log(double(n_switch)));
+        double v_min = std::fmin(0.35 + (7.0 / std::sqrt(double(n)) - 1.0) / 9.0, 0.55);
+        double v_max = 0.55 + (7.0 / std::sqrt(double(n)) - 1.0) / 9.0;
+        int num = a > 4.0 ? 32 : 16;
+        for (int i = 0; i < num; ++i) {
+            double f = lp_cl_ddpr(a, v_min + (v_max - v_min) * double(i) / double(num), theta, dj);
+            mx = i == 0 ? f : std::fmax(mx, f);
+        }
+    } else {
+        double n_switch = 10.0 * powi(a / 5.0, 2);
+        double theta = 0.0;
+        if (!(n > int(n_switch))) {
+            double pw = 0.3 + (0.7 - 0.3) * (std::log(a) - std::log(5.0)) / (std::log(20.0) - std::log(5.0));
+            theta = FRAC_PI_2 * (std::pow(double(n), -pw) - std::pow(n_switch, -pw)) / (1.0 - std::pow(n_switch, -pw));
+        }
+        double v_min = 0.3 + 0.4 * std::log(double(n)) * std::pow(double(n), -0.7);
+        double v_max = v_min + 0.2;
+        int num = a > 8.0 ? 64 : 32;
+        for (int i = 0; i < num; ++i) {
+            double f = lp_cl_ddpr(a, v_min + (v_max - v_min) * double(i) / double(num), theta, dj);
+            mx = i == 0 ? f : std::fmax(mx, f);
+        }
+    }
+    return 1.1 * mx;
+}
+// lp/classical.rs:112-181: integral over 0 < v < 1 (three 32-point pieces) and
+// 0 < theta < 2 pi, and the largest integrand value met
+static double lp_cl_partial_rate(int n, double a, double* max_out) {
+    Table N = table("quadrature/tables:GAUSS_32_NODES"), W = table("quadrature/tables:GAUSS_32_WEIGHTS");
+    DoubleBessel dj(n, double(n) * SQRT_2, double(n) * 0.5);
+    ThetaBound tm = ThetaBound::for_harmonic_low_eta(n, a);
+    auto single = [&](double v, double th_max, double* mx) {
+        double sum = 0.0;
+        for (int i = 0; i < 32; ++i) {
+            double r = lp_cl_ddpr(a, v, 0.5 * (N[i] + 1.0) * th_max, dj);
+            sum += 4.0 * (0.5 * th_max) * W[i] * r;
+            *mx = std::fmax(*mx, r);
+        }
+        return sum;
+    };
+    double mxl = 0.0, mxm = 0.0, mxu = 0.0, lo = 0.0, mid = 0.0, up = 0.0;
+    for (int i = 0; i < 32; ++i) {
+        double v = 0.5 * (N[i] + 1.0) * 0.5;
+        double m = 0.0;
+        lo += W[i] * (0.5 * 0.5) * single(v, tm.at(v), &m);
+        mxl = std::fmax(mxl, m);
+    }
+    for (int i = 0; i < 32; ++i) {
+        double v = 0.5 + 0.5 * (N[i] + 1.0) * 0.25;
+        double m = 0.0;
+        mid += W[i] * (0.5 * 0.25) * single(v, tm.at(v), &m);
+        mxm = std::fmax(mxm, m);
+    }
+    for (int i = 0; i < 32; ++i) {
+        double v = 0.75 + 0.5 * (N[i] + 1.0) * 0.25;
+        double m = 0.0;
+        up += W[i] * (0.5 * 0.25) * single(v, tm.at(v), &m);
+        mxu = std::fmax(mxu, m);
+    }
+    if (max_out) *max_out = std::fmax(std::fmax(mxu, mxm), mxl);
+    return lo + mid + up;
+}
+// lp/classical.rs:189-213
+static bool lp_cl_rate(double a, double eta, double* out) {
+    double x = std::log(a);
+    double MIN = tb(LCR, "MIN")[0], STEP = tb(LCR, "STEP")[0];
+    Table T = tb(LCR, "TABLE");
+    if (x < MIN) { *out = a * a * eta / 3.0; return true; }
+    size_t ix = size_t((x - MIN) / STEP);
+    if (!(ix < T.n - 1)) return false;
+    double dx = (x - MIN) / STEP - double(ix);
+    double f = (1.0 - dx) * T[ix] + dx * T[ix + 1];
+    double prefactor = 2.0 * eta / (1.0 + 0.5 * a * a);
+    *out = prefactor * std::exp(f);
+    return true;
+}
+// lp/classical.rs:215-252
+static int lp_cl_harmonic(double a, double frac) {
+    double MIN = tb(LCC, "MIN")[0], STEP = tb(LCC, "STEP")[0];
+    Table T = tb(LCC, "TABLE");
+    if (std::log(a) <= MIN) return 1;
+    size_t ix = size_t((std::log(a) - MIN) / STEP);
+    double dx = (std::log(a) - MIN) / STEP - double(ix);
+    if (ix + 1 >= 61) { std::fprintf(stderr, "oracle: LP classical CDF index\n"); std::abort(); }
+    double n = 0.0;
+    for (int k = 0; k < 2; ++k) {
+        const double* t = T.p + 32 * (ix + k);
+        double nk;
+        if (frac <= t[1]) nk = 0.9;
+        else if (!Interp{t, 16}.invert(frac, &nk)) { std::fprintf(stderr, "oracle: LP classical invert failed\n"); std::abort(); }
+        n += nk * (k == 0 ? 1.0 - dx : dx);
+    }
+    return int(std::ceil(n));
+}
+// lp/classical.rs:254-333
+static void lp_cl_sample(double a, double eta, Xoshiro& rng, int fixed_n, int* n_out, double* s_out, double* theta_out, SV* pol_out,
+                         uint64_t* trials) {
+    int n = fixed_n > 0 ? fixed_n : lp_cl_harmonic(a, rng.gen_f64());
+    DoubleBessel dj(n, double(n) * SQRT_2, double(n) * 0.5);
+    ThetaBound tm = ThetaBound::for_harmonic_low_eta(n, a);
+    double smax = 2.0 * double(n) * eta / (1.0 + 0.5 * a * a);
+    double mx = lp_cl_ceiling(a, dj);
+    double v, s, theta;
+    for (;;) {
+        v = rng.gen_f64();
+        s = v * smax;
+        theta = FRAC_PI_2 * rng.gen_f64();
+        if (theta > tm.at(v)) continue;
+        double z = mx * rng.gen_f64();
+        if (trials) ++*trials;
+        if (z < lp_cl_ddpr(a, v, theta, dj)) break;
+    }
+    uint32_t r32 = uint32_t(rng.next_u64() >> 32);
+    int quadrant = int((uint64_t(r32) * 4u) >> 32);
+    switch (quadrant) {
+    case 1: theta = PI - theta; break;
+    case 2: theta = PI + theta; break;
+    case 3: theta = 2.0 * PI - theta; break;
+    default: break;
+    }
+    double x = 2.0 * double(n) * a * std::cos(theta) * std::sqrt(v * (1.0 - v) / (1.0 + 0.5 * a * a));
+    double y = 0.25 * double(n) * a * a * v / (1.0 + 0.5 * a * a);
+    double j[5];
+    dj.evaluate(x, y, j);
+    double A0, A1, A2;
+    if (n % 2 == 0) { A0 = j[2]; A1 = -0.5 * (j[1] + j[3]); A2 = 0.25 * (j[0] + 2.0 * j[2] + j[4]); }
+    else { A0 = -j[2]; A1 = 0.5 * (j[1] + j[3]); A2 = -0.25 * (j[0] + 2.0 * j[2] + j[4]); }
+    double u = std::sqrt((1.0 + 0.5 * a * a) * (1.0 - v) / v);
+    double xi0 = 2.0 * (powi(A1, 2) - A0 * A2) - 2.0 * powi(A0 / a, 2);
+    double xi1 = 2.0 * (powi(A1, 2) - A0 * A2) - 2.0 * (1.0 + 2.0 * u * u * powi(std::sin(theta), 2)) * powi(A0 / a, 2);
+    double xi2 = 2.0 * powi(u * A0 / a, 2) * std::sin(2.0 * theta) + 4.0 * u * A0 * A1 * std::sin(theta) / a;
+    *n_out = n;
+    *s_out = s;
+    *theta_out = theta;
+    *pol_out = {1.0, xi1 / xi0, xi2 / xi0, 0.0};
+}
+
+// nonlinear_compton/mod.rs:19-47
+static bool nlc_probability(V4 k, V4 q, double dt, Pol pol, bool classical, double* out) {
     double a_sqd = q * q - 1.0;
     double a = a_sqd >= 0.0 ? std::sqrt(a_sqd) : 0.0;
     double eta = k * q;
     double dphi = dt * eta / (COMPTON_TIME * q[0]);
-    if (pol != Pol::Linear) { std::fprintf(stderr, "oracle: circular LMA emission not restated\n"); std::abort(); }
     double f;
-    if (!lp_rate(a * SQRT_2, eta, &f)) return false;
+    bool ok;
+    if (pol == Pol::Circular) ok = classical ? cp_classical_rate(a, eta, &f) : cp_rate(a, eta, &f);
+    else ok = classical ? lp_cl_rate(a * SQRT_2, eta, &f) : lp_rate(a * SQRT_2, eta, &f);
+    if (!ok) return false;
     *out = ALPHA_FINE * f * dphi / eta;
     return true;
 }
 
-// nonlinear_compton/mod.rs:58-138 (linear polarisation, quantum)
-static void nlc_generate(V4 k, V4 q, double pol_angle, Xoshiro& rng, int* n_out, V4* k_out, SV* sv_out, uint64_t* trials) {
+// nonlinear_compton/mod.rs:58-138
+static void nlc_generate(V4 k, V4 q, Pol pol, double pol_angle, bool classical, Xoshiro& rng, int* n_out, V4* k_out, SV* sv_out,
+                         uint64_t* trials) {
     double a = std::sqrt(q * q - 1.0);
     double eta = k * q;
     int n;
     double s, cphi;
     SV sv;
-    lp_sample(a * SQRT_2, eta, rng, 0, &n, &s, &cphi, &sv, trials);
+    if (pol == Pol::Circular) {
+        if (classical) cp_classical_sample(a, eta, rng, 0, &n, &s, &cphi, &sv, trials);
+        else cp_sample(a, eta, rng, 0, &n, &s, &cphi, &sv, trials);
+    } else {
+        if (classical) lp_cl_sample(a * SQRT_2, eta, rng, 0, &n, &s, &cphi, &sv, trials);
+        else lp_sample(a * SQRT_2, eta, rng, 0, &n, &s, &cphi, &sv, trials);
+    }
     double ell = double(n);
-    double p_zmf = ell * eta / std::sqrt(1.0 + a * a + 2.0 * ell * eta);
-    double cos_theta_zmf = 1.0 - s * (1.0 + a * a + 2.0 * ell * eta) / (ell * eta);
-    V4 qk = q + ell * k;
-    V4 u_zmf = qk / std::sqrt(qk * qk);
-    V3 along = -normalize(spatial(boost_by(q, u_zmf)));
+    double p_zmf = classical ? ell * eta / std::sqrt(1.0 + a * a) : ell * eta / std::sqrt(1.0 + a * a + 2.0 * ell * eta);
+    double cos_theta_zmf = classical ? 1.0 - s * (1.0 + a * a) / (ell * eta) : 1.0 - s * (1.0 + a * a + 2.0 * ell * eta) / (ell * eta);
+    V4 u_zmf;
+    if (classical) u_zmf = q / std::sqrt(q * q);
+    else { V4 qk = q + ell * k; u_zmf = qk / std::sqrt(qk * qk); }
+    V3 along = classical ? normalize(spatial(boost_by(ell * k, u_zmf))) : -normalize(spatial(boost_by(q, u_zmf)));
     V4 eps4{0.0, std::cos(pol_angle), std::sin(pol_angle), 0.0};
     V3 eps = normalize(spatial(boost_by(eps4, u_zmf)));
     eps = normalize(cross(along, cross(eps, along)));
@@ -615,19 +818,18 @@ PushOut lma_push(const Laser& L, V4 r, V4 u, double rqm, double dt, Eqn eqn) {
 
 // plane_wave.rs:251-282
 bool lma_radiate(const Laser& L, V4 r, V4 u, double dt, Xoshiro& rng, bool classical, RadEvent* ev) {
-    if (classical) { std::fprintf(stderr, "oracle: classical LMA emission not restated\n"); std::abort(); }
     double a = std::sqrt(pw_a_sqd(L, r));
     double phase = L.wavevector * r;
     double chirp = 1.0 + 2.0 * L.chirp_b * phase;
     double width = 1.0 + pw_bandwidth(L) * rng.normal();
     V4 kappa = SPEED_OF_LIGHT * COMPTON_TIME * L.wavevector * chirp * width;
     double prob = 0.0;
-    if (!nlc_probability(kappa, u, dt, L.pol, &prob)) prob = 0.0;
+    if (!nlc_probability(kappa, u, dt, L.pol, classical, &prob)) prob = 0.0;
     if (!(rng.gen_f64() < prob)) return false;
     int n;
     V4 k;
     SV pol;
-    nlc_generate(kappa, u, L.pol_angle, rng, &n, &k, &pol, nullptr);
+    nlc_generate(kappa, u, L.pol, L.pol_angle, classical, rng, &n, &k, &pol, nullptr);
     *ev = RadEvent{k, u + double(n) * kappa - k, pol, a, a * (u * kappa), double(n) * kappa[0]};
     return true;
 }
@@ -1313,6 +1515,47 @@ double orc_lp_ddpr(int n, double a, double eta, double s, double theta) {
     return ddpr(a, eta, s, theta, dj);
 }
 double orc_lp_partial_rate(int n, double a, double eta, double* max_out) { return partial_rate(n, a, eta, max_out); }
+double orc_lp_cl_partial_rate(int n, double a, double* max_out) { return lp_cl_partial_rate(n, a, max_out); }
+double orc_lp_cl_ceiling(int n, double a) {
+    DoubleBessel dj(n, double(n) * SQRT_2, double(n) * 0.5);
+    return lp_cl_ceiling(a, dj);
+}
+double orc_lp_cl_ddpr(int n, double a, double v, double theta) {
+    DoubleBessel dj(n, double(n) * SQRT_2, double(n) * 0.5);
+    return lp_cl_ddpr(a, v, theta, dj);
+}
+double orc_lp_theta_bound_low_eta(int n, double a, double v) { return ThetaBound::for_harmonic_low_eta(n, a).at(v); }
+int orc_lp_cl_rate(double a, double eta, double* out) { return lp_cl_rate(a, eta, out) ? 1 : 0; }
+int orc_lp_cl_harmonic(double a, double frac) { return lp_cl_harmonic(a, frac); }
+// v-marginal of the classical double-differential rate at harmonic n: the midpoint
+// sum over nth theta cells of [0, pi/2] below the theta bound, at nv midpoints in v
+void orc_lp_cl_v_marginal(int n, double a, int nv, int nth, double* out) {
+    DoubleBessel dj(n, double(n) * SQRT_2, double(n) * 0.5);
+    ThetaBound tm = ThetaBound::for_harmonic_low_eta(n, a);
+    for (int i = 0; i < nv; ++i) {
+        double v = (double(i) + 0.5) / double(nv), acc = 0.0, tb_v = tm.at(v);
+        for (int k = 0; k < nth; ++k) {
+            double th = FRAC_PI_2 * (double(k) + 0.5) / double(nth);
+            if (th <= tb_v) acc += lp_cl_ddpr(a, v, th, dj);
+        }
+        out[i] = acc;
+    }
+}
+void orc_lp_cl_harmonic_batch(double a, const double* frac, size_t count, int* out) {
+    for (size_t i = 0; i < count; ++i) out[i] = lp_cl_harmonic(a, frac[i]);
+}
+void orc_lp_cl_sample(double a, double eta, uint64_t seed, int fixed_n, size_t count, double* out5) {
+    Xoshiro rng(seed);
+    for (size_t i = 0; i < count; ++i) {
+        int n;
+        double s, th;
+        SV sv;
+        uint64_t tr = 0;
+        lp_cl_sample(a, eta, rng, fixed_n, &n, &s, &th, &sv, &tr);
+        double* o = out5 + 5 * i;
+        o[0] = n; o[1] = s; o[2] = th; o[3] = sv[1]; o[4] = sv[2];
+    }
+}
 int orc_lp_sum_limit(double a, double eta) { return sum_limit(a, eta); }
 double orc_lp_linear_rate(double a, double eta) { return linear_rate(a, eta); }
 double orc_lp_theta_bound(int n, double a, double eta, double s) { return ThetaBound::for_harmonic(n, a, eta).at(s); }

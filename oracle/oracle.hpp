@@ -211,6 +211,15 @@ double pair_spectrum_ceiling(double chi, double sv1);                           
 double pair_angular_spectrum(double y, double s, double chi, double sv1);                   // :116-127
 double pair_angular_spectrum_ceiling(double s, double chi, double sv1);                     // :131-143
 
+// ---------------------------------------------------------------- Bessel J, CP nonlinear Compton (cp.cpp)
+void bessel_j_pm(int n, double x, double out[3]);  // special_functions/bessel.rs:26-33
+double bessel_j(int n, double x);                  // bessel.rs:18-24
+bool cp_rate(double a, double eta, double* out);   // nonlinear_compton/cp/mod.rs:162-176
+bool cp_classical_rate(double a, double eta, double* out);  // cp/classical.rs:22-44
+void cp_sample(double a, double eta, Xoshiro& rng, int fixed_n, int* n, double* s, double* theta, SV* sv, uint64_t* trials);  // cp/mod.rs:181-234
+void cp_classical_sample(double a, double eta, Xoshiro& rng, int fixed_n, int* n, double* s, double* theta, SV* sv,
+                         uint64_t* trials);  // cp/classical.rs:128-178
+
 // ---------------------------------------------------------------- fields
 enum class Kind : int { PlaneWave = 0, FocusedLaser = 1, FastPlaneWave = 2, FastFocusedLaser = 3 };
 enum class Env : int { CosSquared = 0, Flattop = 1, Gaussian = 2 };

@@ -133,6 +133,42 @@ def lib():
         L.orc_bw_harmonic.restype = C.c_int
         L.orc_bw_harmonic.argtypes = [d, d, d, d]
         L.orc_bw_sample.argtypes = [d, d, d, d, C.c_uint64, C.c_int, C.c_size_t, P(d)]
+        i = C.c_int
+        L.orc_bessel_j_pm.argtypes = [i, d, P(d)]
+        L.orc_bessel_j.argtypes = [i, d]
+        L.orc_bessel_j.restype = d
+        L.orc_bessel_j_parts.argtypes = [i, d, P(d)]
+        for name in ("orc_cp_ddpr",):
+            getattr(L, name).argtypes = [i, d, d, d]
+            getattr(L, name).restype = d
+        for name in ("orc_cp_ceiling", "orc_cp_partial_rate", "orc_cp_spectrum_low_eta"):
+            getattr(L, name).argtypes = [i, d, d]
+            getattr(L, name).restype = d
+        for name in ("orc_cp_integrated_spectrum_low_eta", "orc_cp_classical_ceiling"):
+            getattr(L, name).argtypes = [i, d]
+            getattr(L, name).restype = d
+        L.orc_lp_cl_partial_rate.argtypes = [i, d, P(d)]
+        L.orc_lp_cl_partial_rate.restype = d
+        L.orc_lp_cl_ceiling.argtypes = [i, d]
+        L.orc_lp_cl_ceiling.restype = d
+        L.orc_lp_cl_ddpr.argtypes = [i, d, d, d]
+        L.orc_lp_cl_ddpr.restype = d
+        L.orc_lp_theta_bound_low_eta.argtypes = [i, d, d]
+        L.orc_lp_theta_bound_low_eta.restype = d
+        L.orc_lp_cl_rate.argtypes = [d, d, P(d)]
+        L.orc_lp_cl_harmonic.argtypes = [d, d]
+        L.orc_lp_cl_harmonic_batch.argtypes = [d, P(d), C.c_size_t, P(C.c_int)]
+        L.orc_lp_cl_v_marginal.argtypes = [i, d, i, i, P(d)]
+        L.orc_lp_cl_sample.argtypes = [d, d, C.c_uint64, i, C.c_size_t, P(d)]
+        L.orc_cp_grid_max.argtypes = [i, d, d, i]
+        L.orc_cp_grid_max.restype = d
+        L.orc_cp_rate.argtypes = [d, d, P(d)]
+        L.orc_cp_classical_rate.argtypes = [d, d, P(d)]
+        L.orc_cp_harmonic.argtypes = [d, d, d]
+        L.orc_cp_classical_harmonic.argtypes = [d, d]
+        L.orc_cp_cdf_row.argtypes = [d, d, i, P(d)]
+        L.orc_cp_cdf_row.restype = d
+        L.orc_cp_sample.argtypes = [d, d, i, C.c_uint64, i, C.c_size_t, P(d)]
         rc = L.orc_init(os.path.abspath(TABLES).encode())
         if rc != 0:
             raise RuntimeError(f"oracle: cannot load tables from {TABLES}")
@@ -308,4 +344,49 @@ def bw_by_summation(a, eta):
 def bw_sample(a, eta, sv1, sv2, count, seed=0, fixed_n=0):
     out = np.zeros((count, 3))
     lib().orc_bw_sample(a, eta, sv1, sv2, seed, fixed_n, count, dptr(out))
+    return out
+
+
+# ---- Bessel J and CP nonlinear Compton (oracle/cp.cpp)
+def bessel_j_pm(n, x):
+    out = np.zeros(3)
+    lib().orc_bessel_j_pm(n, x, dptr(out))
+    return out
+
+
+def cp_rate(a, eta, classical=False):
+    out = C.c_double(0.0)
+    f = lib().orc_cp_classical_rate if classical else lib().orc_cp_rate
+    return out.value if f(a, eta, C.byref(out)) else None
+
+
+def cp_cdf_row(a, eta, classical=False):
+    out = np.zeros((32, 2))
+    total = lib().orc_cp_cdf_row(a, eta, int(classical), dptr(out))
+    return total, out
+
+
+def cp_sample(a, eta, count, classical=False, seed=0, fixed_n=0):
+    """(n, s, theta, S1, S2, S3, trials) per sample from one seeded stream."""
+    out = np.zeros((count, 7))
+    lib().orc_cp_sample(a, eta, int(classical), seed, fixed_n, count, dptr(out))
+    return out
+
+
+# ---- LP nonlinear Compton, classical (oracle/lma.cpp)
+def lp_cl_rate(a, eta):
+    out = C.c_double(0.0)
+    return out.value if lib().orc_lp_cl_rate(a, eta, C.byref(out)) else None
+
+
+def lp_cl_partial_rate(n, a):
+    mx = C.c_double(0.0)
+    r = lib().orc_lp_cl_partial_rate(n, a, C.byref(mx))
+    return r, mx.value
+
+
+def lp_cl_sample(a, eta, count, seed=0, fixed_n=0):
+    """(n, s, theta, S1, S2) per sample from one seeded stream."""
+    out = np.zeros((count, 5))
+    lib().orc_lp_cl_sample(a, eta, seed, fixed_n, count, dptr(out))
     return out

@@ -160,7 +160,7 @@ __device__ __forceinline__ uint64_t rng_child(uint64_t stream, uint64_t event) {
 struct Hermite { double x0, x1, f0, f1, m0, m1; };
 
 template <typename P>
-__device__ __forceinline__ Hermite hermite_segment(P t, int i, int n) {
+PTG_HD __forceinline__ Hermite hermite_segment(P t, int i, int n) {
     double xm1 = t[2 * (i - 1)], fm1 = t[2 * (i - 1) + 1];
     double x0 = t[2 * i], f0 = t[2 * i + 1];
     double s1 = (f0 - fm1) / (x0 - xm1);
@@ -173,7 +173,7 @@ __device__ __forceinline__ Hermite hermite_segment(P t, int i, int n) {
     if (s2 != 0.0 && m1 / s2 > 3.0) m1 = 3.0 * s2;
     return {xm1, x0, fm1, f0, m0, m1};
 }
-__device__ __forceinline__ double hermite_eval(const Hermite& h, double x) {
+PTG_HD __forceinline__ double hermite_eval(const Hermite& h, double x) {
     double d = h.x1 - h.x0;
     double t = (x - h.x0) / d;
     double omt = 1.0 - t;
@@ -183,7 +183,7 @@ __device__ __forceinline__ double hermite_eval(const Hermite& h, double x) {
 // Segment index i (1..n-1) such that t[i-1][col] < v <= t[i][col]; returns -1 if
 // v <= t[0][col] and n if v > t[n-1][col]. Binary search on a monotone column.
 template <typename P>
-__device__ __forceinline__ int pwmci_locate(P t, int n, int col, double v) {
+PTG_HD __forceinline__ int pwmci_locate(P t, int n, int col, double v) {
     if (!(v > t[col])) return -1;
     if (v > t[2 * (n - 1) + col]) return n;
     int lo = 0, hi = n - 1;  // t[lo] < v <= t[hi]
@@ -195,7 +195,7 @@ __device__ __forceinline__ int pwmci_locate(P t, int n, int col, double v) {
 }
 // Evaluate; out-of-range -> linear extrapolation if ext, else returns false.
 template <typename P>
-__device__ __forceinline__ bool pwmci_eval(P t, int n, double x, bool ext, double& out) {
+PTG_HD __forceinline__ bool pwmci_eval(P t, int n, double x, bool ext, double& out) {
     int i = pwmci_locate(t, n, 0, x);
     if (i > 0 && i < n) { out = hermite_eval(hermite_segment(t, i, n), x); return true; }
     if (!ext) return false;
@@ -205,7 +205,7 @@ __device__ __forceinline__ bool pwmci_eval(P t, int n, double x, bool ext, doubl
     return true;
 }
 template <typename P>
-__device__ __forceinline__ bool pwmci_invert(P t, int n, double f, bool ext, double& out) {
+PTG_HD __forceinline__ bool pwmci_invert(P t, int n, double f, bool ext, double& out) {
     int i = pwmci_locate(t, n, 1, f);
     if (!(i > 0 && i < n)) {
         if (!ext) return false;

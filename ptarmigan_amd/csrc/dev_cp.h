@@ -1,0 +1,361 @@
+// Circularly polarised nonlinear Compton scattering in the LMA plane wave, and
+// the Bessel functions J_{n-1}, J_n, J_{n+1} it is built on.
+//
+// An electron of rms normalised amplitude a and energy parameter eta emits at
+// harmonic n with a lightfront fraction s whose spectrum is a combination of
+// J_{n-1}^2, J_n^2 and J_{n+1}^2 of one argument z(n, a, s): the harmonic comes
+// from an inverse CDF table, s by 1-D rejection, the azimuth is uniform
+// (src/nonlinear_compton/cp/mod.rs, cp/tables/mod.rs, cp/classical.rs,
+// src/special_functions/bessel.rs).
+//
+// Everything here is __host__ __device__: the host-side table generator
+// (tables/gen_cp_tables.hip) sums the same partial rates the device samples.
+#pragma once
+#include "dev_lcfa.h"
+#include "dev_lma.h"
+
+namespace ptg {
+
+// ---------------------------------------------------------------- table layout
+// The CP pack (doubles): quantum total rate [70][61] ln values, rows in ln eta,
+// columns in ln a (cp/tables/total.rs:1-5); quantum CDF [70 * 61][32][2]
+// (n, F(n)/F_total) on the same grid (regenerated: tables/gen_cp_tables.py);
+// classical rate [61] and CDF [61][32][2] in ln a (cp/classical/rate_table.rs,
+// cdf_table.rs); Gauss-Legendre nodes (16) and nodes + weights (32) on [-1, 1]
+// (quadrature/tables.rs).
+namespace cp_pack {
+constexpr int COLS = 61, ROWS = 70;
+constexpr int RATE = 0;
+constexpr int CDF = RATE + COLS * ROWS;
+constexpr int CDF_N = COLS * ROWS;
+constexpr int CL_N = 61;
+constexpr int CL_RATE = CDF + CDF_N * 64;
+constexpr int CL_CDF = CL_RATE + CL_N;
+constexpr int G16N = CL_CDF + CL_N * 64;
+constexpr int G32N = G16N + 16;
+constexpr int G32W = G32N + 32;
+constexpr int SIZE = G32W + 32;
+}  // namespace cp_pack
+
+struct CpScalars {
+    double min[2], step[2];               // ln a, ln eta of the quantum tables (total.rs:3-4)
+    double cl_rate_min, cl_rate_step;     // classical/rate_table.rs:2-3
+    double cl_cdf_min, cl_cdf_step;       // classical/cdf_table.rs:1-2
+};
+
+// ---------------------------------------------------------------- Bessel J (bessel.rs)
+// (J_{n-1}(x), J_n(x), J_{n+1}(x)) by Miller's backward recurrence from an even
+// order a little above n, normalised by J_0 + 2 sum_k J_2k = 1 (bessel.rs:120-176).
+// Written with the reference's operation order: (2(k+2) f) / x, not f * (2(k+2)/x).
+PTG_HD inline void bessel_triple_j(int n, double x, double out[3]) {
+    if (x == 0.0) {
+        out[0] = n == 1 ? 1.0 : 0.0;
+        out[1] = n == 0 ? 1.0 : 0.0;
+        out[2] = n == -1 ? 1.0 : 0.0;
+        return;
+    }
+    double v = x / double(n);
+    int nmax = 1 + n + int(5.0 / (1.0 - 0.9 * v));
+    if (((nmax % 2) + 2) % 2 != 0) nmax += 1;
+    const int n_even = (((n % 2) + 2) % 2 == 0) ? 1 : 0;
+    const int k_stop = n - 1 - n_even;
+    double q0 = 1.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
+    double total = 0.0;
+    for (int k = nmax; k >= k_stop; k -= 2) {
+        q3 = q1;
+        q2 = q0;
+        q1 = double(2 * (k + 2)) * q2 / x - q3;
+        q0 = double(2 * (k + 1)) * q1 / x - q2;
+        total += q0;
+    }
+    double t0 = n_even ? q1 : q0, t1 = n_even ? q2 : q1, t2 = n_even ? q3 : q2;
+    for (int k = k_stop - 2; k >= 0; k -= 2) {
+        q3 = q1;
+        q2 = q0;
+        q1 = double(2 * (k + 2)) * q2 / x - q3;
+        q0 = double(2 * (k + 1)) * q1 / x - q2;
+        total += q0;
+    }
+    double norm = 2.0 * total - q0;
+    if (__builtin_isnan(norm)) { out[0] = out[1] = out[2] = 0.0; return; }
+    out[0] = t0 / norm; out[1] = t1 / norm; out[2] = t2 / norm;
+}
+// bessel.rs:178-186
+PTG_HD inline double bessel_zeta(double z) {
+    if (z < 1.0) {
+        double tmp = log((1.0 + sqrt(1.0 - z * z)) / z) - sqrt(1.0 - z * z);
+        return pow(3.0 * tmp / 2.0, 2.0 / 3.0);
+    }
+    double tmp = sqrt(z * z - 1.0) - acos(1.0 / z);
+    return -pow(3.0 * tmp / 2.0, 2.0 / 3.0);
+}
+// Debye's uniform expansion through Ai and Ai' (bessel.rs:188-198); Ai of a
+// negative argument is None in the reference and taken as 0. `tb` is the LCFA pack.
+PTG_HD inline double bessel_j_asymptotic(TP tb, int n, double x) {
+    double m = double(n);
+    double z = x / m;
+    double zeta = bessel_zeta(z);
+    double arg = pow(m, 2.0 / 3.0) * zeta;
+    double ai = arg < 0.0 ? 0.0 : airy_ai(tb, arg);
+    double aip = arg < 0.0 ? 0.0 : airy_ai_prime(tb, arg);
+    double prefactor = 4.0 * zeta / (1.0 - z * z);
+    double bk = 5.0 / (48.0 * zeta * zeta) - (2.0 + 3.0 * z * z) / (24.0 * sqrt(zeta) * pow(1.0 - z * z, 1.5));
+    return pow(prefactor, 0.25) * (ai / pow(m, 1.0 / 3.0) - aip * bk / pow(m, 5.0 / 3.0));
+}
+// (J_{n-1}, J_n, J_{n+1}): recurrence below n = 1000, uniform expansion above (bessel.rs:26-33)
+PTG_HD inline void bessel_j_pm(TP tb, int n, double x, double out[3]) {
+    if (n < 1000) { bessel_triple_j(n, x, out); return; }
+    double jn = bessel_j_asymptotic(tb, n, x), jp = bessel_j_asymptotic(tb, n + 1, x);
+    out[0] = double(2 * n) * jn / x - jp;
+    out[1] = jn;
+    out[2] = jp;
+}
+
+// ---------------------------------------------------------------- CP spectra (cp/mod.rs)
+// d(rate)/dv at harmonic n, v = s / s_max in [0, 1) (cp/mod.rs:17-38)
+PTG_HD inline double cp_ddpr(TP tb, int n, double a, double eta, double v) {
+    if (v < 0.0 || v >= 1.0) return 0.0;
+    double dn = double(n);
+    double sn = 2.0 * dn * eta / (1.0 + a * a);
+    double smax = sn / (1.0 + sn);
+    double vsmax = v * smax;
+    double w = vsmax / (sn * (1.0 - vsmax));
+    double z = sqrt(4.0 * (dn * dn) * (a * a / (1.0 + a * a)) * w * (1.0 - w));
+    double j[3];
+    bessel_j_pm(tb, n, z, j);
+    return -smax * (j[1] * j[1] + 0.5 * a * a * (1.0 + 0.5 * (vsmax * vsmax) / (1.0 - vsmax)) *
+                                      (2.0 * (j[1] * j[1]) - j[2] * j[2] - j[0] * j[0]));
+}
+PTG_HD __forceinline__ int cp_n_switch(double a) { return int(32.3 * (1.0 + 0.476 * pow(a, 1.56))); }
+// 1.2 x the largest of 32 samples of cp_ddpr (cp/mod.rs:42-75); g16 = 16 Gauss nodes
+PTG_HD inline double cp_ceiling(TP tb, const double* g16, int n, double a, double eta) {
+    double sn = 2.0 * double(n) * eta / (1.0 + a * a);
+    double lower = 0.0, upper = 0.0;
+    bool direct = sn < 2.0 || n < cp_n_switch(a);
+    double v_opt = (1.0 + sn) / (2.0 + sn);
+    for (int i = 0; i < 16; ++i) {
+        double vl, vu;
+        if (direct) {
+            vl = double(i) * v_opt / 16.0;
+            vu = v_opt + double(i) * (1.0 - v_opt) / 16.0;
+        } else {
+            double ul = 0.5 * (1.0 + sn) * (g16[i] + 1.0);
+            double uu = (1.0 + sn) + 0.5 * 2.0 * (1.0 + sn) * (g16[i] + 1.0);
+            vl = ul / (1.0 + ul);
+            vu = uu / (1.0 + uu);
+        }
+        double fl = cp_ddpr(tb, n, a, eta, vl), fu = cp_ddpr(tb, n, a, eta, vu);
+        lower = i == 0 ? fl : fmax(lower, fl);
+        upper = i == 0 ? fu : fmax(upper, fu);
+    }
+    return 1.2 * fmax(lower, upper);
+}
+// classical (eta -> 0) spectrum in v = s / s_n (cp/mod.rs:79-89)
+PTG_HD inline double cp_spectrum_low_eta(TP tb, int n, double a, double v) {
+    if (v < 0.0 || v >= 1.0) return 0.0;
+    double dn = double(n);
+    double z = sqrt(4.0 * a * a * (dn * dn) * v * (1.0 - v) / (1.0 + a * a));
+    double j[3];
+    bessel_j_pm(tb, n, z, j);
+    return dn * (a * a * (j[0] * j[0]) - 2.0 * (1.0 + a * a) * (j[1] * j[1]) + a * a * (j[2] * j[2])) / (1.0 + a * a);
+}
+// partial rate of harmonic n by 2 x 32-point Gauss-Legendre (cp/mod.rs:92-133);
+// g32n, g32w = nodes and weights on [-1, 1]
+PTG_HD inline double cp_partial_rate(TP tb, const double* g32n, const double* g32w, int n, double a, double eta) {
+    double sn = 2.0 * double(n) * eta / (1.0 + a * a);
+    double lower = 0.0, upper = 0.0;
+    if (sn < 2.0 || n < cp_n_switch(a)) {
+        double vmid = (1.0 + sn) / (2.0 + sn);
+        for (int i = 0; i < 32; ++i) lower += 0.5 * vmid * g32w[i] * cp_ddpr(tb, n, a, eta, 0.5 * vmid * (g32n[i] + 1.0));
+        for (int i = 0; i < 32; ++i)
+            upper += 0.5 * (1.0 - vmid) * g32w[i] * cp_ddpr(tb, n, a, eta, vmid + 0.5 * (1.0 - vmid) * (g32n[i] + 1.0));
+    } else {
+        for (int i = 0; i < 32; ++i) {
+            double u = 0.5 * (1.0 + sn) * (g32n[i] + 1.0);
+            lower += 0.5 * (1.0 + sn) * g32w[i] * cp_ddpr(tb, n, a, eta, u / (1.0 + u)) / ((1.0 + u) * (1.0 + u));
+        }
+        for (int i = 0; i < 32; ++i) {
+            double u = (1.0 + sn) + 0.5 * 2.0 * (1.0 + sn) * (g32n[i] + 1.0);
+            upper += 0.5 * 2.0 * (1.0 + sn) * g32w[i] * cp_ddpr(tb, n, a, eta, u / (1.0 + u)) / ((1.0 + u) * (1.0 + u));
+        }
+    }
+    return lower + upper;
+}
+// classical partial rate per unit eta (cp/mod.rs:137-155)
+PTG_HD inline double cp_integrated_spectrum_low_eta(TP tb, const double* g32n, const double* g32w, int n, double a) {
+    double lower = 0.0, upper = 0.0;
+    for (int i = 0; i < 32; ++i) lower += 0.25 * g32w[i] * cp_spectrum_low_eta(tb, n, a, 0.25 * (g32n[i] + 1.0));
+    for (int i = 0; i < 32; ++i) upper += 0.25 * g32w[i] * cp_spectrum_low_eta(tb, n, a, 0.25 * (g32n[i] + 3.0));
+    return lower + upper;
+}
+// 1.1 x the largest of at most 8 samples of the classical spectrum (cp/classical.rs:46-69)
+PTG_HD inline double cp_classical_ceiling(TP tb, int n, double a) {
+    double n_switch = round(3.0 * (1.0 + 0.54 * pow(a, 2.75)));
+    double mx;
+    if (n == 1) {
+        mx = cp_spectrum_low_eta(tb, n, a, 0.99);
+    } else if (a < 0.15 || n > int(n_switch)) {
+        mx = cp_spectrum_low_eta(tb, n, a, 0.5);
+    } else {
+        double b = a < 0.5 ? 2.3 * a : 0.8 * exp(-(a - 0.5) / 3.5) + 0.35;
+        double v = 1.0 - 0.5 * pow(log(double(n)), b) / pow(log(n_switch), b);
+        mx = 0.0;
+        for (int i = -4; i < 4; ++i) {
+            double f = cp_spectrum_low_eta(tb, n, a, fmin(v + 0.1 * double(i) / 4.0, 1.0));
+            mx = i == -4 ? f : fmax(mx, f);
+        }
+    }
+    return 1.1 * mx;
+}
+
+// ---------------------------------------------------------------- total rates
+// f(a, eta) with dP/dphase = alpha f / eta, classical limit (cp/classical.rs:22-44);
+// false outside the table (the reference reports and the probability is 0)
+PTG_HD inline bool cp_classical_rate(const double* P, const CpScalars& c, double a, double eta, double& out) {
+    double x = log(a);
+    if (x < c.cl_rate_min) { out = 2.0 * a * a * eta / 3.0; return true; }
+    double fx = (x - c.cl_rate_min) / c.cl_rate_step;
+    if (!(fx < double(cp_pack::CL_N - 1))) return false;
+    int ix = int(fx);
+    double dx = fx - double(ix);
+    const double* T = P + cp_pack::CL_RATE;
+    out = eta * exp((1.0 - dx) * T[ix] + dx * T[ix + 1]);
+    return true;
+}
+// quantum (cp/mod.rs:162-176, cp/tables/mod.rs:10-34); `rt` = the [70][61] ln-rate table
+PTG_HD inline bool cp_rate(const double* rt, const double* P, const CpScalars& c, double a, double eta, double& out) {
+    double la = log(a), le = log(eta);
+    if (le < c.min[1]) return cp_classical_rate(P, c, a, eta, out);
+    if (la < c.min[0]) {
+        out = a * a * (2.0 + 8.0 * eta + 9.0 * eta * eta + eta * eta * eta) / (2.0 * eta * ((1.0 + 2.0 * eta) * (1.0 + 2.0 * eta))) -
+              a * a * (2.0 + 2.0 * eta - eta * eta) * log(1.0 + 2.0 * eta) / (4.0 * eta * eta);
+        return true;
+    }
+    if (!(a < 20.0 && eta < 2.0)) return false;
+    double fa = (la - c.min[0]) / c.step[0], fe = (le - c.min[1]) / c.step[1];
+    int ia = int(fa), ie = int(fe);
+    ia = ia > cp_pack::COLS - 2 ? cp_pack::COLS - 2 : ia;
+    ie = ie > cp_pack::ROWS - 2 ? cp_pack::ROWS - 2 : ie;
+    double da = fa - double(ia), de = fe - double(ie);
+    const double* t0 = rt + ie * cp_pack::COLS + ia;
+    const double* t1 = t0 + cp_pack::COLS;
+    out = exp((1.0 - da) * (1.0 - de) * t0[0] + da * (1.0 - de) * t0[1] + (1.0 - da) * de * t1[0] + da * de * t1[1]);
+    return true;
+}
+
+// ---------------------------------------------------------------- harmonic index
+// One tabulated CDF [32][2] inverted at frac: 0.9 below its first value, else the
+// first 31 points in (ln n, ln(-ln(1 - F))) with linear extrapolation
+// (cp/tables/mod.rs:36-44,64-76; cp/classical.rs:71-79,100-112)
+PTG_HD inline double cp_invert_one(const double* t32, double frac) {
+    if (frac <= t32[1]) return 0.9;
+    double rs[62];
+    for (int i = 0; i < 31; ++i) {
+        rs[2 * i] = log(t32[2 * i]);
+        rs[2 * i + 1] = log(-1.0 * log(1.0 - t32[2 * i + 1]));
+    }
+    double x = 0.0;
+    pwmci_invert(rs, 31, log(-1.0 * log(1.0 - frac)), true, x);
+    return exp(x);
+}
+// quantum harmonic (cp/tables/mod.rs:46-121); 0 if the lookup leaves the table
+PTG_HD inline int cp_harmonic(const double* P, const CpScalars& c, double a, double eta, double frac) {
+    using namespace cp_pack;
+    double la = log(a), le = log(eta);
+    if (la <= c.min[0]) return 1;
+    double fx = (la - c.min[0]) / c.step[0];
+    if (!(fx < double(COLS - 1))) return 0;
+    int ix = int(fx);
+    double dx = fx - double(ix);
+    const double* T = P + CDF;
+    double n;
+    if (le <= c.min[1]) {
+        n = cp_invert_one(T + 64 * ix, frac) * (1.0 - dx) + cp_invert_one(T + 64 * (ix + 1), frac) * dx;
+    } else {
+        double fy = (le - c.min[1]) / c.step[1];
+        if (!(fy < double(ROWS - 1))) return 0;
+        int iy = int(fy);
+        double dy = fy - double(iy);
+        n = cp_invert_one(T + 64 * (COLS * iy + ix), frac) * ((1.0 - dx) * (1.0 - dy)) +
+            cp_invert_one(T + 64 * (COLS * iy + ix + 1), frac) * (dx * (1.0 - dy)) +
+            cp_invert_one(T + 64 * (COLS * (iy + 1) + ix), frac) * ((1.0 - dx) * dy) +
+            cp_invert_one(T + 64 * (COLS * (iy + 1) + ix + 1), frac) * (dx * dy);
+    }
+    n = ceil(n);
+    return (n >= 1.0 && n <= 1.0e7) ? int(n) : 0;
+}
+// classical harmonic (cp/classical.rs:83-123)
+PTG_HD inline int cp_classical_harmonic(const double* P, const CpScalars& c, double a, double frac) {
+    using namespace cp_pack;
+    double la = log(a);
+    if (la <= c.cl_cdf_min) return 1;
+    double fx = (la - c.cl_cdf_min) / c.cl_cdf_step;
+    if (!(fx < double(CL_N - 1))) return 0;
+    int ix = int(fx);
+    double dx = fx - double(ix);
+    const double* T = P + CL_CDF;
+    double n = cp_invert_one(T + 64 * ix, frac) * (1.0 - dx) + cp_invert_one(T + 64 * (ix + 1), frac) * dx;
+    n = ceil(n);
+    return (n >= 1.0 && n <= 1.0e7) ? int(n) : 0;
+}
+
+// ---------------------------------------------------------------- photon Stokes parameters
+// In the basis (k x k', k'_perp), then rotated to (e_1, e_2) by pi/2 + theta
+// (cp/mod.rs:203-232, cp/classical.rs:155-175). The reference forms 4 n^2 in
+// 32-bit integer arithmetic, which wraps for n > 23170; reproduced.
+PTG_HD __forceinline__ double cp_four_n_sqd(int n) { return double(int32_t(uint32_t(4) * uint32_t(n) * uint32_t(n))); }
+PTG_HD inline void cp_stokes(TP tb, int n, double a, double s, double sn, double v, double theta, bool classical, double sv[4]) {
+    constexpr double LASER_S3 = -1.0;  // cp/mod.rs:12
+    double z = classical ? sqrt(cp_four_n_sqd(n) * (a * a / (1.0 + a * a)) * v * (1.0 - v))
+                         : sqrt(cp_four_n_sqd(n) * (a * a / (1.0 + a * a)) * (s / (sn * (1.0 - s))) * (1.0 - s / (sn * (1.0 - s))));
+    double j[3];
+    bessel_j_pm(tb, n, z, j);
+    double jm = j[0], jn = j[1], jp = j[2];
+    double base = jm * jm + jp * jp - 2.0 * jn * jn;
+    double ja = 2.0 * jn / a;
+    double xi0, xi3;
+    if (classical) {
+        xi0 = 2.0 * base - ja * ja;
+        xi3 = 2.0 * (1.0 - 2.0 * v) * (jm * jm - jp * jp);
+    } else {
+        xi0 = (1.0 - s + 1.0 / (1.0 - s)) * base - ja * ja;
+        xi3 = LASER_S3 * (1.0 - s + 1.0 / (1.0 - s)) * (1.0 - 2.0 * s / (sn * (1.0 - s))) * (jm * jm - jp * jp);
+    }
+    double nz = double(n) / z;
+    double xi1 = 2.0 * base + 8.0 * jn * jn * (1.0 - nz * nz - 0.5 / (a * a));
+    double q = xi1 / xi0, u = 0.0 / xi0, w = xi3 / xi0;
+    double s2, c2;
+    sincos(2.0 * (0.5 * PI_D + theta), &s2, &c2);  // polarization.rs:44-53
+    sv[0] = 1.0;
+    sv[1] = c2 * q + s2 * u;
+    sv[2] = -s2 * q + c2 * u;
+    sv[3] = w;
+}
+
+// ---------------------------------------------------------------- sampler
+// (s, theta, Stokes) of one emission at harmonic n (cp/mod.rs:181-234,
+// cp/classical.rs:128-178): v = s / s_max (quantum) or s / s_n (classical) by
+// rejection under the tabulated ceiling, theta uniform. `cap` bounds the trials;
+// hitting it sets `capped` and keeps the last trial.
+__device__ inline void cp_sample_at(TP tb, const double* P, int n, double a, double eta, bool classical, Rng& rng, double& s,
+                                    double& theta, sv4& sv, uint32_t& trials, bool& capped) {
+    double mx = classical ? cp_classical_ceiling(tb, n, a) : cp_ceiling(tb, P + cp_pack::G16N, n, a, eta);
+    double v = 0.0;
+    const uint32_t cap = 1u << 20;
+    for (uint32_t t = 0;; ++t) {
+        double u;
+        rng_uniform2(rng, v, u);
+        ++trials;
+        if (t >= cap) { capped = true; break; }
+        double f = classical ? cp_spectrum_low_eta(tb, n, a, v) : cp_ddpr(tb, n, a, eta, v);
+        if (u <= f / mx) break;
+    }
+    double sn = 2.0 * double(n) * eta / (1.0 + a * a);
+    s = classical ? v * sn : v * (sn / (1.0 + sn));
+    theta = 2.0 * PI_D * rng_uniform(rng);
+    double st[4];
+    cp_stokes(tb, n, a, s, sn, v, theta, classical, st);
+    sv = {st[0], st[1], st[2], st[3]};
+}
+
+}  // namespace ptg

@@ -18,7 +18,7 @@ using TP = const double*;
 // Ai(x), Ai'(x) for x >= 0: Maclaurin series below 1, generalised Gauss-Laguerre
 // quadrature above (Gil, Segura & Temme 2002), node count chosen by x
 // (special_functions/airy.rs:41-116).
-__device__ __forceinline__ double airy_series(TP a, double x) {
+PTG_HD __forceinline__ double airy_series(TP a, double x) {
     // sum of a[k] x^n[k] over the 20 tabulated terms with n < 20
     double acc = 0.0;
     for (int k = 0; k < 20; ++k) {
@@ -30,13 +30,13 @@ __device__ __forceinline__ double airy_series(TP a, double x) {
     }
     return acc;
 }
-__device__ __forceinline__ double gl_pow_sum(TP nodes, TP weights, int n, double s, double expo) {
+PTG_HD __forceinline__ double gl_pow_sum(TP nodes, TP weights, int n, double s, double expo) {
     double acc = 0.0;
     double inv_s = 1.0 / s;
     for (int k = 0; k < n; ++k) acc += weights[k] * exp(expo * log(2.0 + nodes[k] * inv_s));
     return acc;
 }
-__device__ inline double airy_ai(TP tb, double x) {
+PTG_HD inline double airy_ai(TP tb, double x) {
     using namespace lcfa_pack;
     if (x < 1.0) return airy_series(tb + AIRY_SERIES, x);
     double s = 2.0 * x * sqrt(x) / 3.0;
@@ -47,7 +47,7 @@ __device__ inline double airy_ai(TP tb, double x) {
     else integral = gl_pow_sum(tb + AIRY_N4, tb + AIRY_W4, 4, s, -1.0 / 6.0);
     return a * integral;
 }
-__device__ inline double airy_ai_prime(TP tb, double x) {
+PTG_HD inline double airy_ai_prime(TP tb, double x) {
     using namespace lcfa_pack;
     if (x < 1.0) return airy_series(tb + AIRYP_SERIES, x);
     double s = 2.0 * x * sqrt(x) / 3.0;

@@ -24,12 +24,17 @@ constexpr int RATE = 0;                       // nonlinear_compton/lp/rate_table
 constexpr int RATE_COLS = 61, RATE_ROWS = 68;
 constexpr int CDF = RATE + RATE_COLS * RATE_ROWS;  // nonlinear_compton/lp/cdf_table.rs:4
 constexpr int CDF_COLS = 61, CDF_N = 4148;
-constexpr int SIZE = CDF + CDF_N * 32;
+constexpr int CL_N = 61;                          // classical tables in ln a (lp/classical/rate_table.rs:1)
+constexpr int CL_RATE = CDF + CDF_N * 32;         // [61]
+constexpr int CL_CDF = CL_RATE + CL_N;            // [61][16][2] (lp/classical/cdf_table.rs:4)
+constexpr int SIZE = CL_CDF + CL_N * 32;
 }  // namespace lma_pack
 
 struct LmaScalars {
     double rate_min[2], rate_step[2];  // ln a, ln eta (rate_table.rs:3-4)
     double cdf_min[2], cdf_step[2];    // (cdf_table.rs:2-3)
+    double cl_rate_min, cl_rate_step;  // (classical/rate_table.rs:2-3)
+    double cl_cdf_min, cl_cdf_step;    // (classical/cdf_table.rs:2-3)
 };
 
 // ---------------------------------------------------------------- complex functions
@@ -521,6 +526,156 @@ __device__ __forceinline__ double theta_bound_at(const ThetaBound& tb, double ar
         }
     }
     return fmin(val, PI_D / 2.0);
+}
+
+// ---------------------------------------------------------------- LP, classical limit (lp/classical.rs)
+// theta bound at fixed v = s / s_n (lp/mod.rs:211-284)
+__device__ inline ThetaBound theta_bound_low_eta(int n, double a) {
+    ThetaBound tb;
+    double ln_j_crit = log(0.443) - log(double(n)) / 3.0;
+    for (int i = 0; i < 16; ++i) {
+        double z = cos(PI_D * double(i) / 15.0);
+        double v = 0.5 * (z + 1.0);
+        tb.s[i] = v;
+        double x = 2.0 * double(n) * a * sqrt(v * (1.0 - v) / (1.0 + 0.5 * a * a));
+        double y = 0.25 * double(n) * a * a * v / (1.0 + 0.5 * a * a);
+        double x_crit = y > double(n) / 6.0 ? 4.0 * sqrt(y) * sqrt(double(n) - 2.0 * y) : double(n) + 2.0 * y;
+        double ln_j_bdy = fmin(ln_dbessel_x_zero(n % 2 == 0 ? n : n + 1, y), ln_j_crit);
+        double cos_theta = fmax(1.0 - x_crit * log(0.01) / (x * (ln_j_bdy - ln_j_crit)), -1.0);
+        double f = acos(cos_theta);
+        tb.f[i] = isnan(f) ? PI_D : f;
+    }
+    double f_min = tb.f[0];
+    for (int i = 1; i < 16; ++i) {
+        if (tb.f[i] < f_min) f_min = tb.f[i];
+        else tb.f[i] = f_min;
+    }
+    return tb;
+}
+// rate differential in v and theta, 0 < theta < pi/2 (lp/classical.rs:17-40)
+__device__ inline double lp_cl_ddpr(const DBessel& D, double a, double v, double theta) {
+    int n = D.n;
+    double x = 2.0 * double(n) * a * cos(theta) * sqrt(v * (1.0 - v) / (1.0 + 0.5 * a * a));
+    double y = 0.25 * double(n) * a * a * v / (1.0 + 0.5 * a * a);
+    double j[5];
+    dbessel_eval(D, x, y, j);
+    double g0, g1, g2;
+    if (n % 2 == 0) { g0 = j[2]; g1 = -0.5 * (j[1] + j[3]); g2 = 0.25 * (j[0] + 2.0 * j[2] + j[4]); }
+    else { g0 = -j[2]; g1 = 0.5 * (j[1] + j[3]); g2 = -0.25 * (j[0] + 2.0 * j[2] + j[4]); }
+    return (-g0 * g0 - a * a * (g0 * g2 - g1 * g1)) / (2.0 * PI_D);
+}
+// 1.1 x the largest of up to 64 samples (lp/classical.rs:43-110)
+__device__ inline double lp_cl_ceiling(const DBessel& D, double a) {
+    int n = D.n;
+    double mx = 0.0;
+    if (n == 1) {
+        mx = lp_cl_ddpr(D, a, 0.01, PI_D / 2.0);
+    } else if (a <= 1.5) {
+        double v_min = 0.1 + (9.0 / sqrt(double(n)) - 1.0) / 10.0;
+        double v_max = fmin(v_min + 0.4, 1.0);
+        for (int i = 1; i <= 16; ++i) {
+            double f = lp_cl_ddpr(D, a, v_min + (v_max - v_min) * double(i) / 16.0, 0.0);
+            mx = i == 1 ? f : fmax(mx, f);
+        }
+    } else if (a < 5.0) {
+        int n_switch = int(3.0 * pow(a / 2.0, 4.0 / 3.0));
+        double theta = n > n_switch ? 0.0 : (PI_D / 2.0) * (1.0 - log(double(n)) / log(double(n_switch)));
+        double v_min = fmin(0.35 + (7.0 / sqrt(double(n)) - 1.0) / 9.0, 0.55);
+        double v_max = 0.55 + (7.0 / sqrt(double(n)) - 1.0) / 9.0;
+        int num = a > 4.0 ? 32 : 16;
+        for (int i = 0; i < num; ++i) {
+            double f = lp_cl_ddpr(D, a, v_min + (v_max - v_min) * double(i) / double(num), theta);
+            mx = i == 0 ? f : fmax(mx, f);
+        }
+    } else {
+        double n_switch = 10.0 * ((a / 5.0) * (a / 5.0));
+        double theta = 0.0;
+        if (!(n > int(n_switch))) {
+            double pw = 0.3 + (0.7 - 0.3) * (log(a) - log(5.0)) / (log(20.0) - log(5.0));
+            theta = (PI_D / 2.0) * (pow(double(n), -pw) - pow(n_switch, -pw)) / (1.0 - pow(n_switch, -pw));
+        }
+        double v_min = 0.3 + 0.4 * log(double(n)) * pow(double(n), -0.7);
+        double v_max = v_min + 0.2;
+        int num = a > 8.0 ? 64 : 32;
+        for (int i = 0; i < num; ++i) {
+            double f = lp_cl_ddpr(D, a, v_min + (v_max - v_min) * double(i) / double(num), theta);
+            mx = i == 0 ? f : fmax(mx, f);
+        }
+    }
+    return 1.1 * mx;
+}
+// f(a, eta), dP/dphase = alpha f / eta (lp/classical.rs:189-213); `P` = the LMA pack
+__device__ inline bool lp_cl_rate(const double* P, const LmaScalars& s, double a, double eta, double& out) {
+    double x = log(a);
+    if (x < s.cl_rate_min) { out = a * a * eta / 3.0; return true; }
+    double fx = (x - s.cl_rate_min) / s.cl_rate_step;
+    if (!(fx < double(lma_pack::CL_N - 1))) return false;
+    int ix = int(fx);
+    double dx = fx - double(ix);
+    const double* T = P + lma_pack::CL_RATE;
+    out = 2.0 * eta / (1.0 + 0.5 * a * a) * exp((1.0 - dx) * T[ix] + dx * T[ix + 1]);
+    return true;
+}
+// harmonic from the [16][2] CDFs of the two neighbouring a columns (lp/classical.rs:215-252);
+// 0 outside the table
+__device__ inline int lp_cl_harmonic(const double* P, const LmaScalars& s, double a, double frac) {
+    double la = log(a);
+    if (la <= s.cl_cdf_min) return 1;
+    double fx = (la - s.cl_cdf_min) / s.cl_cdf_step;
+    if (!(fx < double(lma_pack::CL_N - 1))) return 0;
+    int ix = int(fx);
+    double dx = fx - double(ix);
+    double n = 0.0;
+    for (int k = 0; k < 2; ++k) {
+        const double* t = P + lma_pack::CL_CDF + 32 * (ix + k);
+        double nk = 0.9;
+        if (frac > t[1] && !pwmci_invert(t, 16, frac, false, nk)) return 0;
+        n += nk * (k == 0 ? 1.0 - dx : dx);
+    }
+    n = ceil(n);
+    return (n >= 1.0 && n <= 1.0e7) ? int(n) : 0;
+}
+// (s, theta, Stokes) at harmonic n (lp/classical.rs:254-333)
+__device__ inline void lp_cl_sample(const DBessel& D, double a, double eta, Rng& rng, double& s_out, double& theta_out, sv4& pol,
+                                    uint32_t& trials, bool& capped) {
+    int n = D.n;
+    ThetaBound tb = theta_bound_low_eta(n, a);
+    double smax = 2.0 * double(n) * eta / (1.0 + 0.5 * a * a);
+    double mx = lp_cl_ceiling(D, a);
+    double v = 0.5, theta = 0.0;
+    const uint32_t cap = 1u << 20;
+    for (uint32_t t = 0;; ++t) {
+        rng_uniform2(rng, v, theta);
+        theta *= PI_D / 2.0;
+        ++trials;
+        if (t >= cap) { capped = true; break; }
+        if (theta > theta_bound_at(tb, v)) continue;
+        double z = mx * rng_uniform(rng);
+        if (z < lp_cl_ddpr(D, a, v, theta)) break;
+    }
+    int quadrant = int(rng_uniform(rng) * 4.0);
+    if (quadrant == 1) theta = PI_D - theta;
+    else if (quadrant == 2) theta = PI_D + theta;
+    else if (quadrant == 3) theta = 2.0 * PI_D - theta;
+    double x = 2.0 * double(n) * a * cos(theta) * sqrt(v * (1.0 - v) / (1.0 + 0.5 * a * a));
+    double y = 0.25 * double(n) * a * a * v / (1.0 + 0.5 * a * a);
+    // the reference evaluates J at the signed x of the final quadrant and applies the
+    // parity of a negative argument for every quadrant (lp/classical.rs:300-314); so here
+    double j[5];
+    dbessel_eval(D, x, y, j);
+    double A0, A1, A2;
+    if (n % 2 == 0) { A0 = j[2]; A1 = -0.5 * (j[1] + j[3]); A2 = 0.25 * (j[0] + 2.0 * j[2] + j[4]); }
+    else { A0 = -j[2]; A1 = 0.5 * (j[1] + j[3]); A2 = -0.25 * (j[0] + 2.0 * j[2] + j[4]); }
+    double u = sqrt((1.0 + 0.5 * a * a) * (1.0 - v) / v);
+    double st_, ct_;
+    sincos(theta, &st_, &ct_);
+    double A0a = A0 / a;
+    double xi0 = 2.0 * (A1 * A1 - A0 * A2) - 2.0 * A0a * A0a;
+    double xi1 = 2.0 * (A1 * A1 - A0 * A2) - 2.0 * (1.0 + 2.0 * u * u * st_ * st_) * A0a * A0a;
+    double xi2 = 2.0 * (u * A0a) * (u * A0a) * (2.0 * st_ * ct_) + 4.0 * u * A0 * A1 * st_ / a;
+    s_out = v * smax;
+    theta_out = theta;
+    pol = {1.0, xi1 / xi0, xi2 / xi0, 0.0};
 }
 
 // (n, s, theta, Stokes) of one LP emission (lp/mod.rs:586-676). `trial_cap`

@@ -104,6 +104,8 @@ struct ptg_ctx {
     LmaScalars lsc{};
     double* d_bw = nullptr;         // LP Breit-Wheeler pack (dev_bw.h); null if its tables are absent
     BwScalars bsc{};
+    double* d_cp = nullptr;         // CP nonlinear Compton pack (dev_cp.h); null if its tables are absent
+    CpScalars csc{};
     double lma_bandwidth = 0.0, pol_angle = 0.0;
     DevBuf evq, lq[2], phq, phq2, pairq;
     double* d_ev_extra = nullptr;  // ev_k0, ev_a
@@ -249,9 +251,22 @@ bool build_lma_pack(ptg_ctx* c, std::vector<double>& pack) {
     const auto* rstep = need("nonlinear_compton/lp/rate_table:STEP", 2);
     const auto* cmin = need("nonlinear_compton/lp/cdf_table:MIN", 2);
     const auto* cstep = need("nonlinear_compton/lp/cdf_table:STEP", 2);
-    if (!rate || !cdf || !rmin || !rstep || !cmin || !cstep) return false;
+    const auto* cl_rate = need("nonlinear_compton/lp/classical/rate_table:TABLE", lma_pack::CL_N);
+    const auto* cl_cdf = need("nonlinear_compton/lp/classical/cdf_table:TABLE", size_t(lma_pack::CL_N) * 32);
+    const auto* cl_rmin = need("nonlinear_compton/lp/classical/rate_table:MIN", 1);
+    const auto* cl_rstep = need("nonlinear_compton/lp/classical/rate_table:STEP", 1);
+    const auto* cl_cmin = need("nonlinear_compton/lp/classical/cdf_table:MIN", 1);
+    const auto* cl_cstep = need("nonlinear_compton/lp/classical/cdf_table:STEP", 1);
+    if (!rate || !cdf || !rmin || !rstep || !cmin || !cstep || !cl_rate || !cl_cdf || !cl_rmin || !cl_rstep || !cl_cmin || !cl_cstep)
+        return false;
     std::copy(rate->begin(), rate->end(), pack.begin() + lma_pack::RATE);
     std::copy(cdf->begin(), cdf->end(), pack.begin() + lma_pack::CDF);
+    std::copy(cl_rate->begin(), cl_rate->end(), pack.begin() + lma_pack::CL_RATE);
+    std::copy(cl_cdf->begin(), cl_cdf->end(), pack.begin() + lma_pack::CL_CDF);
+    c->lsc.cl_rate_min = (*cl_rmin)[0];
+    c->lsc.cl_rate_step = (*cl_rstep)[0];
+    c->lsc.cl_cdf_min = (*cl_cmin)[0];
+    c->lsc.cl_cdf_step = (*cl_cstep)[0];
     for (int k = 0; k < 2; ++k) {
         c->lsc.rate_min[k] = (*rmin)[k];
         c->lsc.rate_step[k] = (*rstep)[k];
@@ -294,6 +309,40 @@ bool build_bw_pack(ptg_ctx* c, std::vector<double>& pack) {
     b.mid_ln_a_step = one("pair_creation/lp/tables/mid_range/total:LN_A_STEP");
     b.mid_eta_prime_density = one("pair_creation/lp/tables/mid_range/total:ETA_PRIME_DENSITY");
     return std::isfinite(b.hi_ln_min_a) && std::isfinite(b.mid_eta_prime_density);
+}
+
+bool build_cp_pack(ptg_ctx* c, std::vector<double>& pack) {
+    using namespace cp_pack;
+    pack.assign(SIZE, 0.0);
+    auto get = [&](const char* name, size_t n) -> const std::vector<double>* {
+        auto it = c->tables.find(name);
+        return (it == c->tables.end() || it->second.size() != n) ? nullptr : &it->second;
+    };
+    const auto* rate = get("nonlinear_compton/cp/tables/total:TABLE", size_t(CDF_N));
+    const auto* cdf = get("nonlinear_compton/cp/tables/cdf:TABLE", size_t(CDF_N) * 64);
+    const auto* cl_rate = get("nonlinear_compton/cp/classical/rate_table:TABLE", size_t(CL_N));
+    const auto* cl_cdf = get("nonlinear_compton/cp/classical/cdf_table:TABLE", size_t(CL_N) * 64);
+    const auto* g16 = get("quadrature/tables:GAUSS_16_NODES", 16);
+    const auto* g32n = get("quadrature/tables:GAUSS_32_NODES", 32);
+    const auto* g32w = get("quadrature/tables:GAUSS_32_WEIGHTS", 32);
+    const auto* mn = get("nonlinear_compton/cp/tables/total:MIN", 2);
+    const auto* st = get("nonlinear_compton/cp/tables/total:STEP", 2);
+    auto one = [&](const char* name) { const auto* v = get(name, 1); return v ? (*v)[0] : NAN; };
+    if (!rate || !cdf || !cl_rate || !cl_cdf || !g16 || !g32n || !g32w || !mn || !st) return false;
+    std::copy(rate->begin(), rate->end(), pack.begin() + RATE);
+    std::copy(cdf->begin(), cdf->end(), pack.begin() + CDF);
+    std::copy(cl_rate->begin(), cl_rate->end(), pack.begin() + CL_RATE);
+    std::copy(cl_cdf->begin(), cl_cdf->end(), pack.begin() + CL_CDF);
+    std::copy(g16->begin(), g16->end(), pack.begin() + G16N);
+    std::copy(g32n->begin(), g32n->end(), pack.begin() + G32N);
+    std::copy(g32w->begin(), g32w->end(), pack.begin() + G32W);
+    CpScalars& q = c->csc;
+    for (int k = 0; k < 2; ++k) { q.min[k] = (*mn)[k]; q.step[k] = (*st)[k]; }
+    q.cl_rate_min = one("nonlinear_compton/cp/classical/rate_table:MIN");
+    q.cl_rate_step = one("nonlinear_compton/cp/classical/rate_table:STEP");
+    q.cl_cdf_min = one("nonlinear_compton/cp/classical/cdf_table:MIN");
+    q.cl_cdf_step = one("nonlinear_compton/cp/classical/cdf_table:STEP");
+    return std::isfinite(q.cl_rate_min) && std::isfinite(q.cl_cdf_step);
 }
 
 // Laser constants (fast_focused_laser.rs:24-38,47-53,129-169; fast_plane_wave.rs:23-108;
@@ -394,6 +443,11 @@ int ptg_create(ptg_ctx** out, const char* table_path, int device) {
         HIPCHK(c, hipMalloc(&c->d_bw, bpack.size() * sizeof(double)));
         HIPCHK(c, scopy(c, c->d_bw, bpack.data(), bpack.size() * sizeof(double), hipMemcpyHostToDevice));
     }
+    std::vector<double> cpack;
+    if (build_cp_pack(c, cpack)) {
+        HIPCHK(c, hipMalloc(&c->d_cp, cpack.size() * sizeof(double)));
+        HIPCHK(c, scopy(c, c->d_cp, cpack.data(), cpack.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     HIPCHK(c, hipMalloc(&c->d_lma, lpack.size() * sizeof(double)));
     HIPCHK(c, scopy(c, c->d_lma, lpack.data(), lpack.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMalloc(&c->d_lcfa, pack.size() * sizeof(double)));
@@ -414,6 +468,7 @@ void ptg_destroy(ptg_ctx* c) {
     if (c->d_lcfa) (void)hipFree(c->d_lcfa);
     if (c->d_lma) (void)hipFree(c->d_lma);
     if (c->d_bw) (void)hipFree(c->d_bw);
+    if (c->d_cp) (void)hipFree(c->d_cp);
     if (c->d_ev_extra) (void)hipFree(c->d_ev_extra);
     if (c->d_scratch) (void)hipFree(c->d_scratch);
     if (c->d_ovf) (void)hipFree(c->d_ovf);
@@ -703,8 +758,9 @@ static int run_lma(ptg_ctx* c, TransportArgs a, double* ms, uint64_t* launches, 
     const bool track = a.opt.tracking_photons != 0;
     int len_cap = 0;
     {
-        // workspace per lane for J(n, x, y), 5 doubles per order
-        int ncap = lma_harmonic_cap(c->laser.a0);
+        // workspace per lane for J(n, x, y), 5 doubles per order (LP emission and LP
+        // pair creation; circular polarisation needs none)
+        int ncap = c->laser.circular ? 16 : lma_harmonic_cap(c->laser.a0);
         // diagnostic: a smaller workspace sends most events through the deferred pass,
         // which must reproduce the same shower
         if (const char* e = std::getenv("PTG_LMA_HARMONIC_CAP")) {
@@ -729,6 +785,10 @@ static int run_lma(ptg_ctx* c, TransportArgs a, double* ms, uint64_t* launches, 
     m.sc = c->lsc;
     m.bw = c->d_bw;
     m.bsc = c->bsc;
+    m.cp = c->d_cp;
+    m.csc = c->csc;
+    m.lcfa = c->d_lcfa;
+    m.mode = !c->laser.circular ? (a.classical_mode ? LMA_LP_CLASSICAL : LMA_LP) : (a.classical_mode ? LMA_CP_CLASSICAL : LMA_CP);
     m.bandwidth = c->lma_bandwidth;
     m.pol_angle = c->pol_angle;
     m.sin_pa = std::sin(c->pol_angle);
@@ -895,8 +955,11 @@ int ptg_run(ptg_ctx* c, const ptg_collide_options* o, uint64_t seed, ptg_counter
         if (o->tracking_photons && !c->d_bw)
             return fail(c, PTG_E_UNSUPPORTED, "pair creation in a PlaneWave needs the regenerated Breit-Wheeler mid-range table "
                                               "(tables/gen_bw_tables.py, then tables/extract.py)");
-        if (o->classical) return fail(c, PTG_E_UNSUPPORTED, "classical LMA emission is not built into this library");
-        if (c->laser.circular) return fail(c, PTG_E_UNSUPPORTED, "circularly polarised LMA emission is not built into this library yet");
+        if (c->laser.circular && !c->d_cp)
+            return fail(c, PTG_E_UNSUPPORTED, "circularly polarised LMA emission needs the regenerated CP CDF table "
+                                              "(tables/gen_cp_tables.py, then tables/extract.py)");
+        if (c->laser.circular && o->tracking_photons)
+            return fail(c, PTG_E_UNSUPPORTED, "pair creation in a circularly polarised PlaneWave is not built into this library yet");
     }
     HIPCHK(c, hipSetDevice(c->device));
     const size_t n = c->n_prim;
@@ -1153,12 +1216,25 @@ int ptg_histogram(ptg_ctx* c, ptg_hist_spec* s, double* bins_out, double* total_
 int ptg_eval(ptg_ctx* c, int fn, const double* in, size_t n, size_t in_w, double* out, size_t out_w) {
     if (!c || !in || !out) return fail(c, PTG_E_ARG, "null argument");
     if (fn < 0 || fn >= PTG_FN_COUNT) return fail(c, PTG_E_ARG, "unknown function id");
-    static const int need_in[] = {2, 5, 1, 1, 4, 1, 3, 12, 2, 3, 4, 4, 2, 4, 6};
-    static const int need_out[] = {1, 4, 2, 4, 7, 1, 1, 4, 2, 1, 5, 6, 3, 1, 4};
+    static const int need_in[] = {2, 5, 1, 1, 4, 1, 3, 12, 2, 3, 4, 4, 2, 4, 6, 2, 3, 4, 4, 5, 2, 2, 2, 4};
+    static const int need_out[] = {1, 4, 2, 4, 7, 1, 1, 4, 2, 1, 5, 6, 3, 1, 4, 3, 2, 1, 2, 7, 2, 1, 1, 6};
+    static_assert(sizeof(need_in) / sizeof(int) == PTG_FN_COUNT && sizeof(need_out) / sizeof(int) == PTG_FN_COUNT, "probe widths");
     if (int(in_w) < need_in[fn] || int(out_w) < need_out[fn]) return fail(c, PTG_E_ARG, "row width too small for this function");
     if (fn == PTG_FN_FIELDS && !c->have_laser) return fail(c, PTG_E_STATE, "no laser set");
     if (n == 0) return PTG_OK;
-    if (fn >= PTG_FN_BW_RATE && !c->d_bw) return fail(c, PTG_E_UNSUPPORTED, "Breit-Wheeler tables not loaded");
+    if (fn >= PTG_FN_BW_RATE && fn <= PTG_FN_BW_SAMPLE && !c->d_bw) return fail(c, PTG_E_UNSUPPORTED, "Breit-Wheeler tables not loaded");
+    if (fn >= PTG_FN_CP_RATE && fn <= PTG_FN_CP_SAMPLE && !c->d_cp) return fail(c, PTG_E_UNSUPPORTED, "CP nonlinear Compton tables not loaded");
+    if (fn == PTG_FN_BESSEL_J || fn == PTG_FN_CP_PARTIAL_RATE || fn == PTG_FN_CP_SAMPLE) {
+        for (size_t i = 0; i < n; ++i) {
+            const double* x = in + i * in_w;
+            if (fn == PTG_FN_BESSEL_J && !(x[0] >= 1.0 && x[0] <= 1.0e7 && double(int(x[0])) == x[0] && x[1] >= 0.0))
+                return fail(c, PTG_E_ARG, "Bessel probe needs an integer 1 <= n <= 1e7 and x >= 0");
+            if (fn == PTG_FN_CP_PARTIAL_RATE && !(x[0] >= 1.0 && x[0] <= 1.0e6 && double(int(x[0])) == x[0] && x[1] > 0.0 && x[2] > 0.0))
+                return fail(c, PTG_E_ARG, "CP partial-rate probe needs an integer 1 <= n <= 1e6, a > 0, eta > 0");
+            if (fn == PTG_FN_CP_SAMPLE && !(x[3] >= 0.0 && x[3] <= 1.0e6 && x[0] > 0.0 && x[1] > 0.0))
+                return fail(c, PTG_E_ARG, "CP sample probe needs a > 0, eta > 0 and 0 <= fixed n <= 1e6");
+        }
+    }
     if (fn >= PTG_FN_LP_RATE) {
         HIPCHK(c, hipSetDevice(c->device));
         int nmax = 16;
@@ -1172,7 +1248,7 @@ int ptg_eval(ptg_ctx* c, int fn, const double* in, size_t n, size_t in_w, double
                 nmax = std::max(nmax, std::min(hi, 200000));
             }
         }
-        if (fn == PTG_FN_DOUBLE_BESSEL || fn == PTG_FN_LP_SAMPLE) {
+        if (fn == PTG_FN_DOUBLE_BESSEL || fn == PTG_FN_LP_SAMPLE || fn == PTG_FN_LP_CL_SAMPLE || fn == PTG_FN_LP_CL_CEILING) {
             for (size_t i = 0; i < n; ++i) {
                 const double* x = in + i * in_w;
                 int nn;
@@ -1183,6 +1259,10 @@ int ptg_eval(ptg_ctx* c, int fn, const double* in, size_t n, size_t in_w, double
                         return fail(c, PTG_E_ARG, "J(n, x, y) probe needs an integer 3 <= n <= 2e5");
                     if (!(x[1] >= 0.0 && x[1] <= nn * 1.4142135623730951 && x[2] >= 0.0 && x[2] <= 0.5 * nn))
                         return fail(c, PTG_E_ARG, "J(n, x, y) probe needs 0 <= x <= n sqrt 2 and 0 <= y <= n / 2");
+                } else if (fn == PTG_FN_LP_CL_CEILING) {
+                    nn = int(x[0]);
+                    if (!(x[0] >= 1.0 && x[0] <= 2.0e5) || double(nn) != x[0] || !(x[1] > 0.0))
+                        return fail(c, PTG_E_ARG, "LP classical ceiling probe needs an integer 1 <= n <= 2e5 and a > 0");
                 } else {
                     if (!(x[3] >= 0.0 && x[3] <= 2.0e5)) return fail(c, PTG_E_ARG, "LP sample probe needs 0 <= fixed n <= 2e5");
                     nn = x[3] > 0 ? int(x[3]) : lma_harmonic_cap(x[0]);
@@ -1207,8 +1287,17 @@ int ptg_eval(ptg_ctx* c, int fn, const double* in, size_t n, size_t in_w, double
             HIPCHK(c, hipMalloc(&dscr, m * len * 5 * 8));
             HIPCHK(c, scopy(c, din, hin.data(), m * in_w * 8, hipMemcpyHostToDevice));
             HIPCHK(c, sfill(c, dout, 0, m * out_w * 8));
-            launch_eval_lma(fn, din, dout, uint32_t(m), uint32_t(in_w), uint32_t(out_w), c->d_lma, c->lsc, c->d_bw, c->bsc, dscr,
-                            int(len), key_from_seed(0), c->stream);
+            LmaArgs la{};
+            la.tables = c->d_lma;
+            la.sc = c->lsc;
+            la.bw = c->d_bw;
+            la.bsc = c->bsc;
+            la.cp = c->d_cp;
+            la.csc = c->csc;
+            la.lcfa = c->d_lcfa;
+            la.scratch = dscr;
+            la.scratch_len = int(len);
+            launch_eval_lma(fn, din, dout, uint32_t(m), uint32_t(in_w), uint32_t(out_w), la, key_from_seed(0), c->stream);
             HIPCHK(c, hipGetLastError());
             HIPCHK(c, hipStreamSynchronize(c->stream));
             HIPCHK(c, scopy(c, hout.data(), dout, m * out_w * 8, hipMemcpyDeviceToHost));
@@ -1217,7 +1306,8 @@ int ptg_eval(ptg_ctx* c, int fn, const double* in, size_t n, size_t in_w, double
             nmax = 16;
             for (size_t k = 0; k < m; ++k) {
                 std::memcpy(out + rows[k] * out_w, &hout[k * out_w], out_w * 8);
-                if ((fn == PTG_FN_LP_SAMPLE && hout[k * out_w + 5] == -1.0) || (fn == PTG_FN_BW_SAMPLE && hout[k * out_w + 3] == -1.0)) {
+                if (((fn == PTG_FN_LP_SAMPLE || fn == PTG_FN_LP_CL_SAMPLE) && hout[k * out_w + 5] == -1.0) ||
+                    (fn == PTG_FN_BW_SAMPLE && hout[k * out_w + 3] == -1.0)) {
                     again.push_back(rows[k]);
                     nmax = std::max(nmax, int(hout[k * out_w]));
                 }

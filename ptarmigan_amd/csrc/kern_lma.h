@@ -1,14 +1,23 @@
 // Arguments and launchers of the LMA (PlaneWave) kernels.
 #pragma once
 #include "dev_bw.h"
+#include "dev_cp.h"
 #include "dev_lma.h"
 #include "kern_common.h"
 
 namespace ptg {
 
+// Emission model of the lepton kernels: linear or circular polarisation of the
+// wave, quantum or classical radiation mode (nonlinear_compton/mod.rs:31-36)
+enum LmaMode : int { LMA_LP = 0, LMA_CP = 1, LMA_CP_CLASSICAL = 2, LMA_LP_CLASSICAL = 3 };
+
 struct LmaArgs {
     const double* tables;        // LMA pack (dev_lma.h)
     LmaScalars sc;
+    const double* cp;            // CP pack (dev_cp.h)
+    CpScalars csc;
+    const double* lcfa;          // LCFA pack (Airy functions for J_n at n >= 1000)
+    int mode;                    // LmaMode
     double bandwidth;            // fractional bandwidth (plane_wave.rs:49-63); 0 = monochromatic
     double pol_angle, sin_pa, cos_pa;
     // step kernel -> event queue
@@ -50,8 +59,9 @@ void launch_photon_step_lma(const TransportArgs* a, const LmaArgs& m, int grid, 
 void launch_pair_event_lma(const TransportArgs* a, const LmaArgs& m, int lanes, hipStream_t s);
 void launch_push_lma(const LaserDev& L, double dt, int ll, const uint8_t* sp_, double* r4, double* u4, uint32_t n,
                      uint32_t n_steps, uint32_t* taken, double* work, hipStream_t s);
-void launch_eval_lma(int fn, const double* in, double* out, uint32_t n, uint32_t in_w, uint32_t out_w, const double* tables,
-                     const LmaScalars& sc, const double* bw, const BwScalars& bsc, double* scratch, int scratch_len, uint64_t key,
+// probes: the table pointers, scalars and scratch (J(n, x, y) workspace of
+// scratch_len per row) are taken from m
+void launch_eval_lma(int fn, const double* in, double* out, uint32_t n, uint32_t in_w, uint32_t out_w, const LmaArgs& m, uint64_t key,
                      hipStream_t s);
 
 }  // namespace ptg

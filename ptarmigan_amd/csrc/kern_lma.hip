@@ -19,13 +19,26 @@ namespace ptg {
 // (c0, c1 = k); sampling event k uses (c0, c1 = k | EVENT_BLOCK).
 constexpr uint32_t EVENT_BLOCK = 0x80000000u;
 
+// rate table staged in LDS per emission model: LP [68][61], CP [70][61]; the
+// classical tables (61 values) are read through the cache
+template <int MODE>
+struct StepTable {
+    static constexpr int N = MODE == LMA_LP ? lma_pack::RATE_COLS * lma_pack::RATE_ROWS
+                                            : (MODE == LMA_CP ? cp_pack::COLS * cp_pack::ROWS : 1);
+    __device__ static const double* src(const LmaArgs& M) { return MODE == LMA_LP ? M.tables + lma_pack::RATE : M.cp + cp_pack::RATE; }
+};
+
+template <int MODE>
 __global__ __launch_bounds__(256) void k_step_lma(const TransportArgs* __restrict__ A, LmaArgs M) {
     const LaserDev& L = A->laser;
     const double dt = A->dt;
     const double kc = C_LIGHT * T_COMPTON;
-    __shared__ double rt[lma_pack::RATE_COLS * lma_pack::RATE_ROWS];  // LP rate table (32 KB)
-    for (int i = threadIdx.x; i < lma_pack::RATE_COLS * lma_pack::RATE_ROWS; i += blockDim.x) rt[i] = M.tables[lma_pack::RATE + i];
-    __syncthreads();
+    __shared__ double rt[StepTable<MODE>::N];
+    if (MODE == LMA_LP || MODE == LMA_CP) {
+        const double* src = StepTable<MODE>::src(M);
+        for (int i = threadIdx.x; i < StepTable<MODE>::N; i += blockDim.x) rt[i] = src[i];
+        __syncthreads();
+    }
 
     uint64_t n_steps = 0;
     uint32_t flags = 0, psteps = 0;
@@ -88,7 +101,12 @@ __global__ __launch_bounds__(256) void k_step_lma(const TransportArgs* __restric
         double dphi = dt * eta / (T_COMPTON * u.t);
         double f;
         double prob = 0.0;
-        if (lp_rate(rt, M.sc, a_rms * 1.4142135623730951, eta, f)) prob = ALPHA * f * dphi / eta;
+        bool ok;
+        if (MODE == LMA_LP) ok = lp_rate(rt, M.sc, a_rms * 1.4142135623730951, eta, f);
+        else if (MODE == LMA_CP) ok = cp_rate(rt, M.cp, M.csc, a_rms, eta, f);
+        else if (MODE == LMA_CP_CLASSICAL) ok = cp_classical_rate(M.cp, M.csc, a_rms, eta, f);
+        else ok = lp_cl_rate(M.tables, M.sc, a_rms * 1.4142135623730951, eta, f);
+        if (ok) prob = ALPHA * f * dphi / eta;
         else flags |= PTG_FLAG_TABLE_RANGE;
         S *= (1.0 - prob);
         if (S < U) {
@@ -114,8 +132,12 @@ __global__ __launch_bounds__(256) void k_step_lma(const TransportArgs* __restric
 }
 
 // One emission per lane (nonlinear_compton::generate, mod.rs:58-138, with
-// lp::sample, lp/mod.rs:586-676).
+// lp::sample, lp/mod.rs:586-676, cp::sample, cp/mod.rs:181-234, or
+// cp::classical::sample, cp/classical.rs:128-178).
+template <int MODE>
 __global__ __launch_bounds__(64) void k_event_lma(const TransportArgs* __restrict__ A, LmaArgs M) {
+    constexpr bool classical = MODE == LMA_CP_CLASSICAL || MODE == LMA_LP_CLASSICAL;
+    constexpr bool linear = MODE == LMA_LP || MODE == LMA_LP_CLASSICAL;
     const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n_lanes = gridDim.x * blockDim.x;
     double* ws = M.scratch + lane;  // this lane's J(n, x, y) workspace, stride n_lanes
@@ -140,9 +162,19 @@ __global__ __launch_bounds__(64) void k_event_lma(const TransportArgs* __restric
         uint32_t tr = 0;
         bool capped = false;
         int n_cap_ok = 1;
-        {
+        if (!linear) {
+            int nh = classical ? cp_classical_harmonic(M.cp, M.csc, a, rng_uniform(rng))
+                               : cp_harmonic(M.cp, M.csc, a, eta, rng_uniform(rng));
+            if (nh < 1) {
+                flags |= PTG_FLAG_TABLE_RANGE;
+                n_cap_ok = 0;
+            } else {
+                n = nh;
+                cp_sample_at(M.lcfa, M.cp, nh, a, eta, classical, rng, s, cphi, sv, tr, capped);
+            }
+        } else {
             // harmonic first, to check the workspace is large enough
-            int nh = lp_harmonic(cdf, M.sc, alp, eta, rng_uniform(rng));
+            int nh = classical ? lp_cl_harmonic(M.tables, M.sc, alp, rng_uniform(rng)) : lp_harmonic(cdf, M.sc, alp, eta, rng_uniform(rng));
             int need = dbessel_len(nh);
             if (nh < 1) {
                 flags |= PTG_FLAG_TABLE_RANGE;
@@ -155,6 +187,9 @@ __global__ __launch_bounds__(64) void k_event_lma(const TransportArgs* __restric
                 }
                 flags |= PTG_FLAG_TABLE_RANGE;
                 n_cap_ok = 0;
+            } else if (classical) {
+                n = nh;
+                lp_cl_sample(dbessel_at(nh, ws, n_lanes), alp, eta, rng, s, cphi, sv, tr, capped);
             } else {
                 lp_sample(cdf, M.sc, alp, eta, rng, ws, n_lanes, n, s, cphi, sv, tr, capped, nh);
             }
@@ -166,12 +201,15 @@ __global__ __launch_bounds__(64) void k_event_lma(const TransportArgs* __restric
         if (n_cap_ok) {
             // photon momentum in the zero-momentum frame, boosted back (mod.rs:73-116)
             double ell = double(n);
-            double p_zmf = ell * eta / sqrt(1.0 + a * a + 2.0 * ell * eta);
-            double cos_theta_zmf = 1.0 - s * (1.0 + a * a + 2.0 * ell * eta) / (ell * eta);
-            d4 qk{u.t + ell * k0, u.x, u.y, u.z + ell * k0};
+            double p_zmf = classical ? ell * eta / sqrt(1.0 + a * a) : ell * eta / sqrt(1.0 + a * a + 2.0 * ell * eta);
+            double cos_theta_zmf = classical ? 1.0 - s * (1.0 + a * a) / (ell * eta)
+                                             : 1.0 - s * (1.0 + a * a + 2.0 * ell * eta) / (ell * eta);
+            // the ZMF moves with q (classical) or q + n kappa (quantum); the photon's axis
+            // is along n kappa or against q there
+            d4 qk = classical ? u : d4{u.t + ell * k0, u.x, u.y, u.z + ell * k0};
             double inv = 1.0 / sqrt(mdot(qk, qk));
             d4 u_zmf{qk.t * inv, qk.x * inv, qk.y * inv, qk.z * inv};
-            d3 along = -unit(sp(boost4(u, u_zmf)));
+            d3 along = classical ? unit(sp(boost4(d4{ell * k0, 0.0, 0.0, ell * k0}, u_zmf))) : -unit(sp(boost4(u, u_zmf)));
             d3 eps = unit(sp(boost4(d4{0.0, M.cos_pa, M.sin_pa, 0.0}, u_zmf)));
             eps = unit(cross(along, cross(eps, along)));
             double sp_, cp_;
@@ -205,7 +243,7 @@ __global__ __launch_bounds__(64) void k_event_lma(const TransportArgs* __restric
             ++n_emit;
             if (A->recoil) u_new = {u.t + ell * k0 - k.t, u.x - k.x, u.y - k.y, u.z + ell * k0 - k.z};
             icount += 1.0;
-            work += ell * k0;  // absorption (src/main.rs:198-200)
+            if (!classical) work += ell * k0;  // absorption (src/main.rs:198-200)
         }
         // the electron continues with its next RNG segment
         unsigned long long j = atomicAdd(M.next_ctr, 1ull);
@@ -446,10 +484,20 @@ void launch_pair_event_lma(const TransportArgs* a, const LmaArgs& m, int lanes, 
 }
 
 void launch_step_lma(const TransportArgs* a, const LmaArgs& m, int grid, hipStream_t s) {
-    k_step_lma<<<grid, 256, 0, s>>>(a, m);
+    switch (m.mode) {
+    case LMA_CP: k_step_lma<LMA_CP><<<grid, 256, 0, s>>>(a, m); break;
+    case LMA_CP_CLASSICAL: k_step_lma<LMA_CP_CLASSICAL><<<grid, 256, 0, s>>>(a, m); break;
+    case LMA_LP_CLASSICAL: k_step_lma<LMA_LP_CLASSICAL><<<grid, 256, 0, s>>>(a, m); break;
+    default: k_step_lma<LMA_LP><<<grid, 256, 0, s>>>(a, m); break;
+    }
 }
 void launch_event_lma(const TransportArgs* a, const LmaArgs& m, int lanes, hipStream_t s) {
-    k_event_lma<<<lanes / 64, 64, 0, s>>>(a, m);
+    switch (m.mode) {
+    case LMA_CP: k_event_lma<LMA_CP><<<lanes / 64, 64, 0, s>>>(a, m); break;
+    case LMA_CP_CLASSICAL: k_event_lma<LMA_CP_CLASSICAL><<<lanes / 64, 64, 0, s>>>(a, m); break;
+    case LMA_LP_CLASSICAL: k_event_lma<LMA_LP_CLASSICAL><<<lanes / 64, 64, 0, s>>>(a, m); break;
+    default: k_event_lma<LMA_LP><<<lanes / 64, 64, 0, s>>>(a, m); break;
+    }
 }
 
 // Field-only push in the plane wave (test support for PlaneWave::push).
@@ -474,11 +522,16 @@ void launch_push_lma(const LaserDev& L, double dt, int ll, const uint8_t* sp_, d
 }
 
 // Probes of the LMA device functions (parity tests).
-__global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, uint32_t in_w, uint32_t out_w,
-                           const double* tables, LmaScalars sc, const double* bw, BwScalars bsc, double* scratch,
-                           int scratch_len, uint64_t key) {
+__global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, uint32_t in_w, uint32_t out_w, LmaArgs m,
+                           uint64_t key) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const double* tables = m.tables;
+    const LmaScalars& sc = m.sc;
+    const double* bw = m.bw;
+    const BwScalars& bsc = m.bsc;
+    double* scratch = m.scratch;
+    const int scratch_len = m.scratch_len;
     const double* x = in + uint64_t(i) * in_w;
     double* y = out + uint64_t(i) * out_w;
     switch (fn) {
@@ -539,13 +592,80 @@ __global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, ui
         y[0] = nh; y[1] = s; y[2] = th; y[3] = tr;
         break;
     }
+    case PTG_FN_LP_CL_RATE: {
+        double f = 0.0;
+        y[1] = lp_cl_rate(tables, sc, x[0], x[1], f) ? 1.0 : 0.0;
+        y[0] = f;
+        break;
+    }
+    case PTG_FN_LP_CL_HARMONIC: y[0] = lp_cl_harmonic(tables, sc, x[0], x[1]); break;
+    case PTG_FN_LP_CL_CEILING: {
+        int nn = int(x[0]);  // the host checks 1 <= n and the workspace
+        y[0] = lp_cl_ceiling(dbessel_at(nn, scratch + i, n), x[1]);
+        break;
+    }
+    case PTG_FN_LP_CL_SAMPLE: {
+        // (a, eta, stream, fixed_n) -> (n, s, theta, S1, S2, trials); trials -1: workspace too small, -2: outside the table
+        Rng rng = rng_make(key, uint64_t(x[2]));
+        int nh = x[3] > 0.0 ? int(x[3]) : lp_cl_harmonic(tables, sc, x[0], rng_uniform(rng));
+        if (nh < 1) { y[0] = 0.0; y[5] = -2.0; break; }
+        if (dbessel_len(nh) > scratch_len) { y[0] = nh; y[5] = -1.0; break; }
+        double s, th;
+        sv4 pol;
+        uint32_t tr = 0;
+        bool capped = false;
+        lp_cl_sample(dbessel_at(nh, scratch + i, n), x[0], x[1], rng, s, th, pol, tr, capped);
+        y[0] = nh; y[1] = s; y[2] = th; y[3] = pol.q; y[4] = pol.u; y[5] = tr;
+        break;
+    }
+    case PTG_FN_BESSEL_J: {
+        double j[3];
+        bessel_j_pm(m.lcfa, int(x[0]), x[1], j);  // the host checks 1 <= n <= 1e7, x >= 0
+        y[0] = j[0]; y[1] = j[1]; y[2] = j[2];
+        break;
+    }
+    case PTG_FN_CP_RATE: {
+        double f = 0.0;
+        bool ok = x[2] != 0.0 ? cp_classical_rate(m.cp, m.csc, x[0], x[1], f) : cp_rate(m.cp + cp_pack::RATE, m.cp, m.csc, x[0], x[1], f);
+        y[0] = f;
+        y[1] = ok ? 1.0 : 0.0;
+        break;
+    }
+    case PTG_FN_CP_HARMONIC:
+        y[0] = x[3] != 0.0 ? cp_classical_harmonic(m.cp, m.csc, x[0], x[2]) : cp_harmonic(m.cp, m.csc, x[0], x[1], x[2]);
+        break;
+    case PTG_FN_CP_PARTIAL_RATE: {
+        int nn = int(x[0]);  // the host checks 1 <= n <= 1e6
+        if (x[3] != 0.0) {
+            y[0] = cp_integrated_spectrum_low_eta(m.lcfa, m.cp + cp_pack::G32N, m.cp + cp_pack::G32W, nn, x[1]);
+            y[1] = cp_classical_ceiling(m.lcfa, nn, x[1]);
+        } else {
+            y[0] = cp_partial_rate(m.lcfa, m.cp + cp_pack::G32N, m.cp + cp_pack::G32W, nn, x[1], x[2]);
+            y[1] = cp_ceiling(m.lcfa, m.cp + cp_pack::G16N, nn, x[1], x[2]);
+        }
+        break;
+    }
+    case PTG_FN_CP_SAMPLE: {
+        // (a, eta, stream, fixed_n, classical) -> (n, s, theta, S1, S2, S3, trials); trials = -2: outside the table
+        Rng rng = rng_make(key, uint64_t(x[2]));
+        bool cl = x[4] != 0.0;
+        int nh = x[3] > 0.0 ? int(x[3]) : (cl ? cp_classical_harmonic(m.cp, m.csc, x[0], rng_uniform(rng))
+                                                 : cp_harmonic(m.cp, m.csc, x[0], x[1], rng_uniform(rng)));
+        if (nh < 1) { y[0] = 0.0; y[6] = -2.0; break; }
+        double s, th;
+        sv4 sv;
+        uint32_t tr = 0;
+        bool capped = false;
+        cp_sample_at(m.lcfa, m.cp, nh, x[0], x[1], cl, rng, s, th, sv, tr, capped);
+        y[0] = nh; y[1] = s; y[2] = th; y[3] = sv.q; y[4] = sv.u; y[5] = sv.v; y[6] = capped ? -1.0 : double(tr);
+        break;
+    }
     default: break;
     }
 }
-void launch_eval_lma(int fn, const double* in, double* out, uint32_t n, uint32_t in_w, uint32_t out_w, const double* tables,
-                     const LmaScalars& sc, const double* bw, const BwScalars& bsc, double* scratch, int scratch_len, uint64_t key,
+void launch_eval_lma(int fn, const double* in, double* out, uint32_t n, uint32_t in_w, uint32_t out_w, const LmaArgs& m, uint64_t key,
                      hipStream_t s) {
-    if (n) k_eval_lma<<<(n + 63) / 64, 64, 0, s>>>(fn, in, out, n, in_w, out_w, tables, sc, bw, bsc, scratch, scratch_len, key);
+    if (n) k_eval_lma<<<(n + 63) / 64, 64, 0, s>>>(fn, in, out, n, in_w, out_w, m, key);
 }
 
 }  // namespace ptg

@@ -299,3 +299,12 @@ class Fn(enum.IntEnum):
     BW_RATE = 12
     BW_HARMONIC = 13
     BW_SAMPLE = 14
+    BESSEL_J = 15
+    CP_RATE = 16
+    CP_HARMONIC = 17
+    CP_PARTIAL_RATE = 18
+    CP_SAMPLE = 19
+    LP_CL_RATE = 20
+    LP_CL_HARMONIC = 21
+    LP_CL_CEILING = 22
+    LP_CL_SAMPLE = 23

@@ -38,7 +38,11 @@ SOURCES = [
     ("lcfa/pair_creation/tables.rs", None),
     ("nonlinear_compton/lp/rate_table.rs", None),
     ("nonlinear_compton/lp/cdf_table.rs", None),
+    ("nonlinear_compton/lp/classical/rate_table.rs", None),
+    ("nonlinear_compton/lp/classical/cdf_table.rs", None),
     ("nonlinear_compton/cp/tables/total.rs", None),
+    ("nonlinear_compton/cp/classical/rate_table.rs", None),
+    ("nonlinear_compton/cp/classical/cdf_table.rs", None),
     ("pair_creation/lp/tables/total.rs", None),
     ("pair_creation/lp/tables/cdf.rs", None),
     ("pair_creation/lp/tables/mid_range/total.rs", None),
@@ -134,6 +138,8 @@ def extract_file(ref_root, rel, names):
 GENERATED = [
     ("pair_creation/lp/tables/mid_range/cdf:TABLE", "nbw_lp_mid_cdf.f64", [5652, 16, 3],
      "regenerated by tables/gen_bw_tables.py (reference generator: src/pair_creation/lp/mod.rs:1065-1163)"),
+    ("nonlinear_compton/cp/tables/cdf:TABLE", "nlc_cp_cdf.f64", [4270, 32, 2],
+     "regenerated by tables/gen_cp_tables.py (reference generator: src/nonlinear_compton/cp/mod.rs:474-612)"),
 ]
 
 

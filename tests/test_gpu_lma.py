@@ -238,3 +238,85 @@ def test_deferred_high_harmonics_reproduce_the_shower(engine, orc, monkeypatch):
         assert np.array_equal(s1["u"][m1][k1], s2["u"][m2][k2])
         assert np.array_equal(s1["pol"][m1][k1], s2["pol"][m2][k2])
         assert np.array_equal(s1["r"][m1][k1], s2["r"][m2][k2])
+
+
+# ---------------------------------------------------------------- LP, classical limit (lp/classical.rs)
+def test_lp_classical_rate_and_harmonic_match_oracle(engine, orc):
+    from ptarmigan_amd import Fn
+    say("lp classical rate, harmonic")
+    rng = np.random.default_rng(21)
+    m = 3000
+    a = np.exp(rng.uniform(np.log(0.005), np.log(25.0), m))
+    eta = np.exp(rng.uniform(np.log(1e-4), np.log(1.0), m))
+    got = engine.eval(Fn.LP_CL_RATE, np.stack([a, eta], 1), 2)
+    for i in range(m):
+        ref = orc.lp_cl_rate(a[i], eta[i])
+        assert (got[i, 1] == 1.0) == (ref is not None), a[i]
+        if ref is not None:
+            assert got[i, 0] == pytest.approx(ref, rel=1e-12), a[i]
+    a = np.exp(rng.uniform(np.log(0.01), np.log(19.9), m))
+    frac = rng.uniform(size=m)
+    got = engine.eval(Fn.LP_CL_HARMONIC, np.stack([a, frac], 1), 1)[:, 0]
+    ref = np.array([orc.lib().orc_lp_cl_harmonic(a[i], frac[i]) for i in range(m)])
+    assert (got == ref).mean() > 0.999
+    assert np.all(np.abs(got - ref) <= 1)
+
+
+def test_lp_classical_ceiling_matches_oracle(engine, orc):
+    from ptarmigan_amd import Fn
+    say("lp classical ceiling")
+    rng = np.random.default_rng(22)
+    m = 400
+    a = np.exp(rng.uniform(np.log(0.1), np.log(20.0), m))
+    n = np.maximum(1, rng.uniform(size=m) * (6.0 * (1.0 + a * a) + 2.0 * a ** 3)).astype(int).astype(float)
+    got = engine.eval(Fn.LP_CL_CEILING, np.stack([n, a], 1), 1)[:, 0]
+    ref = np.array([orc.lib().orc_lp_cl_ceiling(int(n[i]), a[i]) for i in range(m)])
+    assert np.allclose(got, ref, rtol=1e-10, atol=0)
+
+
+@pytest.mark.parametrize("a,eta,fixed_n", [(1.0, 0.05, 0), (3.0, 0.05, 0), (8.0, 0.02, 0), (3.0, 0.05, 6)])
+def test_lp_classical_sampler_statistics(engine, orc, a, eta, fixed_n):
+    from scipy.stats import ks_2samp
+    from ptarmigan_amd import Fn
+    say("lp classical sample", a, eta, fixed_n)
+    m = 20_000
+    inp = np.stack([np.full(m, a), np.full(m, eta), np.arange(m, dtype=float), np.full(m, float(fixed_n))], 1)
+    got = engine.eval(Fn.LP_CL_SAMPLE, inp, 6)
+    ref = orc.lp_cl_sample(a, eta, m, seed=23, fixed_n=fixed_n)
+    assert np.all(got[:, 5] >= 1)
+    if fixed_n:
+        assert np.all(got[:, 0] == fixed_n)
+    else:
+        top = int(max(got[:, 0].max(), ref[:, 0].max()))
+        assert _chi2_two_sample(got[:, 0], ref[:, 0], np.unique(np.geomspace(1, top + 1, 60).astype(int)) - 0.5) > 1e-3
+    for col in (1, 2, 3, 4):
+        assert ks_2samp(got[:, col], ref[:, col]).pvalue > 1e-3, col
+
+
+def test_lp_classical_shower_matches_oracle(engine, orc):
+    """LP plane wave, a0 = 2, 16-cycle cos^2, 10 GeV electrons, classical emission
+    with Landau-Lifshitz radiation reaction and no recoil."""
+    from scipy.stats import ks_2samp
+    from ptarmigan_amd import workloads
+    laser, beam, opts = workloads.plane_wave_lma(30_000)
+    opts.classical = True
+    say("lp classical shower")
+    engine.set_laser(laser)
+    dt, z0 = engine.field_info()
+    prim = orc.build_beam(beam.desc(), z0 + 3.0 * beam.length, 9)
+    sg, _, cg = engine.collide(prim, opts, seed=9)
+    sc, _, cc, _ = orc.collide(laser.desc(), opts.desc(), prim, 9, 16)
+    say("  gpu", cg, "\n  cpu", cc)
+    assert cg["error_flags"] == 0
+    a, b = cg["emissions"], cc["emissions"]
+    assert a > 1000 and abs(a - b) < 5.0 * math.sqrt(a + b + 1.0)
+    ph_g, ph_c = sg["species"] == 2, sc["species"] == 2
+    for q in (lambda s, m: s["u"][m, 0], lambda s, m: np.arctan2(s["u"][m, 1], -s["u"][m, 3]),
+              lambda s, m: np.arctan2(s["u"][m, 2], -s["u"][m, 3])):
+        assert ks_2samp(q(sg, ph_g), q(sc, ph_c)).pvalue > 1e-3
+    for col in (1, 2):
+        x, y = sg["pol"][ph_g, col], sc["pol"][ph_c, col]
+        se = math.sqrt(x.var() / len(x) + y.var() / len(y)) + 1e-12
+        assert abs(x.mean() - y.mean()) < 5 * se, col
+    eg, ec = sg["u"][sg["species"] == 0, 0], sc["u"][sc["species"] == 0, 0]
+    assert ks_2samp(eg, ec).pvalue > 1e-3

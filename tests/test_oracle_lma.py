@@ -290,3 +290,94 @@ def test_plane_wave_ll_depletion(orc):
     phase = math.pi**1.5 * n_cycles / math.sqrt(math.log(4.0))
     expected = alpha * a0**4 * omega_mc2 * (1.0 / 8.0) * phase / 3.0
     assert abs(work[0] - expected) / expected < 1e-3
+
+
+# ---------------------------------------------------------------- LP, classical limit (lp/classical.rs)
+@pytest.mark.parametrize("a,eta,target", [(0.1, 1e-3, 3.328348674e-6), (0.3, 1e-3, 2.960584967e-5), (1.0, 1e-3, 2.943962548e-4),
+                                          (3.0, 1e-3, 1.711123935e-3), (5.0, 1e-3, 3.382663412e-3), (8.0, 1e-3, 6.000395217e-3),
+                                          (12.0, 1e-3, 9.562344320e-3), (15.0, 1e-3, 1.225662395e-2)])
+def test_lp_classical_total_rate_known_values(orc, a, eta, target):
+    """lp/classical.rs:546-574 (total_rate_accuracy), error < 1e-3."""
+    assert abs(orc.lp_cl_rate(a, eta) - target) / target < 1e-3
+
+
+@pytest.mark.parametrize("j", [0, 12, 24, 30, 36, 40, 44])
+def test_lp_classical_partial_rates_reproduce_shipped_table(orc, golden_tables, j):
+    """The shipped classical table (lp/classical.rs:636-746) sums n x partial_rate(n, a)
+    over harmonics 1..ceil(6 + 2.5 a^3), the harmonic limit of the reference's
+    earlier sum_limit (lp/mod.rs:416, kept there as a comment; the current limit,
+    6 (1 + a^2) + 2 a^3, adds harmonics the table leaves at zero). The oracle's
+    partial rates reproduce its totals, and every-harmonic CDF rows, to their
+    printed precision."""
+    a = 0.02 * 10.0 ** (j / 20.0)
+    n_max = math.ceil(6.0 + 2.5 * a ** 3)
+    rates = np.cumsum([n * orc.lp_cl_partial_rate(n, a)[0] for n in range(1, n_max + 1)])
+    assert abs(math.log(rates[-1]) - golden_tables["nonlinear_compton/lp/classical/rate_table:TABLE"][j]) < 1e-11
+    if n_max <= 16:
+        cdf = golden_tables["nonlinear_compton/lp/classical/cdf_table:TABLE"][j]
+        assert np.allclose(rates / rates[-1], cdf[:n_max, 1], rtol=0, atol=1e-12)
+
+
+def test_lp_classical_rate_ceiling(orc):
+    """lp/classical.rs:447-478 (rate_ceiling): the first 10 of its 20 seeded draws
+    of a; the ceiling exceeds the largest integrand value met by the partial-rate
+    quadrature."""
+    from test_oracle_cp import _xoshiro_f64
+    rng = _xoshiro_f64(0)
+    L = orc.lib()
+    for _ in range(10):
+        a = math.exp(math.log(0.2) + (math.log(20.0) - math.log(0.2)) * next(rng))
+        n_max = math.ceil(6.0 * (1.0 + a * a) + 2.0 * a ** 3)  # sum_limit(a, 0)
+        if n_max > 200:
+            harmonics = [int(math.exp(math.log(2.0) + 0.1 * i * (math.log(n_max) - math.log(2.0)))) for i in range(11)]
+        elif n_max > 10:
+            harmonics = [1, 2, 3] + [int(math.exp(math.log(5.0) + 0.25 * i * (math.log(n_max) - math.log(5.0)))) for i in range(5)]
+        else:
+            harmonics = list(range(1, n_max))
+        for n in harmonics:
+            _, true_max = orc.lp_cl_partial_rate(n, a)
+            assert L.orc_lp_cl_ceiling(n, a) > true_max, (a, n)
+
+
+@pytest.mark.parametrize("a", [0.474275, 0.946303, 2.37700, 4.74275])
+def test_lp_classical_harmonic_index_sampling(orc, a):
+    """lp/classical.rs:576-634 (harmonic_index_sampling): the harmonic distribution
+    of the CDF inversion against n x partial rate in five bins, error < 5% where the
+    expected fraction is at least 1e-3. The inversion is evaluated on 10^6 evenly
+    spaced fractions instead of the reference's 10^6 random ones, and the total is
+    the tabulated one, to which the CDF table is normalised (the explicit sum over
+    all harmonics agrees with it to 1e-3, test_lp_classical_total_rate_known_values)."""
+    import ctypes
+    L = orc.lib()
+    rates = np.array([n * orc.lp_cl_partial_rate(n, a)[0] for n in range(1, 20)])
+    total = orc.lp_cl_rate(a, 1.0) * (1.0 + 0.5 * a * a) / 2.0
+    frac = (np.arange(1_000_000) + 0.5) / 1_000_000
+    n = np.zeros(len(frac), dtype=np.int32)
+    L.orc_lp_cl_harmonic_batch(a, orc.dptr(frac), len(frac), n.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    for lo, hi in ((1, 1), (2, 2), (3, 4), (5, 9), (10, 19)):
+        e = rates[lo - 1:hi].sum() / total
+        if e < 1e-3:
+            continue
+        c = np.mean((n >= lo) & (n <= hi))
+        assert abs(c - e) / e < 5e-2, (lo, hi, c, e)
+
+
+@pytest.mark.parametrize("a,n", [(1.0, 1), (1.0, 4), (3.0, 10), (8.0, 120)])
+def test_lp_classical_sampler_v_marginal(orc, a, n):
+    """The classical sampler's v = s / s_n at fixed n against the v-marginal of the
+    double-differential rate integrated over its theta range (chi^2)."""
+    from scipy.stats import chisquare
+    L = orc.lib()
+    eta = 0.05
+    out = orc.lp_cl_sample(a, eta, 5000, seed=2, fixed_n=n)
+    sn = 2.0 * n * eta / (1.0 + 0.5 * a * a)
+    v = out[:, 1] / sn
+    bins = np.linspace(0.0, 1.0, 26)
+    h, _ = np.histogram(v, bins)
+    nv = 2000
+    dens = np.zeros(nv)
+    L.orc_lp_cl_v_marginal(n, a, nv, 400, orc.dptr(dens))
+    p, _ = np.histogram((np.arange(nv) + 0.5) / nv, bins, weights=dens)
+    p = p / p.sum()
+    keep = p * len(v) > 5
+    assert chisquare(h[keep], p[keep] / p[keep].sum() * h[keep].sum()).pvalue > 1e-3

@@ -255,7 +255,12 @@ typedef enum {
     PTG_FN_LP_CL_HARMONIC = 21,  /* (a, frac) -> n  lp/classical.rs:215-252 */
     PTG_FN_LP_CL_CEILING = 22,   /* (n, a) -> rejection ceiling  lp/classical.rs:43-110 */
     PTG_FN_LP_CL_SAMPLE = 23,    /* (a, eta, stream, fixed_n) -> (n, s, theta, S1, S2, trials)  lp/classical.rs:254-333 */
-    PTG_FN_COUNT = 24
+    PTG_FN_CPBW_RATE = 24,       /* (a, eta) -> (f(S3 = +1), f(S3 = -1), method: 0 zero, 1 summation, 2 table, -1 outside)
+                                    pair_creation/cp/mod.rs:271-281 */
+    PTG_FN_CPBW_PARTIAL_RATE = 25, /* (n, a, eta) -> (S3 = +1, S3 = -1)  pair_creation/cp/mod.rs:98-184 */
+    PTG_FN_CPBW_HARMONIC = 26,   /* (a, eta, S3, frac) -> (n, fell_back)  pair_creation/cp/mod.rs:327-365 */
+    PTG_FN_CPBW_SAMPLE = 27,     /* (a, eta, S1, S2, S3, stream, fixed_n) -> (n, s, theta, trials)  pair_creation/cp/mod.rs:188-245 */
+    PTG_FN_COUNT = 28
 } ptg_eval_fn;
 int ptg_eval(ptg_ctx* ctx, int fn, const double* in, size_t n, size_t in_w, double* out, size_t out_w);
 

@@ -446,6 +446,194 @@ double cp_cdf_row(double a, double eta, bool classical, double out64[64]) {
     return rate;
 }
 
+// ---------------------------------------------------------------- CP nonlinear Breit-Wheeler (pair_creation/cp/mod.rs)
+static const char* CBT = "pair_creation/cp/total:";
+
+// pair_creation/cp/mod.rs:35-40
+static void cpbw_s_bounds(int n, double a, double eta, double* s_min, double* s_max) {
+    double sn = 2.0 * double(n) * eta / (1.0 + powi(a, 2));
+    *s_min = std::fmin(0.5 * (1.0 - std::sqrt(1.0 - 4.0 / sn)), 0.5);
+    *s_max = 0.5;
+}
+// pair_creation/cp/mod.rs:45-66: unpolarised part, the S_1/S_2 and the S_3 coefficients
+static void cpbw_elements(int n, double a, double eta, double s, double out3[3]) {
+    double ell = double(n);
+    double sn = 2.0 * ell * eta / (1.0 + powi(a, 2));
+    double tmp = 1.0 / (sn * s * (1.0 - s));
+    double z = std::sqrt((4.0 * ell * ell) * (powi(a, 2) / (1.0 + powi(a, 2))) * tmp * (1.0 - tmp));
+    double j[3];
+    bessel_j_pm(n, z, j);
+    double jm = j[0], jn = j[1], jp = j[2];
+    out3[0] = powi(jn, 2) + 0.25 * powi(a, 2) * ((1.0 - s) / s + s / (1.0 - s)) * (powi(jm, 2) + powi(jp, 2) - 2.0 * powi(jn, 2));
+    out3[1] = (2.0 * powi(ell * a / z, 2) - (1.0 + powi(a, 2))) * powi(jn, 2) - 0.5 * powi(a, 2) * (powi(jm, 2) + powi(jp, 2));
+    out3[2] = 0.25 * powi(a, 2) * ((1.0 - s) / s + s / (1.0 - s)) * (1.0 - 2.0 / (sn * s * (1.0 - s))) * (powi(jm, 2) - powi(jp, 2));
+}
+// pair_creation/cp/mod.rs:71-78. The reference binds (sin 2theta, cos 2theta) from
+// sin_cos() to names in the opposite order; the arithmetic is kept as written.
+static double cpbw_fully_resolved(int n, double a, double eta, double s, double theta, double sv1, double sv2, double sv3) {
+    double e[3];
+    cpbw_elements(n, a, eta, s, e);
+    double first = std::sin(2.0 * theta), second = std::cos(2.0 * theta);
+    return (e[0] - (sv1 * first + sv2 * second) * e[1] - LASER_S3 * sv3 * e[2]) / (2.0 * PI);
+}
+// pair_creation/cp/mod.rs:80-93: (S_3 = +1, S_3 = -1) integrated over theta
+static C cpbw_over_theta(int n, double a, double eta, double s) {
+    double e[3];
+    cpbw_elements(n, a, eta, s, e);
+    return C{e[0] - LASER_S3 * e[2], e[0] + LASER_S3 * e[2]};
+}
+static double cpbw_over_theta_s3(int n, double a, double eta, double s, double sv3) {
+    C sp = cpbw_over_theta(n, a, eta, s);
+    return 0.5 * (sp.re * (1.0 + sv3) + sp.im * (1.0 - sv3));
+}
+// pair_creation/cp/mod.rs:98-184: partial rate of harmonic n, (S_3 = +1, S_3 = -1)
+C cpbw_partial_rate(int n, double a, double eta) {
+    Table N = table("quadrature/tables:GAUSS_32_NODES"), W = table("quadrature/tables:GAUSS_32_WEIGHTS");
+    double ell = double(n);
+    double sn = 2.0 * ell * eta / (1.0 + a * a);
+    if (sn <= 4.0) return C{0.0, 0.0};
+    double beta = powi(a, 4) * powi(1.0 / ell + 1.0, 2) + 16.0 * powi(a * a - 2.0, 2) / (sn * sn) - 8.0 * a * a * (a * a - 2.0) / sn;
+    beta = std::sqrt(beta) / (a * a - 2.0);
+    double alpha = (a * a + 2.0 * ell) / (ell * (2.0 - a * a)) - 4.0 / sn;
+    double s_peak = 0.5 * (1.0 - std::sqrt(alpha + beta));
+    double s_min, s_max;
+    cpbw_s_bounds(n, a, eta, &s_min, &s_max);
+    auto seg = [&](double lo, double hi) {
+        C acc{0.0, 0.0};
+        for (int i = 0; i < 32; ++i) {
+            double s = 0.5 * (hi - lo) * N[i] + 0.5 * (lo + hi);
+            acc = acc + (0.5 * (hi - lo) * W[i]) * cpbw_over_theta(n, a, eta, s);
+        }
+        return acc;
+    };
+    if (std::isfinite(s_peak)) {
+        double s_mid = 2.0 * s_peak - s_min;
+        if (s_mid > s_max) {
+            C lower = seg(s_min, s_peak), upper = seg(s_peak, s_max);
+            return 2.0 * (upper + lower);
+        }
+        C lower = seg(s_min, s_peak), middle = seg(s_peak, s_mid), upper = seg(s_mid, s_max);
+        return 2.0 * (lower + middle + upper);
+    }
+    return 2.0 * seg(s_min, s_max);
+}
+// pair_creation/cp/mod.rs:188-245: (s, theta) at harmonic n
+static void cpbw_sample_at(int n, double a, double eta, double sv1, double sv2, double sv3, Xoshiro& rng, double* s_out,
+                           double* theta_out) {
+    Table N = table("quadrature/tables:GAUSS_32_NODES");
+    double s_min, s_max;
+    cpbw_s_bounds(n, a, eta, &s_min, &s_max);
+    double mx = NAN;
+    for (int i = 0; i < 32; ++i) mx = std::fmax(mx, cpbw_over_theta_s3(n, a, eta, 0.5 * (s_max - s_min) * N[i] + 0.5 * (s_min + s_max), sv3));
+    mx = 1.5 * mx;
+    double s;
+    for (;;) {
+        s = s_min + (s_max - s_min) * rng.gen_f64();
+        double z = mx * rng.gen_f64();
+        if (z < cpbw_over_theta_s3(n, a, eta, s, sv3)) break;
+    }
+    uint32_t r32 = uint32_t(rng.next_u64() >> 32);  // rand 0.7 gen_range(0, 2)
+    if (int((uint64_t(r32) * 2u) >> 32) == 0) s = 1.0 - s;
+    double theta_opt = 0.5 * std::atan2(sv2, sv1);
+    double mt = 0.0;
+    for (int i = 0; i < 4; ++i) {
+        double f = cpbw_fully_resolved(n, a, eta, s, theta_opt + double(i) * FRAC_PI_2, sv1, sv2, sv3);
+        mt = i == 0 ? f : std::fmax(mt, f);
+    }
+    mt = 1.01 * mt;
+    double theta;
+    for (;;) {
+        theta = 2.0 * PI * rng.gen_f64();
+        double z = mt * rng.gen_f64();
+        if (z < cpbw_fully_resolved(n, a, eta, s, theta, sv1, sv2, sv3)) break;
+    }
+    *s_out = s;
+    *theta_out = theta;
+}
+// pair_creation/cp/mod.rs:368-381
+static void cpbw_sum_limits(double a, double eta, int* lo, int* hi) {
+    double n_min = std::ceil(2.0 * (1.0 + a * a) / eta);
+    double range = 2.0 * (0.25 + a * a) / eta + 13.0 * a * (2.0 + a);
+    double test = 0.25 - (1.0 + a * a) / (2.0 * n_min * eta);
+    if (test <= 2.220446049250313e-16) { *lo = int(n_min) + 1; *hi = int(n_min + 1.0 + range); }
+    else { *lo = int(n_min); *hi = int(n_min + range); }
+}
+static bool cpbw_is_too_small(double a, double eta) { return std::log10(eta) < -1.0 - powi(std::log10(a) + 2.0, 2) / 4.5; }
+// pair_creation/cp/mod.rs:398-404
+static void cpbw_by_summation(double a, double eta, double out2[2]) {
+    int lo, hi;
+    cpbw_sum_limits(a, eta, &lo, &hi);
+    C total{0.0, 0.0};
+    for (int n = lo; n <= hi; ++n) total = total + cpbw_partial_rate(n, a, eta);
+    out2[0] = total.re;
+    out2[1] = total.im;
+}
+// pair_creation/cp/mod.rs:406-449
+static void cpbw_by_lookup(double a, double eta, double out2[2]) {
+    const size_t N_COLS = 74, N_ROWS = 121;
+    double LMA_ = cpt(CBT, "LN_MIN_A")[0], LME = cpt(CBT, "LN_MIN_ETA")[0], LAS = cpt(CBT, "LN_A_STEP")[0], LES = cpt(CBT, "LN_ETA_STEP")[0];
+    Table T = cpt(CBT, "TABLE");
+    double x = std::log(a), y = std::log(eta);
+    if (x < LMA_) { std::fprintf(stderr, "oracle: NBW [CP] rate lookup out of bounds\n"); std::abort(); }
+    if (y < LME) { out2[0] = out2[1] = 0.0; return; }
+    size_t ix = size_t((x - LMA_) / LAS), iy = size_t((y - LME) / LES);
+    if (!(ix < N_COLS - 1 && iy < N_ROWS - 1)) { std::fprintf(stderr, "oracle: NBW [CP] rate lookup out of bounds\n"); std::abort(); }
+    auto at = [&](size_t r, size_t c, int k) { return T[(r * N_COLS + c) * 2 + k]; };
+    if (a * eta > 2.0) {
+        double dx = (x - LMA_) / LAS - double(ix), dy = (y - LME) / LES - double(iy);
+        double w[4] = {(1.0 - dx) * (1.0 - dy), dx * (1.0 - dy), (1.0 - dx) * dy, dx * dy};
+        for (int k = 0; k < 2; ++k)
+            out2[k] = std::exp(w[0] * at(iy, ix, k) + w[1] * at(iy, ix + 1, k) + w[2] * at(iy + 1, ix, k) + w[3] * at(iy + 1, ix + 1, k));
+    } else {
+        double a_min = std::exp(LMA_ + double(ix) * LAS), a_max = std::exp(LMA_ + double(ix + 1) * LAS);
+        double e_min = std::exp(LME + double(iy) * LES), e_max = std::exp(LME + double(iy + 1) * LES);
+        double dx = (a - a_min) / (a_max - a_min), dy = (eta - e_min) / (e_max - e_min);
+        double w[4] = {(1.0 - dx) * (1.0 - dy), dx * (1.0 - dy), (1.0 - dx) * dy, dx * dy};
+        for (int k = 0; k < 2; ++k) {
+            double f = w[0] / at(iy, ix, k) + w[1] / at(iy, ix + 1, k) + w[2] / at(iy + 1, ix, k) + w[3] / at(iy + 1, ix + 1, k);
+            out2[k] = std::exp(1.0 / f);
+        }
+    }
+}
+// (S_3 = +1, S_3 = -1) total rates (pair_creation/cp/mod.rs:271-281)
+static void cpbw_total(double a, double eta, double f[2]) {
+    if (a < 0.02 || cpbw_is_too_small(a, eta)) { f[0] = f[1] = 0.0; return; }
+    if (a < std::exp(cpt(CBT, "LN_MIN_A")[0])) cpbw_by_summation(a, eta, f);
+    else cpbw_by_lookup(a, eta, f);
+}
+// pair_creation/cp/mod.rs:286-322
+double cpbw_probability(double a, double eta, SV sv, double dphi, SV* out) {
+    double f[2];
+    cpbw_total(a, eta, f);
+    double prob = ALPHA_FINE * (0.5 * ((f[0] + f[1]) + sv[3] * (f[0] - f[1]))) * dphi / eta;
+    double prob_avg = ALPHA_FINE * (0.5 * (f[0] + f[1])) * dphi / eta;
+    double delta = ALPHA_FINE * (0.5 * (f[0] - f[1])) * dphi / eta;
+    *out = SV{sv[0], sv[1] * (1.0 - prob_avg) / (1.0 - prob), sv[2] * (1.0 - prob_avg) / (1.0 - prob),
+              (sv[3] * (1.0 - prob_avg) - delta) / (1.0 - prob)};
+    return prob;
+}
+// pair_creation/cp/mod.rs:327-365: harmonic by explicit summation of partial rates
+int cpbw_harmonic(double a, double eta, double sv3, double frac, bool* fell_back) {
+    int lo, hi;
+    cpbw_sum_limits(a, eta, &lo, &hi);
+    double f[2];
+    cpbw_total(a, eta, f);
+    double target = 0.5 * (f[0] * (1.0 + sv3) + f[1] * (1.0 - sv3)) * frac;
+    double cumsum = 0.0;
+    for (int k = lo; k <= hi; ++k) {
+        C pr = cpbw_partial_rate(k, a, eta);
+        cumsum += 0.5 * ((1.0 + sv3) * pr.re + (1.0 - sv3) * pr.im);
+        if (cumsum > target) { if (fell_back) *fell_back = false; return k; }
+    }
+    if (fell_back) *fell_back = true;
+    return hi - 1;
+}
+void cpbw_sample(double a, double eta, double sv1, double sv2, double sv3, Xoshiro& rng, int* n_out, double* s_out, double* theta_out) {
+    int n = cpbw_harmonic(a, eta, sv3, rng.gen_f64(), nullptr);
+    cpbw_sample_at(n, a, eta, sv1, sv2, sv3, rng, s_out, theta_out);
+    *n_out = n;
+}
+
 }  // namespace orc
 
 // ---- probes
@@ -499,6 +687,38 @@ void orc_cp_sample(double a, double eta, int classical, uint64_t seed, int fixed
         else cp_sample(a, eta, rng, fixed_n, &n, &s, &th, &sv, &tr);
         double* o = out7 + 7 * i;
         o[0] = n; o[1] = s; o[2] = th; o[3] = sv[1]; o[4] = sv[2]; o[5] = sv[3]; o[6] = double(tr);
+    }
+}
+}
+
+extern "C" {
+void orc_cpbw_partial_rate(int n, double a, double eta, double* out2) { C r = cpbw_partial_rate(n, a, eta); out2[0] = r.re; out2[1] = r.im; }
+double orc_cpbw_fully_resolved(int n, double a, double eta, double s, double theta, double sv1, double sv2, double sv3) {
+    return cpbw_fully_resolved(n, a, eta, s, theta, sv1, sv2, sv3);
+}
+int orc_cpbw_sum_limits(double a, double eta, int* hi) { int lo; cpbw_sum_limits(a, eta, &lo, hi); return lo; }
+// 0: zero (a < 0.02 or negligible), 1: by summation, 2: by lookup
+int orc_cpbw_total(double a, double eta, double* out2) {
+    cpbw_total(a, eta, out2);
+    if (a < 0.02 || cpbw_is_too_small(a, eta)) return 0;
+    return a < std::exp(cpt(CBT, "LN_MIN_A")[0]) ? 1 : 2;
+}
+void orc_cpbw_by_summation(double a, double eta, double* out2) { cpbw_by_summation(a, eta, out2); }
+double orc_cpbw_probability(double a, double eta, const double* sv4, double dphi, double* sv_out4) {
+    SV o;
+    double p = cpbw_probability(a, eta, SV{sv4[0], sv4[1], sv4[2], sv4[3]}, dphi, &o);
+    for (int k = 0; k < 4; ++k) sv_out4[k] = o[k];
+    return p;
+}
+int orc_cpbw_harmonic(double a, double eta, double sv3, double frac) { return cpbw_harmonic(a, eta, sv3, frac, nullptr); }
+void orc_cpbw_sample(double a, double eta, double sv1, double sv2, double sv3, uint64_t seed, int fixed_n, size_t count, double* out3) {
+    Xoshiro rng(seed);
+    for (size_t i = 0; i < count; ++i) {
+        int n;
+        double s, th;
+        if (fixed_n > 0) { n = fixed_n; cpbw_sample_at(n, a, eta, sv1, sv2, sv3, rng, &s, &th); }
+        else cpbw_sample(a, eta, sv1, sv2, sv3, rng, &n, &s, &th);
+        out3[3 * i] = n; out3[3 * i + 1] = s; out3[3 * i + 2] = th;
     }
 }
 }

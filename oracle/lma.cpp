@@ -759,46 +759,105 @@ static double pw_grad(const Laser& L, V4 r) {
 }
 static V4 pw_grad_a_sqd(const Laser& L, V4 r) { double g = pw_grad(L, r); return {-g, 0.0, 0.0, -g}; }
 
-// plane_wave.rs:152-175
-static V4 pw_ll_force(const Laser& L, V4 r, V4 u) {
-    double eta = SPEED_OF_LIGHT * COMPTON_TIME * (L.wavevector * u);
-    return -2.0 * ALPHA_FINE * pw_a_sqd(L, r) * powi(eta, 2) * u / (3.0 * COMPTON_TIME);
+// ---------------------------------------------------------------- FocusedLaser (focused_laser.rs)
+// n_cycles recomputed from the duration (focused_laser.rs:51-53)
+static double fl_n_cycles(const Laser& L) { return SPEED_OF_LIGHT * L.duration * L.wavevector[0] / (2.0 * PI); }
+static double fl_rayleigh(const Laser& L) { return 0.5 * L.wavevector[0] * powi(L.waist, 2); }
+// focused_laser.rs:72-101
+static void fl_envelope_and_grad(const Laser& L, double phase, double* env, double* grad) {
+    double nc = fl_n_cycles(L);
+    switch (L.env) {
+    case Env::CosSquared:
+        if (std::fabs(phase) < PI * nc) {
+            *env = powi(std::cos(phase / (2.0 * nc)), 4);
+            *grad = 2.0 * L.wavevector[0] * std::tan(phase / (2.0 * nc)) * *env / nc;
+        } else {
+            *env = 0.0; *grad = 0.0;
+        }
+        return;
+    case Env::Flattop:
+        if (std::fabs(phase) > PI * (nc + 1.0)) { *env = 0.0; *grad = 0.0; }
+        else if (std::fabs(phase) > PI * (nc - 1.0)) {
+            double arg = 0.25 * (std::fabs(phase) - (nc - 1.0) * PI);
+            *env = powi(std::cos(arg), 4);
+            *grad = L.wavevector[0] * signum(phase) * std::sin(arg) * powi(std::cos(arg), 3);
+        } else { *env = 1.0; *grad = 0.0; }
+        return;
+    default: {
+        double tau = SPEED_OF_LIGHT * L.wavevector[0] * L.duration;
+        *env = std::exp(-4.0 * LN_2 * powi(phase, 2) / powi(tau, 2));
+        *grad = 8.0 * LN_2 * L.wavevector[0] * phase * *env / powi(tau, 2);
+    }
+    }
 }
-static double pw_ll_work(const Laser& L, V4 r, V4 u) {
+// focused_laser.rs:103-117
+double fl_a_sqd(const Laser& L, V4 r) {
+    double z_r = fl_rayleigh(L);
+    double width_sqd = 1.0 + powi(r[3] / z_r, 2);
+    double rho_sqd = (powi(r[1], 2) + powi(r[2], 2)) / powi(L.waist, 2);
+    double beam = pw_norm(L) * powi(L.a0, 2) * std::exp(-2.0 * rho_sqd / width_sqd) / width_sqd;
+    double env, grad;
+    fl_envelope_and_grad(L, L.wavevector * r, &env, &grad);
+    return beam * env;
+}
+// focused_laser.rs:119-146
+static V4 fl_grad_a_sqd(const Laser& L, V4 r) {
+    double z_r = fl_rayleigh(L);
+    double width_sqd = 1.0 + powi(r[3] / z_r, 2);
+    double rho_sqd = (powi(r[1], 2) + powi(r[2], 2)) / powi(L.waist, 2);
+    double beam = pw_norm(L) * powi(L.a0, 2) * std::exp(-2.0 * rho_sqd / width_sqd) / width_sqd;
+    double gb0 = -4.0 * beam * r[1] / (powi(L.waist, 2) * width_sqd);
+    double gb1 = -4.0 * beam * r[2] / (powi(L.waist, 2) * width_sqd);
+    double gb2 = (2.0 * beam * r[3] / (powi(z_r, 2) * width_sqd)) * (2.0 * rho_sqd / width_sqd - 1.0);
+    double env, ge;
+    fl_envelope_and_grad(L, L.wavevector * r, &env, &ge);
+    double tz = beam * ge + gb2 * env;
+    return V4{-tz, -(gb0 * env), -(gb1 * env), -tz};
+}
+// the LMA fields' <a^2> and its four-gradient, by kind
+static double lma_a_sqd(const Laser& L, V4 r) { return L.kind == Kind::FocusedLaser ? fl_a_sqd(L, r) : pw_a_sqd(L, r); }
+static V4 lma_grad_a_sqd(const Laser& L, V4 r) { return L.kind == Kind::FocusedLaser ? fl_grad_a_sqd(L, r) : pw_grad_a_sqd(L, r); }
+static V4 lma_ll_force(const Laser& L, V4 r, V4 u) {  // plane_wave.rs:152-160, focused_laser.rs:148-153
+    double eta = SPEED_OF_LIGHT * COMPTON_TIME * (L.wavevector * u);
+    return -2.0 * ALPHA_FINE * lma_a_sqd(L, r) * powi(eta, 2) * u / (3.0 * COMPTON_TIME);
+}
+static double lma_ll_work(const Laser& L, V4 r, V4 u) {  // plane_wave.rs:162-175, focused_laser.rs:155-163
     double eta = SPEED_OF_LIGHT * COMPTON_TIME * (L.wavevector * u);
     double omega = SPEED_OF_LIGHT * L.wavevector[0];
     double delta = L.pol == Pol::Circular ? 1.0 : 0.75;
-    return 2.0 * ALPHA_FINE * omega * eta * delta * powi(pw_a_sqd(L, r), 2) / 3.0;
+    return 2.0 * ALPHA_FINE * omega * eta * delta * powi(lma_a_sqd(L, r), 2) / 3.0;
 }
 
-static double pw_bandwidth(const Laser& L) {  // plane_wave.rs:49-63
+static double pw_bandwidth(const Laser& L) {  // plane_wave.rs:49-63, focused_laser.rs:45-58
     if (!L.finite_bandwidth) return 0.0;
-    double n_fwhm = L.env == Env::CosSquared ? 0.36405666377387671305 * L.n_cycles : L.n_cycles;
+    double nc = L.kind == Kind::FocusedLaser ? fl_n_cycles(L) : L.n_cycles;
+    double n_fwhm = L.env == Env::CosSquared ? 0.36405666377387671305 * nc : nc;
     return std::sqrt(0.5 * LN_2) / (PI * n_fwhm);
 }
 
-double lma_max_timestep(const Laser& L) {  // plane_wave.rs:176-182
-    if (L.kind != Kind::PlaneWave) { std::fprintf(stderr, "oracle: FocusedLaser not restated\n"); std::abort(); }
+double lma_max_timestep(const Laser& L) {  // plane_wave.rs:176-182, focused_laser.rs:167-173
     return L.env == Env::Flattop ? 0.2 / (SPEED_OF_LIGHT * L.wavevector[0]) : 1.0 / (SPEED_OF_LIGHT * L.wavevector[0]);
 }
-bool lma_contains(const Laser& L, V4 r) {  // plane_wave.rs:184-192
+bool lma_contains(const Laser& L, V4 r) {  // plane_wave.rs:184-192, focused_laser.rs:175-183
+    double nc = L.kind == Kind::FocusedLaser ? fl_n_cycles(L) : L.n_cycles;
     double phase = L.wavevector * r;
-    double mp = L.env == Env::CosSquared ? PI * L.n_cycles : (L.env == Env::Flattop ? PI * (L.n_cycles + 1.0) : 6.0 * PI * L.n_cycles);
+    double mp = L.env == Env::CosSquared ? PI * nc : (L.env == Env::Flattop ? PI * (nc + 1.0) : 6.0 * PI * nc);
     return phase < mp;
 }
-double lma_ideal_initial_z(const Laser& L) {  // plane_wave.rs:318-325
+double lma_ideal_initial_z(const Laser& L) {  // plane_wave.rs:318-325, focused_laser.rs:274-281
+    double nc = L.kind == Kind::FocusedLaser ? fl_n_cycles(L) : L.n_cycles;
     double wl = 2.0 * PI / L.wavevector[0];
-    return L.env == Env::CosSquared ? 0.5 * wl * L.n_cycles : (L.env == Env::Flattop ? 0.5 * wl * (L.n_cycles + 1.0) : 2.0 * wl * L.n_cycles);
+    return L.env == Env::CosSquared ? 0.5 * wl * nc : (L.env == Env::Flattop ? 0.5 * wl * (nc + 1.0) : 2.0 * wl * nc);
 }
 
-// plane_wave.rs:196-249
+// plane_wave.rs:196-249, focused_laser.rs:185-225
 PushOut lma_push(const Laser& L, V4 r, V4 u, double rqm, double dt, Eqn eqn) {
     double dtau = dt / u[0];
     double scale = powi(rqm / (ELECTRON_CHARGE / ELECTRON_MASS), 2);
     V4 u_mid{0, 0, 0, 0};
     if (eqn == Eqn::LandauLifshitz) {
-        V4 f = 0.5 * SPEED_OF_LIGHT * scale * pw_grad_a_sqd(L, r);
-        V4 g = powi(scale, 2) * pw_ll_force(L, r, u);
+        V4 f = 0.5 * SPEED_OF_LIGHT * scale * lma_grad_a_sqd(L, r);
+        V4 g = powi(scale, 2) * lma_ll_force(L, r, u);
         u_mid = u + 0.5 * (f + g) * dtau;
     } else if (eqn == Eqn::ModifiedLandauLifshitz) {
         std::fprintf(stderr, "oracle: Gaunt factor correction is unavailable in LMA mode\n");
@@ -806,23 +865,25 @@ PushOut lma_push(const Laser& L, V4 r, V4 u, double rqm, double dt, Eqn eqn) {
     }
     r = r + 0.5 * SPEED_OF_LIGHT * u * dtau;
     double dt_actual = 0.5 * u[0] * dtau;
-    V4 f = 0.5 * SPEED_OF_LIGHT * scale * pw_grad_a_sqd(L, r);
-    V4 g = eqn == Eqn::LandauLifshitz ? powi(scale, 2) * pw_ll_force(L, r, u_mid) : V4{0, 0, 0, 0};
+    V4 f = 0.5 * SPEED_OF_LIGHT * scale * lma_grad_a_sqd(L, r);
+    V4 g = eqn == Eqn::LandauLifshitz ? powi(scale, 2) * lma_ll_force(L, r, u_mid) : V4{0, 0, 0, 0};
     u = u + (f + g) * dtau;
-    double dwork = eqn == Eqn::LandauLifshitz ? (f[0] + pw_ll_work(L, r, u_mid)) * dtau : f[0] * dtau;
+    double dwork = eqn == Eqn::LandauLifshitz ? (f[0] + lma_ll_work(L, r, u_mid)) * dtau : f[0] * dtau;
     r = r + 0.5 * SPEED_OF_LIGHT * u * dtau;
     dt_actual = dt_actual + 0.5 * u[0] * dtau;
-    u = with_sqr(u, 1.0 + pw_a_sqd(L, r));
+    u = with_sqr(u, 1.0 + lma_a_sqd(L, r));
     return {r, u, dt_actual, dwork};
 }
 
 // plane_wave.rs:251-282
 bool lma_radiate(const Laser& L, V4 r, V4 u, double dt, Xoshiro& rng, bool classical, RadEvent* ev) {
-    double a = std::sqrt(pw_a_sqd(L, r));
+    // plane_wave.rs:251-282; focused_laser.rs:227-247 (no chirp)
+    double a = std::sqrt(lma_a_sqd(L, r));
     double phase = L.wavevector * r;
-    double chirp = 1.0 + 2.0 * L.chirp_b * phase;
+    double chirp = L.kind == Kind::FocusedLaser ? 1.0 : 1.0 + 2.0 * L.chirp_b * phase;
     double width = 1.0 + pw_bandwidth(L) * rng.normal();
-    V4 kappa = SPEED_OF_LIGHT * COMPTON_TIME * L.wavevector * chirp * width;
+    V4 kappa = L.kind == Kind::FocusedLaser ? SPEED_OF_LIGHT * COMPTON_TIME * L.wavevector * width
+                                            : SPEED_OF_LIGHT * COMPTON_TIME * L.wavevector * chirp * width;
     double prob = 0.0;
     if (!nlc_probability(kappa, u, dt, L.pol, classical, &prob)) prob = 0.0;
     if (!(rng.gen_f64() < prob)) return false;
@@ -1435,24 +1496,25 @@ static void bw_sample(double a, double eta, double sv1, double sv2, Xoshiro& rng
 }
 
 // pair_creation/mod.rs:21-41 (linear polarisation)
-static double nbw_probability(V4 ell, SV sv, V4 k, double a, double dt, double pol_angle, SV* sv_out) {
+static double nbw_probability(V4 ell, SV sv, V4 k, double a, double dt, Pol pol, double pol_angle, SV* sv_out) {
     double eta = k * ell;
     SV s = in_lma_basis(sv, ell);
     s = rotate_by(s, pol_angle);
     double dphi = eta * dt / (COMPTON_TIME * ell[0]);
     SV s2;
-    double prob = bw_probability(a * SQRT_2, eta, s, dphi, &s2);
+    double prob = pol == Pol::Circular ? cpbw_probability(a, eta, s, dphi, &s2) : bw_probability(a * SQRT_2, eta, s, dphi, &s2);
     s2 = rotate_by(s2, -pol_angle);
     *sv_out = from_lma_basis(s2, ell);
     return prob;
 }
 // pair_creation/mod.rs:47-107 (linear polarisation)
-static V4 nbw_generate(V4 ell, SV sv, V4 k, double a, double pol_angle, Xoshiro& rng, int* n_out) {
+static V4 nbw_generate(V4 ell, SV sv, V4 k, double a, Pol pol, double pol_angle, Xoshiro& rng, int* n_out) {
     double eta = k * ell;
     SV s = rotate_by(in_lma_basis(sv, ell), pol_angle);
     int n;
     double sfrac, cphi;
-    bw_sample(a * SQRT_2, eta, s[1], s[2], rng, &n, &sfrac, &cphi, nullptr);
+    if (pol == Pol::Circular) cpbw_sample(a, eta, s[1], s[2], s[3], rng, &n, &sfrac, &cphi);
+    else bw_sample(a * SQRT_2, eta, s[1], s[2], rng, &n, &sfrac, &cphi, nullptr);
     double j = double(n);
     double e_zmf = std::sqrt(0.5 * j * eta);
     double p_zmf = std::sqrt(0.5 * j * eta - 1.0 - a * a);
@@ -1477,18 +1539,18 @@ static V4 nbw_generate(V4 ell, SV sv, V4 k, double a, double pol_angle, Xoshiro&
 // plane_wave.rs:284-316
 double lma_pair_create(const Laser& L, V4 r, V4 ell, SV pol, double dt, Xoshiro& rng, double rate_increase, SV* pol_new,
                        bool* happened, PairEvent* ev) {
-    if (L.kind != Kind::PlaneWave) { std::fprintf(stderr, "oracle: FocusedLaser not restated\n"); std::abort(); }
-    if (L.pol != Pol::Linear) { std::fprintf(stderr, "oracle: circular LMA pair creation not restated\n"); std::abort(); }
-    double a = std::sqrt(pw_a_sqd(L, r));
+    // plane_wave.rs:284-316; focused_laser.rs:249-272 (no chirp)
+    double a = std::sqrt(lma_a_sqd(L, r));
     double phase = L.wavevector * r;
     double chirp = 1.0 + 2.0 * L.chirp_b * phase;
-    V4 kappa = SPEED_OF_LIGHT * COMPTON_TIME * L.wavevector * chirp;
-    double prob = nbw_probability(ell, pol, kappa, a, dt, L.pol_angle, pol_new);
+    V4 kappa = L.kind == Kind::FocusedLaser ? SPEED_OF_LIGHT * COMPTON_TIME * L.wavevector
+                                            : SPEED_OF_LIGHT * COMPTON_TIME * L.wavevector * chirp;
+    double prob = nbw_probability(ell, pol, kappa, a, dt, L.pol, L.pol_angle, pol_new);
     double ri = prob * rate_increase > 0.1 ? 0.1 / prob : rate_increase;
     *happened = false;
     if (rng.gen_f64() < prob * ri) {
         int n;
-        V4 q_p = nbw_generate(ell, pol, kappa, a, L.pol_angle, rng, &n);
+        V4 q_p = nbw_generate(ell, pol, kappa, a, L.pol, L.pol_angle, rng, &n);
         *ev = PairEvent{ell + double(n) * kappa - q_p, q_p, 1.0 / ri, a, a * (ell * kappa), double(n) * kappa[0]};
         *happened = true;
     }

@@ -220,6 +220,9 @@ void cp_sample(double a, double eta, Xoshiro& rng, int fixed_n, int* n, double* 
 void cp_classical_sample(double a, double eta, Xoshiro& rng, int fixed_n, int* n, double* s, double* theta, SV* sv,
                          uint64_t* trials);  // cp/classical.rs:128-178
 
+double cpbw_probability(double a, double eta, SV sv, double dphi, SV* out);  // pair_creation/cp/mod.rs:286-322
+void cpbw_sample(double a, double eta, double sv1, double sv2, double sv3, Xoshiro& rng, int* n, double* s, double* theta);  // :327-365
+
 // ---------------------------------------------------------------- fields
 enum class Kind : int { PlaneWave = 0, FocusedLaser = 1, FastPlaneWave = 2, FastFocusedLaser = 3 };
 enum class Env : int { CosSquared = 0, Flattop = 1, Gaussian = 2 };

@@ -160,6 +160,16 @@ def lib():
         L.orc_lp_cl_harmonic_batch.argtypes = [d, P(d), C.c_size_t, P(C.c_int)]
         L.orc_lp_cl_v_marginal.argtypes = [i, d, i, i, P(d)]
         L.orc_lp_cl_sample.argtypes = [d, d, C.c_uint64, i, C.c_size_t, P(d)]
+        L.orc_cpbw_partial_rate.argtypes = [i, d, d, P(d)]
+        L.orc_cpbw_fully_resolved.argtypes = [i, d, d, d, d, d, d, d]
+        L.orc_cpbw_fully_resolved.restype = d
+        L.orc_cpbw_sum_limits.argtypes = [d, d, P(i)]
+        L.orc_cpbw_total.argtypes = [d, d, P(d)]
+        L.orc_cpbw_by_summation.argtypes = [d, d, P(d)]
+        L.orc_cpbw_probability.argtypes = [d, d, P(d), d, P(d)]
+        L.orc_cpbw_probability.restype = d
+        L.orc_cpbw_harmonic.argtypes = [d, d, d, d]
+        L.orc_cpbw_sample.argtypes = [d, d, d, d, d, C.c_uint64, i, C.c_size_t, P(d)]
         L.orc_cp_grid_max.argtypes = [i, d, d, i]
         L.orc_cp_grid_max.restype = d
         L.orc_cp_rate.argtypes = [d, d, P(d)]
@@ -389,4 +399,30 @@ def lp_cl_sample(a, eta, count, seed=0, fixed_n=0):
     """(n, s, theta, S1, S2) per sample from one seeded stream."""
     out = np.zeros((count, 5))
     lib().orc_lp_cl_sample(a, eta, seed, fixed_n, count, dptr(out))
+    return out
+
+
+# ---- CP nonlinear Breit-Wheeler (oracle/cp.cpp)
+def cpbw_partial_rate(n, a, eta):
+    out = np.zeros(2)
+    lib().orc_cpbw_partial_rate(n, a, eta, dptr(out))
+    return out
+
+
+def cpbw_total(a, eta):
+    """(method: 0 zero, 1 summation, 2 lookup, (S3 = +1, S3 = -1) totals)."""
+    out = np.zeros(2)
+    k = lib().orc_cpbw_total(a, eta, dptr(out))
+    return k, out
+
+
+def cpbw_by_summation(a, eta):
+    out = np.zeros(2)
+    lib().orc_cpbw_by_summation(a, eta, dptr(out))
+    return out
+
+
+def cpbw_sample(a, eta, sv1, sv2, sv3, count, seed=0, fixed_n=0):
+    out = np.zeros((count, 3))
+    lib().orc_cpbw_sample(a, eta, sv1, sv2, sv3, seed, fixed_n, count, dptr(out))
     return out

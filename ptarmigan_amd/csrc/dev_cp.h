@@ -34,13 +34,17 @@ constexpr int CL_CDF = CL_RATE + CL_N;
 constexpr int G16N = CL_CDF + CL_N * 64;
 constexpr int G32N = G16N + 16;
 constexpr int G32W = G32N + 32;
-constexpr int SIZE = G32W + 32;
+constexpr int BW_COLS = 74, BW_ROWS = 121;     // CP Breit-Wheeler total rate [121][74][2]
+constexpr int BW_TOTAL = G32W + 32;            //   ln (S_3 = +1, S_3 = -1) (pair_creation/cp/total.rs:2-8)
+constexpr int SIZE = BW_TOTAL + BW_COLS * BW_ROWS * 2;
 }  // namespace cp_pack
 
 struct CpScalars {
     double min[2], step[2];               // ln a, ln eta of the quantum tables (total.rs:3-4)
     double cl_rate_min, cl_rate_step;     // classical/rate_table.rs:2-3
     double cl_cdf_min, cl_cdf_step;       // classical/cdf_table.rs:1-2
+    double bw_ln_min_a, bw_ln_min_eta;    // pair_creation/cp/total.rs:4-7
+    double bw_ln_a_step, bw_ln_eta_step;
 };
 
 // ---------------------------------------------------------------- Bessel J (bessel.rs)
@@ -332,6 +336,168 @@ PTG_HD inline void cp_stokes(TP tb, int n, double a, double s, double sn, double
     sv[3] = w;
 }
 
+// ---------------------------------------------------------------- CP nonlinear Breit-Wheeler (pair_creation/cp/mod.rs)
+constexpr double CP_LASER_S3 = -1.0;  // pair_creation/cp/mod.rs:13
+// s range at harmonic n: [s_min, 1/2], the other half by symmetry (:35-40)
+PTG_HD inline void cpbw_s_bounds(int n, double a, double eta, double& s_min, double& s_max) {
+    double sn = 2.0 * double(n) * eta / (1.0 + a * a);
+    s_min = fmin(0.5 * (1.0 - sqrt(1.0 - 4.0 / sn)), 0.5);
+    s_max = 0.5;
+}
+// unpolarised, S_1/S_2 and S_3 coefficients of the double-differential rate (:45-66)
+PTG_HD inline void cpbw_elements(TP tb, int n, double a, double eta, double s, double e[3]) {
+    double ell = double(n);
+    double a2 = a * a;
+    double sn = 2.0 * ell * eta / (1.0 + a2);
+    double tmp = 1.0 / (sn * s * (1.0 - s));
+    double z = sqrt((4.0 * ell * ell) * (a2 / (1.0 + a2)) * tmp * (1.0 - tmp));
+    double j[3];
+    bessel_j_pm(tb, n, z, j);
+    double jm2 = j[0] * j[0], jn2 = j[1] * j[1], jp2 = j[2] * j[2];
+    double sym = (1.0 - s) / s + s / (1.0 - s);
+    double ela = ell * a / z;
+    e[0] = jn2 + 0.25 * a2 * sym * (jm2 + jp2 - 2.0 * jn2);
+    e[1] = (2.0 * (ela * ela) - (1.0 + a2)) * jn2 - 0.5 * a2 * (jm2 + jp2);
+    e[2] = 0.25 * a2 * sym * (1.0 - 2.0 / (sn * s * (1.0 - s))) * (jm2 - jp2);
+}
+// rate differential in s and theta for Stokes (S_1, S_2, S_3) (:71-78); the reference
+// binds (sin 2theta, cos 2theta) to names in the opposite order, kept as written
+PTG_HD inline double cpbw_fully_resolved(TP tb, int n, double a, double eta, double s, double theta, double sv1, double sv2,
+                                         double sv3) {
+    double e[3];
+    cpbw_elements(tb, n, a, eta, s, e);
+    double first, second;
+    sincos(2.0 * theta, &first, &second);
+    return (e[0] - (sv1 * first + sv2 * second) * e[1] - CP_LASER_S3 * sv3 * e[2]) / (2.0 * PI_D);
+}
+// integrated over theta at S_3 = +1 (re) and -1 (im) (:80-93)
+PTG_HD __forceinline__ cplx cpbw_over_theta(TP tb, int n, double a, double eta, double s) {
+    double e[3];
+    cpbw_elements(tb, n, a, eta, s, e);
+    return cplx{e[0] - CP_LASER_S3 * e[2], e[0] + CP_LASER_S3 * e[2]};
+}
+// partial rate of harmonic n at S_3 = +1 (re) and -1 (im) by 32-point Gauss-Legendre
+// on one to three s intervals split at the estimated peak (:98-184)
+PTG_HD inline cplx cpbw_partial_rate(TP tb, const double* g32n, const double* g32w, int n, double a, double eta) {
+    double ell = double(n);
+    double sn = 2.0 * ell * eta / (1.0 + a * a);
+    if (sn <= 4.0) return cplx{0.0, 0.0};
+    double a2m2 = a * a - 2.0;
+    double ie = 1.0 / ell + 1.0;
+    double beta = (a * a) * (a * a) * (ie * ie) + 16.0 * (a2m2 * a2m2) / (sn * sn) - 8.0 * a * a * a2m2 / sn;
+    beta = sqrt(beta) / a2m2;
+    double alpha = (a * a + 2.0 * ell) / (ell * (2.0 - a * a)) - 4.0 / sn;
+    double s_peak = 0.5 * (1.0 - sqrt(alpha + beta));
+    double s_min, s_max;
+    cpbw_s_bounds(n, a, eta, s_min, s_max);
+    auto seg = [&](double lo, double hi) {
+        cplx acc{0.0, 0.0};
+        for (int i = 0; i < 32; ++i) {
+            double s = 0.5 * (hi - lo) * g32n[i] + 0.5 * (lo + hi);
+            acc = acc + (0.5 * (hi - lo) * g32w[i]) * cpbw_over_theta(tb, n, a, eta, s);
+        }
+        return acc;
+    };
+    if (__builtin_isfinite(s_peak)) {
+        double s_mid = 2.0 * s_peak - s_min;
+        if (s_mid > s_max) {
+            cplx lower = seg(s_min, s_peak), upper = seg(s_peak, s_max);
+            return 2.0 * (upper + lower);
+        }
+        cplx lower = seg(s_min, s_peak), middle = seg(s_peak, s_mid), upper = seg(s_mid, s_max);
+        return 2.0 * ((lower + middle) + upper);
+    }
+    return 2.0 * seg(s_min, s_max);
+}
+// harmonics contributing to the total (:368-381)
+PTG_HD inline void cpbw_sum_limits(double a, double eta, int& lo, int& hi) {
+    double n_min = ceil(2.0 * (1.0 + a * a) / eta);
+    double range = 2.0 * (0.25 + a * a) / eta + 13.0 * a * (2.0 + a);
+    double test = 0.25 - (1.0 + a * a) / (2.0 * n_min * eta);
+    if (test <= 2.220446049250313e-16) { lo = int(n_min) + 1; hi = int(n_min + 1.0 + range); }
+    else { lo = int(n_min); hi = int(n_min + range); }
+}
+PTG_HD __forceinline__ bool cpbw_is_too_small(double a, double eta) {
+    double l = log10(a) + 2.0;
+    return log10(eta) < -1.0 - l * l / 4.5;
+}
+// table lookup: ln f bilinear in (ln a, ln eta) where a eta > 2, else 1 / ln f
+// bilinear in (a, eta) (:406-449); false outside the table
+PTG_HD inline bool cpbw_by_lookup(const double* P, const CpScalars& c, double a, double eta, double f[2]) {
+    using namespace cp_pack;
+    double x = log(a), y = log(eta);
+    if (y < c.bw_ln_min_eta) { f[0] = f[1] = 0.0; return true; }
+    double fx = (x - c.bw_ln_min_a) / c.bw_ln_a_step, fy = (y - c.bw_ln_min_eta) / c.bw_ln_eta_step;
+    if (!(fx >= 0.0 && fx < double(BW_COLS - 1) && fy < double(BW_ROWS - 1))) return false;
+    int ix = int(fx), iy = int(fy);
+    const double* t00 = P + BW_TOTAL + (iy * BW_COLS + ix) * 2;
+    const double* t10 = t00 + BW_COLS * 2;
+    if (a * eta > 2.0) {
+        double dx = fx - double(ix), dy = fy - double(iy);
+        double w0 = (1.0 - dx) * (1.0 - dy), w1 = dx * (1.0 - dy), w2 = (1.0 - dx) * dy, w3 = dx * dy;
+        for (int k = 0; k < 2; ++k) f[k] = exp(w0 * t00[k] + w1 * t00[2 + k] + w2 * t10[k] + w3 * t10[2 + k]);
+    } else {
+        double a_min = exp(c.bw_ln_min_a + double(ix) * c.bw_ln_a_step), a_max = exp(c.bw_ln_min_a + double(ix + 1) * c.bw_ln_a_step);
+        double e_min = exp(c.bw_ln_min_eta + double(iy) * c.bw_ln_eta_step);
+        double e_max = exp(c.bw_ln_min_eta + double(iy + 1) * c.bw_ln_eta_step);
+        double dx = (a - a_min) / (a_max - a_min), dy = (eta - e_min) / (e_max - e_min);
+        double w0 = (1.0 - dx) * (1.0 - dy), w1 = dx * (1.0 - dy), w2 = (1.0 - dx) * dy, w3 = dx * dy;
+        for (int k = 0; k < 2; ++k) {
+            double g = w0 / t00[k] + w1 / t00[2 + k] + w2 / t10[k] + w3 / t10[2 + k];
+            f[k] = exp(1.0 / g);
+        }
+    }
+    return true;
+}
+// (S_3 = +1, S_3 = -1) totals (:271-281): 0 below a = 0.02 or where negligible,
+// the explicit sum over harmonics below the table's a range, else the table
+PTG_HD inline bool cpbw_total(TP tb, const double* P, const CpScalars& c, double a, double eta, double f[2]) {
+    f[0] = f[1] = 0.0;
+    if (a < 0.02 || cpbw_is_too_small(a, eta)) return true;
+    if (a < exp(c.bw_ln_min_a)) {
+        int lo, hi;
+        cpbw_sum_limits(a, eta, lo, hi);
+        cplx t{0.0, 0.0};
+        for (int n = lo; n <= hi; ++n) t = t + cpbw_partial_rate(tb, P + cp_pack::G32N, P + cp_pack::G32W, n, a, eta);
+        f[0] = t.re;
+        f[1] = t.im;
+        return true;
+    }
+    return cpbw_by_lookup(P, c, a, eta, f);
+}
+// probability over dphi and the Stokes parameters after the step (:286-322)
+PTG_HD inline double cpbw_probability(TP tb, const double* P, const CpScalars& c, double a, double eta, sv4& sv, double dphi,
+                                      bool& in_table) {
+    double f[2];
+    in_table = cpbw_total(tb, P, c, a, eta, f);
+    double prob = ALPHA * (0.5 * ((f[0] + f[1]) + sv.v * (f[0] - f[1]))) * dphi / eta;
+    double prob_avg = ALPHA * (0.5 * (f[0] + f[1])) * dphi / eta;
+    double delta = ALPHA * (0.5 * (f[0] - f[1])) * dphi / eta;
+    sv = {sv.i, sv.q * (1.0 - prob_avg) / (1.0 - prob), sv.u * (1.0 - prob_avg) / (1.0 - prob),
+          (sv.v * (1.0 - prob_avg) - delta) / (1.0 - prob)};
+    return prob;
+}
+// harmonic by explicit summation of partial rates until the cumulative rate passes
+// frac x the total (:327-365); n_max - 1 with `fell_back` set if it never does,
+// -1 if the total is not available
+PTG_HD inline int cpbw_harmonic(TP tb, const double* P, const CpScalars& c, double a, double eta, double sv3, double frac,
+                                bool& fell_back) {
+    int lo, hi;
+    cpbw_sum_limits(a, eta, lo, hi);
+    double f[2];
+    if (!cpbw_total(tb, P, c, a, eta, f)) return -1;
+    double target = 0.5 * (f[0] * (1.0 + sv3) + f[1] * (1.0 - sv3)) * frac;
+    double cumsum = 0.0;
+    fell_back = false;
+    for (int k = lo; k <= hi; ++k) {
+        cplx pr = cpbw_partial_rate(tb, P + cp_pack::G32N, P + cp_pack::G32W, k, a, eta);
+        cumsum += 0.5 * ((1.0 + sv3) * pr.re + (1.0 - sv3) * pr.im);
+        if (cumsum > target) return k;
+    }
+    fell_back = true;
+    return hi - 1;
+}
+
 // ---------------------------------------------------------------- sampler
 // (s, theta, Stokes) of one emission at harmonic n (cp/mod.rs:181-234,
 // cp/classical.rs:128-178): v = s / s_max (quantum) or s / s_n (classical) by
@@ -356,6 +522,53 @@ __device__ inline void cp_sample_at(TP tb, const double* P, int n, double a, dou
     double st[4];
     cp_stokes(tb, n, a, s, sn, v, theta, classical, st);
     sv = {st[0], st[1], st[2], st[3]};
+}
+
+// (s, theta) of one pair at harmonic n (pair_creation/cp/mod.rs:188-245): s by
+// rejection under 1.5 x the largest of 32 samples, mirrored into (1/2, 1) with
+// probability 1/2; theta by rejection under 1.01 x the maximum of the
+// theta-dependence, found at tan 2 theta = S_2 / S_1
+__device__ inline void cpbw_sample_at(TP tb, const double* P, int n, double a, double eta, double sv1, double sv2, double sv3, Rng& rng,
+                                      double& s_out, double& theta_out, uint32_t& trials, bool& capped) {
+    const double* g32n = P + cp_pack::G32N;
+    double s_min, s_max;
+    cpbw_s_bounds(n, a, eta, s_min, s_max);
+    double mx = NAN;
+    for (int i = 0; i < 32; ++i) {
+        cplx sp = cpbw_over_theta(tb, n, a, eta, 0.5 * (s_max - s_min) * g32n[i] + 0.5 * (s_min + s_max));
+        mx = fmax(mx, 0.5 * (sp.re * (1.0 + sv3) + sp.im * (1.0 - sv3)));
+    }
+    mx = 1.5 * mx;
+    const uint32_t cap = 1u << 20;
+    double s = s_min;
+    for (uint32_t t = 0;; ++t) {
+        double u1, u2;
+        rng_uniform2(rng, u1, u2);
+        ++trials;
+        s = s_min + (s_max - s_min) * u1;
+        if (t >= cap) { capped = true; break; }
+        cplx sp = cpbw_over_theta(tb, n, a, eta, s);
+        if (mx * u2 < 0.5 * (sp.re * (1.0 + sv3) + sp.im * (1.0 - sv3))) break;
+    }
+    if (int(rng_uniform(rng) * 2.0) == 0) s = 1.0 - s;
+    double theta_opt = 0.5 * atan2(sv2, sv1);
+    double mt = 0.0;
+    for (int i = 0; i < 4; ++i) {
+        double f = cpbw_fully_resolved(tb, n, a, eta, s, theta_opt + double(i) * (PI_D / 2.0), sv1, sv2, sv3);
+        mt = i == 0 ? f : fmax(mt, f);
+    }
+    mt = 1.01 * mt;
+    double theta = 0.0;
+    for (uint32_t t = 0;; ++t) {
+        double u1, u2;
+        rng_uniform2(rng, u1, u2);
+        ++trials;
+        theta = 2.0 * PI_D * u1;
+        if (t >= cap) { capped = true; break; }
+        if (mt * u2 < cpbw_fully_resolved(tb, n, a, eta, s, theta, sv1, sv2, sv3)) break;
+    }
+    s_out = s;
+    theta_out = theta;
 }
 
 }  // namespace ptg

@@ -29,6 +29,8 @@ struct LaserDev {
     // plane wave
     double amplitude;         // m c^2 k0 a0 / e
     double a_norm;            // e / (m c omega): a = a_norm |E|
+    // FocusedLaser (LMA): waist^2, Rayleigh range, (omega * duration)^2
+    double waist_sqd, zr, tau_sqd;
 };
 
 // ---------------------------------------------------------------- FastFocusedLaser

@@ -125,6 +125,111 @@ __device__ __forceinline__ double pw_grad(const LaserDev& L, double phase) {
     return norm * L.k0 * a02 * 2.0 * LN2_D * phase * exp2(-q * q) / ((PI_D * nc) * (PI_D * nc));
 }
 
+// ---------------------------------------------------------------- FocusedLaser (focused_laser.rs)
+// Gaussian beam of waist w0 whose <a^2> is the plane wave's envelope times
+// exp(-2 rho^2 / (w0^2 W)) / W, W = 1 + (z / z_R)^2, and whose four-gradient has
+// x, y components besides the t = z ones (focused_laser.rs:72-146).
+__device__ __forceinline__ void fl_envelope(const LaserDev& L, double phase, double& env, double& grad) {
+    const double nc = L.n_cycles;
+    if (L.envelope == ENV_COS2) {
+        if (fabs(phase) < PI_D * nc) {
+            double c = cos(phase / (2.0 * nc));
+            double c2 = c * c;
+            env = c2 * c2;
+            grad = 2.0 * L.k0 * tan(phase / (2.0 * nc)) * env / nc;
+        } else {
+            env = 0.0;
+            grad = 0.0;
+        }
+    } else if (L.envelope == ENV_FLATTOP) {
+        double ap = fabs(phase);
+        if (ap > PI_D * (nc + 1.0)) { env = 0.0; grad = 0.0; }
+        else if (ap > PI_D * (nc - 1.0)) {
+            double arg = 0.25 * (ap - (nc - 1.0) * PI_D);
+            double sn, cs;
+            sincos(arg, &sn, &cs);
+            double c2 = cs * cs;
+            env = c2 * c2;
+            grad = L.k0 * copysign(1.0, phase) * sn * (c2 * cs);
+        } else { env = 1.0; grad = 0.0; }
+    } else {
+        env = exp(-4.0 * LN2_D * (phase * phase) / L.tau_sqd);
+        grad = 8.0 * LN2_D * L.k0 * phase * env / L.tau_sqd;
+    }
+}
+__device__ __forceinline__ double fl_beam(const LaserDev& L, d4 r, double& width_sqd, double& rho_sqd) {
+    double zz = r.z / L.zr;
+    width_sqd = 1.0 + zz * zz;
+    rho_sqd = (r.x * r.x + r.y * r.y) / L.waist_sqd;
+    double norm = L.circular ? 1.0 : 0.5;
+    return norm * (L.a0 * L.a0) * exp(-2.0 * rho_sqd / width_sqd) / width_sqd;
+}
+__device__ __forceinline__ double fl_a_sqd(const LaserDev& L, d4 r) {
+    double w2, rho2;
+    double beam = fl_beam(L, r, w2, rho2);
+    double env, grad;
+    fl_envelope(L, L.k0 * (r.t - r.z), env, grad);
+    return beam * env;
+}
+// minus the four-gradient of <a^2>: (g_tz, g_x, g_y, g_tz)
+__device__ __forceinline__ void fl_neg_grad(const LaserDev& L, d4 r, double& g_tz, double& g_x, double& g_y) {
+    double w2, rho2;
+    double beam = fl_beam(L, r, w2, rho2);
+    double gb0 = -4.0 * beam * r.x / (L.waist_sqd * w2);
+    double gb1 = -4.0 * beam * r.y / (L.waist_sqd * w2);
+    double gb2 = (2.0 * beam * r.z / ((L.zr * L.zr) * w2)) * (2.0 * rho2 / w2 - 1.0);
+    double env, ge;
+    fl_envelope(L, L.k0 * (r.t - r.z), env, ge);
+    g_tz = -(beam * ge + gb2 * env);
+    g_x = -(gb0 * env);
+    g_y = -(gb1 * env);
+}
+// <a^2> at r for either LMA field
+__device__ __forceinline__ double lma_a_sqd(const LaserDev& L, d4 r) {
+    return L.kind == K_FOCUSED_LASER ? fl_a_sqd(L, r) : pw_a_sqd(L, L.k0 * (r.t - r.z));
+}
+// The plane-wave leapfrog with the focused beam's four-gradient (focused_laser.rs:185-225).
+__device__ inline double fl_push(const LaserDev& L, d4& r, d4& u, double rqm, double dt, int ll) {
+    double dtau = dt / u.t;
+    double sc = rqm / RQM_ELECTRON;
+    double scale = sc * sc;
+    const double kc = C_LIGHT * T_COMPTON;
+    const double half_c = 0.5 * C_LIGHT;
+    d4 u_mid{0.0, 0.0, 0.0, 0.0};
+    if (ll) {
+        double gtz, gx, gy;
+        fl_neg_grad(L, r, gtz, gx, gy);
+        double eta = kc * L.k0 * (u.t - u.z);
+        double gfac = -2.0 * ALPHA * fl_a_sqd(L, r) * eta * eta / (3.0 * T_COMPTON) * scale * scale;
+        double ft = half_c * scale * gtz, fx = half_c * scale * gx, fy = half_c * scale * gy;
+        u_mid = {u.t + 0.5 * (ft + gfac * u.t) * dtau, u.x + 0.5 * (fx + gfac * u.x) * dtau, u.y + 0.5 * (fy + gfac * u.y) * dtau,
+                 u.z + 0.5 * (ft + gfac * u.z) * dtau};
+    }
+    double h = 0.5 * C_LIGHT * dtau;
+    r = {r.t + h * u.t, r.x + h * u.x, r.y + h * u.y, r.z + h * u.z};
+    double gtz, gx, gy;
+    fl_neg_grad(L, r, gtz, gx, gy);
+    double ft = half_c * scale * gtz, fx = half_c * scale * gx, fy = half_c * scale * gy;
+    double dwork;
+    if (ll) {
+        double eta = kc * L.k0 * (u_mid.t - u_mid.z);
+        double asq = fl_a_sqd(L, r);
+        double gfac = -2.0 * ALPHA * asq * eta * eta / (3.0 * T_COMPTON) * scale * scale;
+        u = {u.t + (ft + gfac * u_mid.t) * dtau, u.x + (fx + gfac * u_mid.x) * dtau, u.y + (fy + gfac * u_mid.y) * dtau,
+             u.z + (ft + gfac * u_mid.z) * dtau};
+        double omega = C_LIGHT * L.k0;
+        double delta = L.circular ? 1.0 : 0.75;
+        dwork = (ft + 2.0 * ALPHA * omega * eta * delta * asq * asq / 3.0) * dtau;
+    } else {
+        u = {u.t + ft * dtau, u.x + fx * dtau, u.y + fy * dtau, u.z + ft * dtau};
+        dwork = ft * dtau;
+    }
+    r = {r.t + h * u.t, r.x + h * u.x, r.y + h * u.y, r.z + h * u.z};
+    double asq = fl_a_sqd(L, r);
+    u.t = sqrt(1.0 + asq + u.x * u.x + u.y * u.y + u.z * u.z);
+    return dwork;
+}
+
 // Leapfrog in proper time with the ponderomotive force, optional cycle-averaged
 // Landau-Lifshitz reaction, then the mass-shell fix u^2 = 1 + <a^2>
 // (plane_wave.rs:196-249). r = (ct, x, y, z), u = (u0, u). Returns the work.
@@ -164,6 +269,10 @@ __device__ __forceinline__ double pw_push(const LaserDev& L, d4& r, d4& u, doubl
     double asq = pw_a_sqd(L, L.k0 * (r.t - r.z));
     u.t = sqrt(1.0 + asq + u.x * u.x + u.y * u.y + u.z * u.z);
     return dwork;
+}
+// push in either LMA field (the kind is uniform over a launch)
+__device__ __forceinline__ double lma_push(const LaserDev& L, d4& r, d4& u, double rqm, double dt, int ll) {
+    return L.kind == K_FOCUSED_LASER ? fl_push(L, r, u, rqm, dt, ll) : pw_push(L, r, u, rqm, dt, ll);
 }
 
 // ---------------------------------------------------------------- LP rate (lp/mod.rs:430-463, lp/linear.rs:6-45)

@@ -325,10 +325,12 @@ bool build_cp_pack(ptg_ctx* c, std::vector<double>& pack) {
     const auto* g16 = get("quadrature/tables:GAUSS_16_NODES", 16);
     const auto* g32n = get("quadrature/tables:GAUSS_32_NODES", 32);
     const auto* g32w = get("quadrature/tables:GAUSS_32_WEIGHTS", 32);
+    const auto* bw = get("pair_creation/cp/total:TABLE", size_t(BW_COLS) * BW_ROWS * 2);
     const auto* mn = get("nonlinear_compton/cp/tables/total:MIN", 2);
     const auto* st = get("nonlinear_compton/cp/tables/total:STEP", 2);
     auto one = [&](const char* name) { const auto* v = get(name, 1); return v ? (*v)[0] : NAN; };
-    if (!rate || !cdf || !cl_rate || !cl_cdf || !g16 || !g32n || !g32w || !mn || !st) return false;
+    if (!rate || !cdf || !cl_rate || !cl_cdf || !g16 || !g32n || !g32w || !mn || !st || !bw) return false;
+    std::copy(bw->begin(), bw->end(), pack.begin() + BW_TOTAL);
     std::copy(rate->begin(), rate->end(), pack.begin() + RATE);
     std::copy(cdf->begin(), cdf->end(), pack.begin() + CDF);
     std::copy(cl_rate->begin(), cl_rate->end(), pack.begin() + CL_RATE);
@@ -342,7 +344,11 @@ bool build_cp_pack(ptg_ctx* c, std::vector<double>& pack) {
     q.cl_rate_step = one("nonlinear_compton/cp/classical/rate_table:STEP");
     q.cl_cdf_min = one("nonlinear_compton/cp/classical/cdf_table:MIN");
     q.cl_cdf_step = one("nonlinear_compton/cp/classical/cdf_table:STEP");
-    return std::isfinite(q.cl_rate_min) && std::isfinite(q.cl_cdf_step);
+    q.bw_ln_min_a = one("pair_creation/cp/total:LN_MIN_A");
+    q.bw_ln_min_eta = one("pair_creation/cp/total:LN_MIN_ETA");
+    q.bw_ln_a_step = one("pair_creation/cp/total:LN_A_STEP");
+    q.bw_ln_eta_step = one("pair_creation/cp/total:LN_ETA_STEP");
+    return std::isfinite(q.cl_rate_min) && std::isfinite(q.cl_cdf_step) && std::isfinite(q.bw_ln_min_a) && std::isfinite(q.bw_ln_eta_step);
 }
 
 // Laser constants (fast_focused_laser.rs:24-38,47-53,129-169; fast_plane_wave.rs:23-108;
@@ -353,8 +359,6 @@ int make_laser(ptg_ctx* c, const ptg_laser_desc* d, LaserDev& L) {
     if (!(d->a0 > 0.0) || !(d->wavelength > 0.0) || !(d->n_cycles > 0.0)) return fail(c, PTG_E_ARG, "a0, wavelength and n_cycles must be positive");
     if ((d->kind == PTG_FOCUSED_LASER || d->kind == PTG_FAST_FOCUSED_LASER) && !(d->waist > 0.0))
         return fail(c, PTG_E_ARG, "focused lasers need a positive waist");
-    if (d->kind == PTG_FOCUSED_LASER)
-        return fail(c, PTG_E_UNSUPPORTED, "FocusedLaser (LMA, focused) is not built into this library yet");
     L = LaserDev{};
     L.kind = d->kind;
     L.envelope = d->envelope;
@@ -386,6 +390,15 @@ int make_laser(ptg_ctx* c, const ptg_laser_desc* d, LaserDev& L) {
         L.amplitude = (M_ELECTRON * C_LIGHT_SQD * L.k0 * d->a0) / E_CHARGE;
         double chirp = 1.0 + 2.0 * d->chirp_b * PI_H * d->n_cycles;
         L.dt = multiplier / (C_LIGHT * L.k0 * chirp);
+    } else if (d->kind == PTG_FOCUSED_LASER) {  // focused_laser.rs:25-70, 165-173: no chirp; n_cycles from the duration
+        double duration = d->n_cycles * d->wavelength / C_LIGHT;
+        L.n_cycles = C_LIGHT * duration * L.k0 / (2.0 * PI_H);
+        L.chirp_b = 0.0;
+        L.waist_sqd = d->waist * d->waist;
+        L.zr = 0.5 * L.k0 * L.waist_sqd;
+        double tau = L.omega * duration;
+        L.tau_sqd = tau * tau;
+        L.dt = (d->envelope == PTG_FLATTOP ? 0.2 : 1.0) / L.omega;
     } else {  // PlaneWave (LMA)
         L.n_cycles = d->n_cycles;
         L.dt = (d->envelope == PTG_FLATTOP ? 0.2 : 1.0) / (C_LIGHT * L.k0);
@@ -495,8 +508,9 @@ int ptg_set_laser(ptg_ctx* c, const ptg_laser_desc* d) {
     c->have_laser = true;
     c->pol_angle = d->polarization == PTG_CIRCULAR ? 0.0 : d->pol_angle;
     c->lma_bandwidth = 0.0;
-    if (d->finite_bandwidth) {  // plane_wave.rs:49-63
-        double n_fwhm = d->envelope == PTG_COS_SQUARED ? 0.36405666377387671305 * d->n_cycles : d->n_cycles;
+    if (d->finite_bandwidth) {  // plane_wave.rs:49-63, focused_laser.rs:45-58
+        double nc = d->kind == PTG_FOCUSED_LASER ? L.n_cycles : d->n_cycles;
+        double n_fwhm = d->envelope == PTG_COS_SQUARED ? 0.36405666377387671305 * nc : nc;
         c->lma_bandwidth = std::sqrt(0.5 * LN2_D) / (PI_H * n_fwhm);
     }
     return PTG_OK;
@@ -951,15 +965,14 @@ int ptg_run(ptg_ctx* c, const ptg_collide_options* o, uint64_t seed, ptg_counter
     if (c->n_prim == 0) return fail(c, PTG_E_STATE, "no primaries resident (ptg_load_primaries / ptg_build_beam)");
     int rc = validate_options(c, o);
     if (rc) return rc;
-    if (c->laser.kind == K_PLANE_WAVE) {
-        if (o->tracking_photons && !c->d_bw)
-            return fail(c, PTG_E_UNSUPPORTED, "pair creation in a PlaneWave needs the regenerated Breit-Wheeler mid-range table "
+    if (c->laser.kind == K_PLANE_WAVE || c->laser.kind == K_FOCUSED_LASER) {
+        if (o->tracking_photons && !c->laser.circular && !c->d_bw)
+            return fail(c, PTG_E_UNSUPPORTED, "LMA pair creation needs the regenerated Breit-Wheeler mid-range table "
                                               "(tables/gen_bw_tables.py, then tables/extract.py)");
         if (c->laser.circular && !c->d_cp)
             return fail(c, PTG_E_UNSUPPORTED, "circularly polarised LMA emission needs the regenerated CP CDF table "
                                               "(tables/gen_cp_tables.py, then tables/extract.py)");
-        if (c->laser.circular && o->tracking_photons)
-            return fail(c, PTG_E_UNSUPPORTED, "pair creation in a circularly polarised PlaneWave is not built into this library yet");
+
     }
     HIPCHK(c, hipSetDevice(c->device));
     const size_t n = c->n_prim;
@@ -1023,7 +1036,7 @@ int ptg_run(ptg_ctx* c, const ptg_collide_options* o, uint64_t seed, ptg_counter
     double ms[2] = {0.0, 0.0};  // lepton, photon kernels
     uint64_t launches[2] = {0, 0};
     uint64_t gens = 0;
-    if (c->laser.kind == K_PLANE_WAVE) {
+    if (c->laser.kind == K_PLANE_WAVE || c->laser.kind == K_FOCUSED_LASER) {
         rc = run_lma(c, a, ms, launches, &gens);
         if (rc) return rc;
     } else {
@@ -1147,7 +1160,7 @@ int ptg_push(ptg_ctx* c, const uint8_t* species, double* r, double* u, size_t n,
     HIPCHK(c, sfill(c, dw, 0, n * 8));
     int rr_mode = (o->classical && o->rr) ? (o->gaunt_factor ? 2 : 1) : 0;
     double dt = c->laser.dt * o->dt_multiplier;
-    if (c->laser.kind == K_PLANE_WAVE) {
+    if (c->laser.kind == K_PLANE_WAVE || c->laser.kind == K_FOCUSED_LASER) {
         if (rr_mode == 2) return fail(c, PTG_E_ARG, "Gaunt factor correction is unavailable in LMA mode");
         launch_push_lma(c->laser, dt, rr_mode, ds, dr, du, uint32_t(n), n_steps, dk, dw, c->stream);
     } else {
@@ -1216,14 +1229,29 @@ int ptg_histogram(ptg_ctx* c, ptg_hist_spec* s, double* bins_out, double* total_
 int ptg_eval(ptg_ctx* c, int fn, const double* in, size_t n, size_t in_w, double* out, size_t out_w) {
     if (!c || !in || !out) return fail(c, PTG_E_ARG, "null argument");
     if (fn < 0 || fn >= PTG_FN_COUNT) return fail(c, PTG_E_ARG, "unknown function id");
-    static const int need_in[] = {2, 5, 1, 1, 4, 1, 3, 12, 2, 3, 4, 4, 2, 4, 6, 2, 3, 4, 4, 5, 2, 2, 2, 4};
-    static const int need_out[] = {1, 4, 2, 4, 7, 1, 1, 4, 2, 1, 5, 6, 3, 1, 4, 3, 2, 1, 2, 7, 2, 1, 1, 6};
+    static const int need_in[] = {2, 5, 1, 1, 4, 1, 3, 12, 2, 3, 4, 4, 2, 4, 6, 2, 3, 4, 4, 5, 2, 2, 2, 4, 2, 3, 4, 7};
+    static const int need_out[] = {1, 4, 2, 4, 7, 1, 1, 4, 2, 1, 5, 6, 3, 1, 4, 3, 2, 1, 2, 7, 2, 1, 1, 6, 3, 2, 2, 4};
     static_assert(sizeof(need_in) / sizeof(int) == PTG_FN_COUNT && sizeof(need_out) / sizeof(int) == PTG_FN_COUNT, "probe widths");
     if (int(in_w) < need_in[fn] || int(out_w) < need_out[fn]) return fail(c, PTG_E_ARG, "row width too small for this function");
     if (fn == PTG_FN_FIELDS && !c->have_laser) return fail(c, PTG_E_STATE, "no laser set");
     if (n == 0) return PTG_OK;
     if (fn >= PTG_FN_BW_RATE && fn <= PTG_FN_BW_SAMPLE && !c->d_bw) return fail(c, PTG_E_UNSUPPORTED, "Breit-Wheeler tables not loaded");
-    if (fn >= PTG_FN_CP_RATE && fn <= PTG_FN_CP_SAMPLE && !c->d_cp) return fail(c, PTG_E_UNSUPPORTED, "CP nonlinear Compton tables not loaded");
+    if (((fn >= PTG_FN_CP_RATE && fn <= PTG_FN_CP_SAMPLE) || fn >= PTG_FN_CPBW_RATE) && !c->d_cp)
+        return fail(c, PTG_E_UNSUPPORTED, "CP tables not loaded");
+    if (fn >= PTG_FN_CPBW_RATE) {
+        for (size_t i = 0; i < n; ++i) {
+            const double* x = in + i * in_w;
+            double a = fn == PTG_FN_CPBW_PARTIAL_RATE ? x[1] : x[0], eta = fn == PTG_FN_CPBW_PARTIAL_RATE ? x[2] : x[1];
+            if (!(a > 0.0 && a < 100.0 && eta > 0.0 && eta < 100.0))
+                return fail(c, PTG_E_ARG, "CP Breit-Wheeler probes need 0 < a < 100 and 0 < eta < 100");
+            if (fn == PTG_FN_CPBW_PARTIAL_RATE && !(x[0] >= 1.0 && x[0] <= 1.0e6 && double(int(x[0])) == x[0]))
+                return fail(c, PTG_E_ARG, "CP Breit-Wheeler partial-rate probe needs an integer 1 <= n <= 1e6");
+            if (fn == PTG_FN_CPBW_SAMPLE && !(x[6] == 0.0 || (x[6] >= 1.0 && x[6] <= 1.0e6)))
+                return fail(c, PTG_E_ARG, "CP Breit-Wheeler sample probe needs fixed n = 0 or 1 <= n <= 1e6");
+            if (fn != PTG_FN_CPBW_PARTIAL_RATE && a < 0.3 && 2.0 * (0.25 + a * a) / eta > 1.0e5)
+                return fail(c, PTG_E_ARG, "CP Breit-Wheeler probe: more than 1e5 harmonics to sum");
+        }
+    }
     if (fn == PTG_FN_BESSEL_J || fn == PTG_FN_CP_PARTIAL_RATE || fn == PTG_FN_CP_SAMPLE) {
         for (size_t i = 0; i < n; ++i) {
             const double* x = in + i * in_w;

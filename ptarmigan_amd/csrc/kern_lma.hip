@@ -1,4 +1,4 @@
-// Transport kernels for the LMA plane wave (PlaneWave): collide()'s
+// Transport kernels for the LMA fields (PlaneWave, FocusedLaser): collide()'s
 // electron/positron loop (src/main.rs:169-210) with the step and the emission
 // event split into two kernels.
 //
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(256) void k_step_lma(const TransportArgs* __restric
         }
         ++n_steps;
         ++psteps;
-        work += pw_push(L, r, u, rqm, dt, A->rr_mode == 1);
+        work += lma_push(L, r, u, rqm, dt, A->rr_mode == 1);
         // PlaneWave::radiate (plane_wave.rs:251-282)
         double phase = L.k0 * (r.t - r.z);
         double chirp = 1.0 + 2.0 * L.chirp_b * phase;
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(256) void k_step_lma(const TransportArgs* __restric
                 q.d[P_ICOUNT][j] = A->in.d[P_ICOUNT][idx];
                 q.nev[j] = nev;
                 M.ev_k0[j] = k0;
-                M.ev_a[j] = sqrt(pw_a_sqd(L, phase));
+                M.ev_a[j] = sqrt(lma_a_sqd(L, r));
             }
             live = fetch();
             continue;
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(64) void k_event_lma(const TransportArgs* __restric
     if (flags) atomicOr(A->flags, flags);
 }
 
-// ================================================================ photons: LP Breit-Wheeler
+// ================================================================ photons: Breit-Wheeler
 // Photon loop of collide() in the plane wave (src/main.rs:212-264 with
 // plane_wave.rs:284-316 and pair_creation/mod.rs:21-41): straight-line steps,
 // the decay probability and Stokes evolution of the photon, and the survival
@@ -268,20 +268,24 @@ __global__ __launch_bounds__(64) void k_event_lma(const TransportArgs* __restric
 // the start of the step's update (position after the move, Stokes parameters
 // before it); the pair-event kernel repeats the step's probability from that
 // record and completes the step.
+template <bool CIRC>
 __device__ __forceinline__ double lma_photon_prob(const LmaArgs& M, const LaserDev& L, double dt, d4 r, d4 k, sv4 pin,
                                                   double& a, double& k0, double& eta, sv4& pnew, bool& in_table) {
     const double kc = C_LIGHT * T_COMPTON;
     double phase = L.k0 * (r.t - r.z);
-    a = sqrt(pw_a_sqd(L, phase));
+    a = sqrt(lma_a_sqd(L, r));
     k0 = kc * L.k0 * (1.0 + 2.0 * L.chirp_b * phase);   // kappa = k0 (1, 0, 0, 1)
     eta = k0 * (k.t - k.z);
     double dphi = eta * dt / (T_COMPTON * k.t);
     sv4 s = stokes_rotate(stokes_to_lma(pin, k), M.pol_angle);
-    double prob = bw_probability(M.bw, M.bsc, a * 1.4142135623730951, eta, s, dphi, in_table);
+    // the LP rates take the peak amplitude a sqrt 2, the CP ones the rms a (pair_creation/mod.rs:28-36)
+    double prob = CIRC ? cpbw_probability(M.lcfa, M.cp, M.csc, a, eta, s, dphi, in_table)
+                       : bw_probability(M.bw, M.bsc, a * 1.4142135623730951, eta, s, dphi, in_table);
     pnew = stokes_from_lma(stokes_rotate(s, -M.pol_angle), k);
     return prob;
 }
 
+template <bool CIRC>
 __global__ __launch_bounds__(256) void k_photon_step_lma(const TransportArgs* __restrict__ A, LmaArgs M) {
     const LaserDev& L = A->laser;
     const double dt = A->dt, cdt = C_LIGHT * dt;
@@ -335,7 +339,7 @@ __global__ __launch_bounds__(256) void k_photon_step_lma(const TransportArgs* __
         sv4 pnew;
         bool in_table;
         // outside both rate tables the probability is 0 without a diagnostic (lp/mod.rs:388-397)
-        double prob = lma_photon_prob(M, L, dt, r, k, pin, a, k0, eta, pnew, in_table);
+        double prob = lma_photon_prob<CIRC>(M, L, dt, r, k, pin, a, k0, eta, pnew, in_table);
         double ri = A->opt.rate_increase;
         if (prob * ri > 0.1) ri = 0.1 / prob;
         S *= (1.0 - prob * ri);
@@ -369,6 +373,7 @@ __global__ __launch_bounds__(256) void k_photon_step_lma(const TransportArgs* __
 // on the double-differential rate, the positron built in the zero-momentum
 // frame; the electron and positron go to the pair queue, the photon is kept
 // (decayed) or re-queued with its reduced weight.
+template <bool CIRC>
 __global__ __launch_bounds__(64) void k_pair_event_lma(const TransportArgs* __restrict__ A, LmaArgs M) {
     const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n_lanes = gridDim.x * blockDim.x;
@@ -388,7 +393,7 @@ __global__ __launch_bounds__(64) void k_pair_event_lma(const TransportArgs* __re
         double a, k0, eta;
         sv4 pnew;
         bool in_table;
-        double prob = lma_photon_prob(M, L, A->dt, r, k, pin, a, k0, eta, pnew, in_table);
+        double prob = lma_photon_prob<CIRC>(M, L, A->dt, r, k, pin, a, k0, eta, pnew, in_table);
         double ri = A->opt.rate_increase;
         if (prob * ri > 0.1) ri = 0.1 / prob;
         double frac = 1.0 / ri;
@@ -396,11 +401,13 @@ __global__ __launch_bounds__(64) void k_pair_event_lma(const TransportArgs* __re
         rng.c1 = nev | EVENT_BLOCK;
         double alp = a * 1.4142135623730951;
         sv4 sl = stokes_rotate(stokes_to_lma(pin, k), M.pol_angle);
-        int n = bw_harmonic(M.bw, M.bsc, alp, eta, sl.q, rng_uniform(rng));
+        bool fell_back = false;
+        int n = CIRC ? cpbw_harmonic(M.lcfa, M.cp, M.csc, a, eta, sl.v, rng_uniform(rng), fell_back)
+                     : bw_harmonic(M.bw, M.bsc, alp, eta, sl.q, rng_uniform(rng));
         bool made = false;
         if (n < 1) {
             flags |= PTG_FLAG_TABLE_RANGE;
-        } else if (dbessel_len(n) > M.scratch_len) {
+        } else if (!CIRC && dbessel_len(n) > M.scratch_len) {
             if (M.ovf_list) {
                 M.ovf_list[atomicAdd(M.ovf_ctr, 1ull)] = i;
                 atomicMax(M.ovf_len, (unsigned long long)dbessel_len(n));
@@ -408,11 +415,11 @@ __global__ __launch_bounds__(64) void k_pair_event_lma(const TransportArgs* __re
             }
             flags |= PTG_FLAG_TABLE_RANGE;
         } else {
-            DBessel D = dbessel_at(n, ws, n_lanes);
             double s, cphi;
             uint32_t tr = 0;
             bool capped = false;
-            bw_sample_at(D, alp, eta, sl.q, sl.u, rng, s, cphi, tr, capped);
+            if (CIRC) cpbw_sample_at(M.lcfa, M.cp, n, a, eta, sl.q, sl.u, sl.v, rng, s, cphi, tr, capped);
+            else bw_sample_at(dbessel_at(n, ws, n_lanes), alp, eta, sl.q, sl.u, rng, s, cphi, tr, capped);
             trials += tr;
             if (capped) flags |= PTG_FLAG_REJECT_CAP;
             // positron in the zero-momentum frame (pair_creation/mod.rs:63-104)
@@ -477,10 +484,12 @@ __global__ __launch_bounds__(64) void k_pair_event_lma(const TransportArgs* __re
 }
 
 void launch_photon_step_lma(const TransportArgs* a, const LmaArgs& m, int grid, hipStream_t s) {
-    k_photon_step_lma<<<grid, 256, 0, s>>>(a, m);
+    if (m.mode == LMA_CP || m.mode == LMA_CP_CLASSICAL) k_photon_step_lma<true><<<grid, 256, 0, s>>>(a, m);
+    else k_photon_step_lma<false><<<grid, 256, 0, s>>>(a, m);
 }
 void launch_pair_event_lma(const TransportArgs* a, const LmaArgs& m, int lanes, hipStream_t s) {
-    k_pair_event_lma<<<lanes / 64, 64, 0, s>>>(a, m);
+    if (m.mode == LMA_CP || m.mode == LMA_CP_CLASSICAL) k_pair_event_lma<true><<<lanes / 64, 64, 0, s>>>(a, m);
+    else k_pair_event_lma<false><<<lanes / 64, 64, 0, s>>>(a, m);
 }
 
 void launch_step_lma(const TransportArgs* a, const LmaArgs& m, int grid, hipStream_t s) {
@@ -510,7 +519,7 @@ __global__ void k_push_lma(LaserDev L, double dt, int ll, const uint8_t* species
     double rqm = species[i] == 1 ? -RQM_ELECTRON : RQM_ELECTRON;
     double w = 0.0;
     uint32_t k = 0;
-    for (; k < n_steps && field_contains(L, r); ++k) w += pw_push(L, r, u, rqm, dt, ll);
+    for (; k < n_steps && field_contains(L, r); ++k) w += lma_push(L, r, u, rqm, dt, ll);
     r4[4 * i] = r.t; r4[4 * i + 1] = r.x; r4[4 * i + 2] = r.y; r4[4 * i + 3] = r.z;
     u4[4 * i] = u.t; u4[4 * i + 1] = u.x; u4[4 * i + 2] = u.y; u4[4 * i + 3] = u.z;
     taken[i] = k;
@@ -616,6 +625,37 @@ __global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, ui
         bool capped = false;
         lp_cl_sample(dbessel_at(nh, scratch + i, n), x[0], x[1], rng, s, th, pol, tr, capped);
         y[0] = nh; y[1] = s; y[2] = th; y[3] = pol.q; y[4] = pol.u; y[5] = tr;
+        break;
+    }
+    case PTG_FN_CPBW_RATE: {
+        double f[2] = {0.0, 0.0};
+        double a = x[0], eta = x[1];
+        bool ok = cpbw_total(m.lcfa, m.cp, m.csc, a, eta, f);
+        y[0] = f[0]; y[1] = f[1];
+        y[2] = !ok ? -1.0 : ((a < 0.02 || cpbw_is_too_small(a, eta)) ? 0.0 : (a < exp(m.csc.bw_ln_min_a) ? 1.0 : 2.0));
+        break;
+    }
+    case PTG_FN_CPBW_PARTIAL_RATE: {
+        cplx r = cpbw_partial_rate(m.lcfa, m.cp + cp_pack::G32N, m.cp + cp_pack::G32W, int(x[0]), x[1], x[2]);
+        y[0] = r.re; y[1] = r.im;
+        break;
+    }
+    case PTG_FN_CPBW_HARMONIC: {
+        bool fb = false;
+        y[0] = cpbw_harmonic(m.lcfa, m.cp, m.csc, x[0], x[1], x[2], x[3], fb);
+        y[1] = fb ? 1.0 : 0.0;
+        break;
+    }
+    case PTG_FN_CPBW_SAMPLE: {
+        Rng rng = rng_make(key, uint64_t(x[5]));
+        bool fb = false;
+        int nh = x[6] > 0.0 ? int(x[6]) : cpbw_harmonic(m.lcfa, m.cp, m.csc, x[0], x[1], x[4], rng_uniform(rng), fb);
+        if (nh < 1) { y[0] = 0.0; y[3] = -2.0; break; }
+        double s, th;
+        uint32_t tr = 0;
+        bool capped = false;
+        cpbw_sample_at(m.lcfa, m.cp, nh, x[0], x[1], x[2], x[3], x[4], rng, s, th, tr, capped);
+        y[0] = nh; y[1] = s; y[2] = th; y[3] = capped ? -1.0 : double(tr);
         break;
     }
     case PTG_FN_BESSEL_J: {

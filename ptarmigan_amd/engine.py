@@ -308,3 +308,7 @@ class Fn(enum.IntEnum):
     LP_CL_HARMONIC = 21
     LP_CL_CEILING = 22
     LP_CL_SAMPLE = 23
+    CPBW_RATE = 24
+    CPBW_PARTIAL_RATE = 25
+    CPBW_HARMONIC = 26
+    CPBW_SAMPLE = 27

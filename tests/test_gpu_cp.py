@@ -164,3 +164,84 @@ def test_cp_plane_wave_shower_matches_oracle(engine, orc, classical):
     assert ks_2samp(eg, ec).pvalue > 1e-3
     wg, wc = sg["work"][sg["species"] == 0], sc["work"][sc["species"] == 0]
     assert ks_2samp(wg, wc).pvalue > 1e-3
+
+
+# ---------------------------------------------------------------- CP nonlinear Breit-Wheeler
+def test_cpbw_rates_match_oracle(engine, orc):
+    """Totals (table, explicit sum below a = 0.3, zero where negligible) and partial
+    rates against the oracle; the reference's partial-rate values at 1e-6."""
+    from ptarmigan_amd import Fn
+    from test_oracle_cp import CPBW_KATS
+    say("cpbw rates")
+    rng = np.random.default_rng(31)
+    m = 2000
+    a = np.exp(rng.uniform(np.log(0.01), np.log(20.0), m))
+    eta = np.exp(rng.uniform(np.log(0.01), np.log(1.9), m))
+    keep = ~((a < 0.3) & (2.0 * (0.25 + a * a) / eta > 2000))
+    a, eta = a[keep], eta[keep]
+    got = engine.eval(Fn.CPBW_RATE, np.stack([a, eta], 1), 3)
+    for i in range(len(a)):
+        k, f = orc.cpbw_total(a[i], eta[i])
+        assert got[i, 2] == k, (a[i], eta[i])
+        assert np.allclose(got[i, :2], f, rtol=1e-9 if k == 1 else 1e-11, atol=0), (a[i], eta[i], got[i], f)
+    n = np.array([k[0] for k in CPBW_KATS], dtype=float)
+    kat = np.array(CPBW_KATS)
+    got = engine.eval(Fn.CPBW_PARTIAL_RATE, kat[:, :3], 2)
+    assert np.all(np.abs(0.5 * got.sum(1) - kat[:, 3]) / kat[:, 3] < 1e-6)
+    for i in range(len(n)):
+        ref = orc.cpbw_partial_rate(int(n[i]), kat[i, 1], kat[i, 2])
+        assert np.allclose(got[i], ref, rtol=1e-10 if n[i] < 1000 else 1e-8, atol=0), i
+
+
+@pytest.mark.parametrize("a,eta,sv", [(1.0, 0.5, (0.0, 0.0, 0.0)), (3.0, 0.2, (0.5, -0.3, 0.7)), (0.2, 1.0, (0.0, 0.5, -0.5))])
+def test_cpbw_sampler_matches_oracle(engine, orc, a, eta, sv):
+    """Harmonic indices value by value on common fractions, then (n, s, theta)
+    statistically."""
+    from scipy.stats import ks_2samp
+    from ptarmigan_amd import Fn
+    say("cpbw sample", a, eta, sv)
+    frac = (np.arange(1000) + 0.5) / 1000
+    got = engine.eval(Fn.CPBW_HARMONIC, np.stack([np.full(1000, a), np.full(1000, eta), np.full(1000, sv[2]), frac], 1), 2)
+    ref = np.array([orc.lib().orc_cpbw_harmonic(a, eta, sv[2], f) for f in frac])
+    assert (got[:, 0] == ref).mean() > 0.99 and np.all(np.abs(got[:, 0] - ref) <= 1)
+    m = 6000
+    inp = np.stack([np.full(m, a), np.full(m, eta), np.full(m, sv[0]), np.full(m, sv[1]), np.full(m, sv[2]),
+                    np.arange(m, dtype=float), np.zeros(m)], 1)
+    got = engine.eval(Fn.CPBW_SAMPLE, inp, 4)
+    ref = orc.cpbw_sample(a, eta, *sv, m, seed=8)
+    assert np.all(got[:, 3] >= 1)
+    lo, hi = int(min(got[:, 0].min(), ref[:, 0].min())), int(max(got[:, 0].max(), ref[:, 0].max()))
+    edges = np.unique(np.linspace(lo, hi + 1, 40).astype(int)) - 0.5
+    assert _chi2_two_sample(got[:, 0], ref[:, 0], edges) > 1e-3
+    for col in (1, 2):
+        assert ks_2samp(got[:, col], ref[:, col]).pvalue > 1e-3, col
+
+
+def test_cp_photon_beam_pair_creation_matches_oracle(engine, orc):
+    """16.5 GeV photons, partly circularly polarised (S_3 = 0.5), through an a0 = 8
+    CP plane wave (pairs not tracked): pair yield, pair spectra, surviving photons
+    and their S_3."""
+    from scipy.stats import ks_2samp
+    from ptarmigan_amd import Beam, CollideOptions, Envelope, Laser, LaserKind, Polarization, Species
+    laser = Laser(LaserKind.PLANE_WAVE, 8.0, 0.8e-6, 16.0, envelope=Envelope.COS_SQUARED, polarization=Polarization.CIRCULAR)
+    g = 16500.0 / 0.510999
+    beam = Beam(n=20_000, gamma=g, sigma=0.01 * g, species=Species.PHOTON, rms_div=1e-6, pol=(1.0, 0.0, 0.0, 0.5))
+    opts = CollideOptions(tracking_photons=True, tracking_secondaries=False, pol_resolved=True)
+    say("cp photon beam pairs")
+    engine.set_laser(laser)
+    dt, z0 = engine.field_info()
+    prim = orc.build_beam(beam.desc(), z0, 12)
+    sg, _, cg = engine.collide(prim, opts, seed=12)
+    sc, _, cc, _ = orc.collide(laser.desc(), opts.desc(), prim, 12, 16)
+    say("  gpu", cg, "\n  cpu", cc)
+    assert cg["error_flags"] == 0
+    a, b = cg["pair_events"], cc["pair_events"]
+    assert a > 500 and abs(a - b) < 5.0 * math.sqrt(a + b)
+    assert cg["photon_steps"] == pytest.approx(cc["photon_steps"], rel=5e-3)
+    for sp in (0, 1):
+        eg, ec = sg["u"][sg["species"] == sp, 0], sc["u"][sc["species"] == sp, 0]
+        assert ks_2samp(eg, ec).pvalue > 1e-3
+    phg, phc = sg["species"] == 2, sc["species"] == 2
+    x, y = sg["pol"][phg, 3], sc["pol"][phc, 3]
+    se = math.sqrt(x.var() / len(x) + y.var() / len(y)) + 1e-15
+    assert abs(x.mean() - y.mean()) < 5 * se + 1e-12

@@ -289,8 +289,16 @@ def test_lp_classical_sampler_statistics(engine, orc, a, eta, fixed_n):
     else:
         top = int(max(got[:, 0].max(), ref[:, 0].max()))
         assert _chi2_two_sample(got[:, 0], ref[:, 0], np.unique(np.geomspace(1, top + 1, 60).astype(int)) - 0.5) > 1e-3
-    for col in (1, 2, 3, 4):
+    for col in (1, 2):
         assert ks_2samp(got[:, col], ref[:, col]).pvalue > 1e-3, col
+    # The reference evaluates J(n, x, y) at the negative x of theta near pi, where its
+    # saddle-point branch returns NaN: those photons carry NaN Stokes parameters, in
+    # the same proportion here as in the oracle; the rest are compared by KS.
+    bad_g, bad_c = ~np.isfinite(got[:, 3]), ~np.isfinite(ref[:, 3])
+    pg, pc = bad_g.mean(), bad_c.mean()
+    assert abs(pg - pc) <= 5.0 * math.sqrt((pg + pc) / m) + 1e-12
+    for col in (3, 4):
+        assert ks_2samp(got[~bad_g, col], ref[~bad_c, col]).pvalue > 1e-3, col
 
 
 def test_lp_classical_shower_matches_oracle(engine, orc):
@@ -316,6 +324,7 @@ def test_lp_classical_shower_matches_oracle(engine, orc):
         assert ks_2samp(q(sg, ph_g), q(sc, ph_c)).pvalue > 1e-3
     for col in (1, 2):
         x, y = sg["pol"][ph_g, col], sc["pol"][ph_c, col]
+        x, y = x[np.isfinite(x)], y[np.isfinite(y)]  # NaN Stokes: see test_lp_classical_sampler_statistics
         se = math.sqrt(x.var() / len(x) + y.var() / len(y)) + 1e-12
         assert abs(x.mean() - y.mean()) < 5 * se, col
     eg, ec = sg["u"][sg["species"] == 0, 0], sc["u"][sc["species"] == 0, 0]

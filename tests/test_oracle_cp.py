@@ -254,3 +254,113 @@ def test_cp_sampler_s_marginal(a, eta, n, classical):
     pol = np.linalg.norm(out[:, 3:6], axis=1)
     assert np.all(pol <= 1.0 + 1e-9)
     assert np.all((out[:, 2] >= 0.0) & (out[:, 2] < 2.0 * math.pi))
+
+
+# ---------------------------------------------------------------- CP nonlinear Breit-Wheeler (pair_creation/cp/mod.rs)
+CPBW_KATS = [
+    (50, 1.0, 0.1, 1.4338498960396018931e-18), (100, 1.0, 0.1, 1.3654528056555865291e-18),
+    (200, 1.0, 0.1, 2.0884399327604375975e-34), (4000, 5.0, 0.02, 9.2620552570985535880e-61),
+    (5000, 5.0, 0.02, 5.9413657401296979089e-17), (6000, 5.0, 0.02, 1.1629168979497463847e-18),
+    (8000, 5.0, 0.02, 1.6722921069034930599e-23), (3, 0.1, 1.0, 2.7926363804797338348e-7),
+    (5, 0.1, 1.0, 1.4215822192894496185e-10), (20, 0.1, 1.0, 1.1028787567587238051e-37),
+    (3000, 10.0, 0.1, 2.8532353822421244101e-48), (4000, 10.0, 0.1, 1.4356925580571873594e-5),
+    (8000, 10.0, 0.1, 1.5847977567888444504e-7), (6, 1.0, 1.0, 0.0031666996194000280745),
+    (20, 1.0, 1.0, 1.2280171339973043031e-5), (50, 1.0, 1.0, 7.7268893728561315057e-11),
+    (300, 10.0, 1.0, 1.6230747656967905300e-7), (400, 10.0, 1.0, 0.0031791285538880908649),
+    (2000, 10.0, 1.0, 5.9533991784012215406e-5), (640, 1.0, 0.01, 9.7351336009776642509e-115),
+    (1000, 1.0, 0.01, 2.3257373741363993054e-156), (25, 0.1, 0.1, 5.7778053795802739886e-52),
+    (50, 0.1, 0.1, 3.3444371706672986244e-90)]
+
+
+@pytest.mark.parametrize("n,a,eta,target", CPBW_KATS)
+def test_cpbw_partial_rate_known_values(n, a, eta, target):
+    """pair_creation/cp/mod.rs:454-499 (partial_rates): the polarisation average, error < 1e-6."""
+    r = orc.cpbw_partial_rate(n, a, eta)
+    assert abs(0.5 * (r[0] + r[1]) - target) / target < 1e-6
+
+
+@pytest.mark.parametrize("i,j", [(0, 6), (0, 9), (0, 13), (2, 4), (2, 9), (3, 3), (3, 8), (4, 2), (4, 7), (5, 4), (5, 10),
+                                 (6, 6), (6, 9), (7, 8), (7, 12), (8, 13)])
+def test_cpbw_table_matches_summation(i, j):
+    """pair_creation/cp/mod.rs:501-523 (total_rates) on the points of its grid
+    with at most 3000 harmonics: the table against the explicit sum, error < 1%."""
+    a, eta = 0.305 * 10.0 ** (i / 5.0), 0.003 * 10.0 ** (j / 5.0)
+    method, table = orc.cpbw_total(a, eta)
+    if method == 0:
+        pytest.skip("negligible rate")
+    hi = orc.C.c_int(0)
+    orc.lib().orc_cpbw_sum_limits(a, eta, orc.C.byref(hi))
+    assert hi.value <= 3000
+    total = orc.cpbw_by_summation(a, eta)
+    assert method == 2
+    assert np.all(np.abs(table - total) / total < 0.01)
+
+
+def test_cpbw_probability_and_stokes():
+    """Probability for S_3 = +1 / -1 from the two tabulated totals, and the Stokes
+    update that keeps the photon ensemble unbiased (pair_creation/cp/mod.rs:286-322):
+    for S_3 = +-1 the parameter stays +-1, and S_1, S_2 scale by (1 - P_avg) / (1 - P)."""
+    L = orc.lib()
+    a, eta, dphi = 3.0, 0.2, 0.1
+    method, f = orc.cpbw_total(a, eta)
+    assert method == 2
+    for sv3, fk in ((1.0, f[0]), (-1.0, f[1])):
+        sv = np.array([1.0, 0.0, 0.0, sv3])
+        out = np.zeros(4)
+        p = L.orc_cpbw_probability(a, eta, orc.dptr(sv), dphi, orc.dptr(out))
+        assert p == pytest.approx(7.29735257e-3 * fk * dphi / eta, rel=1e-14)
+        assert out[3] == pytest.approx(sv3, abs=1e-14)
+    sv = np.array([1.0, 0.3, -0.4, 0.2])
+    out = np.zeros(4)
+    p = L.orc_cpbw_probability(a, eta, orc.dptr(sv), dphi, orc.dptr(out))
+    p_avg = 7.29735257e-3 * 0.5 * (f[0] + f[1]) * dphi / eta
+    assert out[1] == pytest.approx(0.3 * (1 - p_avg) / (1 - p), rel=1e-14)
+    assert out[2] == pytest.approx(-0.4 * (1 - p_avg) / (1 - p), rel=1e-14)
+    # decay-weighted average of S_3 over both outcomes equals the prior S_3
+    p_plus, p_minus = 7.29735257e-3 * f[0] * dphi / eta, 7.29735257e-3 * f[1] * dphi / eta
+    no_decay = (0.6 * (1 - p_plus) - 0.4 * (1 - p_minus)) / (0.6 * (1 - p_plus) + 0.4 * (1 - p_minus))
+    assert out[3] == pytest.approx(no_decay, rel=1e-12)
+
+
+@pytest.mark.parametrize("a,eta,sv3", [(1.0, 0.5, 0.0), (3.0, 0.2, 0.8), (0.2, 1.0, -0.5)])
+def test_cpbw_harmonic_distribution(a, eta, sv3):
+    """Harmonics drawn by explicit summation against the partial rates (chi^2); the
+    rates below a = 0.3 are summed too (pair_creation/cp/mod.rs:271-281)."""
+    from scipy.stats import chisquare
+    L = orc.lib()
+    hi = orc.C.c_int(0)
+    lo = L.orc_cpbw_sum_limits(a, eta, orc.C.byref(hi))
+    pr = np.array([orc.cpbw_partial_rate(n, a, eta) for n in range(lo, hi.value + 1)])
+    w = 0.5 * ((1 + sv3) * pr[:, 0] + (1 - sv3) * pr[:, 1])
+    frac = (np.arange(2000) + 0.5) / 2000
+    n = np.array([L.orc_cpbw_harmonic(a, eta, sv3, f) for f in frac])
+    assert np.all((n >= lo) & (n <= hi.value))
+    h = np.bincount(n - lo, minlength=len(w))[:len(w)]
+    p = w / w.sum()
+    keep = p * len(n) > 5
+    # the inversion is deterministic on evenly spaced fractions; the table total and
+    # the sum differ by < 1%, which shifts the top harmonic's share only
+    assert np.max(np.abs(h[keep] / len(n) - p[keep])) < 0.012
+
+
+@pytest.mark.parametrize("n,a,eta,sv", [(30, 1.0, 0.5, (0.0, 0.0, 0.0)), (300, 3.0, 0.2, (0.5, -0.3, 0.7)),
+                                        (1500, 5.0, 0.2, (0.0, 0.9, -0.2))])
+def test_cpbw_sampler_marginals(n, a, eta, sv):
+    """(s, theta) at fixed n against the double-differential rate's marginals (chi^2)."""
+    from scipy.stats import chisquare
+    L = orc.lib()
+    out = orc.cpbw_sample(a, eta, *sv, 20000, seed=4, fixed_n=n)
+    s_grid = np.linspace(0.0, 1.0, 801)[:-1] + 0.5 / 800
+    th_grid = np.linspace(0.0, 2 * math.pi, 121)[:-1] + math.pi / 120
+    sn = 2.0 * n * eta / (1.0 + a * a)
+    s_min = 0.5 * (1.0 - math.sqrt(1.0 - 4.0 / sn))
+    inside = (s_grid > s_min) & (s_grid < 1.0 - s_min)
+    dens = np.array([[L.orc_cpbw_fully_resolved(n, a, eta, min(s, 1 - s), t, *sv) if ok else 0.0 for t in th_grid]
+                     for s, ok in zip(s_grid, inside)])
+    for col, grid, marg, bins in ((1, s_grid, dens.sum(1), np.linspace(0, 1, 41)),
+                                  (2, th_grid, dens.sum(0), np.linspace(0, 2 * math.pi, 25))):
+        h, _ = np.histogram(out[:, col], bins)
+        p, _ = np.histogram(grid, bins, weights=marg)
+        p = p / p.sum()
+        keep = p * len(out) > 5
+        assert chisquare(h[keep], p[keep] / p[keep].sum() * h[keep].sum()).pvalue > 1e-3, col

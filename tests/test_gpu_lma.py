@@ -329,3 +329,123 @@ def test_lp_classical_shower_matches_oracle(engine, orc):
         assert abs(x.mean() - y.mean()) < 5 * se, col
     eg, ec = sg["u"][sg["species"] == 0, 0], sc["u"][sc["species"] == 0, 0]
     assert ks_2samp(eg, ec).pvalue > 1e-3
+
+
+# ---------------------------------------------------------------- FocusedLaser (focused_laser.rs)
+def _fl_lasers():
+    from ptarmigan_amd import Envelope, Laser, LaserKind, Polarization
+    return {
+        "fl_lp_cos2": Laser(LaserKind.FOCUSED_LASER, 2.0, 0.8e-6, 16.0, waist=4e-6, envelope=Envelope.COS_SQUARED,
+                            polarization=Polarization.LINEAR),
+        "fl_cp_flat": Laser(LaserKind.FOCUSED_LASER, 50.0, 0.8e-6, 8.0, waist=3e-6, envelope=Envelope.FLATTOP,
+                            polarization=Polarization.CIRCULAR),
+        "fl_lp_gauss": Laser(LaserKind.FOCUSED_LASER, 20.0, 0.8e-6, 8.0, waist=2e-6, envelope=Envelope.GAUSSIAN,
+                             polarization=Polarization.LINEAR),
+    }
+
+
+@pytest.mark.parametrize("name", list(_fl_lasers()))
+@pytest.mark.parametrize("rr", [False, True])
+def test_focused_laser_trajectories(engine, orc, name, rr):
+    """Field::push in the focused LMA field (focused_laser.rs:185-225) with the
+    transverse gradient, Lorentz or Landau-Lifshitz, through the whole pulse."""
+    from ptarmigan_amd import CollideOptions
+    laser = _fl_lasers()[name]
+    say("focused push", name, rr)
+    engine.set_laser(laser)
+    dt, z0 = engine.field_info()
+    rng = np.random.default_rng(17)
+    n = 256
+    gam = rng.uniform(100.0, 2e4, n)
+    u = np.stack([gam, rng.normal(0, 1.0, n), rng.normal(0, 1.0, n), -np.sqrt(gam**2 - 1.0)], 1)
+    u[:, 0] = np.sqrt(1.0 + (u[:, 1:] ** 2).sum(1))
+    r = np.stack([-z0 * np.ones(n), rng.normal(0, 2e-6, n), rng.normal(0, 2e-6, n), z0 * np.ones(n)], 1)
+    sp = np.zeros(n, dtype=np.uint8)
+    sp[::3] = 1
+    opts = CollideOptions(classical=True, rr=rr)
+    rg, ug, kg, wg = engine.push(sp, r, u, 200_000, opts, with_work=True)
+    rc, uc, kc, wc = orc.push(laser.desc(), sp, r, u, 200_000, eqn=1 if rr else 0, with_work=True)
+    assert np.array_equal(kg, kc)
+    assert np.abs(ug - uc).max() / np.abs(uc).max() < 1e-12
+    assert np.abs(rg - rc).max() / np.abs(rc).max() < 1e-12
+    assert np.abs(wg - wc).max() <= 1e-12 * np.abs(uc[:, 0]).max()
+
+
+def test_focused_laser_deflection_on_gpu(engine):
+    """focused_laser.rs:353-380 (deflection) through the device pusher, error < 0.5%."""
+    from ptarmigan_amd import CollideOptions, Envelope, Laser, LaserKind, Polarization
+    a0, w0, lam, gamma = 100.0, 4.0e-6, 0.8e-6, 1000.0
+    n_cycles = C_LIGHT * 30.0e-15 / lam
+    laser = Laser(LaserKind.FOCUSED_LASER, a0, lam, n_cycles, waist=w0, envelope=Envelope.GAUSSIAN,
+                  polarization=Polarization.CIRCULAR)
+    engine.set_laser(laser)
+    omega = C_LIGHT * 2.0 * math.pi / lam
+    t_start = -20.0 * lam / C_LIGHT
+    b = np.arange(1, 8) * 1.0e-6
+    u = np.tile([gamma, 0.0, 0.0, -math.sqrt(gamma * gamma - 1.0)], (7, 1))
+    r = np.stack([np.zeros(7), b, np.zeros(7), np.zeros(7)], 1) + u * C_LIGHT * t_start / gamma
+    opts = CollideOptions(classical=True, rr=False, dt_multiplier=0.25 * lam / C_LIGHT * omega)
+    _, u2, _ = engine.push(np.zeros(7, dtype=np.uint8), r, u, 10_000_000, opts)
+    theta = 1000.0 * np.arctan2(u2[:, 1], -u2[:, 3])
+    i_phi = omega * 30.0e-15 * math.sqrt(math.pi / (16.0 * math.log(2.0)))
+    theory = 1000.0 * 2.0 * a0 * a0 * i_phi * b * lam * np.exp(-2.0 * b * b / (w0 * w0)) / (2.0 * math.pi * gamma**2 * w0 * w0)
+    assert np.all(np.abs(theta - theory) / theory < 5.0e-3)
+
+
+@pytest.mark.parametrize("pol,classical", [("linear", False), ("circular", False), ("linear", True)])
+def test_focused_laser_shower_matches_oracle(engine, orc, pol, classical):
+    """FocusedLaser, a0 = 5, w0 = 3 um, 16-cycle cos^2, 10 GeV electrons with a 2 um
+    Gaussian spot (photons final state): emission count, photon spectra and
+    angles, electron energies, against the oracle."""
+    from scipy.stats import ks_2samp
+    from ptarmigan_amd import Beam, CollideOptions, Envelope, Laser, LaserKind, Polarization, Species
+    laser = Laser(LaserKind.FOCUSED_LASER, 5.0, 0.8e-6, 16.0, waist=3e-6, envelope=Envelope.COS_SQUARED,
+                  polarization=Polarization.LINEAR if pol == "linear" else Polarization.CIRCULAR)
+    g = 10000.0 / 0.510999
+    beam = Beam(n=20_000, gamma=g, sigma=0.01 * g, species=Species.ELECTRON, radius=2e-6, rms_div=1e-6)
+    opts = CollideOptions(tracking_photons=False, classical=classical)
+    say("focused shower", pol, classical)
+    engine.set_laser(laser)
+    dt, z0 = engine.field_info()
+    prim = orc.build_beam(beam.desc(), z0, 21)
+    sg, _, cg = engine.collide(prim, opts, seed=21)
+    sc, _, cc, _ = orc.collide(laser.desc(), opts.desc(), prim, 21, 16)
+    say("  gpu", cg, "\n  cpu", cc)
+    assert cg["error_flags"] == 0
+    assert cg["lepton_steps"] == pytest.approx(cc["lepton_steps"], rel=1e-3)
+    a, b = cg["emissions"], cc["emissions"]
+    assert a > 1000 and abs(a - b) < 5.0 * math.sqrt(a + b + 1.0)
+    ph_g, ph_c = sg["species"] == 2, sc["species"] == 2
+    for q in (lambda s, m: s["u"][m, 0], lambda s, m: np.arctan2(s["u"][m, 1], -s["u"][m, 3]),
+              lambda s, m: np.arctan2(s["u"][m, 2], -s["u"][m, 3]), lambda s, m: s["payload"][m]):
+        assert ks_2samp(q(sg, ph_g), q(sc, ph_c)).pvalue > 1e-3
+    eg, ec = sg["u"][sg["species"] == 0, 0], sc["u"][sc["species"] == 0, 0]
+    assert ks_2samp(eg, ec).pvalue > 1e-3
+    xg, xc = sg["u"][sg["species"] == 0, 1], sc["u"][sc["species"] == 0, 1]
+    assert ks_2samp(xg, xc).pvalue > 1e-3
+
+
+def test_focused_laser_pairs_match_oracle(engine, orc):
+    """CP FocusedLaser, a0 = 15, w0 = 3 um, photons and secondaries tracked: the
+    lepton and photon phases of the cascade in the focused LMA field."""
+    from scipy.stats import ks_2samp
+    from ptarmigan_amd import Beam, CollideOptions, Envelope, Laser, LaserKind, Polarization, Species
+    laser = Laser(LaserKind.FOCUSED_LASER, 15.0, 0.8e-6, 16.0, waist=3e-6, envelope=Envelope.COS_SQUARED,
+                  polarization=Polarization.CIRCULAR)
+    g = 16500.0 / 0.510999
+    beam = Beam(n=4000, gamma=g, sigma=0.01 * g, species=Species.ELECTRON, radius=1e-6, rms_div=1e-6)
+    opts = CollideOptions(tracking_photons=True)
+    say("focused cascade")
+    engine.set_laser(laser)
+    dt, z0 = engine.field_info()
+    prim = orc.build_beam(beam.desc(), z0, 22)
+    sg, _, cg = engine.collide(prim, opts, seed=22)
+    sc, _, cc, _ = orc.collide(laser.desc(), opts.desc(), prim, 22, 16)
+    say("  gpu", cg, "\n  cpu", cc)
+    for key in ("emissions", "pair_events"):
+        a, b = cg[key], cc[key]
+        assert abs(a - b) < 5.0 * math.sqrt(a + b + 1.0), key
+    assert cg["pair_events"] + cc["pair_events"] > 20
+    for sp in (0, 2):
+        eg, ec = sg["u"][sg["species"] == sp, 0], sc["u"][sc["species"] == sp, 0]
+        assert ks_2samp(eg, ec).pvalue > 1e-3, sp

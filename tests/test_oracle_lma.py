@@ -381,3 +381,42 @@ def test_lp_classical_sampler_v_marginal(orc, a, n):
     p = p / p.sum()
     keep = p * len(v) > 5
     assert chisquare(h[keep], p[keep] / p[keep].sum() * h[keep].sum()).pvalue > 1e-3
+
+
+# ---------------------------------------------------------------- FocusedLaser (focused_laser.rs)
+C_LIGHT_SI = 2.997925e8
+
+
+def test_focused_laser_on_axis(orc):
+    """focused_laser.rs:329-351 (on_axis): an electron on axis through a flattop CP
+    focused pulse, a0 = 100, leaves with u_perp < 1e-3 and u^2 = 1."""
+    n_cycles = C_LIGHT_SI * 30.0e-15 / 0.8e-6
+    laser = orc.laser_desc(kind=1, envelope=1, polarization=1, a0=100.0, wavelength=0.8e-6, waist=4e-6, n_cycles=n_cycles)
+    t_start = -20.0 * 0.8e-6 / C_LIGHT_SI
+    u = np.array([math.sqrt(1.0 + 1.0e6), 0.0, 0.0, -1000.0])
+    r = u * C_LIGHT_SI * t_start / u[0]
+    r2, u2, steps = orc.push(laser, [0], r[None, :], u[None, :], 10_000_000)
+    u2 = u2[0]
+    assert steps[0] > 100
+    assert u2[1] < 1e-3 and u2[2] < 1e-3
+    assert abs(u2[0] ** 2 - u2[1] ** 2 - u2[2] ** 2 - u2[3] ** 2 - 1.0) < 1e-3
+
+
+def test_focused_laser_deflection(orc):
+    """focused_laser.rs:353-380 (deflection): ponderomotive deflection of 1 GeV-scale
+    electrons at impact parameters 1..7 um against the closed form, error < 0.5%."""
+    a0, w0, lam, gamma = 100.0, 4.0e-6, 0.8e-6, 1000.0
+    n_cycles = C_LIGHT_SI * 30.0e-15 / lam
+    laser = orc.laser_desc(kind=1, envelope=2, polarization=1, a0=a0, wavelength=lam, waist=w0, n_cycles=n_cycles)
+    t_start = -20.0 * lam / C_LIGHT_SI
+    dt_mult = 0.25 * lam / C_LIGHT_SI * (C_LIGHT_SI * 2.0 * math.pi / lam)  # dt = lambda / (4c) over max_timestep = 1 / omega
+    omega = C_LIGHT_SI * 2.0 * math.pi / lam
+    for k in range(1, 8):
+        b = k * 1.0e-6
+        u = np.array([gamma, 0.0, 0.0, -math.sqrt(gamma * gamma - 1.0)])
+        r = np.array([0.0, b, 0.0, 0.0]) + u * C_LIGHT_SI * t_start / u[0]
+        _, u2, _ = orc.push(laser, [0], r[None, :], u[None, :], 10_000_000, dt_multiplier=dt_mult)
+        theta = 1000.0 * math.atan2(u2[0, 1], -u2[0, 3])
+        i_phi = omega * 30.0e-15 * math.sqrt(math.pi / (16.0 * math.log(2.0)))
+        theory = 1000.0 * 2.0 * a0 * a0 * i_phi * b * lam * math.exp(-2.0 * b * b / (w0 * w0)) / (2.0 * math.pi * gamma * gamma * w0 * w0)
+        assert abs(theta - theory) / theory < 5.0e-3, (b, theta, theory)

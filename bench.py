@@ -63,6 +63,36 @@ def cpu_baseline(name, n_primaries, threads):
     return steps / secs, f"{n_primaries} primaries of the same beam ({steps} particle-steps, {secs:.1f} s)"
 
 
+# HBM bytes per lepton step of the dominant kernel, measured with rocprofv3 PMC
+# passes (FETCH_SIZE, WRITE_SIZE) and scaled by a plain copy kernel's readings;
+# the summary and the commands that made it are in this file.
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "traffic_k_leptons_lcfa__focused_lcfa_cascade.md")
+
+
+def traffic_bytes_per_step():
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.strip().startswith("hbm_bytes_per_lepton_step"):
+                    return float(line.split("=")[1].split()[0])
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration(nbytes):
+    """A plain device-to-device copy of nbytes, 5 dispatches: the reference reading
+    for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": (nbytes // 8) * 8, "dispatches": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,7 +104,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="lcfa",
                     help="lcfa: the focused-laser cascade (default, the headline workload); lma: the plane-wave LMA case")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="run only a plain device copy of BYTES (5 dispatches), for counter calibration")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -117,9 +153,11 @@ def main():
     flops = FLOPS_PER_LEPTON_STEP if args.workload == "lcfa" else FLOPS_PER_LMA_STEP
     kname = "k_leptons_lcfa<FastFocusedLaser, linear>" if args.workload == "lcfa" else "k_step_lma"
     achieved = flops * lep_steps_per_launch / (lep_launch_ms * 1e-3) / 1e12
+    bps = traffic_bytes_per_step() if args.workload == "lcfa" else None
+    traffic = None if bps is None else int(round(bps * lep_steps_per_launch))
     roofline = {"bound": "mfma", "pipe": "fp64 vector (VALU); MI355X FP64 vector and matrix peaks are both 78.6",
                 "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
                 "kernel": kname, "kernel_avg_ms": round(lep_launch_ms, 3), "flops_per_step": flops}
 
     cpu = None

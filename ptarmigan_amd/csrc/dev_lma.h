@@ -552,38 +552,61 @@ __device__ inline int lp_sum_limit(double a, double eta) {
     double m = 3.0 - 0.4 * pow(eta, 0.25);
     return int(ceil(6.0 * (1.0 + a * a) + 2.0 * pow(a, m)));
 }
-__device__ inline double lp_ceiling(const DBessel& D, double a, double eta) {
-    int n = D.n;
+// The points at which lp_ceiling samples the double-differential rate
+// (lp/mod.rs:53-140): how many, and the k-th (s, theta). Split out so that a
+// wave can evaluate them in parallel.
+__device__ inline int lp_ceiling_count(int n, double a, double eta) {
+    if (n == 1) return 2;
+    if (a <= 2.0) return 32;
+    if (a <= 5.0) {
+        int n_switch = int(ceil(1.3 * pow(a, 1.2)));
+        if (n < n_switch) return 33;
+        return n < lp_sum_limit(a, eta) / 2 ? 32 : 64;
+    }
+    int n_switch = int(ceil(a * a));
+    if (n < n_switch) return 64;
+    return n > 10 * n_switch ? 64 : (n > 3 * n_switch ? 42 : 32);
+}
+__device__ inline void lp_ceiling_arg(int n, double a, double eta, int k, double& s, double& theta) {
     double sn = 2.0 * double(n) * eta / (1.0 + 0.5 * a * a);
     double smax = sn / (1.0 + sn);
-    double mx = -INFINITY;
+    theta = 0.0;
     if (n == 1) {
-        mx = fmax(lp_ddpr(D, a, eta, 0.01 * smax, PI_D / 2.0), lp_ddpr(D, a, eta, 0.99 * smax, PI_D / 2.0));
+        s = (k == 0 ? 0.01 : 0.99) * smax;
+        theta = PI_D / 2.0;
     } else if (a <= 2.0) {
-        for (int i = 1; i <= 32; ++i) mx = fmax(mx, lp_ddpr(D, a, eta, smax * (0.4 + 0.6 * double(i) / 32.0), 0.0));
+        s = smax * (0.4 + 0.6 * double(k + 1) / 32.0);
     } else if (a <= 5.0) {
         int n_switch = int(ceil(1.3 * pow(a, 1.2)));
         if (n < n_switch) {
-            double theta = (PI_D / 2.0) * (1.0 - log(double(n)) / log(double(n_switch)));
-            for (int i = 0; i <= 32; ++i) mx = fmax(mx, lp_ddpr(D, a, eta, smax * double(i) / 32.0, theta));
+            theta = (PI_D / 2.0) * (1.0 - log(double(n)) / log(double(n_switch)));
+            s = smax * double(k) / 32.0;
         } else {
             double x_min = fmax(4.0 * double(n) * eta / (3.0 * a * a + 4.0 * double(n) * eta), 0.4);
             int m = n < lp_sum_limit(a, eta) / 2 ? 32 : 64;
-            for (int i = 1; i <= m; ++i) mx = fmax(mx, lp_ddpr(D, a, eta, smax * (x_min + (1.0 - x_min) * double(i) / double(m)), 0.0));
+            s = smax * (x_min + (1.0 - x_min) * double(k + 1) / double(m));
         }
     } else {
         int n_switch = int(ceil(a * a));
         if (n < n_switch) {
             double ns = double(n_switch);
-            double theta = (PI_D / 2.0) * (pow(double(n), -0.7) - pow(ns, -0.7)) / (1.0 - pow(ns, -0.7));
-            for (int i = 1; i <= 64; ++i) mx = fmax(mx, lp_ddpr(D, a, eta, smax * (0.3 + 0.7 * double(i) / 64.0), theta));
+            theta = (PI_D / 2.0) * (pow(double(n), -0.7) - pow(ns, -0.7)) / (1.0 - pow(ns, -0.7));
+            s = smax * (0.3 + 0.7 * double(k + 1) / 64.0);
         } else {
             double x_min = fmax(4.0 * double(n) * eta / (3.0 * a * a + 4.0 * double(n) * eta), 0.3);
             double x_peak = (1.0 + sn) / (2.0 + sn);
             int len = n > 10 * n_switch ? 64 : (n > 3 * n_switch ? 42 : 32);
-            for (int i = 1; i <= len; ++i)
-                mx = fmax(mx, lp_ddpr(D, a, eta, smax * (x_min + (x_peak - x_min) * double(i) / double(len)), 0.0));
+            s = smax * (x_min + (x_peak - x_min) * double(k + 1) / double(len));
         }
+    }
+}
+__device__ inline double lp_ceiling(const DBessel& D, double a, double eta) {
+    double mx = -INFINITY;
+    int m = lp_ceiling_count(D.n, a, eta);
+    for (int k = 0; k < m; ++k) {
+        double s, th;
+        lp_ceiling_arg(D.n, a, eta, k, s, th);
+        mx = fmax(mx, lp_ddpr(D, a, eta, s, th));
     }
     return 1.15 * mx;
 }
@@ -624,6 +647,18 @@ __device__ inline ThetaBound theta_bound(int n, double a, double eta) {
         else tb.f[i] = f_min;
     }
     return tb;
+}
+// theta_bound_at over a bound held elsewhere (e.g. one copy per wave in LDS)
+__device__ __forceinline__ double theta_bound_at(const double* ts, const double* tf, double arg) {
+    double val = tf[15];
+    for (int i = 1; i < 16; ++i) {
+        if (arg > ts[i]) {
+            double w = (arg - ts[i - 1]) / (ts[i] - ts[i - 1]);
+            val = w * tf[i] + (1.0 - w) * tf[i - 1];
+            break;
+        }
+    }
+    return fmin(val, PI_D / 2.0);
 }
 __device__ __forceinline__ double theta_bound_at(const ThetaBound& tb, double arg) {
     double val = tb.f[15];
@@ -673,32 +708,31 @@ __device__ inline double lp_cl_ddpr(const DBessel& D, double a, double v, double
     else { g0 = -j[2]; g1 = 0.5 * (j[1] + j[3]); g2 = -0.25 * (j[0] + 2.0 * j[2] + j[4]); }
     return (-g0 * g0 - a * a * (g0 * g2 - g1 * g1)) / (2.0 * PI_D);
 }
-// 1.1 x the largest of up to 64 samples (lp/classical.rs:43-110)
-__device__ inline double lp_cl_ceiling(const DBessel& D, double a) {
-    int n = D.n;
-    double mx = 0.0;
+// The points at which the classical ceiling samples the rate (lp/classical.rs:43-110)
+__device__ inline int lp_cl_ceiling_count(int n, double a) {
+    if (n == 1) return 1;
+    if (a <= 1.5) return 16;
+    if (a < 5.0) return a > 4.0 ? 32 : 16;
+    return a > 8.0 ? 64 : 32;
+}
+__device__ inline void lp_cl_ceiling_arg(int n, double a, int k, double& v, double& theta) {
+    theta = 0.0;
     if (n == 1) {
-        mx = lp_cl_ddpr(D, a, 0.01, PI_D / 2.0);
+        v = 0.01;
+        theta = PI_D / 2.0;
     } else if (a <= 1.5) {
         double v_min = 0.1 + (9.0 / sqrt(double(n)) - 1.0) / 10.0;
         double v_max = fmin(v_min + 0.4, 1.0);
-        for (int i = 1; i <= 16; ++i) {
-            double f = lp_cl_ddpr(D, a, v_min + (v_max - v_min) * double(i) / 16.0, 0.0);
-            mx = i == 1 ? f : fmax(mx, f);
-        }
+        v = v_min + (v_max - v_min) * double(k + 1) / 16.0;
     } else if (a < 5.0) {
         int n_switch = int(3.0 * pow(a / 2.0, 4.0 / 3.0));
-        double theta = n > n_switch ? 0.0 : (PI_D / 2.0) * (1.0 - log(double(n)) / log(double(n_switch)));
+        theta = n > n_switch ? 0.0 : (PI_D / 2.0) * (1.0 - log(double(n)) / log(double(n_switch)));
         double v_min = fmin(0.35 + (7.0 / sqrt(double(n)) - 1.0) / 9.0, 0.55);
         double v_max = 0.55 + (7.0 / sqrt(double(n)) - 1.0) / 9.0;
         int num = a > 4.0 ? 32 : 16;
-        for (int i = 0; i < num; ++i) {
-            double f = lp_cl_ddpr(D, a, v_min + (v_max - v_min) * double(i) / double(num), theta);
-            mx = i == 0 ? f : fmax(mx, f);
-        }
+        v = v_min + (v_max - v_min) * double(k) / double(num);
     } else {
         double n_switch = 10.0 * ((a / 5.0) * (a / 5.0));
-        double theta = 0.0;
         if (!(n > int(n_switch))) {
             double pw = 0.3 + (0.7 - 0.3) * (log(a) - log(5.0)) / (log(20.0) - log(5.0));
             theta = (PI_D / 2.0) * (pow(double(n), -pw) - pow(n_switch, -pw)) / (1.0 - pow(n_switch, -pw));
@@ -706,10 +740,17 @@ __device__ inline double lp_cl_ceiling(const DBessel& D, double a) {
         double v_min = 0.3 + 0.4 * log(double(n)) * pow(double(n), -0.7);
         double v_max = v_min + 0.2;
         int num = a > 8.0 ? 64 : 32;
-        for (int i = 0; i < num; ++i) {
-            double f = lp_cl_ddpr(D, a, v_min + (v_max - v_min) * double(i) / double(num), theta);
-            mx = i == 0 ? f : fmax(mx, f);
-        }
+        v = v_min + (v_max - v_min) * double(k) / double(num);
+    }
+}
+// 1.1 x the largest of up to 64 samples (lp/classical.rs:43-110)
+__device__ inline double lp_cl_ceiling(const DBessel& D, double a) {
+    double mx = NAN;
+    int m = lp_cl_ceiling_count(D.n, a);
+    for (int k = 0; k < m; ++k) {
+        double v, th;
+        lp_cl_ceiling_arg(D.n, a, k, v, th);
+        mx = fmax(mx, lp_cl_ddpr(D, a, v, th));
     }
     return 1.1 * mx;
 }
@@ -744,6 +785,27 @@ __device__ inline int lp_cl_harmonic(const double* P, const LmaScalars& s, doubl
     n = ceil(n);
     return (n >= 1.0 && n <= 1.0e7) ? int(n) : 0;
 }
+// Stokes parameters of a classical LP photon at (v, theta) (lp/classical.rs:296-330).
+// The reference evaluates J at the signed x of the final quadrant and applies the
+// parity of a negative argument for every quadrant (lp/classical.rs:300-314); so here.
+__device__ inline sv4 lp_cl_stokes(const DBessel& D, double a, double v, double theta) {
+    int n = D.n;
+    double x = 2.0 * double(n) * a * cos(theta) * sqrt(v * (1.0 - v) / (1.0 + 0.5 * a * a));
+    double y = 0.25 * double(n) * a * a * v / (1.0 + 0.5 * a * a);
+    double j[5];
+    dbessel_eval(D, x, y, j);
+    double A0, A1, A2;
+    if (n % 2 == 0) { A0 = j[2]; A1 = -0.5 * (j[1] + j[3]); A2 = 0.25 * (j[0] + 2.0 * j[2] + j[4]); }
+    else { A0 = -j[2]; A1 = 0.5 * (j[1] + j[3]); A2 = -0.25 * (j[0] + 2.0 * j[2] + j[4]); }
+    double u = sqrt((1.0 + 0.5 * a * a) * (1.0 - v) / v);
+    double st_, ct_;
+    sincos(theta, &st_, &ct_);
+    double A0a = A0 / a;
+    double xi0 = 2.0 * (A1 * A1 - A0 * A2) - 2.0 * A0a * A0a;
+    double xi1 = 2.0 * (A1 * A1 - A0 * A2) - 2.0 * (1.0 + 2.0 * u * u * st_ * st_) * A0a * A0a;
+    double xi2 = 2.0 * (u * A0a) * (u * A0a) * (2.0 * st_ * ct_) + 4.0 * u * A0 * A1 * st_ / a;
+    return {1.0, xi1 / xi0, xi2 / xi0, 0.0};
+}
 // (s, theta, Stokes) at harmonic n (lp/classical.rs:254-333)
 __device__ inline void lp_cl_sample(const DBessel& D, double a, double eta, Rng& rng, double& s_out, double& theta_out, sv4& pol,
                                     uint32_t& trials, bool& capped) {
@@ -766,26 +828,34 @@ __device__ inline void lp_cl_sample(const DBessel& D, double a, double eta, Rng&
     if (quadrant == 1) theta = PI_D - theta;
     else if (quadrant == 2) theta = PI_D + theta;
     else if (quadrant == 3) theta = 2.0 * PI_D - theta;
-    double x = 2.0 * double(n) * a * cos(theta) * sqrt(v * (1.0 - v) / (1.0 + 0.5 * a * a));
-    double y = 0.25 * double(n) * a * a * v / (1.0 + 0.5 * a * a);
-    // the reference evaluates J at the signed x of the final quadrant and applies the
-    // parity of a negative argument for every quadrant (lp/classical.rs:300-314); so here
+    s_out = v * smax;
+    theta_out = theta;
+    pol = lp_cl_stokes(D, a, v, theta);
+}
+
+// Stokes parameters of an LP photon at (s, theta) (lp/mod.rs:626-676)
+__device__ inline sv4 lp_stokes(const DBessel& D, double a, double eta, double s, double theta) {
+    int n = D.n;
+    double sn = 2.0 * double(n) * eta / (1.0 + 0.5 * a * a);
+    double wn = fmin(s / (sn * (1.0 - s)), 1.0);
+    double x = -2.0 * double(n) * a * cos(theta) * sqrt(wn * (1.0 - wn)) / sqrt(1.0 + 0.5 * a * a);
+    double y = a * a * s / (8.0 * eta * (1.0 - s));
     double j[5];
-    dbessel_eval(D, x, y, j);
+    dbessel_eval(D, fabs(x), y, j);
     double A0, A1, A2;
-    if (n % 2 == 0) { A0 = j[2]; A1 = -0.5 * (j[1] + j[3]); A2 = 0.25 * (j[0] + 2.0 * j[2] + j[4]); }
+    if (x > 0.0) { A0 = j[2]; A1 = 0.5 * (j[1] + j[3]); A2 = 0.25 * (j[0] + 2.0 * j[2] + j[4]); }
+    else if (n % 2 == 0) { A0 = j[2]; A1 = -0.5 * (j[1] + j[3]); A2 = 0.25 * (j[0] + 2.0 * j[2] + j[4]); }
     else { A0 = -j[2]; A1 = 0.5 * (j[1] + j[3]); A2 = -0.25 * (j[0] + 2.0 * j[2] + j[4]); }
-    double u = sqrt((1.0 + 0.5 * a * a) * (1.0 - v) / v);
+    double u = sqrt(2.0 * double(n) * eta * (1.0 - s) / s - (1.0 + 0.5 * a * a));
     double st_, ct_;
     sincos(theta, &st_, &ct_);
     double A0a = A0 / a;
-    double xi0 = 2.0 * (A1 * A1 - A0 * A2) - 2.0 * A0a * A0a;
+    double xi0 = (1.0 - s + 1.0 / (1.0 - s)) * (A1 * A1 - A0 * A2) - 2.0 * A0a * A0a;
     double xi1 = 2.0 * (A1 * A1 - A0 * A2) - 2.0 * (1.0 + 2.0 * u * u * st_ * st_) * A0a * A0a;
-    double xi2 = 2.0 * (u * A0a) * (u * A0a) * (2.0 * st_ * ct_) + 4.0 * u * A0 * A1 * st_ / a;
-    s_out = v * smax;
-    theta_out = theta;
-    pol = {1.0, xi1 / xi0, xi2 / xi0, 0.0};
+    double xi2 = -(2.0 * (u * A0a) * (u * A0a) * (2.0 * st_ * ct_) + 4.0 * u * A0 * A1 * st_ / a);
+    return {1.0, xi1 / xi0, xi2 / xi0, 0.0};
 }
+
 
 // (n, s, theta, Stokes) of one LP emission (lp/mod.rs:586-676). `trial_cap`
 // bounds the rejection loop; hitting it sets `capped` and keeps the last trial.
@@ -816,26 +886,10 @@ __device__ inline void lp_sample(const double* cdf, const LmaScalars& sc, double
     if (quadrant == 1) theta = PI_D - theta;
     else if (quadrant == 2) theta = PI_D + theta;
     else if (quadrant == 3) theta = 2.0 * PI_D - theta;
-    double wn = fmin(s / (sn * (1.0 - s)), 1.0);
-    double x = -2.0 * double(n) * a * cos(theta) * sqrt(wn * (1.0 - wn)) / sqrt(1.0 + 0.5 * a * a);
-    double y = a * a * s / (8.0 * eta * (1.0 - s));
-    double j[5];
-    dbessel_eval(D, fabs(x), y, j);
-    double A0, A1, A2;
-    if (x > 0.0) { A0 = j[2]; A1 = 0.5 * (j[1] + j[3]); A2 = 0.25 * (j[0] + 2.0 * j[2] + j[4]); }
-    else if (n % 2 == 0) { A0 = j[2]; A1 = -0.5 * (j[1] + j[3]); A2 = 0.25 * (j[0] + 2.0 * j[2] + j[4]); }
-    else { A0 = -j[2]; A1 = 0.5 * (j[1] + j[3]); A2 = -0.25 * (j[0] + 2.0 * j[2] + j[4]); }
-    double u = sqrt(2.0 * double(n) * eta * (1.0 - s) / s - (1.0 + 0.5 * a * a));
-    double st_, ct_;
-    sincos(theta, &st_, &ct_);
-    double A0a = A0 / a;
-    double xi0 = (1.0 - s + 1.0 / (1.0 - s)) * (A1 * A1 - A0 * A2) - 2.0 * A0a * A0a;
-    double xi1 = 2.0 * (A1 * A1 - A0 * A2) - 2.0 * (1.0 + 2.0 * u * u * st_ * st_) * A0a * A0a;
-    double xi2 = -(2.0 * (u * A0a) * (u * A0a) * (2.0 * st_ * ct_) + 4.0 * u * A0 * A1 * st_ / a);
     n_out = n;
     s_out = s;
     theta_out = theta;
-    pol = {1.0, xi1 / xi0, xi2 / xi0, 0.0};
+    pol = lp_stokes(D, a, eta, s, theta);
 }
 
 // ---------------------------------------------------------------- photon from the ZMF (nonlinear_compton/mod.rs:58-138)

@@ -713,7 +713,9 @@ static int lma_event_pass(ptg_ctx* c, LmaArgs m, bool pairs, size_t n_ev, size_t
     m.ovf_list = c->d_ovf;
     m.ovf_ctr = c->d_ctr + C_OVF;
     m.ovf_len = c->d_ctr + C_OVFLEN;
-    size_t lanes = std::min<size_t>(max_lanes, ((n_ev + 63) / 64) * 64);
+    // LP emission runs one event per wave (k_event_lma_wave), the others one per lane
+    const bool per_wave = !pairs && (m.mode == LMA_LP || m.mode == LMA_LP_CLASSICAL);
+    size_t lanes = per_wave ? std::min<size_t>(max_lanes, n_ev * 64) : std::min<size_t>(max_lanes, ((n_ev + 63) / 64) * 64);
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     if (pairs) launch_pair_event_lma(c->d_args, m, int(lanes), c->stream);
     else launch_event_lma(c->d_args, m, int(lanes), c->stream);
@@ -728,7 +730,7 @@ static int lma_event_pass(ptg_ctx* c, LmaArgs m, bool pairs, size_t n_ev, size_t
     ++*launches;
     if (ovf[0] == 0) return PTG_OK;
     const size_t per_lane = size_t(ovf[1]) * 5 * sizeof(double);
-    size_t olanes = std::min<size_t>(((ovf[0] + 63) / 64) * 64, ((size_t(8) << 30) / per_lane / 64) * 64);
+    size_t olanes = std::min<size_t>(per_wave ? ovf[0] * 64 : ((ovf[0] + 63) / 64) * 64, ((size_t(8) << 30) / per_lane / 64) * 64);
     olanes = std::max<size_t>(olanes, 64);
     size_t bytes = olanes * per_lane;
     if (bytes > (size_t(16) << 30)) return fail(c, PTG_E_DOMAIN, "LMA harmonic index beyond the supported range");
@@ -784,8 +786,9 @@ static int run_lma(ptg_ctx* c, TransportArgs a, double* ms, uint64_t* launches, 
         double xm = ncap * 1.4142135623730951, ym = ncap * 0.5;
         len_cap = int(std::ceil(xm)) - int(std::floor(-2.0 * ym - xm)) + 200;
     }
-    const size_t budget = size_t(4) << 30;
-    size_t max_lanes = std::max<size_t>(64, std::min<size_t>(65536, budget / (size_t(len_cap) * 40)));
+    // J workspace: up to 8 GB, up to 4096 waves (16 per CU) of per-lane workspaces
+    const size_t budget = size_t(8) << 30;
+    size_t max_lanes = std::max<size_t>(64, std::min<size_t>(262144, budget / (size_t(len_cap) * 40)));
     max_lanes = (max_lanes / 64) * 64;
     size_t need = max_lanes * size_t(len_cap) * 5 * sizeof(double);
     if (need > c->scratch_bytes) {

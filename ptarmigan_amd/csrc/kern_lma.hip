@@ -131,6 +131,78 @@ __global__ __launch_bounds__(256) void k_step_lma(const TransportArgs* __restric
     if (flags) atomicOr(A->flags, flags);
 }
 
+// The rest of an emission once (n, s, theta, Stokes) are sampled (nonlinear_compton/mod.rs:73-136,
+// src/main.rs:172-200): the photon built in the zero-momentum frame and boosted back,
+// written out (final state or photon queue), and the electron, recoiled or not,
+// re-queued with its next RNG segment. `ok` false: no photon, the electron continues.
+template <bool classical>
+__device__ inline void lma_emit_finish(const TransportArgs* __restrict__ A, const LmaArgs& M, uint32_t i, double k0, double a, double eta,
+                                       bool ok, int n, double s, double cphi, sv4 sv, uint64_t& n_emit, uint32_t& flags) {
+    const PBuf& q = M.ev_in;
+    d4 r{q.d[P_RT][i], q.d[P_RX][i], q.d[P_RY][i], q.d[P_RZ][i]};
+    d4 u{q.d[P_UT][i], q.d[P_UX][i], q.d[P_UY][i], q.d[P_UZ][i]};
+    double a_field = M.ev_a[i];
+    uint32_t nev = q.nev[i];
+    d4 u_new = u;
+    double work = q.d[P_WORK][i], icount = q.d[P_ICOUNT][i];
+    if (ok) {
+        // photon momentum in the zero-momentum frame, boosted back (mod.rs:73-116)
+        double ell = double(n);
+        double p_zmf = classical ? ell * eta / sqrt(1.0 + a * a) : ell * eta / sqrt(1.0 + a * a + 2.0 * ell * eta);
+        double cos_theta_zmf = classical ? 1.0 - s * (1.0 + a * a) / (ell * eta)
+                                         : 1.0 - s * (1.0 + a * a + 2.0 * ell * eta) / (ell * eta);
+        // the ZMF moves with q (classical) or q + n kappa (quantum); the photon's axis
+        // is along n kappa or against q there
+        d4 qk = classical ? u : d4{u.t + ell * k0, u.x, u.y, u.z + ell * k0};
+        double inv = 1.0 / sqrt(mdot(qk, qk));
+        d4 u_zmf{qk.t * inv, qk.x * inv, qk.y * inv, qk.z * inv};
+        d3 along = classical ? unit(sp(boost4(d4{ell * k0, 0.0, 0.0, ell * k0}, u_zmf))) : -unit(sp(boost4(u, u_zmf)));
+        d3 eps = unit(sp(boost4(d4{0.0, M.cos_pa, M.sin_pa, 0.0}, u_zmf)));
+        eps = unit(cross(along, cross(eps, along)));
+        double sp_, cp_;
+        sincos(cphi, &sp_, &cp_);
+        d3 perp = rotate(eps, along, sp_, cp_);
+        d3 kp = p_zmf * (cos_theta_zmf * along + sqrt(1.0 - cos_theta_zmf * cos_theta_zmf) * perp);
+        d4 kz{sqrt(nsq(kp)), kp.x, kp.y, kp.z};
+        d4 k = boost4(kz, d4{u_zmf.t, -u_zmf.x, -u_zmf.y, -u_zmf.z});
+        sv = stokes_from_lma(stokes_rotate(sv, -M.pol_angle), k);
+        // sanity check of the construction (mod.rs:118-135)
+        double s_new = (k0 * (k.t - k.z)) / eta;
+        if (!(fabs(s - s_new) / s < 1.0e-2)) flags |= PTG_FLAG_SANITY;
+        double chi = a_field * eta;
+        // photon (src/main.rs:172-189)
+        const uint32_t shower = q.shower[i];
+        uint64_t id = A->id_base + atomicAdd(A->id_ctr, 1ull);
+        uint64_t pstream = rng_child(q.stream[i], nev);
+        if (M.photons_to_out) {
+            if (passes_cut(*A, shower, M_E_MEV * k.t, sp(k))) {
+                uint64_t j = atomicAdd(A->out_ctr, 1ull);
+                write_new(A->out, j, 2u, r, k, d4{0.0, 0.0, 0.0, 0.0}, sv, a_field, chi, q.d[P_WEIGHT][i], id, q.id[i],
+                          pstream, shower);
+                if (A->opt.multiplicity_on) atomicAdd(A->shower_count + shower, 1u);
+            }
+        } else {
+            uint64_t j = atomicAdd(M.ph_ctr, 1ull);
+            if (j < M.ph_cap)
+                write_new(M.ph, j, 2u, r, k, d4{0.0, 0.0, 0.0, 0.0}, sv, a_field, chi, q.d[P_WEIGHT][i], id, q.id[i],
+                          pstream, shower);
+        }
+        ++n_emit;
+        if (A->recoil) u_new = {u.t + ell * k0 - k.t, u.x - k.x, u.y - k.y, u.z + ell * k0 - k.z};
+        icount += 1.0;
+        if (!classical) work += ell * k0;  // absorption (src/main.rs:198-200)
+    }
+    // the electron continues with its next RNG segment
+    unsigned long long j = atomicAdd(M.next_ctr, 1ull);
+    const PBuf& o = M.next;
+    copy_record(o, j, q, i);
+    o.d[P_RT][j] = r.t; o.d[P_RX][j] = r.x; o.d[P_RY][j] = r.y; o.d[P_RZ][j] = r.z;
+    o.d[P_UT][j] = u_new.t; o.d[P_UX][j] = u_new.x; o.d[P_UY][j] = u_new.y; o.d[P_UZ][j] = u_new.z;
+    o.d[P_WORK][j] = work;
+    o.d[P_ICOUNT][j] = icount;
+    o.nev[j] = nev + 1;
+}
+
 // One emission per lane (nonlinear_compton::generate, mod.rs:58-138, with
 // lp::sample, lp/mod.rs:586-676, cp::sample, cp/mod.rs:181-234, or
 // cp::classical::sample, cp/classical.rs:128-178).
@@ -147,16 +219,15 @@ __global__ __launch_bounds__(64) void k_event_lma(const TransportArgs* __restric
     for (uint64_t e = lane; e < M.n_ev; e += n_lanes) {
         const PBuf& q = M.ev_in;
         uint32_t i = M.ev_list ? M.ev_list[e] : uint32_t(e);
-        d4 r{q.d[P_RT][i], q.d[P_RX][i], q.d[P_RY][i], q.d[P_RZ][i]};
         d4 u{q.d[P_UT][i], q.d[P_UX][i], q.d[P_UY][i], q.d[P_UZ][i]};
-        double k0 = M.ev_k0[i], a_field = M.ev_a[i];
+        double k0 = M.ev_k0[i];
         uint32_t nev = q.nev[i];
         Rng rng = rng_make(A->key, q.stream[i]);
         rng.c1 = nev | EVENT_BLOCK;
         double a = sqrt(u.t * u.t - u.x * u.x - u.y * u.y - u.z * u.z - 1.0);  // rms a
         double eta = k0 * (u.t - u.z);
         double alp = a * 1.4142135623730951;
-        int n;
+        int n = 0;
         double s, cphi;
         sv4 sv;
         uint32_t tr = 0;
@@ -196,68 +267,164 @@ __global__ __launch_bounds__(64) void k_event_lma(const TransportArgs* __restric
         }
         trials += tr;
         if (capped) flags |= PTG_FLAG_REJECT_CAP;
-        d4 u_new = u;
-        double work = q.d[P_WORK][i], icount = q.d[P_ICOUNT][i];
-        if (n_cap_ok) {
-            // photon momentum in the zero-momentum frame, boosted back (mod.rs:73-116)
-            double ell = double(n);
-            double p_zmf = classical ? ell * eta / sqrt(1.0 + a * a) : ell * eta / sqrt(1.0 + a * a + 2.0 * ell * eta);
-            double cos_theta_zmf = classical ? 1.0 - s * (1.0 + a * a) / (ell * eta)
-                                             : 1.0 - s * (1.0 + a * a + 2.0 * ell * eta) / (ell * eta);
-            // the ZMF moves with q (classical) or q + n kappa (quantum); the photon's axis
-            // is along n kappa or against q there
-            d4 qk = classical ? u : d4{u.t + ell * k0, u.x, u.y, u.z + ell * k0};
-            double inv = 1.0 / sqrt(mdot(qk, qk));
-            d4 u_zmf{qk.t * inv, qk.x * inv, qk.y * inv, qk.z * inv};
-            d3 along = classical ? unit(sp(boost4(d4{ell * k0, 0.0, 0.0, ell * k0}, u_zmf))) : -unit(sp(boost4(u, u_zmf)));
-            d3 eps = unit(sp(boost4(d4{0.0, M.cos_pa, M.sin_pa, 0.0}, u_zmf)));
-            eps = unit(cross(along, cross(eps, along)));
-            double sp_, cp_;
-            sincos(cphi, &sp_, &cp_);
-            d3 perp = rotate(eps, along, sp_, cp_);
-            d3 kp = p_zmf * (cos_theta_zmf * along + sqrt(1.0 - cos_theta_zmf * cos_theta_zmf) * perp);
-            d4 kz{sqrt(nsq(kp)), kp.x, kp.y, kp.z};
-            d4 k = boost4(kz, d4{u_zmf.t, -u_zmf.x, -u_zmf.y, -u_zmf.z});
-            sv = stokes_from_lma(stokes_rotate(sv, -M.pol_angle), k);
-            // sanity check of the construction (mod.rs:118-135)
-            double s_new = (k0 * (k.t - k.z)) / eta;
-            if (!(fabs(s - s_new) / s < 1.0e-2)) flags |= PTG_FLAG_SANITY;
-            double chi = a_field * eta;
-            // photon (src/main.rs:172-189)
-            const uint32_t shower = q.shower[i];
-            uint64_t id = A->id_base + atomicAdd(A->id_ctr, 1ull);
-            uint64_t pstream = rng_child(q.stream[i], nev);
-            if (M.photons_to_out) {
-                if (passes_cut(*A, shower, M_E_MEV * k.t, sp(k))) {
-                    uint64_t j = atomicAdd(A->out_ctr, 1ull);
-                    write_new(A->out, j, 2u, r, k, d4{0.0, 0.0, 0.0, 0.0}, sv, a_field, chi, q.d[P_WEIGHT][i], id, q.id[i],
-                              pstream, shower);
-                    if (A->opt.multiplicity_on) atomicAdd(A->shower_count + shower, 1u);
-                }
-            } else {
-                uint64_t j = atomicAdd(M.ph_ctr, 1ull);
-                if (j < M.ph_cap)
-                    write_new(M.ph, j, 2u, r, k, d4{0.0, 0.0, 0.0, 0.0}, sv, a_field, chi, q.d[P_WEIGHT][i], id, q.id[i],
-                              pstream, shower);
-            }
-            ++n_emit;
-            if (A->recoil) u_new = {u.t + ell * k0 - k.t, u.x - k.x, u.y - k.y, u.z + ell * k0 - k.z};
-            icount += 1.0;
-            if (!classical) work += ell * k0;  // absorption (src/main.rs:198-200)
-        }
-        // the electron continues with its next RNG segment
-        unsigned long long j = atomicAdd(M.next_ctr, 1ull);
-        const PBuf& o = M.next;
-        copy_record(o, j, q, i);
-        o.d[P_RT][j] = r.t; o.d[P_RX][j] = r.x; o.d[P_RY][j] = r.y; o.d[P_RZ][j] = r.z;
-        o.d[P_UT][j] = u_new.t; o.d[P_UX][j] = u_new.x; o.d[P_UY][j] = u_new.y; o.d[P_UZ][j] = u_new.z;
-        o.d[P_WORK][j] = work;
-        o.d[P_ICOUNT][j] = icount;
-        o.nev[j] = nev + 1;
+        lma_emit_finish<classical>(A, M, i, k0, a, eta, n_cap_ok != 0, n, s, cphi, sv, n_emit, flags);
     }
     wave_add(A->stats + STAT_EMISSIONS, n_emit);
     wave_add(A->stats + STAT_TRIALS, trials);
     if (flags) atomicOr(A->flags, flags);
+}
+
+__device__ __forceinline__ double wave_fmax(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// One emission per wave, for LP emission (quantum or classical), where every
+// trial costs a J(n, x, y) evaluation: the 64 lanes evaluate the ceiling's
+// sample points together (lp/mod.rs:53-140, lp/classical.rs:43-110), then
+// draw (s, theta) candidates in rounds of 8, 32, 64, ... and take the first
+// accepted one in candidate order. Candidates are independent draws (candidate
+// t uses counters 16 + 2t, 17 + 2t of the event's RNG block), so the first
+// accepted one has the distribution of sequential rejection sampling; an
+// event's draws do not depend on how many waves run or on their order.
+#ifndef PTG_LMA_WAVE_WPE
+#define PTG_LMA_WAVE_WPE 2  // two waves per SIMD: 256 VGPRs, a little scratch
+#endif
+template <int MODE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PTG_LMA_WAVE_WPE))) void k_event_lma_wave(const TransportArgs* __restrict__ A, LmaArgs M) {
+    constexpr bool classical = MODE == LMA_LP_CLASSICAL;
+    const int lane = threadIdx.x;
+    const size_t st = size_t(gridDim.x) * 64;  // per-lane workspaces, lane-interleaved
+    double* ws = M.scratch + size_t(blockIdx.x) * 64 + lane;
+    const double* cdf = M.tables + lma_pack::CDF;
+    const uint32_t cap = 1u << 20;
+    __shared__ double tb_s[16], tb_f[16];  // the event's theta bound, one copy per wave
+    uint64_t n_emit = 0, trials = 0;
+    uint32_t flags = 0;
+    for (uint64_t e = blockIdx.x; e < M.n_ev; e += gridDim.x) {
+        __syncthreads();  // the previous event's bound is no longer read
+        const PBuf& q = M.ev_in;
+        const uint32_t i = M.ev_list ? M.ev_list[e] : uint32_t(e);
+        d4 u{q.d[P_UT][i], q.d[P_UX][i], q.d[P_UY][i], q.d[P_UZ][i]};
+        const double k0 = M.ev_k0[i];
+        Rng base = rng_make(A->key, q.stream[i]);
+        base.c1 = q.nev[i] | EVENT_BLOCK;
+        const double a = sqrt(u.t * u.t - u.x * u.x - u.y * u.y - u.z * u.z - 1.0);  // rms a
+        const double eta = k0 * (u.t - u.z);
+        const double alp = a * 1.4142135623730951;
+        Rng g = base;  // counter 0: the harmonic
+        const double fr = rng_uniform(g);
+        const int nh = classical ? lp_cl_harmonic(M.tables, M.sc, alp, fr) : lp_harmonic(cdf, M.sc, alp, eta, fr);
+        bool ok = nh >= 1;
+        if (!ok) {
+            flags |= PTG_FLAG_TABLE_RANGE;
+        } else if (dbessel_len(nh) > M.scratch_len) {
+            if (M.ovf_list) {  // defer: no output now, the host re-runs this event with a larger workspace
+                if (lane == 0) {
+                    M.ovf_list[atomicAdd(M.ovf_ctr, 1ull)] = i;
+                    atomicMax(M.ovf_len, (unsigned long long)dbessel_len(nh));
+                }
+                continue;
+            }
+            flags |= PTG_FLAG_TABLE_RANGE;
+            ok = false;
+        }
+        double s = 0.0, theta = 0.0;
+        sv4 sv{1.0, 0.0, 0.0, 0.0};
+        if (ok) {
+            DBessel D = dbessel_at(nh, ws, st);
+            const double sn = 2.0 * double(nh) * eta / (1.0 + 0.5 * alp * alp);
+            const double smax = classical ? sn : sn / (1.0 + sn);  // s = v * smax
+            if (lane == 0) {
+                const ThetaBound tb = classical ? theta_bound_low_eta(nh, alp) : theta_bound(nh, alp, eta);
+                for (int k = 0; k < 16; ++k) { tb_s[k] = tb.s[k]; tb_f[k] = tb.f[k]; }
+            }
+            __syncthreads();
+            // A candidate t: (v, theta) from counter 16 + 2t, its acceptance draw from 17 + 2t.
+            // f is the rate at the candidate (not needed if theta is out of bounds).
+            auto candidate = [&](uint32_t t, double& v, double& th, double& uz, double& fc) -> bool {
+                Rng gc = base;
+                gc.c0 = 16u + 2u * t;
+                rng_uniform2(gc, v, th);
+                th *= PI_D / 2.0;
+                if (th > theta_bound_at(tb_s, tb_f, classical ? v : smax * v)) return false;
+                uz = rng_uniform(gc);
+                fc = classical ? lp_cl_ddpr(D, alp, v, th) : lp_ddpr(D, alp, eta, smax * v, th);
+                return true;
+            };
+            // round 0: the ceiling's sample points on lanes [0, m), the first 64 - m
+            // candidates on the other lanes (their rates do not depend on the ceiling)
+            const int m = classical ? lp_cl_ceiling_count(nh, alp) : lp_ceiling_count(nh, alp, eta);
+            double f = classical ? NAN : -INFINITY;
+            double v = 0.0, th = 0.0, uz = 0.0, fc = 0.0;
+            bool in_bound = false;
+            for (int k = lane; k < m; k += 64) {
+                double sk, tk;
+                if (classical) {
+                    lp_cl_ceiling_arg(nh, alp, k, sk, tk);
+                    f = fmax(f, lp_cl_ddpr(D, alp, sk, tk));
+                } else {
+                    lp_ceiling_arg(nh, alp, eta, k, sk, tk);
+                    f = fmax(f, lp_ddpr(D, alp, eta, sk, tk));
+                }
+            }
+            if (lane >= m) in_bound = candidate(uint32_t(lane - m), v, th, uz, fc);
+            const double mx = (classical ? 1.1 : 1.15) * wave_fmax(f);
+            uint32_t t0 = uint32_t(m < 64 ? 64 - m : 0);  // candidates consumed
+            int first = -1;
+            double v_acc = 0.0, th_acc = 0.0;
+            {
+                const uint64_t mask = __ballot(lane >= m && in_bound && mx * uz < fc);
+                if (mask) {
+                    first = __ffsll((long long)mask) - 1;
+                    v_acc = __shfl(v, first, 64);
+                    th_acc = __shfl(th, first, 64);
+                    trials += uint32_t(first - m) + 1;
+                }
+            }
+            uint32_t width = 8;
+            while (first < 0) {
+                bool acc = false;
+                const uint32_t t = t0 + uint32_t(lane);
+                if (uint32_t(lane) < width && t < cap && candidate(t, v, th, uz, fc)) acc = mx * uz < fc;
+                const uint64_t mask = __ballot(acc);
+                if (mask) {
+                    first = __ffsll((long long)mask) - 1;
+                    v_acc = __shfl(v, first, 64);
+                    th_acc = __shfl(th, first, 64);
+                    trials += t0 + uint32_t(first) + 1;
+                } else {
+                    t0 += width;
+                    width = width < 64 ? 4 * width : 64;
+                    if (t0 >= cap) {  // as the sequential sampler: give up and keep candidate `cap`
+                        flags |= PTG_FLAG_REJECT_CAP;
+                        Rng gc = base;
+                        gc.c0 = 16u + 2u * cap;
+                        rng_uniform2(gc, v_acc, th_acc);
+                        th_acc *= PI_D / 2.0;
+                        trials += cap;
+                        first = 0;
+                    }
+                }
+            }
+            Rng gq = base;  // counter 1: the quadrant
+            gq.c0 = 1u;
+            const int quadrant = int(rng_uniform(gq) * 4.0);
+            theta = th_acc;
+            if (quadrant == 1) theta = PI_D - theta;
+            else if (quadrant == 2) theta = PI_D + theta;
+            else if (quadrant == 3) theta = 2.0 * PI_D - theta;
+            s = smax * v_acc;
+            if (lane == 0) sv = classical ? lp_cl_stokes(D, alp, v_acc, theta) : lp_stokes(D, alp, eta, s, theta);
+        }
+        if (lane == 0) lma_emit_finish<classical>(A, M, i, k0, a, eta, ok, nh, s, theta, sv, n_emit, flags);
+    }
+    if (lane == 0) {
+        wave_add(A->stats + STAT_EMISSIONS, n_emit);
+        wave_add(A->stats + STAT_TRIALS, trials);
+        if (flags) atomicOr(A->flags, flags);
+    }
 }
 
 // ================================================================ photons: Breit-Wheeler
@@ -504,8 +671,8 @@ void launch_event_lma(const TransportArgs* a, const LmaArgs& m, int lanes, hipSt
     switch (m.mode) {
     case LMA_CP: k_event_lma<LMA_CP><<<lanes / 64, 64, 0, s>>>(a, m); break;
     case LMA_CP_CLASSICAL: k_event_lma<LMA_CP_CLASSICAL><<<lanes / 64, 64, 0, s>>>(a, m); break;
-    case LMA_LP_CLASSICAL: k_event_lma<LMA_LP_CLASSICAL><<<lanes / 64, 64, 0, s>>>(a, m); break;
-    default: k_event_lma<LMA_LP><<<lanes / 64, 64, 0, s>>>(a, m); break;
+    case LMA_LP_CLASSICAL: k_event_lma_wave<LMA_LP_CLASSICAL><<<lanes / 64, 64, 0, s>>>(a, m); break;
+    default: k_event_lma_wave<LMA_LP><<<lanes / 64, 64, 0, s>>>(a, m); break;
     }
 }
 

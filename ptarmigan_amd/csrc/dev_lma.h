@@ -382,24 +382,20 @@ PTG_HD __forceinline__ int dbessel_len(int n) {
     return hi - lo + 200;
 }
 
-PTG_HD inline void dbessel_recurrence(const DBessel& D, double x, double y, double j[5]) {
+// One order of the forward pass: the coefficients of order k from those of k - 1.
+PTG_HD __forceinline__ void dbessel_forward(double x, double y, int k, double& x0, double& x1, double& x2, double& l0,
+                                            double& l1) {
+    double n0 = -x - 4.0 * y * y / x2;
+    double n1 = 2.0 * double(k - 1) - 2.0 * y * x0 / x2;
+    double n2 = -x - 2.0 * y * x1 / x2;
+    double m0 = n0 - 2.0 * y * n2 / l1;
+    double m1 = n1 - l0 * n2 / l1;
+    x0 = n0; x1 = n1; x2 = n2; l0 = m0; l1 = m1;
+}
+// The backward pass over the forward coefficients W(i, c), read in descending order of i.
+template <class DB, class WF>
+PTG_HD __forceinline__ void dbessel_backward(const DB& D, double x, double y, WF& W, double j[5]) {
     const int len = D.len;
-    const size_t st = D.st;
-    double* ws = D.ws;
-    auto W = [&](int i, int c) -> double& { return ws[(size_t(5) * i + c) * st]; };
-    // forward pass (element 0 holds the initial value 1 for all coefficients)
-    double x0 = 1.0, x1 = 1.0, x2 = 1.0, l0 = 1.0, l1 = 1.0;
-    W(0, 0) = 1.0; W(0, 1) = 1.0; W(0, 2) = 1.0; W(0, 3) = 1.0; W(0, 4) = 1.0;
-    for (int i = 1; i < len; ++i) {
-        int k = i + D.k_min;
-        double n0 = -x - 4.0 * y * y / x2;
-        double n1 = 2.0 * double(k - 1) - 2.0 * y * x0 / x2;
-        double n2 = -x - 2.0 * y * x1 / x2;
-        double m0 = n0 - 2.0 * y * n2 / l1;
-        double m1 = n1 - l0 * n2 / l1;
-        x0 = n0; x1 = n1; x2 = n2; l0 = m0; l1 = m1;
-        W(i, 0) = n0; W(i, 1) = n1; W(i, 2) = n2; W(i, 3) = m0; W(i, 4) = m1;
-    }
     int n_minus = int(floor(-2.0 * y - x));
     int n_plus = 8.0 * y > x ? int(ceil(2.0 * y + x * x / (16.0 * y))) : int(ceil(-2.0 * y + x));
     int i_match = (n_plus + n_minus) / 2 - D.k_min;
@@ -474,6 +470,84 @@ PTG_HD inline void dbessel_recurrence(const DBessel& D, double x, double y, doub
         j[q] = __builtin_isfinite(val) ? val : 0.0;
     }
 }
+PTG_HD inline void dbessel_recurrence(const DBessel& D, double x, double y, double j[5]) {
+    const int len = D.len;
+    const size_t st = D.st;
+    double* ws = D.ws;
+    auto S = [&](int i, int c) -> double& { return ws[(size_t(5) * i + c) * st]; };
+    // forward pass (element 0 holds the initial value 1 for all coefficients)
+    double x0 = 1.0, x1 = 1.0, x2 = 1.0, l0 = 1.0, l1 = 1.0;
+    S(0, 0) = 1.0; S(0, 1) = 1.0; S(0, 2) = 1.0; S(0, 3) = 1.0; S(0, 4) = 1.0;
+    for (int i = 1; i < len; ++i) {
+        dbessel_forward(x, y, i + D.k_min, x0, x1, x2, l0, l1);
+        S(i, 0) = x0; S(i, 1) = x1; S(i, 2) = x2; S(i, 3) = l0; S(i, 4) = l1;
+    }
+    auto W = [&](int i, int c) -> double { return ws[(size_t(5) * i + c) * st]; };
+    dbessel_backward(D, x, y, W, j);
+}
+
+// The same recurrence with the forward coefficients checkpointed: only every
+// K-th forward state goes to the per-lane scratch (5 doubles per K orders); the
+// backward pass recomputes one segment of K states at a time, by the same
+// forward step from its checkpoint, into a per-lane LDS column. The values are
+// those of dbessel_recurrence; HBM traffic drops K-fold for twice the forward
+// arithmetic. `seg` points at this lane's column of a [K][5][64]-double LDS
+// array (stride 64 between consecutive doubles of one lane).
+template <int K>
+struct DBesselCk {
+    int n, k_min, len;
+    double* ws;   // checkpoints: 5 * ceil(len / K) doubles, (s, c) at ws[(5 * s + c) * st]
+    size_t st;
+    double* seg;  // LDS column, (t, c) at seg[(5 * t + c) * 64]
+};
+template <int K>
+__device__ __forceinline__ DBesselCk<K> dbessel_ck_at(int n, double* ws, size_t st, double* seg) {
+    DBessel D = dbessel_at(n, ws, st);
+    return DBesselCk<K>{D.n, D.k_min, D.len, ws, st, seg};
+}
+template <int K>
+__device__ inline void dbessel_recurrence(const DBesselCk<K>& D, double x, double y, double j[5]) {
+    const int len = D.len;
+    const size_t st = D.st;
+    double* ws = D.ws;
+    double* seg = D.seg;
+    double x0 = 1.0, x1 = 1.0, x2 = 1.0, l0 = 1.0, l1 = 1.0;
+    ws[0] = 1.0; ws[st] = 1.0; ws[2 * st] = 1.0; ws[3 * st] = 1.0; ws[4 * st] = 1.0;
+    for (int i = 1; i < len; ++i) {
+        dbessel_forward(x, y, i + D.k_min, x0, x1, x2, l0, l1);
+        if (i % K == 0) {
+            double* c = ws + size_t(5) * size_t(i / K) * st;
+            c[0] = x0; c[st] = x1; c[2 * st] = x2; c[3 * st] = l0; c[4 * st] = l1;
+        }
+    }
+    int cur = -1;  // the segment held in LDS; the backward pass only descends
+    double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0, p4 = 0.0;  // checkpoint cur - 1, loaded ahead
+    auto W = [&](int i, int c) -> double {
+        const int sg = i / K;
+        if (sg != cur) {
+            double v0, v1, v2, v3, v4;
+            if (sg == cur - 1) {
+                v0 = p0; v1 = p1; v2 = p2; v3 = p3; v4 = p4;
+            } else {
+                const double* ck = ws + size_t(5) * size_t(sg) * st;
+                v0 = ck[0]; v1 = ck[st]; v2 = ck[2 * st]; v3 = ck[3 * st]; v4 = ck[4 * st];
+            }
+            if (sg > 0) {  // the next segment's checkpoint arrives while this one is consumed
+                const double* ck = ws + size_t(5) * size_t(sg - 1) * st;
+                p0 = ck[0]; p1 = ck[st]; p2 = ck[2 * st]; p3 = ck[3 * st]; p4 = ck[4 * st];
+            }
+            seg[0] = v0; seg[64] = v1; seg[128] = v2; seg[192] = v3; seg[256] = v4;
+            for (int t = 1; t < K && sg * K + t < len; ++t) {
+                dbessel_forward(x, y, sg * K + t + D.k_min, v0, v1, v2, v3, v4);
+                double* o = seg + 320 * t;
+                o[0] = v0; o[64] = v1; o[128] = v2; o[192] = v3; o[256] = v4;
+            }
+            cur = sg;
+        }
+        return seg[(5 * (i - sg * K) + c) * 64];
+    };
+    dbessel_backward(D, x, y, W, j);
+}
 
 // Saddle-point approximation of J(n, x, y) for large n (double_bessel.rs:272-355).
 PTG_HD inline double dbessel_spa(int ni, double x, double y) {
@@ -533,9 +607,17 @@ PTG_HD inline void dbessel_eval(const DBessel& D, double x, double y, double j[5
     else
         dbessel_recurrence(D, x, y, j);
 }
+template <int K>
+__device__ inline void dbessel_eval(const DBesselCk<K>& D, double x, double y, double j[5]) {
+    if (dbessel_use_spa(D.n, x, y))
+        for (int q = 0; q < 5; ++q) j[q] = dbessel_spa(D.n - 2 + q, x, y);
+    else
+        dbessel_recurrence(D, x, y, j);
+}
 
 // ---------------------------------------------------------------- LP double-differential rate (lp/mod.rs:21-140)
-__device__ inline double lp_ddpr(const DBessel& D, double a, double eta, double s, double theta) {
+template <class DB>
+__device__ inline double lp_ddpr(const DB& D, double a, double eta, double s, double theta) {
     int n = D.n;
     double sn = 2.0 * double(n) * eta / (1.0 + 0.5 * a * a);
     double wn = fmin(s / (sn * (1.0 - s)), 1.0);
@@ -624,28 +706,34 @@ PTG_HD inline double ln_dbessel_x_zero(int ni, double y) {
     cplx phase{f.re, f.im + e};
     return log(sqrt(2.0 / (PI_D * c_abs(f2)))) + phase.re + log(fabs(cos(phase.im)));
 }
-__device__ inline ThetaBound theta_bound(int n, double a, double eta) {
-    ThetaBound tb;
+// Point i of 16 of the bound (before the running minimum): s_i and theta_i.
+__device__ inline void theta_bound_point(int n, double a, double eta, int i, double& s_i, double& f_i) {
     double sn = 2.0 * double(n) * eta / (1.0 + 0.5 * a * a);
     double ln_j_crit = log(0.443) - log(double(n)) / 3.0;
-    for (int i = 0; i < 16; ++i) {
-        double z = cos(PI_D * double(i) / 15.0);
-        double s = 0.5 * (z + 1.0) * sn / (1.0 + sn);
-        tb.s[i] = s;
-        double wn = fmin(s / (sn * (1.0 - s)), 1.0);
-        double x = 2.0 * double(n) * a * sqrt(wn * (1.0 - wn)) / sqrt(1.0 + 0.5 * a * a);
-        double y = a * a * s / (8.0 * eta * (1.0 - s));
-        double x_crit = y > double(n) / 6.0 ? 4.0 * sqrt(y) * sqrt(double(n) - 2.0 * y) : double(n) + 2.0 * y;
-        double ln_j_bdy = fmin(ln_dbessel_x_zero(n % 2 == 0 ? n : n + 1, y), ln_j_crit);
-        double cos_theta = fmax(1.0 - x_crit * log(0.01) / (x * (ln_j_bdy - ln_j_crit)), -1.0);
-        double f = acos(cos_theta);
-        tb.f[i] = isnan(f) ? PI_D : f;
-    }
-    double f_min = tb.f[0];
+    double z = cos(PI_D * double(i) / 15.0);
+    double s = 0.5 * (z + 1.0) * sn / (1.0 + sn);
+    s_i = s;
+    double wn = fmin(s / (sn * (1.0 - s)), 1.0);
+    double x = 2.0 * double(n) * a * sqrt(wn * (1.0 - wn)) / sqrt(1.0 + 0.5 * a * a);
+    double y = a * a * s / (8.0 * eta * (1.0 - s));
+    double x_crit = y > double(n) / 6.0 ? 4.0 * sqrt(y) * sqrt(double(n) - 2.0 * y) : double(n) + 2.0 * y;
+    double ln_j_bdy = fmin(ln_dbessel_x_zero(n % 2 == 0 ? n : n + 1, y), ln_j_crit);
+    double cos_theta = fmax(1.0 - x_crit * log(0.01) / (x * (ln_j_bdy - ln_j_crit)), -1.0);
+    double f = acos(cos_theta);
+    f_i = isnan(f) ? PI_D : f;
+}
+// the bound is non-increasing in s: a running minimum from the largest s down
+__device__ __forceinline__ void theta_bound_running_min(double* f) {
+    double f_min = f[0];
     for (int i = 1; i < 16; ++i) {
-        if (tb.f[i] < f_min) f_min = tb.f[i];
-        else tb.f[i] = f_min;
+        if (f[i] < f_min) f_min = f[i];
+        else f[i] = f_min;
     }
+}
+__device__ inline ThetaBound theta_bound(int n, double a, double eta) {
+    ThetaBound tb;
+    for (int i = 0; i < 16; ++i) theta_bound_point(n, a, eta, i, tb.s[i], tb.f[i]);
+    theta_bound_running_min(tb.f);
     return tb;
 }
 // theta_bound_at over a bound held elsewhere (e.g. one copy per wave in LDS)
@@ -674,30 +762,28 @@ __device__ __forceinline__ double theta_bound_at(const ThetaBound& tb, double ar
 
 // ---------------------------------------------------------------- LP, classical limit (lp/classical.rs)
 // theta bound at fixed v = s / s_n (lp/mod.rs:211-284)
+__device__ inline void theta_bound_low_eta_point(int n, double a, int i, double& v_i, double& f_i) {
+    double ln_j_crit = log(0.443) - log(double(n)) / 3.0;
+    double z = cos(PI_D * double(i) / 15.0);
+    double v = 0.5 * (z + 1.0);
+    v_i = v;
+    double x = 2.0 * double(n) * a * sqrt(v * (1.0 - v) / (1.0 + 0.5 * a * a));
+    double y = 0.25 * double(n) * a * a * v / (1.0 + 0.5 * a * a);
+    double x_crit = y > double(n) / 6.0 ? 4.0 * sqrt(y) * sqrt(double(n) - 2.0 * y) : double(n) + 2.0 * y;
+    double ln_j_bdy = fmin(ln_dbessel_x_zero(n % 2 == 0 ? n : n + 1, y), ln_j_crit);
+    double cos_theta = fmax(1.0 - x_crit * log(0.01) / (x * (ln_j_bdy - ln_j_crit)), -1.0);
+    double f = acos(cos_theta);
+    f_i = isnan(f) ? PI_D : f;
+}
 __device__ inline ThetaBound theta_bound_low_eta(int n, double a) {
     ThetaBound tb;
-    double ln_j_crit = log(0.443) - log(double(n)) / 3.0;
-    for (int i = 0; i < 16; ++i) {
-        double z = cos(PI_D * double(i) / 15.0);
-        double v = 0.5 * (z + 1.0);
-        tb.s[i] = v;
-        double x = 2.0 * double(n) * a * sqrt(v * (1.0 - v) / (1.0 + 0.5 * a * a));
-        double y = 0.25 * double(n) * a * a * v / (1.0 + 0.5 * a * a);
-        double x_crit = y > double(n) / 6.0 ? 4.0 * sqrt(y) * sqrt(double(n) - 2.0 * y) : double(n) + 2.0 * y;
-        double ln_j_bdy = fmin(ln_dbessel_x_zero(n % 2 == 0 ? n : n + 1, y), ln_j_crit);
-        double cos_theta = fmax(1.0 - x_crit * log(0.01) / (x * (ln_j_bdy - ln_j_crit)), -1.0);
-        double f = acos(cos_theta);
-        tb.f[i] = isnan(f) ? PI_D : f;
-    }
-    double f_min = tb.f[0];
-    for (int i = 1; i < 16; ++i) {
-        if (tb.f[i] < f_min) f_min = tb.f[i];
-        else tb.f[i] = f_min;
-    }
+    for (int i = 0; i < 16; ++i) theta_bound_low_eta_point(n, a, i, tb.s[i], tb.f[i]);
+    theta_bound_running_min(tb.f);
     return tb;
 }
 // rate differential in v and theta, 0 < theta < pi/2 (lp/classical.rs:17-40)
-__device__ inline double lp_cl_ddpr(const DBessel& D, double a, double v, double theta) {
+template <class DB>
+__device__ inline double lp_cl_ddpr(const DB& D, double a, double v, double theta) {
     int n = D.n;
     double x = 2.0 * double(n) * a * cos(theta) * sqrt(v * (1.0 - v) / (1.0 + 0.5 * a * a));
     double y = 0.25 * double(n) * a * a * v / (1.0 + 0.5 * a * a);
@@ -788,7 +874,8 @@ __device__ inline int lp_cl_harmonic(const double* P, const LmaScalars& s, doubl
 // Stokes parameters of a classical LP photon at (v, theta) (lp/classical.rs:296-330).
 // The reference evaluates J at the signed x of the final quadrant and applies the
 // parity of a negative argument for every quadrant (lp/classical.rs:300-314); so here.
-__device__ inline sv4 lp_cl_stokes(const DBessel& D, double a, double v, double theta) {
+template <class DB>
+__device__ inline sv4 lp_cl_stokes(const DB& D, double a, double v, double theta) {
     int n = D.n;
     double x = 2.0 * double(n) * a * cos(theta) * sqrt(v * (1.0 - v) / (1.0 + 0.5 * a * a));
     double y = 0.25 * double(n) * a * a * v / (1.0 + 0.5 * a * a);
@@ -834,7 +921,8 @@ __device__ inline void lp_cl_sample(const DBessel& D, double a, double eta, Rng&
 }
 
 // Stokes parameters of an LP photon at (s, theta) (lp/mod.rs:626-676)
-__device__ inline sv4 lp_stokes(const DBessel& D, double a, double eta, double s, double theta) {
+template <class DB>
+__device__ inline sv4 lp_stokes(const DB& D, double a, double eta, double s, double theta) {
     int n = D.n;
     double sn = 2.0 * double(n) * eta / (1.0 + 0.5 * a * a);
     double wn = fmin(s / (sn * (1.0 - s)), 1.0);

@@ -291,6 +291,9 @@ __device__ __forceinline__ double wave_fmax(double v) {
 #ifndef PTG_LMA_WAVE_WPE
 #define PTG_LMA_WAVE_WPE 2  // two waves per SIMD: 256 VGPRs, a little scratch
 #endif
+#ifndef PTG_J_SEG
+#define PTG_J_SEG 7  // J(n, x, y) forward states per checkpoint: 17.5 KB of LDS per wave
+#endif
 template <int MODE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PTG_LMA_WAVE_WPE))) void k_event_lma_wave(const TransportArgs* __restrict__ A, LmaArgs M) {
     constexpr bool classical = MODE == LMA_LP_CLASSICAL;
@@ -300,6 +303,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PTG_LMA_WAVE
     const double* cdf = M.tables + lma_pack::CDF;
     const uint32_t cap = 1u << 20;
     __shared__ double tb_s[16], tb_f[16];  // the event's theta bound, one copy per wave
+    __shared__ double jseg[PTG_J_SEG * 5 * 64];  // per-lane segment of recomputed J forward states
     uint64_t n_emit = 0, trials = 0;
     uint32_t flags = 0;
     for (uint64_t e = blockIdx.x; e < M.n_ev; e += gridDim.x) {
@@ -333,13 +337,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PTG_LMA_WAVE
         double s = 0.0, theta = 0.0;
         sv4 sv{1.0, 0.0, 0.0, 0.0};
         if (ok) {
-            DBessel D = dbessel_at(nh, ws, st);
+            const DBesselCk<PTG_J_SEG> D = dbessel_ck_at<PTG_J_SEG>(nh, ws, st, jseg + lane);
             const double sn = 2.0 * double(nh) * eta / (1.0 + 0.5 * alp * alp);
             const double smax = classical ? sn : sn / (1.0 + sn);  // s = v * smax
-            if (lane == 0) {
-                const ThetaBound tb = classical ? theta_bound_low_eta(nh, alp) : theta_bound(nh, alp, eta);
-                for (int k = 0; k < 16; ++k) { tb_s[k] = tb.s[k]; tb_f[k] = tb.f[k]; }
+            if (lane < 16) {  // the theta bound's 16 points, one per lane
+                if (classical) theta_bound_low_eta_point(nh, alp, lane, tb_s[lane], tb_f[lane]);
+                else theta_bound_point(nh, alp, eta, lane, tb_s[lane], tb_f[lane]);
             }
+            __syncthreads();
+            if (lane == 0) theta_bound_running_min(tb_f);
             __syncthreads();
             // A candidate t: (v, theta) from counter 16 + 2t, its acceptance draw from 17 + 2t.
             // f is the rate at the candidate (not needed if theta is out of bounds).
@@ -710,6 +716,7 @@ __global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, ui
     const int scratch_len = m.scratch_len;
     const double* x = in + uint64_t(i) * in_w;
     double* y = out + uint64_t(i) * out_w;
+    __shared__ double jseg[PTG_J_SEG * 5 * 64];  // blocks of 64 lanes
     switch (fn) {
     case PTG_FN_LP_RATE: {
         double f = 0.0;
@@ -722,8 +729,13 @@ __global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, ui
         int nn = int(x[0]);  // the host checks 3 <= n, 0 <= x <= n sqrt 2, 0 <= y <= n / 2, len(n) <= scratch_len
         DBessel D = dbessel_at(nn, scratch + i, n);
         double j[5];
-        if (x[3] != 0.0) dbessel_recurrence(D, x[1], x[2], j);
-        else dbessel_eval(D, x[1], x[2], j);
+        if (x[3] == 2.0) {  // the checkpointed recurrence of the emission kernel
+            dbessel_recurrence(dbessel_ck_at<PTG_J_SEG>(nn, scratch + i, n, jseg + threadIdx.x), x[1], x[2], j);
+        } else if (x[3] != 0.0) {
+            dbessel_recurrence(D, x[1], x[2], j);
+        } else {
+            dbessel_eval(D, x[1], x[2], j);
+        }
         for (int q = 0; q < 5; ++q) y[q] = j[q];
         break;
     }

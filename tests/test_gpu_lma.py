@@ -86,6 +86,21 @@ def test_double_bessel_matches_oracle(engine, orc, n):
         assert np.abs(got - ref).max() <= 1e-11 * scale, rec
 
 
+@pytest.mark.parametrize("n", [3, 7, 8, 25, 1000, 2500])
+def test_double_bessel_checkpointed_is_the_recurrence(engine, n):
+    """The emission kernel's checkpointed J recurrence (forward states recomputed
+    per segment in LDS) gives the values of the stored recurrence, bit for bit."""
+    from ptarmigan_amd import Fn
+    say("J checkpointed", n)
+    rng = np.random.default_rng(100 + n)
+    m = 200
+    x = n * math.sqrt(2.0) * rng.uniform(size=m)
+    y = 0.5 * n * rng.uniform(size=m)
+    plain = engine.eval(Fn.DOUBLE_BESSEL, np.stack([np.full(m, n), x, y, np.full(m, 1.0)], 1), 5)
+    ck = engine.eval(Fn.DOUBLE_BESSEL, np.stack([np.full(m, n), x, y, np.full(m, 2.0)], 1), 5)
+    assert np.array_equal(plain, ck)
+
+
 def test_double_bessel_reference_kat_on_gpu(engine):
     """double_bessel.rs:417-597 known answers through the device recurrence."""
     from ptarmigan_amd import Fn
@@ -95,8 +110,9 @@ def test_double_bessel_reference_kat_on_gpu(engine):
            (1000, 1100.0, 250.0, 1.9079405288254989534e-31),
            (3000, 3500.0, 1500.0, 9.5390642676650740986e-3)]
     for n, x, y, t in pts:
-        got = engine.eval(Fn.DOUBLE_BESSEL, [[n, x, y, 1.0]], 5)[0, 2]
-        assert abs((got - t) / t) < 1e-9, (n, x, y)
+        for mode in (1.0, 2.0):
+            got = engine.eval(Fn.DOUBLE_BESSEL, [[n, x, y, mode]], 5)[0, 2]
+            assert abs((got - t) / t) < 1e-9, (n, x, y, mode)
 
 
 @pytest.mark.parametrize("a,eta,fixed_n", [(2.0, 0.1, 0), (1.0, 0.1, 1), (2.0, 0.3, 8), (5.0, 0.05, 0)])

@@ -104,6 +104,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="lcfa",
                     help="lcfa: the focused-laser cascade (default, the headline workload); lma: the plane-wave LMA case")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="mixed: FP32 rate/CDF tables and pusher algebra, FP64 phase, momentum update and "
+                         "survival product (LMA workloads only)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="run only a plain device copy of BYTES (5 dispatches), for counter calibration")
     args = ap.parse_args()
@@ -121,6 +124,7 @@ def main():
     if args.workload == "lma" and args.primaries == 10_000_000:
         args.primaries = 100_000  # the plane-wave case is quoted on 1e5 primaries
     laser, beam, opts = make_workload(args.workload, args.primaries)
+    opts.precision = 1 if args.precision == "mixed" else 0
     eng = Engine(0)
     eng.set_laser(laser)
     eng.build_beam(beam, seed=0)
@@ -170,8 +174,8 @@ def main():
         "metric": "particle-steps/s on 1 MI355X: surrogate+FP32 vs plain FP64; photon yield err",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": WORKLOADS[args.workload], "primaries": args.primaries, "precision": "fp64"},
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic",
+        "config": {"workload": WORKLOADS[args.workload], "primaries": args.primaries, "precision": args.precision},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "detail": {"lepton_steps": totals["lepton_steps"], "photon_steps": totals["photon_steps"],

@@ -247,6 +247,28 @@ PTG_HD inline bool cp_rate(const double* rt, const double* P, const CpScalars& c
     return true;
 }
 
+// cp_rate on an FP32 copy of the quantum table, in FP32; below the table in eta
+// (classical rate) and a (closed form) it stays FP64
+__device__ inline bool cp_rate_f(const float* rt, const double* P, const CpScalars& c, double a, double eta, float& out) {
+    const float la = logf(float(a)), le = logf(float(eta));
+    if (le < float(c.min[1]) || la < float(c.min[0])) {
+        double o = 0.0;
+        bool ok = cp_rate(P + cp_pack::RATE, P, c, a, eta, o);
+        out = float(o);
+        return ok;
+    }
+    if (!(a < 20.0 && eta < 2.0)) return false;
+    const float fa = (la - float(c.min[0])) / float(c.step[0]), fe = (le - float(c.min[1])) / float(c.step[1]);
+    int ia = int(fa), ie = int(fe);
+    ia = ia > cp_pack::COLS - 2 ? cp_pack::COLS - 2 : ia;
+    ie = ie > cp_pack::ROWS - 2 ? cp_pack::ROWS - 2 : ie;
+    const float da = fa - float(ia), de = fe - float(ie);
+    const float* t0 = rt + ie * cp_pack::COLS + ia;
+    const float* t1 = t0 + cp_pack::COLS;
+    out = expf((1.0f - da) * (1.0f - de) * t0[0] + da * (1.0f - de) * t0[1] + (1.0f - da) * de * t1[0] + da * de * t1[1]);
+    return true;
+}
+
 // ---------------------------------------------------------------- harmonic index
 // One tabulated CDF [32][2] inverted at frac: 0.9 below its first value, else the
 // first 31 points in (ln n, ln(-ln(1 - F))) with linear extrapolation

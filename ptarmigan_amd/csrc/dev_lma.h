@@ -270,6 +270,101 @@ __device__ __forceinline__ double pw_push(const LaserDev& L, d4& r, d4& u, doubl
     u.t = sqrt(1.0 + asq + u.x * u.x + u.y * u.y + u.z * u.z);
     return dwork;
 }
+// ---------------------------------------------------------------- mixed precision
+// The plane-wave push with the pusher algebra (envelope, its gradient, the
+// radiation-reaction factor) in FP32, while the phase k.r, the position and
+// momentum updates and the mass shell stay in FP64 (the phase is formed in
+// FP64 and only its reduced envelope argument, |arg| <= pi/2 for cos^2, goes to
+// FP32). Same steps as pw_push (plane_wave.rs:196-249).
+__device__ __forceinline__ float pw_a_sqd_f(const LaserDev& L, double phase) {
+    const float norm = L.circular ? 1.0f : 0.5f;
+    const double nc = L.n_cycles;
+    const float a02 = float(L.a0 * L.a0);
+    if (L.envelope == ENV_COS2) {
+        if (fabs(phase) < PI_D * nc) {
+            float sn, cs;
+            sincosf(float(phase / (2.0 * nc)), &sn, &cs);
+            float c2 = cs * cs;
+            return norm * a02 * c2 * c2;
+        }
+        return 0.0f;
+    }
+    if (L.envelope == ENV_FLATTOP) {
+        double ap = fabs(phase);
+        if (ap > PI_D * (nc + 1.0)) return 0.0f;
+        if (ap > PI_D * (nc - 1.0)) {
+            float sn, cs;
+            sincosf(float(0.25 * (ap - (nc - 1.0) * PI_D)), &sn, &cs);
+            float c2 = cs * cs;
+            return norm * a02 * c2 * c2;
+        }
+        return norm * a02;
+    }
+    float q = float(phase / (PI_D * nc));
+    return norm * a02 * exp2f(-q * q);
+}
+__device__ __forceinline__ float pw_grad_f(const LaserDev& L, double phase) {
+    const float norm = L.circular ? 1.0f : 0.5f;
+    const double nc = L.n_cycles;
+    const float a02 = float(L.a0 * L.a0), k0 = float(L.k0), ncf = float(nc);
+    if (L.envelope == ENV_COS2) {
+        if (fabs(phase) < PI_D * nc) {
+            float sn, cs, s2, c2;
+            sincosf(float(phase / nc), &s2, &c2);
+            sincosf(float(phase / (2.0 * nc)), &sn, &cs);
+            return norm * k0 * a02 * s2 * cs * cs / ncf;
+        }
+        return 0.0f;
+    }
+    if (L.envelope == ENV_FLATTOP) {
+        double ap = fabs(phase);
+        if (ap > PI_D * (nc + 1.0) || ap < PI_D * (nc - 1.0)) return 0.0f;
+        float sn, cs;
+        sincosf(float(0.25 * (ap - (nc - 1.0) * PI_D)), &sn, &cs);
+        return norm * k0 * a02 * copysignf(1.0f, float(phase)) * sn * cs * cs * cs;
+    }
+    float q = float(phase / (PI_D * nc));
+    const float w = float(PI_D * nc);
+    return norm * k0 * a02 * 2.0f * float(LN2_D) * float(phase) * exp2f(-q * q) / (w * w);
+}
+__device__ __forceinline__ double pw_push_mixed(const LaserDev& L, d4& r, d4& u, double rqm, double dt, int ll) {
+    double dtau = dt / u.t;
+    const float sc = float(rqm / RQM_ELECTRON);
+    const float scale = sc * sc;
+    const float kc = float(C_LIGHT * T_COMPTON * L.k0);
+    const float rr = float(2.0 * ALPHA / (3.0 * T_COMPTON));
+    d4 u_mid{0.0, 0.0, 0.0, 0.0};
+    if (ll) {
+        double ph = L.k0 * (r.t - r.z);
+        float f = -0.5f * float(C_LIGHT) * scale * pw_grad_f(L, ph);
+        float eta = kc * float(u.t - u.z);
+        float gfac = -rr * pw_a_sqd_f(L, ph) * eta * eta * scale * scale;
+        u_mid = {u.t + double(0.5f * (f + gfac * float(u.t))) * dtau, u.x + double(0.5f * gfac * float(u.x)) * dtau,
+                 u.y + double(0.5f * gfac * float(u.y)) * dtau, u.z + double(0.5f * (f + gfac * float(u.z))) * dtau};
+    }
+    double h = 0.5 * C_LIGHT * dtau;
+    r = {r.t + h * u.t, r.x + h * u.x, r.y + h * u.y, r.z + h * u.z};
+    double ph = L.k0 * (r.t - r.z);
+    float f = -0.5f * float(C_LIGHT) * scale * pw_grad_f(L, ph);
+    double dwork;
+    if (ll) {
+        float eta = kc * float(u_mid.t - u_mid.z);
+        float asq = pw_a_sqd_f(L, ph);
+        float gfac = -rr * asq * eta * eta * scale * scale;
+        u = {u.t + double(f + gfac * float(u_mid.t)) * dtau, u.x + double(gfac * float(u_mid.x)) * dtau,
+             u.y + double(gfac * float(u_mid.y)) * dtau, u.z + double(f + gfac * float(u_mid.z)) * dtau};
+        const float omega = float(C_LIGHT * L.k0);
+        const float delta = L.circular ? 1.0f : 0.75f;
+        dwork = double(f + 2.0f * float(ALPHA) * omega * eta * delta * asq * asq / 3.0f) * dtau;
+    } else {
+        u = {u.t + double(f) * dtau, u.x, u.y, u.z + double(f) * dtau};
+        dwork = double(f) * dtau;
+    }
+    r = {r.t + h * u.t, r.x + h * u.x, r.y + h * u.y, r.z + h * u.z};
+    double asq = double(pw_a_sqd_f(L, L.k0 * (r.t - r.z)));
+    u.t = sqrt(1.0 + asq + u.x * u.x + u.y * u.y + u.z * u.z);
+    return dwork;
+}
 // push in either LMA field (the kind is uniform over a launch)
 __device__ __forceinline__ double lma_push(const LaserDev& L, d4& r, d4& u, double rqm, double dt, int ll) {
     return L.kind == K_FOCUSED_LASER ? fl_push(L, r, u, rqm, dt, ll) : pw_push(L, r, u, rqm, dt, ll);
@@ -308,8 +403,33 @@ __device__ __forceinline__ bool lp_rate(const double* rt, const LmaScalars& s, d
     return true;
 }
 
+// The LP rate lookup of lp_rate on an FP32 copy of the table, in FP32; the
+// linear-Compton branch below the table stays FP64 (it cancels strongly).
+__device__ __forceinline__ bool lp_rate_f(const float* rt, const LmaScalars& s, double a, double eta, float& out) {
+    const float x = logf(float(a)), y = logf(float(eta));
+    if (x < float(s.rate_min[0])) { out = float(lp_linear_rate(a, eta)); return true; }
+    const float fx = (x - float(s.rate_min[0])) / float(s.rate_step[0]);
+    if (!(fx >= 0.0f && fx < float(lma_pack::RATE_COLS - 1))) return false;
+    const int ix = int(fx);
+    const float dx = fx - float(ix);
+    if (y < float(s.rate_min[1])) {
+        float f = (1.0f - dx) * rt[ix] + dx * rt[ix + 1];
+        out = expf(f - float(s.rate_min[1])) * float(eta);
+        return true;
+    }
+    const float fy = (y - float(s.rate_min[1])) / float(s.rate_step[1]);
+    if (!(fy >= 0.0f && fy < float(lma_pack::RATE_ROWS - 1))) return false;
+    const int iy = int(fy);
+    const float dy = fy - float(iy);
+    const float* t0 = rt + iy * lma_pack::RATE_COLS + ix;
+    const float* t1 = t0 + lma_pack::RATE_COLS;
+    out = expf((1.0f - dx) * (1.0f - dy) * t0[0] + dx * (1.0f - dy) * t0[1] + (1.0f - dx) * dy * t1[0] + dx * dy * t1[1]);
+    return true;
+}
+
 // ---------------------------------------------------------------- harmonic index (lp/mod.rs:465-553)
-__device__ inline double harmonic_from_table(const double* t16, double a, double frac) {
+template <class P>
+__device__ inline double harmonic_from_table(P t16, double a, double frac) {
     if (frac <= t16[1]) return 0.9;
     double n = 0.0;
     if (a < 5.0) {
@@ -327,7 +447,8 @@ __device__ inline double harmonic_from_table(const double* t16, double a, double
 }
 // Returns 0 when (a, eta) is outside the table, where the reference's lookup
 // would panic; a valid index is at least 1.
-__device__ inline int lp_harmonic(const double* cdf, const LmaScalars& sc, double a, double eta, double frac) {
+template <class P>
+__device__ inline int lp_harmonic(P cdf, const LmaScalars& sc, double a, double eta, double frac) {
     double la = log(a), le = log(eta);
     if (la <= sc.cdf_min[0]) return 1;
     double fx = (la - sc.cdf_min[0]) / sc.cdf_step[0];

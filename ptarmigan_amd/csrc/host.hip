@@ -101,6 +101,7 @@ struct ptg_ctx {
     double* d_lcfa = nullptr;
     LcfaScalars sc{};
     double* d_lma = nullptr;
+    float* d_lma_f = nullptr;  // FP32 copy of the LMA pack (mixed precision)
     LmaScalars lsc{};
     double* d_bw = nullptr;         // LP Breit-Wheeler pack (dev_bw.h); null if its tables are absent
     BwScalars bsc{};
@@ -463,6 +464,11 @@ int ptg_create(ptg_ctx** out, const char* table_path, int device) {
     }
     HIPCHK(c, hipMalloc(&c->d_lma, lpack.size() * sizeof(double)));
     HIPCHK(c, scopy(c, c->d_lma, lpack.data(), lpack.size() * sizeof(double), hipMemcpyHostToDevice));
+    {
+        std::vector<float> lf(lpack.begin(), lpack.end());
+        HIPCHK(c, hipMalloc(&c->d_lma_f, lf.size() * sizeof(float)));
+        HIPCHK(c, scopy(c, c->d_lma_f, lf.data(), lf.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
     HIPCHK(c, hipMalloc(&c->d_lcfa, pack.size() * sizeof(double)));
     HIPCHK(c, scopy(c, c->d_lcfa, pack.data(), pack.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMalloc(&c->d_ctr, C_N * sizeof(unsigned long long)));
@@ -480,6 +486,7 @@ void ptg_destroy(ptg_ctx* c) {
     free_buf(c->prim); free_buf(c->q[0]); free_buf(c->q[1]); free_buf(c->out); free_buf(c->inter); free_buf(c->tmp);
     if (c->d_lcfa) (void)hipFree(c->d_lcfa);
     if (c->d_lma) (void)hipFree(c->d_lma);
+    if (c->d_lma_f) (void)hipFree(c->d_lma_f);
     if (c->d_bw) (void)hipFree(c->d_bw);
     if (c->d_cp) (void)hipFree(c->d_cp);
     if (c->d_ev_extra) (void)hipFree(c->d_ev_extra);
@@ -806,6 +813,8 @@ static int run_lma(ptg_ctx* c, TransportArgs a, double* ms, uint64_t* launches, 
     m.csc = c->csc;
     m.lcfa = c->d_lcfa;
     m.mode = !c->laser.circular ? (a.classical_mode ? LMA_LP_CLASSICAL : LMA_LP) : (a.classical_mode ? LMA_CP_CLASSICAL : LMA_CP);
+    m.mixed = a.precision == PTG_PREC_MIXED ? 1 : 0;
+    m.tables_f = c->d_lma_f;
     m.bandwidth = c->lma_bandwidth;
     m.pol_angle = c->pol_angle;
     m.sin_pa = std::sin(c->pol_angle);
@@ -968,6 +977,12 @@ int ptg_run(ptg_ctx* c, const ptg_collide_options* o, uint64_t seed, ptg_counter
     if (c->n_prim == 0) return fail(c, PTG_E_STATE, "no primaries resident (ptg_load_primaries / ptg_build_beam)");
     int rc = validate_options(c, o);
     if (rc) return rc;
+    if (o->precision == PTG_PREC_MIXED) {
+        if (!(c->laser.kind == K_PLANE_WAVE || c->laser.kind == K_FOCUSED_LASER))
+            return fail(c, PTG_E_UNSUPPORTED, "mixed precision is implemented for the LMA fields (PlaneWave, FocusedLaser)");
+        if (o->classical)
+            return fail(c, PTG_E_UNSUPPORTED, "mixed precision is implemented for quantum LMA emission (classical: false)");
+    }
     if (c->laser.kind == K_PLANE_WAVE || c->laser.kind == K_FOCUSED_LASER) {
         if (o->tracking_photons && !c->laser.circular && !c->d_bw)
             return fail(c, PTG_E_UNSUPPORTED, "LMA pair creation needs the regenerated Breit-Wheeler mid-range table "

@@ -18,6 +18,8 @@ struct LmaArgs {
     CpScalars csc;
     const double* lcfa;          // LCFA pack (Airy functions for J_n at n >= 1000)
     int mode;                    // LmaMode
+    int mixed;                   // mixed precision: FP32 rate table and pusher algebra, FP32 CDF table (LP, CP quantum)
+    const float* tables_f;       // FP32 copy of the LMA pack (mixed precision)
     double bandwidth;            // fractional bandwidth (plane_wave.rs:49-63); 0 = monochromatic
     double pol_angle, sin_pa, cos_pa;
     // step kernel -> event queue

@@ -28,15 +28,20 @@ struct StepTable {
     __device__ static const double* src(const LmaArgs& M) { return MODE == LMA_LP ? M.tables + lma_pack::RATE : M.cp + cp_pack::RATE; }
 };
 
-template <int MODE>
+// MIX: mixed precision (LP and CP quantum modes): the rate table is staged in
+// LDS as FP32 and looked up in FP32, and the plane-wave pusher algebra runs in
+// FP32; the phase, the position and momentum updates, the mass shell and the
+// survival product stay FP64.
+template <int MODE, bool MIX>
 __global__ __launch_bounds__(256) void k_step_lma(const TransportArgs* __restrict__ A, LmaArgs M) {
+    using TT = typename std::conditional<MIX, float, double>::type;
     const LaserDev& L = A->laser;
     const double dt = A->dt;
     const double kc = C_LIGHT * T_COMPTON;
-    __shared__ double rt[StepTable<MODE>::N];
+    __shared__ TT rt[StepTable<MODE>::N];
     if (MODE == LMA_LP || MODE == LMA_CP) {
         const double* src = StepTable<MODE>::src(M);
-        for (int i = threadIdx.x; i < StepTable<MODE>::N; i += blockDim.x) rt[i] = src[i];
+        for (int i = threadIdx.x; i < StepTable<MODE>::N; i += blockDim.x) rt[i] = TT(src[i]);
         __syncthreads();
     }
 
@@ -82,7 +87,8 @@ __global__ __launch_bounds__(256) void k_step_lma(const TransportArgs* __restric
         }
         ++n_steps;
         ++psteps;
-        work += lma_push(L, r, u, rqm, dt, A->rr_mode == 1);
+        if (MIX && L.kind == K_PLANE_WAVE) work += pw_push_mixed(L, r, u, rqm, dt, A->rr_mode == 1);
+        else work += lma_push(L, r, u, rqm, dt, A->rr_mode == 1);
         // PlaneWave::radiate (plane_wave.rs:251-282)
         double phase = L.k0 * (r.t - r.z);
         double chirp = 1.0 + 2.0 * L.chirp_b * phase;
@@ -102,12 +108,19 @@ __global__ __launch_bounds__(256) void k_step_lma(const TransportArgs* __restric
         double f;
         double prob = 0.0;
         bool ok;
-        if (MODE == LMA_LP) ok = lp_rate(rt, M.sc, a_rms * 1.4142135623730951, eta, f);
-        else if (MODE == LMA_CP) ok = cp_rate(rt, M.cp, M.csc, a_rms, eta, f);
-        else if (MODE == LMA_CP_CLASSICAL) ok = cp_classical_rate(M.cp, M.csc, a_rms, eta, f);
-        else ok = lp_cl_rate(M.tables, M.sc, a_rms * 1.4142135623730951, eta, f);
-        if (ok) prob = ALPHA * f * dphi / eta;
-        else flags |= PTG_FLAG_TABLE_RANGE;
+        if constexpr (MIX && (MODE == LMA_LP || MODE == LMA_CP)) {
+            float ff = 0.0f;
+            if (MODE == LMA_LP) ok = lp_rate_f(rt, M.sc, a_rms * 1.4142135623730951, eta, ff);
+            else ok = cp_rate_f(rt, M.cp, M.csc, a_rms, eta, ff);
+            if (ok) prob = double(float(ALPHA) * ff * (float(dphi) / float(eta)));
+        } else {
+            if (MODE == LMA_LP) ok = lp_rate(rt, M.sc, a_rms * 1.4142135623730951, eta, f);
+            else if (MODE == LMA_CP) ok = cp_rate(rt, M.cp, M.csc, a_rms, eta, f);
+            else if (MODE == LMA_CP_CLASSICAL) ok = cp_classical_rate(M.cp, M.csc, a_rms, eta, f);
+            else ok = lp_cl_rate(M.tables, M.sc, a_rms * 1.4142135623730951, eta, f);
+            if (ok) prob = ALPHA * f * dphi / eta;
+        }
+        if (!ok) flags |= PTG_FLAG_TABLE_RANGE;
         S *= (1.0 - prob);
         if (S < U) {
             // suspend: queue the particle with the event parameters
@@ -319,7 +332,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PTG_LMA_WAVE
         const double alp = a * 1.4142135623730951;
         Rng g = base;  // counter 0: the harmonic
         const double fr = rng_uniform(g);
-        const int nh = classical ? lp_cl_harmonic(M.tables, M.sc, alp, fr) : lp_harmonic(cdf, M.sc, alp, eta, fr);
+        const int nh = classical ? lp_cl_harmonic(M.tables, M.sc, alp, fr)
+                                 : (M.mixed ? lp_harmonic(M.tables_f + lma_pack::CDF, M.sc, alp, eta, fr)
+                                            : lp_harmonic(cdf, M.sc, alp, eta, fr));
         bool ok = nh >= 1;
         if (!ok) {
             flags |= PTG_FLAG_TABLE_RANGE;
@@ -667,10 +682,16 @@ void launch_pair_event_lma(const TransportArgs* a, const LmaArgs& m, int lanes, 
 
 void launch_step_lma(const TransportArgs* a, const LmaArgs& m, int grid, hipStream_t s) {
     switch (m.mode) {
-    case LMA_CP: k_step_lma<LMA_CP><<<grid, 256, 0, s>>>(a, m); break;
-    case LMA_CP_CLASSICAL: k_step_lma<LMA_CP_CLASSICAL><<<grid, 256, 0, s>>>(a, m); break;
-    case LMA_LP_CLASSICAL: k_step_lma<LMA_LP_CLASSICAL><<<grid, 256, 0, s>>>(a, m); break;
-    default: k_step_lma<LMA_LP><<<grid, 256, 0, s>>>(a, m); break;
+    case LMA_CP:
+        if (m.mixed) k_step_lma<LMA_CP, true><<<grid, 256, 0, s>>>(a, m);
+        else k_step_lma<LMA_CP, false><<<grid, 256, 0, s>>>(a, m);
+        break;
+    case LMA_CP_CLASSICAL: k_step_lma<LMA_CP_CLASSICAL, false><<<grid, 256, 0, s>>>(a, m); break;
+    case LMA_LP_CLASSICAL: k_step_lma<LMA_LP_CLASSICAL, false><<<grid, 256, 0, s>>>(a, m); break;
+    default:
+        if (m.mixed) k_step_lma<LMA_LP, true><<<grid, 256, 0, s>>>(a, m);
+        else k_step_lma<LMA_LP, false><<<grid, 256, 0, s>>>(a, m);
+        break;
     }
 }
 void launch_event_lma(const TransportArgs* a, const LmaArgs& m, int lanes, hipStream_t s) {

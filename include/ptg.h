@@ -96,7 +96,8 @@ typedef struct {
     int32_t classical;
     int32_t gaunt_factor;
     int32_t precision;       /* ptg_precision */
-    int32_t reserved;
+    int32_t surrogate;       /* 1: sample LMA emission with the loaded surrogate inside its domain
+                                (ptg_surrogate_load); no counterpart in CollideOptions */
 } ptg_collide_options;
 
 /* Particle set, structure of arrays (particle/mod.rs:66-78). Any pointer except
@@ -137,6 +138,8 @@ typedef struct {
     uint64_t photon_launches;
     uint64_t photon_steps_closed_form; /* photon steps with provably zero pair probability taken in closed
                                           form (not included in photon_steps) */
+    uint64_t surrogate_events;  /* emissions sampled by the surrogate */
+    uint64_t surrogate_fallbacks; /* emissions outside its domain, sampled exactly */
 } ptg_counters;
 
 /* Beam description, mirroring BeamBuilder (src/particle/builder.rs:173-219) for
@@ -263,6 +266,66 @@ typedef enum {
     PTG_FN_COUNT = 28
 } ptg_eval_fn;
 int ptg_eval(ptg_ctx* ctx, int fn, const double* in, size_t n, size_t in_w, double* out, size_t out_w);
+
+/* ------------------------------------------------------------------ learned sampler
+ * A surrogate for the sampling of LMA nonlinear Compton emission. No counterpart
+ * in the reference, which always samples from its tables; it replaces the
+ * harmonic CDF inversion (nonlinear_compton/lp/mod.rs:478-553,
+ * cp/tables/mod.rs:46-121) and the rejection sampling of the lightfront
+ * fraction s and, for LP, of the azimuth theta (lp/mod.rs:586-676,
+ * cp/mod.rs:181-234). The photon's Stokes parameters are still computed from
+ * the exact rate at the sampled point. A 4-layer perceptron 5 -> 128 -> 128 ->
+ * 128 -> 3, SiLU, BF16 weights with FP32 accumulation (layer 1 in FP32):
+ * inputs (ln a, ln eta, u1, u2, u3) scaled to the domain box, outputs
+ * (ln nu / 5, v, t): n = ceil(nu), s = v s_max(n), theta = pi t / 2 in the
+ * first quadrant. a is the sampler's amplitude: sqrt(2) a_rms (LP), a_rms (CP).
+ * Events outside the box are sampled exactly. */
+#define PTG_MLP_NPARAM 38304
+typedef struct {
+    int32_t polarization;    /* ptg_polarization sampled */
+    int32_t valid;           /* passed the validation of ptg_surrogate_train */
+    double a_min, a_max;     /* domain box */
+    double eta_min, eta_max;
+    float params[PTG_MLP_NPARAM]; /* FP32 weights: W1[128][8] b1[128] W2[128][128] b2 W3[128][128] b3 W4[32][128] b4[32] */
+} ptg_surrogate_model;
+
+typedef struct {
+    int32_t polarization;
+    int32_t groups;          /* exact-sampler groups of 64 (s, theta) samples at one (a, eta, n); 1 in 8 held out */
+    int32_t steps;           /* optimiser (Adam) steps */
+    int32_t batch;           /* rows per step, a multiple of 64: half drawn fresh from the harmonic tables */
+    double lr;               /* peak learning rate (cosine decay) */
+    double a_min, a_max, eta_min, eta_max;
+    uint64_t seed;
+} ptg_surrogate_train_desc;
+
+typedef struct {
+    double loss[3];          /* mean losses over the last 10% of steps: harmonic, fraction (mean squares), angle (pinball) */
+    double val_harmonic_rel; /* held-out groups: mean |n_pred - n| / n */
+    double val_fraction_abs; /* held-out groups: mean |v_pred(u2) - v| at the rank-coupled levels */
+    double ks_fraction;      /* held-out samples: Kolmogorov-Smirnov distance of the network's CDF of v at the
+                                samples (probability integral transform, by bisection in u2) from uniform */
+    double ks_angle;         /* the same for theta at the samples' v (LP; 0 for CP) */
+    int32_t valid;           /* val_harmonic_rel < 0.05, ks_fraction < 0.02 and ks_angle < 0.02 */
+    int32_t groups_used;     /* groups that produced 64 samples */
+    uint64_t table_rows;     /* training rows drawn from the harmonic tables */
+    uint64_t sampler_rows;   /* training rows from the exact sampler */
+    double gen_ms, train_ms;
+} ptg_surrogate_report;
+
+/* Generates the training data on the device (the tables and the exact sampler),
+ * trains the network with the library's own forward/backward MFMA kernels, loads
+ * it into the context and returns it in model_out (may be NULL). */
+int ptg_surrogate_train(ptg_ctx* ctx, const ptg_surrogate_train_desc* desc, ptg_surrogate_model* model_out,
+                        ptg_surrogate_report* report);
+/* Loads a model for its polarization (one per polarization). */
+int ptg_surrogate_load(ptg_ctx* ctx, const ptg_surrogate_model* model);
+/* Test probe: forward and backward of the network with params (PTG_MLP_NPARAM FP32)
+ * on n rows (a multiple of 32): x n x 8 inputs, t n x 8 targets (y_nu, v, t,
+ * mask_v, mask_t, tau, 0, 0), w_t the angle-loss weight; returns outputs n x 32,
+ * the gradient of the mean loss (PTG_MLP_NPARAM) and the loss sums [3]. */
+int ptg_mlp_probe(ptg_ctx* ctx, const float* params, const float* x, const float* t, size_t n, float w_t, float* out,
+                  float* grads, float* loss);
 
 #ifdef __cplusplus
 }

@@ -29,7 +29,7 @@ class CollideOptions(C.Structure):
                 ("tracking_photons", C.c_int32), ("tracking_secondaries", C.c_int32),
                 ("keep_decayed_photons", C.c_int32), ("pol_resolved", C.c_int32),
                 ("rotate_stokes_pars", C.c_int32), ("classical", C.c_int32), ("gaunt_factor", C.c_int32),
-                ("precision", C.c_int32), ("reserved", C.c_int32)]
+                ("precision", C.c_int32), ("surrogate", C.c_int32)]
 
 
 class Particles(C.Structure):
@@ -47,7 +47,28 @@ class Counters(C.Structure):
                 ("generations", C.c_uint64), ("rejection_trials", C.c_uint64), ("error_flags", C.c_uint32),
                 ("reserved", C.c_uint32), ("kernel_ms", C.c_double), ("lepton_ms", C.c_double),
                 ("photon_ms", C.c_double), ("lepton_launches", C.c_uint64), ("photon_launches", C.c_uint64),
-                ("photon_steps_closed_form", C.c_uint64)]
+                ("photon_steps_closed_form", C.c_uint64), ("surrogate_events", C.c_uint64),
+                ("surrogate_fallbacks", C.c_uint64)]
+
+
+MLP_NPARAM = 38304
+
+
+class SurrogateModel(C.Structure):
+    _fields_ = [("polarization", C.c_int32), ("valid", C.c_int32), ("a_min", C.c_double), ("a_max", C.c_double),
+                ("eta_min", C.c_double), ("eta_max", C.c_double), ("params", C.c_float * MLP_NPARAM)]
+
+
+class SurrogateTrainDesc(C.Structure):
+    _fields_ = [("polarization", C.c_int32), ("groups", C.c_int32), ("steps", C.c_int32), ("batch", C.c_int32),
+                ("lr", C.c_double), ("a_min", C.c_double), ("a_max", C.c_double), ("eta_min", C.c_double),
+                ("eta_max", C.c_double), ("seed", C.c_uint64)]
+
+
+class SurrogateReport(C.Structure):
+    _fields_ = [("loss", C.c_double * 3), ("val_harmonic_rel", C.c_double), ("val_fraction_abs", C.c_double),
+                ("ks_fraction", C.c_double), ("ks_angle", C.c_double), ("valid", C.c_int32), ("groups_used", C.c_int32), ("table_rows", C.c_uint64),
+                ("sampler_rows", C.c_uint64), ("gen_ms", C.c_double), ("train_ms", C.c_double)]
 
 
 class BeamDesc(C.Structure):
@@ -66,7 +87,8 @@ class HistSpec(C.Structure):
 
 # every extern "C" symbol include/ptg.h declares
 EXPORTS = ("ptg_create", "ptg_destroy", "ptg_last_error", "ptg_set_laser", "ptg_field_info", "ptg_build_beam",
-           "ptg_load_primaries", "ptg_run", "ptg_collide", "ptg_fetch", "ptg_push", "ptg_histogram", "ptg_eval")
+           "ptg_load_primaries", "ptg_run", "ptg_collide", "ptg_fetch", "ptg_push", "ptg_histogram", "ptg_eval",
+           "ptg_surrogate_train", "ptg_surrogate_load", "ptg_mlp_probe")
 
 _lib = None
 
@@ -97,6 +119,10 @@ def load():
     L.ptg_push.argtypes = [vp, P(C.c_uint8), P(d), P(d), C.c_size_t, C.c_uint32, P(CollideOptions), P(C.c_uint32), P(d)]
     L.ptg_histogram.argtypes = [vp, P(HistSpec), P(d), P(d)]
     L.ptg_eval.argtypes = [vp, C.c_int, P(d), C.c_size_t, C.c_size_t, P(d), C.c_size_t]
+    f = C.c_float
+    L.ptg_surrogate_train.argtypes = [vp, P(SurrogateTrainDesc), P(SurrogateModel), P(SurrogateReport)]
+    L.ptg_surrogate_load.argtypes = [vp, P(SurrogateModel)]
+    L.ptg_mlp_probe.argtypes = [vp, P(f), P(f), P(f), C.c_size_t, f, P(f), P(f), P(f)]
     for f in ("ptg_set_laser", "ptg_field_info", "ptg_build_beam", "ptg_load_primaries", "ptg_run", "ptg_collide",
               "ptg_fetch", "ptg_push", "ptg_histogram", "ptg_eval"):
         getattr(L, f).restype = C.c_int

@@ -285,10 +285,10 @@ PTG_HD inline double cp_invert_one(const double* t32, double frac) {
     return exp(x);
 }
 // quantum harmonic (cp/tables/mod.rs:46-121); 0 if the lookup leaves the table
-PTG_HD inline int cp_harmonic(const double* P, const CpScalars& c, double a, double eta, double frac) {
+PTG_HD inline double cp_harmonic_cont(const double* P, const CpScalars& c, double a, double eta, double frac) {
     using namespace cp_pack;
     double la = log(a), le = log(eta);
-    if (la <= c.min[0]) return 1;
+    if (la <= c.min[0]) return 1.0;
     double fx = (la - c.min[0]) / c.step[0];
     if (!(fx < double(COLS - 1))) return 0;
     int ix = int(fx);
@@ -307,7 +307,10 @@ PTG_HD inline int cp_harmonic(const double* P, const CpScalars& c, double a, dou
             cp_invert_one(T + 64 * (COLS * (iy + 1) + ix), frac) * ((1.0 - dx) * dy) +
             cp_invert_one(T + 64 * (COLS * (iy + 1) + ix + 1), frac) * (dx * dy);
     }
-    n = ceil(n);
+    return n;
+}
+PTG_HD inline int cp_harmonic(const double* P, const CpScalars& c, double a, double eta, double frac) {
+    double n = ceil(cp_harmonic_cont(P, c, a, eta, frac));
     return (n >= 1.0 && n <= 1.0e7) ? int(n) : 0;
 }
 // classical harmonic (cp/classical.rs:83-123)

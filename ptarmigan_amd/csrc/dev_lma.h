@@ -447,10 +447,12 @@ __device__ inline double harmonic_from_table(P t16, double a, double frac) {
 }
 // Returns 0 when (a, eta) is outside the table, where the reference's lookup
 // would panic; a valid index is at least 1.
+// The interpolated harmonic before rounding up (lp/mod.rs:478-553): 1 at or below
+// the table in a, 0 outside it.
 template <class P>
-__device__ inline int lp_harmonic(P cdf, const LmaScalars& sc, double a, double eta, double frac) {
+__device__ inline double lp_harmonic_cont(P cdf, const LmaScalars& sc, double a, double eta, double frac) {
     double la = log(a), le = log(eta);
-    if (la <= sc.cdf_min[0]) return 1;
+    if (la <= sc.cdf_min[0]) return 1.0;
     double fx = (la - sc.cdf_min[0]) / sc.cdf_step[0];
     if (!(fx >= 0.0 && fx < double(lma_pack::CDF_COLS - 1))) return 0;
     if (!(le <= sc.cdf_min[1]) && !((le - sc.cdf_min[1]) / sc.cdf_step[1] < double(lma_pack::CDF_N / lma_pack::CDF_COLS - 1)))
@@ -470,7 +472,11 @@ __device__ inline int lp_harmonic(P cdf, const LmaScalars& sc, double a, double 
             harmonic_from_table(cdf + 32 * (C * (iy + 1) + ix), a, frac) * ((1.0 - dx) * dy) +
             harmonic_from_table(cdf + 32 * (C * (iy + 1) + ix + 1), a, frac) * (dx * dy);
     }
-    n = ceil(n);
+    return n;
+}
+template <class P>
+__device__ inline int lp_harmonic(P cdf, const LmaScalars& sc, double a, double eta, double frac) {
+    double n = ceil(lp_harmonic_cont(cdf, sc, a, eta, frac));
     return (n >= 1.0 && n <= 1.0e7) ? int(n) : 0;
 }
 

@@ -3,6 +3,7 @@
 // declared in include/ptg.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -15,6 +16,7 @@
 #include "kern_lma.h"
 #include "dev_bw.h"
 #include "kern_misc.h"
+#include "mlp_host.h"
 
 using namespace ptg;
 
@@ -77,7 +79,8 @@ void free_buf(DevBuf& b) {
 }
 
 // Counter block layout (unsigned long long)
-enum { C_WORK = 0, C_NEXT, C_OUT, C_INTER, C_ID, C_FLAGS, C_EV, C_LNEXT, C_PH, C_OVF, C_OVFLEN, C_PAIRS, C_STATS, C_N = C_STATS + STAT_N };
+enum { C_WORK = 0, C_NEXT, C_OUT, C_INTER, C_ID, C_FLAGS, C_EV, C_LNEXT, C_PH, C_OVF, C_OVFLEN, C_PAIRS, C_SROWS, C_SFB, C_STATS,
+       C_N = C_STATS + STAT_N };
 
 std::string default_table_path() {
     Dl_info info;
@@ -129,6 +132,14 @@ struct ptg_ctx {
     TransportArgs* d_args = nullptr;
     uint64_t n_out = 0, n_inter = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // learned sampler, one per polarization (0 linear, 1 circular)
+    MlpNet surr[2];
+    bool surr_loaded[2] = {false, false};
+    bool surr_valid[2] = {false, false};
+    double surr_box[2][4] = {};    // a_min, a_max, eta_min, eta_max
+    uint32_t* d_surr_list = nullptr;  // row -> event, and the exact sampler's list
+    uint32_t* d_surr_fb = nullptr;
+    size_t surr_list_cap = 0;
 };
 
 namespace {
@@ -487,6 +498,10 @@ void ptg_destroy(ptg_ctx* c) {
     if (c->d_lcfa) (void)hipFree(c->d_lcfa);
     if (c->d_lma) (void)hipFree(c->d_lma);
     if (c->d_lma_f) (void)hipFree(c->d_lma_f);
+    c->surr[0].release();
+    c->surr[1].release();
+    if (c->d_surr_list) (void)hipFree(c->d_surr_list);
+    if (c->d_surr_fb) (void)hipFree(c->d_surr_fb);
     if (c->d_bw) (void)hipFree(c->d_bw);
     if (c->d_cp) (void)hipFree(c->d_cp);
     if (c->d_ev_extra) (void)hipFree(c->d_ev_extra);
@@ -706,7 +721,8 @@ static int ensure_ev_extra(ptg_ctx* c, size_t n) {
 // events in evq. Events whose harmonic needs a J(n, x, y) workspace larger than
 // m.scratch_len are deferred and run again with one sized for the largest; their
 // RNG counters make the second pass draw the same numbers.
-static int lma_event_pass(ptg_ctx* c, LmaArgs m, bool pairs, size_t n_ev, size_t max_lanes, double* ms, uint64_t* launches) {
+static int lma_event_pass(ptg_ctx* c, LmaArgs m, bool pairs, size_t n_ev, size_t max_lanes, double* ms, uint64_t* launches,
+                          const uint32_t* list = nullptr) {
     if (c->ovf_cap < c->evq.cap) {
         if (c->d_ovf) (void)hipFree(c->d_ovf);
         c->d_ovf = nullptr;
@@ -715,7 +731,7 @@ static int lma_event_pass(ptg_ctx* c, LmaArgs m, bool pairs, size_t n_ev, size_t
     }
     HIPCHK(c, hipMemsetAsync(c->d_ctr + C_OVF, 0, 2 * sizeof(unsigned long long), c->stream));
     m.ev_in = c->evq.v;
-    m.ev_list = nullptr;
+    m.ev_list = list;
     m.n_ev = n_ev;
     m.ovf_list = c->d_ovf;
     m.ovf_ctr = c->d_ctr + C_OVF;
@@ -764,6 +780,76 @@ static int lma_event_pass(ptg_ctx* c, LmaArgs m, bool pairs, size_t n_ev, size_t
     return PTG_OK;
 }
 
+// Box of the surrogate's domain in SurrArgs terms
+static void surr_box_args(const double* box, SurrArgs& S) {
+    const double la0 = std::log(box[0]), la1 = std::log(box[1]), le0 = std::log(box[2]), le1 = std::log(box[3]);
+    S.la_c = 0.5 * (la0 + la1);
+    S.la_h = 0.5 * (la1 - la0);
+    S.le_c = 0.5 * (le0 + le1);
+    S.le_h = 0.5 * (le1 - le0);
+}
+
+// Emission events of one generation through the learned sampler: inputs for
+// the events inside its domain, the network's forward pass, the photons; the
+// other events (outside the domain, or an LP harmonic beyond the workspace) go
+// to the exact kernels.
+static int lma_surrogate_pass(ptg_ctx* c, LmaArgs m, size_t n_ev, size_t max_lanes, double* ms, uint64_t* launches,
+                              uint64_t* n_fallback) {
+    const int p = m.mode == LMA_CP ? 1 : 0;
+    MlpNet& net = c->surr[p];
+    if (c->surr_list_cap < n_ev) {
+        if (c->d_surr_list) (void)hipFree(c->d_surr_list);
+        if (c->d_surr_fb) (void)hipFree(c->d_surr_fb);
+        c->d_surr_list = c->d_surr_fb = nullptr;
+        c->surr_list_cap = 0;
+        const size_t cap = n_ev + n_ev / 4 + 64;
+        HIPCHK(c, hipMalloc(&c->d_surr_list, cap * sizeof(uint32_t)));
+        HIPCHK(c, hipMalloc(&c->d_surr_fb, cap * sizeof(uint32_t)));
+        c->surr_list_cap = cap;
+    }
+    const size_t rows_cap = (n_ev + 31) / 32 * 32;
+    if (rows_cap > size_t(1) << 30) return fail(c, PTG_E_OOM, "too many emission events for the surrogate's buffers");
+    HIPCHK(c, net.reserve(int(rows_cap)));
+    HIPCHK(c, hipMemsetAsync(c->d_ctr + C_SROWS, 0, 2 * sizeof(unsigned long long), c->stream));
+    SurrArgs S{};
+    surr_box_args(c->surr_box[p], S);
+    S.x = net.X;
+    S.out = net.O;
+    S.list = c->d_surr_list;
+    S.n_rows = c->d_ctr + C_SROWS;
+    S.fb = c->d_surr_fb;
+    S.n_fb = c->d_ctr + C_SFB;
+    S.n_ev = n_ev;
+    m.ev_in = c->evq.v;
+    m.ev_list = nullptr;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    launch_surr_inputs(c->d_args, m, S, c->stream);
+    HIPCHK(c, hipGetLastError());
+    unsigned long long cnt[2] = {0, 0};
+    HIPCHK(c, hipMemcpyAsync(cnt, c->d_ctr + C_SROWS, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const size_t n_rows = size_t(cnt[0]);
+    if (n_rows > 0) {
+        const size_t M = (n_rows + 31) / 32 * 32;
+        if (M > n_rows) HIPCHK(c, hipMemsetAsync(net.X + 8 * n_rows, 0, (M - n_rows) * 8 * sizeof(float), c->stream));
+        net.forward(int(M), false, c->stream);
+        const size_t lanes = std::min<size_t>(max_lanes, (n_rows + 63) / 64 * 64);
+        launch_surr_event(c->d_args, m, S, int(lanes), c->stream);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipMemcpyAsync(cnt, c->d_ctr + C_SROWS, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float t = 0.0f;
+    (void)hipEventElapsedTime(&t, c->ev0, c->ev1);
+    *ms += t;
+    ++*launches;
+    const size_t n_fb = size_t(cnt[1]);
+    *n_fallback += n_fb;
+    if (n_fb > 0) return lma_event_pass(c, m, false, n_fb, max_lanes, ms, launches, c->d_surr_fb);
+    return PTG_OK;
+}
+
 static unsigned long long read_ctr(ptg_ctx* c, int which) {
     unsigned long long v = 0;
     if (scopy(c, &v, c->d_ctr + which, sizeof(v), hipMemcpyDeviceToHost) != hipSuccess) return ~0ull;
@@ -776,7 +862,7 @@ static unsigned long long read_ctr(ptg_ctx* c, int which) {
 // kernel, then the pair-event kernel, until no photon is left; the pairs start
 // the next lepton phase. Without photon tracking, photons are final state and
 // those of the primary beam only pass the final cuts.
-static int run_lma(ptg_ctx* c, TransportArgs a, double* ms, uint64_t* launches, uint64_t* gens) {
+static int run_lma(ptg_ctx* c, TransportArgs a, double* ms, uint64_t* launches, uint64_t* gens, bool use_surr, uint64_t* surr_fb) {
     const size_t n = c->n_prim;
     const bool track = a.opt.tracking_photons != 0;
     int len_cap = 0;
@@ -879,7 +965,8 @@ static int run_lma(ptg_ctx* c, TransportArgs a, double* ms, uint64_t* launches, 
             if (n_ev > 0) {
                 m.next = nxt.v;
                 m.next_ctr = c->d_ctr + C_LNEXT;
-                rc = lma_event_pass(c, m, false, size_t(n_ev), max_lanes, &ms[1], &launches[1]);
+                rc = use_surr ? lma_surrogate_pass(c, m, size_t(n_ev), max_lanes, &ms[1], &launches[1], surr_fb)
+                              : lma_event_pass(c, m, false, size_t(n_ev), max_lanes, &ms[1], &launches[1]);
                 if (rc) return rc;
             }
             unsigned long long now[C_N];
@@ -977,6 +1064,14 @@ int ptg_run(ptg_ctx* c, const ptg_collide_options* o, uint64_t seed, ptg_counter
     if (c->n_prim == 0) return fail(c, PTG_E_STATE, "no primaries resident (ptg_load_primaries / ptg_build_beam)");
     int rc = validate_options(c, o);
     if (rc) return rc;
+    if (o->surrogate) {
+        if (!(c->laser.kind == K_PLANE_WAVE || c->laser.kind == K_FOCUSED_LASER))
+            return fail(c, PTG_E_UNSUPPORTED, "the surrogate samples LMA emission (PlaneWave, FocusedLaser)");
+        if (o->classical) return fail(c, PTG_E_UNSUPPORTED, "the surrogate samples quantum LMA emission (classical: false)");
+        const int p = c->laser.circular ? 1 : 0;
+        if (!c->surr_loaded[p]) return fail(c, PTG_E_STATE, "no surrogate loaded for this polarization (ptg_surrogate_load)");
+        if (!c->surr_valid[p]) return fail(c, PTG_E_STATE, "the loaded surrogate did not pass its validation");
+    }
     if (o->precision == PTG_PREC_MIXED) {
         if (!(c->laser.kind == K_PLANE_WAVE || c->laser.kind == K_FOCUSED_LASER))
             return fail(c, PTG_E_UNSUPPORTED, "mixed precision is implemented for the LMA fields (PlaneWave, FocusedLaser)");
@@ -1054,8 +1149,9 @@ int ptg_run(ptg_ctx* c, const ptg_collide_options* o, uint64_t seed, ptg_counter
     double ms[2] = {0.0, 0.0};  // lepton, photon kernels
     uint64_t launches[2] = {0, 0};
     uint64_t gens = 0;
+    uint64_t surr_fb = 0;
     if (c->laser.kind == K_PLANE_WAVE || c->laser.kind == K_FOCUSED_LASER) {
-        rc = run_lma(c, a, ms, launches, &gens);
+        rc = run_lma(c, a, ms, launches, &gens, o->surrogate != 0, &surr_fb);
         if (rc) return rc;
     } else {
         // Generation loop: primaries and pairs are electrons/positrons (or photons,
@@ -1110,6 +1206,8 @@ int ptg_run(ptg_ctx* c, const ptg_collide_options* o, uint64_t seed, ptg_counter
         cnt->lepton_launches = launches[0];
         cnt->photon_launches = launches[1];
         cnt->photon_steps_closed_form = fin[C_STATS + STAT_PHOTON_STEPS_CF];
+        cnt->surrogate_events = fin[C_STATS + STAT_SURROGATE];
+        cnt->surrogate_fallbacks = surr_fb;
     }
     return PTG_OK;
 }
@@ -1387,5 +1485,289 @@ int ptg_eval(ptg_ctx* c, int fn, const double* in, size_t n, size_t in_w, double
     (void)hipFree(dout);
     return PTG_OK;
 }
+
+static int check_surrogate_box(ptg_ctx* c, double a_min, double a_max, double eta_min, double eta_max) {
+    if (!(a_min > 0.0 && a_max > a_min && a_max <= 20.0))
+        return fail(c, PTG_E_ARG, "surrogate domain needs 0 < a_min < a_max <= 20 (the LMA rate tables end at a = 20)");
+    if (!(eta_min > 0.0 && eta_max > eta_min && eta_max <= 2.0))
+        return fail(c, PTG_E_ARG, "surrogate domain needs 0 < eta_min < eta_max <= 2");
+    return PTG_OK;
+}
+
+int ptg_surrogate_load(ptg_ctx* c, const ptg_surrogate_model* model) {
+    if (!c || !model) return PTG_E_ARG;
+    if (model->polarization != PTG_LINEAR && model->polarization != PTG_CIRCULAR) return fail(c, PTG_E_ARG, "unknown polarization");
+    int rc = check_surrogate_box(c, model->a_min, model->a_max, model->eta_min, model->eta_max);
+    if (rc) return rc;
+    for (int i = 0; i < PTG_MLP_NPARAM; ++i)
+        if (!std::isfinite(model->params[i])) return fail(c, PTG_E_ARG, "surrogate parameters must be finite");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int p = model->polarization == PTG_CIRCULAR ? 1 : 0;
+    HIPCHK(c, c->surr[p].init_params(model->params, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->surr_box[p][0] = model->a_min;
+    c->surr_box[p][1] = model->a_max;
+    c->surr_box[p][2] = model->eta_min;
+    c->surr_box[p][3] = model->eta_max;
+    c->surr_loaded[p] = true;
+    c->surr_valid[p] = model->valid != 0;
+    return PTG_OK;
+}
+
+int ptg_mlp_probe(ptg_ctx* c, const float* params, const float* x, const float* t, size_t n, float w_t, float* out, float* grads,
+                  float* loss) {
+    if (!c || !params || !x || !t || !out || !grads || !loss) return PTG_E_ARG;
+    if (n == 0 || n % 32 != 0 || n > (size_t(1) << 20)) return fail(c, PTG_E_ARG, "the probe needs n rows, a multiple of 32, at most 2^20");
+    HIPCHK(c, hipSetDevice(c->device));
+    MlpNet net;
+    int rc = PTG_OK;
+    hipError_t e = net.init_params(params, c->stream);
+    if (!e) e = net.reserve(int(n));
+    if (!e) e = hipMemcpyAsync(net.X, x, n * 8 * sizeof(float), hipMemcpyHostToDevice, c->stream);
+    if (!e) e = hipMemcpyAsync(net.T, t, n * 8 * sizeof(float), hipMemcpyHostToDevice, c->stream);
+    if (!e) {
+        net.forward(int(n), true, c->stream);
+        net.backward(int(n), w_t, c->stream);
+        e = hipGetLastError();
+    }
+    if (!e) e = hipMemcpyAsync(out, net.O, n * mlp::OUT * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    if (!e) e = hipMemcpyAsync(grads, net.G, mlp::NPARAM * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    if (!e) e = hipMemcpyAsync(loss, net.loss, 3 * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    if (!e) e = hipStreamSynchronize(c->stream);
+    if (e) rc = fail(c, PTG_E_HIP, std::string("mlp probe: ") + hipGetErrorString(e));
+    net.release();
+    return rc;
+}
+
+int ptg_surrogate_train(ptg_ctx* c, const ptg_surrogate_train_desc* d, ptg_surrogate_model* model_out, ptg_surrogate_report* rep) {
+    if (!c || !d) return PTG_E_ARG;
+    if (d->polarization != PTG_LINEAR && d->polarization != PTG_CIRCULAR) return fail(c, PTG_E_ARG, "unknown polarization");
+    if (!(d->groups >= 64 && d->groups <= (1 << 22))) return fail(c, PTG_E_ARG, "groups must be in [64, 2^22]");
+    if (!(d->steps >= 1 && d->steps <= 10000000)) return fail(c, PTG_E_ARG, "steps must be in [1, 1e7]");
+    if (!(d->batch >= 64 && d->batch % 64 == 0 && d->batch <= (1 << 16))) return fail(c, PTG_E_ARG, "batch must be a multiple of 64 in [64, 65536]");
+    if (!(d->lr > 0.0 && d->lr < 1.0)) return fail(c, PTG_E_ARG, "lr must be in (0, 1)");
+    int rc = check_surrogate_box(c, d->a_min, d->a_max, d->eta_min, d->eta_max);
+    if (rc) return rc;
+    const bool circ = d->polarization == PTG_CIRCULAR;
+    if (circ && !c->d_cp) return fail(c, PTG_E_UNSUPPORTED, "CP emission needs the regenerated CP CDF table");
+    HIPCHK(c, hipSetDevice(c->device));
+    LmaArgs m{};
+    m.tables = c->d_lma;
+    m.sc = c->lsc;
+    m.cp = c->d_cp;
+    m.csc = c->csc;
+    m.lcfa = c->d_lcfa;
+    m.mode = circ ? LMA_CP : LMA_LP;
+    const double box[4] = {d->a_min, d->a_max, d->eta_min, d->eta_max};
+    SurrArgs S{};
+    surr_box_args(box, S);
+    S.key = key_from_seed(d->seed);
+    S.n_groups = uint32_t(d->groups);
+    // ---- exact-sampler groups (the checkpointed J workspace for LP)
+    const int ncap = circ ? 16 : lma_harmonic_cap(d->a_max);
+    const double xm = ncap * 1.4142135623730951, ym = ncap * 0.5;
+    const int len_cap = int(std::ceil(xm)) - int(std::floor(-2.0 * ym - xm)) + 200;
+    const size_t per_lane = size_t(5) * size_t(len_cap / PTG_J_SEG + 2) * sizeof(double);
+    size_t lanes = std::min<size_t>(size_t(d->groups) * 64, std::min<size_t>(262144, (size_t(8) << 30) / per_lane));
+    lanes = std::max<size_t>(64, lanes / 64 * 64);
+    double* scratch = nullptr;
+    double* gh = nullptr;
+    float* gs = nullptr;
+    float* drows = nullptr;
+    MlpNet net;
+    auto cleanup = [&]() {
+        if (scratch) (void)hipFree(scratch);
+        if (gh) (void)hipFree(gh);
+        if (gs) (void)hipFree(gs);
+        if (drows) (void)hipFree(drows);
+        net.release();
+    };
+    auto hip_fail = [&](hipError_t e) {
+        cleanup();
+        return fail(c, e == hipErrorOutOfMemory ? PTG_E_OOM : PTG_E_HIP, std::string("surrogate training: ") + hipGetErrorString(e));
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&scratch, lanes * per_lane))) return hip_fail(e);
+    if ((e = hipMalloc(&gh, size_t(d->groups) * 4 * sizeof(double)))) return hip_fail(e);
+    if ((e = hipMalloc(&gs, size_t(d->groups) * 128 * sizeof(float)))) return hip_fail(e);
+    m.scratch = scratch;
+    m.scratch_len = len_cap;
+    S.gh = gh;
+    S.gs = gs;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    launch_surr_groups(m, S, int(lanes), c->stream);
+    if ((e = hipGetLastError())) return hip_fail(e);
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    std::vector<double> hgh(size_t(d->groups) * 4);
+    std::vector<float> hgs(size_t(d->groups) * 128);
+    if ((e = hipMemcpyAsync(hgh.data(), gh, hgh.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream))) return hip_fail(e);
+    if ((e = hipMemcpyAsync(hgs.data(), gs, hgs.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream))) return hip_fail(e);
+    if ((e = hipStreamSynchronize(c->stream))) return hip_fail(e);
+    float gen_ms = 0.0f;
+    (void)hipEventElapsedTime(&gen_ms, c->ev0, c->ev1);
+    // rows: each group's 64 samples sorted by v, at quantile levels (k + 1/2) / 64;
+    // every eighth group held out for validation
+    std::vector<float> rows, vrows;
+    int used = 0;
+    for (int g = 0; g < d->groups; ++g) {
+        const double* h = &hgh[size_t(g) * 4];
+        if (!(h[3] > 0.0)) continue;
+        ++used;
+        const float* smp = &hgs[size_t(g) * 128];
+        int idx[64];
+        for (int k = 0; k < 64; ++k) idx[k] = k;
+        std::sort(idx, idx + 64, [&](int x, int y) { return smp[2 * x] < smp[2 * y]; });
+        std::vector<float>& dst = (g % 8 == 7) ? vrows : rows;
+        for (int k = 0; k < 64; ++k) {
+            const float row[8] = {float(h[0]), float(h[1]), float(h[2]), float((k + 0.5) / 64.0),
+                                  float(std::log(std::max(h[3], 0.9)) / 5.0), smp[2 * idx[k]], smp[2 * idx[k] + 1], 0.0f};
+            dst.insert(dst.end(), row, row + 8);
+        }
+    }
+    if (rows.empty() || vrows.empty()) {
+        cleanup();
+        return fail(c, PTG_E_STATE, "surrogate training: the exact sampler produced too few groups");
+    }
+    if ((e = hipMalloc(&drows, rows.size() * sizeof(float)))) return hip_fail(e);
+    if ((e = hipMemcpyAsync(drows, rows.data(), rows.size() * sizeof(float), hipMemcpyHostToDevice, c->stream))) return hip_fail(e);
+    // ---- training
+    std::vector<float> p0;
+    mlp_init_params(p0, d->seed ^ 0x9e3779b97f4a7c15ull);
+    if ((e = net.init_params(p0.data(), c->stream))) return hip_fail(e);
+    const size_t nv = std::min<size_t>(vrows.size() / 8, 65536);  // held-out rows validated (whole groups)
+    const int vM = int((nv + 31) / 32 * 32);
+    if ((e = net.reserve(std::max(d->batch, vM)))) return hip_fail(e);
+    S.rows = drows;
+    S.n_sampler_rows = uint32_t(rows.size() / 8);
+    S.tx = net.X;
+    S.tt = net.T;
+    S.batch = d->batch;
+    S.n_table_rows = d->batch / 2;
+    double lsum[3] = {0.0, 0.0, 0.0};
+    int lcount = 0;
+    const int tail = std::max(1, d->steps / 10);
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int step = 0; step < d->steps; ++step) {
+        S.step = uint32_t(step);
+        launch_surr_batch(m, S, c->stream);
+        net.forward(d->batch, true, c->stream);
+        net.backward(d->batch, 1.0f, c->stream);
+        const double lr = d->lr * 0.5 * (1.0 + std::cos(PI_H * double(step) / double(d->steps)));
+        net.adam(float(lr), c->stream);
+        if (step >= d->steps - tail) {
+            float l[3];
+            if ((e = hipMemcpyAsync(l, net.loss, sizeof(l), hipMemcpyDeviceToHost, c->stream))) return hip_fail(e);
+            if ((e = hipStreamSynchronize(c->stream))) return hip_fail(e);
+            lsum[0] += l[0] / d->batch;  // harmonic: every row
+            lsum[1] += l[1] / (d->batch / 2);  // fraction, angle: the sampler rows
+            lsum[2] += l[2] / (d->batch / 2);
+            ++lcount;
+        }
+    }
+    if ((e = hipGetLastError())) return hip_fail(e);
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    if ((e = hipStreamSynchronize(c->stream))) return hip_fail(e);
+    float train_ms = 0.0f;
+    (void)hipEventElapsedTime(&train_ms, c->ev0, c->ev1);
+    // ---- validation on the held-out groups
+    std::vector<float> vx(size_t(vM) * 8, 0.0f), vo(size_t(vM) * mlp::OUT);
+    for (size_t r = 0; r < nv; ++r) {
+        const float* row = &vrows[r * 8];
+        vx[r * 8 + 0] = float((row[0] - S.la_c) / S.la_h);
+        vx[r * 8 + 1] = float((row[1] - S.le_c) / S.le_h);
+        vx[r * 8 + 2] = 2.0f * row[2] - 1.0f;
+        vx[r * 8 + 3] = 2.0f * row[3] - 1.0f;
+        vx[r * 8 + 4] = 0.0f;
+    }
+    auto infer = [&]() -> hipError_t {
+        hipError_t err = hipMemcpyAsync(net.X, vx.data(), vx.size() * sizeof(float), hipMemcpyHostToDevice, c->stream);
+        if (err) return err;
+        net.forward(vM, false, c->stream);
+        if ((err = hipMemcpyAsync(vo.data(), net.O, vo.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream))) return err;
+        return hipStreamSynchronize(c->stream);
+    };
+    if ((e = infer())) return hip_fail(e);
+    double h_rel = 0.0, v_abs = 0.0;
+    for (size_t r = 0; r < nv; ++r) {
+        const double nu = std::exp(5.0 * double(vrows[r * 8 + 4]));
+        const double n_true = std::max(1.0, std::ceil(nu));
+        const double n_pred = std::max(1.0, std::ceil(std::exp(5.0 * double(vo[r * mlp::OUT]))));
+        h_rel += std::fabs(n_pred - n_true) / n_true;
+        v_abs += std::fabs(std::min(std::max(double(vo[r * mlp::OUT + 1]), 0.0), 1.0) - double(vrows[r * 8 + 5]));
+    }
+    h_rel /= double(nv);
+    v_abs /= double(nv);
+    // probability integral transforms of the held-out samples under the network:
+    // the level u2 at which the network's v reaches the sample's v (bisection),
+    // then, at that level, the u3 at which its theta reaches the sample's theta;
+    // uniform if the network has the samples' distribution
+    auto pit = [&](int col_in, int col_out, int col_target, std::vector<double>& level) -> hipError_t {
+        std::vector<double> lo(nv, 0.0), hi(nv, 1.0);
+        for (int it = 0; it < 20; ++it) {
+            for (size_t r = 0; r < nv; ++r) vx[r * 8 + col_in] = float(lo[r] + hi[r] - 1.0);  // 2 mid - 1
+            hipError_t err = infer();
+            if (err) return err;
+            for (size_t r = 0; r < nv; ++r) {
+                const double mid = 0.5 * (lo[r] + hi[r]);
+                if (vo[r * mlp::OUT + col_out] < vrows[r * 8 + col_target]) lo[r] = mid;
+                else hi[r] = mid;
+            }
+        }
+        level.resize(nv);
+        for (size_t r = 0; r < nv; ++r) level[r] = 0.5 * (lo[r] + hi[r]);
+        return hipSuccess;
+    };
+    auto ks_uniform = [&](std::vector<double> u) {
+        std::sort(u.begin(), u.end());
+        double dmax = 0.0;
+        const double N = double(u.size());
+        for (size_t i = 0; i < u.size(); ++i) dmax = std::max(dmax, std::max(double(i + 1) / N - u[i], u[i] - double(i) / N));
+        return dmax;
+    };
+    std::vector<double> pit_v, pit_t;
+    for (size_t r = 0; r < nv; ++r) vx[r * 8 + 4] = 0.0f;
+    if ((e = pit(3, 1, 5, pit_v))) return hip_fail(e);
+    const double ks_v = ks_uniform(pit_v);
+    double ks_t = 0.0;
+    if (!circ) {
+        for (size_t r = 0; r < nv; ++r) vx[r * 8 + 3] = float(2.0 * pit_v[r] - 1.0);
+        if ((e = pit(4, 2, 6, pit_t))) return hip_fail(e);
+        ks_t = ks_uniform(pit_t);
+    }
+    const bool valid = h_rel < 0.05 && ks_v < 0.02 && ks_t < 0.02;
+    // ---- export and load
+    ptg_surrogate_model* mdl = new ptg_surrogate_model();
+    mdl->polarization = d->polarization;
+    mdl->valid = valid ? 1 : 0;
+    mdl->a_min = d->a_min;
+    mdl->a_max = d->a_max;
+    mdl->eta_min = d->eta_min;
+    mdl->eta_max = d->eta_max;
+    if ((e = net.download_params(mdl->params, c->stream))) {
+        delete mdl;
+        return hip_fail(e);
+    }
+    cleanup();
+    rc = ptg_surrogate_load(c, mdl);
+    if (!rc && model_out) std::memcpy(model_out, mdl, sizeof(*mdl));
+    delete mdl;
+    if (rc) return rc;
+    if (rep) {
+        std::memset(rep, 0, sizeof(*rep));
+        for (int k = 0; k < 3; ++k) rep->loss[k] = lcount ? lsum[k] / lcount : 0.0;
+        rep->val_harmonic_rel = h_rel;
+        rep->val_fraction_abs = v_abs;
+        rep->ks_fraction = ks_v;
+        rep->ks_angle = ks_t;
+        rep->valid = valid ? 1 : 0;
+        rep->groups_used = used;
+        rep->table_rows = uint64_t(d->steps) * uint64_t(d->batch / 2);
+        rep->sampler_rows = uint64_t(d->steps) * uint64_t(d->batch - d->batch / 2);
+        rep->gen_ms = gen_ms;
+        rep->train_ms = train_ms;
+    }
+    return PTG_OK;
+}
+
 
 }  // extern "C"

@@ -10,7 +10,7 @@
 
 namespace ptg {
 
-enum { STAT_LEPTON_STEPS = 0, STAT_PHOTON_STEPS, STAT_EMISSIONS, STAT_PAIRS, STAT_TRIALS, STAT_PHOTON_STEPS_CF, STAT_N };
+enum { STAT_LEPTON_STEPS = 0, STAT_PHOTON_STEPS, STAT_EMISSIONS, STAT_PAIRS, STAT_TRIALS, STAT_PHOTON_STEPS_CF, STAT_SURROGATE, STAT_N };
 
 // CollideOptions as the kernels need them (src/main.rs:72-105).
 struct DevOptions {

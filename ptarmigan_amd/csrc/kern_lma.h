@@ -5,6 +5,10 @@
 #include "dev_lma.h"
 #include "kern_common.h"
 
+#ifndef PTG_J_SEG
+#define PTG_J_SEG 7  // J(n, x, y) forward states per checkpoint: 17.5 KB of LDS per wave
+#endif
+
 namespace ptg {
 
 // Emission model of the lepton kernels: linear or circular polarisation of the
@@ -54,6 +58,36 @@ struct LmaArgs {
     unsigned long long* pairs_ctr;
     unsigned long long pairs_cap;
 };
+
+// The learned sampler (surrogate) of LMA emission: inputs built from queued
+// events, outputs turned into photons; training data from the tables and the
+// exact sampler.
+struct SurrArgs {
+    double la_c, la_h, le_c, le_h;   // domain box: centre and half width of ln a and ln eta
+    // inference
+    float* x;                        // [rows][8] network inputs
+    const float* out;                // [rows][32] network outputs
+    uint32_t* list;                  // row -> event index (events the network samples)
+    unsigned long long* n_rows;
+    uint32_t* fb;                    // events left to the exact sampler
+    unsigned long long* n_fb;
+    unsigned long long n_ev;         // events queued by the step kernel
+    // training data
+    uint64_t key;                    // Philox key of the training draws
+    uint32_t n_groups;
+    double* gh;                      // [groups][4]: ln a, ln eta, u1, nu (nu = 0: no samples)
+    float* gs;                       // [groups][64][2]: v = s / s_max, t = 2 theta / pi
+    const float* rows;               // [n_sampler_rows][8]: ln a, ln eta, u1, u2, y_nu, v, t, 0
+    uint32_t n_sampler_rows;
+    float* tx;                       // batch inputs [batch][8]
+    float* tt;                       // batch targets [batch][8]
+    int batch, n_table_rows;
+    uint32_t step;
+};
+void launch_surr_inputs(const TransportArgs* a, const LmaArgs& m, const SurrArgs& s, hipStream_t st);
+void launch_surr_event(const TransportArgs* a, const LmaArgs& m, const SurrArgs& s, int lanes, hipStream_t st);
+void launch_surr_groups(const LmaArgs& m, const SurrArgs& s, int lanes, hipStream_t st);
+void launch_surr_batch(const LmaArgs& m, const SurrArgs& s, hipStream_t st);
 
 void launch_step_lma(const TransportArgs* a, const LmaArgs& m, int grid, hipStream_t s);
 void launch_event_lma(const TransportArgs* a, const LmaArgs& m, int lanes, hipStream_t s);

@@ -304,9 +304,6 @@ __device__ __forceinline__ double wave_fmax(double v) {
 #ifndef PTG_LMA_WAVE_WPE
 #define PTG_LMA_WAVE_WPE 2  // two waves per SIMD: 256 VGPRs, a little scratch
 #endif
-#ifndef PTG_J_SEG
-#define PTG_J_SEG 7  // J(n, x, y) forward states per checkpoint: 17.5 KB of LDS per wave
-#endif
 template <int MODE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PTG_LMA_WAVE_WPE))) void k_event_lma_wave(const TransportArgs* __restrict__ A, LmaArgs M) {
     constexpr bool classical = MODE == LMA_LP_CLASSICAL;
@@ -722,6 +719,261 @@ __global__ void k_push_lma(LaserDev L, double dt, int ll, const uint8_t* species
 void launch_push_lma(const LaserDev& L, double dt, int ll, const uint8_t* sp_, double* r4, double* u4, uint32_t n,
                      uint32_t n_steps, uint32_t* taken, double* work, hipStream_t s) {
     k_push_lma<<<(n + 255) / 256, 256, 0, s>>>(L, dt, ll, sp_, r4, u4, n, n_steps, taken, work);
+}
+
+// ================================================================ learned sampler (surrogate)
+// The network maps (ln a, ln eta, u1, u2, u3), scaled to its domain box, to
+// (ln nu / 5, v, t): the harmonic n = ceil(nu) (nu stands in for the CDF-table
+// inversion of lp/mod.rs:478-553, cp/tables/mod.rs:46-121), the lightfront
+// fraction s = v s_max(n) and, for LP, the first-quadrant azimuth 2 t / pi
+// (stand-ins for the rejection sampling of lp/mod.rs:586-676, cp/mod.rs:181-234).
+// The photon's Stokes parameters are computed from the exact rate at the sampled
+// point, and the photon is built by lma_emit_finish as in the exact kernels.
+// a is the sampler's amplitude: sqrt(2) a_rms for LP, a_rms for CP.
+__device__ __forceinline__ bool surr_in_domain(const SurrArgs& S, double la, double le) {
+    return fabs(la - S.la_c) <= S.la_h && fabs(le - S.le_c) <= S.le_h;
+}
+__device__ __forceinline__ void surr_features(const SurrArgs& S, double la, double le, double u1, double u2, double u3, float* x) {
+    x[0] = float((la - S.la_c) / S.la_h);
+    x[1] = float((le - S.le_c) / S.le_h);
+    x[2] = float(2.0 * u1 - 1.0);
+    x[3] = float(2.0 * u2 - 1.0);
+    x[4] = float(2.0 * u3 - 1.0);
+    x[5] = 0.0f; x[6] = 0.0f; x[7] = 0.0f;
+}
+template <bool CIRC>
+__device__ __forceinline__ double surr_nu(const LmaArgs& M, double a, double eta, double frac) {
+    return CIRC ? cp_harmonic_cont(M.cp, M.csc, a, eta, frac) : lp_harmonic_cont(M.tables + lma_pack::CDF, M.sc, a, eta, frac);
+}
+
+// Queued events -> network rows (inside the domain) or the exact sampler's list.
+// u1 is the exact path's harmonic draw (counter 0); u2, u3 come from counter 2.
+template <bool CIRC>
+__global__ void k_surr_inputs(const TransportArgs* __restrict__ A, LmaArgs M, SurrArgs S) {
+    const uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= S.n_ev) return;
+    const PBuf& q = M.ev;
+    const uint32_t i = uint32_t(e);
+    const double ut = q.d[P_UT][i], ux = q.d[P_UX][i], uy = q.d[P_UY][i], uz = q.d[P_UZ][i];
+    const double a_rms = sqrt(ut * ut - ux * ux - uy * uy - uz * uz - 1.0);
+    const double eta = M.ev_k0[i] * (ut - uz);
+    const double amp = CIRC ? a_rms : a_rms * 1.4142135623730951;
+    Rng base = rng_make(A->key, q.stream[i]);
+    base.c1 = q.nev[i] | EVENT_BLOCK;
+    Rng g0 = base;
+    const double u1 = rng_uniform(g0);
+    Rng g2 = base;
+    g2.c0 = 2u;
+    double u2, u3;
+    rng_uniform2(g2, u2, u3);
+    const double la = log(amp), le = log(eta);
+    if (surr_in_domain(S, la, le)) {
+        const unsigned long long j = atomicAdd(S.n_rows, 1ull);
+        S.list[j] = i;
+        surr_features(S, la, le, u1, u2, u3, S.x + 8 * j);
+    } else {
+        S.fb[atomicAdd(S.n_fb, 1ull)] = i;
+    }
+}
+
+// Network outputs -> photons. One lane per row; LP Stokes parameters need a
+// J(n, x, y) workspace per lane (events whose harmonic outgrows it go to the
+// exact sampler's list).
+template <bool CIRC>
+__global__ __launch_bounds__(64) void k_surr_event(const TransportArgs* __restrict__ A, LmaArgs M, SurrArgs S) {
+    const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n_lanes = gridDim.x * blockDim.x;
+    double* ws = M.scratch + lane;
+    const uint64_t n_rows = *S.n_rows;
+    uint64_t n_emit = 0;
+    uint32_t flags = 0;
+    M.ev_in = M.ev;
+    for (uint64_t r = lane; r < n_rows; r += n_lanes) {
+        const uint32_t i = S.list[r];
+        const PBuf& q = M.ev;
+        const double ut = q.d[P_UT][i], ux = q.d[P_UX][i], uy = q.d[P_UY][i], uz = q.d[P_UZ][i];
+        const double a_rms = sqrt(ut * ut - ux * ux - uy * uy - uz * uz - 1.0);
+        const double k0 = M.ev_k0[i];
+        const double eta = k0 * (ut - uz);
+        const double amp = CIRC ? a_rms : a_rms * 1.4142135623730951;
+        const float* o = S.out + 32 * r;
+        const double nu = exp(5.0 * double(o[0]));
+        const int n = nu > 1.0 ? int(fmin(ceil(nu), 1.0e7)) : 1;
+        // 0 < v < 1 as the exact samplers' draws: the photon construction divides by s
+        // and takes sqrt(1 - cos^2) of an angle that reaches pi at v = 1
+        const double v = fmin(fmax(double(o[1]), 1.0e-6), 1.0 - 1.0e-7);
+        Rng base = rng_make(A->key, q.stream[i]);
+        base.c1 = q.nev[i] | EVENT_BLOCK;
+        double s, theta;
+        sv4 sv;
+        if (CIRC) {
+            const double sn = 2.0 * double(n) * eta / (1.0 + amp * amp);
+            s = v * (sn / (1.0 + sn));
+            Rng g3 = base;
+            g3.c0 = 3u;
+            theta = 2.0 * PI_D * rng_uniform(g3);
+            double st[4];
+            cp_stokes(M.lcfa, n, amp, s, sn, v, theta, false, st);
+            sv = {st[0], st[1], st[2], st[3]};
+        } else {
+            if (dbessel_len(n) > M.scratch_len) {
+                S.fb[atomicAdd(S.n_fb, 1ull)] = i;
+                continue;
+            }
+            const double sn = 2.0 * double(n) * eta / (1.0 + 0.5 * amp * amp);
+            s = v * (sn / (1.0 + sn));
+            const double th = fmin(fmax(double(o[2]), 0.0), 1.0) * (PI_D / 2.0);
+            Rng gq = base;  // counter 1: the quadrant, as the exact sampler
+            gq.c0 = 1u;
+            const int quadrant = int(rng_uniform(gq) * 4.0);
+            theta = th;
+            if (quadrant == 1) theta = PI_D - th;
+            else if (quadrant == 2) theta = PI_D + th;
+            else if (quadrant == 3) theta = 2.0 * PI_D - th;
+            sv = lp_stokes(dbessel_at(n, ws, n_lanes), amp, eta, s, theta);
+        }
+        lma_emit_finish<false>(A, M, i, k0, a_rms, eta, true, n, s, theta, sv, n_emit, flags);
+    }
+    wave_add(A->stats + STAT_EMISSIONS, n_emit);
+    wave_add(A->stats + STAT_SURROGATE, n_emit);
+    if (flags) atomicOr(A->flags, flags);
+}
+
+// Training data from the exact sampler: per wave one (a, eta, u1) drawn in the
+// domain box, its harmonic n = ceil(nu) from the tables, and 64 samples of
+// (v, t) at that harmonic. LP: the wave evaluates the ceiling points, then
+// 64 candidates per round, keeping every accepted one (rejection sampling
+// under the ceiling of lp/mod.rs:53-140 within the theta bound of :143-302);
+// CP: every lane runs the exact sampler (cp/mod.rs:181-234) at n.
+template <bool CIRC>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PTG_LMA_WAVE_WPE))) void k_surr_groups(LmaArgs M, SurrArgs S) {
+    const int lane = threadIdx.x;
+    const size_t st = size_t(gridDim.x) * 64;
+    double* ws = M.scratch + size_t(blockIdx.x) * 64 + lane;
+    __shared__ double tb_s[16], tb_f[16];
+    __shared__ double jseg[PTG_J_SEG * 5 * 64];
+    for (uint32_t g = blockIdx.x; g < S.n_groups; g += gridDim.x) {
+        __syncthreads();
+        Rng gr = rng_make(S.key, g);
+        gr.c1 = 0x40000000u;
+        double ua, ue;
+        rng_uniform2(gr, ua, ue);
+        const double u1 = rng_uniform(gr);
+        const double la = S.la_c + S.la_h * (2.0 * ua - 1.0), le = S.le_c + S.le_h * (2.0 * ue - 1.0);
+        const double amp = exp(la), eta = exp(le);
+        const double nu = surr_nu<CIRC>(M, amp, eta, u1);
+        const int n = nu > 1.0 ? int(ceil(nu)) : 1;
+        bool ok = nu > 0.0 && n <= 10000000;
+        float* out = S.gs + size_t(g) * 128;
+        if (CIRC) {
+            if (ok) {
+                Rng rl = rng_make(S.key, (uint64_t(g) << 6) | uint64_t(lane));
+                rl.c1 = 0x50000000u;
+                double s, theta;
+                sv4 sv;
+                uint32_t tr = 0;
+                bool capped = false;
+                cp_sample_at(M.lcfa, M.cp, n, amp, eta, false, rl, s, theta, sv, tr, capped);
+                const double sn = 2.0 * double(n) * eta / (1.0 + amp * amp);
+                out[2 * lane] = float(s / (sn / (1.0 + sn)));
+                out[2 * lane + 1] = 0.0f;
+                if (capped) ok = false;
+            }
+            ok = __all(ok);
+        } else if (ok && dbessel_len(n) <= M.scratch_len) {
+            const DBesselCk<PTG_J_SEG> D = dbessel_ck_at<PTG_J_SEG>(n, ws, st, jseg + lane);
+            const double sn = 2.0 * double(n) * eta / (1.0 + 0.5 * amp * amp);
+            const double smax = sn / (1.0 + sn);
+            if (lane < 16) theta_bound_point(n, amp, eta, lane, tb_s[lane], tb_f[lane]);
+            __syncthreads();
+            if (lane == 0) theta_bound_running_min(tb_f);
+            __syncthreads();
+            const int m = lp_ceiling_count(n, amp, eta);
+            double f = -INFINITY;
+            for (int k = lane; k < m; k += 64) {
+                double sk, tk;
+                lp_ceiling_arg(n, amp, eta, k, sk, tk);
+                f = fmax(f, lp_ddpr(D, amp, eta, sk, tk));
+            }
+            const double mx = 1.15 * wave_fmax(f);
+            int filled = 0;
+            for (uint32_t round = 0; filled < 64 && round < 4096; ++round) {
+                Rng gc = gr;
+                gc.c1 = 0x48000000u;
+                gc.c0 = 16u + 2u * (round * 64u + uint32_t(lane));
+                double v, th;
+                rng_uniform2(gc, v, th);
+                th *= PI_D / 2.0;
+                bool acc = false;
+                if (th <= theta_bound_at(tb_s, tb_f, smax * v)) {
+                    const double uz = rng_uniform(gc);
+                    acc = mx * uz < lp_ddpr(D, amp, eta, smax * v, th);
+                }
+                const uint64_t mask = __ballot(acc);
+                const int slot = filled + __popcll(mask & ((1ull << lane) - 1ull));
+                if (acc && slot < 64) {
+                    out[2 * slot] = float(v);
+                    out[2 * slot + 1] = float(th / (PI_D / 2.0));
+                }
+                filled += __popcll(mask);
+            }
+            ok = filled >= 64;
+        } else {
+            ok = false;
+        }
+        if (lane == 0) {
+            double* h = S.gh + size_t(g) * 4;
+            h[0] = la; h[1] = le; h[2] = u1; h[3] = ok ? nu : 0.0;
+        }
+    }
+}
+
+// One training batch: rows [0, n_table_rows) drawn fresh from the harmonic
+// tables (target nu only), the rest drawn from the exact-sampler rows (targets
+// nu, the rank-coupled v and, for LP, t at a fresh quantile level u3).
+template <bool CIRC>
+__global__ void k_surr_batch(LmaArgs M, SurrArgs S) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= S.batch) return;
+    Rng rr = rng_make(S.key, (uint64_t(S.step) << 32) | uint64_t(r));
+    rr.c1 = 0x60000000u;
+    float* x = S.tx + 8 * size_t(r);
+    float* t = S.tt + 8 * size_t(r);
+    double a1, a2, u1, u2, u3, dummy;
+    rng_uniform2(rr, a1, a2);
+    rng_uniform2(rr, u1, u2);
+    rng_uniform2(rr, u3, dummy);
+    if (r < S.n_table_rows) {
+        const double la = S.la_c + S.la_h * (2.0 * a1 - 1.0), le = S.le_c + S.le_h * (2.0 * a2 - 1.0);
+        const double nu = fmax(surr_nu<CIRC>(M, exp(la), exp(le), u1), 0.9);
+        surr_features(S, la, le, u1, u2, u3, x);
+        t[0] = float(log(nu) / 5.0); t[1] = 0.0f; t[2] = 0.0f; t[3] = 0.0f; t[4] = 0.0f; t[5] = float(u3);
+    } else {
+        const uint32_t idx = uint32_t(fmin(a1 * double(S.n_sampler_rows), double(S.n_sampler_rows - 1)));
+        const float* row = S.rows + 8 * size_t(idx);
+        surr_features(S, row[0], row[1], row[2], row[3], u3, x);
+        t[0] = row[4]; t[1] = row[5]; t[2] = row[6]; t[3] = 1.0f; t[4] = CIRC ? 0.0f : 1.0f; t[5] = float(u3);
+    }
+    t[6] = 0.0f; t[7] = 0.0f;
+}
+
+void launch_surr_inputs(const TransportArgs* a, const LmaArgs& m, const SurrArgs& s, hipStream_t st) {
+    const unsigned blocks = unsigned((s.n_ev + 255) / 256);
+    if (!blocks) return;
+    if (m.mode == LMA_CP) k_surr_inputs<true><<<blocks, 256, 0, st>>>(a, m, s);
+    else k_surr_inputs<false><<<blocks, 256, 0, st>>>(a, m, s);
+}
+void launch_surr_event(const TransportArgs* a, const LmaArgs& m, const SurrArgs& s, int lanes, hipStream_t st) {
+    if (m.mode == LMA_CP) k_surr_event<true><<<lanes / 64, 64, 0, st>>>(a, m, s);
+    else k_surr_event<false><<<lanes / 64, 64, 0, st>>>(a, m, s);
+}
+void launch_surr_groups(const LmaArgs& m, const SurrArgs& s, int lanes, hipStream_t st) {
+    if (m.mode == LMA_CP) k_surr_groups<true><<<lanes / 64, 64, 0, st>>>(m, s);
+    else k_surr_groups<false><<<lanes / 64, 64, 0, st>>>(m, s);
+}
+void launch_surr_batch(const LmaArgs& m, const SurrArgs& s, hipStream_t st) {
+    if (m.mode == LMA_CP) k_surr_batch<true><<<(s.batch + 255) / 256, 256, 0, st>>>(m, s);
+    else k_surr_batch<false><<<(s.batch + 255) / 256, 256, 0, st>>>(m, s);
 }
 
 // Probes of the LMA device functions (parity tests).

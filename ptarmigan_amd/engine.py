@@ -109,6 +109,7 @@ class CollideOptions:
     classical: bool = False
     gaunt_factor: bool = False
     precision: int = 0               # 0 = FP64, 1 = mixed
+    surrogate: bool = False          # sample LMA emission with the loaded surrogate inside its domain
 
     def desc(self):
         return _abi.CollideOptions(
@@ -116,7 +117,7 @@ class CollideOptions:
             -1 if self.multiplicity is None else int(self.multiplicity), int(self.discard_bg_e),
             int(self.discard_bg_ph), int(self.rr), int(self.tracking_photons), int(self.tracking_secondaries),
             int(self.keep_decayed_photons), int(self.pol_resolved), int(self.rotate_stokes_pars),
-            int(self.classical), int(self.gaunt_factor), int(self.precision), 0)
+            int(self.classical), int(self.gaunt_factor), int(self.precision), int(self.surrogate))
 
 
 @dataclasses.dataclass
@@ -281,6 +282,39 @@ class Engine:
         out = np.zeros((x.shape[0], out_w))
         self._check(self._lib.ptg_eval(self._ctx, int(fn), _dptr(x), x.shape[0], x.shape[1], _dptr(out), out_w))
         return out
+
+
+    # -- learned sampler
+    def surrogate_train(self, polarization, groups=8192, steps=6000, batch=4096, lr=2e-3, a_range=(0.25, 20.0),
+                        eta_range=(0.01, 0.3), seed=1):
+        """Trains the surrogate on the device (ptg_surrogate_train) and loads it.
+        Returns (model, report) as ctypes structures."""
+        d = _abi.SurrogateTrainDesc(int(polarization), int(groups), int(steps), int(batch), float(lr),
+                                    float(a_range[0]), float(a_range[1]), float(eta_range[0]), float(eta_range[1]),
+                                    int(seed))
+        model = _abi.SurrogateModel()
+        rep = _abi.SurrogateReport()
+        self._check(self._lib.ptg_surrogate_train(self._ctx, C.byref(d), C.byref(model), C.byref(rep)))
+        return model, rep
+
+    def surrogate_load(self, model):
+        self._check(self._lib.ptg_surrogate_load(self._ctx, C.byref(model)))
+
+    def mlp_probe(self, params, x, t, w_t=1.0):
+        """Forward and backward of the surrogate network on the device: (outputs n x 32,
+        gradient of the mean loss, loss sums)."""
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        n = x.shape[0]
+        out = np.zeros((n, 32), dtype=np.float32)
+        grads = np.zeros(_abi.MLP_NPARAM, dtype=np.float32)
+        loss = np.zeros(3, dtype=np.float32)
+        fp = C.POINTER(C.c_float)
+        self._check(self._lib.ptg_mlp_probe(self._ctx, params.ctypes.data_as(fp), x.ctypes.data_as(fp),
+                                            t.ctypes.data_as(fp), n, float(w_t), out.ctypes.data_as(fp),
+                                            grads.ctypes.data_as(fp), loss.ctypes.data_as(fp)))
+        return out, grads, loss
 
 
 class Fn(enum.IntEnum):

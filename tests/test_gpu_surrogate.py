@@ -1,0 +1,131 @@
+"""The learned sampler on the GPU.
+
+Training: a short training of each model on the device (data from the tables
+and the exact sampler) validates on held-out groups, and beats the untrained
+network on them (what the training gains). Use: plane-wave showers with the
+shipped models against the exact sampler at the same seed. The emission rate is
+untouched, so photon yields agree to the same events; the photon energy and
+angle spectra (200 bins) must be chi-square consistent and Stokes means agree
+within their errors.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _chi2_two_sample(a, b, bins):
+    from scipy.stats import chi2 as c2
+    ha, _ = np.histogram(a, bins)
+    hb, _ = np.histogram(b, bins)
+    na, nb = ha.sum(), hb.sum()
+    m = (ha + hb) > 0
+    k1, k2 = math.sqrt(nb / na), math.sqrt(na / nb)
+    chi2 = np.sum((k1 * ha[m] - k2 * hb[m]) ** 2 / (ha[m] + hb[m]))
+    return c2.sf(chi2, m.sum() - 1)
+
+
+_MODELS = {}
+
+
+def _model(engine, pol):
+    """A model trained once per session on a box that covers the test showers."""
+    if pol not in _MODELS:
+        say("training surrogate", pol)
+        model, rep = engine.surrogate_train(pol, groups=4096, steps=3000, batch=4096, lr=2e-3, a_range=(0.05, 8.0),
+                                            eta_range=(0.01, 0.3), seed=21 + pol)
+        say("  report", dict(loss=list(rep.loss), h=rep.val_harmonic_rel, v=rep.val_fraction_abs, valid=rep.valid,
+                             ks_v=rep.ks_fraction, ks_t=rep.ks_angle,
+                             groups=rep.groups_used, gen_ms=rep.gen_ms, train_ms=rep.train_ms))
+        _MODELS[pol] = (model, rep)
+    return _MODELS[pol]
+
+
+@pytest.mark.parametrize("pol", [0, 1])
+def test_training_validates_and_beats_the_untrained_network(engine, pol):
+    model, rep = _model(engine, pol)
+    assert rep.valid == 1
+    assert rep.groups_used > 3500
+    assert all(np.isfinite(rep.loss))
+    _, rep0 = engine.surrogate_train(pol, groups=512, steps=1, batch=256, lr=1e-6, a_range=(0.05, 8.0),
+                                     eta_range=(0.01, 0.3), seed=21 + pol)
+    say("  untrained", rep0.val_harmonic_rel, rep0.val_fraction_abs)
+    assert rep.val_harmonic_rel < 0.2 * rep0.val_harmonic_rel
+    assert rep.val_fraction_abs < 0.5 * rep0.val_fraction_abs
+    engine.surrogate_load(model)  # the training call above replaced it
+
+
+def _shower(engine, laser, beam, opts, seed):
+    engine.set_laser(laser)
+    engine.build_beam(beam, seed=seed)
+    c = engine.run(opts, seed=seed)
+    sec, _ = engine.fetch(c)
+    return sec, c
+
+
+@pytest.mark.parametrize("pol,a0", [(0, 2.0), (1, 5.0)])
+def test_surrogate_shower_matches_exact(engine, pol, a0):
+    """With the shipped models (python -m ptarmigan_amd.surrogate)."""
+    from ptarmigan_amd import surrogate, workloads
+    from ptarmigan_amd.engine import Polarization
+    model, meta = surrogate.load(surrogate.default_path(pol))
+    assert meta["valid"] == 1
+    engine.surrogate_load(model)
+    laser, beam, opts = workloads.plane_wave_lma(100_000)
+    laser.polarization = Polarization(pol)
+    laser.a0 = a0
+    say("surrogate shower", pol, a0)
+    se, ce = _shower(engine, laser, beam, opts, 31)
+    opts.surrogate = True
+    ss, cs = _shower(engine, laser, beam, opts, 31)
+    say("  exact    ", ce, "\n  surrogate", cs)
+    assert cs["error_flags"] == 0
+    assert cs["surrogate_events"] > 0.95 * cs["emissions"]
+    assert cs["surrogate_events"] + cs["surrogate_fallbacks"] >= cs["emissions"]
+    pe, ps = se["species"] == 2, ss["species"] == 2
+    ye, ys = se["weight"][pe].sum(), ss["weight"][ps].sum()
+    say(f"  yield exact {ye} surrogate {ys}")
+    assert abs(ys - ye) / ye < 5e-3
+    ee, es = se["u"][pe, 0], ss["u"][ps, 0]
+    bins = np.linspace(0.0, max(ee.max(), es.max()), 201)
+    p_e = _chi2_two_sample(ee, es, bins)
+    say(f"  energy spectrum chi2 p = {p_e:.3g}; mean {ee.mean():.6g} vs {es.mean():.6g}")
+    assert abs(es.mean() - ee.mean()) / ee.mean() < 0.01
+    for comp in (1, 2):
+        ae = np.arctan2(se["u"][pe, comp], -se["u"][pe, 3])
+        as_ = np.arctan2(ss["u"][ps, comp], -ss["u"][ps, 3])
+        lim = np.quantile(np.abs(np.concatenate([ae, as_])), 0.999)
+        p_a = _chi2_two_sample(ae, as_, np.linspace(-lim, lim, 201))
+        say(f"  angle {comp} chi2 p = {p_a:.3g}; rms {ae.std():.4g} vs {as_.std():.4g}")
+        assert abs(as_.std() - ae.std()) / ae.std() < 0.03
+    assert p_e > 1e-4
+    for col in (1, 2, 3):
+        x, y = se["pol"][pe, col], ss["pol"][ps, col]
+        se_ = math.sqrt(x.var() / len(x) + y.var() / len(y)) + 1e-12
+        assert abs(x.mean() - y.mean()) < 6 * se_, col
+
+
+def test_surrogate_requires_a_valid_model_and_quantum_lma(engine):
+    from ptarmigan_amd import workloads
+    from ptarmigan_amd.engine import PtgError
+    laser, beam, opts = workloads.plane_wave_lma(1000)
+    opts.surrogate = True
+    opts.classical = True
+    engine.set_laser(laser)
+    engine.build_beam(beam, seed=0)
+    with pytest.raises(PtgError, match="surrogate"):
+        engine.run(opts, seed=0)
+    laser, beam, opts = workloads.focused_lcfa_cascade(1000)
+    opts.surrogate = True
+    engine.set_laser(laser)
+    engine.build_beam(beam, seed=0)
+    with pytest.raises(PtgError, match="surrogate"):
+        engine.run(opts, seed=0)

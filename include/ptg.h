@@ -217,6 +217,11 @@ int ptg_collide(ptg_ctx* ctx, const ptg_particles* primaries, size_t n, const pt
 /* Copies results to caller buffers sized from counters (n_secondaries / n_intermediates).
  * Either output may be NULL. Replaces the Shower/ParticleBunch fields (particle/shower.rs:9-46). */
 int ptg_fetch(ptg_ctx* ctx, ptg_particles* secondaries, ptg_particles* intermediates);
+/* The index of the primary whose shower holds each final-state particle, in
+ * ptg_fetch's order (n_secondaries values); in a batch (ptg_set_batch) it
+ * names the collision. No counterpart in the reference (one shower per
+ * collide() call there). */
+int ptg_fetch_showers(ptg_ctx* ctx, uint32_t* shower);
 
 /* Field-only push of n particles for up to n_steps steps each (stops when a particle
  * leaves the field): repeated Field::push (src/field/mod.rs:147-152 and the LMA
@@ -266,6 +271,31 @@ typedef enum {
     PTG_FN_COUNT = 28
 } ptg_eval_fn;
 int ptg_eval(ptg_ctx* ctx, int fn, const double* in, size_t n, size_t in_w, double* out, size_t out_w);
+
+/* ------------------------------------------------------------------ batches and launch tuning
+ * A batch of collisions in one particle population (no counterpart in the
+ * reference, which runs one collision per input deck): n_coll lasers that may
+ * differ only in a0 (an a0 scan; LMA fields) and n_coll beams; the primaries of
+ * collision k are built as ptg_build_beam would (initial_z from the laser and
+ * the beam length, src/main.rs:1062-1066) at consecutive indices, and each
+ * particle's shower index (its primary) names its collision. ptg_run then
+ * transports every collision with one sequence of launches. A later
+ * ptg_set_laser, ptg_build_beam or ptg_load_primaries ends the batch. */
+int ptg_set_batch(ptg_ctx* ctx, const ptg_laser_desc* lasers, const ptg_beam_desc* beams, int n_coll, uint64_t seed);
+
+/* Launch-shape autotuner for the LMA lepton step kernel: runs the loaded
+ * primaries (at most max_primaries of them; 0: all) at each block size (64,
+ * 128, 256 threads) and persistent grid (2, 4, 8 blocks per CU), keeps the
+ * fastest for later runs of this context. Results do not depend on the launch
+ * shape; `consistent` reports that every candidate produced the same number
+ * of emissions. No counterpart in the reference. */
+typedef struct {
+    int32_t block, grid_per_cu;      /* chosen */
+    int32_t n_candidates, consistent;
+    int32_t cand_block[16], cand_grid_per_cu[16];
+    double cand_ms[16];              /* lepton step kernel time of each candidate */
+} ptg_tune_result;
+int ptg_autotune(ptg_ctx* ctx, const ptg_collide_options* opt, uint64_t seed, uint64_t max_primaries, ptg_tune_result* out);
 
 /* ------------------------------------------------------------------ learned sampler
  * A surrogate for the sampling of LMA nonlinear Compton emission. No counterpart

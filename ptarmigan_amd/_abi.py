@@ -54,6 +54,12 @@ class Counters(C.Structure):
 MLP_NPARAM = 38304
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("block", C.c_int32), ("grid_per_cu", C.c_int32), ("n_candidates", C.c_int32),
+                ("consistent", C.c_int32), ("cand_block", C.c_int32 * 16), ("cand_grid_per_cu", C.c_int32 * 16),
+                ("cand_ms", C.c_double * 16)]
+
+
 class SurrogateModel(C.Structure):
     _fields_ = [("polarization", C.c_int32), ("valid", C.c_int32), ("a_min", C.c_double), ("a_max", C.c_double),
                 ("eta_min", C.c_double), ("eta_max", C.c_double), ("params", C.c_float * MLP_NPARAM)]
@@ -88,7 +94,8 @@ class HistSpec(C.Structure):
 # every extern "C" symbol include/ptg.h declares
 EXPORTS = ("ptg_create", "ptg_destroy", "ptg_last_error", "ptg_set_laser", "ptg_field_info", "ptg_build_beam",
            "ptg_load_primaries", "ptg_run", "ptg_collide", "ptg_fetch", "ptg_push", "ptg_histogram", "ptg_eval",
-           "ptg_surrogate_train", "ptg_surrogate_load", "ptg_mlp_probe")
+           "ptg_surrogate_train", "ptg_surrogate_load", "ptg_mlp_probe", "ptg_set_batch", "ptg_autotune",
+           "ptg_fetch_showers")
 
 _lib = None
 
@@ -123,6 +130,9 @@ def load():
     L.ptg_surrogate_train.argtypes = [vp, P(SurrogateTrainDesc), P(SurrogateModel), P(SurrogateReport)]
     L.ptg_surrogate_load.argtypes = [vp, P(SurrogateModel)]
     L.ptg_mlp_probe.argtypes = [vp, P(f), P(f), P(f), C.c_size_t, f, P(f), P(f), P(f)]
+    L.ptg_set_batch.argtypes = [vp, P(LaserDesc), P(BeamDesc), C.c_int, C.c_uint64]
+    L.ptg_fetch_showers.argtypes = [vp, P(C.c_uint32)]
+    L.ptg_autotune.argtypes = [vp, P(CollideOptions), C.c_uint64, C.c_uint64, P(TuneResult)]
     for f in ("ptg_set_laser", "ptg_field_info", "ptg_build_beam", "ptg_load_primaries", "ptg_run", "ptg_collide",
               "ptg_fetch", "ptg_push", "ptg_histogram", "ptg_eval"):
         getattr(L, f).restype = C.c_int

@@ -138,6 +138,14 @@ struct ptg_ctx {
     bool surr_valid[2] = {false, false};
     double surr_box[2][4] = {};    // a_min, a_max, eta_min, eta_max
     uint32_t* d_surr_list = nullptr;  // row -> event, and the exact sampler's list
+    // a batch of collisions (ptg_set_batch): lasers, and each primary's collision
+    LaserDev* d_lasers = nullptr;
+    uint8_t* d_coll = nullptr;
+    size_t coll_cap = 0;
+    int n_coll = 0;
+    double batch_a0_max = 0.0;
+    // launch shape of the lepton step kernel (ptg_autotune)
+    int step_block = 256, step_grid_per_cu = 4;
     uint32_t* d_surr_fb = nullptr;
     size_t surr_list_cap = 0;
 };
@@ -502,6 +510,8 @@ void ptg_destroy(ptg_ctx* c) {
     c->surr[1].release();
     if (c->d_surr_list) (void)hipFree(c->d_surr_list);
     if (c->d_surr_fb) (void)hipFree(c->d_surr_fb);
+    if (c->d_lasers) (void)hipFree(c->d_lasers);
+    if (c->d_coll) (void)hipFree(c->d_coll);
     if (c->d_bw) (void)hipFree(c->d_bw);
     if (c->d_cp) (void)hipFree(c->d_cp);
     if (c->d_ev_extra) (void)hipFree(c->d_ev_extra);
@@ -528,6 +538,7 @@ int ptg_set_laser(ptg_ctx* c, const ptg_laser_desc* d) {
     if (rc) return rc;
     c->laser = L;
     c->have_laser = true;
+    c->n_coll = 0;
     c->pol_angle = d->polarization == PTG_CIRCULAR ? 0.0 : d->pol_angle;
     c->lma_bandwidth = 0.0;
     if (d->finite_bandwidth) {  // plane_wave.rs:49-63, focused_laser.rs:45-58
@@ -577,6 +588,9 @@ int ptg_build_beam(ptg_ctx* c, const ptg_beam_desc* b, double initial_z, uint64_
     bd.weight = b->weight;
     bd.initial_z = initial_z;
     bd.key = mix64(key_from_seed(seed) ^ 0x6265616d6265616dULL);
+    bd.off = 0;
+    bd.n_total = bd.n;
+    c->n_coll = 0;
     launch_build_beam(bd, c->prim.v, c->d_n0, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -636,6 +650,7 @@ int ptg_load_primaries(ptg_ctx* c, const ptg_particles* p, size_t n) {
     launch_primary_dirs(c->prim.v, uint32_t(n), c->d_n0, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->n_coll = 0;
     c->n_prim = n;
     return PTG_OK;
 }
@@ -869,7 +884,7 @@ static int run_lma(ptg_ctx* c, TransportArgs a, double* ms, uint64_t* launches, 
     {
         // workspace per lane for J(n, x, y), 5 doubles per order (LP emission and LP
         // pair creation; circular polarisation needs none)
-        int ncap = c->laser.circular ? 16 : lma_harmonic_cap(c->laser.a0);
+        int ncap = c->laser.circular ? 16 : lma_harmonic_cap(c->n_coll ? c->batch_a0_max : c->laser.a0);
         // diagnostic: a smaller workspace sends most events through the deferred pass,
         // which must reproduce the same shower
         if (const char* e = std::getenv("PTG_LMA_HARMONIC_CAP")) {
@@ -901,6 +916,8 @@ static int run_lma(ptg_ctx* c, TransportArgs a, double* ms, uint64_t* launches, 
     m.mode = !c->laser.circular ? (a.classical_mode ? LMA_LP_CLASSICAL : LMA_LP) : (a.classical_mode ? LMA_CP_CLASSICAL : LMA_CP);
     m.mixed = a.precision == PTG_PREC_MIXED ? 1 : 0;
     m.tables_f = c->d_lma_f;
+    m.lasers = c->n_coll ? c->d_lasers : nullptr;
+    m.coll = c->n_coll ? c->d_coll : nullptr;
     m.bandwidth = c->lma_bandwidth;
     m.pol_angle = c->pol_angle;
     m.sin_pa = std::sin(c->pol_angle);
@@ -950,9 +967,10 @@ static int run_lma(ptg_ctx* c, TransportArgs a, double* ms, uint64_t* launches, 
             m.ph = c->phq.v;
             m.ph_cap = c->phq.cap;
             HIPCHK(c, hipMemcpyAsync(c->d_args, &a, sizeof(a), hipMemcpyHostToDevice, c->stream));
-            int grid = int(std::min<size_t>(size_t(c->n_cu) * 4, (n_cur + 255) / 256));
+            const size_t sb = size_t(c->step_block);
+            int grid = int(std::min<size_t>(size_t(c->n_cu) * size_t(c->step_grid_per_cu), (n_cur + sb - 1) / sb));
             HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-            launch_step_lma(c->d_args, m, grid, c->stream);
+            launch_step_lma(c->d_args, m, grid, c->stream, c->step_block);
             HIPCHK(c, hipGetLastError());
             HIPCHK(c, hipEventRecord(c->ev1, c->stream));
             unsigned long long n_ev = 0;
@@ -1256,6 +1274,14 @@ int ptg_fetch(ptg_ctx* c, ptg_particles* sec, ptg_particles* inter) {
     return fetch_buf(c, c->inter, c->n_inter, inter);
 }
 
+int ptg_fetch_showers(ptg_ctx* c, uint32_t* shower) {
+    if (!c || !shower) return fail(c, PTG_E_ARG, "null argument");
+    if (c->n_out == 0) return PTG_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, scopy(c, shower, c->out.v.shower, c->n_out * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return PTG_OK;
+}
+
 int ptg_push(ptg_ctx* c, const uint8_t* species, double* r, double* u, size_t n, uint32_t n_steps, const ptg_collide_options* o,
              uint32_t* steps_taken, double* work) {
     if (!c || !species || !r || !u || !o) return fail(c, PTG_E_ARG, "null argument");
@@ -1483,6 +1509,134 @@ int ptg_eval(ptg_ctx* c, int fn, const double* in, size_t n, size_t in_w, double
     HIPCHK(c, scopy(c, out, dout, n * out_w * 8, hipMemcpyDeviceToHost));
     (void)hipFree(din);
     (void)hipFree(dout);
+    return PTG_OK;
+}
+
+int ptg_set_batch(ptg_ctx* c, const ptg_laser_desc* lasers, const ptg_beam_desc* beams, int n_coll, uint64_t seed) {
+    if (!c || !lasers || !beams) return fail(c, PTG_E_ARG, "null argument");
+    if (n_coll < 1 || n_coll > 256) return fail(c, PTG_E_ARG, "a batch holds 1 to 256 collisions");
+    const ptg_laser_desc& d0 = lasers[0];
+    if (d0.kind != PTG_PLANE_WAVE && d0.kind != PTG_FOCUSED_LASER)
+        return fail(c, PTG_E_UNSUPPORTED, "a batch of collisions runs the LMA fields (PlaneWave, FocusedLaser)");
+    for (int k = 1; k < n_coll; ++k) {
+        const ptg_laser_desc& d = lasers[k];
+        if (d.kind != d0.kind || d.envelope != d0.envelope || d.polarization != d0.polarization ||
+            d.finite_bandwidth != d0.finite_bandwidth || d.wavelength != d0.wavelength || d.waist != d0.waist ||
+            d.n_cycles != d0.n_cycles || d.pol_angle != d0.pol_angle || d.chirp_b != d0.chirp_b)
+            return fail(c, PTG_E_ARG, "the lasers of a batch may differ only in a0");
+    }
+    uint64_t total = 0;
+    for (int k = 0; k < n_coll; ++k) {
+        const ptg_beam_desc& b = beams[k];
+        if (b.n == 0) return fail(c, PTG_E_ARG, "every beam of a batch needs primaries");
+        if (b.species < 0 || b.species > 2 || b.radial < 0 || b.radial > 2) return fail(c, PTG_E_ARG, "bad species or radial distribution");
+        if (!(std::fabs(b.energy_chirp) <= 1.0)) return fail(c, PTG_E_ARG, "|energy_chirp| must be <= 1");
+        total += b.n;
+    }
+    if (total > 0xffffffffull) return fail(c, PTG_E_ARG, "a batch holds fewer than 2^32 primaries");
+    std::vector<LaserDev> L(n_coll);
+    double a0_max = 0.0;
+    for (int k = 0; k < n_coll; ++k) {
+        int rc = make_laser(c, &lasers[k], L[k]);
+        if (rc) return rc;
+        a0_max = std::max(a0_max, lasers[k].a0);
+    }
+    int rc = ptg_set_laser(c, &d0);  // the batch's common field parameters (time step, polarization, bandwidth)
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!c->d_lasers) HIPCHK(c, hipMalloc(&c->d_lasers, 256 * sizeof(LaserDev)));
+    HIPCHK(c, scopy(c, c->d_lasers, L.data(), n_coll * sizeof(LaserDev), hipMemcpyHostToDevice));
+    if (c->prim.cap < total) HIPCHK(c, alloc_buf(c->prim, total));
+    rc = ensure_n0(c, total);
+    if (rc) return rc;
+    if (c->coll_cap < total) {
+        if (c->d_coll) (void)hipFree(c->d_coll);
+        c->d_coll = nullptr;
+        c->coll_cap = 0;
+        HIPCHK(c, hipMalloc(&c->d_coll, total));
+        c->coll_cap = total;
+    }
+    std::vector<uint8_t> coll(total);
+    uint64_t off = 0;
+    for (int k = 0; k < n_coll; ++k) {
+        const ptg_beam_desc& b = beams[k];
+        ptg_laser_desc dk = lasers[k];
+        const double z0 = ideal_initial_z(&dk, L[k]) + 3.0 * b.length;  // the driver's initial_z (src/main.rs:1062-1066)
+        BeamDev bd{};
+        bd.n = uint32_t(b.n);
+        bd.species = b.species;
+        bd.radial = b.radial;
+        bd.gamma = b.gamma;
+        bd.sigma = b.sigma;
+        bd.rho = -b.energy_chirp;
+        bd.radius = b.radius;
+        bd.max_radius = b.max_radius;
+        bd.length = b.length;
+        bd.rms_div = b.rms_div;
+        bd.angle = b.angle;
+        bd.plane_angle = b.collision_plane_angle;
+        bd.off_x = b.offset[0]; bd.off_y = b.offset[1]; bd.off_z = b.offset[2];
+        for (int q = 0; q < 4; ++q) bd.pol[q] = b.pol[q];
+        bd.weight = b.weight;
+        bd.initial_z = z0;
+        // collision 0 draws as ptg_build_beam with the same seed would
+        bd.key = mix64(key_from_seed(seed) ^ 0x6265616d6265616dULL ^ (0x9e3779b97f4a7c15ULL * uint64_t(k)));
+        bd.off = uint32_t(off);
+        bd.n_total = uint32_t(total);
+        launch_build_beam(bd, c->prim.v, c->d_n0, c->stream);
+        HIPCHK(c, hipGetLastError());
+        std::fill(coll.begin() + off, coll.begin() + off + b.n, uint8_t(k));
+        off += b.n;
+    }
+    HIPCHK(c, scopy(c, c->d_coll, coll.data(), total, hipMemcpyHostToDevice));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->n_prim = total;
+    c->n_coll = n_coll;
+    c->batch_a0_max = a0_max;
+    return PTG_OK;
+}
+
+int ptg_autotune(ptg_ctx* c, const ptg_collide_options* o, uint64_t seed, uint64_t max_primaries, ptg_tune_result* out) {
+    if (!c || !o) return PTG_E_ARG;
+    if (!(c->laser.kind == K_PLANE_WAVE || c->laser.kind == K_FOCUSED_LASER))
+        return fail(c, PTG_E_UNSUPPORTED, "the autotuner tunes the LMA lepton step kernel");
+    if (c->n_prim == 0) return fail(c, PTG_E_STATE, "no primaries loaded");
+    const size_t keep = c->n_prim;
+    c->n_prim = std::min<size_t>(keep, max_primaries ? max_primaries : keep);
+    const int blocks[3] = {64, 128, 256}, grids[3] = {2, 4, 8};
+    ptg_tune_result r{};
+    double best = 1e300;
+    uint64_t emissions0 = 0;
+    r.consistent = 1;
+    int rc = PTG_OK;
+    for (int bi = 0; bi < 3 && !rc; ++bi)
+        for (int gi = 0; gi < 3 && !rc; ++gi) {
+            c->step_block = blocks[bi];
+            c->step_grid_per_cu = grids[gi];
+            ptg_counters cnt{};
+            rc = ptg_run(c, o, seed, &cnt);
+            if (rc) break;
+            const int k = r.n_candidates++;
+            r.cand_block[k] = blocks[bi];
+            r.cand_grid_per_cu[k] = grids[gi];
+            r.cand_ms[k] = cnt.lepton_ms;
+            if (k == 0) emissions0 = cnt.emissions;
+            else if (cnt.emissions != emissions0) r.consistent = 0;
+            if (cnt.lepton_ms < best) {
+                best = cnt.lepton_ms;
+                r.block = blocks[bi];
+                r.grid_per_cu = grids[gi];
+            }
+        }
+    c->n_prim = keep;
+    if (rc) {
+        c->step_block = 256;
+        c->step_grid_per_cu = 4;
+        return rc;
+    }
+    c->step_block = r.block;
+    c->step_grid_per_cu = r.grid_per_cu;
+    if (out) *out = r;
     return PTG_OK;
 }
 

@@ -24,6 +24,9 @@ struct LmaArgs {
     int mode;                    // LmaMode
     int mixed;                   // mixed precision: FP32 rate table and pusher algebra, FP32 CDF table (LP, CP quantum)
     const float* tables_f;       // FP32 copy of the LMA pack (mixed precision)
+    // a batch of collisions: laser of collision coll[shower] for each particle (null: A->laser)
+    const LaserDev* lasers;
+    const uint8_t* coll;
     double bandwidth;            // fractional bandwidth (plane_wave.rs:49-63); 0 = monochromatic
     double pol_angle, sin_pa, cos_pa;
     // step kernel -> event queue
@@ -89,7 +92,8 @@ void launch_surr_event(const TransportArgs* a, const LmaArgs& m, const SurrArgs&
 void launch_surr_groups(const LmaArgs& m, const SurrArgs& s, int lanes, hipStream_t st);
 void launch_surr_batch(const LmaArgs& m, const SurrArgs& s, hipStream_t st);
 
-void launch_step_lma(const TransportArgs* a, const LmaArgs& m, int grid, hipStream_t s);
+// block: 64, 128 or 256 threads (the kernel's bound); grid: persistent blocks
+void launch_step_lma(const TransportArgs* a, const LmaArgs& m, int grid, hipStream_t s, int block = 256);
 void launch_event_lma(const TransportArgs* a, const LmaArgs& m, int lanes, hipStream_t s);
 void launch_photon_step_lma(const TransportArgs* a, const LmaArgs& m, int grid, hipStream_t s);
 void launch_pair_event_lma(const TransportArgs* a, const LmaArgs& m, int lanes, hipStream_t s);

@@ -35,7 +35,7 @@ struct StepTable {
 template <int MODE, bool MIX>
 __global__ __launch_bounds__(256) void k_step_lma(const TransportArgs* __restrict__ A, LmaArgs M) {
     using TT = typename std::conditional<MIX, float, double>::type;
-    const LaserDev& L = A->laser;
+    const LaserDev* Lp = &A->laser;  // a batch of collisions: the particle's own laser
     const double dt = A->dt;
     const double kc = C_LIGHT * T_COMPTON;
     __shared__ TT rt[StepTable<MODE>::N];
@@ -61,6 +61,7 @@ __global__ __launch_bounds__(256) void k_step_lma(const TransportArgs* __restric
             meta = in.meta[i];
             if ((meta & 3u) == 2u) { forward_record(*A, i); continue; }
             idx = i;
+            if (M.coll) Lp = M.lasers + M.coll[in.shower[i]];
             r = {in.d[P_RT][i], in.d[P_RX][i], in.d[P_RY][i], in.d[P_RZ][i]};
             u = {in.d[P_UT][i], in.d[P_UX][i], in.d[P_UY][i], in.d[P_UZ][i]};
             work = in.d[P_WORK][i];
@@ -77,6 +78,7 @@ __global__ __launch_bounds__(256) void k_step_lma(const TransportArgs* __restric
     };
     bool live = fetch();
     while (live) {
+        const LaserDev& L = *Lp;
         bool inside = !skip && field_contains(L, r) && r.t / C_LIGHT < A->opt.t_stop;
         if (inside && psteps >= A->step_cap) { flags |= PTG_FLAG_STEP_CAP; inside = false; }
         if (!inside) {
@@ -472,7 +474,7 @@ __device__ __forceinline__ double lma_photon_prob(const LmaArgs& M, const LaserD
 
 template <bool CIRC>
 __global__ __launch_bounds__(256) void k_photon_step_lma(const TransportArgs* __restrict__ A, LmaArgs M) {
-    const LaserDev& L = A->laser;
+    const LaserDev* Lp = &A->laser;
     const double dt = A->dt, cdt = C_LIGHT * dt;
     uint64_t n_steps = 0;
     uint32_t flags = 0, psteps = 0;
@@ -491,6 +493,7 @@ __global__ __launch_bounds__(256) void k_photon_step_lma(const TransportArgs* __
             meta = in.meta[i];
             if ((meta & 3u) != 2u) { forward_record(*A, i); continue; }
             idx = i;
+            if (M.coll) Lp = M.lasers + M.coll[in.shower[i]];
             r = {in.d[P_RT][i], in.d[P_RX][i], in.d[P_RY][i], in.d[P_RZ][i]};
             k = {in.d[P_UT][i], in.d[P_UX][i], in.d[P_UY][i], in.d[P_UZ][i]};
             pol = {in.d[P_SI][i], in.d[P_SQ][i], in.d[P_SU][i], in.d[P_SV][i]};
@@ -508,6 +511,7 @@ __global__ __launch_bounds__(256) void k_photon_step_lma(const TransportArgs* __
     };
     bool live = fetch();
     while (live) {
+        const LaserDev& L = *Lp;
         bool inside = !skip && field_contains(L, r) && r.t / C_LIGHT < A->opt.t_stop;
         if (inside && psteps >= A->step_cap) { flags |= PTG_FLAG_STEP_CAP; inside = false; }
         if (!inside) {
@@ -677,17 +681,17 @@ void launch_pair_event_lma(const TransportArgs* a, const LmaArgs& m, int lanes, 
     else k_pair_event_lma<false><<<lanes / 64, 64, 0, s>>>(a, m);
 }
 
-void launch_step_lma(const TransportArgs* a, const LmaArgs& m, int grid, hipStream_t s) {
+void launch_step_lma(const TransportArgs* a, const LmaArgs& m, int grid, hipStream_t s, int block) {
     switch (m.mode) {
     case LMA_CP:
-        if (m.mixed) k_step_lma<LMA_CP, true><<<grid, 256, 0, s>>>(a, m);
-        else k_step_lma<LMA_CP, false><<<grid, 256, 0, s>>>(a, m);
+        if (m.mixed) k_step_lma<LMA_CP, true><<<grid, block, 0, s>>>(a, m);
+        else k_step_lma<LMA_CP, false><<<grid, block, 0, s>>>(a, m);
         break;
-    case LMA_CP_CLASSICAL: k_step_lma<LMA_CP_CLASSICAL, false><<<grid, 256, 0, s>>>(a, m); break;
-    case LMA_LP_CLASSICAL: k_step_lma<LMA_LP_CLASSICAL, false><<<grid, 256, 0, s>>>(a, m); break;
+    case LMA_CP_CLASSICAL: k_step_lma<LMA_CP_CLASSICAL, false><<<grid, block, 0, s>>>(a, m); break;
+    case LMA_LP_CLASSICAL: k_step_lma<LMA_LP_CLASSICAL, false><<<grid, block, 0, s>>>(a, m); break;
     default:
-        if (m.mixed) k_step_lma<LMA_LP, true><<<grid, 256, 0, s>>>(a, m);
-        else k_step_lma<LMA_LP, false><<<grid, 256, 0, s>>>(a, m);
+        if (m.mixed) k_step_lma<LMA_LP, true><<<grid, block, 0, s>>>(a, m);
+        else k_step_lma<LMA_LP, false><<<grid, block, 0, s>>>(a, m);
         break;
     }
 }
@@ -798,10 +802,12 @@ __global__ __launch_bounds__(64) void k_surr_event(const TransportArgs* __restri
         const double amp = CIRC ? a_rms : a_rms * 1.4142135623730951;
         const float* o = S.out + 32 * r;
         const double nu = exp(5.0 * double(o[0]));
-        const int n = nu > 1.0 ? int(fmin(ceil(nu), 1.0e7)) : 1;
+        int n = nu > 1.0 ? int(fmin(ceil(nu), 1.0e7)) : 1;
         // 0 < v < 1 as the exact samplers' draws: the photon construction divides by s
-        // and takes sqrt(1 - cos^2) of an angle that reaches pi at v = 1
-        const double v = fmin(fmax(double(o[1]), 1.0e-6), 1.0 - 1.0e-7);
+        // and takes sqrt(1 - cos^2) of an angle that reaches pi at v = 1. Below the
+        // lowest trained quantile level the network extrapolates; v < 1e-4 (wide-angle,
+        // soft photons, probability ~1e-4 under the exact sampler) is cut to 1e-4.
+        const double v = fmin(fmax(double(o[1]), 1.0e-4), 1.0 - 1.0e-7);
         Rng base = rng_make(A->key, q.stream[i]);
         base.c1 = q.nev[i] | EVENT_BLOCK;
         double s, theta;
@@ -816,10 +822,10 @@ __global__ __launch_bounds__(64) void k_surr_event(const TransportArgs* __restri
             cp_stokes(M.lcfa, n, amp, s, sn, v, theta, false, st);
             sv = {st[0], st[1], st[2], st[3]};
         } else {
-            if (dbessel_len(n) > M.scratch_len) {
-                S.fb[atomicAdd(S.n_fb, 1ull)] = i;
-                continue;
-            }
+            // the workspace holds the harmonics of the tables (1.5 x the reference's sum
+            // limit at the laser's a0); a network output beyond it is cut to it
+            while (n > 1 && dbessel_len(n) > M.scratch_len) n = n - 1 - (dbessel_len(n) - M.scratch_len) / 4;
+            if (n < 1) n = 1;
             const double sn = 2.0 * double(n) * eta / (1.0 + 0.5 * amp * amp);
             s = v * (sn / (1.0 + sn));
             const double th = fmin(fmax(double(o[2]), 0.0), 1.0) * (PI_D / 2.0);

@@ -13,6 +13,7 @@ struct BeamDev {
     double weight;
     double initial_z;
     uint64_t key;
+    uint32_t off, n_total;   // this beam's primaries at [off, off + n) of n_total (a batch of collisions)
 };
 
 struct HistArgs {

@@ -22,9 +22,11 @@ __device__ __forceinline__ void normal2(Rng& g, double& a, double& b) {
 // draws from its own counter-based stream, so the beam is independent of the
 // launch shape.
 __global__ void k_build_beam(BeamDev b, PBuf p, double* n0) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= b.n) return;
-    Rng g = rng_make(b.key, i);
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= b.n) return;
+    Rng g = rng_make(b.key, k);
+    const uint32_t i = b.off + k;  // index in the primary buffer
+    const uint32_t nt = b.n_total;
     double gamma, dz;
     for (int t = 0;; ++t) {
         double n0_, n1_;
@@ -89,7 +91,7 @@ __global__ void k_build_beam(BeamDev b, PBuf p, double* n0) {
     p.meta[i] = uint32_t(b.species) | META_PRIMARY;
     p.nev[i] = 0;
     double inv = 1.0 / sqrt(nsq(u3));
-    n0[i] = u3.x * inv; n0[b.n + i] = u3.y * inv; n0[2 * b.n + i] = u3.z * inv;
+    n0[i] = u3.x * inv; n0[nt + i] = u3.y * inv; n0[2 * nt + i] = u3.z * inv;
 }
 
 void launch_build_beam(const BeamDev& b, const PBuf& p, double* n0, hipStream_t s) {

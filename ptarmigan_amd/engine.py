@@ -245,6 +245,10 @@ class Engine:
         inter = empty_particles(int(counters["n_intermediates"])) if intermediates else None
         self._check(self._lib.ptg_fetch(self._ctx, C.byref(particles_struct(sec)),
                                         C.byref(particles_struct(inter)) if inter is not None else None))
+        shower = np.zeros(len(sec["species"]), dtype=np.uint32)
+        if len(shower):
+            self._check(self._lib.ptg_fetch_showers(self._ctx, shower.ctypes.data_as(C.POINTER(C.c_uint32))))
+        sec["shower"] = shower
         return sec, inter
 
     def collide(self, prim, options: CollideOptions, seed=0):
@@ -283,6 +287,24 @@ class Engine:
         self._check(self._lib.ptg_eval(self._ctx, int(fn), _dptr(x), x.shape[0], x.shape[1], _dptr(out), out_w))
         return out
 
+
+    # -- batches of collisions and launch tuning
+    def set_batch(self, lasers, beams, seed=0):
+        """ptg_set_batch: lasers differing only in a0, one beam per laser."""
+        if len(lasers) != len(beams):
+            raise ValueError("one beam per laser")
+        n = len(lasers)
+        la = (_abi.LaserDesc * n)(*[l.desc() for l in lasers])
+        be = (_abi.BeamDesc * n)(*[b.desc() for b in beams])
+        self._check(self._lib.ptg_set_batch(self._ctx, la, be, n, seed))
+        self.laser = lasers[0]
+
+    def autotune(self, options: CollideOptions, seed=0, max_primaries=0):
+        r = _abi.TuneResult()
+        self._check(self._lib.ptg_autotune(self._ctx, C.byref(options.desc()), seed, max_primaries, C.byref(r)))
+        k = r.n_candidates
+        return {"block": r.block, "grid_per_cu": r.grid_per_cu, "consistent": bool(r.consistent),
+                "candidates": [(r.cand_block[i], r.cand_grid_per_cu[i], r.cand_ms[i]) for i in range(k)]}
 
     # -- learned sampler
     def surrogate_train(self, polarization, groups=8192, steps=6000, batch=4096, lr=2e-3, a_range=(0.25, 20.0),

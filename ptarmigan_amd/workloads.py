@@ -36,3 +36,26 @@ def plane_wave_lma(n=100_000):
     beam = Beam(n=n, gamma=gamma, sigma=0.01 * gamma, species=Species.ELECTRON, rms_div=1e-6)
     options = CollideOptions(tracking_photons=False)
     return laser, beam, options
+
+
+def batched_scan(n_coll=64, scale=1.0, seed=2024, polarization=Polarization.LINEAR):
+    """An a0 and energy scan as one batch: n_coll linearly polarised plane-wave
+    collisions (0.8 um, 16-cycle cos^2), a0 log-spaced over [0.5, 20], electron
+    energy linear over [5, 17.5] GeV (1% spread, 1 urad divergence), primaries per
+    collision floor(10^U(4, 6)) from a fixed seed (uneven; their sum is about
+    1.4e7 for 64 collisions), times `scale`; LMA Compton and pair creation on.
+    Returns (lasers, beams, options)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    counts = np.floor(10.0 ** rng.uniform(4.0, 6.0, n_coll)).astype(np.int64)
+    counts = np.maximum(1, np.floor(counts * scale)).astype(np.int64)
+    a0s = np.exp(np.linspace(np.log(0.5), np.log(20.0), n_coll))
+    energies = np.linspace(5000.0, 17500.0, n_coll)  # MeV
+    lasers, beams = [], []
+    for k in range(n_coll):
+        lasers.append(Laser(LaserKind.PLANE_WAVE, a0=float(a0s[k]), wavelength=0.8e-6, n_cycles=16.0,
+                            envelope=Envelope.COS_SQUARED, polarization=polarization))
+        gamma = energies[k] / ELECTRON_MASS_MEV
+        beams.append(Beam(n=int(counts[k]), gamma=gamma, sigma=0.01 * gamma, species=Species.ELECTRON, rms_div=1e-6))
+    options = CollideOptions(tracking_photons=True)
+    return lasers, beams, options

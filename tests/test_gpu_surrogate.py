@@ -102,10 +102,15 @@ def test_surrogate_shower_matches_exact(engine, pol, a0):
     for comp in (1, 2):
         ae = np.arctan2(se["u"][pe, comp], -se["u"][pe, 3])
         as_ = np.arctan2(ss["u"][ps, comp], -ss["u"][ps, 3])
-        lim = np.quantile(np.abs(np.concatenate([ae, as_])), 0.999)
+        lim = np.quantile(np.abs(ae), 0.999)
         p_a = _chi2_two_sample(ae, as_, np.linspace(-lim, lim, 201))
-        say(f"  angle {comp} chi2 p = {p_a:.3g}; rms {ae.std():.4g} vs {as_.std():.4g}")
-        assert abs(as_.std() - ae.std()) / ae.std() < 0.03
+        core_e, core_s = ae[np.abs(ae) <= lim], as_[np.abs(as_) <= lim]
+        # the wide-angle tail (soft photons at small s): counts beyond the exact 99.9% range
+        te, ts = np.sum(np.abs(ae) > lim), np.sum(np.abs(as_) > lim)
+        say(f"  angle {comp} chi2 p = {p_a:.3g}; core rms {core_e.std():.4g} vs {core_s.std():.4g}; tail {te} vs {ts}")
+        assert abs(core_s.std() - core_e.std()) / core_e.std() < 0.03
+        assert abs(ts - te) < 5.0 * math.sqrt(te + ts + 1.0)
+        assert p_a > 1e-4
     assert p_e > 1e-4
     for col in (1, 2, 3):
         x, y = se["pol"][pe, col], ss["pol"][ps, col]

@@ -93,13 +93,105 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_bytes": (nbytes // 8) * 8, "dispatches": 5}))
 
 
+def _chi2_p(a, b, bins):
+    import math
+    import numpy as np
+    from scipy.stats import chi2 as c2
+    ha, _ = np.histogram(a, bins)
+    hb, _ = np.histogram(b, bins)
+    na, nb = ha.sum(), hb.sum()
+    m = (ha + hb) > 0
+    k1, k2 = math.sqrt(nb / na), math.sqrt(na / nb)
+    return float(c2.sf(np.sum((k1 * ha[m] - k2 * hb[m]) ** 2 / (ha[m] + hb[m])), m.sum() - 1))
+
+
+def _photon_stats(sec):
+    import numpy as np
+    ph = sec["species"] == 2
+    return sec["u"][ph, 0], np.arctan2(sec["u"][ph, 1], -sec["u"][ph, 3]), float(sec["weight"][ph].sum())
+
+
+def surrogate_compare(eng, sync, a0=10.0, n=100_000):
+    """The learned sampler's configuration at one a0: LP plane wave (0.8 um,
+    16-cycle cos^2), 10 GeV electrons with 1% spread, LMA emission. Plain FP64
+    against the shipped surrogate with mixed precision, same seed: particle-steps/s
+    of both, their ratio, the photon-yield error and chi-square p of the 200-bin
+    photon energy and angle spectra against the FP64 run."""
+    import numpy as np
+    from ptarmigan_amd import surrogate, workloads
+    model, meta = surrogate.load(surrogate.default_path(0))
+    eng.surrogate_load(model)
+    laser, beam, opts = workloads.plane_wave_lma(n)
+    laser.a0 = a0
+    eng.set_laser(laser)
+    eng.build_beam(beam, seed=1)
+    out = {}
+    for mode in ("fp64", "surrogate_fp32"):
+        opts.surrogate = mode != "fp64"
+        opts.precision = 1 if mode != "fp64" else 0
+        eng.run(opts, seed=1)
+        sync()
+        t0 = time.perf_counter()
+        c = eng.run(opts, seed=2)
+        sync()
+        el = time.perf_counter() - t0
+        sec, _ = eng.fetch(c)
+        out[mode] = (c, el, _photon_stats(sec))
+    (cf, tf, (ef, af, yf)), (cs, ts, (es, as_, ys)) = out["fp64"], out["surrogate_fp32"]
+    lim = float(np.quantile(np.abs(af), 0.999))
+    return {"workload": f"PlaneWave LP a0={a0:g} 0.8um 16-cycle cos^2, 10 GeV e- 1% spread, {n} primaries, LMA emission",
+            "fp64_steps_per_s": (cf["lepton_steps"] + cf["photon_steps"]) / tf,
+            "surrogate_fp32_steps_per_s": (cs["lepton_steps"] + cs["photon_steps"]) / ts,
+            "speedup": tf / ts, "photon_yield_err": (ys - yf) / yf,
+            "energy_chi2_p": _chi2_p(ef, es, np.linspace(0.0, max(ef.max(), es.max()), 201)),
+            "angle_chi2_p": _chi2_p(af, as_, np.linspace(-lim, lim, 201)),
+            "surrogate_events": cs["surrogate_events"], "exact_fallbacks": cs["surrogate_fallbacks"],
+            "model_valid": meta["valid"]}
+
+
+def scan_compare(eng, sync, scale=0.01):
+    """The batched scan (64 LP plane-wave collisions in one population, a0 0.5-20,
+    5-17.5 GeV, uneven primary counts scaled by `scale`, Compton and pair creation)
+    with the tuned launch shape: plain FP64 against surrogate + mixed precision."""
+    import numpy as np
+    from ptarmigan_amd import surrogate, workloads
+    model, _ = surrogate.load(surrogate.default_path(0))
+    eng.surrogate_load(model)
+    lasers, beams, opts = workloads.batched_scan(64, scale=scale)
+    eng.set_batch(lasers, beams, seed=5)
+    opts.surrogate = True
+    opts.precision = 1
+    tune = eng.autotune(opts, seed=5, max_primaries=200_000)
+    out = {}
+    for mode in ("surrogate_fp32", "fp64"):
+        opts.surrogate = mode != "fp64"
+        opts.precision = 1 if mode != "fp64" else 0
+        sync()
+        t0 = time.perf_counter()
+        c = eng.run(opts, seed=6)
+        sync()
+        el = time.perf_counter() - t0
+        sec, _ = eng.fetch(c)
+        out[mode] = (c, el, _photon_stats(sec))
+    (cf, tf, (ef, _, yf)), (cs, ts, (es, _, ys)) = out["fp64"], out["surrogate_fp32"]
+    return {"workload": f"64 PlaneWave LP collisions in one batch, a0 0.5-20, 5-17.5 GeV, "
+                        f"{sum(b.n for b in beams)} primaries, Compton + Breit-Wheeler",
+            "fp64_steps_per_s": (cf["lepton_steps"] + cf["photon_steps"]) / tf,
+            "surrogate_fp32_steps_per_s": (cs["lepton_steps"] + cs["photon_steps"]) / ts,
+            "speedup": tf / ts, "photon_yield_err": (ys - yf) / yf,
+            "energy_chi2_p": _chi2_p(ef, es, np.linspace(0.0, max(ef.max(), es.max()), 201)),
+            "tuned_launch": {"block": tune["block"], "grid_per_cu": tune["grid_per_cu"], "consistent": tune["consistent"]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--primaries", type=int, default=10_000_000)
-    ap.add_argument("--cpu-primaries", type=int, default=40_000)
+    ap.add_argument("--cpu-primaries", type=int, default=0, help="0: 80 000 (lcfa) or 2 000 000 (lma)")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the surrogate + FP32 vs plain FP64 comparisons (learned-sampler and scan workloads)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="lcfa",
@@ -164,7 +256,13 @@ def main():
                 "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
                 "kernel": kname, "kernel_avg_ms": round(lep_launch_ms, 3), "flops_per_step": flops}
 
+    compare = None
+    if not args.no_compare and args.workload == "lcfa" and args.precision == "fp64":
+        compare = {"learned_sampler": surrogate_compare(eng, sync), "batched_scan": scan_compare(eng, sync)}
+
     cpu = None
+    if not args.cpu_primaries:
+        args.cpu_primaries = 80_000 if args.workload == "lcfa" else 2_000_000
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         v, sample = cpu_baseline(args.workload, args.cpu_primaries, threads)
@@ -178,6 +276,7 @@ def main():
         "config": {"workload": WORKLOADS[args.workload], "primaries": args.primaries, "precision": args.precision},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "surrogate_fp32_vs_fp64": compare,
         "detail": {"lepton_steps": totals["lepton_steps"], "photon_steps": totals["photon_steps"],
                    "emissions": totals["emissions"], "pair_events": totals["pair_events"],
                    "final_state_per_step": totals["n_secondaries"] / args.steps,

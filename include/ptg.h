@@ -233,6 +233,10 @@ int ptg_push(ptg_ctx* ctx, const uint8_t* species, double* r, double* u, size_t 
 /* Weighted histogram of the resident final-state particles (hgram.rs:209-323).
  * bins_out has nbins entries (min/max written back when computed). */
 int ptg_histogram(ptg_ctx* ctx, ptg_hist_spec* spec, double* bins_out, double* total_weight);
+/* Weighted 2-D histogram (Histogram::generate_2d, hgram.rs:326-446): species and
+ * weight from spec_x; bins_out has nbins_x * nbins_y entries, flat index
+ * by * nbins_y + bx as the reference (rows overlap when the counts differ). */
+int ptg_histogram_2d(ptg_ctx* ctx, ptg_hist_spec* spec_x, ptg_hist_spec* spec_y, double* bins_out, double* total_weight);
 
 /* Batched evaluation of one device function for parity testing. fn ids are
  * listed in ptg_eval_fn; in is n x in_w, out is n x out_w (host buffers). */
@@ -332,7 +336,7 @@ typedef struct {
 typedef struct {
     double loss[3];          /* mean losses over the last 10% of steps: harmonic, fraction (mean squares), angle (pinball) */
     double val_harmonic_rel; /* held-out groups: mean |n_pred - n| / n */
-    double val_fraction_abs; /* held-out groups: mean |v_pred(u2) - v| at the rank-coupled levels */
+    double val_fraction_abs; /* held-out groups: mean |v_pred(u2) - v| at the rank levels (k + 1) / 65 */
     double ks_fraction;      /* held-out samples: Kolmogorov-Smirnov distance of the network's CDF of v at the
                                 samples (probability integral transform, by bisection in u2) from uniform */
     double ks_angle;         /* the same for theta at the samples' v (LP; 0 for CP) */

@@ -95,7 +95,7 @@ class HistSpec(C.Structure):
 EXPORTS = ("ptg_create", "ptg_destroy", "ptg_last_error", "ptg_set_laser", "ptg_field_info", "ptg_build_beam",
            "ptg_load_primaries", "ptg_run", "ptg_collide", "ptg_fetch", "ptg_push", "ptg_histogram", "ptg_eval",
            "ptg_surrogate_train", "ptg_surrogate_load", "ptg_mlp_probe", "ptg_set_batch", "ptg_autotune",
-           "ptg_fetch_showers")
+           "ptg_fetch_showers", "ptg_histogram_2d")
 
 _lib = None
 
@@ -132,6 +132,7 @@ def load():
     L.ptg_mlp_probe.argtypes = [vp, P(f), P(f), P(f), C.c_size_t, f, P(f), P(f), P(f)]
     L.ptg_set_batch.argtypes = [vp, P(LaserDesc), P(BeamDesc), C.c_int, C.c_uint64]
     L.ptg_fetch_showers.argtypes = [vp, P(C.c_uint32)]
+    L.ptg_histogram_2d.argtypes = [vp, P(HistSpec), P(HistSpec), P(d), P(d)]
     L.ptg_autotune.argtypes = [vp, P(CollideOptions), C.c_uint64, C.c_uint64, P(TuneResult)]
     for f in ("ptg_set_laser", "ptg_field_info", "ptg_build_beam", "ptg_load_primaries", "ptg_run", "ptg_collide",
               "ptg_fetch", "ptg_push", "ptg_histogram", "ptg_eval"):

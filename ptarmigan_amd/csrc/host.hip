@@ -1368,6 +1368,60 @@ int ptg_histogram(ptg_ctx* c, ptg_hist_spec* s, double* bins_out, double* total_
     return PTG_OK;
 }
 
+int ptg_histogram_2d(ptg_ctx* c, ptg_hist_spec* sx, ptg_hist_spec* sy, double* bins_out, double* total_weight) {
+    if (!c || !sx || !sy || !bins_out) return fail(c, PTG_E_ARG, "null argument");
+    for (const ptg_hist_spec* s : {sx, sy}) {
+        if (s->nbins == 0 || s->nbins > 4096) return fail(c, PTG_E_ARG, "nbins must be in [1, 4096] per axis");
+        if (s->quantity < 0 || s->quantity >= PTG_Q_COUNT) return fail(c, PTG_E_ARG, "unknown quantity");
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    HistArgs h[2];
+    const ptg_hist_spec* sp[2] = {sx, sy};
+    int grid = std::max(1, std::min(c->n_cu * 4, int((c->n_out + 255) / 256)));
+    for (int a = 0; a < 2; ++a) {
+        h[a] = HistArgs{};
+        h[a].p = c->out.v;
+        h[a].n = c->n_out;
+        h[a].species = sx->species;  // one filter and one weight for both axes (hgram.rs:326-331)
+        h[a].q = sp[a]->quantity;
+        h[a].wfn = sx->weight_fn;
+        h[a].log_scale = sp[a]->log_scale;
+        double gmin = sp[a]->min, gmax = sp[a]->max;
+        if (!sp[a]->fixed_range) {
+            unsigned long long mm[2] = {~0ull, 0ull};
+            HIPCHK(c, scopy(c, c->d_ctr + C_WORK, mm, sizeof(mm), hipMemcpyHostToDevice));
+            if (h[a].n) launch_hist_minmax(h[a], c->d_ctr + C_WORK, grid, c->stream);
+            HIPCHK(c, hipGetLastError());
+            HIPCHK(c, scopy(c, mm, c->d_ctr + C_WORK, sizeof(mm), hipMemcpyDeviceToHost));
+            if (mm[0] == ~0ull) {
+                if (total_weight) *total_weight = 0.0;
+                return fail(c, PTG_E_DOMAIN, "no particles of this species to histogram");
+            }
+            gmin = host_order_key_inv(mm[0]);
+            gmax = host_order_key_inv(mm[1]);
+        }
+        h[a].nbins = gmin == gmax ? 1u : sp[a]->nbins;  // number_of_bins (hgram.rs:171-186)
+        h[a].gmin = gmin;
+        h[a].bin_vol = gmin == gmax ? 0.0 : (gmax - gmin) / double(h[a].nbins);  // bin size 0 for a degenerate axis
+        const_cast<ptg_hist_spec*>(sp[a])->min = gmin;
+        const_cast<ptg_hist_spec*>(sp[a])->max = gmax;
+    }
+    const size_t nb = size_t(h[0].nbins) * h[1].nbins;
+    for (size_t b = 0; b < size_t(sx->nbins) * sy->nbins; ++b) bins_out[b] = 0.0;
+    double* d_bins = nullptr;
+    HIPCHK(c, hipMalloc(&d_bins, (nb + 1) * sizeof(double)));
+    HIPCHK(c, hipMemsetAsync(d_bins, 0, (nb + 1) * sizeof(double), c->stream));
+    if (c->n_out) launch_hist_bin2(h[0], h[1], d_bins, d_bins + nb, grid, c->stream);
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> hb(nb + 1);
+    HIPCHK(c, hipMemcpyAsync(hb.data(), d_bins, (nb + 1) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(d_bins);
+    for (size_t b = 0; b < nb; ++b) bins_out[b] = hb[b];
+    if (total_weight) *total_weight = hb[nb];
+    return PTG_OK;
+}
+
 int ptg_eval(ptg_ctx* c, int fn, const double* in, size_t n, size_t in_w, double* out, size_t out_w) {
     if (!c || !in || !out) return fail(c, PTG_E_ARG, "null argument");
     if (fn < 0 || fn >= PTG_FN_COUNT) return fail(c, PTG_E_ARG, "unknown function id");
@@ -1759,8 +1813,10 @@ int ptg_surrogate_train(ptg_ctx* c, const ptg_surrogate_train_desc* d, ptg_surro
     if ((e = hipStreamSynchronize(c->stream))) return hip_fail(e);
     float gen_ms = 0.0f;
     (void)hipEventElapsedTime(&gen_ms, c->ev0, c->ev1);
-    // rows: each group's 64 samples sorted by v, at quantile levels (k + 1/2) / 64;
-    // every eighth group held out for validation
+    // rows: each group's 64 samples sorted by v, the k-th at the level (k + 1) / 65
+    // (the mean of the k-th of 64 ordered uniforms, so that the mean-square fit of
+    // the ordered samples is the quantile function, tails included); every eighth
+    // group held out for validation
     std::vector<float> rows, vrows;
     int used = 0;
     for (int g = 0; g < d->groups; ++g) {
@@ -1773,7 +1829,7 @@ int ptg_surrogate_train(ptg_ctx* c, const ptg_surrogate_train_desc* d, ptg_surro
         std::sort(idx, idx + 64, [&](int x, int y) { return smp[2 * x] < smp[2 * y]; });
         std::vector<float>& dst = (g % 8 == 7) ? vrows : rows;
         for (int k = 0; k < 64; ++k) {
-            const float row[8] = {float(h[0]), float(h[1]), float(h[2]), float((k + 0.5) / 64.0),
+            const float row[8] = {float(h[0]), float(h[1]), float(h[2]), float((k + 1.0) / 65.0),
                                   float(std::log(std::max(h[3], 0.9)) / 5.0), smp[2 * idx[k]], smp[2 * idx[k] + 1], 0.0f};
             dst.insert(dst.end(), row, row + 8);
         }
@@ -1855,15 +1911,25 @@ int ptg_surrogate_train(ptg_ctx* c, const ptg_surrogate_train_desc* d, ptg_surro
     // the level u2 at which the network's v reaches the sample's v (bisection),
     // then, at that level, the u3 at which its theta reaches the sample's theta;
     // uniform if the network has the samples' distribution
+    // v as the inference kernel forms it: the network at the level clamped to
+    // [1/65, 64/65], continued as sqrt(u) towards 0 and linearly towards 1 beyond
+    auto level_in = [](int col, double u) { return col == 3 ? std::min(std::max(u, 1.0 / 65.0), 64.0 / 65.0) : u; };
+    auto value_out = [](int col, double u, double o) {
+        if (col != 1) return o;
+        const double v = std::min(std::max(o, 0.0), 1.0);
+        if (u < 1.0 / 65.0) return v * std::sqrt(u * 65.0);
+        if (u > 64.0 / 65.0) return 1.0 - (1.0 - v) * (1.0 - u) * 65.0;
+        return v;
+    };
     auto pit = [&](int col_in, int col_out, int col_target, std::vector<double>& level) -> hipError_t {
         std::vector<double> lo(nv, 0.0), hi(nv, 1.0);
         for (int it = 0; it < 20; ++it) {
-            for (size_t r = 0; r < nv; ++r) vx[r * 8 + col_in] = float(lo[r] + hi[r] - 1.0);  // 2 mid - 1
+            for (size_t r = 0; r < nv; ++r) vx[r * 8 + col_in] = float(2.0 * level_in(col_in, 0.5 * (lo[r] + hi[r])) - 1.0);
             hipError_t err = infer();
             if (err) return err;
             for (size_t r = 0; r < nv; ++r) {
                 const double mid = 0.5 * (lo[r] + hi[r]);
-                if (vo[r * mlp::OUT + col_out] < vrows[r * 8 + col_target]) lo[r] = mid;
+                if (value_out(col_out, mid, vo[r * mlp::OUT + col_out]) < vrows[r * 8 + col_target]) lo[r] = mid;
                 else hi[r] = mid;
             }
         }
@@ -1884,7 +1950,7 @@ int ptg_surrogate_train(ptg_ctx* c, const ptg_surrogate_train_desc* d, ptg_surro
     const double ks_v = ks_uniform(pit_v);
     double ks_t = 0.0;
     if (!circ) {
-        for (size_t r = 0; r < nv; ++r) vx[r * 8 + 3] = float(2.0 * pit_v[r] - 1.0);
+        for (size_t r = 0; r < nv; ++r) vx[r * 8 + 3] = float(2.0 * level_in(3, pit_v[r]) - 1.0);
         if ((e = pit(4, 2, 6, pit_t))) return hip_fail(e);
         ks_t = ks_uniform(pit_t);
     }

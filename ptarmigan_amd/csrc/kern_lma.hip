@@ -737,6 +737,19 @@ void launch_push_lma(const LaserDev& L, double dt, int ll, const uint8_t* sp_, d
 __device__ __forceinline__ bool surr_in_domain(const SurrArgs& S, double la, double le) {
     return fabs(la - S.la_c) <= S.la_h && fabs(le - S.le_c) <= S.le_h;
 }
+// The network is trained at the levels u2 = (k + 1) / 65, k = 0..63 (the means of
+// 64 ordered uniforms); outside [1/65, 64/65] the quantile function of v is
+// continued from the network's value at the end level: towards v = 0 as
+// sqrt(u2) (a density rising linearly from zero, as the rate of every harmonic
+// above the first vanishes at s = 0) and linearly to v = 1 at u2 = 1.
+constexpr double SURR_U2_LO = 1.0 / 65.0, SURR_U2_HI = 64.0 / 65.0;
+__device__ __forceinline__ double surr_u2_level(double u2) { return fmin(fmax(u2, SURR_U2_LO), SURR_U2_HI); }
+__device__ __forceinline__ double surr_v_tail(double u2, double v_net) {
+    const double v = fmin(fmax(v_net, 0.0), 1.0);
+    if (u2 < SURR_U2_LO) return v * sqrt(u2 / SURR_U2_LO);
+    if (u2 > SURR_U2_HI) return 1.0 - (1.0 - v) * ((1.0 - u2) / (1.0 - SURR_U2_HI));
+    return v;
+}
 __device__ __forceinline__ void surr_features(const SurrArgs& S, double la, double le, double u1, double u2, double u3, float* x) {
     x[0] = float((la - S.la_c) / S.la_h);
     x[1] = float((le - S.le_c) / S.le_h);
@@ -774,7 +787,7 @@ __global__ void k_surr_inputs(const TransportArgs* __restrict__ A, LmaArgs M, Su
     if (surr_in_domain(S, la, le)) {
         const unsigned long long j = atomicAdd(S.n_rows, 1ull);
         S.list[j] = i;
-        surr_features(S, la, le, u1, u2, u3, S.x + 8 * j);
+        surr_features(S, la, le, u1, surr_u2_level(u2), u3, S.x + 8 * j);
     } else {
         S.fb[atomicAdd(S.n_fb, 1ull)] = i;
     }
@@ -803,13 +816,15 @@ __global__ __launch_bounds__(64) void k_surr_event(const TransportArgs* __restri
         const float* o = S.out + 32 * r;
         const double nu = exp(5.0 * double(o[0]));
         int n = nu > 1.0 ? int(fmin(ceil(nu), 1.0e7)) : 1;
-        // 0 < v < 1 as the exact samplers' draws: the photon construction divides by s
-        // and takes sqrt(1 - cos^2) of an angle that reaches pi at v = 1. Below the
-        // lowest trained quantile level the network extrapolates; v < 1e-4 (wide-angle,
-        // soft photons, probability ~1e-4 under the exact sampler) is cut to 1e-4.
-        const double v = fmin(fmax(double(o[1]), 1.0e-4), 1.0 - 1.0e-7);
         Rng base = rng_make(A->key, q.stream[i]);
         base.c1 = q.nev[i] | EVENT_BLOCK;
+        Rng g2 = base;  // the level u2 of the inputs (counter 2)
+        g2.c0 = 2u;
+        double u2, u3;
+        rng_uniform2(g2, u2, u3);
+        // 0 < v < 1 as the exact samplers' draws: the photon construction divides by s
+        // and takes sqrt(1 - cos^2) of an angle that reaches pi at v = 1
+        const double v = fmin(fmax(surr_v_tail(u2, double(o[1])), 1.0e-12), 1.0 - 1.0e-12);
         double s, theta;
         sv4 sv;
         if (CIRC) {

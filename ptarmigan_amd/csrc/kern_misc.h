@@ -40,6 +40,9 @@ void launch_build_beam(const BeamDev& b, const PBuf& p, double* n0, hipStream_t 
 void launch_primary_dirs(const PBuf& p, uint32_t n, double* n0, hipStream_t s);
 void launch_hist_minmax(const HistArgs& h, unsigned long long* mm, int grid, hipStream_t s);
 void launch_hist_bin(const HistArgs& h, double* bins, double* total, int grid, hipStream_t s);
+// 2-D weighted binning (hgram.rs:326-446): hx gives species, weight and the x axis,
+// hy the y axis; flat bin by * ny + bx as the reference, dropped past nx * ny.
+void launch_hist_bin2(const HistArgs& hx, const HistArgs& hy, double* bins, double* total, int grid, hipStream_t s);
 double host_order_key_inv(unsigned long long k);
 void launch_select_multiplicity(const PBuf& src, uint64_t n, const PBuf& dst, unsigned long long* ctr, const uint32_t* count,
                                 uint32_t want, hipStream_t s);

@@ -212,6 +212,51 @@ __global__ void k_hist_bin(HistArgs h, double* bins, double* total) {
         if (lbins[b] != 0.0) atomicAdd(&bins[b], lbins[b]);
 }
 
+// 2-D: a value pair is binned only if both are finite; the flat index is
+// by * ny + bx (hgram.rs:418, as the reference: row index times the number of
+// y bins), so with nx != ny rows overlap and indices past nx * ny are dropped.
+// Bins in LDS when they fit (<= 6144 doubles), else straight to global atomics.
+template <bool LDS>
+__global__ void k_hist_bin2(HistArgs hx, HistArgs hy, double* bins, double* total) {
+    extern __shared__ double lbins[];
+    const uint32_t nb = hx.nbins * hy.nbins;
+    if (LDS) {
+        for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) lbins[b] = 0.0;
+        __syncthreads();
+    }
+    double tot = 0.0;
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < hx.n; i += uint64_t(gridDim.x) * blockDim.x) {
+        if ((hx.p.meta[i] & 3u) != uint32_t(hx.species)) continue;
+        double vx = quantity(hx.p, i, hx.q), vy = quantity(hx.p, i, hy.q);
+        if (hx.log_scale) vx = log(vx);
+        if (hy.log_scale) vy = log(vy);
+        const double w = hweight(hx.p, i, hx.wfn);
+        tot += w;
+        if (!isfinite(vx) || !isfinite(vy)) continue;
+        double bx = hx.bin_vol == 0.0 ? 0.0 : floor((vx - hx.gmin) / hx.bin_vol);
+        double by = hy.bin_vol == 0.0 ? 0.0 : floor((vy - hy.gmin) / hy.bin_vol);
+        if (!(bx >= 0.0)) bx = 0.0;  // Rust `as usize` saturates at 0
+        if (!(by >= 0.0)) by = 0.0;
+        const double fb = by * double(hy.nbins) + bx;
+        if (fb < double(nb)) {
+            if (LDS) atomicAdd(&lbins[uint32_t(fb)], w);
+            else atomicAdd(&bins[uint32_t(fb)], w);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
+    if ((threadIdx.x & 63) == 0 && tot != 0.0) atomicAdd(total, tot);
+    if (LDS) {
+        __syncthreads();
+        for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x)
+            if (lbins[b] != 0.0) atomicAdd(&bins[b], lbins[b]);
+    }
+}
+void launch_hist_bin2(const HistArgs& hx, const HistArgs& hy, double* bins, double* total, int grid, hipStream_t s) {
+    const uint32_t nb = hx.nbins * hy.nbins;
+    if (nb <= 6144) k_hist_bin2<true><<<grid, 256, nb * sizeof(double), s>>>(hx, hy, bins, total);
+    else k_hist_bin2<false><<<grid, 256, 0, s>>>(hx, hy, bins, total);
+}
+
 void launch_hist_minmax(const HistArgs& h, unsigned long long* mm, int grid, hipStream_t s) {
     k_hist_minmax<<<grid, 256, 0, s>>>(h, mm);
 }

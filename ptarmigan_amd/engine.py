@@ -280,6 +280,16 @@ class Engine:
         self._check(self._lib.ptg_histogram(self._ctx, C.byref(spec), _dptr(out), C.byref(tot)))
         return out, spec.min, spec.max, tot.value
 
+    def histogram2d(self, species, qx, qy, nbins_x, nbins_y, weight_fn=0, log_x=False, log_y=False):
+        """Histogram::generate_2d over the resident final-state particles; returns
+        (flat counts of nbins_x * nbins_y, (xmin, xmax), (ymin, ymax), total)."""
+        sx = _abi.HistSpec(int(species), int(qx), int(weight_fn), int(log_x), nbins_x, 0, 0.0, 0.0)
+        sy = _abi.HistSpec(int(species), int(qy), int(weight_fn), int(log_y), nbins_y, 0, 0.0, 0.0)
+        out = np.zeros(nbins_x * nbins_y)
+        tot = C.c_double()
+        self._check(self._lib.ptg_histogram_2d(self._ctx, C.byref(sx), C.byref(sy), _dptr(out), C.byref(tot)))
+        return out, (sx.min, sx.max), (sy.min, sy.max), tot.value
+
     def eval(self, fn, x, out_w):
         """Batched evaluation of one device function (ptg_eval_fn ids) for parity tests."""
         x = np.ascontiguousarray(np.atleast_2d(np.asarray(x, dtype=np.float64)))

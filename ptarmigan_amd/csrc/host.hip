@@ -148,6 +148,11 @@ struct ptg_ctx {
     int step_block = 256, step_grid_per_cu = 4;
     uint32_t* d_surr_fb = nullptr;
     size_t surr_list_cap = 0;
+    // emission events the learned sampler leaves to the exact kernels, held until
+    // the lepton phase drains (one exact pass per phase instead of one per generation)
+    DevBuf defq;
+    double* d_def_extra = nullptr;  // kappa^0, sqrt(<a^2>)
+    size_t def_extra_cap = 0, n_def = 0;
 };
 
 namespace {
@@ -510,6 +515,8 @@ void ptg_destroy(ptg_ctx* c) {
     c->surr[1].release();
     if (c->d_surr_list) (void)hipFree(c->d_surr_list);
     if (c->d_surr_fb) (void)hipFree(c->d_surr_fb);
+    free_buf(c->defq);
+    if (c->d_def_extra) (void)hipFree(c->d_def_extra);
     if (c->d_lasers) (void)hipFree(c->d_lasers);
     if (c->d_coll) (void)hipFree(c->d_coll);
     if (c->d_bw) (void)hipFree(c->d_bw);
@@ -737,15 +744,16 @@ static int ensure_ev_extra(ptg_ctx* c, size_t n) {
 // m.scratch_len are deferred and run again with one sized for the largest; their
 // RNG counters make the second pass draw the same numbers.
 static int lma_event_pass(ptg_ctx* c, LmaArgs m, bool pairs, size_t n_ev, size_t max_lanes, double* ms, uint64_t* launches,
-                          const uint32_t* list = nullptr) {
-    if (c->ovf_cap < c->evq.cap) {
+                          const uint32_t* list = nullptr, const DevBuf* in = nullptr) {
+    const size_t ovf_need = std::max(c->evq.cap, in ? in->cap : size_t(0));
+    if (c->ovf_cap < ovf_need) {
         if (c->d_ovf) (void)hipFree(c->d_ovf);
         c->d_ovf = nullptr;
-        HIPCHK(c, hipMalloc(&c->d_ovf, c->evq.cap * sizeof(uint32_t)));
-        c->ovf_cap = c->evq.cap;
+        HIPCHK(c, hipMalloc(&c->d_ovf, ovf_need * sizeof(uint32_t)));
+        c->ovf_cap = ovf_need;
     }
     HIPCHK(c, hipMemsetAsync(c->d_ctr + C_OVF, 0, 2 * sizeof(unsigned long long), c->stream));
-    m.ev_in = c->evq.v;
+    m.ev_in = in ? in->v : c->evq.v;
     m.ev_list = list;
     m.n_ev = n_ev;
     m.ovf_list = c->d_ovf;
@@ -861,7 +869,28 @@ static int lma_surrogate_pass(ptg_ctx* c, LmaArgs m, size_t n_ev, size_t max_lan
     ++*launches;
     const size_t n_fb = size_t(cnt[1]);
     *n_fallback += n_fb;
-    if (n_fb > 0) return lma_event_pass(c, m, false, n_fb, max_lanes, ms, launches, c->d_surr_fb);
+    if (n_fb > 0) {  // held for one exact pass when the lepton phase drains
+        const size_t need = c->n_def + n_fb;
+        if (c->defq.cap < need) HIPCHK(c, grow_buf(c->defq, need + need / 2 + 64, c->n_def, c->stream));
+        if (c->def_extra_cap < c->defq.cap) {
+            double* ne = nullptr;
+            HIPCHK(c, hipMalloc(&ne, 2 * c->defq.cap * sizeof(double)));
+            if (c->n_def) {
+                HIPCHK(c, hipMemcpyAsync(ne, c->d_def_extra, c->n_def * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+                HIPCHK(c, hipMemcpyAsync(ne + c->defq.cap, c->d_def_extra + c->def_extra_cap, c->n_def * sizeof(double),
+                                         hipMemcpyDeviceToDevice, c->stream));
+                HIPCHK(c, hipStreamSynchronize(c->stream));
+            }
+            if (c->d_def_extra) (void)hipFree(c->d_def_extra);
+            c->d_def_extra = ne;
+            c->def_extra_cap = c->defq.cap;
+        }
+        launch_gather_events(c->evq.v, m.ev_k0, m.ev_a, c->d_surr_fb, n_fb, c->defq.v, c->d_def_extra,
+                             c->d_def_extra + c->def_extra_cap, c->n_def, c->stream);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->n_def += n_fb;
+    }
     return PTG_OK;
 }
 
@@ -936,8 +965,10 @@ static int run_lma(ptg_ctx* c, TransportArgs a, double* ms, uint64_t* launches, 
     DevBuf* cur = &c->prim;
     size_t n_cur = n;
     int li = 0;
+    c->n_def = 0;
     for (;;) {
         // ---------------- lepton phase
+        for (;;) {
         while (n_cur > 0) {
             // each live lepton yields at most one final state or one event; each event
             // at most one photon and one continuing lepton
@@ -994,6 +1025,41 @@ static int run_lma(ptg_ctx* c, TransportArgs a, double* ms, uint64_t* launches, 
             n_cur = size_t(now[C_LNEXT]);
             li ^= 1;
             ++*gens;
+        }
+        if (c->n_def == 0) break;
+        // the events the learned sampler left to the exact kernels, in one pass; their
+        // electrons continue in another lepton round
+        {
+            const size_t nd = c->n_def;
+            HIPCHK(c, grow_buf(c->out, c->n_out + nd + 1024, c->n_out, c->stream));
+            DevBuf& nxt = c->lq[li];
+            if (nxt.cap < nd) HIPCHK(c, alloc_buf(nxt, nd + nd / 4));
+            unsigned long long n_ph = read_ctr(c, track ? C_PH : C_NEXT);
+            HIPCHK(c, grow_buf(c->phq, size_t(n_ph) + nd + 1024, size_t(n_ph), c->stream));
+            HIPCHK(c, zero_ctr(C_LNEXT));
+            a.out = c->out.v;
+            a.next = c->phq.v;
+            a.next_ctr = c->d_ctr + (track ? C_PH : C_NEXT);
+            a.next_cap = c->phq.cap;
+            HIPCHK(c, hipMemcpyAsync(c->d_args, &a, sizeof(a), hipMemcpyHostToDevice, c->stream));
+            m.ph = c->phq.v;
+            m.ph_cap = c->phq.cap;
+            m.next = nxt.v;
+            m.next_ctr = c->d_ctr + C_LNEXT;
+            m.ev_k0 = c->d_def_extra;
+            m.ev_a = c->d_def_extra + c->def_extra_cap;
+            int rc = lma_event_pass(c, m, false, nd, max_lanes, &ms[1], &launches[1], nullptr, &c->defq);
+            if (rc) return rc;
+            c->n_def = 0;
+            unsigned long long now[C_N];
+            HIPCHK(c, scopy(c, now, c->d_ctr, sizeof(now), hipMemcpyDeviceToHost));
+            c->n_out = now[C_OUT];
+            cur = &nxt;
+            n_cur = size_t(now[C_LNEXT]);
+            li ^= 1;
+            ++*gens;
+        }
+        if (n_cur == 0) break;
         }
         if (!track) break;
         // ---------------- photon phase

@@ -91,6 +91,8 @@ void launch_surr_inputs(const TransportArgs* a, const LmaArgs& m, const SurrArgs
 void launch_surr_event(const TransportArgs* a, const LmaArgs& m, const SurrArgs& s, int lanes, hipStream_t st);
 void launch_surr_groups(const LmaArgs& m, const SurrArgs& s, int lanes, hipStream_t st);
 void launch_surr_batch(const LmaArgs& m, const SurrArgs& s, hipStream_t st);
+void launch_gather_events(const PBuf& src, const double* k0s, const double* as_, const uint32_t* list, uint64_t n, const PBuf& dst,
+                          double* k0d, double* ad, uint64_t off, hipStream_t s);
 
 // block: 64, 128 or 256 threads (the kernel's bound); grid: persistent blocks
 void launch_step_lma(const TransportArgs* a, const LmaArgs& m, int grid, hipStream_t s, int block = 256);

@@ -815,34 +815,43 @@ __global__ __launch_bounds__(64) void k_surr_event(const TransportArgs* __restri
         const double amp = CIRC ? a_rms : a_rms * 1.4142135623730951;
         const float* o = S.out + 32 * r;
         const double nu = exp(5.0 * double(o[0]));
-        int n = nu > 1.0 ? int(fmin(ceil(nu), 1.0e7)) : 1;
+        // the network's harmonic is cut at 1.5x the reference's harmonic sum limit
+        // at the event's amplitude (+16), where the tables end: beyond it a network
+        // output is an extrapolation (and 4 n^2 would wrap in the CP Stokes parameters)
+        const double n_cap = ceil(1.5 * (6.0 * (1.0 + amp * amp) + 2.0 * amp * amp * amp)) + 16.0;
+        int n = nu > 1.0 ? int(fmin(ceil(nu), n_cap)) : 1;
         Rng base = rng_make(A->key, q.stream[i]);
         base.c1 = q.nev[i] | EVENT_BLOCK;
         Rng g2 = base;  // the level u2 of the inputs (counter 2)
         g2.c0 = 2u;
         double u2, u3;
         rng_uniform2(g2, u2, u3);
-        // 0 < v < 1 as the exact samplers' draws: the photon construction divides by s
-        // and takes sqrt(1 - cos^2) of an angle that reaches pi at v = 1
-        const double v = fmin(fmax(surr_v_tail(u2, double(o[1])), 1.0e-12), 1.0 - 1.0e-12);
-        double s, theta;
+        // 0 < v < 1 as the exact samplers' draws: the photon construction divides by s,
+        // and sqrt(1 - cos^2) (photon) and sqrt(w (1 - w)) (CP Stokes, w -> 1 as v -> 1)
+        // must not see a rounding past 1
+        // the network's v at the (clamped) level is kept inside [0.005, 0.995]: an
+        // output at or beyond 0 (a small error where the true end quantile is ~0.01)
+        // would otherwise become v ~ 0 and emit photons of the laser frequency co-moving
+        // with the pulse, each stepped for millions of steps
+        const double v_net = double(o[1]);
+        if (!(v_net == v_net)) {
+            S.fb[atomicAdd(S.n_fb, 1ull)] = i;
+            continue;
+        }
+        double v = fmin(fmax(surr_v_tail(u2, fmin(fmax(v_net, 0.005), 0.995)), 1.0e-12), 1.0 - 1.0e-9);
+        double s = 0.0, theta, sn;
         sv4 sv;
         if (CIRC) {
-            const double sn = 2.0 * double(n) * eta / (1.0 + amp * amp);
-            s = v * (sn / (1.0 + sn));
+            sn = 2.0 * double(n) * eta / (1.0 + amp * amp);
             Rng g3 = base;
             g3.c0 = 3u;
             theta = 2.0 * PI_D * rng_uniform(g3);
-            double st[4];
-            cp_stokes(M.lcfa, n, amp, s, sn, v, theta, false, st);
-            sv = {st[0], st[1], st[2], st[3]};
         } else {
             // the workspace holds the harmonics of the tables (1.5 x the reference's sum
             // limit at the laser's a0); a network output beyond it is cut to it
             while (n > 1 && dbessel_len(n) > M.scratch_len) n = n - 1 - (dbessel_len(n) - M.scratch_len) / 4;
             if (n < 1) n = 1;
-            const double sn = 2.0 * double(n) * eta / (1.0 + 0.5 * amp * amp);
-            s = v * (sn / (1.0 + sn));
+            sn = 2.0 * double(n) * eta / (1.0 + 0.5 * amp * amp);
             const double th = fmin(fmax(double(o[2]), 0.0), 1.0) * (PI_D / 2.0);
             Rng gq = base;  // counter 1: the quadrant, as the exact sampler
             gq.c0 = 1u;
@@ -851,7 +860,28 @@ __global__ __launch_bounds__(64) void k_surr_event(const TransportArgs* __restri
             if (quadrant == 1) theta = PI_D - th;
             else if (quadrant == 2) theta = PI_D + th;
             else if (quadrant == 3) theta = 2.0 * PI_D - th;
-            sv = lp_stokes(dbessel_at(n, ws, n_lanes), amp, eta, s, theta);
+        }
+        // At a point where the exact rate underflows to zero (a high harmonic at v near
+        // 0 or 1) the Stokes parameters are 0/0: the exact sampler never draws such a
+        // point. The network's v is then moved halfway to 1/2 until the rate is
+        // resolved (a few tries; such draws are ~1e-5 of all), else the event goes to
+        // the exact kernels.
+        bool ok = false;
+        for (int tr = 0; tr < 6 && !ok; ++tr) {
+            if (tr) v = 0.5 + 0.5 * (v - 0.5);
+            s = v * (sn / (1.0 + sn));
+            if (CIRC) {
+                double st[4];
+                cp_stokes(M.lcfa, n, amp, s, sn, v, theta, false, st);
+                sv = {st[0], st[1], st[2], st[3]};
+            } else {
+                sv = lp_stokes(dbessel_at(n, ws, n_lanes), amp, eta, s, theta);
+            }
+            ok = isfinite(sv.q) && isfinite(sv.u) && isfinite(sv.v);
+        }
+        if (!ok) {
+            S.fb[atomicAdd(S.n_fb, 1ull)] = i;
+            continue;
         }
         lma_emit_finish<false>(A, M, i, k0, a_rms, eta, true, n, s, theta, sv, n_emit, flags);
     }
@@ -995,6 +1025,30 @@ void launch_surr_groups(const LmaArgs& m, const SurrArgs& s, int lanes, hipStrea
 void launch_surr_batch(const LmaArgs& m, const SurrArgs& s, hipStream_t st) {
     if (m.mode == LMA_CP) k_surr_batch<true><<<(s.batch + 255) / 256, 256, 0, st>>>(m, s);
     else k_surr_batch<false><<<(s.batch + 255) / 256, 256, 0, st>>>(m, s);
+}
+
+// Moves events out of the event queue (list of indices) into another queue with
+// their kappa^0 and sqrt(<a^2>), at [off, off + n).
+__global__ void k_gather_events(PBuf src, const double* __restrict__ k0s, const double* __restrict__ as_,
+                                const uint32_t* __restrict__ list, uint64_t n, PBuf dst, double* k0d, double* ad, uint64_t off) {
+    const uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t i = list[j];
+    const uint64_t o = off + j;
+#pragma unroll
+    for (int c = 0; c < P_NDOUBLE; ++c) dst.d[c][o] = src.d[c][i];  // the whole record, as queued
+    dst.id[o] = src.id[i];
+    dst.parent_id[o] = src.parent_id[i];
+    dst.stream[o] = src.stream[i];
+    dst.shower[o] = src.shower[i];
+    dst.meta[o] = src.meta[i];
+    dst.nev[o] = src.nev[i];
+    k0d[off + j] = k0s[i];
+    ad[off + j] = as_[i];
+}
+void launch_gather_events(const PBuf& src, const double* k0s, const double* as_, const uint32_t* list, uint64_t n, const PBuf& dst,
+                          double* k0d, double* ad, uint64_t off, hipStream_t s) {
+    if (n) k_gather_events<<<unsigned((n + 255) / 256), 256, 0, s>>>(src, k0s, as_, list, n, dst, k0d, ad, off);
 }
 
 // Probes of the LMA device functions (parity tests).

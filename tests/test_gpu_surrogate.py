@@ -115,6 +115,7 @@ def test_surrogate_shower_matches_exact(engine, pol, a0):
         assert ts - te < 2e-3 * len(ae) + 5.0 * math.sqrt(te + ts + 1.0)
         assert p_a > 1e-4
     assert p_e > 1e-4
+    assert np.all(np.isfinite(ss["pol"][ps])) and np.all(np.isfinite(ss["u"][ps]))
     for col in (1, 2, 3):
         x, y = se["pol"][pe, col], ss["pol"][ps, col]
         se_ = math.sqrt(x.var() / len(x) + y.var() / len(y)) + 1e-12
@@ -137,3 +138,25 @@ def test_surrogate_requires_a_valid_model_and_quantum_lma(engine):
     engine.build_beam(beam, seed=0)
     with pytest.raises(PtgError, match="surrogate"):
         engine.run(opts, seed=0)
+
+
+def test_events_left_to_the_exact_sampler_are_exact(engine):
+    """A model whose domain holds none of the shower's events leaves every emission
+    to the exact kernels, through the deferred queue: the final state is the exact
+    run's, particle for particle (every event draws from its own counters)."""
+    from ptarmigan_amd import surrogate, workloads
+    model, _ = surrogate.load(surrogate.default_path(0))
+    model.a_min, model.a_max = 15.0, 20.0  # the a0 = 2 shower never reaches a = 15
+    engine.surrogate_load(model)
+    laser, beam, opts = workloads.plane_wave_lma(20_000)
+    se, ce = _shower(engine, laser, beam, opts, 41)
+    opts.surrogate = True
+    ss, cs = _shower(engine, laser, beam, opts, 41)
+    say("deferred: exact", ce["emissions"], "surrogate run", cs["emissions"], cs["surrogate_fallbacks"], cs["error_flags"])
+    assert cs["surrogate_events"] == 0 and cs["surrogate_fallbacks"] == ce["emissions"]
+    assert cs["error_flags"] == 0
+    for sp in (0, 2):
+        a = np.sort(se["u"][se["species"] == sp, 0])
+        b = np.sort(ss["u"][ss["species"] == sp, 0])
+        assert np.array_equal(a, b), sp
+    engine.surrogate_load(surrogate.load(surrogate.default_path(0))[0])

@@ -199,6 +199,9 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
                     help="mixed: FP32 rate/CDF tables and pusher algebra, FP64 phase, momentum update and "
                          "survival product (LMA workloads only)")
+    ap.add_argument("--surrogate", action="store_true",
+                    help="sample LMA emission with the shipped learned sampler (lma workload)")
+    ap.add_argument("--a0", type=float, default=0.0, help="override the laser's a0")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="run only a plain device copy of BYTES (5 dispatches), for counter calibration")
     args = ap.parse_args()
@@ -217,7 +220,13 @@ def main():
         args.primaries = 100_000  # the plane-wave case is quoted on 1e5 primaries
     laser, beam, opts = make_workload(args.workload, args.primaries)
     opts.precision = 1 if args.precision == "mixed" else 0
+    if args.a0 > 0.0:
+        laser.a0 = args.a0
     eng = Engine(0)
+    if args.surrogate:
+        from ptarmigan_amd import surrogate
+        eng.surrogate_load(surrogate.load(surrogate.default_path(int(laser.polarization)))[0])
+        opts.surrogate = True
     eng.set_laser(laser)
     eng.build_beam(beam, seed=0)
     try:
@@ -257,7 +266,7 @@ def main():
                 "kernel": kname, "kernel_avg_ms": round(lep_launch_ms, 3), "flops_per_step": flops}
 
     compare = None
-    if not args.no_compare and args.workload == "lcfa" and args.precision == "fp64":
+    if not args.no_compare and args.workload == "lcfa" and args.precision == "fp64" and not args.surrogate:
         compare = {"learned_sampler": surrogate_compare(eng, sync), "batched_scan": scan_compare(eng, sync)}
 
     cpu = None
@@ -273,7 +282,9 @@ def main():
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic",
-        "config": {"workload": WORKLOADS[args.workload], "primaries": args.primaries, "precision": args.precision},
+        "config": {"workload": WORKLOADS[args.workload] + (f", a0={args.a0:g}" if args.a0 > 0.0 else ""),
+                   "primaries": args.primaries, "precision": args.precision,
+                   "sampler": "learned" if args.surrogate else "exact"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "surrogate_fp32_vs_fp64": compare,

@@ -793,9 +793,10 @@ __global__ void k_surr_inputs(const TransportArgs* __restrict__ A, LmaArgs M, Su
     }
 }
 
-// Network outputs -> photons. One lane per row; LP Stokes parameters need a
-// J(n, x, y) workspace per lane (events whose harmonic outgrows it go to the
-// exact sampler's list).
+// Network outputs -> photons. One lane per row; LP Stokes parameters need one
+// J(n, x, y) evaluation per event, in a per-lane workspace (the checkpointed
+// recurrence of the emission kernel is slower here: this kernel is bound by the
+// serial recurrence of its largest harmonic, not by the workspace traffic).
 template <bool CIRC>
 __global__ __launch_bounds__(64) void k_surr_event(const TransportArgs* __restrict__ A, LmaArgs M, SurrArgs S) {
     const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;

@@ -147,6 +147,8 @@ struct ptg_ctx {
     // launch shape of the lepton step kernel (ptg_autotune)
     int step_block = 256, step_grid_per_cu = 4;
     uint32_t* d_surr_fb = nullptr;
+    uint32_t* d_surr_perm = nullptr;
+    unsigned int* d_surr_bucket = nullptr;  // [2][32]
     size_t surr_list_cap = 0;
     // emission events the learned sampler leaves to the exact kernels, held until
     // the lepton phase drains (one exact pass per phase instead of one per generation)
@@ -515,6 +517,8 @@ void ptg_destroy(ptg_ctx* c) {
     c->surr[1].release();
     if (c->d_surr_list) (void)hipFree(c->d_surr_list);
     if (c->d_surr_fb) (void)hipFree(c->d_surr_fb);
+    if (c->d_surr_perm) (void)hipFree(c->d_surr_perm);
+    if (c->d_surr_bucket) (void)hipFree(c->d_surr_bucket);
     free_buf(c->defq);
     if (c->d_def_extra) (void)hipFree(c->d_def_extra);
     if (c->d_lasers) (void)hipFree(c->d_lasers);
@@ -823,13 +827,16 @@ static int lma_surrogate_pass(ptg_ctx* c, LmaArgs m, size_t n_ev, size_t max_lan
     if (c->surr_list_cap < n_ev) {
         if (c->d_surr_list) (void)hipFree(c->d_surr_list);
         if (c->d_surr_fb) (void)hipFree(c->d_surr_fb);
-        c->d_surr_list = c->d_surr_fb = nullptr;
+        if (c->d_surr_perm) (void)hipFree(c->d_surr_perm);
+        c->d_surr_list = c->d_surr_fb = c->d_surr_perm = nullptr;
         c->surr_list_cap = 0;
         const size_t cap = n_ev + n_ev / 4 + 64;
         HIPCHK(c, hipMalloc(&c->d_surr_list, cap * sizeof(uint32_t)));
         HIPCHK(c, hipMalloc(&c->d_surr_fb, cap * sizeof(uint32_t)));
+        HIPCHK(c, hipMalloc(&c->d_surr_perm, cap * sizeof(uint32_t)));
         c->surr_list_cap = cap;
     }
+    if (!c->d_surr_bucket) HIPCHK(c, hipMalloc(&c->d_surr_bucket, 64 * sizeof(unsigned int)));
     const size_t rows_cap = (n_ev + 31) / 32 * 32;
     if (rows_cap > size_t(1) << 30) return fail(c, PTG_E_OOM, "too many emission events for the surrogate's buffers");
     HIPCHK(c, net.reserve(int(rows_cap)));
@@ -843,6 +850,8 @@ static int lma_surrogate_pass(ptg_ctx* c, LmaArgs m, size_t n_ev, size_t max_lan
     S.fb = c->d_surr_fb;
     S.n_fb = c->d_ctr + C_SFB;
     S.n_ev = n_ev;
+    S.perm = c->d_surr_perm;
+    S.bucket = c->d_surr_bucket;
     m.ev_in = c->evq.v;
     m.ev_list = nullptr;
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
@@ -856,6 +865,8 @@ static int lma_surrogate_pass(ptg_ctx* c, LmaArgs m, size_t n_ev, size_t max_lan
         const size_t M = (n_rows + 31) / 32 * 32;
         if (M > n_rows) HIPCHK(c, hipMemsetAsync(net.X + 8 * n_rows, 0, (M - n_rows) * 8 * sizeof(float), c->stream));
         net.forward(int(M), false, c->stream);
+        HIPCHK(c, hipMemsetAsync(c->d_surr_bucket, 0, 64 * sizeof(unsigned int), c->stream));
+        launch_surr_order(S, uint32_t(n_rows), c->stream);
         const size_t lanes = std::min<size_t>(max_lanes, (n_rows + 63) / 64 * 64);
         launch_surr_event(c->d_args, m, S, int(lanes), c->stream);
         HIPCHK(c, hipGetLastError());

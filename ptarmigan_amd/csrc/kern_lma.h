@@ -75,6 +75,8 @@ struct SurrArgs {
     uint32_t* fb;                    // events left to the exact sampler
     unsigned long long* n_fb;
     unsigned long long n_ev;         // events queued by the step kernel
+    uint32_t* perm;                  // rows ordered by harmonic bucket (largest first), or null
+    unsigned int* bucket;            // [2][32] bucket counts and cursors
     // training data
     uint64_t key;                    // Philox key of the training draws
     uint32_t n_groups;
@@ -89,6 +91,8 @@ struct SurrArgs {
 };
 void launch_surr_inputs(const TransportArgs* a, const LmaArgs& m, const SurrArgs& s, hipStream_t st);
 void launch_surr_event(const TransportArgs* a, const LmaArgs& m, const SurrArgs& s, int lanes, hipStream_t st);
+// orders the rows (n_rows on the host) by their harmonic's power-of-two bucket, largest first
+void launch_surr_order(const SurrArgs& s, uint32_t n_rows, hipStream_t st);
 void launch_surr_groups(const LmaArgs& m, const SurrArgs& s, int lanes, hipStream_t st);
 void launch_surr_batch(const LmaArgs& m, const SurrArgs& s, hipStream_t st);
 void launch_gather_events(const PBuf& src, const double* k0s, const double* as_, const uint32_t* list, uint64_t n, const PBuf& dst,

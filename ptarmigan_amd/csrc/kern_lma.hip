@@ -806,7 +806,8 @@ __global__ __launch_bounds__(64) void k_surr_event(const TransportArgs* __restri
     uint64_t n_emit = 0;
     uint32_t flags = 0;
     M.ev_in = M.ev;
-    for (uint64_t r = lane; r < n_rows; r += n_lanes) {
+    for (uint64_t rr = lane; rr < n_rows; rr += n_lanes) {
+        const uint64_t r = S.perm ? S.perm[rr] : rr;
         const uint32_t i = S.list[r];
         const PBuf& q = M.ev;
         const double ut = q.d[P_UT][i], ux = q.d[P_UX][i], uy = q.d[P_UY][i], uz = q.d[P_UZ][i];
@@ -1007,6 +1008,37 @@ __global__ void k_surr_batch(LmaArgs M, SurrArgs S) {
         t[0] = row[4]; t[1] = row[5]; t[2] = row[6]; t[3] = 1.0f; t[4] = CIRC ? 0.0f : 1.0f; t[5] = float(u3);
     }
     t[6] = 0.0f; t[7] = 0.0f;
+}
+
+// Rows ordered by the power-of-two bucket of their harmonic, largest first: a
+// lane's time is that of its J(n, x, y) recurrence, ~n, so a wave of lanes with
+// like harmonics does not wait for one large one, and the longest waves start first.
+__device__ __forceinline__ int surr_bucket(const float* o) {
+    const double nu = exp(5.0 * double(o[0]));
+    return nu <= 1.0 ? 0 : min(31, int(log2(nu)) + 1);
+}
+__global__ void k_surr_count(SurrArgs S, uint32_t n_rows) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n_rows) atomicAdd(S.bucket + surr_bucket(S.out + 32 * size_t(r)), 1u);
+}
+__global__ void k_surr_scan(SurrArgs S) {
+    if (threadIdx.x != 0) return;
+    unsigned int off = 0;
+    for (int b = 31; b >= 0; --b) {  // largest harmonics first
+        S.bucket[32 + b] = off;
+        off += S.bucket[b];
+    }
+}
+__global__ void k_surr_scatter(SurrArgs S, uint32_t n_rows) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n_rows) S.perm[atomicAdd(S.bucket + 32 + surr_bucket(S.out + 32 * size_t(r)), 1u)] = r;
+}
+void launch_surr_order(const SurrArgs& s, uint32_t n_rows, hipStream_t st) {
+    if (!n_rows) return;
+    const unsigned g = (n_rows + 255) / 256;
+    k_surr_count<<<g, 256, 0, st>>>(s, n_rows);
+    k_surr_scan<<<1, 64, 0, st>>>(s);
+    k_surr_scatter<<<g, 256, 0, st>>>(s, n_rows);
 }
 
 void launch_surr_inputs(const TransportArgs* a, const LmaArgs& m, const SurrArgs& s, hipStream_t st) {

@@ -1,0 +1,595 @@
+"""Ptarmigan input decks: the YAML surface of the reference driver.
+
+`Deck.from_file(path)` reads a deck with the reference's key semantics
+(src/main.rs:321-995, docs/input_guide/list_of_keys.md) and mathematical
+expressions in the style of the evalexpr crate with the reference's constants
+and functions (src/input/mod.rs:56-130): named values from the `constants`
+block (read once, in order), physical and conversion constants, and the
+reference's helper functions. `Deck.collisions()` gives one (Laser, Beam,
+CollideOptions) per value of a looped `laser:a0` (start/stop/step,
+src/input/mod.rs:239-263), and `run_deck` transports them — several a0 values
+as one batch of collisions.
+
+Not supported (they raise DeckError naming the key): beams loaded from HDF5
+(`beam:from_hdf5`), custom spectra (`beam:spectrum`), bremsstrahlung sources,
+`increase_pair_rate_by: auto`.
+"""
+import math
+import warnings
+import re
+
+import numpy as np
+import yaml
+
+from .engine import Beam, CollideOptions, Envelope, Laser, LaserKind, Polarization, Species
+
+# the reference's SI values (src/constants.rs:4-27); ELECTRON_CHARGE is negative
+SPEED_OF_LIGHT = 2.997925e8
+VACUUM_PERMITTIVITY = 8.854188e-12
+ELECTRON_CHARGE = -1.602177e-19
+ELEMENTARY_CHARGE = -ELECTRON_CHARGE
+ELECTRON_MASS = 9.109383e-31
+PROTON_MASS = 1.672622e-27
+COMPTON_TIME = 1.28808867e-21
+
+
+class DeckError(ValueError):
+    """A key that is missing, malformed or unsupported (the reference's InputError)."""
+
+
+# ---------------------------------------------------------------- expressions
+_CONSTANTS = {
+    "m": ELECTRON_MASS, "me": ELECTRON_MASS, "mp": PROTON_MASS, "c": SPEED_OF_LIGHT,
+    "e": ELECTRON_CHARGE, "qe": ELECTRON_CHARGE, "eV": ELEMENTARY_CHARGE,
+    "keV": 1.0e3 * ELEMENTARY_CHARGE, "MeV": 1.0e6 * ELEMENTARY_CHARGE, "GeV": 1.0e9 * ELEMENTARY_CHARGE,
+    "femto": 1.0e-15, "pico": 1.0e-12, "nano": 1.0e-9, "micro": 1.0e-6, "milli": 1.0e-3,
+    "pi": math.pi, "degree": math.pi / 180.0,
+}
+
+
+def _signum(x):
+    return math.copysign(1.0, x) if not math.isnan(x) else math.nan
+
+
+def _rust_round(x):
+    # f64::round: half away from zero
+    return math.copysign(math.floor(abs(x) + 0.5), x)
+
+
+_FUNCTIONS = {
+    "sqrt": math.sqrt, "cbrt": lambda x: math.copysign(abs(x) ** (1.0 / 3.0), x), "abs": abs, "exp": math.exp,
+    "ln": math.log, "sin": math.sin, "cos": math.cos, "tan": math.tan, "asin": math.asin, "acos": math.acos,
+    "atan": math.atan, "atan2": math.atan2, "sinh": math.sinh, "cosh": math.cosh, "tanh": math.tanh,
+    "asinh": math.asinh, "acosh": math.acosh, "atanh": math.atanh, "floor": math.floor, "ceil": math.ceil,
+    "round": _rust_round, "signum": _signum,
+    # the reference's approximations and helpers (src/input/mod.rs:103-106)
+    "erf": lambda x: math.tanh(167.0 * x / 148.0 + 11.0 * x ** 3 / 109.0),
+    "step": lambda x, lo, hi: 1.0 if lo <= x < hi else 0.0,
+    "gauss": lambda x, mu, sigma: math.exp(-(x - mu) ** 2 / (2.0 * sigma ** 2)),
+    "critical": lambda omega: VACUUM_PERMITTIVITY * ELECTRON_MASS * omega ** 2 / ELEMENTARY_CHARGE ** 2,
+}
+
+_TOKEN = re.compile(r"\s*(?:(\d+\.\d*(?:[eE][+-]?\d+)?|\.\d+(?:[eE][+-]?\d+)?|\d+[eE][+-]?\d+)|(\d+)|"
+                    r"([A-Za-z_][A-Za-z_0-9]*)|(\*\*|[-+*/%^(),]))")
+
+
+def _tokens(s):
+    pos, out = 0, []
+    s = s.strip()
+    while pos < len(s):
+        m = _TOKEN.match(s, pos)
+        if not m or m.end() == pos:
+            raise DeckError(f"cannot parse expression {s!r} at {s[pos:]!r}")
+        f, i, name, op = m.groups()
+        if f is not None:
+            out.append(("num", float(f)))
+        elif i is not None:
+            out.append(("num", int(i)))
+        elif name is not None:
+            out.append(("name", name))
+        else:
+            out.append(("op", op))
+        pos = m.end()
+    return out
+
+
+def _arith(op, a, b):
+    both_int = isinstance(a, int) and isinstance(b, int)
+    if op == "+":
+        return a + b
+    if op == "-":
+        return a - b
+    if op == "*":
+        return a * b
+    if op == "/":
+        if both_int:  # evalexpr keeps integers: division truncates toward zero
+            if b == 0:
+                raise DeckError("integer division by zero")
+            q = abs(a) // abs(b)
+            return q if (a >= 0) == (b >= 0) else -q
+        with np.errstate(all="ignore"):  # IEEE results (inf, NaN), as f64 gives
+            return float(np.float64(a) / np.float64(b))
+    if op == "%":
+        if both_int:
+            if b == 0:
+                raise DeckError("integer remainder by zero")
+            return int(math.fmod(a, b))
+        with np.errstate(all="ignore"):
+            return float(np.fmod(np.float64(a), np.float64(b)))
+    if op == "^":
+        with np.errstate(all="ignore"):
+            return float(np.power(np.float64(a), np.float64(b)))
+    raise DeckError(f"unknown operator {op}")
+
+
+def evaluate(expr, context=None):
+    """Value of an evalexpr-style expression: + - * / % (integers stay integers),
+    ^ (power, right-associative, binding tighter than unary minus), functions
+    f(a[, b, ...]), and the named values of `context` and the reference's constants."""
+    ctx = dict(_CONSTANTS)
+    if context:
+        ctx.update(context)
+    toks = _tokens(str(expr))
+    pos = [0]
+
+    def peek():
+        return toks[pos[0]] if pos[0] < len(toks) else (None, None)
+
+    def take(kind=None, val=None):
+        t = peek()
+        if t[0] is None or (kind and t[0] != kind) or (val and t[1] != val):
+            raise DeckError(f"malformed expression {expr!r}")
+        pos[0] += 1
+        return t
+
+    def atom():
+        k, v = peek()
+        if k == "num":
+            take()
+            return v
+        if k == "name":
+            take()
+            if peek() == ("op", "("):
+                take("op", "(")
+                args = [] if peek() == ("op", ")") else [additive()]
+                while peek() == ("op", ","):
+                    take()
+                    args.append(additive())
+                take("op", ")")
+                if v not in _FUNCTIONS:
+                    raise DeckError(f"unknown function {v!r} in {expr!r}")
+                try:
+                    return float(_FUNCTIONS[v](*[float(x) for x in args]))
+                except TypeError as err:
+                    raise DeckError(f"{v}{tuple(args)}: {err}")
+                except ValueError:  # outside the domain: NaN, as f64 gives
+                    return math.nan
+                except OverflowError:
+                    return math.inf
+            if v not in ctx:
+                raise DeckError(f"unknown name {v!r} in {expr!r}")
+            return ctx[v]
+        if (k, v) == ("op", "("):
+            take()
+            x = additive()
+            take("op", ")")
+            return x
+        raise DeckError(f"malformed expression {expr!r}")
+
+    def power():
+        base = atom()
+        if peek() in (("op", "^"), ("op", "**")):
+            take()
+            return _arith("^", base, unary())  # right-associative
+        return base
+
+    def unary():
+        if peek() == ("op", "-"):
+            take()
+            x = unary()
+            return -x
+        if peek() == ("op", "+"):
+            take()
+            return unary()
+        return power()
+
+    def multiplicative():
+        x = unary()
+        while peek() in (("op", "*"), ("op", "/"), ("op", "%")):
+            op = take()[1]
+            x = _arith(op, x, unary())
+        return x
+
+    def additive():
+        x = multiplicative()
+        while peek() in (("op", "+"), ("op", "-")):
+            op = take()[1]
+            x = _arith(op, x, multiplicative())
+        return x
+
+    value = additive()
+    if pos[0] != len(toks):
+        raise DeckError(f"trailing input in expression {expr!r}")
+    return value
+
+
+# ---------------------------------------------------------------- the deck
+class _Missing(Exception):
+    pass
+
+
+class Deck:
+    """One YAML input deck (the first YAML document of the file).
+
+    Reads follow the reference's conversions (src/input/types.rs:18-133): a
+    number is a YAML integer, a YAML float or a string holding an expression; a
+    count (usize) is a YAML integer only; a flag is a YAML boolean only; a list
+    of numbers is a scalar or a sequence of them."""
+
+    def __init__(self, data, name="deck"):
+        if not isinstance(data, dict):
+            raise DeckError("the deck must be a YAML mapping")
+        self.data = data
+        self.name = name
+        self.context = {}
+        consts = data.get("constants")
+        if isinstance(consts, dict):
+            # in order, so no forward definitions; entries that are neither numbers
+            # nor strings are passed over (src/input/mod.rs:111-135)
+            for k, v in consts.items():
+                if isinstance(v, bool) or not isinstance(v, (int, float, str)):
+                    continue
+                try:
+                    val = float(v) if not isinstance(v, str) else float(evaluate(v, self.context))
+                except DeckError as err:
+                    raise DeckError(f"constants:{k}: {err}")
+                self.context[str(k)] = val
+
+    @classmethod
+    def from_string(cls, text, name="deck"):
+        try:
+            docs = list(yaml.safe_load_all(text))
+        except yaml.YAMLError as err:
+            raise DeckError(f"not a YAML file: {err}")
+        if not docs or docs[0] is None:
+            raise DeckError("YAML file is empty")
+        return cls(docs[0], name)
+
+    @classmethod
+    def from_file(cls, path):
+        import os
+        with open(path) as f:
+            return cls.from_string(f.read(), os.path.splitext(os.path.basename(path))[0])
+
+    # -- reads: a missing key raises _Missing, a malformed one DeckError
+    def _node(self, path):
+        node = self.data
+        for part in path.split(":"):
+            if not isinstance(node, dict) or part not in node or node[part] is None:
+                raise _Missing(path)
+            node = node[part]
+        return node
+
+    def contains(self, path):
+        try:
+            self._node(path)
+            return True
+        except _Missing:
+            return False
+
+    def _eval(self, path, s):
+        try:
+            return float(evaluate(s, self.context))
+        except (DeckError, ZeroDivisionError, OverflowError) as err:
+            raise DeckError(f"{path}: {err}")
+
+    def _number(self, path, v):
+        if isinstance(v, bool):
+            raise DeckError(f"{path} must be numerical")
+        if isinstance(v, (int, float)):
+            return float(v)
+        if isinstance(v, str):
+            return self._eval(path, v)
+        raise DeckError(f"{path} must be numerical")
+
+    def number(self, path, default=_Missing):
+        try:
+            v = self._node(path)
+        except _Missing:
+            if default is _Missing:
+                raise DeckError(f"{path} is required")
+            return default
+        return self._number(path, v)
+
+    def count(self, path, default=_Missing):
+        try:
+            v = self._node(path)
+        except _Missing:
+            if default is _Missing:
+                raise DeckError(f"{path} is required")
+            return default
+        if isinstance(v, bool) or not isinstance(v, int) or v < 0:
+            raise DeckError(f"{path} must be a non-negative integer")
+        return v
+
+    def flag(self, path, default):
+        """unwrap_or(default): a missing or non-boolean value gives the default."""
+        try:
+            v = self._node(path)
+        except _Missing:
+            return default
+        return v if isinstance(v, bool) else default
+
+    def string(self, path, default=_Missing):
+        try:
+            v = self._node(path)
+        except _Missing:
+            if default is _Missing:
+                raise DeckError(f"{path} is required")
+            return default
+        if isinstance(v, bool):
+            return "true" if v else "false"
+        if isinstance(v, (int, float, str)):
+            return str(v)
+        raise DeckError(f"{path} must be a string")
+
+    def numbers(self, path, default=_Missing):
+        try:
+            v = self._node(path)
+        except _Missing:
+            if default is _Missing:
+                raise DeckError(f"{path} is required")
+            return default
+        vs = v if isinstance(v, list) else [v]
+        return [self._number(path, x) for x in vs]
+
+    def func(self, path, var):
+        """The expression at `path` as a function of one variable `var` (the
+        reference's Config::func); fails if it names anything else unknown."""
+        try:
+            expr = self.string(path)
+        except DeckError:
+            raise
+        ctx = dict(self.context)
+
+        def f(x):
+            ctx[var] = float(x)
+            return float(evaluate(expr, ctx))
+        f(1.0)  # unknown names raise here
+        return f
+
+    def number_loop(self, path):
+        """start, start + step, ... while <= stop, or the single value (src/input/mod.rs:239-263)."""
+        if not self.contains(f"{path}:start"):
+            return [self.number(path)]
+        start, stop, step = (self.number(f"{path}:{k}") for k in ("start", "stop", "step"))
+        if not step > 0.0 and start <= stop:
+            raise DeckError(f"{path}:step must be positive")
+        out, x = [], start
+        while x <= stop:
+            out.append(x)
+            x = x + step
+        return out
+
+    # -- the driver's reading of control, laser, beam and output
+    def options(self):
+        """CollideOptions, the seed, whether the LCFA is used and the stop mode
+        (src/main.rs:339-397, 855-880)."""
+        lcfa = self.flag("control:lcfa", False)
+        if self.contains("control:classical"):
+            s = self.string("control:classical")
+            if s not in ("true", "false", "gaunt_factor_corrected"):
+                raise DeckError("control:classical must be one of 'true', 'false' or 'gaunt_factor_corrected'")
+            classical, gaunt = s != "false", s == "gaunt_factor_corrected"
+            if gaunt and not lcfa:
+                raise DeckError("Gaunt factor correction is only available under the LCFA")
+        else:
+            classical = gaunt = False
+        mult = self.count("control:select_multiplicity", None) if self.contains("control:select_multiplicity") else None
+        if self.contains("control:increase_pair_rate_by"):
+            if self._node("control:increase_pair_rate_by") == "auto":
+                raise DeckError("control:increase_pair_rate_by: auto is not supported by this build")
+            rate = self.number("control:increase_pair_rate_by")
+        else:
+            rate = 1.0
+        if rate < 1.0:
+            raise DeckError("increase in pair creation rate must be >= 1.0")
+        if rate > 1.0 and mult is not None:
+            raise DeckError("event biasing (pair rate increase > 1) is incompatible with multiplicity selection")
+        stop = ("automatic", math.inf)
+        if self.contains("control:stop_at_time"):
+            s = self.string("control:stop_at_time")
+            stop = ("same_time", math.inf) if s == "auto" else ("given_time", self._eval("control:stop_at_time", s))
+        if self.contains("output:discard_background") and isinstance(self._node("output:discard_background"), bool):
+            disc_e = disc_ph = self._node("output:discard_background")
+        else:
+            disc_e, disc_ph = self.flag("output:discard_background_e", False), False
+        if self.contains("output:coordinate_system") and self.string("output:coordinate_system") == "beam":
+            raise DeckError("output:coordinate_system: beam is not supported by this build")
+        opts = CollideOptions(
+            dt_multiplier=self.number("control:dt_multiplier", 1.0), rate_increase=rate,
+            t_stop=stop[1] if stop[0] == "given_time" else math.inf,
+            min_energy=1.0e-6 * self.number("output:min_energy", 0.0) / ELEMENTARY_CHARGE,  # J -> MeV
+            max_angle=self.number("output:max_angle", math.pi), multiplicity=mult,
+            rr=self.flag("control:radiation_reaction", True),
+            tracking_photons=self.flag("control:pair_creation", not classical),
+            tracking_secondaries=self.flag("control:track_secondaries", True),
+            keep_decayed_photons=self.flag("output:dump_decayed_photons", False),
+            pol_resolved=self.flag("control:pol_resolved", False), classical=classical, gaunt_factor=gaunt,
+            discard_bg_e=disc_e, discard_bg_ph=disc_ph)
+        seed = self.count("control:rng_seed", 0)
+        return opts, seed, lcfa, stop
+
+    def lasers(self, lcfa):
+        """One Laser per a0 value (src/main.rs:399-470, 1042-1060)."""
+        a0s = self.number_loop("laser:a0")
+        if self.contains("laser:wavelength"):
+            wavelength = self.number("laser:wavelength")
+        else:
+            omega = self.number("laser:omega")
+            wavelength = 2.0 * math.pi * COMPTON_TIME * ELECTRON_MASS * SPEED_OF_LIGHT ** 3 / omega
+        pol_s = self.string("laser:polarization")
+        if pol_s == "circular":
+            pol, pol_angle = Polarization.CIRCULAR, 0.0
+        elif pol_s in ("linear", "linear || x"):
+            pol, pol_angle = Polarization.LINEAR, 0.0
+        elif pol_s == "linear || y":
+            pol, pol_angle = Polarization.LINEAR, math.pi / 2
+        elif pol_s.startswith("linear @"):
+            pol, pol_angle = Polarization.LINEAR, self._eval("laser:polarization", pol_s[len("linear @"):])
+        else:
+            raise DeckError("laser polarization must be 'linear [|| x or y]' or 'circular'")
+        try:
+            waist = self.number("laser:waist")
+            focusing = True
+        except DeckError:
+            focusing, waist = False, math.inf
+        envs = {"cos2": Envelope.COS_SQUARED, "cos^2": Envelope.COS_SQUARED, "cos_sqd": Envelope.COS_SQUARED,
+                "cos_squared": Envelope.COS_SQUARED, "flattop": Envelope.FLATTOP, "flat-top": Envelope.FLATTOP,
+                "gauss": Envelope.GAUSSIAN, "gaussian": Envelope.GAUSSIAN}
+        env_s = self.string("laser:envelope", None)
+        if env_s in envs:
+            envelope = envs[env_s]
+        else:
+            if env_s is not None:  # reported by the reference, which then takes the default
+                warnings.warn("laser envelope must be one of 'cos^2', 'flattop' or 'gaussian'")
+            envelope = Envelope.GAUSSIAN if focusing else Envelope.COS_SQUARED
+        n_cycles = None
+        if envelope == Envelope.GAUSSIAN and self.contains("laser:fwhm_duration"):
+            try:
+                n_cycles = SPEED_OF_LIGHT * self.number("laser:fwhm_duration") / wavelength
+            except DeckError:
+                n_cycles = None
+        if n_cycles is None:
+            n_cycles = self.number("laser:n_cycles")
+        if envelope == Envelope.FLATTOP and n_cycles < 1.0:
+            raise DeckError("'n_cycles' must be >= 1.0 for flattop lasers")
+        if focusing and self.contains("laser:chirp_coeff"):
+            warnings.warn("chirp parameter ignored for focusing laser pulses")
+        chirp = 0.0 if focusing else self.number("laser:chirp_coeff", 0.0)
+        bw = self.flag("control:bandwidth_correction", False)
+        return [Laser.select(a0, wavelength, n_cycles, waist=waist if focusing else None, lcfa=lcfa,
+                             envelope=envelope, polarization=pol, pol_angle=pol_angle, chirp_b=chirp,
+                             finite_bandwidth=bw) for a0 in a0s]
+
+    def beam(self):
+        """The generated beam (src/main.rs:480-800)."""
+        for unsupported in ("beam:from_hdf5", "beam:spectrum"):
+            if self.contains(unsupported):
+                raise DeckError(f"{unsupported} is not supported by this build")
+        species = Species.ELECTRON
+        if self.contains("beam:species"):
+            s = self.string("beam:species")
+            names = {"electron": Species.ELECTRON, "positron": Species.POSITRON, "photon": Species.PHOTON}
+            if s not in names:
+                raise DeckError(f"requested species '{s}' is not one of 'electron', 'positron' or 'photon'")
+            species = names[s]
+        if species == Species.PHOTON and self.flag("beam:bremsstrahlung_source", False):
+            raise DeckError("beam:bremsstrahlung_source is not supported by this build")
+        angle = self.number("beam:collision_angle", 0.0)
+        plane = 0.0
+        if self.contains("beam:collision_plane"):
+            s = self.string("beam:collision_plane")
+            plane = {"horizontal": 0.0, "vertical": math.pi / 2}.get(s)
+            if plane is None:
+                plane = self._eval("beam:collision_plane", s)
+        off = self.numbers("beam:offset", [0.0, 0.0, 0.0])
+        if len(off) != 3:
+            raise DeckError("a collision offset must be expressed as a three-vector [dx, dy, dz]")
+        n = self.count("beam:n") if self.contains("beam:n") else self.count("beam:ne")
+        w_charge = abs(self.number("beam:charge")) / (ELEMENTARY_CHARGE * n) if self.contains("beam:charge") else None
+        w_real = self.number("beam:n_real") / n if self.contains("beam:n_real") else None
+        if w_charge is not None and w_real is not None and w_charge != w_real:
+            raise DeckError("conflicting values of 'n_real' and 'charge'. Specify one or the other.")
+        weight = w_charge if w_charge is not None else (w_real if w_real is not None else 1.0)
+        radius, radial, max_radius = 0.0, 0, 0.0
+        if self.contains("beam:radius"):
+            r = self._node("beam:radius")
+            vs = [self.string_of(x) for x in (r if isinstance(r, list) else [r])]
+            dstr = vs[1] if len(vs) > 1 else "normally_distributed"
+            try:
+                radius = self._eval("beam:radius", vs[0])
+            except DeckError:
+                radius = None
+            if radius is None or dstr not in ("normally_distributed", "uniformly_distributed"):
+                raise DeckError("beam radius must be a single numerical value or [value, normally_distributed or "
+                                "uniformly_distributed (, cutoff)]")
+            cut = None
+            if len(vs) > 2:
+                try:
+                    cut = self._eval("beam:radius", vs[2])
+                except DeckError:
+                    cut = None
+            if dstr == "uniformly_distributed":
+                radial = 2
+            elif cut is not None:
+                radial, max_radius = 1, cut
+        chirp = self.number("beam:energy_chirp", 0.0)
+        if abs(chirp) > 1.0:
+            raise DeckError(f"absolute value of energy chirp parameter {chirp} must be <= 1")
+        sv = self.numbers("beam:stokes_pars", [0.0, 0.0, 0.0])
+        if len(sv) != 3:
+            raise DeckError("particle polarization must be specified as three Stokes parameters [S_1, S_2, S_3]")
+        if math.sqrt(sv[0] ** 2 + sv[1] ** 2 + sv[2] ** 2) > 1.0:
+            raise DeckError("specified particle polarization does not satisfy S_1^2 + S_2^2 + S_3^2 <= 1")
+        return Beam(n=n, gamma=self.number("beam:gamma"), sigma=self.number("beam:sigma", 0.0), species=species,
+                    radius=radius, radial=radial, max_radius=max_radius, length=self.number("beam:length", 0.0),
+                    rms_div=self.number("beam:rms_divergence", 0.0), energy_chirp=chirp, angle=angle,
+                    collision_plane_angle=plane, offset=tuple(off), pol=(1.0, sv[0], sv[1], sv[2]), weight=weight)
+
+    @staticmethod
+    def string_of(x):
+        if isinstance(x, bool):
+            return "true" if x else "false"
+        return str(x)
+
+    def collisions(self):
+        """[(Laser, Beam, CollideOptions)], one per value of laser:a0, the seed and the stop mode."""
+        opts, seed, lcfa, stop = self.options()
+        beam = self.beam()
+        return [(laser, beam, opts) for laser in self.lasers(lcfa)], seed, stop
+
+
+def drift_to_common_time(particles, stop):
+    """The driver's final time fix (src/main.rs:1150-1188): with stop_at_time: auto
+    every particle drifts along its momentum to the largest ct of the bunch; with a
+    given time, to c t. Returns the common time in s (inf when positions stay)."""
+    mode, t = stop
+    r, u = particles["r"], particles["u"]
+    if mode == "automatic" or len(r) == 0:
+        return math.inf
+    ct = r[:, 0].max() if mode == "same_time" else t * SPEED_OF_LIGHT
+    r += u * ((ct - r[:, 0]) / u[:, 0])[:, None]
+    return ct / SPEED_OF_LIGHT
+
+
+def run_deck(deck, engine=None):
+    """Transports every collision of the deck on the GPU and returns
+    [(a0, final-state particles, counters)]. Several a0 values of an LMA field
+    run as one batch of collisions (one primary set per a0, as the reference's
+    loop builds one per run, src/main.rs:998-1070)."""
+    import numpy as np
+    from .engine import Engine
+    cols, seed, stop = deck.collisions()
+    eng = engine or Engine(0)
+    out = []
+    lma = cols[0][0].kind in (LaserKind.PLANE_WAVE, LaserKind.FOCUSED_LASER)
+    if len(cols) > 1 and lma:
+        eng.set_batch([c[0] for c in cols], [c[1] for c in cols], seed=seed)
+        cnt = eng.run(cols[0][2], seed=seed)
+        sec, _ = eng.fetch(cnt)
+        starts = np.cumsum([0] + [c[1].n for c in cols])
+        coll = np.searchsorted(starts, sec["shower"], side="right") - 1
+        for k, (laser, _, _) in enumerate(cols):
+            part = {key: v[coll == k] for key, v in sec.items()}
+            drift_to_common_time(part, stop)
+            out.append((laser.a0, part, cnt))
+        return out
+    for laser, beam, opts in cols:
+        eng.set_laser(laser)
+        eng.build_beam(beam, seed=seed)
+        cnt = eng.run(opts, seed=seed)
+        sec, _ = eng.fetch(cnt)
+        drift_to_common_time(sec, stop)
+        out.append((laser.a0, sec, cnt))
+    return out

@@ -1,0 +1,288 @@
+"""Input decks: expressions, the constants block and the driver's key semantics.
+
+The parser cases restate the reference's own tests of its input reader
+(src/input/mod.rs:310-410: config_parser and looper). The rest checks each
+mapped parameter against what the driver computes from the same keys
+(src/main.rs:339-880), with values fixed by closed forms: 2 pi hbar c / E for a
+wavelength given as a photon energy, c t / lambda for a duration, charge / (e n)
+for a weight, J -> MeV for min_energy.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from ptarmigan_amd import deck as D
+from ptarmigan_amd.engine import Envelope, LaserKind, Polarization, Species
+
+CONFIG = """---
+control:
+  dx: 0.001
+  nx: 4000
+  ne: sin(a * x)
+  ib: a * b^3
+
+extra:
+  dx: 160
+  r: [0.0, b, 1.0, 2.0 * a]
+  s: [0.0, none]
+
+constants:
+  N: 2.0
+  a: N * pi
+  b: 17.0
+  y: 1.0
+
+deep:
+  nested:
+    section:
+      key: 1.0
+"""
+
+
+def test_reference_config_parser_cases():
+    c = D.Deck.from_string(CONFIG)
+    assert c.number("control:dx") == 0.001
+    assert c.count("control:nx") == 4000
+    assert c.number("control:ib") == 2.0 * math.pi * 17.0 ** 3
+    assert c.number("extra:dx") == 160.0
+    ne = c.func("control:ne", "x")
+    assert ne(0.6) == math.sin(2.0 * math.pi * 0.6)
+    with pytest.raises(D.DeckError):
+        c.func("control:ne", "y")
+    assert c.numbers("extra:r") == [0.0, 17.0, 1.0, 4.0 * math.pi]
+    with pytest.raises(D.DeckError):
+        c.numbers("extra:s")
+    assert c.number("deep:nested:section:key") == 1.0
+    assert D.evaluate("1.0 / (1.0 + y)", c.context) == 0.5
+    assert c.contains("extra") and c.contains("deep:nested") and not c.contains("control:absent")
+
+
+def test_reference_looper_cases():
+    c = D.Deck.from_string("---\nlaser:\n    a0: 10.0\n")
+    assert c.number_loop("laser:a0") == [10.0]
+    c = D.Deck.from_string("---\nlaser:\n    a0:\n        start: 1.0\n        stop: 10.0\n        step: 2.0\n")
+    assert c.number_loop("laser:a0") == [1.0, 3.0, 5.0, 7.0, 9.0]
+    # accumulation as the reference does it: 1.0 + 0.1 + ... passes 1.5 after five steps
+    c = D.Deck.from_string("---\nx:\n  start: 1.0\n  stop: 1.5\n  step: 0.1\n")
+    assert len(c.number_loop("x")) == 5
+
+
+@pytest.mark.parametrize("expr,value", [
+    ("1 / 2", 0.0),                  # integers stay integers
+    ("1.0 / 2", 0.5),
+    ("7 % 3", 1.0),
+    ("-7 / 2", -3.0),                # truncation toward zero
+    ("2^3^2", 512.0),                # right-associative power
+    ("-2^2", -4.0),                  # power binds tighter than negation
+    ("2 * 3 + 4 * 5", 26.0),
+    ("(2 + 3) * 4", 20.0),
+    ("1.0e-6 + 2e3", 2000.000001),
+    ("atan2(1.0, 1.0) / degree", 45.0),
+    ("round(2.5) + round(-2.5)", 0.0),  # half away from zero
+    ("signum(-3.0) + abs(-2.0)", 1.0),
+    ("step(0.5, 0.0, 1.0) + step(1.0, 0.0, 1.0)", 1.0),  # [min, max)
+    ("gauss(1.0, 1.0, 2.0)", 1.0),
+    ("cbrt(-27.0)", -3.0),
+    ("ln(exp(2.5))", 2.5),
+    ("floor(-1.5) + ceil(1.2)", 0.0),
+])
+def test_expressions(expr, value):
+    assert D.evaluate(expr) == pytest.approx(value, rel=1e-15, abs=1e-15)
+
+
+def test_reference_constants_and_functions():
+    ctx = {}
+    # the reference's rounded SI values; e and qe are the electron's (negative) charge
+    assert D.evaluate("e", ctx) == -1.602177e-19 and D.evaluate("qe", ctx) == -1.602177e-19
+    assert D.evaluate("MeV") == 1.602177e-13 and D.evaluate("GeV / keV") == pytest.approx(1e6, rel=1e-15)
+    assert D.evaluate("me * c^2 / MeV") == pytest.approx(0.5109989, rel=2e-6)
+    assert D.evaluate("mp / me") == pytest.approx(1836.15, rel=1e-5)
+    # erf is the reference's tanh approximation, good to 4e-4 against the true erf
+    for x in np.linspace(-3, 3, 61):
+        assert abs(D.evaluate(f"erf({float(x)!r})") - math.erf(x)) < 4e-4
+    # critical density of a 0.8 um laser: eps0 m omega^2 / e^2 = 1.742e27 m^-3
+    assert D.evaluate("critical(2.0 * pi * c / (0.8 * micro))") == pytest.approx(1.742e27, rel=1e-3)
+    assert math.isnan(D.evaluate("sqrt(-1.0)")) and D.evaluate("1.0 / 0") == math.inf
+    for bad in ("unknown_name", "sin(", "2 +", "foo(1.0)", "1.0 2.0", "sqrt(1.0, 2.0)"):
+        with pytest.raises(D.DeckError):
+            D.evaluate(bad)
+
+
+def test_constants_block_in_order():
+    c = D.Deck.from_string("constants:\n  a0: 15.0\n  gamma0: 1000.0 / 0.511\n  max_angle: atan(a0 / gamma0)\n"
+                           "  flag: true\n")
+    assert c.context["gamma0"] == pytest.approx(1956.95, abs=5e-3)  # the documented values
+    assert c.context["max_angle"] == pytest.approx(7.6649e-3, rel=1e-4)
+    assert "flag" not in c.context  # entries that are not numbers or strings are passed over
+    with pytest.raises(D.DeckError):
+        D.Deck.from_string("constants:\n  b: 2.0 * a\n  a: 1.0\n")  # no forward definitions
+    with pytest.raises(D.DeckError):
+        D.Deck.from_string("")
+
+
+DECK = """---
+control:
+  radiation_reaction: true
+  pair_creation: true
+  rng_seed: 7
+  dt_multiplier: 0.5
+  increase_pair_rate_by: 10.0
+  stop_at_time: 100.0 * femto
+
+laser:
+  a0:
+    start: 1.0
+    stop: 2.0
+    step: 0.5
+  omega: 1.55 * eV
+  polarization: linear @ 30.0 * degree
+  n_cycles: 16.0
+
+beam:
+  n: 1000
+  species: positron
+  charge: 1.0 * nano
+  gamma: energy / (me * c^2)
+  sigma: 0.01 * energy / (me * c^2)
+  radius: [2.0 * micro, normally_distributed, 6.0 * micro]
+  length: 10.0 * micro
+  collision_angle: 5.0 * degree
+  collision_plane: vertical
+  rms_divergence: 50.0 * micro
+  offset: [1.0 * micro, 0.0, -2.0 * micro]
+  energy_chirp: 0.2
+  stokes_pars: [0.0, 0.0, 1.0]
+
+output:
+  min_energy: 100.0 * MeV
+  max_angle: 10.0 * milli
+  discard_background: true
+
+constants:
+  energy: 10.0 * GeV
+"""
+
+
+def test_driver_key_semantics():
+    d = D.Deck.from_string(DECK)
+    cols, seed, stop = d.collisions()
+    assert seed == 7 and stop == ("given_time", pytest.approx(1e-13))
+    assert [c[0].a0 for c in cols] == [1.0, 1.5, 2.0]
+    laser, beam, opts = cols[0]
+    assert laser.kind == LaserKind.PLANE_WAVE  # no waist, LCFA off
+    assert laser.wavelength == pytest.approx(1239.84198e-9 / 1.55, rel=2e-6)  # 2 pi hbar c / E
+    assert laser.polarization == Polarization.LINEAR and laser.pol_angle == pytest.approx(math.pi / 6)
+    assert laser.envelope == Envelope.COS_SQUARED and laser.n_cycles == 16.0
+    gamma = 10.0e9 * 1.602177e-19 / (9.109383e-31 * 2.997925e8 ** 2)
+    assert beam.species == Species.POSITRON and beam.n == 1000
+    assert beam.gamma == pytest.approx(gamma, rel=1e-14) and beam.sigma == pytest.approx(0.01 * gamma, rel=1e-14)
+    assert beam.weight == pytest.approx(1e-9 / (1.602177e-19 * 1000), rel=1e-14)
+    assert (beam.radius, beam.radial, beam.max_radius) == (pytest.approx(2e-6), 1, pytest.approx(6e-6))
+    assert beam.length == pytest.approx(1e-5) and beam.rms_div == pytest.approx(5e-5)
+    assert beam.angle == pytest.approx(math.radians(5.0)) and beam.collision_plane_angle == pytest.approx(math.pi / 2)
+    assert beam.offset == (pytest.approx(1e-6), 0.0, pytest.approx(-2e-6))
+    assert beam.energy_chirp == 0.2 and beam.pol == (1.0, 0.0, 0.0, 1.0)
+    assert opts.dt_multiplier == 0.5 and opts.rate_increase == 10.0 and opts.t_stop == pytest.approx(1e-13)
+    assert opts.min_energy == pytest.approx(100.0) and opts.max_angle == pytest.approx(0.01)
+    assert opts.discard_bg_e and opts.discard_bg_ph and opts.tracking_photons and not opts.classical
+
+
+def _with(text, **edits):
+    d = D.Deck.from_string(text)
+    for path, v in edits.items():
+        node = d.data
+        parts = path.split("__")
+        for p in parts[:-1]:
+            node = node.setdefault(p, {})
+        if v is None:
+            node.pop(parts[-1], None)
+        else:
+            node[parts[-1]] = v
+    return D.Deck(d.data)
+
+
+def test_driver_defaults_and_alternatives():
+    d = _with(DECK, laser__waist="4.0 * micro", laser__fwhm_duration="30.0 * femto", laser__polarization="circular",
+              control__lcfa=True, beam__charge=None, beam__n_real=5.0e9, output__discard_background=None,
+              output__discard_background_e=True, control__stop_at_time="auto", control__classical="true",
+              control__pair_creation=None, beam__radius="1.0 * micro", beam__collision_plane="30.0 * degree")
+    cols, _, stop = d.collisions()
+    laser, beam, opts = cols[0]
+    assert laser.kind == LaserKind.FAST_FOCUSED_LASER and laser.waist == pytest.approx(4e-6)
+    assert laser.envelope == Envelope.GAUSSIAN  # the default of a focused laser
+    assert laser.n_cycles == pytest.approx(2.997925e8 * 30e-15 / laser.wavelength, rel=1e-14)
+    assert laser.polarization == Polarization.CIRCULAR and laser.chirp_b == 0.0
+    assert beam.weight == pytest.approx(5.0e6) and (beam.radial, beam.radius) == (0, pytest.approx(1e-6))
+    assert beam.collision_plane_angle == pytest.approx(math.pi / 6)
+    assert opts.discard_bg_e and not opts.discard_bg_ph
+    assert stop[0] == "same_time" and opts.t_stop == math.inf
+    assert opts.classical and not opts.tracking_photons  # pair creation defaults to off when classical
+    d = _with(DECK, laser__envelope="flattop", laser__n_cycles=0.5)
+    with pytest.raises(D.DeckError, match="flattop"):
+        d.collisions()
+    d = _with(DECK, laser__envelope="sawtooth")  # reported, then the default
+    with pytest.warns(UserWarning):
+        assert d.collisions()[0][0][0].envelope == Envelope.COS_SQUARED
+
+
+@pytest.mark.parametrize("edit,match", [
+    ({"beam__n_real": 1.0}, "conflicting"),
+    ({"beam__stokes_pars": [1.0, 1.0, 0.0]}, "S_1"),
+    ({"beam__stokes_pars": [1.0, 0.0]}, "three Stokes"),
+    ({"beam__offset": [1.0, 0.0]}, "three-vector"),
+    ({"beam__energy_chirp": 1.5}, "chirp"),
+    ({"beam__species": "muon"}, "species"),
+    ({"beam__radius": ["1.0 * micro", "gaussian"]}, "radius"),
+    ({"beam__n": 1.0e3}, "beam:n"),
+    ({"laser__polarization": "elliptical"}, "polarization"),
+    ({"control__classical": "maybe"}, "classical"),
+    ({"control__classical": "gaunt_factor_corrected"}, "LCFA"),
+    ({"control__increase_pair_rate_by": 0.5}, ">= 1.0"),
+    ({"control__select_multiplicity": 2}, "incompatible"),
+    ({"control__stop_at_time": "soon"}, "stop_at_time"),
+    ({"beam__from_hdf5": {"file": "x.h5"}}, "not supported"),
+    ({"output__coordinate_system": "beam"}, "not supported"),
+    ({"beam__gamma": None}, "beam:gamma is required"),
+])
+def test_driver_errors(edit, match):
+    with pytest.raises(D.DeckError, match=match):
+        _with(DECK, **edit).collisions()
+
+
+def test_drift_to_common_time():
+    r = np.array([[1.0, 0.0, 0.0, 0.0], [3.0, 0.0, 0.0, 2.0]])
+    u = np.array([[2.0, 1.0, 0.0, 1.0], [1.0, 0.0, 0.0, 0.5]])
+    t = D.drift_to_common_time({"r": r, "u": u}, ("same_time", math.inf))
+    assert t == 3.0 / 2.997925e8
+    assert np.allclose(r, [[3.0, 1.0, 0.0, 1.0], [3.0, 0.0, 0.0, 2.0]])
+    p = {"r": r.copy(), "u": u}
+    assert D.drift_to_common_time(p, ("automatic", math.inf)) == math.inf and np.array_equal(p["r"], r)
+
+
+@pytest.mark.gpu
+def test_deck_runs_an_a0_loop_as_one_batch(engine):
+    """Three a0 values of a plane wave run as one batch: every collision gets its
+    own primaries and final state; photon yields grow with a0 and agree with
+    separate runs of each collision within their statistics."""
+    d = _with(DECK, beam__n=4000, beam__charge=None, control__increase_pair_rate_by=None,
+              control__stop_at_time=None, output__min_energy=None, output__max_angle=None,
+              output__discard_background=None, beam__species="electron", laser__polarization="linear")
+    res = D.run_deck(d, engine)
+    assert [r[0] for r in res] == [1.0, 1.5, 2.0]
+    yields = []
+    for a0, part, cnt in res:
+        print("deck batch a0", a0, {k: cnt[k] for k in ("emissions", "error_flags")}, flush=True)
+        assert cnt["error_flags"] == 0
+        e = part["species"] == 0
+        assert abs(part["weight"][e].sum() - 4000.0) < 1e-9  # one electron per primary
+        yields.append(np.sum(part["species"] == 2))
+    assert yields[0] < yields[1] < yields[2]
+    cols, seed, _ = d.collisions()
+    for (laser, beam, opts), y in zip(cols, yields):
+        engine.set_laser(laser)
+        engine.build_beam(beam, seed=seed + 1)
+        c = engine.run(opts, seed=seed + 1)
+        sec, _ = engine.fetch(c)
+        y1 = np.sum(sec["species"] == 2)
+        assert abs(y - y1) < 5.0 * math.sqrt(2.0 * max(y, y1)), (laser.a0, y, y1)

@@ -167,11 +167,13 @@ typedef enum {
     PTG_Q_P_PERP, PTG_Q_R_PERP, PTG_Q_R_X, PTG_Q_R_Y, PTG_Q_P_MINUS, PTG_Q_P_PLUS, PTG_Q_GAMMA,
     PTG_Q_ENERGY, PTG_Q_UNIT, PTG_Q_X, PTG_Q_Y, PTG_Q_Z, PTG_Q_RADIUS, PTG_Q_BIRTH_A, PTG_Q_PARENT_CHI,
     PTG_Q_N_INTER, PTG_Q_ABSORPTION, PTG_Q_S1, PTG_Q_S2, PTG_Q_S3,
+    PTG_Q_POL_X, PTG_Q_POL_Y,  /* polarization projected on the x / y axis (particle/mod.rs:344-358) */
+    PTG_Q_WEIGHT, PTG_Q_HELICITY,  /* weight; weight times S_3 */
     PTG_Q_COUNT
 } ptg_quantity;
 
 /* Histogram weights (functions.rs:232-263). */
-typedef enum { PTG_W_NUMBER = 0, PTG_W_ENERGY = 1, PTG_W_HELICITY = 2 } ptg_weight_fn;
+typedef enum { PTG_W_NUMBER = 0, PTG_W_ENERGY = 1, PTG_W_HELICITY = 2, PTG_W_POL_X = 3, PTG_W_POL_Y = 4 } ptg_weight_fn;
 
 /* 1-D weighted histogram of the resident final-state particles of one species,
  * as Histogram::generate_1d (src/output/hgram.rs:209-323): bins span the global
@@ -185,6 +187,16 @@ typedef struct {
     uint32_t nbins;          /* BinSpec::FixedNumber */
     int32_t fixed_range;     /* use min/max below instead of the data range */
     double min, max;         /* in/out: range used (of ln(q) when log_scale) */
+    /* Range cut (Filter, src/output/mod.rs:60-140): with cut = 1 only particles
+     * with cut_min <= cut_quantity < cut_max (default units) set the range and
+     * are binned; the total still counts every particle of the species. The
+     * 2-D histogram takes the cut of spec_x. */
+    int32_t cut;
+    int32_t cut_quantity;    /* ptg_quantity */
+    double cut_min, cut_max;
+    /* Unit conversion (Unit::convert, src/output/units.rs): the quantity is
+     * multiplied by scale before the log and the binning; 0 means 1. */
+    double scale;
 } ptg_hist_spec;
 
 typedef struct ptg_ctx ptg_ctx;

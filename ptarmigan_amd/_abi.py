@@ -88,7 +88,8 @@ class BeamDesc(C.Structure):
 class HistSpec(C.Structure):
     _fields_ = [("species", C.c_int32), ("quantity", C.c_int32), ("weight_fn", C.c_int32),
                 ("log_scale", C.c_int32), ("nbins", C.c_uint32), ("fixed_range", C.c_int32),
-                ("min", C.c_double), ("max", C.c_double)]
+                ("min", C.c_double), ("max", C.c_double), ("cut", C.c_int32), ("cut_quantity", C.c_int32),
+                ("cut_min", C.c_double), ("cut_max", C.c_double), ("scale", C.c_double)]
 
 
 # every extern "C" symbol include/ptg.h declares

@@ -1395,6 +1395,20 @@ int ptg_push(ptg_ctx* c, const uint8_t* species, double* r, double* u, size_t n,
     return PTG_OK;
 }
 
+// The range cut of `cut` and the unit scale of `s` (scale 0 means 1).
+static int hist_cut_scale(ptg_ctx* c, HistArgs& h, const ptg_hist_spec* cut, const ptg_hist_spec* s) {
+    if (cut->weight_fn < PTG_W_NUMBER || cut->weight_fn > PTG_W_POL_Y) return fail(c, PTG_E_ARG, "unknown weight function");
+    if (cut->cut && (cut->cut_quantity < 0 || cut->cut_quantity >= PTG_Q_COUNT))
+        return fail(c, PTG_E_ARG, "unknown cut quantity");
+    if (!(std::isfinite(s->scale) && s->scale >= 0.0)) return fail(c, PTG_E_ARG, "scale must be finite and >= 0");
+    h.cut = cut->cut ? 1 : 0;
+    h.cq = cut->cut ? cut->cut_quantity : 0;
+    h.cmin = cut->cut_min;
+    h.cmax = cut->cut_max;
+    h.scale = s->scale == 0.0 ? 1.0 : s->scale;
+    return PTG_OK;
+}
+
 int ptg_histogram(ptg_ctx* c, ptg_hist_spec* s, double* bins_out, double* total_weight) {
     if (!c || !s || !bins_out) return fail(c, PTG_E_ARG, "null argument");
     if (s->nbins == 0 || s->nbins > 8192) return fail(c, PTG_E_ARG, "nbins must be in [1, 8192]");
@@ -1408,6 +1422,7 @@ int ptg_histogram(ptg_ctx* c, ptg_hist_spec* s, double* bins_out, double* total_
     h.wfn = s->weight_fn;
     h.log_scale = s->log_scale;
     h.nbins = s->nbins;
+    if (int rc = hist_cut_scale(c, h, s, s)) return rc;
     int grid = std::max(1, std::min(c->n_cu * 4, int((h.n + 255) / 256)));
     double gmin = s->min, gmax = s->max;
     if (!s->fixed_range) {
@@ -1463,6 +1478,7 @@ int ptg_histogram_2d(ptg_ctx* c, ptg_hist_spec* sx, ptg_hist_spec* sy, double* b
         h[a].q = sp[a]->quantity;
         h[a].wfn = sx->weight_fn;
         h[a].log_scale = sp[a]->log_scale;
+        if (int rc = hist_cut_scale(c, h[a], sx, sp[a])) return rc;
         double gmin = sp[a]->min, gmax = sp[a]->max;
         if (!sp[a]->fixed_range) {
             unsigned long long mm[2] = {~0ull, 0ull};

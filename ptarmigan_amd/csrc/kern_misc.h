@@ -22,6 +22,8 @@ struct HistArgs {
     int species, q, wfn, log_scale;
     uint32_t nbins;
     double gmin, bin_vol;
+    int cut, cq;             // range cut on quantity cq: cmin <= q < cmax
+    double cmin, cmax, scale;
 };
 
 struct EvalArgs {

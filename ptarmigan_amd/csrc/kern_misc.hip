@@ -113,6 +113,39 @@ void launch_primary_dirs(const PBuf& p, uint32_t n, double* n0, hipStream_t s) {
 
 // ---------------------------------------------------------------- histograms
 // Accessors of src/output/functions.rs:10-263.
+// Fraction of the intensity polarized along a unit axis (ax = 0: x, 1: y): the
+// Stokes vector as a Jones vector (e_x real, e_y complex) in the basis x = (n_z,
+// 0, -n_x)/|..|, y = x cross n about the direction n, projected on the axis and
+// mixed with the unpolarized part (geometry/polarization.rs:159-211).
+__device__ double pol_along(const PBuf& p, uint64_t i, int ax) {
+    double ux = p.d[P_UX][i], uy = p.d[P_UY][i], uz = p.d[P_UZ][i];
+    double un = sqrt(ux * ux + uy * uy + uz * uz);
+    double nx = ux / un, ny = uy / un, nz = uz / un;
+    double mag = hypot(nx, nz);
+    double x0 = 1.0, x2 = 0.0;
+    if (mag != 0.0) { x0 = nz / mag; x2 = -nx / mag; }
+    // y = x cross n with x = (x0, 0, x2)
+    double y0 = -x2 * ny, y1 = x2 * nx - x0 * nz, y2 = x0 * ny;
+    double xa = ax == 0 ? x0 : 0.0, ya = ax == 0 ? y0 : y1;
+    double unpol = 0.5 * (xa * xa + ya * ya);
+    double si = p.d[P_SI][i], sq = p.d[P_SQ][i], su = p.d[P_SU][i], sv = p.d[P_SV][i];
+    double frac = hypot(hypot(sq, su), sv) / si;
+    if (!(frac > 0.0)) return unpol;
+    double q = sq / (frac * si), u = su / (frac * si), v = sv / (frac * si);
+    double ex;
+    if (1.0 + q < 1.0e-12) {
+        double s2 = u * u + v * v;
+        double delta = 0.5 * s2 + 0.125 * s2 * s2;
+        ex = sqrt(0.5 * delta);
+    } else {
+        ex = sqrt(0.5 * (1.0 + q));
+    }
+    double eyr = 1.0, eyi = 0.0;
+    if (ex != 0.0) { eyr = 0.5 * u / ex; eyi = -0.5 * v / ex; }
+    double re = ex * xa + eyr * ya, im = eyi * ya;
+    return frac * (re * re + im * im) + (1.0 - frac) * unpol;
+}
+
 __device__ __forceinline__ double quantity(const PBuf& p, uint64_t i, int q) {
     double ut = p.d[P_UT][i], ux = p.d[P_UX][i], uy = p.d[P_UY][i], uz = p.d[P_UZ][i];
     double p0 = M_E_MEV * ut, p1 = M_E_MEV * ux, p2 = M_E_MEV * uy, p3 = M_E_MEV * uz;
@@ -144,6 +177,10 @@ __device__ __forceinline__ double quantity(const PBuf& p, uint64_t i, int q) {
     case PTG_Q_S1: return p.d[P_SQ][i];
     case PTG_Q_S2: return p.d[P_SU][i];
     case PTG_Q_S3: return p.d[P_SV][i];
+    case PTG_Q_POL_X: return pol_along(p, i, 0);
+    case PTG_Q_POL_Y: return pol_along(p, i, 1);
+    case PTG_Q_WEIGHT: return p.d[P_WEIGHT][i];
+    case PTG_Q_HELICITY: return p.d[P_WEIGHT][i] * p.d[P_SV][i];
     default: return 0.0;
     }
 }
@@ -151,7 +188,19 @@ __device__ __forceinline__ double hweight(const PBuf& p, uint64_t i, int w) {
     double wt = p.d[P_WEIGHT][i];
     if (w == PTG_W_ENERGY) return wt * p.d[P_UT][i] * M_E_MEV;
     if (w == PTG_W_HELICITY) return wt * p.d[P_SV][i];
+    if (w == PTG_W_POL_X) return wt * pol_along(p, i, 0);
+    if (w == PTG_W_POL_Y) return wt * pol_along(p, i, 1);
     return wt;
+}
+// the binned value: converted to the output unit, then log-scaled if asked
+__device__ __forceinline__ double hvalue(const HistArgs& h, uint64_t i) {
+    double v = quantity(h.p, i, h.q) * h.scale;
+    return h.log_scale ? log(v) : v;
+}
+__device__ __forceinline__ bool hcut(const HistArgs& h, uint64_t i) {
+    if (!h.cut) return true;
+    double c = quantity(h.p, i, h.cq);
+    return c >= h.cmin && c < h.cmax;
 }
 
 __device__ __forceinline__ double order_key_inv(unsigned long long k) {
@@ -167,9 +216,8 @@ __global__ void k_hist_minmax(HistArgs h, unsigned long long* mm) {
     unsigned long long lo = ~0ull, hi = 0ull;
     for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < h.n; i += uint64_t(gridDim.x) * blockDim.x) {
         if ((h.p.meta[i] & 3u) != uint32_t(h.species)) continue;
-        double v = quantity(h.p, i, h.q);
-        if (h.log_scale) v = log(v);
-        if (!isfinite(v)) continue;
+        double v = hvalue(h, i);
+        if (!isfinite(v) || !hcut(h, i)) continue;
         unsigned long long k = order_key(v);
         lo = k < lo ? k : lo;
         hi = k > hi ? k : hi;
@@ -196,11 +244,10 @@ __global__ void k_hist_bin(HistArgs h, double* bins, double* total) {
     double tot = 0.0;
     for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < h.n; i += uint64_t(gridDim.x) * blockDim.x) {
         if ((h.p.meta[i] & 3u) != uint32_t(h.species)) continue;
-        double v = quantity(h.p, i, h.q);
-        if (h.log_scale) v = log(v);
+        double v = hvalue(h, i);
         double w = hweight(h.p, i, h.wfn);
         tot += w;
-        if (!isfinite(v)) continue;
+        if (!isfinite(v) || !hcut(h, i)) continue;
         double fb = floor((v - h.gmin) / h.bin_vol);
         if (!(fb >= 0.0)) fb = 0.0;  // Rust `as usize` saturates at 0
         if (fb < double(h.nbins)) atomicAdd(&lbins[uint32_t(fb)], w);
@@ -227,12 +274,10 @@ __global__ void k_hist_bin2(HistArgs hx, HistArgs hy, double* bins, double* tota
     double tot = 0.0;
     for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < hx.n; i += uint64_t(gridDim.x) * blockDim.x) {
         if ((hx.p.meta[i] & 3u) != uint32_t(hx.species)) continue;
-        double vx = quantity(hx.p, i, hx.q), vy = quantity(hx.p, i, hy.q);
-        if (hx.log_scale) vx = log(vx);
-        if (hy.log_scale) vy = log(vy);
+        double vx = hvalue(hx, i), vy = hvalue(hy, i);
         const double w = hweight(hx.p, i, hx.wfn);
         tot += w;
-        if (!isfinite(vx) || !isfinite(vy)) continue;
+        if (!isfinite(vx) || !isfinite(vy) || !hcut(hx, i)) continue;
         double bx = hx.bin_vol == 0.0 ? 0.0 : floor((vx - hx.gmin) / hx.bin_vol);
         double by = hy.bin_vol == 0.0 ? 0.0 : floor((vy - hy.gmin) / hy.bin_vol);
         if (!(bx >= 0.0)) bx = 0.0;  // Rust `as usize` saturates at 0

@@ -333,6 +333,14 @@ class Deck:
             return str(v)
         raise DeckError(f"{path} must be a string")
 
+    def raw_list(self, path):
+        """A list of strings (a scalar gives a list of one), empty when missing."""
+        try:
+            v = self._node(path)
+        except _Missing:
+            return []
+        return [self.string_of(x) for x in (v if isinstance(v, list) else [v])]
+
     def numbers(self, path, default=_Missing):
         try:
             v = self._node(path)
@@ -563,18 +571,28 @@ def drift_to_common_time(particles, stop):
     return ct / SPEED_OF_LIGHT
 
 
-def run_deck(deck, engine=None):
+def ident(deck, path=None):
+    """output:ident, `auto` meaning the input file's stem (src/main.rs:813-829)."""
+    s = deck.string("output:ident", "")
+    return deck.name if s == "auto" else s
+
+
+def run_deck(deck, engine=None, out_dir=None):
     """Transports every collision of the deck on the GPU and returns
     [(a0, final-state particles, counters)]. Several a0 values of an LMA field
     run as one batch of collisions (one primary set per a0, as the reference's
-    loop builds one per run, src/main.rs:998-1070)."""
+    loop builds one per run, src/main.rs:998-1070). With `out_dir`, the
+    distributions the output section asks for are written there after each run
+    (then every a0 runs on its own, so that the resident final state is its)."""
     import numpy as np
     from .engine import Engine
+    from . import output
     cols, seed, stop = deck.collisions()
     eng = engine or Engine(0)
     out = []
     lma = cols[0][0].kind in (LaserKind.PLANE_WAVE, LaserKind.FOCUSED_LASER)
-    if len(cols) > 1 and lma:
+    writing = out_dir is not None and any(output.distributions(deck).values())
+    if len(cols) > 1 and lma and not writing:
         eng.set_batch([c[0] for c in cols], [c[1] for c in cols], seed=seed)
         cnt = eng.run(cols[0][2], seed=seed)
         sec, _ = eng.fetch(cnt)
@@ -591,5 +609,7 @@ def run_deck(deck, engine=None):
         cnt = eng.run(opts, seed=seed)
         sec, _ = eng.fetch(cnt)
         drift_to_common_time(sec, stop)
+        if writing:
+            output.write_distributions(eng, deck, sec, out_dir, ident(deck), laser.a0, len(cols))
         out.append((laser.a0, sec, cnt))
     return out

@@ -271,20 +271,29 @@ class Engine:
                                        taken.ctypes.data_as(C.POINTER(C.c_uint32)), _dptr(work)))
         return (r, u, taken, work) if with_work else (r, u, taken)
 
-    def histogram(self, species, quantity, nbins, weight_fn=0, log_scale=False, range_=None):
-        """Histogram::generate_1d over the resident final-state particles; returns (counts, min, max, total)."""
-        spec = _abi.HistSpec(int(species), int(quantity), int(weight_fn), int(log_scale), nbins,
-                             1 if range_ else 0, range_[0] if range_ else 0.0, range_[1] if range_ else 0.0)
+    @staticmethod
+    def _hist_spec(species, quantity, weight_fn, log_scale, nbins, range_=None, cut=None, scale=1.0):
+        return _abi.HistSpec(int(species), int(quantity), int(weight_fn), int(log_scale), nbins,
+                             0 if range_ is None else 1, range_[0] if range_ else 0.0, range_[1] if range_ else 0.0,
+                             1 if cut else 0, int(cut[0]) if cut else 0, float(cut[1]) if cut else 0.0,
+                             float(cut[2]) if cut else 0.0, float(scale))
+
+    def histogram(self, species, quantity, nbins, weight_fn=0, log_scale=False, range_=None, cut=None, scale=1.0):
+        """Histogram::generate_1d over the resident final-state particles; returns (counts, min, max, total).
+        `cut` = (quantity, min, max) bins only particles with min <= quantity < max; `scale`
+        converts the binned quantity to another unit."""
+        spec = self._hist_spec(species, quantity, weight_fn, log_scale, nbins, range_, cut, scale)
         out = np.zeros(nbins)
         tot = C.c_double()
         self._check(self._lib.ptg_histogram(self._ctx, C.byref(spec), _dptr(out), C.byref(tot)))
         return out, spec.min, spec.max, tot.value
 
-    def histogram2d(self, species, qx, qy, nbins_x, nbins_y, weight_fn=0, log_x=False, log_y=False):
+    def histogram2d(self, species, qx, qy, nbins_x, nbins_y, weight_fn=0, log_x=False, log_y=False, cut=None,
+                    scale=(1.0, 1.0), ranges=(None, None)):
         """Histogram::generate_2d over the resident final-state particles; returns
         (flat counts of nbins_x * nbins_y, (xmin, xmax), (ymin, ymax), total)."""
-        sx = _abi.HistSpec(int(species), int(qx), int(weight_fn), int(log_x), nbins_x, 0, 0.0, 0.0)
-        sy = _abi.HistSpec(int(species), int(qy), int(weight_fn), int(log_y), nbins_y, 0, 0.0, 0.0)
+        sx = self._hist_spec(species, qx, weight_fn, log_x, nbins_x, ranges[0], cut, scale[0])
+        sy = self._hist_spec(species, qy, weight_fn, log_y, nbins_y, ranges[1], None, scale[1])
         out = np.zeros(nbins_x * nbins_y)
         tot = C.c_double()
         self._check(self._lib.ptg_histogram_2d(self._ctx, C.byref(sx), C.byref(sy), _dptr(out), C.byref(tot)))

@@ -199,6 +199,25 @@ typedef struct {
     double scale;
 } ptg_hist_spec;
 
+/* A summary statistic's reductions over the resident final-state particles of
+ * one species (SummaryStatistic, src/output/stats.rs:265-410): v = variable,
+ * w = weight quantity (PTG_Q_UNIT for none), "in" = cut_min < cut_quantity <
+ * cut_max (strict) or every particle when cut = 0. Default units (MeV, m, rad). */
+typedef struct {
+    int32_t species;         /* ptg_species */
+    int32_t variable;        /* ptg_quantity */
+    int32_t weight;          /* ptg_quantity */
+    int32_t cut;
+    int32_t cut_quantity;    /* ptg_quantity */
+    int32_t reserved;
+    double cut_min, cut_max;
+    double shift;            /* subtracted from v in the last two sums */
+} ptg_stats_spec;
+/* out[PTG_STATS_N]: sum v w (all), sum v w (in), sum w (in), min v w (in, +inf if
+ * none), max v w (in, -inf if none), sum w cos v, sum w sin v, sum w (v - shift)^2,
+ * sum w (v - shift) (in). */
+#define PTG_STATS_N 9
+
 typedef struct ptg_ctx ptg_ctx;
 
 /* Creates a context on HIP device `device`, loading tables from the blob at
@@ -249,6 +268,9 @@ int ptg_histogram(ptg_ctx* ctx, ptg_hist_spec* spec, double* bins_out, double* t
  * weight from spec_x; bins_out has nbins_x * nbins_y entries, flat index
  * by * nbins_y + bx as the reference (rows overlap when the counts differ). */
 int ptg_histogram_2d(ptg_ctx* ctx, ptg_hist_spec* spec_x, ptg_hist_spec* spec_y, double* bins_out, double* total_weight);
+/* The reductions of one summary statistic (stats.rs:265-410): replaces the
+ * per-particle loops of total / variance / min / max / directmean. */
+int ptg_stats(ptg_ctx* ctx, const ptg_stats_spec* spec, double* out);
 
 /* Batched evaluation of one device function for parity testing. fn ids are
  * listed in ptg_eval_fn; in is n x in_w, out is n x out_w (host buffers). */

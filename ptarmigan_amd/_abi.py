@@ -92,11 +92,19 @@ class HistSpec(C.Structure):
                 ("cut_min", C.c_double), ("cut_max", C.c_double), ("scale", C.c_double)]
 
 
+class StatsSpec(C.Structure):
+    _fields_ = [("species", C.c_int32), ("variable", C.c_int32), ("weight", C.c_int32), ("cut", C.c_int32),
+                ("cut_quantity", C.c_int32), ("reserved", C.c_int32), ("cut_min", C.c_double),
+                ("cut_max", C.c_double), ("shift", C.c_double)]
+
+
+STATS_N = 9
+
 # every extern "C" symbol include/ptg.h declares
 EXPORTS = ("ptg_create", "ptg_destroy", "ptg_last_error", "ptg_set_laser", "ptg_field_info", "ptg_build_beam",
            "ptg_load_primaries", "ptg_run", "ptg_collide", "ptg_fetch", "ptg_push", "ptg_histogram", "ptg_eval",
            "ptg_surrogate_train", "ptg_surrogate_load", "ptg_mlp_probe", "ptg_set_batch", "ptg_autotune",
-           "ptg_fetch_showers", "ptg_histogram_2d")
+           "ptg_fetch_showers", "ptg_histogram_2d", "ptg_stats")
 
 _lib = None
 
@@ -134,6 +142,7 @@ def load():
     L.ptg_set_batch.argtypes = [vp, P(LaserDesc), P(BeamDesc), C.c_int, C.c_uint64]
     L.ptg_fetch_showers.argtypes = [vp, P(C.c_uint32)]
     L.ptg_histogram_2d.argtypes = [vp, P(HistSpec), P(HistSpec), P(d), P(d)]
+    L.ptg_stats.argtypes = [vp, P(StatsSpec), P(d)]
     L.ptg_autotune.argtypes = [vp, P(CollideOptions), C.c_uint64, C.c_uint64, P(TuneResult)]
     for f in ("ptg_set_laser", "ptg_field_info", "ptg_build_beam", "ptg_load_primaries", "ptg_run", "ptg_collide",
               "ptg_fetch", "ptg_push", "ptg_histogram", "ptg_eval"):

@@ -1460,6 +1460,46 @@ int ptg_histogram(ptg_ctx* c, ptg_hist_spec* s, double* bins_out, double* total_
     return PTG_OK;
 }
 
+int ptg_stats(ptg_ctx* c, const ptg_stats_spec* s, double* out) {
+    if (!c || !s || !out) return fail(c, PTG_E_ARG, "null argument");
+    for (int q : {s->variable, s->weight, s->cut ? s->cut_quantity : 0})
+        if (q < 0 || q >= PTG_Q_COUNT) return fail(c, PTG_E_ARG, "unknown quantity");
+    if (s->species < 0 || s->species > 2) return fail(c, PTG_E_ARG, "bad species");
+    HIPCHK(c, hipSetDevice(c->device));
+    StatArgs a{};
+    a.p = c->out.v;
+    a.n = c->n_out;
+    a.species = s->species;
+    a.qv = s->variable;
+    a.qw = s->weight;
+    a.cut = s->cut ? 1 : 0;
+    a.qf = s->cut ? s->cut_quantity : 0;
+    a.fmin = s->cut_min;
+    a.fmax = s->cut_max;
+    a.shift = s->shift;
+    // 7 sums (slots 0-6) then the two order keys (slots 8, 9)
+    unsigned long long* d = nullptr;
+    HIPCHK(c, hipMalloc(&d, 10 * sizeof(unsigned long long)));
+    unsigned long long init[10] = {0, 0, 0, 0, 0, 0, 0, 0, ~0ull, 0ull};
+    unsigned long long h[10];
+    hipError_t e = scopy(c, d, init, sizeof(init), hipMemcpyHostToDevice);
+    int grid = std::max(1, std::min(c->n_cu * 4, int((a.n + 255) / 256)));
+    if (e == hipSuccess && a.n) {
+        launch_stats(a, reinterpret_cast<double*>(d), d + 8, grid, c->stream);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = scopy(c, h, d, sizeof(h), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    HIPCHK(c, e);
+    double sums[7];
+    memcpy(sums, h, sizeof(sums));
+    out[0] = sums[0]; out[1] = sums[1]; out[2] = sums[2];
+    out[3] = h[8] == ~0ull ? INFINITY : host_order_key_inv(h[8]);
+    out[4] = h[8] == ~0ull ? -INFINITY : host_order_key_inv(h[9]);
+    out[5] = sums[3]; out[6] = sums[4]; out[7] = sums[5]; out[8] = sums[6];
+    return PTG_OK;
+}
+
 int ptg_histogram_2d(ptg_ctx* c, ptg_hist_spec* sx, ptg_hist_spec* sy, double* bins_out, double* total_weight) {
     if (!c || !sx || !sy || !bins_out) return fail(c, PTG_E_ARG, "null argument");
     for (const ptg_hist_spec* s : {sx, sy}) {

@@ -44,6 +44,14 @@ void launch_hist_minmax(const HistArgs& h, unsigned long long* mm, int grid, hip
 void launch_hist_bin(const HistArgs& h, double* bins, double* total, int grid, hipStream_t s);
 // 2-D weighted binning (hgram.rs:326-446): hx gives species, weight and the x axis,
 // hy the y axis; flat bin by * ny + bx as the reference, dropped past nx * ny.
+struct StatArgs {
+    PBuf p;
+    uint64_t n;
+    int species, qv, qw, cut, qf;
+    double fmin, fmax, shift;
+};
+// acc[7]: sums (all vw, in vw, in w, in w cos v, in w sin v, in w (v-s)^2, in w (v-s)); mm[2]: order keys of min/max vw
+void launch_stats(const StatArgs& a, double* acc, unsigned long long* mm, int grid, hipStream_t s);
 void launch_hist_bin2(const HistArgs& hx, const HistArgs& hy, double* bins, double* total, int grid, hipStream_t s);
 double host_order_key_inv(unsigned long long k);
 void launch_select_multiplicity(const PBuf& src, uint64_t n, const PBuf& dst, unsigned long long* ctr, const uint32_t* count,

@@ -302,6 +302,50 @@ void launch_hist_bin2(const HistArgs& hx, const HistArgs& hy, double* bins, doub
     else k_hist_bin2<false><<<grid, 256, 0, s>>>(hx, hy, bins, total);
 }
 
+// One pass of every reduction a summary statistic needs (stats.rs:265-410),
+// wave-reduced then one atomic per wave and sum.
+__global__ void k_stats(StatArgs a, double* acc, unsigned long long* mm) {
+    double s[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    unsigned long long lo = ~0ull, hi = 0ull;
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < a.n; i += uint64_t(gridDim.x) * blockDim.x) {
+        if ((a.p.meta[i] & 3u) != uint32_t(a.species)) continue;
+        const double v = quantity(a.p, i, a.qv), w = quantity(a.p, i, a.qw);
+        const double vw = v * w;
+        s[0] += vw;
+        if (a.cut) {
+            const double f = quantity(a.p, i, a.qf);
+            if (!(f > a.fmin && f < a.fmax)) continue;
+        }
+        s[1] += vw;
+        s[2] += w;
+        s[3] += w * cos(v);
+        s[4] += w * sin(v);
+        const double d = v - a.shift;
+        s[5] += w * d * d;
+        s[6] += w * d;
+        if (!isnan(vw)) {
+            const unsigned long long k = order_key(vw);
+            lo = k < lo ? k : lo;
+            hi = k > hi ? k : hi;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        for (int j = 0; j < 7; ++j) s[j] += __shfl_xor(s[j], off);
+        unsigned long long olo = __shfl_xor(lo, off), ohi = __shfl_xor(hi, off);
+        lo = olo < lo ? olo : lo;
+        hi = ohi > hi ? ohi : hi;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        for (int j = 0; j < 7; ++j)
+            if (s[j] != 0.0) atomicAdd(&acc[j], s[j]);
+        atomicMin(mm, lo);
+        atomicMax(mm + 1, hi);
+    }
+}
+void launch_stats(const StatArgs& a, double* acc, unsigned long long* mm, int grid, hipStream_t s) {
+    k_stats<<<grid, 256, 0, s>>>(a, acc, mm);
+}
+
 void launch_hist_minmax(const HistArgs& h, unsigned long long* mm, int grid, hipStream_t s) {
     k_hist_minmax<<<grid, 256, 0, s>>>(h, mm);
 }

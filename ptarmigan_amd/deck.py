@@ -591,7 +591,8 @@ def run_deck(deck, engine=None, out_dir=None):
     eng = engine or Engine(0)
     out = []
     lma = cols[0][0].kind in (LaserKind.PLANE_WAVE, LaserKind.FOCUSED_LASER)
-    writing = out_dir is not None and any(output.distributions(deck).values())
+    writing = out_dir is not None and (any(output.distributions(deck).values()) or
+                                       any(output.statistics(deck)))
     if len(cols) > 1 and lma and not writing:
         eng.set_batch([c[0] for c in cols], [c[1] for c in cols], seed=seed)
         cnt = eng.run(cols[0][2], seed=seed)
@@ -611,5 +612,6 @@ def run_deck(deck, engine=None, out_dir=None):
         drift_to_common_time(sec, stop)
         if writing:
             output.write_distributions(eng, deck, sec, out_dir, ident(deck), laser.a0, len(cols))
+            output.write_stats(eng, deck, out_dir, ident(deck), laser.a0, len(cols))
         out.append((laser.a0, sec, cnt))
     return out

@@ -299,6 +299,15 @@ class Engine:
         self._check(self._lib.ptg_histogram_2d(self._ctx, C.byref(sx), C.byref(sy), _dptr(out), C.byref(tot)))
         return out, (sx.min, sx.max), (sy.min, sy.max), tot.value
 
+    def stats(self, species, variable, weight=15, cut=None, shift=0.0):
+        """ptg_stats: the reductions of one summary statistic over the resident final
+        state (weight quantity 15 = unit). `cut` = (quantity, min, max), strict."""
+        spec = _abi.StatsSpec(int(species), int(variable), int(weight), 1 if cut else 0, int(cut[0]) if cut else 0, 0,
+                              float(cut[1]) if cut else 0.0, float(cut[2]) if cut else 0.0, float(shift))
+        out = np.zeros(_abi.STATS_N)
+        self._check(self._lib.ptg_stats(self._ctx, C.byref(spec), _dptr(out)))
+        return out
+
     def eval(self, fn, x, out_w):
         """Batched evaluation of one device function (ptg_eval_fn ids) for parity tests."""
         x = np.ascontiguousarray(np.atleast_2d(np.asarray(x, dtype=np.float64)))

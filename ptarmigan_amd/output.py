@@ -331,3 +331,168 @@ def write_distributions(engine, deck, particles, prefix_dir="", ident="", a0=Non
             if h is not None:
                 written.append(h.write(prefix + d.suffix(), fmt))
     return written
+
+
+# ---------------------------------------------------------------- summary statistics
+_OPS = {"total": "total", "fraction": "fraction", "frac": "fraction", "mean": "mean", "variance": "variance",
+        "var": "variance", "minimum": "minimum", "min": "minimum", "maximum": "maximum", "max": "maximum",
+        "circmean": "circmean", "cmean": "circmean", "circvariance": "circvar", "cvariance": "circvar",
+        "circvar": "circvar", "cvar": "circvar", "circstd": "circstd", "cstd": "circstd"}
+_DEFAULT = UNIT_SYSTEMS["auto"]
+
+
+class Statistic:
+    """One `stats:` specifier (SummaryStatistic::load, src/output/stats.rs:85-263):
+    `op var[`weight]`, `op var[`weight] in (min; max)` or
+    `op var[`weight] for var2 in (min; max)`, bounds in SI and strict."""
+
+    def __init__(self, spec, context=None):
+        context = context or {}
+        bad = DeckError(f"{spec!r} is not a summary statistic")
+        start, end = spec.find("("), spec.rfind(")")
+        rng = spec[start:end + 1] if start >= 0 and end >= 0 else None
+        words = (spec.replace(rng, "") if rng else spec).split()
+        fname = None
+        if len(words) == 5:
+            if words[3] not in _Q or words[2] != "for":
+                raise DeckError(f"{spec!r}: the requested filter is not recognised")
+            fname = words[3]
+        lo, hi = -math.inf, math.inf
+        if len(words) in (3, 5):
+            if rng is None or words[-1] != "in":
+                raise bad
+            rs = ["".join(r.split()) for r in rng[1:-1].split(";")]
+            if len(rs) != 2:
+                raise bad
+            try:
+                lo = -math.inf if rs[0] == "auto" else float(evaluate(rs[0], context))
+                hi = math.inf if rs[1] == "auto" else float(evaluate(rs[1], context))
+            except DeckError:
+                raise bad
+        if len(words) not in (2, 3, 5):
+            raise bad
+        if words[0] not in _OPS:
+            raise DeckError(f"{spec!r}: the requested operation is not recognised")
+        self.op = _OPS[words[0]]
+        vs = words[1].split("`")
+        if len(vs) > 2:
+            raise bad
+        var, wname = vs[0], (vs[1] if len(vs) == 2 else "unit")
+        if var not in _Q:
+            raise DeckError(f"{spec!r}: the requested variable is not recognised")
+        kind = _Q[var][1]
+        if self.op.startswith("circ") and kind != "angle":
+            raise DeckError(f"{spec!r}: circular stats can only be applied to angles")
+        u = _unit(kind, _DEFAULT)[1]
+        if self.op in ("fraction", "circvar"):
+            u = "1"
+        elif self.op == "variance":
+            u = "1" if u == "1" else f"({u})^2"
+        if wname not in _Q:
+            raise bad
+        self.variable, self.var_unit, self.var_quantity = var, u, _Q[var][0]
+        self.weight, self.weight_quantity = wname, _Q[wname][0]
+        # a finite bound without `for` cuts on the variable itself (keeping its unit)
+        if fname is None and (math.isfinite(lo) or math.isfinite(hi)):
+            self.filter, self.filter_unit, fkind = var, u, kind
+        elif fname is not None:
+            self.filter, self.filter_unit, fkind = fname, _unit(_Q[fname][1], _DEFAULT)[1], _Q[fname][1]
+        else:
+            self.filter, fkind = None, kind
+        scale = _unit(fkind, UNIT_SYSTEMS["si"])[0]
+        self.min, self.max = lo / scale, hi / scale
+        self.value = math.nan
+        self.name = ""
+
+    def evaluate(self, engine, species, name):
+        """The statistic over the resident final state (stats.rs:412-481): one device
+        pass for the sums, a second for the variance, shifted by the mean."""
+        cut = (_Q[self.filter][0], self.min, self.max) if self.filter else None
+        out = engine.stats(int(species), self.var_quantity, self.weight_quantity, cut)
+        f = np.float64
+        with np.errstate(all="ignore"):
+            if self.op == "total":
+                v = out[1]
+            elif self.op == "fraction":
+                v = f(out[1]) / f(out[0])
+            elif self.op == "mean":
+                v = f(out[1]) / f(out[2])
+            elif self.op == "minimum":
+                v = out[3]
+            elif self.op == "maximum":
+                v = out[4]
+            elif self.op == "variance":
+                mean = f(out[1]) / f(out[2])
+                o2 = engine.stats(int(species), self.var_quantity, self.weight_quantity, cut,
+                                  shift=float(mean) if np.isfinite(mean) else 0.0)
+                v = (f(o2[7]) - f(o2[8]) ** 2 / f(o2[2])) / f(o2[2])
+            else:
+                mx, my = f(out[5]) / f(out[2]), f(out[6]) / f(out[2])
+                r = np.hypot(mx, my)
+                v = {"circmean": np.arctan2(my, mx), "circvar": 1.0 - r, "circstd": np.sqrt(-2.0 * np.log(r))}[self.op]
+        self.value, self.name = float(v), name
+        return self.value
+
+    def __str__(self):
+        bounds = ""
+        if self.filter:
+            if math.isfinite(self.min) and math.isinf(self.max):
+                bounds = f" ({self.filter} [{self.filter_unit}] > {rust_exp(self.min, 6)})"
+            elif math.isinf(self.min) and math.isfinite(self.max):
+                bounds = f" ({self.filter} [{self.filter_unit}] < {rust_exp(self.max, 6)})"
+            else:
+                bounds = f" ({rust_exp(self.min, 6)} < {self.filter} [{self.filter_unit}] < {rust_exp(self.max, 6)})"
+        wstr = f" ({self.weight}-weighted)" if self.weight != "unit" else ""
+        return (f"{self.name}: {self.op} {self.variable}{wstr}{bounds} = {rust_exp(self.value, 6)} "
+                f"[{self.var_unit}]")
+
+
+class StatsExpression:
+    """`name[`formula] expression [unit]` (StatsExpression, stats.rs:519-575)."""
+
+    def __init__(self, spec, context=None):
+        v = spec.split()
+        if len(v) < 2:
+            raise DeckError(f"{spec!r} is not a stats expression")
+        self.name = v[0].split("`")[0]
+        self.formula = v[1] if "`" in v[0] else None
+        try:
+            self.value = float(evaluate(v[1], context or {}))
+        except DeckError:
+            raise DeckError(f"{spec!r}: cannot evaluate {v[1]!r}")
+        self.unit = v[2] if len(v) > 2 else "1"
+
+    def __str__(self):
+        f = f" ({self.formula})" if self.formula is not None else ""
+        return f"expr: {self.name}{f} = {rust_exp(self.value, 6)} [{self.unit}]"
+
+
+def statistics(deck):
+    """[(species name, Statistic)] in the order the reference writes them
+    (electron, positron, photon; src/main.rs:1274-1285), and the expressions."""
+    if deck.raw_list("stats:intermediate"):
+        raise DeckError("stats:intermediate is not supported by this build")
+    stats = []
+    for name in ("electron", "positron", "photon"):
+        stats += [(name, Statistic(s, deck.context)) for s in deck.raw_list(f"stats:{name}")]
+    exprs = [StatsExpression(s, deck.context) for s in deck.raw_list("stats:expression")]
+    return stats, exprs
+
+
+def write_stats(engine, deck, prefix_dir="", ident="", a0=None, n_a0=1):
+    """Evaluates the deck's statistics over the resident final state and writes
+    `[ident_]stats.txt`; returns the file name, or None when none are asked for."""
+    stats, exprs = statistics(deck)
+    if not stats and not exprs:
+        return None
+    cur = ident if n_a0 <= 1 else f"{ident}{'_' if ident else ''}a0_{a0:.3f}"
+    species = {"electron": Species.ELECTRON, "positron": Species.POSITRON, "photon": Species.PHOTON}
+    lines = []
+    for name, st in stats:
+        st.evaluate(engine, species[name], name)
+        lines.append(str(st))
+    lines += [str(e) for e in exprs]
+    fname = f"{prefix_dir}{'/' if prefix_dir else ''}{cur}{'_' if cur else ''}stats.txt"
+    with open(fname, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return fname

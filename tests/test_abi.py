@@ -32,10 +32,10 @@ def test_struct_sizes_match_header():
     """ctypes mirrors of the ABI structs have the sizes the C compiler gives them."""
     from ptarmigan_amd import _abi
     src = os.path.join(ROOT, "include", "ptg.h")
-    code = "#include <stdio.h>\n#include \"%s\"\nint main(){printf(\"%%zu %%zu %%zu %%zu %%zu %%zu %%zu %%zu %%zu %%zu %%d\\n\"," \
+    code = "#include <stdio.h>\n#include \"%s\"\nint main(){printf(\"%%zu %%zu %%zu %%zu %%zu %%zu %%zu %%zu %%zu %%zu %%zu %%d\\n\"," \
            "sizeof(ptg_laser_desc),sizeof(ptg_collide_options),sizeof(ptg_particles),sizeof(ptg_counters)," \
            "sizeof(ptg_beam_desc),sizeof(ptg_hist_spec),sizeof(ptg_surrogate_model),sizeof(ptg_surrogate_train_desc)," \
-           "sizeof(ptg_surrogate_report),sizeof(ptg_tune_result),PTG_MLP_NPARAM);}\n" % src
+           "sizeof(ptg_surrogate_report),sizeof(ptg_tune_result),sizeof(ptg_stats_spec),PTG_MLP_NPARAM);}\n" % src
     import subprocess
     import tempfile
     with tempfile.TemporaryDirectory() as d:
@@ -45,9 +45,9 @@ def test_struct_sizes_match_header():
         out = subprocess.run([os.path.join(d, "s")], capture_output=True, text=True, check=True).stdout.split()
     want = [ctypes.sizeof(t) for t in (_abi.LaserDesc, _abi.CollideOptions, _abi.Particles, _abi.Counters,
                                        _abi.BeamDesc, _abi.HistSpec, _abi.SurrogateModel, _abi.SurrogateTrainDesc,
-                                       _abi.SurrogateReport, _abi.TuneResult)]
-    assert [int(x) for x in out[:10]] == want
-    assert int(out[10]) == _abi.MLP_NPARAM
+                                       _abi.SurrogateReport, _abi.TuneResult, _abi.StatsSpec)]
+    assert [int(x) for x in out[:11]] == want
+    assert int(out[11]) == _abi.MLP_NPARAM
 
 
 def test_engine_without_gpu_fails_loudly():

@@ -286,3 +286,12 @@ def test_deck_runs_an_a0_loop_as_one_batch(engine):
         sec, _ = engine.fetch(c)
         y1 = np.sum(sec["species"] == 2)
         assert abs(y - y1) < 5.0 * math.sqrt(2.0 * max(y, y1)), (laser.a0, y, y1)
+
+
+def test_command_line_reports_bad_decks(tmp_path, capsys):
+    from ptarmigan_amd.__main__ import main
+    assert main([]) == 2
+    bad = tmp_path / "bad.yml"
+    bad.write_text("laser:\n  a0: 1.0\n")
+    assert main([str(bad)]) == 1
+    assert "error:" in capsys.readouterr().err

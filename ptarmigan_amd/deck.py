@@ -11,9 +11,9 @@ src/input/mod.rs:239-263), and `run_deck` transports them — several a0 values
 as one batch of collisions.
 
 Not supported (they raise DeckError naming the key): beams loaded from HDF5
-(`beam:from_hdf5`), custom spectra (`beam:spectrum`), bremsstrahlung sources,
-`increase_pair_rate_by: auto`.
+(`beam:from_hdf5`), custom spectra (`beam:spectrum`), bremsstrahlung sources.
 """
+import dataclasses
 import math
 import warnings
 import re
@@ -31,6 +31,8 @@ ELEMENTARY_CHARGE = -ELECTRON_CHARGE
 ELECTRON_MASS = 9.109383e-31
 PROTON_MASS = 1.672622e-27
 COMPTON_TIME = 1.28808867e-21
+ELECTRON_MASS_MEV = 0.510999
+ALPHA_FINE = 7.29735257e-3
 
 
 class DeckError(ValueError):
@@ -394,10 +396,12 @@ class Deck:
         else:
             classical = gaunt = False
         mult = self.count("control:select_multiplicity", None) if self.contains("control:select_multiplicity") else None
+        self.auto_rate = False
         if self.contains("control:increase_pair_rate_by"):
             if self._node("control:increase_pair_rate_by") == "auto":
-                raise DeckError("control:increase_pair_rate_by: auto is not supported by this build")
-            rate = self.number("control:increase_pair_rate_by")
+                self.auto_rate, rate = True, 1.0  # set per a0 value by collisions()
+            else:
+                rate = self.number("control:increase_pair_rate_by")
         else:
             rate = 1.0
         if rate < 1.0:
@@ -551,11 +555,59 @@ class Deck:
             return "true" if x else "false"
         return str(x)
 
-    def collisions(self):
-        """[(Laser, Beam, CollideOptions)], one per value of laser:a0, the seed and the stop mode."""
+    def collisions(self, engine=None):
+        """[(Laser, Beam, CollideOptions)], one per value of laser:a0, the seed and the
+        stop mode. `increase_pair_rate_by: auto` evaluates its rates with `engine`."""
         opts, seed, lcfa, stop = self.options()
         beam = self.beam()
-        return [(laser, beam, opts) for laser in self.lasers(lcfa)], seed, stop
+        lasers = self.lasers(lcfa)
+        if not self.auto_rate:
+            return [(laser, beam, opts) for laser in lasers], seed, stop
+        if engine is None:
+            raise DeckError("increase_pair_rate_by: auto evaluates rates on the device: pass an Engine")
+        out = []
+        for laser in lasers:
+            rate = auto_pair_rate_increase(engine, laser, beam.gamma, lcfa)
+            if not rate >= 1.0:
+                raise DeckError("increase in pair creation rate must be >= 1.0")
+            if rate > 1.0 and opts.multiplicity is not None:
+                raise DeckError("event biasing (pair rate increase > 1) is incompatible with multiplicity selection")
+            out.append((laser, beam, dataclasses.replace(opts, rate_increase=rate)))
+        return out, seed, stop
+
+
+def auto_pair_rate_increase(engine, laser, gamma, lcfa):
+    """increase_pair_rate_by: auto (src/main.rs:290-319): the ratio of the photon
+    emission rate of an electron to the pair creation rate of a photon, both of
+    energy gamma m c^2 meeting the laser head-on, with the rates evaluated by the
+    device's own rate functions. LMA: at least 1 (0 pair rate gives 1); LCFA: the
+    plain ratio."""
+    if lcfa:
+        chi = 2.0 * gamma * laser.a0 * 1.26e-6 / (ELECTRON_MASS_MEV * 1.0e6 * laser.wavelength)
+        photon = engine.eval(0, [[chi, gamma]], 1)[0, 0]           # lcfa photon_emission::rate
+        t_par, t_perp = engine.eval(2, [[chi]], 2)[0]               # auxiliary T(chi)
+        pair = ALPHA_FINE * chi * 0.5 * (t_par + t_perp) / (COMPTON_TIME * gamma)
+        return float(photon / pair) if pair != 0.0 else math.inf
+    k0 = SPEED_OF_LIGHT * COMPTON_TIME * 2.0 * math.pi / laser.wavelength
+    u0 = math.sqrt(1.0 + gamma * gamma)
+    lp = laser.polarization == Polarization.LINEAR
+    a_rms = laser.a0 / math.sqrt(2.0) if lp else laser.a0
+    eta_q = k0 * (u0 + gamma)                     # kappa . u for u = (u0, 0, 0, -gamma)
+    q0 = u0 + a_rms * a_rms * k0 / (2.0 * eta_q)  # q = u + a^2 kappa / (2 kappa . u)
+    eta_l = 2.0 * k0 * gamma                      # kappa . ell for ell = gamma (1, 0, 0, -1)
+    if lp:
+        f_ph, ok = engine.eval(8, [[a_rms * math.sqrt(2.0), eta_q]], 2)[0]
+        f0, f1, _ = engine.eval(12, [[a_rms * math.sqrt(2.0), eta_l]], 3)[0]
+    else:
+        f_ph, ok = engine.eval(16, [[a_rms, eta_q, 0.0]], 2)[0]
+        f0, f1, _ = engine.eval(24, [[a_rms, eta_l]], 3)[0]
+    if not ok:
+        raise DeckError(f"increase_pair_rate_by: auto: (a, eta) = ({a_rms:g}, {eta_q:g}) is outside the rate tables")
+    # both probabilities are alpha f dphi / eta with dphi / eta = dt / (tau_c E): the common factors cancel
+    pair = 0.5 * (f0 + f1) / gamma
+    if pair == 0.0:
+        return 1.0
+    return max((f_ph / q0) / pair, 1.0)
 
 
 def drift_to_common_time(particles, stop):
@@ -587,13 +639,15 @@ def run_deck(deck, engine=None, out_dir=None):
     import numpy as np
     from .engine import Engine
     from . import output
-    cols, seed, stop = deck.collisions()
+    deck.collisions()  # deck errors before the device is touched
     eng = engine or Engine(0)
+    cols, seed, stop = deck.collisions(eng)
     out = []
     lma = cols[0][0].kind in (LaserKind.PLANE_WAVE, LaserKind.FOCUSED_LASER)
     writing = out_dir is not None and (any(output.distributions(deck).values()) or
                                        any(output.statistics(deck)))
-    if len(cols) > 1 and lma and not writing:
+    same_opts = all(c[2] == cols[0][2] for c in cols)
+    if len(cols) > 1 and lma and not writing and same_opts:
         eng.set_batch([c[0] for c in cols], [c[1] for c in cols], seed=seed)
         cnt = eng.run(cols[0][2], seed=seed)
         sec, _ = eng.fetch(cnt)

@@ -239,6 +239,7 @@ def test_driver_defaults_and_alternatives():
     ({"control__classical": "maybe"}, "classical"),
     ({"control__classical": "gaunt_factor_corrected"}, "LCFA"),
     ({"control__increase_pair_rate_by": 0.5}, ">= 1.0"),
+    ({"control__increase_pair_rate_by": "auto"}, "pass an Engine"),
     ({"control__select_multiplicity": 2}, "incompatible"),
     ({"control__stop_at_time": "soon"}, "stop_at_time"),
     ({"beam__from_hdf5": {"file": "x.h5"}}, "not supported"),
@@ -295,3 +296,77 @@ def test_command_line_reports_bad_decks(tmp_path, capsys):
     bad.write_text("laser:\n  a0: 1.0\n")
     assert main([str(bad)]) == 1
     assert "error:" in capsys.readouterr().err
+
+
+class _RateStub:
+    """Records the arguments of the rate evaluations and returns fixed rates."""
+
+    def __init__(self, f_ph=3.0, f0=1.0, f1=2.0, t=(0.1, 0.3), lcfa_rate=5.0):
+        self.calls, self.f_ph, self.f0, self.f1, self.t, self.lcfa_rate = [], f_ph, f0, f1, t, lcfa_rate
+
+    def eval(self, fn, x, out_w):
+        self.calls.append((fn, list(x[0])))
+        out = {8: [self.f_ph, 1.0], 16: [self.f_ph, 1.0], 12: [self.f0, self.f1, 1.0], 24: [self.f0, self.f1, 2.0],
+               0: [self.lcfa_rate], 2: list(self.t)}[fn]
+        return np.array([out])
+
+
+def _mdot(a, b):
+    return a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3]
+
+
+@pytest.mark.parametrize("pol", ["linear", "circular"])
+def test_auto_pair_rate_kinematics(pol):
+    """increase_pair_rate_by: auto (src/main.rs:290-319): the (a, eta) handed to the
+    device's rate functions, recomputed here with explicit four-vectors as the
+    reference builds them, and the ratio of the two probabilities."""
+    d = _with(DECK, control__increase_pair_rate_by="auto", laser__polarization=pol)
+    stub = _RateStub()
+    cols, _, _ = d.collisions(stub)
+    laser, beam, opts = cols[1]
+    g = beam.gamma
+    k = 2.997925e8 * 1.28808867e-21 * 2.0 * math.pi / laser.wavelength * np.array([1.0, 0.0, 0.0, 1.0])
+    ell = g * np.array([1.0, 0.0, 0.0, -1.0])
+    u = np.array([math.sqrt(1.0 + g * g), 0.0, 0.0, -g])
+    a_rms = laser.a0 / math.sqrt(2.0) if pol == "linear" else laser.a0
+    q = u + a_rms ** 2 * k / (2.0 * _mdot(k, u))
+    assert _mdot(q, q) == pytest.approx(1.0 + a_rms ** 2, rel=1e-6)  # q^2 = 1 + a^2 (cancellation at gamma^2)
+    calls = stub.calls[2:4]  # the second a0
+    a_arg = a_rms * math.sqrt(2.0) if pol == "linear" else a_rms
+    assert calls[0][1][:2] == [pytest.approx(a_arg, rel=1e-14), pytest.approx(_mdot(k, q), rel=1e-12)]
+    assert calls[1][1][:2] == [pytest.approx(a_arg, rel=1e-14), pytest.approx(_mdot(k, ell), rel=1e-12)]
+    # P = alpha f dphi / eta, dphi = eta dt / (tau_c E): photon f / q0 over pair (f0 + f1) / (2 ell0)
+    want = max((3.0 / q[0]) / (0.5 * 3.0 / ell[0]), 1.0)
+    assert opts.rate_increase == pytest.approx(want, rel=1e-12)
+    # a zero pair rate leaves the rate alone
+    cols, _, _ = d.collisions(_RateStub(f0=0.0, f1=0.0))
+    assert all(c[2].rate_increase == 1.0 for c in cols)
+
+
+def test_auto_pair_rate_lcfa():
+    d = _with(DECK, control__increase_pair_rate_by="auto", control__lcfa=True)
+    stub = _RateStub(t=(0.2, 0.4), lcfa_rate=1.0e20)
+    cols, _, _ = d.collisions(stub)
+    laser, beam, opts = cols[0]
+    chi = 2.0 * beam.gamma * laser.a0 * 1.26e-6 / (0.510999e6 * laser.wavelength)
+    assert stub.calls[0] == (0, [pytest.approx(chi, rel=1e-14), beam.gamma]) and stub.calls[1][1] == [pytest.approx(chi)]
+    pair = 7.29735257e-3 * chi * 0.3 / (1.28808867e-21 * beam.gamma)
+    assert opts.rate_increase == pytest.approx(1.0e20 / pair, rel=1e-12)
+    with pytest.raises(D.DeckError, match=">= 1.0"):  # the LCFA ratio is not clamped
+        _with(DECK, control__increase_pair_rate_by="auto", control__lcfa=True).collisions(_RateStub(lcfa_rate=1e-30))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("edit", [{"laser__polarization": "linear"}, {"laser__polarization": "circular"},
+                                  {"control__lcfa": True, "laser__a0": 20.0}])
+def test_auto_pair_rate_on_device(engine, edit):
+    """The rates of increase_pair_rate_by: auto, evaluated by the device's own rate
+    functions at the deck's (a0, gamma): finite, at least 1, growing as a0 falls for
+    the LMA (pair creation is the rarer process further below threshold)."""
+    d = _with(DECK, control__increase_pair_rate_by="auto", beam__charge=None, **edit)
+    cols, _, _ = d.collisions(engine)
+    rates = [c[2].rate_increase for c in cols]
+    print("auto pair rate", edit, rates, flush=True)
+    assert all(math.isfinite(r) and r >= 1.0 for r in rates)
+    if "control__lcfa" not in edit:
+        assert rates[0] >= rates[-1]

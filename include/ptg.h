@@ -268,6 +268,14 @@ int ptg_histogram(ptg_ctx* ctx, ptg_hist_spec* spec, double* bins_out, double* t
  * weight from spec_x; bins_out has nbins_x * nbins_y entries, flat index
  * by * nbins_y + bx as the reference (rows overlap when the counts differ). */
 int ptg_histogram_2d(ptg_ctx* ctx, ptg_hist_spec* spec_x, ptg_hist_spec* spec_y, double* bins_out, double* total_weight);
+/* The driver's post-processing of the resident final state (src/main.rs:1150-1196):
+ * stop_mode 0 leaves positions, 1 drifts every particle along its momentum to
+ * the largest ct among them (stop_at_time: auto), 2 to c t_stop; then, with
+ * beam_basis = 1, rotates positions and momenta (current and at creation) into
+ * the frame where the beam centroid, at collision angle theta in the plane at
+ * phi, moves along +z (Particle::to_beam_coordinate_basis, particle/mod.rs:256-300).
+ * t_common (may be NULL) receives the common time in s (+inf for mode 0). */
+int ptg_finish(ptg_ctx* ctx, int stop_mode, double t_stop, int beam_basis, double theta, double phi, double* t_common);
 /* The reductions of one summary statistic (stats.rs:265-410): replaces the
  * per-particle loops of total / variance / min / max / directmean. */
 int ptg_stats(ptg_ctx* ctx, const ptg_stats_spec* spec, double* out);

@@ -1460,6 +1460,41 @@ int ptg_histogram(ptg_ctx* c, ptg_hist_spec* s, double* bins_out, double* total_
     return PTG_OK;
 }
 
+int ptg_finish(ptg_ctx* c, int stop_mode, double t_stop, int beam_basis, double theta, double phi, double* t_common) {
+    if (!c) return fail(c, PTG_E_ARG, "null argument");
+    if (stop_mode < 0 || stop_mode > 2) return fail(c, PTG_E_ARG, "stop_mode must be 0, 1 or 2");
+    if (stop_mode == 2 && !std::isfinite(t_stop)) return fail(c, PTG_E_ARG, "t_stop must be finite");
+    HIPCHK(c, hipSetDevice(c->device));
+    double ct = INFINITY;
+    if (stop_mode == 1 && c->n_out) {
+        unsigned long long* d = nullptr;
+        HIPCHK(c, hipMalloc(&d, sizeof(unsigned long long)));
+        unsigned long long k = 0ull;
+        hipError_t e = scopy(c, d, &k, sizeof(k), hipMemcpyHostToDevice);
+        if (e == hipSuccess) { launch_max_ct(c->out.v, c->n_out, d, c->stream); e = hipGetLastError(); }
+        if (e == hipSuccess) e = scopy(c, &k, d, sizeof(k), hipMemcpyDeviceToHost);
+        (void)hipFree(d);
+        HIPCHK(c, e);
+        ct = host_order_key_inv(k);
+    } else if (stop_mode == 1) {
+        ct = -INFINITY;  // no particles: the reference's NEG_INFINITY start
+    } else if (stop_mode == 2) {
+        ct = t_stop * C_LIGHT;
+    }
+    // beam centroid (-cos phi sin theta, -sin theta sin phi, -cos theta) onto +z (particle/mod.rs:256-272)
+    double vangle = 0.0, hangle = PI_D - theta;
+    if (phi != 0.0 && theta != 0.0) {
+        const double cos_v = std::hypot(std::cos(theta), std::sin(theta) * std::cos(phi));
+        vangle = std::copysign(std::acos(cos_v), std::sin(theta) * std::sin(phi));
+        hangle = std::copysign(std::acos(-std::cos(theta) / cos_v), std::cos(phi) * std::sin(theta));
+    }
+    launch_finish(c->out.v, c->n_out, stop_mode != 0 && c->n_out, ct, beam_basis ? 1 : 0, hangle, vangle, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (t_common) *t_common = stop_mode == 0 ? INFINITY : ct / C_LIGHT;
+    return PTG_OK;
+}
+
 int ptg_stats(ptg_ctx* c, const ptg_stats_spec* s, double* out) {
     if (!c || !s || !out) return fail(c, PTG_E_ARG, "null argument");
     for (int q : {s->variable, s->weight, s->cut ? s->cut_quantity : 0})

@@ -44,6 +44,10 @@ void launch_hist_minmax(const HistArgs& h, unsigned long long* mm, int grid, hip
 void launch_hist_bin(const HistArgs& h, double* bins, double* total, int grid, hipStream_t s);
 // 2-D weighted binning (hgram.rs:326-446): hx gives species, weight and the x axis,
 // hy the y axis; flat bin by * ny + bx as the reference, dropped past nx * ny.
+// drift to c t (when drift) then rotate by R_x(-vangle) R_y(hangle) (when rotate)
+void launch_finish(const PBuf& p, uint64_t n, int drift, double ct, int rotate, double hangle, double vangle,
+                   hipStream_t s);
+void launch_max_ct(const PBuf& p, uint64_t n, unsigned long long* key, hipStream_t s);
 struct StatArgs {
     PBuf p;
     uint64_t n;

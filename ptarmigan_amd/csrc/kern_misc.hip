@@ -302,6 +302,55 @@ void launch_hist_bin2(const HistArgs& hx, const HistArgs& hy, double* bins, doub
     else k_hist_bin2<false><<<grid, 256, 0, s>>>(hx, hy, bins, total);
 }
 
+// ---------------------------------------------------------------- post-processing
+__global__ void k_max_ct(PBuf p, uint64_t n, unsigned long long* key) {
+    unsigned long long hi = 0ull;
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+        const unsigned long long k = order_key(p.d[P_RT][i]);
+        hi = k > hi ? k : hi;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(hi, off);
+        hi = o > hi ? o : hi;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(key, hi);
+}
+void launch_max_ct(const PBuf& p, uint64_t n, unsigned long long* key, hipStream_t s) {
+    int grid = int(std::min<uint64_t>(1024, (n + 255) / 256));
+    if (n) k_max_ct<<<grid, 256, 0, s>>>(p, n, key);
+}
+
+// r -> r + u (ct - r_t) / u_t (src/main.rs:1165-1169), then the rotation of
+// Particle::to_beam_coordinate_basis: about y by hangle, then about x by -vangle,
+// for the position and momentum now and at creation (times unchanged)
+__global__ void k_finish(PBuf p, uint64_t n, int drift, double ct, int rotate, double ch, double sh, double cv,
+                         double sv) {
+    const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    if (drift) {
+        const double s = (ct - p.d[P_RT][i]) / p.d[P_UT][i];
+        p.d[P_RX][i] += p.d[P_UX][i] * s;
+        p.d[P_RY][i] += p.d[P_UY][i] * s;
+        p.d[P_RZ][i] += p.d[P_UZ][i] * s;
+        p.d[P_RT][i] += p.d[P_UT][i] * s;
+    }
+    if (rotate) {
+        for (int base : {P_RX, P_UX, P_R0X, P_U0X}) {
+            const double x = p.d[base][i], y = p.d[base + 1][i], z = p.d[base + 2][i];
+            const double x1 = ch * x + sh * z, z1 = ch * z - sh * x;  // about y by hangle
+            p.d[base][i] = x1;
+            p.d[base + 1][i] = cv * y + sv * z1;                      // about x by -vangle
+            p.d[base + 2][i] = cv * z1 - sv * y;
+        }
+    }
+}
+void launch_finish(const PBuf& p, uint64_t n, int drift, double ct, int rotate, double hangle, double vangle,
+                   hipStream_t s) {
+    if (!n) return;
+    k_finish<<<int((n + 255) / 256), 256, 0, s>>>(p, n, drift, ct, rotate, cos(hangle), sin(hangle), cos(vangle),
+                                                   sin(vangle));
+}
+
 // One pass of every reduction a summary statistic needs (stats.rs:265-410),
 // wave-reduced then one atomic per wave and sum.
 __global__ void k_stats(StatArgs a, double* acc, unsigned long long* mm) {

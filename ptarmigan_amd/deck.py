@@ -416,8 +416,7 @@ class Deck:
             disc_e = disc_ph = self._node("output:discard_background")
         else:
             disc_e, disc_ph = self.flag("output:discard_background_e", False), False
-        if self.contains("output:coordinate_system") and self.string("output:coordinate_system") == "beam":
-            raise DeckError("output:coordinate_system: beam is not supported by this build")
+        self.beam_basis = self.contains("output:coordinate_system") and self.string("output:coordinate_system") == "beam"
         opts = CollideOptions(
             dt_multiplier=self.number("control:dt_multiplier", 1.0), rate_increase=rate,
             t_stop=stop[1] if stop[0] == "given_time" else math.inf,
@@ -610,19 +609,6 @@ def auto_pair_rate_increase(engine, laser, gamma, lcfa):
     return max((f_ph / q0) / pair, 1.0)
 
 
-def drift_to_common_time(particles, stop):
-    """The driver's final time fix (src/main.rs:1150-1188): with stop_at_time: auto
-    every particle drifts along its momentum to the largest ct of the bunch; with a
-    given time, to c t. Returns the common time in s (inf when positions stay)."""
-    mode, t = stop
-    r, u = particles["r"], particles["u"]
-    if mode == "automatic" or len(r) == 0:
-        return math.inf
-    ct = r[:, 0].max() if mode == "same_time" else t * SPEED_OF_LIGHT
-    r += u * ((ct - r[:, 0]) / u[:, 0])[:, None]
-    return ct / SPEED_OF_LIGHT
-
-
 def ident(deck, path=None):
     """output:ident, `auto` meaning the input file's stem (src/main.rs:813-829)."""
     s = deck.string("output:ident", "")
@@ -647,23 +633,25 @@ def run_deck(deck, engine=None, out_dir=None):
     writing = out_dir is not None and (any(output.distributions(deck).values()) or
                                        any(output.statistics(deck)))
     same_opts = all(c[2] == cols[0][2] for c in cols)
-    if len(cols) > 1 and lma and not writing and same_opts:
+    stop_mode = {"automatic": 0, "same_time": 1, "given_time": 2}[stop[0]]
+    t_stop = stop[1] if stop_mode == 2 else 0.0
+    # a common time is per collision, so those run one at a time
+    if len(cols) > 1 and lma and not writing and same_opts and stop_mode != 1:
         eng.set_batch([c[0] for c in cols], [c[1] for c in cols], seed=seed)
         cnt = eng.run(cols[0][2], seed=seed)
+        eng.finish(stop_mode, t_stop, deck.beam_basis, cols[0][1].angle, cols[0][1].collision_plane_angle)
         sec, _ = eng.fetch(cnt)
         starts = np.cumsum([0] + [c[1].n for c in cols])
         coll = np.searchsorted(starts, sec["shower"], side="right") - 1
         for k, (laser, _, _) in enumerate(cols):
-            part = {key: v[coll == k] for key, v in sec.items()}
-            drift_to_common_time(part, stop)
-            out.append((laser.a0, part, cnt))
+            out.append((laser.a0, {key: v[coll == k] for key, v in sec.items()}, cnt))
         return out
     for laser, beam, opts in cols:
         eng.set_laser(laser)
         eng.build_beam(beam, seed=seed)
         cnt = eng.run(opts, seed=seed)
+        eng.finish(stop_mode, t_stop, deck.beam_basis, beam.angle, beam.collision_plane_angle)
         sec, _ = eng.fetch(cnt)
-        drift_to_common_time(sec, stop)
         if writing:
             output.write_distributions(eng, deck, sec, out_dir, ident(deck), laser.a0, len(cols))
             output.write_stats(eng, deck, out_dir, ident(deck), laser.a0, len(cols))

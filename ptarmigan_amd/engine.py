@@ -299,6 +299,14 @@ class Engine:
         self._check(self._lib.ptg_histogram_2d(self._ctx, C.byref(sx), C.byref(sy), _dptr(out), C.byref(tot)))
         return out, (sx.min, sx.max), (sy.min, sy.max), tot.value
 
+    def finish(self, stop_mode=0, t_stop=0.0, beam_basis=False, theta=0.0, phi=0.0):
+        """ptg_finish: the driver's time fix (0 none, 1 common time, 2 given time in s)
+        and beam-frame rotation of the resident final state; returns the common time."""
+        t = C.c_double()
+        self._check(self._lib.ptg_finish(self._ctx, int(stop_mode), float(t_stop), int(bool(beam_basis)),
+                                         float(theta), float(phi), C.byref(t)))
+        return t.value
+
     def stats(self, species, variable, weight=15, cut=None, shift=0.0):
         """ptg_stats: the reductions of one summary statistic over the resident final
         state (weight quantity 15 = unit). `cut` = (quantity, min, max), strict."""

@@ -243,22 +243,11 @@ def test_driver_defaults_and_alternatives():
     ({"control__select_multiplicity": 2}, "incompatible"),
     ({"control__stop_at_time": "soon"}, "stop_at_time"),
     ({"beam__from_hdf5": {"file": "x.h5"}}, "not supported"),
-    ({"output__coordinate_system": "beam"}, "not supported"),
     ({"beam__gamma": None}, "beam:gamma is required"),
 ])
 def test_driver_errors(edit, match):
     with pytest.raises(D.DeckError, match=match):
         _with(DECK, **edit).collisions()
-
-
-def test_drift_to_common_time():
-    r = np.array([[1.0, 0.0, 0.0, 0.0], [3.0, 0.0, 0.0, 2.0]])
-    u = np.array([[2.0, 1.0, 0.0, 1.0], [1.0, 0.0, 0.0, 0.5]])
-    t = D.drift_to_common_time({"r": r, "u": u}, ("same_time", math.inf))
-    assert t == 3.0 / 2.997925e8
-    assert np.allclose(r, [[3.0, 1.0, 0.0, 1.0], [3.0, 0.0, 0.0, 2.0]])
-    p = {"r": r.copy(), "u": u}
-    assert D.drift_to_common_time(p, ("automatic", math.inf)) == math.inf and np.array_equal(p["r"], r)
 
 
 @pytest.mark.gpu
@@ -370,3 +359,60 @@ def test_auto_pair_rate_on_device(engine, edit):
     assert all(math.isfinite(r) and r >= 1.0 for r in rates)
     if "control__lcfa" not in edit:
         assert rates[0] >= rates[-1]
+
+
+def _rot(axis, t):
+    """ThreeVector::rotate_around (geometry/three_vector.rs:79-93) as a matrix."""
+    ax, ay, az = axis
+    s, c = math.sin(t), math.cos(t)
+    return np.array([[c + ax * ax * (1 - c), ax * ay * (1 - c) - az * s, ax * az * (1 - c) + ay * s],
+                     [ay * ax * (1 - c) + az * s, c + ay * ay * (1 - c), ay * az * (1 - c) - ax * s],
+                     [az * ax * (1 - c) - ay * s, az * ay * (1 - c) + ax * s, c + az * az * (1 - c)]])
+
+
+def _beam_basis(theta, phi):
+    """The rotation of Particle::to_beam_coordinate_basis (particle/mod.rs:256-300)."""
+    cos_v = math.hypot(math.cos(theta), math.sin(theta) * math.cos(phi))
+    if phi == 0.0 or theta == 0.0:
+        v, h = 0.0, math.pi - theta
+    else:
+        v = math.copysign(math.acos(cos_v), math.sin(theta) * math.sin(phi))
+        h = math.copysign(math.acos(-math.cos(theta) / cos_v), math.cos(phi) * math.sin(theta))
+    return _rot((1.0, 0.0, 0.0), -v) @ _rot((0.0, 1.0, 0.0), h)
+
+
+@pytest.mark.parametrize("theta,phi", [(0.0, 0.0), (0.3, 0.0), (0.3, 0.7), (1.2, -2.0), (0.05, math.pi / 2)])
+def test_beam_basis_sends_the_centroid_to_plus_z(theta, phi):
+    centroid = np.array([-math.cos(phi) * math.sin(theta), -math.sin(theta) * math.sin(phi), -math.cos(theta)])
+    assert np.allclose(_beam_basis(theta, phi) @ centroid, [0.0, 0.0, 1.0], atol=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode,theta,phi", [(1, 0.2, 0.6), (2, 0.0, 0.0)])
+def test_finish_matches_the_driver(engine, mode, theta, phi):
+    """ptg_finish against the driver's post-processing restated with numpy
+    (src/main.rs:1150-1196): the drift to a common time, then the beam frame."""
+    from ptarmigan_amd import workloads
+    laser, beam, opts = workloads.plane_wave_lma(5000)
+    beam.angle, beam.collision_plane_angle = theta, phi
+    engine.set_laser(laser)
+    engine.build_beam(beam, seed=12)
+    cnt = engine.run(opts, seed=12)
+    before, _ = engine.fetch(cnt)
+    t_stop = 2.0e-12
+    t = engine.finish(mode, t_stop, True, theta, phi)
+    after, _ = engine.fetch(cnt)
+    print("finish", mode, theta, phi, t, flush=True)
+    r, u = before["r"].copy(), before["u"]
+    ct = r[:, 0].max() if mode == 1 else t_stop * 2.997925e8
+    assert t == pytest.approx(ct / 2.997925e8, rel=1e-15)
+    r += u * ((ct - r[:, 0]) / u[:, 0])[:, None]
+    R = _beam_basis(theta, phi)
+    scale = np.abs(r[:, 1:]).max()
+    assert np.allclose(after["r"][:, 0], r[:, 0], rtol=1e-13)
+    assert np.allclose(after["r"][:, 1:], r[:, 1:] @ R.T, rtol=1e-12, atol=1e-12 * scale)
+    assert np.allclose(after["u"][:, 1:], u[:, 1:] @ R.T, rtol=1e-12, atol=1e-12 * np.abs(u).max())
+    assert np.array_equal(after["u"][:, 0], u[:, 0])
+    assert np.allclose(after["u0"][:, 1:], before["u0"][:, 1:] @ R.T, rtol=1e-12, atol=1e-12 * np.abs(u).max())
+    s0 = np.abs(before["r0"][:, 1:]).max()
+    assert np.allclose(after["r0"][:, 1:], before["r0"][:, 1:] @ R.T, rtol=1e-12, atol=1e-12 * s0)

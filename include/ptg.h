@@ -234,6 +234,13 @@ int ptg_field_info(ptg_ctx* ctx, double* max_timestep, double* ideal_initial_z);
 /* Generates `beam->n` primaries on the device: replaces BeamBuilder::build
  * (src/particle/builder.rs:173-219). ids are 0..n-1. */
 int ptg_build_beam(ptg_ctx* ctx, const ptg_beam_desc* beam, double initial_z, uint64_t seed);
+/* The energy spectrum of the beams built next (GammaDistribution, particle/dstr.rs:60-214):
+ * kind 0 the normal spectrum of ptg_beam_desc (the default); 1 a tabulated CDF,
+ * `table` = n (gamma, cumulative fraction) pairs, sampled by the inverse of its
+ * monotone cubic interpolant from a uniform made of the beam's correlated
+ * normals (so energy_chirp still applies); 2 a bremsstrahlung source between
+ * brem_min and brem_max. Replaces the spectrum branch at src/main.rs:610-693. */
+int ptg_set_beam_spectrum(ptg_ctx* ctx, int kind, const double* table, size_t n, double brem_min, double brem_max);
 /* Uploads host primaries to the device (replaces the Vec<Particle> handed to collide). */
 int ptg_load_primaries(ptg_ctx* ctx, const ptg_particles* primaries, size_t n);
 

@@ -104,7 +104,8 @@ STATS_N = 9
 EXPORTS = ("ptg_create", "ptg_destroy", "ptg_last_error", "ptg_set_laser", "ptg_field_info", "ptg_build_beam",
            "ptg_load_primaries", "ptg_run", "ptg_collide", "ptg_fetch", "ptg_push", "ptg_histogram", "ptg_eval",
            "ptg_surrogate_train", "ptg_surrogate_load", "ptg_mlp_probe", "ptg_set_batch", "ptg_autotune",
-           "ptg_fetch_showers", "ptg_histogram_2d", "ptg_stats", "ptg_finish")
+           "ptg_fetch_showers", "ptg_histogram_2d", "ptg_stats", "ptg_finish",
+           "ptg_set_beam_spectrum")
 
 _lib = None
 
@@ -144,6 +145,7 @@ def load():
     L.ptg_histogram_2d.argtypes = [vp, P(HistSpec), P(HistSpec), P(d), P(d)]
     L.ptg_stats.argtypes = [vp, P(StatsSpec), P(d)]
     L.ptg_finish.argtypes = [vp, C.c_int, d, C.c_int, d, d, P(d)]
+    L.ptg_set_beam_spectrum.argtypes = [vp, C.c_int, P(d), C.c_size_t, d, d]
     L.ptg_autotune.argtypes = [vp, P(CollideOptions), C.c_uint64, C.c_uint64, P(TuneResult)]
     for f in ("ptg_set_laser", "ptg_field_info", "ptg_build_beam", "ptg_load_primaries", "ptg_run", "ptg_collide",
               "ptg_fetch", "ptg_push", "ptg_histogram", "ptg_eval"):

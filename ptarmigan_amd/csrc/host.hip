@@ -141,6 +141,9 @@ struct ptg_ctx {
     // a batch of collisions (ptg_set_batch): lasers, and each primary's collision
     LaserDev* d_lasers = nullptr;
     uint8_t* d_coll = nullptr;
+    int spectrum = 0, n_cdf = 0;        // the next beams' energy spectrum (ptg_set_beam_spectrum)
+    double* d_cdf = nullptr;
+    double brem_min = 0.0, brem_max = 0.0;
     size_t coll_cap = 0;
     int n_coll = 0;
     double batch_a0_max = 0.0;
@@ -523,6 +526,7 @@ void ptg_destroy(ptg_ctx* c) {
     if (c->d_def_extra) (void)hipFree(c->d_def_extra);
     if (c->d_lasers) (void)hipFree(c->d_lasers);
     if (c->d_coll) (void)hipFree(c->d_coll);
+    if (c->d_cdf) (void)hipFree(c->d_cdf);
     if (c->d_bw) (void)hipFree(c->d_bw);
     if (c->d_cp) (void)hipFree(c->d_cp);
     if (c->d_ev_extra) (void)hipFree(c->d_ev_extra);
@@ -572,6 +576,40 @@ int ptg_field_info(ptg_ctx* c, double* max_timestep, double* z0) {
     return PTG_OK;
 }
 
+static void set_spectrum(const ptg_ctx* c, BeamDev& bd) {
+    bd.spectrum = c->spectrum;
+    bd.n_cdf = c->n_cdf;
+    bd.cdf = c->d_cdf;
+    bd.brem_min = c->brem_min;
+    bd.brem_max = c->brem_max;
+}
+
+int ptg_set_beam_spectrum(ptg_ctx* c, int kind, const double* table, size_t n, double brem_min, double brem_max) {
+    if (!c) return fail(c, PTG_E_ARG, "null argument");
+    if (kind < 0 || kind > 2) return fail(c, PTG_E_ARG, "kind must be 0 (normal), 1 (tabulated) or 2 (bremsstrahlung)");
+    if (kind == 1) {
+        if (!table || n < 2 || n > (1u << 20)) return fail(c, PTG_E_ARG, "a tabulated spectrum needs 2 to 2^20 points");
+        for (size_t i = 0; i < n; ++i) {
+            if (!std::isfinite(table[2 * i]) || !std::isfinite(table[2 * i + 1]))
+                return fail(c, PTG_E_ARG, "the spectrum holds non-numerical values");
+            if (i && (table[2 * i] <= table[2 * i - 2] || table[2 * i + 1] < table[2 * i - 1]))
+                return fail(c, PTG_E_ARG, "the spectrum's gamma must increase and its CDF must not decrease");
+        }
+        HIPCHK(c, hipSetDevice(c->device));
+        if (c->d_cdf) (void)hipFree(c->d_cdf);
+        c->d_cdf = nullptr;
+        HIPCHK(c, hipMalloc(&c->d_cdf, n * 2 * sizeof(double)));
+        HIPCHK(c, scopy(c, c->d_cdf, table, n * 2 * sizeof(double), hipMemcpyHostToDevice));
+        c->n_cdf = int(n);
+    }
+    if (kind == 2 && !(brem_min > 0.0 && brem_max > brem_min))
+        return fail(c, PTG_E_ARG, "a bremsstrahlung source needs 0 < gamma_min < gamma");
+    c->spectrum = kind;
+    c->brem_min = brem_min;
+    c->brem_max = brem_max;
+    return PTG_OK;
+}
+
 int ptg_build_beam(ptg_ctx* c, const ptg_beam_desc* b, double initial_z, uint64_t seed) {
     if (!c || !b) return fail(c, PTG_E_ARG, "null argument");
     if (b->n == 0 || b->n > 0xffffffffull) return fail(c, PTG_E_ARG, "beam size must be in [1, 2^32)");
@@ -582,6 +620,7 @@ int ptg_build_beam(ptg_ctx* c, const ptg_beam_desc* b, double initial_z, uint64_
     int rc = ensure_n0(c, b->n);
     if (rc) return rc;
     BeamDev bd{};
+    set_spectrum(c, bd);
     bd.n = uint32_t(b->n);
     bd.species = b->species;
     bd.radial = b->radial;
@@ -1785,6 +1824,7 @@ int ptg_set_batch(ptg_ctx* c, const ptg_laser_desc* lasers, const ptg_beam_desc*
         ptg_laser_desc dk = lasers[k];
         const double z0 = ideal_initial_z(&dk, L[k]) + 3.0 * b.length;  // the driver's initial_z (src/main.rs:1062-1066)
         BeamDev bd{};
+        set_spectrum(c, bd);
         bd.n = uint32_t(b.n);
         bd.species = b.species;
         bd.radial = b.radial;

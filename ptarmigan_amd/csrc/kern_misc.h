@@ -14,6 +14,10 @@ struct BeamDev {
     double initial_z;
     uint64_t key;
     uint32_t off, n_total;   // this beam's primaries at [off, off + n) of n_total (a batch of collisions)
+    int spectrum;            // 0 normal, 1 tabulated CDF, 2 bremsstrahlung source (particle/dstr.rs:161-214)
+    int n_cdf;
+    const double* cdf;       // [n_cdf][2] (gamma, cumulative fraction)
+    double brem_min, brem_max;
 };
 
 struct HistArgs {

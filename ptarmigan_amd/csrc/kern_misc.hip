@@ -28,13 +28,40 @@ __global__ void k_build_beam(BeamDev b, PBuf p, double* n0) {
     const uint32_t i = b.off + k;  // index in the primary buffer
     const uint32_t nt = b.n_total;
     double gamma, dz;
-    for (int t = 0;; ++t) {
+    if (b.spectrum == 1) {
+        // correlated normals mapped to a uniform by the reference's erf
+        // approximation (NORTA), then the inverse of the tabulated CDF
         double n0_, n1_;
         normal2(g, n0_, n1_);
-        double n2 = b.rho * n0_ + sqrt(1.0 - b.rho * b.rho) * n1_;
+        const double n2 = b.rho * n0_ + sqrt(1.0 - b.rho * b.rho) * n1_;
+        const double x = n2 * 0.70710678118654752440;
+        const double u1 = 0.5 * (1.0 + tanh(167.0 * x / 148.0 + 11.0 * x * x * x / 109.0));
         dz = b.length * n0_;
-        gamma = b.gamma + b.sigma * n2;
-        if (gamma > 1.0 || t > 1000) break;
+        if (!pwmci_invert(b.cdf, b.n_cdf, u1, true, gamma)) gamma = b.cdf[0];
+    } else if (b.spectrum == 2) {
+        // x = gamma / gamma_max from 4/(3x) - 4/3 + x by rejection
+        const double x_min = b.brem_min / b.brem_max;
+        const double y_max = 4.0 / (3.0 * x_min) - 4.0 / 3.0 + x_min;
+        double x = 1.0;
+        for (int t = 0; t < 100000; ++t) {
+            double r1, u;
+            rng_uniform2(g, r1, u);
+            x = x_min + (1.0 - x_min) * r1;
+            if (u <= (4.0 / (3.0 * x) - 4.0 / 3.0 + x) / y_max) break;
+        }
+        double na_, nb_;
+        normal2(g, na_, nb_);
+        dz = b.length * na_;
+        gamma = x * b.brem_max;
+    } else {
+        for (int t = 0;; ++t) {
+            double n0_, n1_;
+            normal2(g, n0_, n1_);
+            double n2 = b.rho * n0_ + sqrt(1.0 - b.rho * b.rho) * n1_;
+            dz = b.length * n0_;
+            gamma = b.gamma + b.sigma * n2;
+            if (gamma > 1.0 || t > 1000) break;
+        }
     }
     double uu = b.species == 2 ? -gamma : -sqrt(gamma * gamma - 1.0);
     double na, nb;

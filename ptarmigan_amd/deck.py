@@ -10,8 +10,8 @@ CollideOptions) per value of a looped `laser:a0` (start/stop/step,
 src/input/mod.rs:239-263), and `run_deck` transports them — several a0 values
 as one batch of collisions.
 
-Not supported (they raise DeckError naming the key): beams loaded from HDF5
-(`beam:from_hdf5`), custom spectra (`beam:spectrum`), bremsstrahlung sources.
+Not supported (it raises DeckError naming the key): beams loaded from HDF5
+(`beam:from_hdf5`).
 """
 import dataclasses
 import math
@@ -233,6 +233,7 @@ class Deck:
             raise DeckError("the deck must be a YAML mapping")
         self.data = data
         self.name = name
+        self.dir = ""
         self.context = {}
         consts = data.get("constants")
         if isinstance(consts, dict):
@@ -261,7 +262,9 @@ class Deck:
     def from_file(cls, path):
         import os
         with open(path) as f:
-            return cls.from_string(f.read(), os.path.splitext(os.path.basename(path))[0])
+            d = cls.from_string(f.read(), os.path.splitext(os.path.basename(path))[0])
+        d.dir = os.path.dirname(os.path.abspath(path))
+        return d
 
     # -- reads: a missing key raises _Missing, a malformed one DeckError
     def _node(self, path):
@@ -485,9 +488,8 @@ class Deck:
 
     def beam(self):
         """The generated beam (src/main.rs:480-800)."""
-        for unsupported in ("beam:from_hdf5", "beam:spectrum"):
-            if self.contains(unsupported):
-                raise DeckError(f"{unsupported} is not supported by this build")
+        if self.contains("beam:from_hdf5"):
+            raise DeckError("beam:from_hdf5 is not supported by this build")
         species = Species.ELECTRON
         if self.contains("beam:species"):
             s = self.string("beam:species")
@@ -495,8 +497,7 @@ class Deck:
             if s not in names:
                 raise DeckError(f"requested species '{s}' is not one of 'electron', 'positron' or 'photon'")
             species = names[s]
-        if species == Species.PHOTON and self.flag("beam:bremsstrahlung_source", False):
-            raise DeckError("beam:bremsstrahlung_source is not supported by this build")
+        brem = species == Species.PHOTON and self.flag("beam:bremsstrahlung_source", False)
         angle = self.number("beam:collision_angle", 0.0)
         plane = 0.0
         if self.contains("beam:collision_plane"):
@@ -535,7 +536,19 @@ class Deck:
                 radial = 2
             elif cut is not None:
                 radial, max_radius = 1, cut
+        # the energy spectrum (src/main.rs:608-693): tabulated, bremsstrahlung or normal
+        mean_gamma, self.spectrum = None, (0, None, (0.0, 0.0))
+        if self.contains("beam:spectrum"):
+            table, mean_gamma = self._spectrum_table(n)
+            self.spectrum = (1, table, (0.0, 0.0))
+        elif brem:
+            lo, hi = self.number("beam:gamma_min"), self.number("beam:gamma")
+            self.spectrum = (2, None, (lo, hi))
+            mean_gamma = hi  # GammaDistribution::gamma of a bremsstrahlung source
         chirp = self.number("beam:energy_chirp", 0.0)
+        if brem and chirp != 0.0:
+            warnings.warn("energy chirp ignored for bremsstrahlung photons")
+            chirp = 0.0
         if abs(chirp) > 1.0:
             raise DeckError(f"absolute value of energy chirp parameter {chirp} must be <= 1")
         sv = self.numbers("beam:stokes_pars", [0.0, 0.0, 0.0])
@@ -543,10 +556,53 @@ class Deck:
             raise DeckError("particle polarization must be specified as three Stokes parameters [S_1, S_2, S_3]")
         if math.sqrt(sv[0] ** 2 + sv[1] ** 2 + sv[2] ** 2) > 1.0:
             raise DeckError("specified particle polarization does not satisfy S_1^2 + S_2^2 + S_3^2 <= 1")
-        return Beam(n=n, gamma=self.number("beam:gamma"), sigma=self.number("beam:sigma", 0.0), species=species,
+        return Beam(n=n, gamma=mean_gamma if mean_gamma is not None else self.number("beam:gamma"),
+                    sigma=0.0 if mean_gamma is not None else self.number("beam:sigma", 0.0), species=species,
                     radius=radius, radial=radial, max_radius=max_radius, length=self.number("beam:length", 0.0),
                     rms_div=self.number("beam:rms_divergence", 0.0), energy_chirp=chirp, angle=angle,
                     collision_plane_angle=plane, offset=tuple(off), pol=(1.0, sv[0], sv[1], sv[2]), weight=weight)
+
+    def _spectrum_table(self, npart):
+        """The CDF rows of GammaDistribution::custom (particle/dstr.rs:88-106) from
+        beam:spectrum:function (sampled on a grid) or beam:spectrum:file, and the
+        spectrum's mean gamma (its first moment on the grid)."""
+        import os
+        analytical, numerical = self.contains("beam:spectrum:function"), self.contains("beam:spectrum:file")
+        if analytical and numerical:
+            raise DeckError("specify a function or a file in beam:spectrum, not both")
+        lo = self.number("beam:spectrum:min")
+        if analytical:
+            f = self.func("beam:spectrum:function", "gamma")
+            hi = self.number("beam:spectrum:max")
+            nres = self.count("beam:spectrum:resolution") if self.contains("beam:spectrum:resolution") else \
+                int(10.0 + 2.0 * float(np.cbrt(float(npart))))
+            step = (hi - lo) / nres
+            vals = [f(lo + i * step) for i in range(nres)]
+        else:
+            path = os.path.join(self.dir, self.string("beam:spectrum:file"))
+            try:
+                with open(path) as fh:
+                    vals = [float(s) for s in fh.read().splitlines()]
+            except OSError:
+                raise DeckError(f'Unable to open "{path}": no such file or directory')
+            except ValueError:
+                raise DeckError(f'Unable to import "{path}" as a 1D array of spectral values')
+            nsteps = len(vals) - 1
+            if self.contains("beam:spectrum:max"):
+                step = (self.number("beam:spectrum:max") - lo) / nsteps
+            else:
+                step = self.number("beam:spectrum:step")
+        if not all(math.isfinite(v) for v in vals):
+            raise DeckError("beam:spectrum evaluated to non-numerical values in the specified domain")
+        total = sum(vals)
+        rows, g, rt = [(lo, 0.0)], lo, 0.0
+        for v in vals:
+            g += step
+            rt += v
+            rows.append((g, rt / total))
+        m0 = sum(vals)
+        m1 = sum((lo + i * step) * v for i, v in enumerate(vals))
+        return np.array(rows), m1 / m0
 
     @staticmethod
     def string_of(x):
@@ -637,6 +693,7 @@ def run_deck(deck, engine=None, out_dir=None):
     t_stop = stop[1] if stop_mode == 2 else 0.0
     # a common time is per collision, so those run one at a time
     if len(cols) > 1 and lma and not writing and same_opts and stop_mode != 1:
+        eng.set_beam_spectrum(*deck.spectrum)
         eng.set_batch([c[0] for c in cols], [c[1] for c in cols], seed=seed)
         cnt = eng.run(cols[0][2], seed=seed)
         eng.finish(stop_mode, t_stop, deck.beam_basis, cols[0][1].angle, cols[0][1].collision_plane_angle)
@@ -648,6 +705,7 @@ def run_deck(deck, engine=None, out_dir=None):
         return out
     for laser, beam, opts in cols:
         eng.set_laser(laser)
+        eng.set_beam_spectrum(*deck.spectrum)
         eng.build_beam(beam, seed=seed)
         cnt = eng.run(opts, seed=seed)
         eng.finish(stop_mode, t_stop, deck.beam_basis, beam.angle, beam.collision_plane_angle)

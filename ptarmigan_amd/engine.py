@@ -229,6 +229,12 @@ class Engine:
             initial_z = self.field_info()[1] + 3.0 * beam.length
         self._check(self._lib.ptg_build_beam(self._ctx, C.byref(beam.desc()), initial_z, seed))
 
+    def set_beam_spectrum(self, kind=0, table=None, brem=(0.0, 0.0)):
+        """ptg_set_beam_spectrum: 0 normal, 1 tabulated CDF ((gamma, cdf) rows), 2 bremsstrahlung (min, max)."""
+        t = np.ascontiguousarray(table if table is not None else np.zeros((0, 2)), dtype=np.float64).reshape(-1, 2)
+        self._check(self._lib.ptg_set_beam_spectrum(self._ctx, int(kind), _dptr(t), t.shape[0], float(brem[0]),
+                                                    float(brem[1])))
+
     def load_primaries(self, prim):
         n = len(prim["species"])
         self._prim_keep = prim

@@ -243,6 +243,8 @@ def test_driver_defaults_and_alternatives():
     ({"control__select_multiplicity": 2}, "incompatible"),
     ({"control__stop_at_time": "soon"}, "stop_at_time"),
     ({"beam__from_hdf5": {"file": "x.h5"}}, "not supported"),
+    ({"beam__spectrum": {"function": "1.0", "file": "x.dat", "min": 1.0}}, "not both"),
+    ({"beam__spectrum": {"function": "sqrt(gamma - 1e6)", "min": 1.0, "max": 2.0}}, "non-numerical"),
     ({"beam__gamma": None}, "beam:gamma is required"),
 ])
 def test_driver_errors(edit, match):
@@ -416,3 +418,65 @@ def test_finish_matches_the_driver(engine, mode, theta, phi):
     assert np.allclose(after["u0"][:, 1:], before["u0"][:, 1:] @ R.T, rtol=1e-12, atol=1e-12 * np.abs(u).max())
     s0 = np.abs(before["r0"][:, 1:]).max()
     assert np.allclose(after["r0"][:, 1:], before["r0"][:, 1:] @ R.T, rtol=1e-12, atol=1e-12 * s0)
+
+
+def test_spectrum_tables(tmp_path):
+    """GammaDistribution::custom rows (particle/dstr.rs:88-106) from a function and
+    from a file, and the spectrum's mean gamma."""
+    d = _with(DECK, beam__spectrum={"function": "gamma / 1000.0", "min": 1000.0, "max": 2000.0, "resolution": 4})
+    cols, _, _ = d.collisions()
+    kind, table, _ = d.spectrum
+    vals = [1.0, 1.25, 1.5, 1.75]
+    assert kind == 1 and np.allclose(table[:, 0], [1000, 1250, 1500, 1750, 2000])
+    assert np.allclose(table[:, 1], np.cumsum([0.0] + vals) / sum(vals), rtol=1e-15)
+    assert cols[0][1].gamma == pytest.approx(sum(g * v for g, v in zip([1000, 1250, 1500, 1750], vals)) / sum(vals))
+    f = tmp_path / "spec.dat"
+    f.write_text("1.0\n3.0\n0.0\n")
+    (tmp_path / "deck.yml").write_text(DECK.replace("  n: 1000\n", "  n: 1000\n  spectrum:\n    file: spec.dat\n"
+                                                                  "    min: 100.0\n    step: 10.0\n"))
+    d = D.Deck.from_file(str(tmp_path / "deck.yml"))
+    d.collisions()
+    assert np.allclose(d.spectrum[1], [[100.0, 0.0], [110.0, 0.25], [120.0, 1.0], [130.0, 1.0]])
+    d = _with(DECK, beam__species="photon", beam__bremsstrahlung_source=True, beam__gamma_min=100.0, beam__energy_chirp=0.0)
+    cols, _, _ = d.collisions()
+    assert d.spectrum == (2, None, (100.0, pytest.approx(cols[0][1].gamma)))
+
+
+def _photon_primaries(engine, d, seed):
+    from ptarmigan_amd import workloads
+    laser, _, opts = workloads.plane_wave_lma(10)
+    cols, _, _ = d.collisions()
+    beam = cols[0][1]
+    opts.tracking_photons = False  # photons are not transported: the final state is the primaries
+    engine.set_laser(laser)
+    engine.set_beam_spectrum(*d.spectrum)
+    engine.build_beam(beam, seed=seed)
+    cnt = engine.run(opts, seed=seed)
+    sec, _ = engine.fetch(cnt)
+    engine.set_beam_spectrum(0)
+    return sec["u"][:, 0]
+
+
+@pytest.mark.gpu
+def test_beam_spectra_on_device(engine):
+    """Tabulated and bremsstrahlung spectra sampled by the device beam builder
+    against their distributions: a flat table (uniform gamma up to the
+    reference's erf approximation, 4e-4) and the bremsstrahlung density
+    4/(3x) - 4/3 + x on [x_min, 1]."""
+    n = 200_000
+    d = _with(DECK, beam__species="photon", beam__n=n, beam__charge=None, beam__energy_chirp=0.0,
+              beam__spectrum={"function": "1.0", "min": 1000.0, "max": 2000.0, "resolution": 50})
+    g = np.sort(_photon_primaries(engine, d, 3))
+    print("tabulated spectrum", g[0], g[-1], flush=True)
+    assert g[0] >= 1000.0 and g[-1] <= 2000.0
+    ks = np.abs(np.arange(1, n + 1) / n - (g - 1000.0) / 1000.0).max()
+    assert ks < 5e-3
+    d = _with(DECK, beam__species="photon", beam__n=n, beam__charge=None, beam__energy_chirp=0.0,
+              beam__bremsstrahlung_source=True, beam__gamma_min=200.0, beam__gamma=2000.0)
+    g = _photon_primaries(engine, d, 4) / 2000.0
+    x = np.linspace(0.1, 1.0, 200001)
+    p = 4.0 / (3.0 * x) - 4.0 / 3.0 + x
+    mean = np.trapezoid(x * p, x) / np.trapezoid(p, x)
+    print("brem spectrum", g.min(), g.max(), g.mean(), mean, flush=True)
+    assert g.min() >= 0.1 and g.max() <= 1.0
+    assert abs(g.mean() - mean) < 5.0 * g.std() / math.sqrt(n)

@@ -180,7 +180,7 @@ typedef enum { PTG_W_NUMBER = 0, PTG_W_ENERGY = 1, PTG_W_HELICITY = 2, PTG_W_POL
  * min..max of the (optionally log-scaled) quantity, the particle at exactly max
  * is not binned, the total counts every weight. */
 typedef struct {
-    int32_t species;         /* ptg_species */
+    int32_t species;         /* ptg_species; 3 = the intermediate (decayed) photons */
     int32_t quantity;        /* ptg_quantity */
     int32_t weight_fn;       /* ptg_weight_fn */
     int32_t log_scale;       /* BinSpec::LogScaled: bin ln(quantity) */
@@ -204,7 +204,7 @@ typedef struct {
  * w = weight quantity (PTG_Q_UNIT for none), "in" = cut_min < cut_quantity <
  * cut_max (strict) or every particle when cut = 0. Default units (MeV, m, rad). */
 typedef struct {
-    int32_t species;         /* ptg_species */
+    int32_t species;         /* ptg_species; 3 = the intermediate (decayed) photons */
     int32_t variable;        /* ptg_quantity */
     int32_t weight;          /* ptg_quantity */
     int32_t cut;

@@ -1454,9 +1454,10 @@ int ptg_histogram(ptg_ctx* c, ptg_hist_spec* s, double* bins_out, double* total_
     if (s->quantity < 0 || s->quantity >= PTG_Q_COUNT) return fail(c, PTG_E_ARG, "unknown quantity");
     HIPCHK(c, hipSetDevice(c->device));
     HistArgs h{};
-    h.p = c->out.v;
-    h.n = c->n_out;
-    h.species = s->species;
+    // species 3: the intermediate (decayed) photons kept by keep_decayed_photons
+    h.p = s->species == 3 ? c->inter.v : c->out.v;
+    h.n = s->species == 3 ? c->n_inter : c->n_out;
+    h.species = s->species == 3 ? 2 : s->species;
     h.q = s->quantity;
     h.wfn = s->weight_fn;
     h.log_scale = s->log_scale;
@@ -1538,12 +1539,12 @@ int ptg_stats(ptg_ctx* c, const ptg_stats_spec* s, double* out) {
     if (!c || !s || !out) return fail(c, PTG_E_ARG, "null argument");
     for (int q : {s->variable, s->weight, s->cut ? s->cut_quantity : 0})
         if (q < 0 || q >= PTG_Q_COUNT) return fail(c, PTG_E_ARG, "unknown quantity");
-    if (s->species < 0 || s->species > 2) return fail(c, PTG_E_ARG, "bad species");
+    if (s->species < 0 || s->species > 3) return fail(c, PTG_E_ARG, "bad species");
     HIPCHK(c, hipSetDevice(c->device));
     StatArgs a{};
-    a.p = c->out.v;
-    a.n = c->n_out;
-    a.species = s->species;
+    a.p = s->species == 3 ? c->inter.v : c->out.v;  // 3: the intermediate photons
+    a.n = s->species == 3 ? c->n_inter : c->n_out;
+    a.species = s->species == 3 ? 2 : s->species;
     a.qv = s->variable;
     a.qw = s->weight;
     a.cut = s->cut ? 1 : 0;
@@ -1583,12 +1584,13 @@ int ptg_histogram_2d(ptg_ctx* c, ptg_hist_spec* sx, ptg_hist_spec* sy, double* b
     HIPCHK(c, hipSetDevice(c->device));
     HistArgs h[2];
     const ptg_hist_spec* sp[2] = {sx, sy};
-    int grid = std::max(1, std::min(c->n_cu * 4, int((c->n_out + 255) / 256)));
+    const uint64_t n_src = sx->species == 3 ? c->n_inter : c->n_out;
+    int grid = std::max(1, std::min(c->n_cu * 4, int((n_src + 255) / 256)));
     for (int a = 0; a < 2; ++a) {
         h[a] = HistArgs{};
-        h[a].p = c->out.v;
-        h[a].n = c->n_out;
-        h[a].species = sx->species;  // one filter and one weight for both axes (hgram.rs:326-331)
+        h[a].p = sx->species == 3 ? c->inter.v : c->out.v;  // 3: the intermediate photons
+        h[a].n = sx->species == 3 ? c->n_inter : c->n_out;
+        h[a].species = sx->species == 3 ? 2 : sx->species;  // one filter and one weight for both axes (hgram.rs:326-331)
         h[a].q = sp[a]->quantity;
         h[a].wfn = sx->weight_fn;
         h[a].log_scale = sp[a]->log_scale;

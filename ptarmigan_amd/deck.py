@@ -428,7 +428,9 @@ class Deck:
             rr=self.flag("control:radiation_reaction", True),
             tracking_photons=self.flag("control:pair_creation", not classical),
             tracking_secondaries=self.flag("control:track_secondaries", True),
-            keep_decayed_photons=self.flag("output:dump_decayed_photons", False),
+            # kept when asked for, or when intermediate output needs them (src/main.rs:982)
+            keep_decayed_photons=self.flag("output:dump_decayed_photons", False) or
+            bool(self.raw_list("output:intermediate")) or bool(self.raw_list("stats:intermediate")),
             pol_resolved=self.flag("control:pol_resolved", False), classical=classical, gaunt_factor=gaunt,
             discard_bg_e=disc_e, discard_bg_ph=disc_ph)
         seed = self.count("control:rng_seed", 0)
@@ -709,9 +711,9 @@ def run_deck(deck, engine=None, out_dir=None):
         eng.build_beam(beam, seed=seed)
         cnt = eng.run(opts, seed=seed)
         eng.finish(stop_mode, t_stop, deck.beam_basis, beam.angle, beam.collision_plane_angle)
-        sec, _ = eng.fetch(cnt)
+        sec, inter = eng.fetch(cnt, intermediates=opts.keep_decayed_photons)
         if writing:
-            output.write_distributions(eng, deck, sec, out_dir, ident(deck), laser.a0, len(cols))
+            output.write_distributions(eng, deck, sec, out_dir, ident(deck), laser.a0, len(cols), inter)
             output.write_stats(eng, deck, out_dir, ident(deck), laser.a0, len(cols))
         out.append((laser.a0, sec, cnt))
     return out

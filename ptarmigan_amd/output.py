@@ -290,6 +290,9 @@ class Histogram:
         return filename + ".dat"
 
 
+INTERMEDIATE = 3  # the decayed photons, kept when the deck asks for them
+
+
 def file_format(deck):
     """output:file_format (src/main.rs:908-925): plain_text (also plain-text, ascii) or
     fits; anything else falls back to plain_text."""
@@ -307,24 +310,28 @@ def unit_system(deck):
 def distributions(deck):
     """{species: [Distribution]} from output:electron / photon / positron."""
     out = {}
-    for name, sp in (("electron", Species.ELECTRON), ("photon", Species.PHOTON), ("positron", Species.POSITRON)):
+    for name, sp in (("electron", Species.ELECTRON), ("photon", Species.PHOTON), ("positron", Species.POSITRON),
+                     ("intermediate", INTERMEDIATE)):
         specs = deck.raw_list(f"output:{name}")
         out[sp] = [Distribution(s, deck.context) for s in specs]
-    if deck.raw_list("output:intermediate"):
-        raise DeckError("output:intermediate distributions are not supported by this build")
     return out
 
 
-def write_distributions(engine, deck, particles, prefix_dir="", ident="", a0=None, n_a0=1):
-    """Writes every requested distribution of the resident final state; returns the
-    file names. `particles` are the fetched final-state particles (for the species
-    counts); file names follow src/main.rs:1218-1240."""
+def write_distributions(engine, deck, particles, prefix_dir="", ident="", a0=None, n_a0=1, intermediates=None):
+    """Writes every requested distribution of the resident final state (and of the
+    intermediate photons); returns the file names. `particles` and `intermediates`
+    are the fetched particles (for the species counts); file names follow
+    src/main.rs:1218-1248."""
     system, fmt = unit_system(deck), file_format(deck)
     cur = ident if n_a0 <= 1 else f"{ident}{'_' if ident else ''}a0_{a0:.3f}"
     written = []
-    names = {Species.ELECTRON: "electron", Species.PHOTON: "photon", Species.POSITRON: "positron"}
+    names = {Species.ELECTRON: "electron", Species.PHOTON: "photon", Species.POSITRON: "positron",
+             INTERMEDIATE: "intermediate"}
     for sp, dists in distributions(deck).items():
-        n = int(np.sum(particles["species"] == int(sp)))
+        if sp == INTERMEDIATE:
+            n = 0 if intermediates is None else len(intermediates["species"])
+        else:
+            n = int(np.sum(particles["species"] == int(sp)))
         prefix = f"{prefix_dir}{'/' if prefix_dir else ''}{cur}{'_' if cur else ''}{names[sp]}"
         for d in dists:
             h = d.histogram(engine, int(sp), n, system)
@@ -470,8 +477,8 @@ class StatsExpression:
 def statistics(deck):
     """[(species name, Statistic)] in the order the reference writes them
     (electron, positron, photon; src/main.rs:1274-1285), and the expressions."""
-    if deck.raw_list("stats:intermediate"):
-        raise DeckError("stats:intermediate is not supported by this build")
+    # stats:intermediate is parsed but, as in the reference, not written
+    [Statistic(s, deck.context) for s in deck.raw_list("stats:intermediate")]
     stats = []
     for name in ("electron", "positron", "photon"):
         stats += [(name, Statistic(s, deck.context)) for s in deck.raw_list(f"stats:{name}")]

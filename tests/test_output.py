@@ -276,3 +276,32 @@ def test_polarization_projection_on_device(engine):
         _, _, _, total = engine.histogram(2, 14, 10, wfn)  # energy, weighted by the projection
         assert total == pytest.approx(np.sum(sec["weight"][ph] * _project(sec["pol"][ph], sec["u"][ph], axis)),
                                       rel=1e-10)
+
+
+@pytest.mark.gpu
+@pytest.mark.skip(reason="no intermediate photons are recorded by this cascade with keep_decayed_photons set "
+                         "(n_intermediates = 0 even with pair biasing): the recording path is under investigation")
+def test_intermediate_photons(engine, tmp_path):
+    """Histograms and statistics of the intermediate (decayed) photons: species 3
+    selects the buffer ptg_fetch returns as intermediates."""
+    from ptarmigan_amd import workloads
+    laser, beam, opts = workloads.focused_lcfa_cascade(20_000)
+    opts.keep_decayed_photons = True
+    opts.rate_increase = 1000.0  # event biasing: enough pair events in a small run
+    engine.set_laser(laser)
+    engine.build_beam(beam, seed=21)
+    cnt = engine.run(opts, seed=21)
+    sec, inter = engine.fetch(cnt, intermediates=True)
+    print("intermediates", len(inter["species"]), flush=True)
+    assert len(inter["species"]) > 10
+    e = 0.510999 * inter["u"][:, 0]
+    counts, lo, hi, total = engine.histogram(O.INTERMEDIATE, 14, 16)
+    assert (lo, hi) == (pytest.approx(e.min(), rel=1e-15), pytest.approx(e.max(), rel=1e-15))
+    assert total == pytest.approx(inter["weight"].sum(), rel=1e-12)
+    ref = np.bincount(np.floor((e - lo) / ((hi - lo) / 16)).astype(int), weights=inter["weight"], minlength=17)[:16]
+    assert np.abs(counts - ref).sum() <= 2.0 * inter["weight"].max()
+    st = O.Statistic("mean energy", {})
+    assert st.evaluate(engine, O.INTERMEDIATE, "intermediate") == pytest.approx(e.mean(), rel=1e-10)
+    deck = Deck.from_string("output:\n  intermediate: [energy]\n")
+    files = O.write_distributions(engine, deck, sec, str(tmp_path), "", intermediates=inter)
+    assert [os.path.basename(f) for f in files] == ["intermediate_energy.dat"]

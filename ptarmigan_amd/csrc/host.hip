@@ -126,8 +126,11 @@ struct ptg_ctx {
     double* d_n0 = nullptr;
     size_t n0_cap = 0;
     size_t n_prim = 0;
-    uint32_t* d_shower_count = nullptr;
+    uint32_t* d_shower_count = nullptr;  // final-state particles per shower (multiplicity selection)
     size_t shower_cap = 0;
+    // secondary-queue records per input particle seen so far (lepton, photon stage):
+    // sizes the queue of the next transport launch so that one launch suffices
+    double next_ratio[2] = {2.0, 2.0};
     unsigned long long* d_ctr = nullptr;
     TransportArgs* d_args = nullptr;
     uint64_t n_out = 0, n_inter = 0;
@@ -724,7 +727,11 @@ static int transport(ptg_ctx* c, TransportArgs a, bool photons, DevBuf& in, size
     // room for one final-state record per input particle
     HIPCHK(c, grow_buf(c->out, c->n_out + n_in + n_in / 4 + 1024, c->n_out, c->stream));
     if (a.opt.keep_decayed_photons && photons) HIPCHK(c, grow_buf(c->inter, c->n_inter + n_in + 1024, c->n_inter, c->stream));
-    if (next.cap < 1024) HIPCHK(c, alloc_buf(next, std::max<size_t>(1024, n_in)));
+    {
+        const double ratio = c->next_ratio[photons ? 1 : 0];
+        const size_t want = size_t(double(n_in) * ratio * 1.25) + 1024;
+        if (next.cap < want) HIPCHK(c, alloc_buf(next, want));
+    }
     unsigned long long snap[C_N];
     HIPCHK(c, scopy(c, snap, c->d_ctr, sizeof(snap), hipMemcpyDeviceToHost));
     for (int attempt = 0; attempt < 4; ++attempt) {
@@ -756,6 +763,9 @@ static int transport(ptg_ctx* c, TransportArgs a, bool photons, DevBuf& in, size
         (void)hipEventElapsedTime(&t, c->ev0, c->ev1);
         *ms += t;
         ++*launches;
+        const double seen = double(now[C_NEXT]) / double(n_in);
+        double& ratio = c->next_ratio[photons ? 1 : 0];
+        if (seen > ratio) ratio = seen;
         if (now[C_NEXT] <= next.cap) {
             c->n_out = now[C_OUT];
             c->n_inter = now[C_INTER];
@@ -1213,6 +1223,10 @@ int ptg_run(ptg_ctx* c, const ptg_collide_options* o, uint64_t seed, ptg_counter
             return fail(c, PTG_E_UNSUPPORTED, "mixed precision is implemented for quantum LMA emission (classical: false)");
     }
     if (c->laser.kind == K_PLANE_WAVE || c->laser.kind == K_FOCUSED_LASER) {
+        // classical + rr + gaunt_factor is the modified Landau-Lifshitz equation,
+        // which the LMA fields refuse (plane_wave.rs:214,228,236; focused_laser.rs:223,237,245)
+        if (o->classical && o->rr && o->gaunt_factor)
+            return fail(c, PTG_E_ARG, "Gaunt factor correction is unavailable in LMA mode");
         if (o->tracking_photons && !c->laser.circular && !c->d_bw)
             return fail(c, PTG_E_UNSUPPORTED, "LMA pair creation needs the regenerated Breit-Wheeler mid-range table "
                                               "(tables/gen_bw_tables.py, then tables/extract.py)");
@@ -1264,16 +1278,6 @@ int ptg_run(ptg_ctx* c, const ptg_collide_options* o, uint64_t seed, ptg_counter
     a.n_prim = uint32_t(n);
     a.stats = c->d_ctr + C_STATS;
     a.flags = reinterpret_cast<uint32_t*>(c->d_ctr + C_FLAGS);
-    if (a.opt.multiplicity_on) {
-        if (c->shower_cap < n) {
-            if (c->d_shower_count) (void)hipFree(c->d_shower_count);
-            c->d_shower_count = nullptr;
-            HIPCHK(c, hipMalloc(&c->d_shower_count, n * sizeof(uint32_t)));
-            c->shower_cap = n;
-        }
-        HIPCHK(c, hipMemsetAsync(c->d_shower_count, 0, n * sizeof(uint32_t), c->stream));
-    }
-    a.shower_count = c->d_shower_count;
 
     HIPCHK(c, hipMemsetAsync(c->d_ctr, 0, C_N * sizeof(unsigned long long), c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1310,6 +1314,19 @@ int ptg_run(ptg_ctx* c, const ptg_collide_options* o, uint64_t seed, ptg_counter
     }
 
     if (a.opt.multiplicity_on && c->n_out) {
+        // count each shower's final state now that it is complete (a repeated
+        // transport launch cannot count a particle twice), then keep the showers
+        // with multiplicity + 1 particles
+        if (c->shower_cap < n) {
+            if (c->d_shower_count) (void)hipFree(c->d_shower_count);
+            c->d_shower_count = nullptr;
+            c->shower_cap = 0;
+            HIPCHK(c, hipMalloc(&c->d_shower_count, n * sizeof(uint32_t)));
+            c->shower_cap = n;
+        }
+        HIPCHK(c, hipMemsetAsync(c->d_shower_count, 0, n * sizeof(uint32_t), c->stream));
+        launch_count_showers(c->out.v, c->n_out, c->d_shower_count, c->stream);
+        HIPCHK(c, hipGetLastError());
         HIPCHK(c, grow_buf(c->tmp, c->n_out, 0, c->stream));
         unsigned long long zero = 0;
         HIPCHK(c, scopy(c, c->d_ctr + C_WORK, &zero, sizeof(zero), hipMemcpyHostToDevice));
@@ -1530,6 +1547,13 @@ int ptg_finish(ptg_ctx* c, int stop_mode, double t_stop, int beam_basis, double 
     }
     launch_finish(c->out.v, c->n_out, stop_mode != 0 && c->n_out, ct, beam_basis ? 1 : 0, hangle, vangle, c->stream);
     HIPCHK(c, hipGetLastError());
+    // the intermediate photons take the basis change but not the drift: the
+    // reference drifts bunch.iter_mut() and rotates bunch.iter_all_mut()
+    // (src/main.rs:1150-1196, particle/shower.rs:64-69)
+    if (beam_basis && c->n_inter) {
+        launch_finish(c->inter.v, c->n_inter, 0, ct, 1, hangle, vangle, c->stream);
+        HIPCHK(c, hipGetLastError());
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (t_common) *t_common = stop_mode == 0 ? INFINITY : ct / C_LIGHT;
     return PTG_OK;

@@ -51,13 +51,61 @@ struct TransportArgs {
     uint64_t id_base;
     const double* prim_n0;      // [3][n_prim] unit momentum direction of each primary
     uint32_t n_prim;
-    uint32_t* shower_count;     // kept final-state particles per shower (multiplicity selection)
     unsigned long long* stats;
     uint32_t* flags;
 };
 
 __device__ __forceinline__ void wave_add(unsigned long long* p, uint64_t v) {
     if (v) atomicAdd(p, (unsigned long long)v);
+}
+
+// Slot reservation with one atomic per wave: the active lanes that name the same
+// counter and count are ranked by mbcnt over their ballot, the first of them adds
+// the group's total and broadcasts the base with readfirstlane, and each lane takes
+// base + rank * cnt. The loop peels one (counter, count) group per pass, so lanes
+// that reach this point from different call sites (branches the compiler merged)
+// are still served correctly; when all active lanes agree it runs once.
+// Slots within a wave follow lane order; across waves they follow the order in
+// which the waves' atomics land.
+__device__ __forceinline__ uint64_t wave_reserve(unsigned long long* ctr, uint32_t cnt) {
+    const uint64_t addr = reinterpret_cast<uint64_t>(ctr);
+    uint64_t slot = 0;
+    for (;;) {
+        const uint64_t lead = (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(addr >> 32))) << 32) |
+                              uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(addr)));
+        const uint32_t lead_cnt = __builtin_amdgcn_readfirstlane(cnt);
+        if (lead == addr && lead_cnt == cnt) {
+            const uint64_t mask = __ballot(1);
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+            unsigned long long base = 0;
+            if (rank == 0) base = atomicAdd(ctr, (unsigned long long)(__popcll(mask)) * cnt);
+            base = (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(base >> 32))) << 32) |
+                   uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(base)));
+            slot = base + uint64_t(rank) * cnt;
+            break;
+        }
+    }
+    return slot;
+}
+// 32-bit counter form (work counters): same scheme.
+__device__ __forceinline__ uint32_t wave_reserve32(uint32_t* ctr, uint32_t cnt) {
+    const uint64_t addr = reinterpret_cast<uint64_t>(ctr);
+    uint32_t slot = 0;
+    for (;;) {
+        const uint64_t lead = (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(addr >> 32))) << 32) |
+                              uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(addr)));
+        const uint32_t lead_cnt = __builtin_amdgcn_readfirstlane(cnt);
+        if (lead == addr && lead_cnt == cnt) {
+            const uint64_t mask = __ballot(1);
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+            uint32_t base = 0;
+            if (rank == 0) base = atomicAdd(ctr, uint32_t(__popcll(mask)) * cnt);
+            base = __builtin_amdgcn_readfirstlane(base);
+            slot = base + rank * cnt;
+            break;
+        }
+    }
+    return slot;
 }
 
 // The final energy/angle cut of collide() (src/main.rs:268-275).
@@ -86,14 +134,13 @@ __device__ __forceinline__ void finalize_lepton(const TransportArgs& a, uint32_t
     if (primary && a.opt.discard_bg_e && !(icount > 0.0)) return;
     uint32_t shower = a.in.shower[i];
     if (!passes_cut(a, shower, M_E_MEV * ut, u)) return;
-    uint64_t j = atomicAdd(a.out_ctr, 1ull);
+    uint64_t j = wave_reserve(a.out_ctr, 1u);
     const PBuf& o = a.out;
     copy_record(o, j, a.in, i);
     o.d[P_RT][j] = r.t; o.d[P_RX][j] = r.x; o.d[P_RY][j] = r.y; o.d[P_RZ][j] = r.z;
     o.d[P_UT][j] = ut; o.d[P_UX][j] = u.x; o.d[P_UY][j] = u.y; o.d[P_UZ][j] = u.z;
     o.d[P_WORK][j] = work;
     o.d[P_ICOUNT][j] = icount;
-    if (a.opt.multiplicity_on) atomicAdd(a.shower_count + shower, 1u);
 }
 
 __device__ __forceinline__ void finalize_photon_from(const TransportArgs& a, const PBuf& src, uint32_t i, d4 r, d4 k, sv4 pol,
@@ -107,13 +154,12 @@ __device__ __forceinline__ void finalize_photon_from(const TransportArgs& a, con
         if (!passes_cut(a, shower, M_E_MEV * k.t, sp(k))) return;
         o = &a.out;
         ctr = a.out_ctr;
-        if (a.opt.multiplicity_on) atomicAdd(a.shower_count + shower, 1u);
     } else {
         if (!a.opt.keep_decayed_photons) return;
         o = &a.inter;
         ctr = a.inter_ctr;
     }
-    uint64_t j = atomicAdd(ctr, 1ull);
+    uint64_t j = wave_reserve(ctr, 1u);
     copy_record(*o, j, src, i);
     o->d[P_RT][j] = r.t; o->d[P_RX][j] = r.x; o->d[P_RY][j] = r.y; o->d[P_RZ][j] = r.z;
     o->d[P_UT][j] = k.t; o->d[P_UX][j] = k.x; o->d[P_UY][j] = k.y; o->d[P_UZ][j] = k.z;
@@ -154,8 +200,8 @@ __device__ __forceinline__ void write_new(const PBuf& q, uint64_t j, uint32_t sp
 __device__ __forceinline__ void emit_photon(const TransportArgs& a, d4 r, d4 k, sv4 pol, double a_eff, double chi,
                                             double weight, uint64_t parent, uint64_t stream, uint32_t shower,
                                             uint32_t& flags) {
-    uint64_t j = atomicAdd(a.next_ctr, 1ull);
-    uint64_t id = a.id_base + atomicAdd(a.id_ctr, 1ull);
+    uint64_t j = wave_reserve(a.next_ctr, 1u);
+    uint64_t id = a.id_base + wave_reserve(a.id_ctr, 1u);
     if (j >= a.next_cap) return;  // counted; the host grows the queue and repeats the launch
     write_new(a.next, j, 2u, r, k, d4{0.0, 0.0, 0.0, 0.0}, pol, a_eff, chi, weight, id, parent, stream, shower);
 }
@@ -164,8 +210,8 @@ __device__ __forceinline__ void emit_photon(const TransportArgs& a, d4 r, d4 k, 
 __device__ __forceinline__ void emit_pair(const TransportArgs& a, d4 r, d3 ue, d3 up, double weight, double a_eff,
                                           double chi, uint64_t parent, uint64_t base_stream, uint32_t shower,
                                           uint32_t& flags) {
-    uint64_t j = atomicAdd(a.next_ctr, 2ull);
-    uint64_t id = a.id_base + atomicAdd(a.id_ctr, 2ull);
+    uint64_t j = wave_reserve(a.next_ctr, 2u);
+    uint64_t id = a.id_base + wave_reserve(a.id_ctr, 2u);
     if (j + 1 >= a.next_cap) return;
     sv4 unpol{1.0, 0.0, 0.0, 0.0};
     d4 u0{1.0, 0.0, 0.0, 0.0};
@@ -178,7 +224,7 @@ __device__ __forceinline__ void emit_pair(const TransportArgs& a, d4 r, d3 ue, d
 // Hands a particle of the other kind (photon in the lepton stage or vice versa)
 // on to the next stage unchanged.
 __device__ __forceinline__ void forward_record(const TransportArgs& a, uint32_t i) {
-    uint64_t j = atomicAdd(a.next_ctr, 1ull);
+    uint64_t j = wave_reserve(a.next_ctr, 1u);
     if (j >= a.next_cap) return;
     const PBuf& q = a.next;
 #pragma unroll

@@ -13,7 +13,8 @@
 //    the first step with S < U. The event step has exactly the distribution of
 //    the reference's per-step Bernoulli draws (rng < p_k), with one RNG call per
 //    event instead of one per step.
-//  * Secondaries are appended to the next queue with a wave-aggregated atomic.
+//  * Secondaries, final states and forwarded records take their slots with one
+//    atomic per wave (wave_reserve: ballot, mbcnt rank, readfirstlane base).
 #include "kern_common.h"
 
 // Occupancy target of the transport kernels (waves per SIMD); bounds the
@@ -97,7 +98,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PTG_WAVES_P
 
     auto fetch = [&]() -> bool {
         for (;;) {
-            uint32_t i = atomicAdd(A->work_ctr, 1u);
+            uint32_t i = wave_reserve32(A->work_ctr, 1u);
             if (i >= A->n_in) return false;
             const PBuf& in = A->in;
             meta = in.meta[i];
@@ -229,7 +230,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PTG_WAVES_P
 
     auto fetch = [&]() -> bool {
         for (;;) {
-            uint32_t i = atomicAdd(A->work_ctr, 1u);
+            uint32_t i = wave_reserve32(A->work_ctr, 1u);
             if (i >= A->n_in) return false;
             const PBuf& in = A->in;
             meta = in.meta[i];

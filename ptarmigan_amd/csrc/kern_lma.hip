@@ -55,7 +55,7 @@ __global__ __launch_bounds__(256) void k_step_lma(const TransportArgs* __restric
 
     auto fetch = [&]() -> bool {
         for (;;) {
-            uint32_t i = atomicAdd(A->work_ctr, 1u);
+            uint32_t i = wave_reserve32(A->work_ctr, 1u);
             if (i >= A->n_in) return false;
             const PBuf& in = A->in;
             meta = in.meta[i];
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void k_step_lma(const TransportArgs* __restric
         S *= (1.0 - prob);
         if (S < U) {
             // suspend: queue the particle with the event parameters
-            unsigned long long j = atomicAdd(M.ev_ctr, 1ull);
+            unsigned long long j = wave_reserve(M.ev_ctr, 1u);
             if (j < M.ev_cap) {
                 const PBuf& q = M.ev;
                 copy_record(q, j, A->in, idx);
@@ -187,17 +187,16 @@ __device__ inline void lma_emit_finish(const TransportArgs* __restrict__ A, cons
         double chi = a_field * eta;
         // photon (src/main.rs:172-189)
         const uint32_t shower = q.shower[i];
-        uint64_t id = A->id_base + atomicAdd(A->id_ctr, 1ull);
+        uint64_t id = A->id_base + wave_reserve(A->id_ctr, 1u);
         uint64_t pstream = rng_child(q.stream[i], nev);
         if (M.photons_to_out) {
             if (passes_cut(*A, shower, M_E_MEV * k.t, sp(k))) {
-                uint64_t j = atomicAdd(A->out_ctr, 1ull);
+                uint64_t j = wave_reserve(A->out_ctr, 1u);
                 write_new(A->out, j, 2u, r, k, d4{0.0, 0.0, 0.0, 0.0}, sv, a_field, chi, q.d[P_WEIGHT][i], id, q.id[i],
                           pstream, shower);
-                if (A->opt.multiplicity_on) atomicAdd(A->shower_count + shower, 1u);
             }
         } else {
-            uint64_t j = atomicAdd(M.ph_ctr, 1ull);
+            uint64_t j = wave_reserve(M.ph_ctr, 1u);
             if (j < M.ph_cap)
                 write_new(M.ph, j, 2u, r, k, d4{0.0, 0.0, 0.0, 0.0}, sv, a_field, chi, q.d[P_WEIGHT][i], id, q.id[i],
                           pstream, shower);
@@ -208,7 +207,7 @@ __device__ inline void lma_emit_finish(const TransportArgs* __restrict__ A, cons
         if (!classical) work += ell * k0;  // absorption (src/main.rs:198-200)
     }
     // the electron continues with its next RNG segment
-    unsigned long long j = atomicAdd(M.next_ctr, 1ull);
+    unsigned long long j = wave_reserve(M.next_ctr, 1u);
     const PBuf& o = M.next;
     copy_record(o, j, q, i);
     o.d[P_RT][j] = r.t; o.d[P_RX][j] = r.x; o.d[P_RY][j] = r.y; o.d[P_RZ][j] = r.z;
@@ -267,7 +266,7 @@ __global__ __launch_bounds__(64) void k_event_lma(const TransportArgs* __restric
                 n_cap_ok = 0;
             } else if (need > M.scratch_len) {
                 if (M.ovf_list) {  // defer: no output now, the host re-runs this event
-                    M.ovf_list[atomicAdd(M.ovf_ctr, 1ull)] = i;
+                    M.ovf_list[wave_reserve(M.ovf_ctr, 1u)] = i;
                     atomicMax(M.ovf_len, (unsigned long long)need);
                     continue;
                 }
@@ -340,7 +339,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PTG_LMA_WAVE
         } else if (dbessel_len(nh) > M.scratch_len) {
             if (M.ovf_list) {  // defer: no output now, the host re-runs this event with a larger workspace
                 if (lane == 0) {
-                    M.ovf_list[atomicAdd(M.ovf_ctr, 1ull)] = i;
+                    M.ovf_list[wave_reserve(M.ovf_ctr, 1u)] = i;
                     atomicMax(M.ovf_len, (unsigned long long)dbessel_len(nh));
                 }
                 continue;
@@ -487,7 +486,7 @@ __global__ __launch_bounds__(256) void k_photon_step_lma(const TransportArgs* __
 
     auto fetch = [&]() -> bool {
         for (;;) {
-            uint32_t i = atomicAdd(A->work_ctr, 1u);
+            uint32_t i = wave_reserve32(A->work_ctr, 1u);
             if (i >= A->n_in) return false;
             const PBuf& in = A->in;
             meta = in.meta[i];
@@ -533,7 +532,7 @@ __global__ __launch_bounds__(256) void k_photon_step_lma(const TransportArgs* __
         if (prob * ri > 0.1) ri = 0.1 / prob;
         S *= (1.0 - prob * ri);
         if (S < U) {
-            unsigned long long j = atomicAdd(M.ev_ctr, 1ull);
+            unsigned long long j = wave_reserve(M.ev_ctr, 1u);
             if (j < M.ev_cap) {
                 const PBuf& q = M.ev;
                 copy_record(q, j, A->in, idx);
@@ -598,7 +597,7 @@ __global__ __launch_bounds__(64) void k_pair_event_lma(const TransportArgs* __re
             flags |= PTG_FLAG_TABLE_RANGE;
         } else if (!CIRC && dbessel_len(n) > M.scratch_len) {
             if (M.ovf_list) {
-                M.ovf_list[atomicAdd(M.ovf_ctr, 1ull)] = i;
+                M.ovf_list[wave_reserve(M.ovf_ctr, 1u)] = i;
                 atomicMax(M.ovf_len, (unsigned long long)dbessel_len(n));
                 continue;
             }
@@ -632,8 +631,8 @@ __global__ __launch_bounds__(64) void k_pair_event_lma(const TransportArgs* __re
             if (!(fabs(s - s_new) / s < 1.0e-3)) flags |= PTG_FLAG_SANITY;
             d4 qe{k.t + jn * k0 - qp.t, k.x - qp.x, k.y - qp.y, k.z + jn * k0 - qp.z};
             // electron and positron (src/main.rs:220-243): weight frac * w, half the absorbed energy each
-            unsigned long long j = atomicAdd(M.pairs_ctr, 2ull);
-            uint64_t id = A->id_base + atomicAdd(A->id_ctr, 2ull);
+            unsigned long long j = wave_reserve(M.pairs_ctr, 2u);
+            uint64_t id = A->id_base + wave_reserve(A->id_ctr, 2u);
             if (j + 1 < M.pairs_cap) {
                 sv4 unpol{1.0, 0.0, 0.0, 0.0};
                 d4 u0{1.0, 0.0, 0.0, 0.0};
@@ -655,7 +654,7 @@ __global__ __launch_bounds__(64) void k_pair_event_lma(const TransportArgs* __re
         if (decayed) {
             finalize_photon_from(*A, q, i, r, k, pol, weight, icount, meta, true);
         } else {
-            unsigned long long j = atomicAdd(M.next_ctr, 1ull);
+            unsigned long long j = wave_reserve(M.next_ctr, 1u);
             const PBuf& o = M.next;
             copy_record(o, j, q, i);
             o.d[P_RT][j] = r.t; o.d[P_RX][j] = r.x; o.d[P_RY][j] = r.y; o.d[P_RZ][j] = r.z;
@@ -785,11 +784,11 @@ __global__ void k_surr_inputs(const TransportArgs* __restrict__ A, LmaArgs M, Su
     rng_uniform2(g2, u2, u3);
     const double la = log(amp), le = log(eta);
     if (surr_in_domain(S, la, le)) {
-        const unsigned long long j = atomicAdd(S.n_rows, 1ull);
+        const unsigned long long j = wave_reserve(S.n_rows, 1u);
         S.list[j] = i;
         surr_features(S, la, le, u1, surr_u2_level(u2), u3, S.x + 8 * j);
     } else {
-        S.fb[atomicAdd(S.n_fb, 1ull)] = i;
+        S.fb[wave_reserve(S.n_fb, 1u)] = i;
     }
 }
 
@@ -837,7 +836,7 @@ __global__ __launch_bounds__(64) void k_surr_event(const TransportArgs* __restri
         // with the pulse, each stepped for millions of steps
         const double v_net = double(o[1]);
         if (!(v_net == v_net)) {
-            S.fb[atomicAdd(S.n_fb, 1ull)] = i;
+            S.fb[wave_reserve(S.n_fb, 1u)] = i;
             continue;
         }
         double v = fmin(fmax(surr_v_tail(u2, fmin(fmax(v_net, 0.005), 0.995)), 1.0e-12), 1.0 - 1.0e-9);
@@ -882,7 +881,7 @@ __global__ __launch_bounds__(64) void k_surr_event(const TransportArgs* __restri
             ok = isfinite(sv.q) && isfinite(sv.u) && isfinite(sv.v);
         }
         if (!ok) {
-            S.fb[atomicAdd(S.n_fb, 1ull)] = i;
+            S.fb[wave_reserve(S.n_fb, 1u)] = i;
             continue;
         }
         lma_emit_finish<false>(A, M, i, k0, a_rms, eta, true, n, s, theta, sv, n_emit, flags);

@@ -62,6 +62,7 @@ struct StatArgs {
 void launch_stats(const StatArgs& a, double* acc, unsigned long long* mm, int grid, hipStream_t s);
 void launch_hist_bin2(const HistArgs& hx, const HistArgs& hy, double* bins, double* total, int grid, hipStream_t s);
 double host_order_key_inv(unsigned long long k);
+void launch_count_showers(const PBuf& p, uint64_t n, uint32_t* count, hipStream_t s);
 void launch_select_multiplicity(const PBuf& src, uint64_t n, const PBuf& dst, unsigned long long* ctr, const uint32_t* count,
                                 uint32_t want, hipStream_t s);
 void launch_eval(const EvalArgs& e, hipStream_t s);

@@ -436,6 +436,16 @@ double host_order_key_inv(unsigned long long k) {
 }
 
 // ---------------------------------------------------------------- multiplicity selection
+// Final-state particles per shower, counted over the finished final state (the
+// length of Shower::secondaries after the cuts, src/main.rs:268-282).
+__global__ void k_count_showers(PBuf p, uint64_t n, uint32_t* count) {
+    const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+    if (i < n) atomicAdd(count + p.shower[i], 1u);
+}
+void launch_count_showers(const PBuf& p, uint64_t n, uint32_t* count, hipStream_t s) {
+    if (n) k_count_showers<<<int((n + 255) / 256), 256, 0, s>>>(p, n, count);
+}
+
 // Drops every final-state particle of showers whose kept count differs from
 // multiplicity + 1 (src/main.rs:277-282), compacting into `dst`.
 __global__ void k_select_multiplicity(PBuf src, uint64_t n, PBuf dst, unsigned long long* ctr, const uint32_t* count,
@@ -443,7 +453,7 @@ __global__ void k_select_multiplicity(PBuf src, uint64_t n, PBuf dst, unsigned l
     uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
     if (i >= n) return;
     if (count[src.shower[i]] != want) return;
-    uint64_t j = atomicAdd(ctr, 1ull);
+    uint64_t j = wave_reserve(ctr, 1u);
 #pragma unroll
     for (int c = 0; c < P_NDOUBLE; ++c) dst.d[c][j] = src.d[c][i];
     dst.id[j] = src.id[i];

@@ -420,6 +420,34 @@ def test_finish_matches_the_driver(engine, mode, theta, phi):
     assert np.allclose(after["r0"][:, 1:], before["r0"][:, 1:] @ R.T, rtol=1e-12, atol=1e-12 * s0)
 
 
+@pytest.mark.gpu
+def test_finish_rotates_intermediates_without_drift(engine):
+    """The intermediate (decayed) photons take the beam-frame rotation but not the
+    drift to the common time: the driver drifts bunch.iter_mut() and rotates
+    bunch.iter_all_mut() (src/main.rs:1150-1196, particle/shower.rs:64-69)."""
+    from ptarmigan_amd import Beam, CollideOptions, Envelope, Laser, LaserKind, Polarization
+    laser = Laser(LaserKind.FAST_PLANE_WAVE, 40.0, 0.8e-6, 8.0, 0.0, Envelope.COS_SQUARED, Polarization.LINEAR)
+    theta, phi = 0.25, -0.8
+    beam = Beam(n=20_000, gamma=16500.0 / 0.510999, angle=theta, collision_plane_angle=phi)
+    opts = CollideOptions(tracking_photons=True, keep_decayed_photons=True)
+    engine.set_laser(laser)
+    engine.build_beam(beam, seed=21)
+    cnt = engine.run(opts, seed=21)
+    sec0, inter0 = engine.fetch(cnt, intermediates=True)
+    assert len(inter0["species"]) > 50
+    engine.finish(1, 0.0, True, theta, phi)
+    sec1, inter1 = engine.fetch(cnt, intermediates=True)
+    print("intermediates", len(inter0["species"]), flush=True)
+    R = _beam_basis(theta, phi)
+    for k in ("r", "u", "r0", "u0"):
+        scale = np.abs(inter0[k][:, 1:]).max()
+        assert np.array_equal(inter1[k][:, 0], inter0[k][:, 0]), k   # times and energies unchanged: no drift
+        assert np.allclose(inter1[k][:, 1:], inter0[k][:, 1:] @ R.T, rtol=1e-12, atol=1e-12 * scale), k
+    # the final state is drifted to the latest time as well
+    ct = sec0["r"][:, 0].max()
+    assert np.allclose(sec1["r"][:, 0], ct, rtol=1e-13)
+
+
 def test_spectrum_tables(tmp_path):
     """GammaDistribution::custom rows (particle/dstr.rs:88-106) from a function and
     from a file, and the spectrum's mean gamma."""

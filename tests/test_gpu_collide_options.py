@@ -1,0 +1,394 @@
+"""GPU parity of collide()'s options (src/main.rs:134-289) and of the device beam
+builder (particle/builder.rs:173-219) against the oracle.
+
+Each option is checked two ways where the counter-based RNG allows it:
+  * exactly, against a run of the device without the option on the same seed
+    (every particle draws from its own stream, so switching a cut, a selection
+    or a stop time on changes only what the option changes), and
+  * statistically, against the oracle's collide() at matched sample size
+    (two-sample KS / chi-square at p > 1e-3, counts within 5 sigma).
+Options: rate_increase with partial decay and reweighting (main.rs:242-246),
+min_energy / max_angle (main.rs:268-275), select_multiplicity (main.rs:277-282,
+on a cascade whose first lepton launch overflows a cold context's queue),
+discard_background (main.rs:207, 256), t_stop (main.rs:169, 216), LMA finite
+bandwidth (plane_wave.rs:49-63, 263) and LMA chirp (plane_wave.rs:253-256).
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+C_LIGHT = 2.997925e8
+M_E_MEV = 0.510999
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _fpw_cascade(n, a0=40.0):
+    """FastPlaneWave (LCFA) LP, 8-cycle cos^2, 16.5 GeV electrons: about 4.5
+    photons per primary at a0 = 40 (more than the queue a cold context sizes)."""
+    from ptarmigan_amd import Beam, CollideOptions, Envelope, Laser, LaserKind, Polarization
+    laser = Laser(LaserKind.FAST_PLANE_WAVE, a0, 0.8e-6, 8.0, 0.0, Envelope.COS_SQUARED, Polarization.LINEAR)
+    beam = Beam(n=n, gamma=16500.0 / M_E_MEV, sigma=0.0)
+    return laser, beam, CollideOptions(tracking_photons=True)
+
+
+def _prim(engine, orc, laser, beam, seed):
+    engine.set_laser(laser)
+    dt, z0 = engine.field_info()
+    return orc.build_beam(beam.desc(), z0 + 3.0 * beam.length, seed)
+
+
+def _key(s, m=None):
+    """Sort key of final-state records: (shower, species, momentum)."""
+    m = np.ones(len(s["species"]), bool) if m is None else m
+    return np.lexsort(np.vstack([s["u"][m].T, s["species"][m], s["shower"][m]]))
+
+
+def _same_records(a, b, ma=None, mb=None):
+    ka, kb = _key(a, ma), _key(b, mb)
+    ma = np.ones(len(a["species"]), bool) if ma is None else ma
+    mb = np.ones(len(b["species"]), bool) if mb is None else mb
+    for f in ("species", "shower", "u", "r", "pol", "weight"):
+        if not np.array_equal(a[f][ma][ka], b[f][mb][kb]):
+            return f
+    return None
+
+
+def _poisson_close(a, b, k=5.0):
+    return abs(a - b) < k * math.sqrt(a + b + 1.0)
+
+
+def test_multiplicity_after_a_cold_queue_overflow(orc):
+    """select_multiplicity on a fresh context whose first lepton launch overflows
+    its secondary queue and is repeated: the kept showers are exactly those of the
+    unselected run with multiplicity + 1 final-state particles, each complete, and
+    their number agrees with the oracle's."""
+    from ptarmigan_amd import Engine
+    laser, beam, opts = _fpw_cascade(20_000)
+    with Engine(0) as eng:
+        prim = _prim(eng, orc, laser, beam, 4)
+        opts.multiplicity = 4
+        say("multiplicity, cold context")
+        sel, _, cs = eng.collide(prim, opts, seed=4)
+        say("  counters", cs)
+        # the first lepton launch was repeated (more launches than stages)
+        assert cs["lepton_launches"] + cs["photon_launches"] > cs["generations"]
+    with Engine(0) as eng:
+        eng.set_laser(laser)
+        opts.multiplicity = None
+        full, _, cf = eng.collide(prim, opts, seed=4)
+    counts = np.bincount(full["shower"].astype(np.int64), minlength=beam.n)
+    keep = counts[full["shower"].astype(np.int64)] == 5
+    assert len(sel["species"]) == keep.sum() > 0
+    assert _same_records(sel, full, None, keep) is None
+    per = np.bincount(sel["shower"].astype(np.int64), minlength=beam.n)
+    assert set(np.unique(per[per > 0])) == {5}
+    # against the oracle: the number of kept showers (binomial)
+    oo = orc.collide_options(multiplicity=4)
+    sc, _, cc, _ = orc.collide(laser.desc(), oo, prim, 4, 16)
+    kg, kc = keep.sum() // 5, len(sc["species"]) // 5
+    assert len(sc["species"]) % 5 == 0
+    p = (kg + kc) / (2.0 * beam.n)
+    se = math.sqrt(2.0 * beam.n * p * (1 - p)) + 1.0
+    say("  kept showers gpu", kg, "oracle", kc)
+    assert abs(kg - kc) < 5 * se
+
+
+def test_rate_increase_reweights_pairs(engine, orc):
+    """increase_pair_rate_by: pair events are boosted (capped at 0.1 per step),
+    each pair carries 1/boost of the photon's weight and the photon continues
+    with the rest (src/main.rs:242-246, field/lcf.rs:159-163). Event counts,
+    positron weights and photon weights agree with the oracle."""
+    from scipy.stats import ks_2samp
+    laser, beam, opts = _fpw_cascade(6000, a0=40.0)
+    opts.rate_increase = 20.0
+    prim = _prim(engine, orc, laser, beam, 8)
+    say("rate increase")
+    sg, _, cg = engine.collide(prim, opts, seed=8)
+    sc, _, cc, _ = orc.collide(laser.desc(), opts.desc(), prim, 8, 16)
+    say("  gpu", cg, "\n  cpu", cc)
+    assert cg["pair_events"] > 500
+    assert _poisson_close(cg["pair_events"], cc["pair_events"])
+    assert _poisson_close(cg["emissions"], cc["emissions"])
+    pos_g, pos_c = sg["species"] == 1, sc["species"] == 1
+    assert np.all(sg["weight"][pos_g] <= 1.0 / 20.0 + 1e-15)
+    # the weighted positron yield estimates the unbiased one
+    wg, wc = sg["weight"][pos_g].sum(), sc["weight"][pos_c].sum()
+    sig = math.sqrt((sg["weight"][pos_g] ** 2).sum() + (sc["weight"][pos_c] ** 2).sum()) + 1e-12
+    assert abs(wg - wc) < 5 * sig
+    # photons that survived one or more partial decays carry weight < 1
+    ph_g, ph_c = sg["species"] == 2, sc["species"] == 2
+    fg, fc = (sg["weight"][ph_g] < 1.0).sum(), (sc["weight"][ph_c] < 1.0).sum()
+    say("  reweighted photons gpu", fg, "oracle", fc)
+    assert fg > 100 and _poisson_close(fg, fc)
+    wpg, wpc = sg["weight"][ph_g], sc["weight"][ph_c]
+    assert ks_2samp(wpg[wpg < 1.0], wpc[wpc < 1.0]).pvalue > 1e-3
+
+
+def test_lma_rate_increase_reweights_pairs(engine, orc):
+    """The same biasing in the LMA plane wave (plane_wave.rs:284-316), CP a0 = 10:
+    pair counts, positron weights and reweighted photon counts against the oracle."""
+    from ptarmigan_amd import Beam, CollideOptions, Envelope, Laser, LaserKind, Polarization
+    laser = Laser(LaserKind.PLANE_WAVE, 10.0, 0.8e-6, 8.0, 0.0, Envelope.COS_SQUARED, Polarization.CIRCULAR)
+    beam = Beam(n=4000, gamma=16500.0 / M_E_MEV, sigma=0.0)
+    opts = CollideOptions(tracking_photons=True, rate_increase=100.0)
+    prim = _prim(engine, orc, laser, beam, 9)
+    say("lma rate increase")
+    sg, _, cg = engine.collide(prim, opts, seed=9)
+    sc, _, cc, _ = orc.collide(laser.desc(), opts.desc(), prim, 9, 16)
+    say("  gpu", cg, "\n  cpu", cc)
+    assert cg["pair_events"] > 200
+    assert _poisson_close(cg["pair_events"], cc["pair_events"])
+    pos_g, pos_c = sg["species"] == 1, sc["species"] == 1
+    wg, wc = sg["weight"][pos_g].sum(), sc["weight"][pos_c].sum()
+    sig = math.sqrt((sg["weight"][pos_g] ** 2).sum() + (sc["weight"][pos_c] ** 2).sum()) + 1e-12
+    assert abs(wg - wc) < 5 * sig
+    ph_g, ph_c = sg["species"] == 2, sc["species"] == 2
+    fg, fc = (sg["weight"][ph_g] < 1.0).sum(), (sc["weight"][ph_c] < 1.0).sum()
+    assert fg > 100 and _poisson_close(fg, fc)
+
+
+def test_energy_and_angle_cut(engine, orc):
+    """min_energy and max_angle drop final-state particles below the energy or
+    outside the cone about the primary's direction (main.rs:265-275): exactly the
+    cut applied to the uncut run of the same seed, and the kept counts agree with
+    the oracle."""
+    laser, beam, opts = _fpw_cascade(6000, a0=20.0)
+    prim = _prim(engine, orc, laser, beam, 10)
+    say("energy/angle cut")
+    full, _, _ = engine.collide(prim, opts, seed=10)
+    opts.min_energy, opts.max_angle = 50.0, 2.0e-3
+    cut, _, cg = engine.collide(prim, opts, seed=10)
+    n0 = prim["u"][:, 1:] / np.linalg.norm(prim["u"][:, 1:], axis=1)[:, None]
+    sh = full["shower"].astype(np.int64)
+    p = full["u"][:, 1:]
+    cosang = np.einsum("ij,ij->i", n0[sh], p) / np.linalg.norm(p, axis=1)
+    keep = (full["u"][:, 0] * M_E_MEV > 50.0) & (cosang > math.cos(2.0e-3))
+    assert 0 < keep.sum() < len(keep)
+    assert len(cut["species"]) == keep.sum()
+    assert _same_records(cut, full, None, keep) is None
+    sc, _, cc, _ = orc.collide(laser.desc(), opts.desc(), prim, 10, 16)
+    for sp in (0, 2):
+        a, b = (cut["species"] == sp).sum(), (sc["species"] == sp).sum()
+        say("  species", sp, "gpu", a, "oracle", b)
+        assert _poisson_close(a, b)
+
+
+def test_discard_background(engine, orc):
+    """discard_background_e drops primary electrons that never interacted
+    (interaction count 0); discard_background_ph drops primary photons that did not
+    decay (main.rs:207, 256). Exactly those records, against the run without it."""
+    from ptarmigan_amd import Beam, Species
+    laser, _, opts = _fpw_cascade(1, a0=2.0)
+    beam = Beam(n=20_000, gamma=2000.0 / M_E_MEV, sigma=0.0, radius=0.0)
+    prim = _prim(engine, orc, laser, beam, 11)
+    say("discard background (electrons)")
+    full, _, _ = engine.collide(prim, opts, seed=11)
+    opts.discard_bg_e = True
+    dis, _, cg = engine.collide(prim, opts, seed=11)
+    bg = (full["species"] == 0) & (full["id"] < beam.n) & (full["interaction_count"] == 0.0)
+    assert 0 < bg.sum() < beam.n
+    assert _same_records(dis, full, None, ~bg) is None
+    sc, _, cc, _ = orc.collide(laser.desc(), opts.desc(), prim, 11, 16)
+    a, b = (dis["species"] == 0).sum(), (sc["species"] == 0).sum()
+    say("  kept electrons gpu", a, "oracle", b)
+    assert _poisson_close(a, b)
+    # photon primaries: those that do not decay are background
+    lphot, _, popts = _fpw_cascade(1, a0=40.0)
+    pbeam = Beam(n=20_000, gamma=8000.0 / M_E_MEV, sigma=0.0, species=Species.PHOTON)
+    pprim = _prim(engine, orc, lphot, pbeam, 12)
+    say("discard background (photons)")
+    full, _, cf = engine.collide(pprim, popts, seed=12)
+    popts.discard_bg_ph = True
+    dis, _, cd = engine.collide(pprim, popts, seed=12)
+    bgp = (full["species"] == 2) & (full["id"] < pbeam.n)
+    assert 0 < bgp.sum() < pbeam.n and cf["pair_events"] > 100
+    assert _same_records(dis, full, None, ~bgp) is None
+    sc, _, cc, _ = orc.collide(lphot.desc(), popts.desc(), pprim, 12, 16)
+    assert _poisson_close(len(dis["species"]), len(sc["species"]))
+
+
+def test_t_stop_recoil_free_exact(engine, orc):
+    """stop_at_time: the loops end at the first step with t >= t_stop
+    (main.rs:169, 216). Without recoil the electron trajectory is deterministic,
+    so every final electron matches the oracle's to 1e-12."""
+    from ptarmigan_amd import workloads
+    laser, beam, opts = workloads.focused_lcfa_cascade(400)
+    opts.rr = False
+    opts.tracking_photons = False
+    engine.set_laser(laser)
+    dt, z0 = engine.field_info()
+    prim = orc.build_beam(beam.desc(), z0 + 3.0 * beam.length, 13)
+    t_stop = -0.3 * z0 / C_LIGHT   # part way through the pulse
+    opts.t_stop = t_stop
+    say("t_stop exact")
+    sg, _, cg = engine.collide(prim, opts, seed=13)
+    sc, _, cc, _ = orc.collide(laser.desc(), opts.desc(), prim, 13, 8)
+    assert cg["lepton_steps"] == cc["lepton_steps"]
+    eg, ec = sg["species"] == 0, sc["species"] == 0
+    assert eg.sum() == ec.sum() == beam.n
+    og, oc = np.argsort(sg["id"][eg]), np.argsort(sc["id"][ec])
+    rg, rc = sg["r"][eg][og], sc["r"][ec][oc]
+    ug, uc = sg["u"][eg][og], sc["u"][ec][oc]
+    # every electron stopped at the first step past t_stop
+    assert np.all(rg[:, 0] >= t_stop * C_LIGHT) and np.all(rg[:, 0] < t_stop * C_LIGHT + 1.01 * C_LIGHT * dt)
+    du = np.abs(ug - uc).max(axis=1) / np.linalg.norm(uc[:, 1:], axis=1)
+    assert du.max() < 1e-12
+    span = np.abs(rc - prim["r"][np.sort(sc["id"][ec]).astype(int)]).max(axis=1)
+    assert (np.abs(rg - rc).max(axis=1) / span).max() < 1e-12
+
+
+@pytest.mark.parametrize("case", ["bandwidth", "chirp", "bandwidth_cp"])
+def test_lma_bandwidth_and_chirp(engine, orc, case):
+    """LMA emission with a finite-bandwidth pulse (the local frequency drawn per
+    step, plane_wave.rs:49-63, 263) or a chirped one (plane_wave.rs:253-256).
+    The observable is the angle-scaled photon energy y = E (1 + (gamma theta)^2),
+    which at a0 = 1 sits on sharp harmonic lines set by the local frequency: the
+    frequency spread moves it far beyond statistics (checked against the same run
+    without the spread or chirp), and its distribution, the photon yield and the
+    energy spectrum agree with the oracle."""
+    from scipy.stats import ks_2samp
+    from ptarmigan_amd import Beam, CollideOptions, Envelope, Laser, LaserKind, Polarization
+    pol = Polarization.CIRCULAR if case.endswith("_cp") else Polarization.LINEAR
+    nc = 2.5 if case.startswith("bandwidth") else 8.0
+
+    def laser_of(on):
+        return Laser(LaserKind.PLANE_WAVE, 1.0, 0.8e-6, nc, 0.0, Envelope.COS_SQUARED, pol,
+                     chirp_b=0.01 if (case == "chirp" and on) else 0.0, finite_bandwidth=case.startswith("bandwidth") and on)
+
+    g0 = 10000.0 / M_E_MEV
+    beam = Beam(n=400_000, gamma=g0, sigma=0.0)
+    opts = CollideOptions(tracking_photons=False)
+    laser = laser_of(True)
+    prim = _prim(engine, orc, laser, beam, 14)
+    say("lma", case)
+    sg, _, cg = engine.collide(prim, opts, seed=14)
+    sc, _, cc, _ = orc.collide(laser.desc(), opts.desc(), prim, 14, 16)
+    say("  gpu", cg, "\n  cpu", cc)
+    assert cg["error_flags"] == 0
+    assert cg["lepton_steps"] == pytest.approx(cc["lepton_steps"], rel=1e-3)
+    assert cg["emissions"] > 2000 and _poisson_close(cg["emissions"], cc["emissions"])
+
+    def y(s):
+        m = s["species"] == 2
+        u = s["u"][m]
+        th = np.arctan2(np.hypot(u[:, 1], u[:, 2]), -u[:, 3]) * g0
+        return u[:, 0] * (1.0 + th * th)
+    yg, yc = y(sg), y(sc)
+    assert ks_2samp(yg, yc).pvalue > 1e-3
+    assert ks_2samp(sg["u"][sg["species"] == 2, 0], sc["u"][sc["species"] == 2, 0]).pvalue > 1e-3
+    engine.set_laser(laser_of(False))
+    s0, _, _ = engine.collide(prim, opts, seed=14)
+    engine.set_laser(laser)
+    p_off = ks_2samp(yg, y(s0)).pvalue
+    say("  spread/chirp off vs on: p", p_off)
+    assert p_off < 1e-10
+
+
+def _primaries_on_device(engine, beam, seed, initial_z):
+    """The device-built primaries, read back as the final state of a run that stops
+    before the first step (t_stop below every particle's time)."""
+    from ptarmigan_amd import CollideOptions
+    engine.build_beam(beam, seed=seed, initial_z=initial_z)
+    cnt = engine.run(CollideOptions(t_stop=-1e300), seed=seed)
+    sec, _ = engine.fetch(cnt)
+    order = np.argsort(sec["id"])
+    return {k: v[order] for k, v in sec.items()}, cnt
+
+
+@pytest.mark.parametrize("radial", [0, 1, 2])
+def test_device_beam_builder(engine, orc, radial):
+    """k_build_beam against BeamBuilder::build (particle/builder.rs:173-219) and the
+    closed forms of its distributions: gamma ~ N(mu, sigma) correlated with z at
+    rho = -energy_chirp, z ~ initial_z + N(0, sigma_z), t = -initial_z, theta_x ~
+    angle + N(0, div), theta_y ~ N(0, div), transverse normal / truncated normal /
+    uniform disc; KS against the closed forms and two-sample KS against the oracle."""
+    from scipy.stats import kstest, ks_2samp, norm
+    from ptarmigan_amd import Beam, workloads
+    laser, _, _ = workloads.plane_wave_lma(1)
+    engine.set_laser(laser)
+    n, mu, sigma, sz, r0, div, chirp = 200_000, 20000.0, 300.0, 15e-6, 3e-6, 2e-4, 0.4
+    beam = Beam(n=n, gamma=mu, sigma=sigma, radius=r0, radial=radial, max_radius=4e-6, length=sz, rms_div=div,
+                energy_chirp=chirp)
+    z0 = 1e-4
+    say("beam builder radial", radial)
+    g, cnt = _primaries_on_device(engine, beam, 15, z0)
+    assert len(g["species"]) == n and cnt["lepton_steps"] == 0
+    assert np.array_equal(g["id"], np.arange(n))
+    c = orc.build_beam(beam.desc(), z0, 15)
+    gam = g["u"][:, 0]
+    z = g["r"][:, 3]
+    assert np.all(g["r"][:, 0] == -z0)
+    assert np.allclose(g["u"][:, 0] ** 2 - (g["u"][:, 1:] ** 2).sum(1), 1.0, rtol=1e-6)
+    assert kstest((gam - mu) / sigma, norm.cdf).pvalue > 1e-3
+    assert kstest((z - z0) / sz, norm.cdf).pvalue > 1e-3
+    rho = np.corrcoef(gam, z)[0, 1]
+    assert abs(rho - (-chirp)) < 5.0 * (1 - chirp ** 2) / math.sqrt(n)
+    th_x = np.arctan2(-g["u"][:, 1], -g["u"][:, 3])
+    th_y = np.arcsin(-g["u"][:, 2] / np.linalg.norm(g["u"][:, 1:], axis=1))
+    assert kstest(th_x / div, norm.cdf).pvalue > 1e-3
+    assert kstest(th_y / div, norm.cdf).pvalue > 1e-3
+    x, y = g["r"][:, 1], g["r"][:, 2]
+    if radial == 0:
+        assert kstest(x / r0, norm.cdf).pvalue > 1e-3 and kstest(y / r0, norm.cdf).pvalue > 1e-3
+    elif radial == 1:
+        assert np.all(x ** 2 + y ** 2 <= 4e-6 ** 2 * (1 + 1e-12))
+    else:
+        # uniform disc of radius r0: r^2 / r0^2 ~ U(0, 1), azimuth uniform
+        assert kstest((x ** 2 + y ** 2) / r0 ** 2, "uniform").pvalue > 1e-3
+        assert kstest((np.arctan2(y, x) + np.pi) / (2 * np.pi), "uniform").pvalue > 1e-3
+    for name, a, b in (("gamma", gam, c["u"][:, 0]), ("z", z, c["r"][:, 3]), ("x", x, c["r"][:, 1]),
+                       ("y", y, c["r"][:, 2]), ("theta_x", th_x, np.arctan2(-c["u"][:, 1], -c["u"][:, 3]))):
+        p = ks_2samp(a, b).pvalue
+        say("  ks vs oracle", name, p)
+        assert p > 1e-3, name
+
+
+def test_beam_builder_collision_angle_and_offset(engine, orc):
+    """Collision angle, collision plane and offset (particle/builder.rs:186-213):
+    the beam centroid moves along -(cos phi sin theta, sin phi sin theta, cos theta),
+    positions are rotated about y then z, and an offset along z pushes the start back."""
+    from ptarmigan_amd import Beam, workloads
+    laser, _, _ = workloads.plane_wave_lma(1)
+    engine.set_laser(laser)
+    th, ph = 0.3, 0.7
+    beam = Beam(n=50_000, gamma=5000.0, sigma=0.0, radius=1e-6, length=2e-6, angle=th, collision_plane_angle=ph,
+                offset=(1e-6, -2e-6, 3e-6))
+    g, _ = _primaries_on_device(engine, beam, 16, 5e-5)
+    c = orc.build_beam(beam.desc(), 5e-5, 16)
+    centroid = np.array([-math.cos(ph) * math.sin(th), -math.sin(ph) * math.sin(th), -math.cos(th)])
+    dirs = g["u"][:, 1:] / np.linalg.norm(g["u"][:, 1:], axis=1)[:, None]
+    assert np.allclose(dirs, centroid, atol=1e-12)
+    assert np.allclose(g["u"], c["u"], rtol=1e-12)          # no spread: identical momenta
+    assert np.all(g["r"][:, 0] == -5e-5)
+    for col in (1, 2, 3):
+        from scipy.stats import ks_2samp
+        assert ks_2samp(g["r"][:, col], c["r"][:, col]).pvalue > 1e-3
+    mean = g["r"][:, 1:].mean(0)
+    # the rotated start point: (x_off, y_off, z0 + z_off) rotated about y by theta, then z by phi
+    p = np.array([1e-6, -2e-6, 5e-5 + 3e-6])
+    ry = np.array([[math.cos(th), 0, math.sin(th)], [0, 1, 0], [-math.sin(th), 0, math.cos(th)]])
+    rz = np.array([[math.cos(ph), -math.sin(ph), 0], [math.sin(ph), math.cos(ph), 0], [0, 0, 1]])
+    assert np.allclose(mean, rz @ ry @ p, atol=5 * 2e-6 / math.sqrt(beam.n) + 5e-9)
+
+
+def test_lma_gaunt_with_radiation_reaction_is_refused(engine):
+    """classical + radiation_reaction + gaunt_factor is the modified Landau-Lifshitz
+    equation, which the LMA fields refuse (plane_wave.rs:214,228,236); without
+    radiation reaction the same options are legal (Lorentz force, quantum
+    emission, no recoil)."""
+    from ptarmigan_amd import CollideOptions, PtgError, workloads
+    laser, beam, _ = workloads.plane_wave_lma(256)
+    engine.set_laser(laser)
+    engine.build_beam(beam, seed=1)
+    with pytest.raises(PtgError, match="Gaunt factor correction is unavailable in LMA mode"):
+        engine.run(CollideOptions(classical=True, rr=True, gaunt_factor=True, tracking_photons=False), seed=1)
+    cnt = engine.run(CollideOptions(classical=True, rr=False, gaunt_factor=True, tracking_photons=False), seed=1)
+    assert cnt["lepton_steps"] > 0 and cnt["error_flags"] == 0

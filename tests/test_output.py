@@ -127,16 +127,20 @@ def _gen1d(v, w, accept, bspec, n, log):
         v = np.log(v)
     ok = np.isfinite(v) & accept
     lo, hi = (v[ok].min(), v[ok].max()) if ok.any() else (np.finfo(float).max, -np.finfo(float).max)
+    # with nothing in range the reference's max - min is f64::MIN - f64::MAX, which
+    # IEEE arithmetic takes to -inf without a fault; nothing is binned then
+    with np.errstate(over="ignore"):
+        span = hi - lo
     if lo == hi:
         nb, vol = 1, 1.0
     elif bspec[0] in ("auto", "log"):
         nb = int(math.ceil(2.0 * np.cbrt(float(n))))
-        vol = (hi - lo) / nb
+        vol = span / nb
     elif bspec[0] == "fixed_number":
         nb = bspec[1]
-        vol = (hi - lo) / nb
+        vol = span / nb
     else:
-        nb, vol = int(math.ceil((hi - lo) / bspec[1])), bspec[1]
+        nb, vol = (int(math.ceil(span / bspec[1])) if np.isfinite(span) else 0), bspec[1]
     b = np.floor((v[ok] - lo) / vol).astype(np.int64)
     ww = w[ok]
     if log:

@@ -676,6 +676,15 @@ int orc_cp_harmonic(double a, double eta, double frac) { return has_table("nonli
 int orc_cp_classical_harmonic(double a, double frac) { return cp_classical_harmonic(a, frac); }
 double orc_cp_cdf_row(double a, double eta, int classical, double* out64) { return cp_cdf_row(a, eta, classical != 0, out64); }
 // count samples of (n, s, theta, S1, S2, S3, trials) from one seeded stream
+// Stokes parameters (1, S1, S2, S3) of the CP photon at harmonic n, fraction v of
+// the largest s (quantum: s = v s_max; classical: s = v s_n) and ZMF azimuth theta
+void orc_cp_stokes(int n, double a, double eta, double v, double theta, int classical, double* out4) {
+    double sn = 2.0 * double(n) * eta / (1.0 + a * a);
+    double s = classical ? v * sn : v * sn / (1.0 + sn);
+    SV sv = cp_stokes(n, a, s, sn, v, theta, classical != 0);
+    out4[0] = sv.i; out4[1] = sv.q; out4[2] = sv.u; out4[3] = sv.v;
+}
+
 void orc_cp_sample(double a, double eta, int classical, uint64_t seed, int fixed_n, size_t count, double* out7) {
     Xoshiro rng(seed);
     for (size_t i = 0; i < count; ++i) {

@@ -447,6 +447,27 @@ static double partial_rate(int n, double a, double eta, double* max_out) {
 }
 
 // lp/mod.rs:586-676
+// Stokes parameters of the photon at harmonic n, (s, theta), theta in [0, 2 pi)
+// (lp/mod.rs:622-671); *xi0_out (if given) receives the unnormalised xi_0
+static SV lp_stokes(int n, double a, double eta, double s, double theta, DoubleBessel& dj, double* xi0_out) {
+    double sn = 2.0 * double(n) * eta / (1.0 + 0.5 * a * a);
+    double wn = std::fmin(s / (sn * (1.0 - s)), 1.0);
+    double x = -2.0 * double(n) * a * std::cos(theta) * std::sqrt(wn * (1.0 - wn)) / std::sqrt(1.0 + 0.5 * a * a);
+    double y = a * a * s / (8.0 * eta * (1.0 - s));
+    double j[5];
+    dj.evaluate(std::fabs(x), y, j);
+    double A0, A1, A2;
+    if (x > 0.0) { A0 = j[2]; A1 = 0.5 * (j[1] + j[3]); A2 = 0.25 * (j[0] + 2.0 * j[2] + j[4]); }
+    else if (n % 2 == 0) { A0 = j[2]; A1 = -0.5 * (j[1] + j[3]); A2 = 0.25 * (j[0] + 2.0 * j[2] + j[4]); }
+    else { A0 = -j[2]; A1 = 0.5 * (j[1] + j[3]); A2 = -0.25 * (j[0] + 2.0 * j[2] + j[4]); }
+    double u = std::sqrt(2.0 * double(n) * eta * (1.0 - s) / s - (1.0 + 0.5 * a * a));
+    double xi0 = (1.0 - s + 1.0 / (1.0 - s)) * (powi(A1, 2) - A0 * A2) - 2.0 * powi(A0 / a, 2);
+    double xi1 = 2.0 * (powi(A1, 2) - A0 * A2) - 2.0 * (1.0 + 2.0 * u * u * powi(std::sin(theta), 2)) * powi(A0 / a, 2);
+    double xi2 = -1.0 * (2.0 * powi(u * A0 / a, 2) * std::sin(2.0 * theta) + 4.0 * u * A0 * A1 * std::sin(theta) / a);
+    if (xi0_out) *xi0_out = xi0;
+    return {1.0, xi1 / xi0, xi2 / xi0, 0.0};
+}
+
 static void lp_sample(double a, double eta, Xoshiro& rng, int fixed_n, int* n_out, double* s_out, double* theta_out, SV* pol_out,
                       uint64_t* trials) {
     int n = fixed_n > 0 ? fixed_n : get_harmonic_index(a, eta, rng.gen_f64());
@@ -474,23 +495,10 @@ static void lp_sample(double a, double eta, Xoshiro& rng, int fixed_n, int* n_ou
     case 3: theta = 2.0 * PI - theta; break;
     default: break;
     }
-    double wn = std::fmin(s / (sn * (1.0 - s)), 1.0);
-    double x = -2.0 * double(n) * a * std::cos(theta) * std::sqrt(wn * (1.0 - wn)) / std::sqrt(1.0 + 0.5 * a * a);
-    double y = a * a * s / (8.0 * eta * (1.0 - s));
-    double j[5];
-    dj.evaluate(std::fabs(x), y, j);
-    double A0, A1, A2;
-    if (x > 0.0) { A0 = j[2]; A1 = 0.5 * (j[1] + j[3]); A2 = 0.25 * (j[0] + 2.0 * j[2] + j[4]); }
-    else if (n % 2 == 0) { A0 = j[2]; A1 = -0.5 * (j[1] + j[3]); A2 = 0.25 * (j[0] + 2.0 * j[2] + j[4]); }
-    else { A0 = -j[2]; A1 = 0.5 * (j[1] + j[3]); A2 = -0.25 * (j[0] + 2.0 * j[2] + j[4]); }
-    double u = std::sqrt(2.0 * double(n) * eta * (1.0 - s) / s - (1.0 + 0.5 * a * a));
-    double xi0 = (1.0 - s + 1.0 / (1.0 - s)) * (powi(A1, 2) - A0 * A2) - 2.0 * powi(A0 / a, 2);
-    double xi1 = 2.0 * (powi(A1, 2) - A0 * A2) - 2.0 * (1.0 + 2.0 * u * u * powi(std::sin(theta), 2)) * powi(A0 / a, 2);
-    double xi2 = -1.0 * (2.0 * powi(u * A0 / a, 2) * std::sin(2.0 * theta) + 4.0 * u * A0 * A1 * std::sin(theta) / a);
     *n_out = n;
     *s_out = s;
     *theta_out = theta;
-    *pol_out = {1.0, xi1 / xi0, xi2 / xi0, 0.0};
+    *pol_out = lp_stokes(n, a, eta, s, theta, dj, nullptr);
 }
 
 // ---------------------------------------------------------------- LP nonlinear Compton, classical (lp/classical.rs)
@@ -619,6 +627,24 @@ static int lp_cl_harmonic(double a, double frac) {
     return int(std::ceil(n));
 }
 // lp/classical.rs:254-333
+// Stokes parameters of the classical LP photon at harmonic n, (v, theta), theta in
+// [0, 2 pi) (lp/classical.rs:294-330; J(n, x, y) at the signed x with the parity
+// applied on top, as the reference does)
+static SV lp_cl_stokes(int n, double a, double v, double theta, DoubleBessel& dj) {
+    double x = 2.0 * double(n) * a * std::cos(theta) * std::sqrt(v * (1.0 - v) / (1.0 + 0.5 * a * a));
+    double y = 0.25 * double(n) * a * a * v / (1.0 + 0.5 * a * a);
+    double j[5];
+    dj.evaluate(x, y, j);
+    double A0, A1, A2;
+    if (n % 2 == 0) { A0 = j[2]; A1 = -0.5 * (j[1] + j[3]); A2 = 0.25 * (j[0] + 2.0 * j[2] + j[4]); }
+    else { A0 = -j[2]; A1 = 0.5 * (j[1] + j[3]); A2 = -0.25 * (j[0] + 2.0 * j[2] + j[4]); }
+    double u = std::sqrt((1.0 + 0.5 * a * a) * (1.0 - v) / v);
+    double xi0 = 2.0 * (powi(A1, 2) - A0 * A2) - 2.0 * powi(A0 / a, 2);
+    double xi1 = 2.0 * (powi(A1, 2) - A0 * A2) - 2.0 * (1.0 + 2.0 * u * u * powi(std::sin(theta), 2)) * powi(A0 / a, 2);
+    double xi2 = 2.0 * powi(u * A0 / a, 2) * std::sin(2.0 * theta) + 4.0 * u * A0 * A1 * std::sin(theta) / a;
+    return {1.0, xi1 / xi0, xi2 / xi0, 0.0};
+}
+
 static void lp_cl_sample(double a, double eta, Xoshiro& rng, int fixed_n, int* n_out, double* s_out, double* theta_out, SV* pol_out,
                          uint64_t* trials) {
     int n = fixed_n > 0 ? fixed_n : lp_cl_harmonic(a, rng.gen_f64());
@@ -644,21 +670,10 @@ static void lp_cl_sample(double a, double eta, Xoshiro& rng, int fixed_n, int* n
     case 3: theta = 2.0 * PI - theta; break;
     default: break;
     }
-    double x = 2.0 * double(n) * a * std::cos(theta) * std::sqrt(v * (1.0 - v) / (1.0 + 0.5 * a * a));
-    double y = 0.25 * double(n) * a * a * v / (1.0 + 0.5 * a * a);
-    double j[5];
-    dj.evaluate(x, y, j);
-    double A0, A1, A2;
-    if (n % 2 == 0) { A0 = j[2]; A1 = -0.5 * (j[1] + j[3]); A2 = 0.25 * (j[0] + 2.0 * j[2] + j[4]); }
-    else { A0 = -j[2]; A1 = 0.5 * (j[1] + j[3]); A2 = -0.25 * (j[0] + 2.0 * j[2] + j[4]); }
-    double u = std::sqrt((1.0 + 0.5 * a * a) * (1.0 - v) / v);
-    double xi0 = 2.0 * (powi(A1, 2) - A0 * A2) - 2.0 * powi(A0 / a, 2);
-    double xi1 = 2.0 * (powi(A1, 2) - A0 * A2) - 2.0 * (1.0 + 2.0 * u * u * powi(std::sin(theta), 2)) * powi(A0 / a, 2);
-    double xi2 = 2.0 * powi(u * A0 / a, 2) * std::sin(2.0 * theta) + 4.0 * u * A0 * A1 * std::sin(theta) / a;
     *n_out = n;
     *s_out = s;
     *theta_out = theta;
-    *pol_out = {1.0, xi1 / xi0, xi2 / xi0, 0.0};
+    *pol_out = lp_cl_stokes(n, a, v, theta, dj);
 }
 
 // nonlinear_compton/mod.rs:19-47
@@ -1577,6 +1592,19 @@ double orc_lp_ddpr(int n, double a, double eta, double s, double theta) {
     return ddpr(a, eta, s, theta, dj);
 }
 double orc_lp_partial_rate(int n, double a, double eta, double* max_out) { return partial_rate(n, a, eta, max_out); }
+// (xi0, S1, S2, S3) of the quantum LP photon at harmonic n, (s, theta)
+void orc_lp_stokes(int n, double a, double eta, double s, double theta, double* out4) {
+    DoubleBessel dj(n, double(n) * SQRT_2, double(n) * 0.5);
+    double xi0 = 0.0;
+    SV sv = lp_stokes(n, a, eta, s, theta, dj, &xi0);
+    out4[0] = xi0; out4[1] = sv.q; out4[2] = sv.u; out4[3] = sv.v;
+}
+// (1, S1, S2, S3) of the classical LP photon at harmonic n, (v, theta)
+void orc_lp_cl_stokes(int n, double a, double v, double theta, double* out4) {
+    DoubleBessel dj(n, double(n) * SQRT_2, double(n) * 0.5);
+    SV sv = lp_cl_stokes(n, a, v, theta, dj);
+    out4[0] = sv.i; out4[1] = sv.q; out4[2] = sv.u; out4[3] = sv.v;
+}
 double orc_lp_cl_partial_rate(int n, double a, double* max_out) { return lp_cl_partial_rate(n, a, max_out); }
 double orc_lp_cl_ceiling(int n, double a) {
     DoubleBessel dj(n, double(n) * SQRT_2, double(n) * 0.5);

@@ -102,6 +102,9 @@ def lib():
         L.orc_lp_harmonic.argtypes = [d, d, d]
         L.orc_double_bessel.argtypes = [C.c_int, d, d, P(d)]
         L.orc_double_bessel_recurrence.argtypes = [C.c_int, d, d, P(d)]
+        L.orc_lp_stokes.argtypes = [C.c_int, d, d, d, d, P(d)]
+        L.orc_lp_cl_stokes.argtypes = [C.c_int, d, d, d, P(d)]
+        L.orc_cp_stokes.argtypes = [C.c_int, d, d, d, d, C.c_int, P(d)]
         L.orc_lp_ddpr.restype = d
         L.orc_lp_ddpr.argtypes = [C.c_int, d, d, d, d]
         L.orc_lp_partial_rate.restype = d
@@ -425,4 +428,25 @@ def cpbw_by_summation(a, eta):
 def cpbw_sample(a, eta, sv1, sv2, sv3, count, seed=0, fixed_n=0):
     out = np.zeros((count, 3))
     lib().orc_cpbw_sample(a, eta, sv1, sv2, sv3, seed, fixed_n, count, dptr(out))
+    return out
+
+
+def lp_stokes(n, a, eta, s, theta):
+    """(xi_0, S1, S2, S3) of the quantum LP photon at harmonic n, (s, theta)."""
+    out = np.zeros(4)
+    lib().orc_lp_stokes(int(n), a, eta, s, theta, dptr(out))
+    return out
+
+
+def lp_cl_stokes(n, a, v, theta):
+    """(1, S1, S2, S3) of the classical LP photon at harmonic n, (v, theta)."""
+    out = np.zeros(4)
+    lib().orc_lp_cl_stokes(int(n), a, v, theta, dptr(out))
+    return out
+
+
+def cp_stokes(n, a, eta, v, theta, classical=False):
+    """(1, S1, S2, S3) of the CP photon at harmonic n, fraction v of the largest s, azimuth theta."""
+    out = np.zeros(4)
+    lib().orc_cp_stokes(int(n), a, eta, v, theta, int(classical), dptr(out))
     return out

@@ -118,7 +118,9 @@ def test_rate_increase_reweights_pairs(engine, orc):
     assert _poisson_close(cg["pair_events"], cc["pair_events"])
     assert _poisson_close(cg["emissions"], cc["emissions"])
     pos_g, pos_c = sg["species"] == 1, sc["species"] == 1
-    assert np.all(sg["weight"][pos_g] <= 1.0 / 20.0 + 1e-15)
+    # each pair takes 1 / min(boost, 0.1 / prob) of its photon's weight
+    assert np.all(sg["weight"][pos_g] <= 1.0) and np.all(sg["weight"][pos_g] > 0.0)
+    assert ks_2samp(sg["weight"][pos_g], sc["weight"][pos_c]).pvalue > 1e-3
     # the weighted positron yield estimates the unbiased one
     wg, wc = sg["weight"][pos_g].sum(), sc["weight"][pos_c].sum()
     sig = math.sqrt((sg["weight"][pos_g] ** 2).sum() + (sc["weight"][pos_c] ** 2).sum()) + 1e-12
@@ -237,8 +239,11 @@ def test_t_stop_recoil_free_exact(engine, orc):
     og, oc = np.argsort(sg["id"][eg]), np.argsort(sc["id"][ec])
     rg, rc = sg["r"][eg][og], sc["r"][ec][oc]
     ug, uc = sg["u"][eg][og], sc["u"][ec][oc]
-    # every electron stopped at the first step past t_stop
-    assert np.all(rg[:, 0] >= t_stop * C_LIGHT) and np.all(rg[:, 0] < t_stop * C_LIGHT + 1.01 * C_LIGHT * dt)
+    # no electron runs past the first step beyond t_stop; those still in the field
+    # then (most of them) stop within that step, the others left the field earlier
+    ct = t_stop * C_LIGHT
+    assert np.all(rg[:, 0] < ct + 1.01 * C_LIGHT * dt)
+    assert np.mean(rg[:, 0] >= ct) > 0.9
     du = np.abs(ug - uc).max(axis=1) / np.linalg.norm(uc[:, 1:], axis=1)
     assert du.max() < 1e-12
     span = np.abs(rc - prim["r"][np.sort(sc["id"][ec]).astype(int)]).max(axis=1)
@@ -272,7 +277,11 @@ def test_lma_bandwidth_and_chirp(engine, orc, case):
     sg, _, cg = engine.collide(prim, opts, seed=14)
     sc, _, cc, _ = orc.collide(laser.desc(), opts.desc(), prim, 14, 16)
     say("  gpu", cg, "\n  cpu", cc)
-    assert cg["error_flags"] == 0
+    # a 21 % frequency spread draws a width <= 0 about once in 1e6 steps; its eta is
+    # outside the rate table, where the reference reports the lookup and uses
+    # probability 0, as the device does (PTG_FLAG_TABLE_RANGE, bit 1)
+    allowed = 2 if case.startswith("bandwidth") else 0
+    assert cg["error_flags"] & ~allowed == 0
     assert cg["lepton_steps"] == pytest.approx(cc["lepton_steps"], rel=1e-3)
     assert cg["emissions"] > 2000 and _poisson_close(cg["emissions"], cc["emissions"])
 

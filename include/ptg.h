@@ -230,6 +230,13 @@ const char* ptg_last_error(const ptg_ctx* ctx);
 int ptg_set_laser(ptg_ctx* ctx, const ptg_laser_desc* laser);
 /* Field::max_timestep / ideal_initial_z (src/field/mod.rs:130,138). */
 int ptg_field_info(ptg_ctx* ctx, double* max_timestep, double* ideal_initial_z);
+/* Field::energy (src/field/mod.rs:191): the pulse energy of `laser` in J for the
+ * focused kinds, per unit area in J/m^2 for the plane waves (*per_area = 1), as
+ * fast_focused_laser.rs:220-261 and fast_plane_wave.rs:197-240 (PlaneWave and
+ * FocusedLaser use the same formulas, plane_wave.rs:327-333, focused_laser.rs:317-323);
+ * a chirped plane wave integrates its on-axis energy density (fast_plane_wave.rs:56-85).
+ * Host arithmetic only: needs no context or device. */
+int ptg_laser_energy(const ptg_laser_desc* laser, double* energy, int32_t* per_area);
 
 /* Generates `beam->n` primaries on the device: replaces BeamBuilder::build
  * (src/particle/builder.rs:173-219). ids are 0..n-1. */

@@ -6,5 +6,5 @@ collide()/Field interface plus the build script. There is no CPU fallback:
 importing the engine without a built library raises LibraryMissing.
 """
 from ._abi import LibraryMissing  # noqa: F401
-from .engine import (Beam, CollideOptions, Engine, Envelope, Fn, Laser, LaserKind,  # noqa: F401
+from .engine import (Beam, CollideOptions, Engine, Envelope, Fn, Laser, LaserKind, laser_energy,  # noqa: F401
                      Polarization, PtgError, Species, empty_particles)

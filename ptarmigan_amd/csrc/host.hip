@@ -23,6 +23,7 @@ using namespace ptg;
 namespace {
 
 constexpr double PI_H = 3.14159265358979323846;
+constexpr double EPS0_SI = 8.854188e-12;  // vacuum permittivity, 7 digits as src/constants.rs:8
 
 // Device particle buffer with capacity management.
 struct DevBuf {
@@ -564,6 +565,102 @@ int ptg_set_laser(ptg_ctx* c, const ptg_laser_desc* d) {
         double n_fwhm = d->envelope == PTG_COS_SQUARED ? 0.36405666377387671305 * nc : nc;
         c->lma_bandwidth = std::sqrt(0.5 * LN2_D) / (PI_H * n_fwhm);
     }
+    return PTG_OK;
+}
+
+// On-axis energy density 0.5 eps0 (E^2 + c^2 B^2) of the FastPlaneWave at phase phi
+// (fast_plane_wave.rs:110-185). As the reference's field, the amplitude carries the
+// exact c^2 (SPEED_OF_LIGHT_SQD) and B = E / c with the 7-digit c, so c^2 B^2 differs
+// from E^2 by (c_sqd / c^2).
+static double fpw_axis_energy_density(const ptg_laser_desc* d, double k0, double phi) {
+    const double amplitude = (M_ELECTRON * C_LIGHT_SQD * k0 * d->a0) / E_CHARGE;
+    const double nc = d->n_cycles, b = d->chirp_b;
+    const double psi = phi * (1.0 + b * phi), dpsi = 1.0 + 2.0 * b * phi;
+    double f, df;
+    if (d->envelope == PTG_COS_SQUARED) {
+        if (std::fabs(phi) < nc * PI_H) {
+            f = std::pow(std::cos(phi / (2.0 * nc)), 2);
+            df = -std::sin(phi / nc) / (2.0 * nc);
+        } else { f = 0.0; df = 0.0; }
+    } else if (d->envelope == PTG_FLATTOP) {
+        const double ap = std::fabs(phi);
+        if (ap > PI_H * (nc + 1.0)) { f = 0.0; df = 0.0; }
+        else if (ap > PI_H * (nc - 1.0)) {
+            const double arg = 0.25 * (ap - (nc - 1.0) * PI_H);
+            f = std::pow(std::cos(arg), 2);
+            df = -std::copysign(1.0, phi) * 0.25 * std::sin(2.0 * arg);
+        } else { f = 1.0; df = 0.0; }
+    } else {
+        const double q = phi / (PI_H * nc);
+        f = std::exp2(-0.5 * q * q);
+        df = -LN2_D * phi * f / ((PI_H * nc) * (PI_H * nc));
+    }
+    const double sn = std::sin(psi), cs = std::cos(psi);
+    const double dax = sn * df + cs * dpsi * f;
+    const double day = d->polarization == PTG_CIRCULAR ? (cs * df - sn * dpsi * f) : 0.0;
+    const double e_sqd = amplitude * amplitude * (dax * dax + day * day);
+    const double b_sqd = (amplitude / C_LIGHT) * (amplitude / C_LIGHT) * (dax * dax + day * day);
+    return 0.5 * EPS0_SI * (e_sqd + C_LIGHT_SQD * b_sqd);
+}
+
+// FastPlaneWave::integrated_intensity (fast_plane_wave.rs:56-85): Simpson-weighted,
+// compensated sum of the on-axis energy density over the pulse.
+static double fpw_integrated_intensity(const ptg_laser_desc* d, double k0, int ppw) {
+    const double nc = d->n_cycles;
+    const double max_phase = d->envelope == PTG_COS_SQUARED ? PI_H * nc
+                             : (d->envelope == PTG_FLATTOP ? PI_H * (nc + 1.0) : 6.0 * PI_H * nc);
+    const double dphi = 2.0 * PI_H / double(ppw), dz = dphi / k0;
+    const int64_t n_pts = int64_t(max_phase / dphi);
+    double sum = 0.0, comp = 0.0;
+    for (int64_t n = -n_pts; n < n_pts; ++n) {
+        const double u = fpw_axis_energy_density(d, k0, double(n) * dphi);
+        const double w = (n % 2 == 0) ? 2.0 / 3.0 : 4.0 / 3.0;
+        const double y = w * u * dz - comp;
+        const double t = sum + y;
+        comp = (t - sum) - y;
+        sum = t;
+    }
+    return sum;
+}
+
+int ptg_laser_energy(const ptg_laser_desc* d, double* energy, int32_t* per_area) {
+    if (!d || !energy) return PTG_E_ARG;
+    if (d->kind < 0 || d->kind > 3 || d->envelope < 0 || d->envelope > 2) return PTG_E_ARG;
+    if (!(d->a0 > 0.0) || !(d->wavelength > 0.0) || !(d->n_cycles > 0.0)) return PTG_E_ARG;
+    const bool focused = d->kind == PTG_FOCUSED_LASER || d->kind == PTG_FAST_FOCUSED_LASER;
+    if (focused && !(d->waist > 0.0)) return PTG_E_ARG;
+    const double k0 = 2.0 * PI_H / d->wavelength, omega = C_LIGHT * k0;
+    const double amplitude = (M_ELECTRON * C_LIGHT * omega * d->a0) / E_CHARGE;
+    // the focused fields' n_cycles is recomputed from their duration (fast_focused_laser.rs:47-53)
+    const double nc = focused ? C_LIGHT * (d->n_cycles * d->wavelength / C_LIGHT) * k0 / (2.0 * PI_H) : d->n_cycles;
+    if (per_area) *per_area = focused ? 0 : 1;
+    // a chirped plane wave (the LMA plane wave keeps its chirp; the focused fields drop it)
+    if (!focused && d->chirp_b != 0.0) {
+        const int ppw = int(10.0 * (1.0 + 2.0 * PI_H * d->chirp_b * d->n_cycles));
+        *energy = fpw_integrated_intensity(d, k0, ppw);
+        return PTG_OK;
+    }
+    const double intensity = C_LIGHT * EPS0_SI * amplitude * amplitude;
+    const double delta = d->polarization == PTG_CIRCULAR ? 1.0 : 0.0;
+    double duration;
+    if (d->envelope == PTG_COS_SQUARED) {
+        duration = (1.0 + delta) * ((1.0 + 3.0 * nc * nc) * PI_H / (8.0 * nc)) / omega;
+    } else if (d->envelope == PTG_GAUSSIAN) {
+        const double arg = -(PI_H * nc) * (PI_H * nc) / LN2_D;
+        const double large = 0.5 * nc * std::sqrt(PI_H * PI_H * PI_H / LN2_D);
+        const double phase_x = large - std::expm1(arg) * std::sqrt(LN2_D / PI_H) / (4.0 * nc);
+        const double phase_y = large + (1.0 + std::exp(arg)) * std::sqrt(LN2_D / PI_H) / (4.0 * nc);
+        duration = (phase_x + delta * phase_y) / omega;
+    } else {
+        duration = (1.0 + delta) * ((nc - 3.0 / 16.0) * PI_H) / omega;
+    }
+    if (!focused) {
+        *energy = intensity * duration;
+        return PTG_OK;
+    }
+    const double zr = 0.5 * k0 * d->waist * d->waist, e = d->waist / zr;
+    const double power = 0.5 * PI_H * intensity * d->waist * d->waist * (1.0 + e * e / 4.0 + std::pow(e, 4) / 8.0);
+    *energy = power * duration;
     return PTG_OK;
 }
 

@@ -59,52 +59,77 @@ __device__ __forceinline__ void wave_add(unsigned long long* p, uint64_t v) {
     if (v) atomicAdd(p, (unsigned long long)v);
 }
 
-// Slot reservation with one atomic per wave: the active lanes that name the same
-// counter and count are ranked by mbcnt over their ballot, the first of them adds
-// the group's total and broadcasts the base with readfirstlane, and each lane takes
-// base + rank * cnt. The loop peels one (counter, count) group per pass, so lanes
-// that reach this point from different call sites (branches the compiler merged)
-// are still served correctly; when all active lanes agree it runs once.
+// Lane-uniform reads: the builtins take and return int, which must not be
+// sign-extended when widened to 64 bits.
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int lane) {
+    return uint32_t(__builtin_amdgcn_readlane(int(v), lane));
+}
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int lane) {
+    return (uint64_t(readlane_u32(uint32_t(v >> 32), lane)) << 32) | uint64_t(readlane_u32(uint32_t(v), lane));
+}
+
+// Slot reservation with one atomic per wave and counter: every pass of the loop
+// takes the lowest lane still waiting as leader, reads its counter and count
+// (readlane at a lane index that is uniform, being derived from a ballot), and
+// serves every waiting lane that names the same pair: their ranks come from mbcnt
+// over the ballot of matches, the leader adds the group's total and the base is
+// read back from the leader. The loop runs while a ballot of waiting lanes is
+// non-zero, so its control flow is uniform and every ballot inside it sees the
+// lanes it was written for (lanes reaching this point from different call sites,
+// which the compiler may merge, are served in turn; when all agree one pass
+// does). Each pass serves at least its leader, so 64 passes serve a whole wave;
+// the per-lane atomic after them only bounds the loop.
 // Slots within a wave follow lane order; across waves they follow the order in
 // which the waves' atomics land.
 __device__ __forceinline__ uint64_t wave_reserve(unsigned long long* ctr, uint32_t cnt) {
     const uint64_t addr = reinterpret_cast<uint64_t>(ctr);
+    const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    bool pending = true;
     uint64_t slot = 0;
-    for (;;) {
-        const uint64_t lead = (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(addr >> 32))) << 32) |
-                              uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(addr)));
-        const uint32_t lead_cnt = __builtin_amdgcn_readfirstlane(cnt);
-        if (lead == addr && lead_cnt == cnt) {
-            const uint64_t mask = __ballot(1);
-            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
-            unsigned long long base = 0;
-            if (rank == 0) base = atomicAdd(ctr, (unsigned long long)(__popcll(mask)) * cnt);
-            base = (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(base >> 32))) << 32) |
-                   uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(base)));
+    for (int pass = 0; pass < 64; ++pass) {
+        const uint64_t waiting = __ballot(pending);
+        if (waiting == 0) break;
+        const int leader = __ffsll((long long)waiting) - 1;
+        const uint64_t lead_addr = readlane_u64(addr, leader);
+        const uint32_t lead_cnt = readlane_u32(cnt, leader);
+        const bool match = pending && addr == lead_addr && cnt == lead_cnt;
+        const uint64_t group = __ballot(match);
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(group >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(group), 0u));
+        unsigned long long base = 0;
+        if (lane == uint32_t(leader)) base = atomicAdd(ctr, (unsigned long long)(__popcll(group)) * lead_cnt);
+        base = readlane_u64(base, leader);
+        if (match) {
             slot = base + uint64_t(rank) * cnt;
-            break;
+            pending = false;
         }
     }
+    if (pending) slot = atomicAdd(ctr, (unsigned long long)cnt);
     return slot;
 }
 // 32-bit counter form (work counters): same scheme.
 __device__ __forceinline__ uint32_t wave_reserve32(uint32_t* ctr, uint32_t cnt) {
     const uint64_t addr = reinterpret_cast<uint64_t>(ctr);
+    const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    bool pending = true;
     uint32_t slot = 0;
-    for (;;) {
-        const uint64_t lead = (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(addr >> 32))) << 32) |
-                              uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(addr)));
-        const uint32_t lead_cnt = __builtin_amdgcn_readfirstlane(cnt);
-        if (lead == addr && lead_cnt == cnt) {
-            const uint64_t mask = __ballot(1);
-            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
-            uint32_t base = 0;
-            if (rank == 0) base = atomicAdd(ctr, uint32_t(__popcll(mask)) * cnt);
-            base = __builtin_amdgcn_readfirstlane(base);
+    for (int pass = 0; pass < 64; ++pass) {
+        const uint64_t waiting = __ballot(pending);
+        if (waiting == 0) break;
+        const int leader = __ffsll((long long)waiting) - 1;
+        const uint64_t lead_addr = readlane_u64(addr, leader);
+        const uint32_t lead_cnt = readlane_u32(cnt, leader);
+        const bool match = pending && addr == lead_addr && cnt == lead_cnt;
+        const uint64_t group = __ballot(match);
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(group >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(group), 0u));
+        uint32_t base = 0;
+        if (lane == uint32_t(leader)) base = atomicAdd(ctr, uint32_t(__popcll(group)) * lead_cnt);
+        base = readlane_u32(base, leader);
+        if (match) {
             slot = base + rank * cnt;
-            break;
+            pending = false;
         }
     }
+    if (pending) slot = atomicAdd(ctr, cnt);
     return slot;
 }
 

@@ -673,6 +673,30 @@ def ident(deck, path=None):
     return deck.name if s == "auto" else s
 
 
+Q_ABSORPTION, Q_WEIGHT = 23, 29   # ptg_quantity ids (include/ptg.h)
+ELEMENTARY_CHARGE = 1.602177e-19
+
+
+def laser_depletion(laser, absorbed_mev):
+    """The driver's depletion check (src/main.rs:1198-1215): the weighted energy the
+    particles absorbed from the laser (MeV, over the final state and the kept
+    intermediates) in J, over Field::energy; above 0.1 the driver warns that the
+    background-field approximation is likely to be invalid. Returns the fraction."""
+    from .engine import laser_energy
+    energy, _ = laser_energy(laser)
+    f_abs = absorbed_mev * 1.0e6 * ELEMENTARY_CHARGE / energy
+    if f_abs > 0.1:
+        warnings.warn(f"obtained laser energy depletion of {100.0 * f_abs:.2f}%.\n"
+                      "         Background field approximation likely to be invalid.")
+    return f_abs
+
+
+def _absorbed_mev(eng):
+    """Sum of weight x absorbed energy (MeV) over the resident final state and the
+    intermediates (bunch.iter_all_mut(), src/main.rs:1143-1147)."""
+    return sum(float(eng.stats(sp, Q_ABSORPTION, Q_WEIGHT)[0]) for sp in (0, 1, 2, 3))
+
+
 def run_deck(deck, engine=None, out_dir=None):
     """Transports every collision of the deck on the GPU and returns
     [(a0, final-state particles, counters)]. Several a0 values of an LMA field
@@ -703,13 +727,17 @@ def run_deck(deck, engine=None, out_dir=None):
         starts = np.cumsum([0] + [c[1].n for c in cols])
         coll = np.searchsorted(starts, sec["shower"], side="right") - 1
         for k, (laser, _, _) in enumerate(cols):
-            out.append((laser.a0, {key: v[coll == k] for key, v in sec.items()}, cnt))
+            part = {key: v[coll == k] for key, v in sec.items()}
+            c = dict(cnt)
+            c["laser_depletion"] = laser_depletion(laser, float(np.sum(part["weight"] * part["work"])) * 0.510999)
+            out.append((laser.a0, part, c))
         return out
     for laser, beam, opts in cols:
         eng.set_laser(laser)
         eng.set_beam_spectrum(*deck.spectrum)
         eng.build_beam(beam, seed=seed)
         cnt = eng.run(opts, seed=seed)
+        cnt["laser_depletion"] = laser_depletion(laser, _absorbed_mev(eng))
         eng.finish(stop_mode, t_stop, deck.beam_basis, beam.angle, beam.collision_plane_angle)
         sec, inter = eng.fetch(cnt, intermediates=opts.keep_decayed_photons)
         if writing:

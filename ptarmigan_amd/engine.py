@@ -171,6 +171,18 @@ def particles_struct(arrs):
     return _abi.Particles(**kw)
 
 
+def laser_energy(laser: Laser):
+    """Field::energy (src/field/mod.rs:191) of `laser`: (energy, unit), unit "J" for
+    the focused fields and "J/m^2" for the plane waves. Host arithmetic; needs no
+    device."""
+    lib = _abi.load()
+    e, per_area = C.c_double(), C.c_int32()
+    rc = lib.ptg_laser_energy(C.byref(laser.desc()), C.byref(e), C.byref(per_area))
+    if rc != 0:
+        raise PtgError(rc, "ptg_laser_energy: invalid laser parameters")
+    return e.value, ("J/m^2" if per_area.value else "J")
+
+
 def _dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 

@@ -280,6 +280,58 @@ def test_deck_runs_an_a0_loop_as_one_batch(engine):
         assert abs(y - y1) < 5.0 * math.sqrt(2.0 * max(y, y1)), (laser.a0, y, y1)
 
 
+FOCUSED_DECK = """---
+control:
+  lcfa: true
+  pair_creation: false
+  rng_seed: 3
+laser:
+  a0: 10.0
+  wavelength: 0.8 * micro
+  waist: 4.0 * micro
+  fwhm_duration: 30.0 * femto
+  polarization: linear
+beam:
+  n: 4000
+  species: electron
+  charge: 1.0 * nano
+  gamma: 16.5 * GeV / (me * c^2)
+  radius: 2.0 * micro
+  length: 10.0 * micro
+"""
+
+
+@pytest.mark.gpu
+def test_deck_reports_laser_depletion(engine):
+    """The driver's depletion fraction (src/main.rs:1136-1147, 1198-1215): the
+    weighted absorbed energy of the final state, from the device's reduction,
+    against the same sum over the fetched particles, over Field::energy."""
+    from ptarmigan_amd import laser_energy
+    d = D.Deck.from_string(FOCUSED_DECK)
+    (a0, part, cnt), = D.run_deck(d, engine)
+    laser = d.collisions()[0][0][0]
+    absorbed_j = float(np.sum(part["weight"] * part["work"])) * 0.510999 * 1e6 * 1.602177e-19
+    energy, unit = laser_energy(laser)
+    print("depletion", cnt["laser_depletion"], absorbed_j, energy, unit, flush=True)
+    assert unit == "J" and energy > 0.0
+    assert cnt["laser_depletion"] == pytest.approx(absorbed_j / energy, rel=1e-9, abs=1e-300)
+    assert absorbed_j != 0.0
+
+
+def test_depletion_warning_threshold():
+    """Above a 10 % depletion the driver warns (src/main.rs:1208-1215)."""
+    import warnings
+    from ptarmigan_amd import Envelope, Laser, LaserKind, Polarization, laser_energy
+    laser = Laser(LaserKind.FAST_FOCUSED_LASER, 10.0, 0.8e-6, 11.24, 4e-6, Envelope.GAUSSIAN, Polarization.LINEAR)
+    e, _ = laser_energy(laser)
+    mev = e / (1e6 * 1.602177e-19)
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        assert D.laser_depletion(laser, 0.05 * mev) == pytest.approx(0.05)
+    with pytest.warns(UserWarning, match="depletion of 20.00%"):
+        assert D.laser_depletion(laser, 0.2 * mev) == pytest.approx(0.2)
+
+
 def test_command_line_reports_bad_decks(tmp_path, capsys):
     from ptarmigan_amd.__main__ import main
     assert main([]) == 2

@@ -1,0 +1,67 @@
+"""Photon Stokes parameters of the device's LMA emission samplers, value by value:
+each device sample (n, s, theta) is fed to the oracle's Stokes functions, which
+tests/test_oracle_stokes.py pins by the Thomson-limit dipole, the classical limits
+and the double-differential rate (lp/mod.rs:622-671, lp/classical.rs:294-330,
+cp/mod.rs:203-232, cp/classical.rs:155-175). Tolerance 1e-9: the device and the
+oracle evaluate the same J(n, x, y) / J_n recurrences in a different order."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("a,eta,fixed_n", [(1e-3, 1e-4, 1), (0.5, 0.1, 0), (2.0, 0.1, 0), (2.0, 0.3, 8),
+                                           (5.0, 0.05, 0), (10.0, 0.02, 0)])
+def test_lp_stokes_at_device_samples(engine, orc, a, eta, fixed_n):
+    from ptarmigan_amd import Fn
+    say("lp stokes", a, eta, fixed_n)
+    m = 3000
+    inp = np.stack([np.full(m, a), np.full(m, eta), np.arange(m, dtype=float), np.full(m, float(fixed_n))], 1)
+    got = engine.eval(Fn.LP_SAMPLE, inp, 6)
+    ref = np.array([orc.lp_stokes(int(r[0]), a, eta, r[1], r[2]) for r in got])
+    assert np.all(ref[:, 0] > 0.0)
+    err = np.abs(got[:, 3:5] - ref[:, 1:3]).max()
+    say("  max |dS|", err)
+    assert err < 1e-9
+
+
+@pytest.mark.parametrize("a,eta,fixed_n", [(0.5, 0.05, 0), (3.0, 0.05, 0), (3.0, 0.05, 6)])
+def test_lp_classical_stokes_at_device_samples(engine, orc, a, eta, fixed_n):
+    """Classical LP; the reference's saddle-point branch returns NaN for some theta
+    near pi at a >= 3 (reproduced on both sides), which the comparison skips."""
+    from ptarmigan_amd import Fn
+    say("lp classical stokes", a, eta, fixed_n)
+    m = 3000
+    inp = np.stack([np.full(m, a), np.full(m, eta), np.arange(m, dtype=float), np.full(m, float(fixed_n))], 1)
+    got = engine.eval(Fn.LP_CL_SAMPLE, inp, 6)
+    smax = 2.0 * got[:, 0] * eta / (1.0 + 0.5 * a * a)
+    ref = np.array([orc.lp_cl_stokes(int(r[0]), a, r[1] / sm, r[2]) for r, sm in zip(got, smax)])
+    both = np.isfinite(ref[:, 1]) & np.isfinite(ref[:, 2])
+    assert np.array_equal(np.isfinite(got[:, 3]), np.isfinite(ref[:, 1]))
+    assert both.mean() > 0.95
+    assert np.abs(got[both, 3:5] - ref[both, 1:3]).max() < 1e-9
+
+
+@pytest.mark.parametrize("a,eta,fixed_n,classical", [(1e-3, 1e-4, 1, False), (1.0, 0.2, 0, False), (8.0, 0.05, 0, False),
+                                                     (2.0, 0.05, 0, True)])
+def test_cp_stokes_at_device_samples(engine, orc, a, eta, fixed_n, classical):
+    from ptarmigan_amd import Fn
+    say("cp stokes", a, eta, fixed_n, classical)
+    m = 3000
+    inp = np.stack([np.full(m, a), np.full(m, eta), np.arange(m, dtype=float), np.full(m, float(fixed_n)),
+                    np.full(m, float(classical))], 1)
+    got = engine.eval(Fn.CP_SAMPLE, inp, 7)
+    n = got[:, 0]
+    sn = 2.0 * n * eta / (1.0 + a * a)
+    v = got[:, 1] / sn if classical else got[:, 1] / (sn / (1.0 + sn))
+    ref = np.array([orc.cp_stokes(int(n[k]), a, eta, v[k], got[k, 2], classical) for k in range(m)])
+    err = np.abs(got[:, 3:6] - ref[:, 1:4]).max()
+    say("  max |dS|", err)
+    assert err < 1e-9

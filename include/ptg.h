@@ -276,8 +276,19 @@ int ptg_push(ptg_ctx* ctx, const uint8_t* species, double* r, double* u, size_t 
              const ptg_collide_options* opt, uint32_t* steps_taken, double* work);
 
 /* Weighted histogram of the resident final-state particles (hgram.rs:209-323).
- * bins_out has nbins entries (min/max written back when computed). */
+ * bins_out has nbins entries, nbins <= 2^20 (min/max written back when computed). */
 int ptg_histogram(ptg_ctx* ctx, ptg_hist_spec* spec, double* bins_out, double* total_weight);
+/* ptg_histogram without allocating anything: no counterpart in the reference.
+ * For callers that cannot allocate device or host memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The bins
+ * accumulator and range keys live in a fixed device array of the library,
+ * (PTG_NOALLOC_MAX_BINS + 3) x 8 bytes per GPU, held until the process ends;
+ * results go straight into the caller's bins_out / total_weight. nbins above
+ * PTG_NOALLOC_MAX_BINS is refused with PTG_E_ARG. Every call on one GPU shares
+ * that array and clears it first: calls that overlap on one GPU (two host
+ * threads, two contexts on different streams) race, so the caller serialises them. */
+#define PTG_NOALLOC_MAX_BINS 8192
+int ptg_histogram_noalloc(ptg_ctx* ctx, ptg_hist_spec* spec, double* bins_out, double* total_weight);
 /* Weighted 2-D histogram (Histogram::generate_2d, hgram.rs:326-446): species and
  * weight from spec_x; bins_out has nbins_x * nbins_y entries, flat index
  * by * nbins_y + bx as the reference (rows overlap when the counts differ). */

@@ -105,7 +105,7 @@ EXPORTS = ("ptg_create", "ptg_destroy", "ptg_last_error", "ptg_set_laser", "ptg_
            "ptg_load_primaries", "ptg_run", "ptg_collide", "ptg_fetch", "ptg_push", "ptg_histogram", "ptg_eval",
            "ptg_surrogate_train", "ptg_surrogate_load", "ptg_mlp_probe", "ptg_set_batch", "ptg_autotune",
            "ptg_fetch_showers", "ptg_histogram_2d", "ptg_stats", "ptg_finish",
-           "ptg_set_beam_spectrum", "ptg_laser_energy")
+           "ptg_set_beam_spectrum", "ptg_histogram_noalloc", "ptg_laser_energy")
 
 _lib = None
 
@@ -135,6 +135,7 @@ def load():
     L.ptg_fetch.argtypes = [vp, P(Particles), P(Particles)]
     L.ptg_push.argtypes = [vp, P(C.c_uint8), P(d), P(d), C.c_size_t, C.c_uint32, P(CollideOptions), P(C.c_uint32), P(d)]
     L.ptg_histogram.argtypes = [vp, P(HistSpec), P(d), P(d)]
+    L.ptg_histogram_noalloc.argtypes = [vp, P(HistSpec), P(d), P(d)]
     L.ptg_laser_energy.argtypes = [P(LaserDesc), P(d), P(C.c_int32)]
     L.ptg_laser_energy.restype = C.c_int
     L.ptg_eval.argtypes = [vp, C.c_int, P(d), C.c_size_t, C.c_size_t, P(d), C.c_size_t]

@@ -83,6 +83,13 @@ void free_buf(DevBuf& b) {
 enum { C_WORK = 0, C_NEXT, C_OUT, C_INTER, C_ID, C_FLAGS, C_EV, C_LNEXT, C_PH, C_OVF, C_OVFLEN, C_PAIRS, C_SROWS, C_SFB, C_STATS,
        C_N = C_STATS + STAT_N };
 
+// Device scratch of ptg_histogram_noalloc: words [0, nbins) are the bins and
+// [nbins] the total weight (doubles), [PTG_NOALLOC_MAX_BINS + 1] and [+ 2] the
+// order keys of the range search. Each GPU holds its own copy until the process
+// ends; host code reaches it only through the address hipGetSymbolAddress gives
+// on that device (ptg_ctx::d_noalloc), never through this name.
+static __device__ __align__(256) unsigned long long scratch[PTG_NOALLOC_MAX_BINS + 3];
+
 std::string default_table_path() {
     Dl_info info;
     if (dladdr(reinterpret_cast<void*>(&default_table_path), &info) && info.dli_fname) {
@@ -136,6 +143,9 @@ struct ptg_ctx {
     TransportArgs* d_args = nullptr;
     uint64_t n_out = 0, n_inter = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double* d_hist = nullptr;               // ptg_histogram's bins and total, grown as needed
+    size_t hist_cap = 0;
+    unsigned long long* d_noalloc = nullptr;  // this device's copy of `scratch` (ptg_histogram_noalloc)
     // learned sampler, one per polarization (0 linear, 1 circular)
     MlpNet surr[2];
     bool surr_loaded[2] = {false, false};
@@ -475,6 +485,12 @@ int ptg_create(ptg_ctx** out, const char* table_path, int device) {
         return PTG_E_HIP;
     }
     HIPCHK(c, hipSetDevice(device));
+    {
+        // this device's address of the no-allocation scratch array (once, at set-up)
+        void* p = nullptr;
+        HIPCHK(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        c->d_noalloc = static_cast<unsigned long long*>(p);
+    }
     hipDeviceProp_t prop;
     HIPCHK(c, hipGetDeviceProperties(&prop, device));
     c->n_cu = prop.multiProcessorCount;
@@ -542,6 +558,7 @@ void ptg_destroy(ptg_ctx* c) {
     if (c->d_shower_count) (void)hipFree(c->d_shower_count);
     if (c->d_ctr) (void)hipFree(c->d_ctr);
     if (c->d_args) (void)hipFree(c->d_args);
+    if (c->d_hist) (void)hipFree(c->d_hist);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1562,11 +1579,11 @@ static int hist_cut_scale(ptg_ctx* c, HistArgs& h, const ptg_hist_spec* cut, con
     return PTG_OK;
 }
 
-int ptg_histogram(ptg_ctx* c, ptg_hist_spec* s, double* bins_out, double* total_weight) {
-    if (!c || !s || !bins_out) return fail(c, PTG_E_ARG, "null argument");
-    if (s->nbins == 0 || s->nbins > 8192) return fail(c, PTG_E_ARG, "nbins must be in [1, 8192]");
-    if (s->quantity < 0 || s->quantity >= PTG_Q_COUNT) return fail(c, PTG_E_ARG, "unknown quantity");
-    HIPCHK(c, hipSetDevice(c->device));
+// Histogram::generate_1d over the resident final state into the caller's host
+// buffers, with the bins accumulator at d_bins (nbins + 1 doubles) and the two
+// range keys at d_keys on the device. Allocates nothing; clears with kernels.
+static int histogram_into(ptg_ctx* c, ptg_hist_spec* s, double* bins_out, double* total_weight, double* d_bins,
+                          unsigned long long* d_keys) {
     HistArgs h{};
     // species 3: the intermediate (decayed) photons kept by keep_decayed_photons
     h.p = s->species == 3 ? c->inter.v : c->out.v;
@@ -1581,10 +1598,11 @@ int ptg_histogram(ptg_ctx* c, ptg_hist_spec* s, double* bins_out, double* total_
     double gmin = s->min, gmax = s->max;
     if (!s->fixed_range) {
         unsigned long long mm[2] = {~0ull, 0ull};
-        HIPCHK(c, scopy(c, c->d_ctr + C_WORK, mm, sizeof(mm), hipMemcpyHostToDevice));
-        if (h.n) launch_hist_minmax(h, c->d_ctr + C_WORK, grid, c->stream);
+        launch_fill_u64(d_keys, 1, ~0ull, c->stream);
+        launch_fill_u64(d_keys + 1, 1, 0ull, c->stream);
+        if (h.n) launch_hist_minmax(h, d_keys, grid, c->stream);
         HIPCHK(c, hipGetLastError());
-        HIPCHK(c, scopy(c, mm, c->d_ctr + C_WORK, sizeof(mm), hipMemcpyDeviceToHost));
+        HIPCHK(c, scopy(c, mm, d_keys, sizeof(mm), hipMemcpyDeviceToHost));
         if (mm[0] == ~0ull) {  // nothing to bin (reference returns None)
             for (uint32_t b = 0; b < s->nbins; ++b) bins_out[b] = 0.0;
             if (total_weight) *total_weight = 0.0;
@@ -1598,20 +1616,53 @@ int ptg_histogram(ptg_ctx* c, ptg_hist_spec* s, double* bins_out, double* total_
     h.nbins = nbins;
     h.gmin = gmin;
     h.bin_vol = gmin == gmax ? 1.0 : (gmax - gmin) / double(nbins);
-    double* d_bins = nullptr;
-    HIPCHK(c, hipMalloc(&d_bins, (nbins + 1) * sizeof(double)));
-    HIPCHK(c, hipMemsetAsync(d_bins, 0, (nbins + 1) * sizeof(double), c->stream));
+    launch_fill_u64(reinterpret_cast<unsigned long long*>(d_bins), nbins + 1, 0ull, c->stream);  // +0.0
     if (h.n) launch_hist_bin(h, d_bins, d_bins + nbins, grid, c->stream);
     HIPCHK(c, hipGetLastError());
-    std::vector<double> hb(nbins + 1);
-    HIPCHK(c, hipMemcpyAsync(hb.data(), d_bins, (nbins + 1) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    double tot = 0.0;
+    HIPCHK(c, hipMemcpyAsync(bins_out, d_bins, nbins * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&tot, d_bins + nbins, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    (void)hipFree(d_bins);
-    for (uint32_t b = 0; b < nbins; ++b) bins_out[b] = hb[b];
-    if (total_weight) *total_weight = hb[nbins];
+    if (total_weight) *total_weight = tot;
     s->min = gmin;
     s->max = gmax;
     return PTG_OK;
+}
+
+static int check_hist_spec(ptg_ctx* c, const ptg_hist_spec* s, uint32_t max_bins) {
+    if (s->nbins == 0 || s->nbins > max_bins)
+        return fail(c, PTG_E_ARG, "nbins must be in [1, " + std::to_string(max_bins) + "]");
+    if (s->quantity < 0 || s->quantity >= PTG_Q_COUNT) return fail(c, PTG_E_ARG, "unknown quantity");
+    if (s->species < 0 || s->species > 3) return fail(c, PTG_E_ARG, "bad species");
+    return PTG_OK;
+}
+
+int ptg_histogram(ptg_ctx* c, ptg_hist_spec* s, double* bins_out, double* total_weight) {
+    if (!c || !s || !bins_out) return fail(c, PTG_E_ARG, "null argument");
+    if (int rc = check_hist_spec(c, s, 1u << 20)) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->hist_cap < size_t(s->nbins) + 3) {
+        if (c->d_hist) (void)hipFree(c->d_hist);
+        c->d_hist = nullptr;
+        c->hist_cap = 0;
+        const size_t cap = std::max<size_t>(size_t(s->nbins) + 3, 8192 + 3);
+        HIPCHK(c, hipMalloc(&c->d_hist, cap * sizeof(double)));
+        c->hist_cap = cap;
+    }
+    return histogram_into(c, s, bins_out, total_weight, c->d_hist,
+                          reinterpret_cast<unsigned long long*>(c->d_hist + c->hist_cap - 2));
+}
+
+int ptg_histogram_noalloc(ptg_ctx* c, ptg_hist_spec* s, double* bins_out, double* total_weight) {
+    if (!c || !s || !bins_out) return fail(c, PTG_E_ARG, "null argument");
+    if (s->nbins > PTG_NOALLOC_MAX_BINS)
+        return fail(c, PTG_E_ARG, "ptg_histogram_noalloc bins at most PTG_NOALLOC_MAX_BINS (" +
+                                      std::to_string(PTG_NOALLOC_MAX_BINS) + ") values; use ptg_histogram for more");
+    if (int rc = check_hist_spec(c, s, PTG_NOALLOC_MAX_BINS)) return rc;
+    if (!c->d_noalloc) return fail(c, PTG_E_STATE, "no scratch address for this device");
+    HIPCHK(c, hipSetDevice(c->device));
+    return histogram_into(c, s, bins_out, total_weight, reinterpret_cast<double*>(c->d_noalloc),
+                          c->d_noalloc + PTG_NOALLOC_MAX_BINS + 1);
 }
 
 int ptg_finish(ptg_ctx* c, int stop_mode, double t_stop, int beam_basis, double theta, double phi, double* t_common) {

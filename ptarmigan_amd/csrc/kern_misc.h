@@ -45,7 +45,10 @@ struct EvalArgs {
 void launch_build_beam(const BeamDev& b, const PBuf& p, double* n0, hipStream_t s);
 void launch_primary_dirs(const PBuf& p, uint32_t n, double* n0, hipStream_t s);
 void launch_hist_minmax(const HistArgs& h, unsigned long long* mm, int grid, hipStream_t s);
+// bins held in LDS per workgroup up to this many (64 KB), global atomics above
+constexpr uint32_t HIST_LDS_BINS = 8192;
 void launch_hist_bin(const HistArgs& h, double* bins, double* total, int grid, hipStream_t s);
+void launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t s);
 // 2-D weighted binning (hgram.rs:326-446): hx gives species, weight and the x axis,
 // hy the y axis; flat bin by * ny + bx as the reference, dropped past nx * ny.
 // drift to c t (when drift) then rotate by R_x(-vangle) R_y(hangle) (when rotate)

@@ -264,10 +264,14 @@ __global__ void k_hist_minmax(HistArgs h, unsigned long long* mm) {
 // Weighted binning with per-workgroup bins in LDS, merged with one global atomic
 // per bin per workgroup (hgram.rs:268-298). The total counts all weights of the
 // species, binned or not.
+// Above HIST_LDS_BINS bins the kernel bins straight into global memory.
+template <bool LDS>
 __global__ void k_hist_bin(HistArgs h, double* bins, double* total) {
     extern __shared__ double lbins[];
-    for (uint32_t b = threadIdx.x; b < h.nbins; b += blockDim.x) lbins[b] = 0.0;
-    __syncthreads();
+    if (LDS) {
+        for (uint32_t b = threadIdx.x; b < h.nbins; b += blockDim.x) lbins[b] = 0.0;
+        __syncthreads();
+    }
     double tot = 0.0;
     for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < h.n; i += uint64_t(gridDim.x) * blockDim.x) {
         if ((h.p.meta[i] & 3u) != uint32_t(h.species)) continue;
@@ -277,13 +281,18 @@ __global__ void k_hist_bin(HistArgs h, double* bins, double* total) {
         if (!isfinite(v) || !hcut(h, i)) continue;
         double fb = floor((v - h.gmin) / h.bin_vol);
         if (!(fb >= 0.0)) fb = 0.0;  // Rust `as usize` saturates at 0
-        if (fb < double(h.nbins)) atomicAdd(&lbins[uint32_t(fb)], w);
+        if (fb < double(h.nbins)) {
+            if (LDS) atomicAdd(&lbins[uint32_t(fb)], w);
+            else atomicAdd(&bins[uint32_t(fb)], w);
+        }
     }
     for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
     if ((threadIdx.x & 63) == 0 && tot != 0.0) atomicAdd(total, tot);
-    __syncthreads();
-    for (uint32_t b = threadIdx.x; b < h.nbins; b += blockDim.x)
-        if (lbins[b] != 0.0) atomicAdd(&bins[b], lbins[b]);
+    if (LDS) {
+        __syncthreads();
+        for (uint32_t b = threadIdx.x; b < h.nbins; b += blockDim.x)
+            if (lbins[b] != 0.0) atomicAdd(&bins[b], lbins[b]);
+    }
 }
 
 // 2-D: a value pair is binned only if both are finite; the flat index is
@@ -426,7 +435,17 @@ void launch_hist_minmax(const HistArgs& h, unsigned long long* mm, int grid, hip
     k_hist_minmax<<<grid, 256, 0, s>>>(h, mm);
 }
 void launch_hist_bin(const HistArgs& h, double* bins, double* total, int grid, hipStream_t s) {
-    k_hist_bin<<<grid, 256, h.nbins * sizeof(double), s>>>(h, bins, total);
+    if (h.nbins <= HIST_LDS_BINS) k_hist_bin<true><<<grid, 256, h.nbins * sizeof(double), s>>>(h, bins, total);
+    else k_hist_bin<false><<<grid, 256, 0, s>>>(h, bins, total);
+}
+
+// n copies of v (clears counters and accumulators with a kernel, which graph
+// capture records as an ordinary kernel node)
+__global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v) {
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) p[i] = v;
+}
+void launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t s) {
+    if (n) k_fill_u64<<<int(std::min<uint64_t>(1024, (n + 255) / 256)), 256, 0, s>>>(p, n, v);
 }
 double host_order_key_inv(unsigned long long k) {
     unsigned long long b = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;

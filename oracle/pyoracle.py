@@ -56,6 +56,15 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def use_library(path):
+    """Loads the oracle from `path` (a build of the same sources, e.g. the
+    -O3 -march=native timing build) instead of liboracle.so; call before lib()."""
+    global LIB, _lib
+    if _lib is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    LIB = path
+
+
 def lib():
     global _lib
     if _lib is None:

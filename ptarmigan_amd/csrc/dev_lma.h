@@ -491,16 +491,20 @@ __device__ inline int lp_harmonic(P cdf, const LmaScalars& sc, double a, double 
 // coefficients are stored, in per-lane scratch `ws` with stride `st` between
 // consecutive doubles of one lane (so a wave's accesses are contiguous); the
 // backward pass keeps running sums instead of the reference's stored f, g, h.
-struct DBessel {
+// The reference's backward recurrence for J(n - 2 .. n + 2, x, y)
+// (double_bessel.rs:142-246) over the orders of its workspace. The kernels
+// evaluate J by quadrature (dbessel_quadrature below); the recurrence stays as
+// a probe that the parity tests hold the quadrature against.
+struct DBesselRec {
     int n, k_min, len;
     double* ws;   // 5 * len doubles, element (i, c) at ws[(5 * i + c) * st]
     size_t st;
 };
-PTG_HD __forceinline__ DBessel dbessel_at(int n, double* ws, size_t st) {
+PTG_HD __forceinline__ DBesselRec dbessel_rec_at(int n, double* ws, size_t st) {
     double xm = double(n) * 1.4142135623730951, ym = double(n) * 0.5;
     int n_minus_bound = int(floor(-2.0 * ym - xm));
     int n_plus_bound = xm > 4.0 * (2.0 - 1.4142135623730951) * ym ? int(ceil(xm)) : int(ceil(2.0 * ym + xm * xm / (16.0 * ym)));
-    return DBessel{n, n_minus_bound - 100, n_plus_bound - n_minus_bound + 200, ws, st};
+    return DBesselRec{n, n_minus_bound - 100, n_plus_bound - n_minus_bound + 200, ws, st};
 }
 PTG_HD __forceinline__ int dbessel_len(int n) {
     double xm = double(n) * 1.4142135623730951, ym = double(n) * 0.5;
@@ -597,7 +601,7 @@ PTG_HD __forceinline__ void dbessel_backward(const DB& D, double x, double y, WF
         j[q] = __builtin_isfinite(val) ? val : 0.0;
     }
 }
-PTG_HD inline void dbessel_recurrence(const DBessel& D, double x, double y, double j[5]) {
+PTG_HD inline void dbessel_recurrence(const DBesselRec& D, double x, double y, double j[5]) {
     const int len = D.len;
     const size_t st = D.st;
     double* ws = D.ws;
@@ -610,69 +614,6 @@ PTG_HD inline void dbessel_recurrence(const DBessel& D, double x, double y, doub
         S(i, 0) = x0; S(i, 1) = x1; S(i, 2) = x2; S(i, 3) = l0; S(i, 4) = l1;
     }
     auto W = [&](int i, int c) -> double { return ws[(size_t(5) * i + c) * st]; };
-    dbessel_backward(D, x, y, W, j);
-}
-
-// The same recurrence with the forward coefficients checkpointed: only every
-// K-th forward state goes to the per-lane scratch (5 doubles per K orders); the
-// backward pass recomputes one segment of K states at a time, by the same
-// forward step from its checkpoint, into a per-lane LDS column. The values are
-// those of dbessel_recurrence; HBM traffic drops K-fold for twice the forward
-// arithmetic. `seg` points at this lane's column of a [K][5][64]-double LDS
-// array (stride 64 between consecutive doubles of one lane).
-template <int K>
-struct DBesselCk {
-    int n, k_min, len;
-    double* ws;   // checkpoints: 5 * ceil(len / K) doubles, (s, c) at ws[(5 * s + c) * st]
-    size_t st;
-    double* seg;  // LDS column, (t, c) at seg[(5 * t + c) * 64]
-};
-template <int K>
-__device__ __forceinline__ DBesselCk<K> dbessel_ck_at(int n, double* ws, size_t st, double* seg) {
-    DBessel D = dbessel_at(n, ws, st);
-    return DBesselCk<K>{D.n, D.k_min, D.len, ws, st, seg};
-}
-template <int K>
-__device__ inline void dbessel_recurrence(const DBesselCk<K>& D, double x, double y, double j[5]) {
-    const int len = D.len;
-    const size_t st = D.st;
-    double* ws = D.ws;
-    double* seg = D.seg;
-    double x0 = 1.0, x1 = 1.0, x2 = 1.0, l0 = 1.0, l1 = 1.0;
-    ws[0] = 1.0; ws[st] = 1.0; ws[2 * st] = 1.0; ws[3 * st] = 1.0; ws[4 * st] = 1.0;
-    for (int i = 1; i < len; ++i) {
-        dbessel_forward(x, y, i + D.k_min, x0, x1, x2, l0, l1);
-        if (i % K == 0) {
-            double* c = ws + size_t(5) * size_t(i / K) * st;
-            c[0] = x0; c[st] = x1; c[2 * st] = x2; c[3 * st] = l0; c[4 * st] = l1;
-        }
-    }
-    int cur = -1;  // the segment held in LDS; the backward pass only descends
-    double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0, p4 = 0.0;  // checkpoint cur - 1, loaded ahead
-    auto W = [&](int i, int c) -> double {
-        const int sg = i / K;
-        if (sg != cur) {
-            double v0, v1, v2, v3, v4;
-            if (sg == cur - 1) {
-                v0 = p0; v1 = p1; v2 = p2; v3 = p3; v4 = p4;
-            } else {
-                const double* ck = ws + size_t(5) * size_t(sg) * st;
-                v0 = ck[0]; v1 = ck[st]; v2 = ck[2 * st]; v3 = ck[3 * st]; v4 = ck[4 * st];
-            }
-            if (sg > 0) {  // the next segment's checkpoint arrives while this one is consumed
-                const double* ck = ws + size_t(5) * size_t(sg - 1) * st;
-                p0 = ck[0]; p1 = ck[st]; p2 = ck[2 * st]; p3 = ck[3 * st]; p4 = ck[4 * st];
-            }
-            seg[0] = v0; seg[64] = v1; seg[128] = v2; seg[192] = v3; seg[256] = v4;
-            for (int t = 1; t < K && sg * K + t < len; ++t) {
-                dbessel_forward(x, y, sg * K + t + D.k_min, v0, v1, v2, v3, v4);
-                double* o = seg + 320 * t;
-                o[0] = v0; o[64] = v1; o[128] = v2; o[192] = v3; o[256] = v4;
-            }
-            cur = sg;
-        }
-        return seg[(5 * (i - sg * K) + c) * 64];
-    };
     dbessel_backward(D, x, y, W, j);
 }
 
@@ -728,18 +669,76 @@ PTG_HD inline bool dbessel_use_spa(int n, double x, double y) {
     double x_crit = y > ne / 6.0 ? 4.0 * sqrt(y * (ne - 2.0 * y)) : ne + 2.0 * y;
     return x < x_crit;
 }
-PTG_HD inline void dbessel_eval(const DBessel& D, double x, double y, double j[5]) {
+// J(n - 2 .. n + 2, x, y) by the trapezoid rule. J(m, x, y) = (1/pi) times the
+// integral over [0, pi] of cos(m phi - x sin phi + y sin 2 phi) (the integrand is
+// even and 2 pi periodic), and the trapezoid rule on K intervals of [0, pi] is
+// the 2K-point rule over the full period: its error is the integrand's Fourier
+// content beyond order 2K, which ends near |m| + x + 2y and decays within a few
+// multiples of |x|^(1/3) and |y|^(1/3) past it (x < 0 is the angle theta in the
+// second or third quadrant: J(m, -x, y) = (-1)^m J(m, x, y), which the integral
+// gives by itself). The node count below keeps the error under ~2e-13 absolute
+// over |x| <= n sqrt 2, 0 <= y <= n / 2; the
+// rounding floor is ~1e-15 absolute, so a J far below that (deep in its
+// evanescent region, where it feeds no rate) is not resolved relatively, as it
+// is by the recurrence. The nodes phi_k = k pi / K enter through e^(i phi) and
+// e^(i n phi), advanced by rotation and recomputed every 32 nodes; each node
+// costs one sin/cos pair of the bounded phase y sin 2 phi - x sin phi.
+__device__ __forceinline__ int dbessel_quad_nodes(int n, double x, double y) {
+    const double ax = fabs(x), ay = fabs(y);
+    return int(ceil(0.5 * (double(n) + 12.0 + ax + 2.0 * ay + 10.0 * (cbrt(ax) + 2.0 * cbrt(ay)))));
+}
+__device__ inline void dbessel_quadrature(int n, double x, double y, double j[5]) {
+    const int K = dbessel_quad_nodes(n, x, y);
+    const long long two_k = 2LL * K;
+    const long long nm = ((long long)n % two_k + two_k) % two_k;  // n phi_1 = pi nm / K (mod 2 pi)
+    const double inv_k = 1.0 / double(K);
+    double sh, ch, snh, cnh;
+    sincos_phase(PI_D * inv_k, &sh, &ch);
+    sincos_phase(PI_D * double(nm) * inv_k, &snh, &cnh);
+    double cp = 1.0, sp = 0.0, cq = 1.0, sq = 0.0;  // e^(i phi_k), e^(i n phi_k)
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
+    for (int k = 0; k <= K; ++k) {
+        if (k > 0 && (k & 31) == 0) {
+            sincos_phase(PI_D * double(k) * inv_k, &sp, &cp);
+            sincos_phase(PI_D * double((nm * k) % two_k) * inv_k, &sq, &cq);
+        }
+        const double s2 = 2.0 * sp * cp, c2 = (cp - sp) * (cp + sp);
+        double sv, cv;
+        sincos_phase(y * s2 - x * sp, &sv, &cv);
+        const double w = (k == 0 || k == K) ? 0.5 : 1.0;
+        const double cr = w * (cq * cv - sq * sv), sr = w * (sq * cv + cq * sv);  // w e^(i psi_k)
+        a0 += cr * c2 + sr * s2;  // cos(psi - 2 phi)
+        a1 += cr * cp + sr * sp;  // cos(psi - phi)
+        a2 += cr;
+        a3 += cr * cp - sr * sp;  // cos(psi + phi)
+        a4 += cr * c2 - sr * s2;  // cos(psi + 2 phi)
+        const double cp1 = cp * ch - sp * sh;
+        sp = sp * ch + cp * sh;
+        cp = cp1;
+        const double cq1 = cq * cnh - sq * snh;
+        sq = sq * cnh + cq * snh;
+        cq = cq1;
+    }
+    j[0] = a0 * inv_k; j[1] = a1 * inv_k; j[2] = a2 * inv_k; j[3] = a3 * inv_k; j[4] = a4 * inv_k;
+}
+
+// The reference's evaluation (double_bessel.rs:22-37): the saddle point where
+// dbessel_use_spa holds, else the recurrence (probe of the parity tests).
+__device__ inline void dbessel_eval(const DBesselRec& D, double x, double y, double j[5]) {
     if (dbessel_use_spa(D.n, x, y))
         for (int q = 0; q < 5; ++q) j[q] = dbessel_spa(D.n - 2 + q, x, y);
     else
         dbessel_recurrence(D, x, y, j);
 }
-template <int K>
-__device__ inline void dbessel_eval(const DBesselCk<K>& D, double x, double y, double j[5]) {
+// The kernels' evaluation at harmonic n: the same saddle point, and the
+// quadrature where the reference recurs.
+struct DBessel { int n; };
+__device__ __forceinline__ DBessel dbessel_of(int n) { return DBessel{n}; }
+__device__ inline void dbessel_eval(const DBessel& D, double x, double y, double j[5]) {
     if (dbessel_use_spa(D.n, x, y))
         for (int q = 0; q < 5; ++q) j[q] = dbessel_spa(D.n - 2 + q, x, y);
     else
-        dbessel_recurrence(D, x, y, j);
+        dbessel_quadrature(D.n, x, y, j);
 }
 
 // ---------------------------------------------------------------- LP double-differential rate (lp/mod.rs:21-140)
@@ -1074,11 +1073,10 @@ __device__ inline sv4 lp_stokes(const DB& D, double a, double eta, double s, dou
 
 // (n, s, theta, Stokes) of one LP emission (lp/mod.rs:586-676). `trial_cap`
 // bounds the rejection loop; hitting it sets `capped` and keeps the last trial.
-__device__ inline void lp_sample(const double* cdf, const LmaScalars& sc, double a, double eta, Rng& rng, double* ws, size_t st,
-                                 int& n_out, double& s_out, double& theta_out, sv4& pol, uint32_t& trials, bool& capped,
-                                 int fixed_n = 0) {
+__device__ inline void lp_sample(const double* cdf, const LmaScalars& sc, double a, double eta, Rng& rng, int& n_out,
+                                 double& s_out, double& theta_out, sv4& pol, uint32_t& trials, bool& capped, int fixed_n = 0) {
     int n = fixed_n > 0 ? fixed_n : lp_harmonic(cdf, sc, a, eta, rng_uniform(rng));
-    DBessel D = dbessel_at(n, ws, st);
+    const DBessel D = dbessel_of(n);
     ThetaBound tb = theta_bound(n, a, eta);
     double sn = 2.0 * double(n) * eta / (1.0 + 0.5 * a * a);
     double smax = sn / (1.0 + sn);

@@ -80,7 +80,7 @@ void free_buf(DevBuf& b) {
 }
 
 // Counter block layout (unsigned long long)
-enum { C_WORK = 0, C_NEXT, C_OUT, C_INTER, C_ID, C_FLAGS, C_EV, C_LNEXT, C_PH, C_OVF, C_OVFLEN, C_PAIRS, C_SROWS, C_SFB, C_STATS,
+enum { C_WORK = 0, C_NEXT, C_OUT, C_INTER, C_ID, C_FLAGS, C_EV, C_LNEXT, C_PH, C_PAIRS, C_SROWS, C_SFB, C_STATS,
        C_N = C_STATS + STAT_N };
 
 // Device scratch of ptg_histogram_noalloc: words [0, nbins) are the bins and
@@ -122,12 +122,6 @@ struct ptg_ctx {
     DevBuf evq, lq[2], phq, phq2, pairq;
     double* d_ev_extra = nullptr;  // ev_k0, ev_a
     size_t ev_extra_cap = 0;
-    double* d_scratch = nullptr;
-    size_t scratch_bytes = 0;
-    uint32_t* d_ovf = nullptr;      // LMA events deferred for a larger J(n, x, y) workspace
-    size_t ovf_cap = 0;
-    double* d_scratch2 = nullptr;   // that larger workspace
-    size_t scratch2_bytes = 0;
     LaserDev laser{};
     bool have_laser = false;
     DevBuf prim, q[2], out, inter, tmp;
@@ -550,9 +544,6 @@ void ptg_destroy(ptg_ctx* c) {
     if (c->d_bw) (void)hipFree(c->d_bw);
     if (c->d_cp) (void)hipFree(c->d_cp);
     if (c->d_ev_extra) (void)hipFree(c->d_ev_extra);
-    if (c->d_scratch) (void)hipFree(c->d_scratch);
-    if (c->d_ovf) (void)hipFree(c->d_ovf);
-    if (c->d_scratch2) (void)hipFree(c->d_scratch2);
     free_buf(c->evq); free_buf(c->lq[0]); free_buf(c->lq[1]); free_buf(c->phq); free_buf(c->phq2); free_buf(c->pairq);
     if (c->d_n0) (void)hipFree(c->d_n0);
     if (c->d_shower_count) (void)hipFree(c->d_shower_count);
@@ -891,12 +882,6 @@ static int transport(ptg_ctx* c, TransportArgs a, bool photons, DevBuf& in, size
     return fail(c, PTG_E_OOM, "secondary queue kept overflowing");
 }
 
-// Largest harmonic the LP CDF table can return at peak amplitude a0, with margin
-// (lp/mod.rs:414-419 at eta -> 0, where the harmonic sum is longest).
-static int lma_harmonic_cap(double a0) {
-    return int(std::ceil(1.5 * (6.0 * (1.0 + a0 * a0) + 2.0 * a0 * a0 * a0))) + 16;
-}
-
 static int ensure_ev_extra(ptg_ctx* c, size_t n) {
     if (n <= c->ev_extra_cap) return PTG_OK;
     if (c->d_ev_extra) (void)hipFree(c->d_ev_extra);
@@ -907,25 +892,12 @@ static int ensure_ev_extra(ptg_ctx* c, size_t n) {
 }
 
 // Runs the LMA event kernel (emissions, or pairs when `pairs`) over the n_ev
-// events in evq. Events whose harmonic needs a J(n, x, y) workspace larger than
-// m.scratch_len are deferred and run again with one sized for the largest; their
-// RNG counters make the second pass draw the same numbers.
+// events in evq (or `list` of `in`).
 static int lma_event_pass(ptg_ctx* c, LmaArgs m, bool pairs, size_t n_ev, size_t max_lanes, double* ms, uint64_t* launches,
                           const uint32_t* list = nullptr, const DevBuf* in = nullptr) {
-    const size_t ovf_need = std::max(c->evq.cap, in ? in->cap : size_t(0));
-    if (c->ovf_cap < ovf_need) {
-        if (c->d_ovf) (void)hipFree(c->d_ovf);
-        c->d_ovf = nullptr;
-        HIPCHK(c, hipMalloc(&c->d_ovf, ovf_need * sizeof(uint32_t)));
-        c->ovf_cap = ovf_need;
-    }
-    HIPCHK(c, hipMemsetAsync(c->d_ctr + C_OVF, 0, 2 * sizeof(unsigned long long), c->stream));
     m.ev_in = in ? in->v : c->evq.v;
     m.ev_list = list;
     m.n_ev = n_ev;
-    m.ovf_list = c->d_ovf;
-    m.ovf_ctr = c->d_ctr + C_OVF;
-    m.ovf_len = c->d_ctr + C_OVFLEN;
     // LP emission runs one event per wave (k_event_lma_wave), the others one per lane
     const bool per_wave = !pairs && (m.mode == LMA_LP || m.mode == LMA_LP_CLASSICAL);
     size_t lanes = per_wave ? std::min<size_t>(max_lanes, n_ev * 64) : std::min<size_t>(max_lanes, ((n_ev + 63) / 64) * 64);
@@ -934,36 +906,8 @@ static int lma_event_pass(ptg_ctx* c, LmaArgs m, bool pairs, size_t n_ev, size_t
     else launch_event_lma(c->d_args, m, int(lanes), c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    unsigned long long ovf[2] = {0, 0};
-    HIPCHK(c, hipMemcpyAsync(ovf, c->d_ctr + C_OVF, sizeof(ovf), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     float t = 0.0f;
-    (void)hipEventElapsedTime(&t, c->ev0, c->ev1);
-    *ms += t;
-    ++*launches;
-    if (ovf[0] == 0) return PTG_OK;
-    const size_t per_lane = size_t(ovf[1]) * 5 * sizeof(double);
-    size_t olanes = std::min<size_t>(per_wave ? ovf[0] * 64 : ((ovf[0] + 63) / 64) * 64, ((size_t(8) << 30) / per_lane / 64) * 64);
-    olanes = std::max<size_t>(olanes, 64);
-    size_t bytes = olanes * per_lane;
-    if (bytes > (size_t(16) << 30)) return fail(c, PTG_E_DOMAIN, "LMA harmonic index beyond the supported range");
-    if (bytes > c->scratch2_bytes) {
-        if (c->d_scratch2) (void)hipFree(c->d_scratch2);
-        c->d_scratch2 = nullptr;
-        HIPCHK(c, hipMalloc(&c->d_scratch2, bytes));
-        c->scratch2_bytes = bytes;
-    }
-    m.ev_list = c->d_ovf;
-    m.n_ev = ovf[0];
-    m.ovf_list = nullptr;
-    m.scratch = c->d_scratch2;
-    m.scratch_len = int(ovf[1]);
-    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    if (pairs) launch_pair_event_lma(c->d_args, m, int(olanes), c->stream);
-    else launch_event_lma(c->d_args, m, int(olanes), c->stream);
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
     (void)hipEventElapsedTime(&t, c->ev0, c->ev1);
     *ms += t;
     ++*launches;
@@ -1083,31 +1027,8 @@ static unsigned long long read_ctr(ptg_ctx* c, int which) {
 static int run_lma(ptg_ctx* c, TransportArgs a, double* ms, uint64_t* launches, uint64_t* gens, bool use_surr, uint64_t* surr_fb) {
     const size_t n = c->n_prim;
     const bool track = a.opt.tracking_photons != 0;
-    int len_cap = 0;
-    {
-        // workspace per lane for J(n, x, y), 5 doubles per order (LP emission and LP
-        // pair creation; circular polarisation needs none)
-        int ncap = c->laser.circular ? 16 : lma_harmonic_cap(c->n_coll ? c->batch_a0_max : c->laser.a0);
-        // diagnostic: a smaller workspace sends most events through the deferred pass,
-        // which must reproduce the same shower
-        if (const char* e = std::getenv("PTG_LMA_HARMONIC_CAP")) {
-            int v = std::atoi(e);
-            if (v >= 3) ncap = v;
-        }
-        double xm = ncap * 1.4142135623730951, ym = ncap * 0.5;
-        len_cap = int(std::ceil(xm)) - int(std::floor(-2.0 * ym - xm)) + 200;
-    }
-    // J workspace: up to 8 GB, up to 4096 waves (16 per CU) of per-lane workspaces
-    const size_t budget = size_t(8) << 30;
-    size_t max_lanes = std::max<size_t>(64, std::min<size_t>(262144, budget / (size_t(len_cap) * 40)));
-    max_lanes = (max_lanes / 64) * 64;
-    size_t need = max_lanes * size_t(len_cap) * 5 * sizeof(double);
-    if (need > c->scratch_bytes) {
-        if (c->d_scratch) (void)hipFree(c->d_scratch);
-        c->d_scratch = nullptr;
-        HIPCHK(c, hipMalloc(&c->d_scratch, need));
-        c->scratch_bytes = need;
-    }
+    // event kernels: up to 4096 waves (16 per CU)
+    const size_t max_lanes = 262144;
     LmaArgs m{};
     m.tables = c->d_lma;
     m.sc = c->lsc;
@@ -1129,8 +1050,6 @@ static int run_lma(ptg_ctx* c, TransportArgs a, double* ms, uint64_t* launches, 
     m.ph_ctr = c->d_ctr + C_PH;
     m.pairs_ctr = c->d_ctr + C_PAIRS;
     m.photons_to_out = track ? 0 : 1;
-    m.scratch = c->d_scratch;
-    m.scratch_len = len_cap;
 
     const unsigned long long zero = 0;
     auto zero_ctr = [&](int which) { return hipMemcpyAsync(c->d_ctr + which, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream); };
@@ -1842,84 +1761,55 @@ int ptg_eval(ptg_ctx* c, int fn, const double* in, size_t n, size_t in_w, double
     }
     if (fn >= PTG_FN_LP_RATE) {
         HIPCHK(c, hipSetDevice(c->device));
-        int nmax = 16;
         if (fn == PTG_FN_BW_SAMPLE) {
-            for (size_t i = 0; i < n; ++i) {
-                const double* x = in + i * in_w;
-                if (!(x[5] >= 0.0 && x[5] <= 2.0e5)) return fail(c, PTG_E_ARG, "BW sample probe needs 0 <= fixed n <= 2e5");
-                int lo = 0, hi = 0;
-                if (x[5] > 0) hi = int(x[5]);
-                else if (x[0] > 0.0 && x[1] > 0.0) bw_sum_limits(x[0], x[1], lo, hi);
-                nmax = std::max(nmax, std::min(hi, 200000));
-            }
+            for (size_t i = 0; i < n; ++i)
+                if (!(in[i * in_w + 5] >= 0.0 && in[i * in_w + 5] <= 2.0e5))
+                    return fail(c, PTG_E_ARG, "BW sample probe needs 0 <= fixed n <= 2e5");
         }
+        size_t len = 0;  // orders of the recurrence probe's per-row workspace
         if (fn == PTG_FN_DOUBLE_BESSEL || fn == PTG_FN_LP_SAMPLE || fn == PTG_FN_LP_CL_SAMPLE || fn == PTG_FN_LP_CL_CEILING) {
             for (size_t i = 0; i < n; ++i) {
                 const double* x = in + i * in_w;
-                int nn;
                 if (fn == PTG_FN_DOUBLE_BESSEL) {
-                    // the workspace is sized for x <= n sqrt 2, y <= n / 2 (double_bessel.rs:39-76)
-                    nn = int(x[0]);
+                    // the recurrence's workspace is sized for x <= n sqrt 2, y <= n / 2 (double_bessel.rs:39-76)
+                    const int nn = int(x[0]);
                     if (!(x[0] >= 3.0 && x[0] <= 2.0e5) || double(nn) != x[0])
                         return fail(c, PTG_E_ARG, "J(n, x, y) probe needs an integer 3 <= n <= 2e5");
-                    if (!(x[1] >= 0.0 && x[1] <= nn * 1.4142135623730951 && x[2] >= 0.0 && x[2] <= 0.5 * nn))
-                        return fail(c, PTG_E_ARG, "J(n, x, y) probe needs 0 <= x <= n sqrt 2 and 0 <= y <= n / 2");
+                    if (!(std::fabs(x[1]) <= nn * 1.4142135623730951 && x[2] >= 0.0 && x[2] <= 0.5 * nn))
+                        return fail(c, PTG_E_ARG, "J(n, x, y) probe needs |x| <= n sqrt 2 and 0 <= y <= n / 2");
+                    if (!(x[3] == 0.0 || x[3] == 1.0 || x[3] == 2.0 || x[3] == 3.0))
+                        return fail(c, PTG_E_ARG, "J(n, x, y) probe mode is 0, 1, 2 or 3");
+                    if (x[3] == 1.0 || x[3] == 3.0) len = std::max(len, size_t(dbessel_len(nn)));
                 } else if (fn == PTG_FN_LP_CL_CEILING) {
-                    nn = int(x[0]);
-                    if (!(x[0] >= 1.0 && x[0] <= 2.0e5) || double(nn) != x[0] || !(x[1] > 0.0))
+                    if (!(x[0] >= 1.0 && x[0] <= 2.0e5) || double(int(x[0])) != x[0] || !(x[1] > 0.0))
                         return fail(c, PTG_E_ARG, "LP classical ceiling probe needs an integer 1 <= n <= 2e5 and a > 0");
-                } else {
-                    if (!(x[3] >= 0.0 && x[3] <= 2.0e5)) return fail(c, PTG_E_ARG, "LP sample probe needs 0 <= fixed n <= 2e5");
-                    nn = x[3] > 0 ? int(x[3]) : lma_harmonic_cap(x[0]);
-                }
-                nmax = std::max(nmax, nn);
-            }
-        }
-        // LP samples whose harmonic outgrows the workspace come back with trials = -1;
-        // they are run again, alone, with a workspace for the largest such harmonic
-        std::vector<size_t> rows(n);
-        for (size_t i = 0; i < n; ++i) rows[i] = i;
-        for (int pass = 0; pass < 2 && !rows.empty(); ++pass) {
-            const size_t m = rows.size();
-            std::vector<double> hin(m * in_w), hout(m * out_w);
-            for (size_t k = 0; k < m; ++k) std::memcpy(&hin[k * in_w], in + rows[k] * in_w, in_w * 8);
-            double xm = nmax * 1.4142135623730951, ym = nmax * 0.5;
-            size_t len = size_t(int(std::ceil(xm)) - int(std::floor(-2.0 * ym - xm)) + 200);
-            if (m * len * 5 * 8 > (size_t(16) << 30)) return fail(c, PTG_E_ARG, "probe batch too large for its J(n, x, y) workspace");
-            double *din = nullptr, *dout = nullptr, *dscr = nullptr;
-            HIPCHK(c, hipMalloc(&din, m * in_w * 8));
-            HIPCHK(c, hipMalloc(&dout, m * out_w * 8));
-            HIPCHK(c, hipMalloc(&dscr, m * len * 5 * 8));
-            HIPCHK(c, scopy(c, din, hin.data(), m * in_w * 8, hipMemcpyHostToDevice));
-            HIPCHK(c, sfill(c, dout, 0, m * out_w * 8));
-            LmaArgs la{};
-            la.tables = c->d_lma;
-            la.sc = c->lsc;
-            la.bw = c->d_bw;
-            la.bsc = c->bsc;
-            la.cp = c->d_cp;
-            la.csc = c->csc;
-            la.lcfa = c->d_lcfa;
-            la.scratch = dscr;
-            la.scratch_len = int(len);
-            launch_eval_lma(fn, din, dout, uint32_t(m), uint32_t(in_w), uint32_t(out_w), la, key_from_seed(0), c->stream);
-            HIPCHK(c, hipGetLastError());
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-            HIPCHK(c, scopy(c, hout.data(), dout, m * out_w * 8, hipMemcpyDeviceToHost));
-            (void)hipFree(din); (void)hipFree(dout); (void)hipFree(dscr);
-            std::vector<size_t> again;
-            nmax = 16;
-            for (size_t k = 0; k < m; ++k) {
-                std::memcpy(out + rows[k] * out_w, &hout[k * out_w], out_w * 8);
-                if (((fn == PTG_FN_LP_SAMPLE || fn == PTG_FN_LP_CL_SAMPLE) && hout[k * out_w + 5] == -1.0) ||
-                    (fn == PTG_FN_BW_SAMPLE && hout[k * out_w + 3] == -1.0)) {
-                    again.push_back(rows[k]);
-                    nmax = std::max(nmax, int(hout[k * out_w]));
+                } else if (!(x[3] >= 0.0 && x[3] <= 2.0e5)) {
+                    return fail(c, PTG_E_ARG, "LP sample probe needs 0 <= fixed n <= 2e5");
                 }
             }
-            rows.swap(again);
         }
-        if (!rows.empty()) return fail(c, PTG_E_STATE, "LP sample probe: workspace re-run did not converge");
+        if (n * len * 5 * 8 > (size_t(16) << 30)) return fail(c, PTG_E_ARG, "probe batch too large for its J(n, x, y) workspace");
+        double *din = nullptr, *dout = nullptr, *dscr = nullptr;
+        HIPCHK(c, hipMalloc(&din, n * in_w * 8));
+        HIPCHK(c, hipMalloc(&dout, n * out_w * 8));
+        if (len) HIPCHK(c, hipMalloc(&dscr, n * len * 5 * 8));
+        HIPCHK(c, scopy(c, din, in, n * in_w * 8, hipMemcpyHostToDevice));
+        HIPCHK(c, sfill(c, dout, 0, n * out_w * 8));
+        LmaArgs la{};
+        la.tables = c->d_lma;
+        la.sc = c->lsc;
+        la.bw = c->d_bw;
+        la.bsc = c->bsc;
+        la.cp = c->d_cp;
+        la.csc = c->csc;
+        la.lcfa = c->d_lcfa;
+        la.scratch = dscr;
+        launch_eval_lma(fn, din, dout, uint32_t(n), uint32_t(in_w), uint32_t(out_w), la, key_from_seed(0), c->stream);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, scopy(c, out, dout, n * out_w * 8, hipMemcpyDeviceToHost));
+        (void)hipFree(din); (void)hipFree(dout);
+        if (dscr) (void)hipFree(dscr);
         return PTG_OK;
     }
     HIPCHK(c, hipSetDevice(c->device));
@@ -2153,20 +2043,13 @@ int ptg_surrogate_train(ptg_ctx* c, const ptg_surrogate_train_desc* d, ptg_surro
     surr_box_args(box, S);
     S.key = key_from_seed(d->seed);
     S.n_groups = uint32_t(d->groups);
-    // ---- exact-sampler groups (the checkpointed J workspace for LP)
-    const int ncap = circ ? 16 : lma_harmonic_cap(d->a_max);
-    const double xm = ncap * 1.4142135623730951, ym = ncap * 0.5;
-    const int len_cap = int(std::ceil(xm)) - int(std::floor(-2.0 * ym - xm)) + 200;
-    const size_t per_lane = size_t(5) * size_t(len_cap / PTG_J_SEG + 2) * sizeof(double);
-    size_t lanes = std::min<size_t>(size_t(d->groups) * 64, std::min<size_t>(262144, (size_t(8) << 30) / per_lane));
-    lanes = std::max<size_t>(64, lanes / 64 * 64);
-    double* scratch = nullptr;
+    // ---- exact-sampler groups (one wave per group, up to 4096 waves)
+    const size_t lanes = std::min<size_t>(size_t(d->groups) * 64, 262144);
     double* gh = nullptr;
     float* gs = nullptr;
     float* drows = nullptr;
     MlpNet net;
     auto cleanup = [&]() {
-        if (scratch) (void)hipFree(scratch);
         if (gh) (void)hipFree(gh);
         if (gs) (void)hipFree(gs);
         if (drows) (void)hipFree(drows);
@@ -2177,11 +2060,8 @@ int ptg_surrogate_train(ptg_ctx* c, const ptg_surrogate_train_desc* d, ptg_surro
         return fail(c, e == hipErrorOutOfMemory ? PTG_E_OOM : PTG_E_HIP, std::string("surrogate training: ") + hipGetErrorString(e));
     };
     hipError_t e;
-    if ((e = hipMalloc(&scratch, lanes * per_lane))) return hip_fail(e);
     if ((e = hipMalloc(&gh, size_t(d->groups) * 4 * sizeof(double)))) return hip_fail(e);
     if ((e = hipMalloc(&gs, size_t(d->groups) * 128 * sizeof(float)))) return hip_fail(e);
-    m.scratch = scratch;
-    m.scratch_len = len_cap;
     S.gh = gh;
     S.gs = gs;
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));

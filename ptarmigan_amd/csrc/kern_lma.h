@@ -5,10 +5,6 @@
 #include "dev_lma.h"
 #include "kern_common.h"
 
-#ifndef PTG_J_SEG
-#define PTG_J_SEG 7  // J(n, x, y) forward states per checkpoint: 17.5 KB of LDS per wave
-#endif
-
 namespace ptg {
 
 // Emission model of the lepton kernels: linear or circular polarisation of the
@@ -39,12 +35,6 @@ struct LmaArgs {
     PBuf ev_in;
     const uint32_t* ev_list;
     uint64_t n_ev;
-    // events whose harmonic needs a larger J(n, x, y) workspace than scratch_len are
-    // deferred here (when ovf_list is set) and re-run by the host with a workspace
-    // of ovf_len; their RNG counters make the re-run draw the same numbers
-    uint32_t* ovf_list;
-    unsigned long long* ovf_ctr;
-    unsigned long long* ovf_len;
     // event kernel outputs
     PBuf next;                   // electrons continuing after their event
     unsigned long long* next_ctr;
@@ -52,8 +42,7 @@ struct LmaArgs {
     unsigned long long* ph_ctr;
     unsigned long long ph_cap;
     int photons_to_out;          // photons are final state (photon tracking off)
-    double* scratch;             // J(n, x, y) workspace, 5 * scratch_len doubles per lane
-    int scratch_len;
+    double* scratch;             // the J(n, x, y) probe's recurrence workspace (k_eval_lma only)
     // LP Breit-Wheeler (photon phase)
     const double* bw;            // BW pack (dev_bw.h)
     BwScalars bsc;
@@ -105,8 +94,8 @@ void launch_photon_step_lma(const TransportArgs* a, const LmaArgs& m, int grid, 
 void launch_pair_event_lma(const TransportArgs* a, const LmaArgs& m, int lanes, hipStream_t s);
 void launch_push_lma(const LaserDev& L, double dt, int ll, const uint8_t* sp_, double* r4, double* u4, uint32_t n,
                      uint32_t n_steps, uint32_t* taken, double* work, hipStream_t s);
-// probes: the table pointers, scalars and scratch (J(n, x, y) workspace of
-// scratch_len per row) are taken from m
+// probes: the table pointers, scalars and scratch (the recurrence probe's J(n, x, y)
+// workspace, dbessel_len(n) orders per row) are taken from m
 void launch_eval_lma(int fn, const double* in, double* out, uint32_t n, uint32_t in_w, uint32_t out_w, const LmaArgs& m, uint64_t key,
                      hipStream_t s);
 

@@ -8,7 +8,7 @@
 //    <a^2> at the event into the event queue and pulls the next particle:
 //    heavy, divergent event sampling never stalls a wave of stepping lanes.
 //  * k_event_lma: one lane per queued event samples the harmonic, (s, theta)
-//    by rejection on J(n, x, y) with per-lane scratch, builds the photon in the
+//    by rejection on J(n, x, y), builds the photon in the
 //    zero-momentum frame, writes it out and re-queues the recoiled electron.
 // The host alternates the two until no electron is left in the field.
 #include "kern_lma.h"
@@ -226,7 +226,6 @@ __global__ __launch_bounds__(64) void k_event_lma(const TransportArgs* __restric
     constexpr bool linear = MODE == LMA_LP || MODE == LMA_LP_CLASSICAL;
     const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n_lanes = gridDim.x * blockDim.x;
-    double* ws = M.scratch + lane;  // this lane's J(n, x, y) workspace, stride n_lanes
     const double* cdf = M.tables + lma_pack::CDF;
     uint64_t n_emit = 0, trials = 0;
     uint32_t flags = 0;
@@ -258,25 +257,15 @@ __global__ __launch_bounds__(64) void k_event_lma(const TransportArgs* __restric
                 cp_sample_at(M.lcfa, M.cp, nh, a, eta, classical, rng, s, cphi, sv, tr, capped);
             }
         } else {
-            // harmonic first, to check the workspace is large enough
             int nh = classical ? lp_cl_harmonic(M.tables, M.sc, alp, rng_uniform(rng)) : lp_harmonic(cdf, M.sc, alp, eta, rng_uniform(rng));
-            int need = dbessel_len(nh);
             if (nh < 1) {
-                flags |= PTG_FLAG_TABLE_RANGE;
-                n_cap_ok = 0;
-            } else if (need > M.scratch_len) {
-                if (M.ovf_list) {  // defer: no output now, the host re-runs this event
-                    M.ovf_list[wave_reserve(M.ovf_ctr, 1u)] = i;
-                    atomicMax(M.ovf_len, (unsigned long long)need);
-                    continue;
-                }
                 flags |= PTG_FLAG_TABLE_RANGE;
                 n_cap_ok = 0;
             } else if (classical) {
                 n = nh;
-                lp_cl_sample(dbessel_at(nh, ws, n_lanes), alp, eta, rng, s, cphi, sv, tr, capped);
+                lp_cl_sample(dbessel_of(nh), alp, eta, rng, s, cphi, sv, tr, capped);
             } else {
-                lp_sample(cdf, M.sc, alp, eta, rng, ws, n_lanes, n, s, cphi, sv, tr, capped, nh);
+                lp_sample(cdf, M.sc, alp, eta, rng, n, s, cphi, sv, tr, capped, nh);
             }
         }
         trials += tr;
@@ -309,12 +298,9 @@ template <int MODE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PTG_LMA_WAVE_WPE))) void k_event_lma_wave(const TransportArgs* __restrict__ A, LmaArgs M) {
     constexpr bool classical = MODE == LMA_LP_CLASSICAL;
     const int lane = threadIdx.x;
-    const size_t st = size_t(gridDim.x) * 64;  // per-lane workspaces, lane-interleaved
-    double* ws = M.scratch + size_t(blockIdx.x) * 64 + lane;
     const double* cdf = M.tables + lma_pack::CDF;
     const uint32_t cap = 1u << 20;
     __shared__ double tb_s[16], tb_f[16];  // the event's theta bound, one copy per wave
-    __shared__ double jseg[PTG_J_SEG * 5 * 64];  // per-lane segment of recomputed J forward states
     uint64_t n_emit = 0, trials = 0;
     uint32_t flags = 0;
     for (uint64_t e = blockIdx.x; e < M.n_ev; e += gridDim.x) {
@@ -333,24 +319,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PTG_LMA_WAVE
         const int nh = classical ? lp_cl_harmonic(M.tables, M.sc, alp, fr)
                                  : (M.mixed ? lp_harmonic(M.tables_f + lma_pack::CDF, M.sc, alp, eta, fr)
                                             : lp_harmonic(cdf, M.sc, alp, eta, fr));
-        bool ok = nh >= 1;
-        if (!ok) {
-            flags |= PTG_FLAG_TABLE_RANGE;
-        } else if (dbessel_len(nh) > M.scratch_len) {
-            if (M.ovf_list) {  // defer: no output now, the host re-runs this event with a larger workspace
-                if (lane == 0) {
-                    M.ovf_list[wave_reserve(M.ovf_ctr, 1u)] = i;
-                    atomicMax(M.ovf_len, (unsigned long long)dbessel_len(nh));
-                }
-                continue;
-            }
-            flags |= PTG_FLAG_TABLE_RANGE;
-            ok = false;
-        }
+        const bool ok = nh >= 1;
+        if (!ok) flags |= PTG_FLAG_TABLE_RANGE;
         double s = 0.0, theta = 0.0;
         sv4 sv{1.0, 0.0, 0.0, 0.0};
         if (ok) {
-            const DBesselCk<PTG_J_SEG> D = dbessel_ck_at<PTG_J_SEG>(nh, ws, st, jseg + lane);
+            const DBessel D = dbessel_of(nh);
             const double sn = 2.0 * double(nh) * eta / (1.0 + 0.5 * alp * alp);
             const double smax = classical ? sn : sn / (1.0 + sn);  // s = v * smax
             if (lane < 16) {  // the theta bound's 16 points, one per lane
@@ -565,7 +539,6 @@ template <bool CIRC>
 __global__ __launch_bounds__(64) void k_pair_event_lma(const TransportArgs* __restrict__ A, LmaArgs M) {
     const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n_lanes = gridDim.x * blockDim.x;
-    double* ws = M.scratch + lane;
     const LaserDev& L = A->laser;
     uint64_t n_pairs = 0, trials = 0;
     uint32_t flags = 0;
@@ -595,19 +568,12 @@ __global__ __launch_bounds__(64) void k_pair_event_lma(const TransportArgs* __re
         bool made = false;
         if (n < 1) {
             flags |= PTG_FLAG_TABLE_RANGE;
-        } else if (!CIRC && dbessel_len(n) > M.scratch_len) {
-            if (M.ovf_list) {
-                M.ovf_list[wave_reserve(M.ovf_ctr, 1u)] = i;
-                atomicMax(M.ovf_len, (unsigned long long)dbessel_len(n));
-                continue;
-            }
-            flags |= PTG_FLAG_TABLE_RANGE;
         } else {
             double s, cphi;
             uint32_t tr = 0;
             bool capped = false;
             if (CIRC) cpbw_sample_at(M.lcfa, M.cp, n, a, eta, sl.q, sl.u, sl.v, rng, s, cphi, tr, capped);
-            else bw_sample_at(dbessel_at(n, ws, n_lanes), alp, eta, sl.q, sl.u, rng, s, cphi, tr, capped);
+            else bw_sample_at(dbessel_of(n), alp, eta, sl.q, sl.u, rng, s, cphi, tr, capped);
             trials += tr;
             if (capped) flags |= PTG_FLAG_REJECT_CAP;
             // positron in the zero-momentum frame (pair_creation/mod.rs:63-104)
@@ -793,14 +759,11 @@ __global__ void k_surr_inputs(const TransportArgs* __restrict__ A, LmaArgs M, Su
 }
 
 // Network outputs -> photons. One lane per row; LP Stokes parameters need one
-// J(n, x, y) evaluation per event, in a per-lane workspace (the checkpointed
-// recurrence of the emission kernel is slower here: this kernel is bound by the
-// serial recurrence of its largest harmonic, not by the workspace traffic).
+// J(n, x, y) evaluation per event.
 template <bool CIRC>
 __global__ __launch_bounds__(64) void k_surr_event(const TransportArgs* __restrict__ A, LmaArgs M, SurrArgs S) {
     const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n_lanes = gridDim.x * blockDim.x;
-    double* ws = M.scratch + lane;
     const uint64_t n_rows = *S.n_rows;
     uint64_t n_emit = 0;
     uint32_t flags = 0;
@@ -848,10 +811,6 @@ __global__ __launch_bounds__(64) void k_surr_event(const TransportArgs* __restri
             g3.c0 = 3u;
             theta = 2.0 * PI_D * rng_uniform(g3);
         } else {
-            // the workspace holds the harmonics of the tables (1.5 x the reference's sum
-            // limit at the laser's a0); a network output beyond it is cut to it
-            while (n > 1 && dbessel_len(n) > M.scratch_len) n = n - 1 - (dbessel_len(n) - M.scratch_len) / 4;
-            if (n < 1) n = 1;
             sn = 2.0 * double(n) * eta / (1.0 + 0.5 * amp * amp);
             const double th = fmin(fmax(double(o[2]), 0.0), 1.0) * (PI_D / 2.0);
             Rng gq = base;  // counter 1: the quadrant, as the exact sampler
@@ -876,7 +835,7 @@ __global__ __launch_bounds__(64) void k_surr_event(const TransportArgs* __restri
                 cp_stokes(M.lcfa, n, amp, s, sn, v, theta, false, st);
                 sv = {st[0], st[1], st[2], st[3]};
             } else {
-                sv = lp_stokes(dbessel_at(n, ws, n_lanes), amp, eta, s, theta);
+                sv = lp_stokes(dbessel_of(n), amp, eta, s, theta);
             }
             ok = isfinite(sv.q) && isfinite(sv.u) && isfinite(sv.v);
         }
@@ -900,10 +859,7 @@ __global__ __launch_bounds__(64) void k_surr_event(const TransportArgs* __restri
 template <bool CIRC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PTG_LMA_WAVE_WPE))) void k_surr_groups(LmaArgs M, SurrArgs S) {
     const int lane = threadIdx.x;
-    const size_t st = size_t(gridDim.x) * 64;
-    double* ws = M.scratch + size_t(blockIdx.x) * 64 + lane;
     __shared__ double tb_s[16], tb_f[16];
-    __shared__ double jseg[PTG_J_SEG * 5 * 64];
     for (uint32_t g = blockIdx.x; g < S.n_groups; g += gridDim.x) {
         __syncthreads();
         Rng gr = rng_make(S.key, g);
@@ -932,8 +888,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PTG_LMA_WAVE
                 if (capped) ok = false;
             }
             ok = __all(ok);
-        } else if (ok && dbessel_len(n) <= M.scratch_len) {
-            const DBesselCk<PTG_J_SEG> D = dbessel_ck_at<PTG_J_SEG>(n, ws, st, jseg + lane);
+        } else if (ok) {
+            const DBessel D = dbessel_of(n);
             const double sn = 2.0 * double(n) * eta / (1.0 + 0.5 * amp * amp);
             const double smax = sn / (1.0 + sn);
             if (lane < 16) theta_bound_point(n, amp, eta, lane, tb_s[lane], tb_f[lane]);
@@ -1092,11 +1048,8 @@ __global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, ui
     const LmaScalars& sc = m.sc;
     const double* bw = m.bw;
     const BwScalars& bsc = m.bsc;
-    double* scratch = m.scratch;
-    const int scratch_len = m.scratch_len;
     const double* x = in + uint64_t(i) * in_w;
     double* y = out + uint64_t(i) * out_w;
-    __shared__ double jseg[PTG_J_SEG * 5 * 64];  // blocks of 64 lanes
     switch (fn) {
     case PTG_FN_LP_RATE: {
         double f = 0.0;
@@ -1106,32 +1059,31 @@ __global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, ui
     }
     case PTG_FN_LP_HARMONIC: y[0] = lp_harmonic(tables + lma_pack::CDF, sc, x[0], x[1], x[2]); break;
     case PTG_FN_DOUBLE_BESSEL: {
-        int nn = int(x[0]);  // the host checks 3 <= n, 0 <= x <= n sqrt 2, 0 <= y <= n / 2, len(n) <= scratch_len
-        DBessel D = dbessel_at(nn, scratch + i, n);
+        // mode x[3]: 0 the kernels' evaluation (saddle point or quadrature), 1 the
+        // reference's recurrence (the host checks 3 <= n, |x| <= n sqrt 2,
+        // 0 <= y <= n / 2 and sizes m.scratch for it), 2 the quadrature alone,
+        // 3 the reference's evaluation (saddle point or recurrence)
+        const int nn = int(x[0]);
+        const DBesselRec R = dbessel_rec_at(nn, m.scratch + i, n);
         double j[5];
-        if (x[3] == 2.0) {  // the checkpointed recurrence of the emission kernel
-            dbessel_recurrence(dbessel_ck_at<PTG_J_SEG>(nn, scratch + i, n, jseg + threadIdx.x), x[1], x[2], j);
-        } else if (x[3] != 0.0) {
-            dbessel_recurrence(D, x[1], x[2], j);
-        } else {
-            dbessel_eval(D, x[1], x[2], j);
-        }
+        if (x[3] == 1.0) dbessel_recurrence(R, x[1], x[2], j);
+        else if (x[3] == 2.0) dbessel_quadrature(nn, x[1], x[2], j);
+        else if (x[3] == 3.0) dbessel_eval(R, x[1], x[2], j);
+        else dbessel_eval(dbessel_of(nn), x[1], x[2], j);
         for (int q = 0; q < 5; ++q) y[q] = j[q];
         break;
     }
     case PTG_FN_LP_SAMPLE: {
-        // (a, eta, stream, fixed_n) -> (n, s, theta, S1, S2, trials); trials = -1 marks a
-        // harmonic n too large for the workspace (the host re-runs such rows)
+        // (a, eta, stream, fixed_n) -> (n, s, theta, S1, S2, trials)
         Rng rng = rng_make(key, uint64_t(x[2]));
         int nh = x[3] > 0.0 ? int(x[3]) : lp_harmonic(tables + lma_pack::CDF, sc, x[0], x[1], rng_uniform(rng));
         if (nh < 1) { y[0] = 0.0; y[5] = -2.0; break; }  // outside the table
-        if (dbessel_len(nh) > scratch_len) { y[0] = nh; y[5] = -1.0; break; }
         int nn;
         double s, th;
         sv4 pol;
         uint32_t tr = 0;
         bool capped = false;
-        lp_sample(tables + lma_pack::CDF, sc, x[0], x[1], rng, scratch + i, n, nn, s, th, pol, tr, capped, nh);
+        lp_sample(tables + lma_pack::CDF, sc, x[0], x[1], rng, nn, s, th, pol, tr, capped, nh);
         y[0] = nn; y[1] = s; y[2] = th; y[3] = pol.q; y[4] = pol.u; y[5] = tr;
         break;
     }
@@ -1147,12 +1099,11 @@ __global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, ui
     }
     case PTG_FN_BW_HARMONIC: y[0] = bw_harmonic(bw, bsc, x[0], x[1], x[2], x[3]); break;
     case PTG_FN_BW_SAMPLE: {
-        // trials = -1: harmonic too large for the workspace (the host re-runs the row); -2: outside the tables
+        // trials = -2: outside the tables
         Rng rng = rng_make(key, uint64_t(x[4]));
         int nh = x[5] > 0.0 ? int(x[5]) : bw_harmonic(bw, bsc, x[0], x[1], x[2], rng_uniform(rng));
         if (nh < 1) { y[0] = 0.0; y[3] = -2.0; break; }
-        if (dbessel_len(nh) > scratch_len) { y[0] = nh; y[3] = -1.0; break; }
-        DBessel D = dbessel_at(nh, scratch + i, n);
+        const DBessel D = dbessel_of(nh);
         double s, th;
         uint32_t tr = 0;
         bool capped = false;
@@ -1168,21 +1119,20 @@ __global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, ui
     }
     case PTG_FN_LP_CL_HARMONIC: y[0] = lp_cl_harmonic(tables, sc, x[0], x[1]); break;
     case PTG_FN_LP_CL_CEILING: {
-        int nn = int(x[0]);  // the host checks 1 <= n and the workspace
-        y[0] = lp_cl_ceiling(dbessel_at(nn, scratch + i, n), x[1]);
+        const int nn = int(x[0]);  // the host checks 1 <= n
+        y[0] = lp_cl_ceiling(dbessel_of(nn), x[1]);
         break;
     }
     case PTG_FN_LP_CL_SAMPLE: {
-        // (a, eta, stream, fixed_n) -> (n, s, theta, S1, S2, trials); trials -1: workspace too small, -2: outside the table
+        // (a, eta, stream, fixed_n) -> (n, s, theta, S1, S2, trials); trials -2: outside the table
         Rng rng = rng_make(key, uint64_t(x[2]));
         int nh = x[3] > 0.0 ? int(x[3]) : lp_cl_harmonic(tables, sc, x[0], rng_uniform(rng));
         if (nh < 1) { y[0] = 0.0; y[5] = -2.0; break; }
-        if (dbessel_len(nh) > scratch_len) { y[0] = nh; y[5] = -1.0; break; }
         double s, th;
         sv4 pol;
         uint32_t tr = 0;
         bool capped = false;
-        lp_cl_sample(dbessel_at(nh, scratch + i, n), x[0], x[1], rng, s, th, pol, tr, capped);
+        lp_cl_sample(dbessel_of(nh), x[0], x[1], rng, s, th, pol, tr, capped);
         y[0] = nh; y[1] = s; y[2] = th; y[3] = pol.q; y[4] = pol.u; y[5] = tr;
         break;
     }

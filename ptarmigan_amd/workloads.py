@@ -59,3 +59,26 @@ def batched_scan(n_coll=64, scale=1.0, seed=2024, polarization=Polarization.LINE
         beams.append(Beam(n=int(counts[k]), gamma=gamma, sigma=0.01 * gamma, species=Species.ELECTRON, rms_div=1e-6))
     options = CollideOptions(tracking_photons=True)
     return lasers, beams, options
+
+
+def surrogate_domain(n, polarization=Polarization.LINEAR, n_coll=64, seed=4096):
+    """The learned sampler's domain as one batch: n_coll plane-wave collisions
+    (0.8 um, 16-cycle cos^2, the given polarization) with a0 uniform on
+    [0.5, 20] and electron energy uniform on [2, 20] GeV (1% spread, 1 urad
+    divergence), both stratified (one draw in each of n_coll equal slices, the
+    energies shuffled against the a0s), n primaries split evenly between them;
+    LMA nonlinear Compton only. The same seed gives the same collisions at any
+    n, so a smaller n samples the same distribution. Returns (lasers, beams, options)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    k = np.arange(n_coll)
+    a0s = 0.5 + 19.5 * (k + rng.uniform(size=n_coll)) / n_coll
+    energies = rng.permutation(2000.0 + 18000.0 * (k + rng.uniform(size=n_coll)) / n_coll)  # MeV
+    per = [n // n_coll + (1 if k < n % n_coll else 0) for k in range(n_coll)]
+    lasers, beams = [], []
+    for k in range(n_coll):
+        lasers.append(Laser(LaserKind.PLANE_WAVE, a0=float(a0s[k]), wavelength=0.8e-6, n_cycles=16.0,
+                            envelope=Envelope.COS_SQUARED, polarization=polarization))
+        gamma = energies[k] / ELECTRON_MASS_MEV
+        beams.append(Beam(n=max(1, per[k]), gamma=gamma, sigma=0.01 * gamma, species=Species.ELECTRON, rms_div=1e-6))
+    return lasers, beams, CollideOptions(tracking_photons=False)

@@ -2,7 +2,8 @@
 
 Deterministic pieces are compared value by value: the LP rate table lookup and
 harmonic CDF inversion (1e-12 relative; harmonic indices exactly), J(n, x, y)
-by recurrence and saddle point (1e-11 of the largest |J| in the batch),
+by the kernels' saddle point and quadrature and by the reference's recurrence
+(1e-11 of the largest |J| in the batch),
 field-only plane-wave trajectories (1e-12 relative). The sampler and the full
 shower are compared statistically at matched sample size (two-sample KS /
 chi-square at p > 1e-3): the RNG streams differ (xoshiro256** on the CPU,
@@ -86,23 +87,39 @@ def test_double_bessel_matches_oracle(engine, orc, n):
         assert np.abs(got - ref).max() <= 1e-11 * scale, rec
 
 
-@pytest.mark.parametrize("n", [3, 7, 8, 25, 1000, 2500])
-def test_double_bessel_checkpointed_is_the_recurrence(engine, n):
-    """The emission kernel's checkpointed J recurrence (forward states recomputed
-    per segment in LDS) gives the values of the stored recurrence, bit for bit."""
+@pytest.mark.parametrize("n", [3, 7, 8, 25, 1000, 2500, 10000])
+def test_double_bessel_quadrature_matches_recurrence(engine, orc, n):
+    """The kernels' trapezoid quadrature of J(n - 2 .. n + 2, x, y) against the
+    reference's recurrence (double_bessel.rs:142-246), both on the device, over
+    the whole domain |x| <= n sqrt 2, 0 <= y <= n / 2 (the saddle-point region
+    included), and against the oracle's recurrence (built without FMA contraction,
+    as the reference): within 1e-12 absolute (|J| <= 1). The quadrature's own
+    rounding floor is ~1e-15 times the square root of its ~n nodes; the rest is
+    the recurrences' rounding (the device's contracts a * b + c into FMAs)."""
     from ptarmigan_amd import Fn
-    say("J checkpointed", n)
+    say("J quadrature", n)
     rng = np.random.default_rng(100 + n)
-    m = 200
+    m = 200 if n < 5000 else 40
     x = n * math.sqrt(2.0) * rng.uniform(size=m)
     y = 0.5 * n * rng.uniform(size=m)
-    plain = engine.eval(Fn.DOUBLE_BESSEL, np.stack([np.full(m, n), x, y, np.full(m, 1.0)], 1), 5)
-    ck = engine.eval(Fn.DOUBLE_BESSEL, np.stack([np.full(m, n), x, y, np.full(m, 2.0)], 1), 5)
-    assert np.array_equal(plain, ck)
+    rec = engine.eval(Fn.DOUBLE_BESSEL, np.stack([np.full(m, n), x, y, np.full(m, 1.0)], 1), 5)
+    quad = engine.eval(Fn.DOUBLE_BESSEL, np.stack([np.full(m, n), x, y, np.full(m, 2.0)], 1), 5)
+    err = np.abs(quad - rec).max()
+    # negative x (theta beyond pi / 2 in the rates): J(m, -x, y) = (-1)^m J(m, x, y)
+    neg = engine.eval(Fn.DOUBLE_BESSEL, np.stack([np.full(m, n), -x, y, np.full(m, 2.0)], 1), 5)
+    sign = (-1.0) ** (n - 2 + np.arange(5))
+    err_neg = np.abs(neg - sign * rec).max()
+    k = np.arange(0, m, 4)
+    ora = np.array([orc.double_bessel(n, x[i], y[i], recurrence=True) for i in k])
+    err_orc = np.abs(quad[k] - ora).max()
+    say("  max |dJ|", err, "negative x", err_neg, "oracle", err_orc)
+    assert err < 1e-12 and err_neg < 1e-12 and err_orc < 1e-12
 
 
 def test_double_bessel_reference_kat_on_gpu(engine):
-    """double_bessel.rs:417-597 known answers through the device recurrence."""
+    """double_bessel.rs:417-597 known answers through the device recurrence (1e-9
+    relative) and through the kernels' evaluation (1e-9 relative where |J| > 1e-6;
+    the 1.9e-31 value lies below the quadrature's 1e-15 absolute floor)."""
     from ptarmigan_amd import Fn
     pts = [(10, 0.5 * 10 * math.sqrt(2), 0.5 * 5, 0.027510433345552824743),
            (100, 0.9375 * 100 * math.sqrt(2), 0.5 * 50, -0.024581413776829650737),
@@ -110,9 +127,13 @@ def test_double_bessel_reference_kat_on_gpu(engine):
            (1000, 1100.0, 250.0, 1.9079405288254989534e-31),
            (3000, 3500.0, 1500.0, 9.5390642676650740986e-3)]
     for n, x, y, t in pts:
-        for mode in (1.0, 2.0):
-            got = engine.eval(Fn.DOUBLE_BESSEL, [[n, x, y, mode]], 5)[0, 2]
-            assert abs((got - t) / t) < 1e-9, (n, x, y, mode)
+        got = engine.eval(Fn.DOUBLE_BESSEL, [[n, x, y, 1.0]], 5)[0, 2]
+        assert abs((got - t) / t) < 1e-9, (n, x, y)
+        got = engine.eval(Fn.DOUBLE_BESSEL, [[n, x, y, 0.0]], 5)[0, 2]
+        if abs(t) > 1e-6:
+            assert abs((got - t) / t) < 1e-9, (n, x, y)
+        else:
+            assert abs(got) < 1e-13, (n, x, y)
 
 
 @pytest.mark.parametrize("a,eta,fixed_n", [(2.0, 0.1, 0), (1.0, 0.1, 1), (2.0, 0.3, 8), (5.0, 0.05, 0)])
@@ -227,35 +248,6 @@ def test_lma_shower_statistics_match_oracle(engine, orc):
     assert ks_2samp(eg, ec).pvalue > 1e-3
 
 
-def test_deferred_high_harmonics_reproduce_the_shower(engine, orc, monkeypatch):
-    """Events whose harmonic outgrows the per-lane J(n, x, y) workspace are
-    re-run in a second pass with a larger one. Forcing that for almost every
-    event (workspace for n <= 3) must give the same shower bit for bit, since
-    every draw comes from the event's own counter-based RNG block."""
-    from ptarmigan_amd import workloads
-    laser, beam, opts = workloads.plane_wave_lma(20_000)
-    laser.a0 = 5.0
-    engine.set_laser(laser)
-    dt, z0 = engine.field_info()
-    prim = orc.build_beam(beam.desc(), z0 + 3.0 * beam.length, 3)
-    say("deferred pass: normal run")
-    s1, _, c1 = engine.collide(prim, opts, seed=3)
-    monkeypatch.setenv("PTG_LMA_HARMONIC_CAP", "3")
-    say("deferred pass: forced run")
-    s2, _, c2 = engine.collide(prim, opts, seed=3)
-    assert c1["emissions"] == c2["emissions"] > 1000
-    # in the LMA path the second launch counter counts event-kernel launches
-    assert c2["photon_launches"] > c1["photon_launches"]
-    assert c1["error_flags"] == c2["error_flags"] == 0
-    for sp in (0, 2):
-        m1, m2 = s1["species"] == sp, s2["species"] == sp
-        k1 = np.lexsort(s1["u"][m1].T)
-        k2 = np.lexsort(s2["u"][m2].T)
-        assert np.array_equal(s1["u"][m1][k1], s2["u"][m2][k2])
-        assert np.array_equal(s1["pol"][m1][k1], s2["pol"][m2][k2])
-        assert np.array_equal(s1["r"][m1][k1], s2["r"][m2][k2])
-
-
 # ---------------------------------------------------------------- LP, classical limit (lp/classical.rs)
 def test_lp_classical_rate_and_harmonic_match_oracle(engine, orc):
     from ptarmigan_amd import Fn
@@ -279,6 +271,10 @@ def test_lp_classical_rate_and_harmonic_match_oracle(engine, orc):
 
 
 def test_lp_classical_ceiling_matches_oracle(engine, orc):
+    """lp/classical.rs:43-110. The rate is (-g0^2 - a^2 (g0 g2 - g1^2)) / 2 pi in
+    J(n - 2 .. n + 2); the device's J (quadrature) and the oracle's (recurrence)
+    differ by up to ~5e-13 absolute, which moves the rate by up to about
+    (1 + a^2) sqrt(rate) 1e-12: the bound is 1e-10 relative plus that."""
     from ptarmigan_amd import Fn
     say("lp classical ceiling")
     rng = np.random.default_rng(22)
@@ -287,7 +283,9 @@ def test_lp_classical_ceiling_matches_oracle(engine, orc):
     n = np.maximum(1, rng.uniform(size=m) * (6.0 * (1.0 + a * a) + 2.0 * a ** 3)).astype(int).astype(float)
     got = engine.eval(Fn.LP_CL_CEILING, np.stack([n, a], 1), 1)[:, 0]
     ref = np.array([orc.lib().orc_lp_cl_ceiling(int(n[i]), a[i]) for i in range(m)])
-    assert np.allclose(got, ref, rtol=1e-10, atol=0)
+    bound = 1e-10 * np.abs(ref) + 1e-12 * (1.0 + a * a) * np.sqrt(np.abs(ref))
+    say("  max |d| / bound", (np.abs(got - ref) / bound).max())
+    assert np.all(np.abs(got - ref) <= bound)
 
 
 @pytest.mark.parametrize("a,eta,fixed_n", [(1.0, 0.05, 0), (3.0, 0.05, 0), (8.0, 0.02, 0), (3.0, 0.05, 6)])

@@ -2,8 +2,11 @@
 each device sample (n, s, theta) is fed to the oracle's Stokes functions, which
 tests/test_oracle_stokes.py pins by the Thomson-limit dipole, the classical limits
 and the double-differential rate (lp/mod.rs:622-671, lp/classical.rs:294-330,
-cp/mod.rs:203-232, cp/classical.rs:155-175). Tolerance 1e-9: the device and the
-oracle evaluate the same J(n, x, y) / J_n recurrences in a different order."""
+cp/mod.rs:203-232, cp/classical.rs:155-175). The device evaluates J(n, x, y) by
+quadrature and the oracle by the reference's recurrence; they agree to ~1e-13
+absolute, which moves xi = O((1 + a^2) J dJ) and so S = xi / xi0 by up to about
+1e-13 (1 + a^2) / xi0: quantum LP tolerance 1e-9 plus that, per sample; classical
+LP and CP 1e-9 (CP evaluates J_n by the same recurrence in a different order)."""
 import sys
 
 import numpy as np
@@ -27,9 +30,10 @@ def test_lp_stokes_at_device_samples(engine, orc, a, eta, fixed_n):
     got = engine.eval(Fn.LP_SAMPLE, inp, 6)
     ref = np.array([orc.lp_stokes(int(r[0]), a, eta, r[1], r[2]) for r in got])
     assert np.all(ref[:, 0] > 0.0)
-    err = np.abs(got[:, 3:5] - ref[:, 1:3]).max()
-    say("  max |dS|", err)
-    assert err < 1e-9
+    err = np.abs(got[:, 3:5] - ref[:, 1:3]).max(axis=1)
+    bound = 1e-9 + 1e-13 * (1.0 + a * a) / ref[:, 0]
+    say("  max |dS|", err.max(), "max |dS| / bound", (err / bound).max())
+    assert np.all(err < bound)
 
 
 @pytest.mark.parametrize("a,eta,fixed_n", [(0.5, 0.05, 0), (3.0, 0.05, 0), (3.0, 0.05, 6)])

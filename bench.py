@@ -129,6 +129,11 @@ def cpu_baseline(name, n_primaries, threads):
     return steps / secs, f"{n_primaries} primaries of the same beam ({steps} particle-steps, {secs:.1f} s)", flags
 
 
+def log(*a):
+    """Progress on stderr (the JSON line is the only stdout)."""
+    print("bench:", *a, file=sys.stderr, flush=True)
+
+
 def _profile_figure(path, key):
     """A `key = value` line of a committed counter summary under profiles/."""
     try:
@@ -162,7 +167,7 @@ def copy_calibration(nbytes):
 
 
 # ptg_quantity / ptg_species ids (include/ptg.h)
-Q_ANGLE_X, Q_ENERGY, Q_UNIT, Q_S1, Q_S2, Q_S3 = 0, 14, 15, 24, 25, 26
+Q_THETA, Q_ENERGY, Q_UNIT, Q_S1, Q_S2, Q_S3 = 2, 14, 15, 24, 25, 26
 PHOTON = 2
 
 
@@ -179,17 +184,20 @@ def _chi2_hist(ha, hb):
 
 def _final_photons(eng, e_max, a_lim):
     """The resident final-state photons: 200-bin histograms of energy over
-    [0, e_max) MeV and of angle_x over [-a_lim, a_lim) rad, their number, and the
-    mean Stokes parameters with their standard errors (device reductions)."""
+    [0, e_max) MeV and of their polar angle to the beam axis (theta, from -z)
+    over [0, a_lim) rad, their number, and the mean Stokes parameters with their
+    standard errors (device reductions)."""
     import math
     he = eng.histogram(PHOTON, Q_ENERGY, 200, range_=(0.0, e_max))[0]
-    ha = eng.histogram(PHOTON, Q_ANGLE_X, 200, range_=(-a_lim, a_lim))[0]
+    ha = eng.histogram(PHOTON, Q_THETA, 200, range_=(0.0, a_lim))[0]
     n = eng.stats(PHOTON, Q_UNIT, weight=Q_UNIT)[2]
     st = []
     for q in (Q_S1, Q_S2, Q_S3):
         o = eng.stats(PHOTON, q, weight=Q_UNIT)
         m = o[1] / o[2]
-        st.append((m, math.sqrt(max(o[7] / o[2] - m * m, 0.0) / o[2])))
+        # NaN when any photon's Stokes parameters are (the reference's CP Stokes
+        # parameters overflow at the highest harmonics, on both paths)
+        st.append((m, math.sqrt(max(o[7] / o[2] - m * m, 0.0) / o[2]) if math.isfinite(m) else math.nan))
     return he, ha, n, st
 
 
@@ -200,8 +208,9 @@ def _against_fp64(ref, ref_primaries, got, got_primaries):
     y_ref, y_got = ref[2] / ref_primaries, got[2] / got_primaries
     return {"photon_yield_err": y_got / y_ref - 1.0, "photon_yield_err_sigma": math.sqrt(1.0 / ref[2] + 1.0 / got[2]),
             "energy_chi2_p": _chi2_hist(ref[0], got[0]), "angle_chi2_p": _chi2_hist(ref[1], got[1]),
-            "stokes_mean_diff_in_sigma": [(a[0] - b[0]) / max(math.hypot(a[1], b[1]), 1e-300)
-                                          for a, b in zip(ref[3], got[3])]}
+            "stokes_mean_diff": [b[0] - a[0] for a, b in zip(ref[3], got[3])],
+            "stokes_mean_diff_in_sigma": [(b[0] - a[0]) / math.hypot(a[1], b[1]) if math.hypot(a[1], b[1]) > 0.0
+                                          else None for a, b in zip(ref[3], got[3])]}
 
 
 def _timed(eng, opts, seed, sync):
@@ -249,7 +258,7 @@ def config3_compare(eng, sync, n=10_000_000):
         opts.precision = 1 if mode == "mixed" else 0
         eng.run(opts, seed=seed + 100)  # warm
         c, el, sps = _timed(eng, opts, seed, sync)
-        res[mode] = (c, el, sps, _final_photons(eng, 1.05e4, 5.0 * laser.a0 / gamma))
+        res[mode] = (c, el, sps, _final_photons(eng, 1.05e4, 2.0e-3))
     (cf, tf, sf, pf), (cm, tm, sm, pm) = res["fp64"], res["mixed"]
     return {"workload": WORKLOADS["lma"] + f", {n} primaries", "fp64_steps_per_s": sf, "mixed_steps_per_s": sm,
             "fp64_ms": 1e3 * tf, "mixed_ms": 1e3 * tm, "speedup": tf / tm, **_against_fp64(pf, n, pm, n)}
@@ -268,13 +277,13 @@ def config4_compare(eng, sync, n, fp64_fraction):
         model, meta = surrogate.load(surrogate.default_path(int(pol)))
         eng.surrogate_load(model)
         res = {}
-        for mode in ("surrogate_fp32", "fp64"):
+        for mode, seed in (("surrogate_fp32", 7), ("fp64", 8)):  # independent samples
             m = n // 2 if mode != "fp64" else max(64, int(n // 2 * fp64_fraction))
             lasers, beams, opts = workloads.surrogate_domain(m, pol)
-            eng.set_batch(lasers, beams, seed=7)
+            eng.set_batch(lasers, beams, seed=seed)
             opts.surrogate = mode != "fp64"
             opts.precision = 1 if mode != "fp64" else 0
-            c, el, sps = _timed(eng, opts, 8, sync)
+            c, el, sps = _timed(eng, opts, seed, sync)
             res[mode] = (m, c, el, sps, _final_photons(eng, 2.05e4, 0.05))
         (ms, cs, ts, ss, ps), (mf, cf, tf, sf, pf) = res["surrogate_fp32"], res["fp64"]
         out["lp" if pol == Polarization.LINEAR else "cp"] = {
@@ -297,14 +306,14 @@ def config5_compare(eng, sync, scale, fp64_scale):
     eng.surrogate_load(model)
     res = {}
     tune = None
-    for mode, sc in (("surrogate_fp32", scale), ("fp64", fp64_scale)):
+    for mode, sc, seed in (("surrogate_fp32", scale, 5), ("fp64", fp64_scale, 6)):  # independent samples
         lasers, beams, opts = workloads.batched_scan(64, scale=sc)
-        eng.set_batch(lasers, beams, seed=5)
+        eng.set_batch(lasers, beams, seed=seed)
         opts.surrogate = mode != "fp64"
         opts.precision = 1 if mode != "fp64" else 0
         if tune is None:
             tune = eng.autotune(opts, seed=5, max_primaries=200_000)
-        c, el, sps = _timed(eng, opts, 6, sync)
+        c, el, sps = _timed(eng, opts, seed, sync)
         res[mode] = (sum(b.n for b in beams), c, el, sps, _final_photons(eng, 1.8e4, 0.05))
     (ms, cs, ts, ss, ps), (mf, cf, tf, sf, pf) = res["surrogate_fp32"], res["fp64"]
     return {"workload": "64 PlaneWave LP collisions in one batch, a0 0.5-20, 5-17.5 GeV, Compton + Breit-Wheeler",
@@ -334,7 +343,7 @@ def main():
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="lcfa",
                     help="lcfa: the focused-laser cascade (default, the headline workload); lma: the plane-wave LMA case")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
-                    help="mixed: FP32 rate/CDF tables and pusher algebra, FP64 phase, momentum update and "
+                    help="mixed: FP32 rate tables and pusher algebra, FP64 phase, momentum update and "
                          "survival product (LMA workloads only)")
     ap.add_argument("--surrogate", action="store_true",
                     help="sample LMA emission with the shipped learned sampler (lma workload)")
@@ -414,12 +423,14 @@ def main():
                 "valu_busy": _profile_figure(VALU_FILE, "valu_busy") if lcfa else None,
                 "hbm_bytes_per_step": bps}
 
+    log(f"timed {args.steps} runs: {value:.4g} particle-steps/s")
     info = host_cpu()
     threads = args.cpu_threads or baseline_threads(info)
     if not args.cpu_primaries:
         args.cpu_primaries = 80_000 if lcfa else 4_000_000
     cpu = None
     if not args.no_cpu:
+        log(f"CPU baseline on {threads} threads")
         v, sample, flags = cpu_baseline(args.workload, args.cpu_primaries, threads)
         cpu = {"value": v, "unit": "particle-steps/s", "cores": threads, "kind": "port", "sample": sample,
                "build": flags, "host": info,
@@ -427,12 +438,17 @@ def main():
 
     compare = None
     if not args.no_compare and lcfa and args.precision == "fp64" and not args.surrogate:
+        log("plane-wave LMA vs CPU")
         compare = {"config1_plane_wave_lma_vs_cpu": config1_compare(eng, sync, threads, 4_000_000, args.no_cpu)}
+        log("mixed precision at 1e7")
         compare["config3_mixed_vs_fp64"] = config3_compare(eng, sync)
+        log("learned sampler over its domain")
         compare["config4_learned_sampler_vs_fp64"] = config4_compare(eng, sync, args.config4_primaries,
                                                                      args.config4_fp64_fraction)
+        log("batched scan")
         compare["config5_batched_scan_vs_fp64"] = config5_compare(eng, sync, args.config5_scale,
                                                                   args.config5_fp64_scale)
+        log("comparisons done")
 
     line = {
         "metric": "particle-steps/s on 1 MI355X: surrogate+FP32 vs plain FP64; photon yield err",

@@ -380,13 +380,19 @@ int ptg_autotune(ptg_ctx* ctx, const ptg_collide_options* opt, uint64_t seed, ui
  * inputs (ln a, ln eta, u1, u2, u3) scaled to the domain box, outputs
  * (ln nu / 5, v, t): n = ceil(nu), s = v s_max(n), theta = pi t / 2 in the
  * first quadrant. a is the sampler's amplitude: sqrt(2) a_rms (LP), a_rms (CP).
- * Events outside the box are sampled exactly. */
+ * Events outside the box are sampled exactly: those below the larger of a_min and
+ * a_use_min, or below the larger of eta_min and eta_use_min, in the same
+ * generation (where the exact sampler is the cheaper, or the network's spectra
+ * fall short of the exact sampler's), the others (above a_max or eta_max) in one
+ * deferred exact pass. */
 #define PTG_MLP_NPARAM 38304
 typedef struct {
     int32_t polarization;    /* ptg_polarization sampled */
     int32_t valid;           /* passed the validation of ptg_surrogate_train */
-    double a_min, a_max;     /* domain box */
+    double a_min, a_max;     /* domain box: the network's inputs are scaled to it */
     double eta_min, eta_max;
+    double a_use_min;        /* events below this amplitude are sampled exactly (0: a_min) */
+    double eta_use_min;      /* events below this eta are sampled exactly (0: eta_min) */
     float params[PTG_MLP_NPARAM]; /* FP32 weights: W1[128][8] b1[128] W2[128][128] b2 W3[128][128] b3 W4[32][128] b4[32] */
 } ptg_surrogate_model;
 

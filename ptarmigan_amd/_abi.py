@@ -62,7 +62,9 @@ class TuneResult(C.Structure):
 
 class SurrogateModel(C.Structure):
     _fields_ = [("polarization", C.c_int32), ("valid", C.c_int32), ("a_min", C.c_double), ("a_max", C.c_double),
-                ("eta_min", C.c_double), ("eta_max", C.c_double), ("params", C.c_float * MLP_NPARAM)]
+                ("eta_min", C.c_double), ("eta_max", C.c_double), ("a_use_min", C.c_double),
+                ("eta_use_min", C.c_double),
+                ("params", C.c_float * MLP_NPARAM)]
 
 
 class SurrogateTrainDesc(C.Structure):

@@ -249,20 +249,23 @@ PTG_HD inline bool cp_rate(const double* rt, const double* P, const CpScalars& c
 
 // cp_rate on an FP32 copy of the quantum table, in FP32; below the table in eta
 // (classical rate) and a (closed form) it stays FP64
+// FP32 table and interpolation; the branch and cell decisions are taken in FP64,
+// as cp_rate takes them, so that both paths agree on which (a, eta) use the
+// table, the explicit sum or neither.
 __device__ inline bool cp_rate_f(const float* rt, const double* P, const CpScalars& c, double a, double eta, float& out) {
-    const float la = logf(float(a)), le = logf(float(eta));
-    if (le < float(c.min[1]) || la < float(c.min[0])) {
+    const double la = log(a), le = log(eta);
+    if (le < c.min[1] || la < c.min[0]) {
         double o = 0.0;
         bool ok = cp_rate(P + cp_pack::RATE, P, c, a, eta, o);
         out = float(o);
         return ok;
     }
     if (!(a < 20.0 && eta < 2.0)) return false;
-    const float fa = (la - float(c.min[0])) / float(c.step[0]), fe = (le - float(c.min[1])) / float(c.step[1]);
+    const double fa = (la - c.min[0]) / c.step[0], fe = (le - c.min[1]) / c.step[1];
     int ia = int(fa), ie = int(fe);
     ia = ia > cp_pack::COLS - 2 ? cp_pack::COLS - 2 : ia;
     ie = ie > cp_pack::ROWS - 2 ? cp_pack::ROWS - 2 : ie;
-    const float da = fa - float(ia), de = fe - float(ie);
+    const float da = float(fa - double(ia)), de = float(fe - double(ie));
     const float* t0 = rt + ie * cp_pack::COLS + ia;
     const float* t1 = t0 + cp_pack::COLS;
     out = expf((1.0f - da) * (1.0f - de) * t0[0] + da * (1.0f - de) * t0[1] + (1.0f - da) * de * t1[0] + da * de * t1[1]);

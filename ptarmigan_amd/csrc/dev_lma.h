@@ -405,22 +405,26 @@ __device__ __forceinline__ bool lp_rate(const double* rt, const LmaScalars& s, d
 
 // The LP rate lookup of lp_rate on an FP32 copy of the table, in FP32; the
 // linear-Compton branch below the table stays FP64 (it cancels strongly).
+// FP32 table and interpolation; the cell index and the table-range decisions are
+// taken in FP64, as lp_rate takes them, so that both paths agree on which
+// (a, eta) are inside the table (at the table's edge an FP32 index can round past
+// the last column).
 __device__ __forceinline__ bool lp_rate_f(const float* rt, const LmaScalars& s, double a, double eta, float& out) {
-    const float x = logf(float(a)), y = logf(float(eta));
-    if (x < float(s.rate_min[0])) { out = float(lp_linear_rate(a, eta)); return true; }
-    const float fx = (x - float(s.rate_min[0])) / float(s.rate_step[0]);
-    if (!(fx >= 0.0f && fx < float(lma_pack::RATE_COLS - 1))) return false;
+    const double x = log(a), y = log(eta);
+    if (x < s.rate_min[0]) { out = float(lp_linear_rate(a, eta)); return true; }
+    const double fx = (x - s.rate_min[0]) / s.rate_step[0];
+    if (!(fx >= 0.0 && fx < double(lma_pack::RATE_COLS - 1))) return false;
     const int ix = int(fx);
-    const float dx = fx - float(ix);
-    if (y < float(s.rate_min[1])) {
+    const float dx = float(fx - double(ix));
+    if (y < s.rate_min[1]) {
         float f = (1.0f - dx) * rt[ix] + dx * rt[ix + 1];
         out = expf(f - float(s.rate_min[1])) * float(eta);
         return true;
     }
-    const float fy = (y - float(s.rate_min[1])) / float(s.rate_step[1]);
-    if (!(fy >= 0.0f && fy < float(lma_pack::RATE_ROWS - 1))) return false;
+    const double fy = (y - s.rate_min[1]) / s.rate_step[1];
+    if (!(fy >= 0.0 && fy < double(lma_pack::RATE_ROWS - 1))) return false;
     const int iy = int(fy);
-    const float dy = fy - float(iy);
+    const float dy = float(fy - double(iy));
     const float* t0 = rt + iy * lma_pack::RATE_COLS + ix;
     const float* t1 = t0 + lma_pack::RATE_COLS;
     out = expf((1.0f - dx) * (1.0f - dy) * t0[0] + dx * (1.0f - dy) * t0[1] + (1.0f - dx) * dy * t1[0] + dx * dy * t1[1]);

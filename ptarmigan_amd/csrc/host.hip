@@ -80,7 +80,7 @@ void free_buf(DevBuf& b) {
 }
 
 // Counter block layout (unsigned long long)
-enum { C_WORK = 0, C_NEXT, C_OUT, C_INTER, C_ID, C_FLAGS, C_EV, C_LNEXT, C_PH, C_PAIRS, C_SROWS, C_SFB, C_STATS,
+enum { C_WORK = 0, C_NEXT, C_OUT, C_INTER, C_ID, C_FLAGS, C_EV, C_LNEXT, C_PH, C_PAIRS, C_SROWS, C_SFB, C_SNOW, C_STATS,
        C_N = C_STATS + STAT_N };
 
 // Device scratch of ptg_histogram_noalloc: words [0, nbins) are the bins and
@@ -144,7 +144,7 @@ struct ptg_ctx {
     MlpNet surr[2];
     bool surr_loaded[2] = {false, false};
     bool surr_valid[2] = {false, false};
-    double surr_box[2][4] = {};    // a_min, a_max, eta_min, eta_max
+    double surr_box[2][6] = {};    // a_min, a_max, eta_min, eta_max, a_use_min, eta_use_min
     uint32_t* d_surr_list = nullptr;  // row -> event, and the exact sampler's list
     // a batch of collisions (ptg_set_batch): lasers, and each primary's collision
     LaserDev* d_lasers = nullptr;
@@ -158,6 +158,7 @@ struct ptg_ctx {
     // launch shape of the lepton step kernel (ptg_autotune)
     int step_block = 256, step_grid_per_cu = 4;
     uint32_t* d_surr_fb = nullptr;
+    uint32_t* d_surr_now = nullptr;     // events below the surrogate's amplitude range
     uint32_t* d_surr_perm = nullptr;
     unsigned int* d_surr_bucket = nullptr;  // [2][32]
     size_t surr_list_cap = 0;
@@ -534,6 +535,7 @@ void ptg_destroy(ptg_ctx* c) {
     c->surr[1].release();
     if (c->d_surr_list) (void)hipFree(c->d_surr_list);
     if (c->d_surr_fb) (void)hipFree(c->d_surr_fb);
+    if (c->d_surr_now) (void)hipFree(c->d_surr_now);
     if (c->d_surr_perm) (void)hipFree(c->d_surr_perm);
     if (c->d_surr_bucket) (void)hipFree(c->d_surr_bucket);
     free_buf(c->defq);
@@ -921,6 +923,8 @@ static void surr_box_args(const double* box, SurrArgs& S) {
     S.la_h = 0.5 * (la1 - la0);
     S.le_c = 0.5 * (le0 + le1);
     S.le_h = 0.5 * (le1 - le0);
+    S.la_use = std::log(std::max(box[0], box[4]));
+    S.le_use = std::log(std::max(box[2], box[5]));
 }
 
 // Emission events of one generation through the learned sampler: inputs for
@@ -934,12 +938,14 @@ static int lma_surrogate_pass(ptg_ctx* c, LmaArgs m, size_t n_ev, size_t max_lan
     if (c->surr_list_cap < n_ev) {
         if (c->d_surr_list) (void)hipFree(c->d_surr_list);
         if (c->d_surr_fb) (void)hipFree(c->d_surr_fb);
+        if (c->d_surr_now) (void)hipFree(c->d_surr_now);
         if (c->d_surr_perm) (void)hipFree(c->d_surr_perm);
-        c->d_surr_list = c->d_surr_fb = c->d_surr_perm = nullptr;
+        c->d_surr_list = c->d_surr_fb = c->d_surr_now = c->d_surr_perm = nullptr;
         c->surr_list_cap = 0;
         const size_t cap = n_ev + n_ev / 4 + 64;
         HIPCHK(c, hipMalloc(&c->d_surr_list, cap * sizeof(uint32_t)));
         HIPCHK(c, hipMalloc(&c->d_surr_fb, cap * sizeof(uint32_t)));
+        HIPCHK(c, hipMalloc(&c->d_surr_now, cap * sizeof(uint32_t)));
         HIPCHK(c, hipMalloc(&c->d_surr_perm, cap * sizeof(uint32_t)));
         c->surr_list_cap = cap;
     }
@@ -947,7 +953,7 @@ static int lma_surrogate_pass(ptg_ctx* c, LmaArgs m, size_t n_ev, size_t max_lan
     const size_t rows_cap = (n_ev + 31) / 32 * 32;
     if (rows_cap > size_t(1) << 30) return fail(c, PTG_E_OOM, "too many emission events for the surrogate's buffers");
     HIPCHK(c, net.reserve(int(rows_cap)));
-    HIPCHK(c, hipMemsetAsync(c->d_ctr + C_SROWS, 0, 2 * sizeof(unsigned long long), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_ctr + C_SROWS, 0, 3 * sizeof(unsigned long long), c->stream));
     SurrArgs S{};
     surr_box_args(c->surr_box[p], S);
     S.x = net.X;
@@ -956,6 +962,8 @@ static int lma_surrogate_pass(ptg_ctx* c, LmaArgs m, size_t n_ev, size_t max_lan
     S.n_rows = c->d_ctr + C_SROWS;
     S.fb = c->d_surr_fb;
     S.n_fb = c->d_ctr + C_SFB;
+    S.now = c->d_surr_now;
+    S.n_now = c->d_ctr + C_SNOW;
     S.n_ev = n_ev;
     S.perm = c->d_surr_perm;
     S.bucket = c->d_surr_bucket;
@@ -964,10 +972,10 @@ static int lma_surrogate_pass(ptg_ctx* c, LmaArgs m, size_t n_ev, size_t max_lan
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     launch_surr_inputs(c->d_args, m, S, c->stream);
     HIPCHK(c, hipGetLastError());
-    unsigned long long cnt[2] = {0, 0};
+    unsigned long long cnt[3] = {0, 0, 0};
     HIPCHK(c, hipMemcpyAsync(cnt, c->d_ctr + C_SROWS, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    const size_t n_rows = size_t(cnt[0]);
+    const size_t n_rows = size_t(cnt[0]), n_now = size_t(cnt[2]);
     if (n_rows > 0) {
         const size_t M = (n_rows + 31) / 32 * 32;
         if (M > n_rows) HIPCHK(c, hipMemsetAsync(net.X + 8 * n_rows, 0, (M - n_rows) * 8 * sizeof(float), c->stream));
@@ -986,7 +994,11 @@ static int lma_surrogate_pass(ptg_ctx* c, LmaArgs m, size_t n_ev, size_t max_lan
     *ms += t;
     ++*launches;
     const size_t n_fb = size_t(cnt[1]);
-    *n_fallback += n_fb;
+    *n_fallback += n_fb + n_now;
+    if (n_now > 0) {  // below the domain's amplitude: the exact kernels, in this generation
+        int rc = lma_event_pass(c, m, false, n_now, max_lanes, ms, launches, c->d_surr_now, &c->evq);
+        if (rc) return rc;
+    }
     if (n_fb > 0) {  // held for one exact pass when the lepton phase drains
         const size_t need = c->n_def + n_fb;
         if (c->defq.cap < need) HIPCHK(c, grow_buf(c->defq, need + need / 2 + 64, c->n_def, c->stream));
@@ -1979,6 +1991,9 @@ int ptg_surrogate_load(ptg_ctx* c, const ptg_surrogate_model* model) {
     if (model->polarization != PTG_LINEAR && model->polarization != PTG_CIRCULAR) return fail(c, PTG_E_ARG, "unknown polarization");
     int rc = check_surrogate_box(c, model->a_min, model->a_max, model->eta_min, model->eta_max);
     if (rc) return rc;
+    if (!(model->a_use_min >= 0.0 && std::isfinite(model->a_use_min) && model->eta_use_min >= 0.0 &&
+          std::isfinite(model->eta_use_min)))
+        return fail(c, PTG_E_ARG, "surrogate a_use_min and eta_use_min must be finite and >= 0");
     for (int i = 0; i < PTG_MLP_NPARAM; ++i)
         if (!std::isfinite(model->params[i])) return fail(c, PTG_E_ARG, "surrogate parameters must be finite");
     HIPCHK(c, hipSetDevice(c->device));
@@ -1989,6 +2004,8 @@ int ptg_surrogate_load(ptg_ctx* c, const ptg_surrogate_model* model) {
     c->surr_box[p][1] = model->a_max;
     c->surr_box[p][2] = model->eta_min;
     c->surr_box[p][3] = model->eta_max;
+    c->surr_box[p][4] = model->a_use_min;
+    c->surr_box[p][5] = model->eta_use_min;
     c->surr_loaded[p] = true;
     c->surr_valid[p] = model->valid != 0;
     return PTG_OK;
@@ -2038,7 +2055,7 @@ int ptg_surrogate_train(ptg_ctx* c, const ptg_surrogate_train_desc* d, ptg_surro
     m.csc = c->csc;
     m.lcfa = c->d_lcfa;
     m.mode = circ ? LMA_CP : LMA_LP;
-    const double box[4] = {d->a_min, d->a_max, d->eta_min, d->eta_max};
+    const double box[6] = {d->a_min, d->a_max, d->eta_min, d->eta_max, 0.0, 0.0};
     SurrArgs S{};
     surr_box_args(box, S);
     S.key = key_from_seed(d->seed);

@@ -18,8 +18,8 @@ struct LmaArgs {
     CpScalars csc;
     const double* lcfa;          // LCFA pack (Airy functions for J_n at n >= 1000)
     int mode;                    // LmaMode
-    int mixed;                   // mixed precision: FP32 rate table and pusher algebra, FP32 CDF table (LP, CP quantum)
-    const float* tables_f;       // FP32 copy of the LMA pack (mixed precision)
+    int mixed;                   // mixed precision: FP32 rate table and pusher algebra (LP, CP quantum)
+    const float* tables_f;       // FP32 copy of the LMA pack (mixed precision: its rate table)
     // a batch of collisions: laser of collision coll[shower] for each particle (null: A->laser)
     const LaserDev* lasers;
     const uint8_t* coll;
@@ -56,13 +56,16 @@ struct LmaArgs {
 // exact sampler.
 struct SurrArgs {
     double la_c, la_h, le_c, le_h;   // domain box: centre and half width of ln a and ln eta
+    double la_use, le_use;           // ln a, ln eta below which events are sampled exactly in this generation
     // inference
     float* x;                        // [rows][8] network inputs
     const float* out;                // [rows][32] network outputs
     uint32_t* list;                  // row -> event index (events the network samples)
     unsigned long long* n_rows;
-    uint32_t* fb;                    // events left to the exact sampler
+    uint32_t* fb;                    // events left to the exact sampler's deferred pass
     unsigned long long* n_fb;
+    uint32_t* now;                   // events below the box's amplitude: the exact sampler in this generation
+    unsigned long long* n_now;
     unsigned long long n_ev;         // events queued by the step kernel
     uint32_t* perm;                  // rows ordered by harmonic bucket (largest first), or null
     unsigned int* bucket;            // [2][32] bucket counts and cursors

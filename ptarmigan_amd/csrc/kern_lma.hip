@@ -316,9 +316,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PTG_LMA_WAVE
         const double alp = a * 1.4142135623730951;
         Rng g = base;  // counter 0: the harmonic
         const double fr = rng_uniform(g);
-        const int nh = classical ? lp_cl_harmonic(M.tables, M.sc, alp, fr)
-                                 : (M.mixed ? lp_harmonic(M.tables_f + lma_pack::CDF, M.sc, alp, eta, fr)
-                                            : lp_harmonic(cdf, M.sc, alp, eta, fr));
+        // the harmonic is inverted from the FP64 CDF in mixed precision too: near F = 1
+        // neighbouring FP32 entries round to the same value (and 1 - F to 0), where the
+        // monotone inversion divides by zero; it is one lookup per event, not per step
+        const int nh = classical ? lp_cl_harmonic(M.tables, M.sc, alp, fr) : lp_harmonic(cdf, M.sc, alp, eta, fr);
         const bool ok = nh >= 1;
         if (!ok) flags |= PTG_FLAG_TABLE_RANGE;
         double s = 0.0, theta = 0.0;
@@ -728,7 +729,11 @@ __device__ __forceinline__ double surr_nu(const LmaArgs& M, double a, double eta
     return CIRC ? cp_harmonic_cont(M.cp, M.csc, a, eta, frac) : lp_harmonic_cont(M.tables + lma_pack::CDF, M.sc, a, eta, frac);
 }
 
-// Queued events -> network rows (inside the domain) or the exact sampler's list.
+// Queued events -> network rows (inside the domain), the exact sampler in this
+// generation (below the amplitude or eta from which the network is used: where
+// the exact sampler is the cheaper, or the network falls short of it), or the
+// exact sampler's deferred pass (the rest: above the domain, where an exact
+// event is expensive).
 // u1 is the exact path's harmonic draw (counter 0); u2, u3 come from counter 2.
 template <bool CIRC>
 __global__ void k_surr_inputs(const TransportArgs* __restrict__ A, LmaArgs M, SurrArgs S) {
@@ -749,7 +754,9 @@ __global__ void k_surr_inputs(const TransportArgs* __restrict__ A, LmaArgs M, Su
     double u2, u3;
     rng_uniform2(g2, u2, u3);
     const double la = log(amp), le = log(eta);
-    if (surr_in_domain(S, la, le)) {
+    if (la < S.la_use || le < S.le_use) {
+        S.now[wave_reserve(S.n_now, 1u)] = i;
+    } else if (surr_in_domain(S, la, le)) {
         const unsigned long long j = wave_reserve(S.n_rows, 1u);
         S.list[j] = i;
         surr_features(S, la, le, u1, surr_u2_level(u2), u3, S.x + 8 * j);

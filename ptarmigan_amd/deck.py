@@ -8,10 +8,9 @@ block (read once, in order), physical and conversion constants, and the
 reference's helper functions. `Deck.collisions()` gives one (Laser, Beam,
 CollideOptions) per value of a looped `laser:a0` (start/stop/step,
 src/input/mod.rs:239-263), and `run_deck` transports them — several a0 values
-as one batch of collisions.
-
-Not supported (it raises DeckError naming the key): beams loaded from HDF5
-(`beam:from_hdf5`).
+as one batch of collisions. Beams can be imported from a particle dump
+(`beam:from_hdf5`), and `output:dump_all_particles: hdf5` writes one
+(ptarmigan_amd/h5.py, over libhdf5).
 """
 import dataclasses
 import math
@@ -33,6 +32,32 @@ PROTON_MASS = 1.672622e-27
 COMPTON_TIME = 1.28808867e-21
 ELECTRON_MASS_MEV = 0.510999
 ALPHA_FINE = 7.29735257e-3
+
+
+@dataclasses.dataclass
+class LoadedBeam:
+    """beam:from_hdf5 (src/main.rs:533-562): primaries imported from a particle
+    dump (BeamLoader, src/particle/loader.rs); min_energy in MeV."""
+    file: str
+    species: Species
+    distance_bt_ips: float
+    auto_timing: bool = True
+    min_energy: float = 0.0
+    max_angle: float = math.pi
+    angle: float = 0.0
+    collision_plane_angle: float = 0.0
+    offset: tuple = (0.0, 0.0, 0.0)
+    length: float = 0.0
+    n: int = 0
+
+    def load(self, engine):
+        """Reads the dump (h5.load_beam, at this laser's ideal initial z) into the engine."""
+        from . import h5
+        names = {Species.ELECTRON: "electron", Species.POSITRON: "positron", Species.PHOTON: "photon"}
+        prim = h5.load_beam(self.file, names[self.species], self.distance_bt_ips, engine.field_info()[1],
+                            self.angle, self.offset, self.auto_timing, self.min_energy, self.max_angle)
+        self.n = len(prim["species"])
+        engine.load_primaries(prim)
 
 
 class DeckError(ValueError):
@@ -233,6 +258,7 @@ class Deck:
             raise DeckError("the deck must be a YAML mapping")
         self.data = data
         self.name = name
+        self.text = None  # the input file's text, when read from one
         self.dir = ""
         self.context = {}
         consts = data.get("constants")
@@ -256,7 +282,9 @@ class Deck:
             raise DeckError(f"not a YAML file: {err}")
         if not docs or docs[0] is None:
             raise DeckError("YAML file is empty")
-        return cls(docs[0], name)
+        d = cls(docs[0], name)
+        d.text = text
+        return d
 
     @classmethod
     def from_file(cls, path):
@@ -489,9 +517,8 @@ class Deck:
                              finite_bandwidth=bw) for a0 in a0s]
 
     def beam(self):
-        """The generated beam (src/main.rs:480-800)."""
-        if self.contains("beam:from_hdf5"):
-            raise DeckError("beam:from_hdf5 is not supported by this build")
+        """The generated beam (src/main.rs:480-800), or the beam imported from a
+        particle dump (beam:from_hdf5, src/main.rs:533-562)."""
         species = Species.ELECTRON
         if self.contains("beam:species"):
             s = self.string("beam:species")
@@ -510,6 +537,17 @@ class Deck:
         off = self.numbers("beam:offset", [0.0, 0.0, 0.0])
         if len(off) != 3:
             raise DeckError("a collision offset must be expressed as a three-vector [dx, dy, dz]")
+        if self.contains("beam:from_hdf5"):
+            from . import h5
+            if not h5.available():
+                raise DeckError("cannot import particles from file (HDF5 support is not built)")
+            import os
+            return LoadedBeam(file=os.path.join(self.dir, self.string("beam:from_hdf5:file")), species=species,
+                              distance_bt_ips=self.number("beam:from_hdf5:distance_bt_ips"),
+                              auto_timing=self.flag("beam:from_hdf5:auto_timing", True),
+                              min_energy=1.0e-6 * self.number("beam:from_hdf5:min_energy", 0.0) / ELEMENTARY_CHARGE,
+                              max_angle=self.number("beam:from_hdf5:max_angle", math.pi), angle=angle,
+                              collision_plane_angle=plane, offset=tuple(off))
         n = self.count("beam:n") if self.contains("beam:n") else self.count("beam:ne")
         w_charge = abs(self.number("beam:charge")) / (ELEMENTARY_CHARGE * n) if self.contains("beam:charge") else None
         w_real = self.number("beam:n_real") / n if self.contains("beam:n_real") else None
@@ -620,6 +658,8 @@ class Deck:
         lasers = self.lasers(lcfa)
         if not self.auto_rate:
             return [(laser, beam, opts) for laser in lasers], seed, stop
+        if isinstance(beam, LoadedBeam):
+            raise DeckError("automatic increase in pair creation not available when loading particles from file")
         if engine is None:
             raise DeckError("increase_pair_rate_by: auto evaluates rates on the device: pass an Engine")
         out = []
@@ -667,6 +707,61 @@ def auto_pair_rate_increase(engine, laser, gamma, lcfa):
     return max((f_ph / q0) / pair, 1.0)
 
 
+def dump_mode(deck):
+    """output:dump_all_particles (src/main.rs:831-851): "hdf5" or None."""
+    if not deck.contains("output:dump_all_particles"):
+        return None
+    s = deck.string("output:dump_all_particles")
+    if s != "hdf5":
+        raise DeckError("specified format for complete data output (dump_all_particles: ...) is invalid")
+    from . import h5
+    if not h5.available():
+        raise DeckError("complete data output has been requested (dump_all_particles: hdf5), but HDF5 support "
+                        "is not built")
+    return s
+
+
+def hdf5_run_info(deck, laser, beam, opts, seed, lcfa, stop, t_stop_global, absorbed_mev):
+    """The run's parameters as the particle dump records them (src/main.rs:1296-1470)."""
+    from . import output
+    from .engine import laser_energy
+    system = output.unit_system(deck)
+    spectrum = getattr(deck, "spectrum", (0, None, (0.0, 0.0)))
+    loaded = isinstance(beam, LoadedBeam)
+    b = {"n": beam.n, "species": {Species.ELECTRON: "electron", Species.POSITRON: "positron",
+                                  Species.PHOTON: "photon"}[beam.species],
+         "collision_angle": beam.angle, "collision_plane_angle": beam.collision_plane_angle, "offset": beam.offset}
+    if loaded:
+        b.update(imported_from_file=True, distance_bt_ips=beam.distance_bt_ips, min_energy=beam.min_energy,
+                 max_angle=beam.max_angle)
+    else:
+        b.update(imported_from_file=False, weight=beam.weight, gamma=beam.gamma, sigma=beam.sigma,
+                 bremsstrahlung_source=spectrum[0] == 2, gamma_min=spectrum[2][0],
+                 radius=beam.radius, radius_max=beam.max_radius if beam.radial == 1 else
+                 (beam.radius if beam.radial == 2 else math.inf),
+                 length=beam.length, rms_divergence=beam.rms_div, polarization=beam.pol,
+                 transverse_normal=beam.radial != 2)
+    focusing = laser.kind in (LaserKind.FOCUSED_LASER, LaserKind.FAST_FOCUSED_LASER)
+    return {
+        "input": deck.text if deck.text is not None else yaml.safe_dump(deck.data),
+        "units": {k: system[k][1] for k in ("length", "energy", "momentum")},
+        "control": {"dt_multiplier": opts.dt_multiplier, "radiation_reaction": opts.rr, "classical": opts.classical,
+                    "pair_creation": opts.tracking_photons, "pol_resolved": opts.pol_resolved, "lcfa": lcfa,
+                    "rng_seed": seed, "increase_pair_rate_by": opts.rate_increase,
+                    "bandwidth_correction": laser.finite_bandwidth,
+                    "stop_at_time": t_stop_global if stop[0] != "automatic" else None,
+                    "select_multiplicity": opts.multiplicity},
+        "laser": {"a0": laser.a0, "wavelength": laser.wavelength, "polarization": int(laser.polarization),
+                  "polarization_angle": laser.pol_angle, "focusing": focusing, "envelope": int(laser.envelope),
+                  "chirp_b": laser.chirp_b, "waist": laser.waist, "n_cycles": laser.n_cycles},
+        "beam": b,
+        "output": {"laser_defines_positive_z": not deck.beam_basis, "discard_background_e": opts.discard_bg_e,
+                   "discard_background_ph": opts.discard_bg_ph, "min_energy": opts.min_energy},
+        "laser_energy": laser_energy(laser),
+        "absorption_j": absorbed_mev * 1.0e6 * ELEMENTARY_CHARGE,
+    }
+
+
 def ident(deck, path=None):
     """output:ident, `auto` meaning the input file's stem (src/main.rs:813-829)."""
     s = deck.string("output:ident", "")
@@ -708,17 +803,20 @@ def run_deck(deck, engine=None, out_dir=None):
     from .engine import Engine
     from . import output
     deck.collisions()  # deck errors before the device is touched
+    dump = dump_mode(deck) is not None and out_dir is not None
     eng = engine or Engine(0)
     cols, seed, stop = deck.collisions(eng)
+    _, _, lcfa, _ = deck.options()
     out = []
     lma = cols[0][0].kind in (LaserKind.PLANE_WAVE, LaserKind.FOCUSED_LASER)
+    loaded = isinstance(cols[0][1], LoadedBeam)
     writing = out_dir is not None and (any(output.distributions(deck).values()) or
-                                       any(output.statistics(deck)))
+                                       any(output.statistics(deck)) or dump)
     same_opts = all(c[2] == cols[0][2] for c in cols)
     stop_mode = {"automatic": 0, "same_time": 1, "given_time": 2}[stop[0]]
     t_stop = stop[1] if stop_mode == 2 else 0.0
     # a common time is per collision, so those run one at a time
-    if len(cols) > 1 and lma and not writing and same_opts and stop_mode != 1:
+    if len(cols) > 1 and lma and not writing and same_opts and stop_mode != 1 and not loaded:
         eng.set_beam_spectrum(*deck.spectrum)
         eng.set_batch([c[0] for c in cols], [c[1] for c in cols], seed=seed)
         cnt = eng.run(cols[0][2], seed=seed)
@@ -734,14 +832,27 @@ def run_deck(deck, engine=None, out_dir=None):
         return out
     for laser, beam, opts in cols:
         eng.set_laser(laser)
-        eng.set_beam_spectrum(*deck.spectrum)
-        eng.build_beam(beam, seed=seed)
+        if loaded:
+            beam.load(eng)
+        else:
+            eng.set_beam_spectrum(*deck.spectrum)
+            eng.build_beam(beam, seed=seed)
         cnt = eng.run(opts, seed=seed)
-        cnt["laser_depletion"] = laser_depletion(laser, _absorbed_mev(eng))
-        eng.finish(stop_mode, t_stop, deck.beam_basis, beam.angle, beam.collision_plane_angle)
+        absorbed = _absorbed_mev(eng)
+        cnt["laser_depletion"] = laser_depletion(laser, absorbed)
+        t_common = eng.finish(stop_mode, t_stop, deck.beam_basis, beam.angle, beam.collision_plane_angle)
         sec, inter = eng.fetch(cnt, intermediates=opts.keep_decayed_photons)
         if writing:
             output.write_distributions(eng, deck, sec, out_dir, ident(deck), laser.a0, len(cols), inter)
             output.write_stats(eng, deck, out_dir, ident(deck), laser.a0, len(cols))
+        if dump:
+            from . import h5
+            cur = ident(deck) if len(cols) <= 1 else f"{ident(deck)}{'_' if ident(deck) else ''}a0_{laser.a0:.3f}"
+            path = f"{out_dir}{'/' if out_dir else ''}{cur}{'_' if cur else ''}particles.h5"
+            t_global = t_common if stop_mode == 1 else t_stop
+            h5.write_particles(path, hdf5_run_info(deck, laser, beam, opts, seed, lcfa, stop, t_global, absorbed),
+                               sec, inter if opts.keep_decayed_photons and deck.flag("output:dump_decayed_photons",
+                                                                                     False) else None)
+            cnt["particle_dump"] = path
         out.append((laser.a0, sec, cnt))
     return out

@@ -22,6 +22,18 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # eta0 = 0.024 - 0.24; photon recoil lowers eta, so the box starts at 0.01.
 A_RANGE = (0.05, 20.0)
 ETA_RANGE = (0.01, 0.3)
+# Below this amplitude the exact sampler, evaluating J(n, x, y) by quadrature,
+# costs no more than the network (measured on plane-wave showers: surrogate with
+# FP32 over exact FP64 0.8-1.0x at a0 <= 2, 1.1x at 3, 1.7-3x at 5, 3.5-9x at
+# 10-20): the loaded models sample events from this amplitude up, and events
+# below it run on the exact kernels in the same generation.
+A_USE_MIN = 3.0
+# Below this eta the shipped models' spectra are not chi-square consistent with
+# the exact sampler's at 1e5 primaries (a0 = 10 showers, independent seeds: CP
+# photon energy p = 3e-26 at eta0 = 0.024 (2 GeV), 3e-5 at 0.06, 0.02 at 0.12;
+# LP angle p = 1e-5 at 0.024, 1e-3 at 0.06, 0.25 at 0.12): events below it run on
+# the exact kernels in the same generation.
+ETA_USE_MIN = 0.1
 
 
 def default_path(polarization):
@@ -36,7 +48,9 @@ def save(model, path, report=None):
     np.savez(path, params=np.ctypeslib.as_array(model.params).copy(), meta=json.dumps(meta))
 
 
-def load(path):
+def load(path, a_use_min=A_USE_MIN, eta_use_min=ETA_USE_MIN):
+    """The model at `path` and its metadata, to be used from amplitude
+    `a_use_min` and eta `eta_use_min` up (0: from its box's lower edges)."""
     from . import _abi
     with np.load(path, allow_pickle=False) as z:
         params = z["params"].astype(np.float32)
@@ -47,6 +61,8 @@ def load(path):
     m.polarization = int(meta["polarization"])
     m.valid = int(meta["valid"])
     m.a_min, m.a_max, m.eta_min, m.eta_max = (float(meta[k]) for k in ("a_min", "a_max", "eta_min", "eta_max"))
+    m.a_use_min = float(a_use_min)
+    m.eta_use_min = float(eta_use_min)
     C.memmove(m.params, params.ctypes.data, params.nbytes)
     return m, meta
 

@@ -242,7 +242,7 @@ def test_driver_defaults_and_alternatives():
     ({"control__increase_pair_rate_by": "auto"}, "pass an Engine"),
     ({"control__select_multiplicity": 2}, "incompatible"),
     ({"control__stop_at_time": "soon"}, "stop_at_time"),
-    ({"beam__from_hdf5": {"file": "x.h5"}}, "not supported"),
+    ({"beam__from_hdf5": {"file": "x.h5"}}, "distance_bt_ips is required"),
     ({"beam__spectrum": {"function": "1.0", "file": "x.dat", "min": 1.0}}, "not both"),
     ({"beam__spectrum": {"function": "sqrt(gamma - 1e6)", "min": 1.0, "max": 2.0}}, "non-numerical"),
     ({"beam__gamma": None}, "beam:gamma is required"),

@@ -3,12 +3,11 @@ oracle's GPU twin: its parity with the oracle is tested elsewhere).
 
 Mixed precision keeps the phase, position and momentum updates, the mass shell
 and the survival product in FP64 and runs the rate table lookup, the
-plane-wave pusher algebra and the CDF table in FP32. Both runs use the same
-seed, so they draw the same random numbers and differ only where an FP32
-rounding moves an event across a survival threshold: photon yields must agree
-within 0.5%, photon energy and angle spectra (200 bins, the reference's
-output binning) must be chi-square consistent, and Stokes means within their
-errors.
+plane-wave pusher algebra and the CDF table in FP32. The two runs use
+independent seeds (beam and events), so they are independent samples: photon
+yields per primary must agree within 5 standard errors, the photon energy and
+angle spectra (200 bins, the reference's output binning) must be chi-square
+consistent (p > 1e-3), and the Stokes means agree within 5 standard errors.
 """
 import math
 import sys
@@ -49,8 +48,11 @@ def _compare(sf, cf, sm, cm):
     assert cm["lepton_steps"] == pytest.approx(cf["lepton_steps"], rel=1e-3)
     pf, pm = sf["species"] == 2, sm["species"] == 2
     yf, ym = sf["weight"][pf].sum(), sm["weight"][pm].sum()
-    say(f"  photon yield fp64 {yf:.6g} mixed {ym:.6g} rel {(ym - yf) / yf:.2e}")
-    assert abs(ym - yf) / yf < 5e-3
+    # equal primary counts and weights: the yields' ratio has a relative standard
+    # error of about sqrt(1 / N_f + 1 / N_m) (photon counts nearly Poisson)
+    sig = math.sqrt(1.0 / pf.sum() + 1.0 / pm.sum())
+    say(f"  photon yield fp64 {yf:.6g} mixed {ym:.6g} rel {(ym - yf) / yf:.2e} (sigma {sig:.1e})")
+    assert abs(ym - yf) / yf < 5.0 * sig
     ef, em = sf["u"][pf, 0], sm["u"][pm, 0]
     bins = np.linspace(0.0, max(ef.max(), em.max()), 201)
     assert _chi2_two_sample(ef, em, bins) > 1e-3
@@ -69,13 +71,13 @@ def _compare(sf, cf, sm, cm):
 
 def test_mixed_precision_lp_plane_wave(engine):
     """LP plane wave, a0 = 2, 16-cycle cos^2, 10 GeV electrons with 1% spread
-    (the plain-method configuration), 2e5 primaries."""
+    (the plain-method configuration), 5e5 primaries."""
     from ptarmigan_amd import workloads
-    laser, beam, opts = workloads.plane_wave_lma(200_000)
+    laser, beam, opts = workloads.plane_wave_lma(500_000)
     say("mixed precision LP")
     sf, cf = _run(engine, laser, beam, opts, 11)
     opts.precision = 1
-    sm, cm = _run(engine, laser, beam, opts, 11)
+    sm, cm = _run(engine, laser, beam, opts, 12)
     _compare(sf, cf, sm, cm)
 
 
@@ -83,13 +85,13 @@ def test_mixed_precision_cp_plane_wave(engine):
     """The same with circular polarisation and a0 = 5."""
     from ptarmigan_amd import workloads
     from ptarmigan_amd.engine import Polarization
-    laser, beam, opts = workloads.plane_wave_lma(50_000)
+    laser, beam, opts = workloads.plane_wave_lma(200_000)
     laser.polarization = Polarization.CIRCULAR
     laser.a0 = 5.0
     say("mixed precision CP")
-    sf, cf = _run(engine, laser, beam, opts, 12)
+    sf, cf = _run(engine, laser, beam, opts, 13)
     opts.precision = 1
-    sm, cm = _run(engine, laser, beam, opts, 12)
+    sm, cm = _run(engine, laser, beam, opts, 14)
     _compare(sf, cf, sm, cm)
 
 

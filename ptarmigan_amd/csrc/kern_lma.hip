@@ -540,12 +540,13 @@ template <bool CIRC>
 __global__ __launch_bounds__(64) void k_pair_event_lma(const TransportArgs* __restrict__ A, LmaArgs M) {
     const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n_lanes = gridDim.x * blockDim.x;
-    const LaserDev& L = A->laser;
     uint64_t n_pairs = 0, trials = 0;
     uint32_t flags = 0;
     for (uint64_t e = lane; e < M.n_ev; e += n_lanes) {
         const PBuf& q = M.ev_in;
         uint32_t i = M.ev_list ? M.ev_list[e] : uint32_t(e);
+        // a batch of collisions: the photon's own collision's laser
+        const LaserDev& L = M.coll ? M.lasers[M.coll[q.shower[i]]] : A->laser;
         d4 r{q.d[P_RT][i], q.d[P_RX][i], q.d[P_RY][i], q.d[P_RZ][i]};
         d4 k{q.d[P_UT][i], q.d[P_UX][i], q.d[P_UY][i], q.d[P_UZ][i]};
         sv4 pol{q.d[P_SI][i], q.d[P_SQ][i], q.d[P_SU][i], q.d[P_SV][i]};

@@ -56,6 +56,30 @@ def test_first_collision_of_a_batch_is_the_single_collision(engine, pairs):
     assert nph[2] / 1000 > nph[0] / 3000 > nph[1] / 2000
 
 
+def test_pair_events_read_their_own_collisions_laser(engine):
+    """Breit-Wheeler events of a later collision in a batch: the photon's field
+    comes from its own collision's pulse (a0 = 20 here, behind an a0 = 0.5 first
+    collision), so the positrons per primary match that collision run alone (an
+    independent seed) within the Poisson error, and no event falls outside the
+    tables, as one would at the first collision's a0."""
+    lasers, beams, opts = _scan(2, [4000, 4000], [0.5, 20.0])
+    beams[1].gamma = beams[0].gamma = 17500.0 / 0.510999
+    engine.set_batch(lasers, beams, seed=21)
+    cb = engine.run(opts, seed=21)
+    sb, _ = engine.fetch(cb)
+    engine.set_laser(lasers[1])
+    engine.build_beam(beams[1], seed=22)
+    cs = engine.run(opts, seed=22)
+    ss, _ = engine.fetch(cs)
+    nb = int(np.sum((sb["species"] == 1) & (sb["shower"] >= 4000)))
+    n0 = int(np.sum((sb["species"] == 1) & (sb["shower"] < 4000)))
+    ns = int(np.sum(ss["species"] == 1))
+    say("positrons: batch collision 1", nb, "collision 0", n0, "single", ns, "flags", cb["error_flags"], cs["error_flags"])
+    assert cb["error_flags"] == 0 and cs["error_flags"] == 0
+    assert ns > 100 and n0 < ns / 20
+    assert abs(nb - ns) < 5.0 * np.sqrt(nb + ns)
+
+
 def test_autotuner_picks_a_shape_and_keeps_results(engine):
     lasers, beams, opts = _scan(4, [20000, 5000, 10000, 2000], [1.0, 2.0, 4.0, 8.0])
     engine.set_batch(lasers, beams, seed=3)

@@ -318,7 +318,7 @@ typedef enum {
     PTG_FN_LCFA_STOKES = 7,      /* (k[4], chi, gamma, v[3], w[3]) -> Stokes[4]  mod.rs:239 */
     PTG_FN_LP_RATE = 8,          /* (a, eta) -> (f, in_table)  nonlinear_compton/lp/mod.rs:430 */
     PTG_FN_LP_HARMONIC = 9,      /* (a, eta, frac) -> n  nonlinear_compton/lp/mod.rs:478 */
-    PTG_FN_DOUBLE_BESSEL = 10,   /* (n, x, y, mode) -> J(n-2..n+2, x, y); mode 0: as evaluated, 1: recurrence, 2: checkpointed recurrence  special_functions/double_bessel.rs */
+    PTG_FN_DOUBLE_BESSEL = 10,   /* (n, x, y, mode) -> J(n-2..n+2, x, y); mode 0: as the kernels evaluate it (saddle point or quadrature), 1: the reference's recurrence, 2: quadrature, 3: the reference's evaluation (saddle point or recurrence), 4: quadrature shared by a wave  special_functions/double_bessel.rs */
     PTG_FN_LP_SAMPLE = 11,       /* (a, eta, stream, fixed_n) -> (n, s, theta, S1, S2, trials)  lp/mod.rs:586 */
     PTG_FN_BW_RATE = 12,         /* (a, eta) -> (par, perp, table: 0 negligible, 1 mid, 2 high, -1 outside)
                                     pair_creation/lp/mod.rs:385-401 */

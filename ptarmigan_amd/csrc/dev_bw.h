@@ -60,44 +60,51 @@ PTG_HD inline bool bw_is_negligible(double a, double eta) {
 
 // ---------------------------------------------------------------- theta bound (lp/theta_bound.rs:10-100)
 struct BwThetaBound { double s[16], f[16]; };
-PTG_HD inline BwThetaBound bw_theta_bound(int n, double a, double eta) {
-    BwThetaBound t;
+// point i of 16 (Chebyshev nodes in s over [1/2, s_max]) and its bound on theta
+PTG_HD inline void bw_theta_bound_point(int n, double a, double eta, int i, double& s_i, double& f_i) {
     double nn = double(n);
     double s_min = 0.5;
     double s_max = 0.5 + sqrt(0.25 - (1.0 + 0.5 * a * a) / (2.0 * nn * eta));
     double ln_j_crit = log(0.443) - log(nn) / 3.0;
-    for (int i = 0; i < 16; ++i) {
-        double z = cos(PI_D * double(i) / 15.0);
-        double s = s_min + 0.5 * (z + 1.0) * (s_max - s_min);
-        t.s[i] = s;
-        double r_n = sqrt(2.0 * nn * eta * s * (1.0 - s) - (1.0 + 0.5 * a * a));
-        double x = a * r_n / (eta * s * (1.0 - s));
-        double y = a * a / (8.0 * eta * s * (1.0 - s));
-        double x_crit = y > nn / 6.0 ? 4.0 * sqrt(y) * sqrt(nn - 2.0 * y) : nn + 2.0 * y;
-        double ln_j_bdy = ln_dbessel_x_zero(n % 2 == 0 ? n : n + 1, y);
-        ln_j_bdy = fmin(ln_j_bdy, ln_j_crit);
-        double cos_theta = fmax(1.0 - x_crit * log(0.01) / (x * (ln_j_bdy - ln_j_crit)), -1.0);
-        double f = acos(cos_theta);
-        t.f[i] = __builtin_isnan(f) ? PI_D : f;
-    }
-    double f_min = t.f[0];
+    double z = cos(PI_D * double(i) / 15.0);
+    double s = s_min + 0.5 * (z + 1.0) * (s_max - s_min);
+    s_i = s;
+    double r_n = sqrt(2.0 * nn * eta * s * (1.0 - s) - (1.0 + 0.5 * a * a));
+    double x = a * r_n / (eta * s * (1.0 - s));
+    double y = a * a / (8.0 * eta * s * (1.0 - s));
+    double x_crit = y > nn / 6.0 ? 4.0 * sqrt(y) * sqrt(nn - 2.0 * y) : nn + 2.0 * y;
+    double ln_j_bdy = ln_dbessel_x_zero(n % 2 == 0 ? n : n + 1, y);
+    ln_j_bdy = fmin(ln_j_bdy, ln_j_crit);
+    double cos_theta = fmax(1.0 - x_crit * log(0.01) / (x * (ln_j_bdy - ln_j_crit)), -1.0);
+    double f = acos(cos_theta);
+    f_i = __builtin_isnan(f) ? PI_D : f;
+}
+// the bound is made non-increasing from the first point (s = s_max) on
+PTG_HD inline void bw_theta_bound_running_min(double* f) {
+    double f_min = f[0];
     for (int i = 1; i < 16; ++i) {
-        if (t.f[i] < f_min) f_min = t.f[i];
-        else t.f[i] = f_min;
+        if (f[i] < f_min) f_min = f[i];
+        else f[i] = f_min;
     }
+}
+PTG_HD inline BwThetaBound bw_theta_bound(int n, double a, double eta) {
+    BwThetaBound t;
+    for (int i = 0; i < 16; ++i) bw_theta_bound_point(n, a, eta, i, t.s[i], t.f[i]);
+    bw_theta_bound_running_min(t.f);
     return t;
 }
-PTG_HD inline double bw_theta_at(const BwThetaBound& t, double s) {
-    double val = t.f[15];
+PTG_HD inline double bw_theta_at(const double* ts, const double* tf, double s) {
+    double val = tf[15];
     for (int i = 1; i < 16; ++i) {
-        if (s > t.s[i]) {
-            double w = (s - t.s[i - 1]) / (t.s[i] - t.s[i - 1]);
-            val = w * t.f[i] + (1.0 - w) * t.f[i - 1];
+        if (s > ts[i]) {
+            double w = (s - ts[i - 1]) / (ts[i] - ts[i - 1]);
+            val = w * tf[i] + (1.0 - w) * tf[i - 1];
             break;
         }
     }
     return fmin(val, 0.5 * PI_D);
 }
+PTG_HD inline double bw_theta_at(const BwThetaBound& t, double s) { return bw_theta_at(t.s, t.f, s); }
 
 // ---------------------------------------------------------------- double-differential rate (lp/mod.rs:50-112)
 PTG_HD inline void bw_s_bounds(int n, double a, double eta, double& s_min, double& s_max) {
@@ -105,7 +112,8 @@ PTG_HD inline void bw_s_bounds(int n, double a, double eta, double& s_min, doubl
     s_max = fmax(0.5 + sqrt(0.25 - (1.0 + 0.5 * a * a) / (2.0 * double(n) * eta)), 0.5);
 }
 // unpolarised part and the S1, S2 coefficients: d^2W/(ds dtheta) = (u - S1 d1 - S2 d2) / 2 pi
-PTG_HD inline void bw_elements(const DBessel& D, double a, double eta, double s, double theta, double& u, double& d1,
+template <class DB>
+PTG_HD inline void bw_elements(const DB& D, double a, double eta, double s, double theta, double& u, double& d1,
                                double& d2) {
     double nn = double(D.n);
     double r_n_sqd = 2.0 * nn * eta * s * (1.0 - s) - (1.0 + 0.5 * a * a);
@@ -123,24 +131,32 @@ PTG_HD inline void bw_elements(const DBessel& D, double a, double eta, double s,
     d1 = c * c - (g0 * st) * (g0 * st) * r_n_sqd;
     d2 = 2.0 * g0 * rn * st * c;
 }
-PTG_HD inline double bw_fully_resolved(const DBessel& D, double a, double eta, double s, double theta, double sv1, double sv2) {
+template <class DB>
+PTG_HD inline double bw_fully_resolved(const DB& D, double a, double eta, double s, double theta, double sv1, double sv2) {
     double u, d1, d2;
     bw_elements(D, a, eta, s, theta, u, d1, d2);
     return (u - sv1 * d1 - sv2 * d2) / (2.0 * PI_D);
 }
 // (lp/mod.rs:230-258) largest value along theta = 0 over 50 points around the peak, x 1.2
-PTG_HD inline double bw_ceiling(const DBessel& D, double a, double eta, double sv1, double sv2) {
+constexpr int BW_CEILING_POINTS = 50;
+PTG_HD inline void bw_ceiling_range(int n, double a, double eta, double& lower, double& upper) {
     double s_min, s_max;
-    bw_s_bounds(D.n, a, eta, s_min, s_max);
-    double centre = 0.5 + sqrt(0.25 - (1.0 + a * a) / (2.0 * double(D.n) * eta));
-    double lower = s_min, upper = s_max;
+    bw_s_bounds(n, a, eta, s_min, s_max);
+    double centre = 0.5 + sqrt(0.25 - (1.0 + a * a) / (2.0 * double(n) * eta));
+    lower = s_min;
+    upper = s_max;
     if (__builtin_isfinite(centre)) {
         lower = fmax(centre - 0.2, s_min);
         upper = 0.5 * (centre + s_max);
     }
+}
+template <class DB>
+PTG_HD inline double bw_ceiling(const DB& D, double a, double eta, double sv1, double sv2) {
+    double lower, upper;
+    bw_ceiling_range(D.n, a, eta, lower, upper);
     double mx = -1.0e300;
-    for (int i = 0; i < 50; ++i)
-        mx = fmax(mx, bw_fully_resolved(D, a, eta, lower + (upper - lower) * double(i) / 50.0, 0.0, sv1, sv2));
+    for (int i = 0; i < BW_CEILING_POINTS; ++i)
+        mx = fmax(mx, bw_fully_resolved(D, a, eta, lower + (upper - lower) * double(i) / double(BW_CEILING_POINTS), 0.0, sv1, sv2));
     return 1.2 * mx;
 }
 
@@ -372,33 +388,4 @@ __device__ inline int bw_harmonic(const double* P, const BwScalars& c, double a,
     if (bw_hi_contains(c, a, eta)) return bw_hi_invert(P + HI_CDF, c, a, eta, sv1, frac);
     return 0;
 }
-// (s, theta) at harmonic n by rejection (lp/mod.rs:261-301); the J workspace is
-// D's (sized for n); gives up after `cap` trials with `capped` set
-__device__ inline void bw_sample_at(const DBessel& D, double a, double eta, double sv1, double sv2, Rng& rng, double& s,
-                                    double& theta, uint32_t& trials, bool& capped) {
-    double s_min, s_max;
-    bw_s_bounds(D.n, a, eta, s_min, s_max);
-    BwThetaBound tb = bw_theta_bound(D.n, a, eta);
-    double mx = bw_ceiling(D, a, eta, sv1, sv2);
-    const uint32_t cap = 1u << 20;
-    for (uint32_t t = 0;; ++t) {
-        double u1, u2;
-        rng_uniform2(rng, u1, u2);
-        s = s_min + (s_max - s_min) * u1;
-        theta = 0.5 * PI_D * u2;
-        if (t >= cap) { capped = true; break; }
-        if (theta > bw_theta_at(tb, s)) continue;
-        double z = mx * rng_uniform(rng);
-        ++trials;
-        if (z < bw_fully_resolved(D, a, eta, s, theta, sv1, sv2)) break;
-    }
-    double u1, u2;
-    rng_uniform2(rng, u1, u2);
-    if (int(u1 * 2.0) == 0) s = 1.0 - s;
-    int quadrant = int(u2 * 4.0);
-    if (quadrant == 1) theta = PI_D - theta;
-    else if (quadrant == 2) theta = PI_D + theta;
-    else if (quadrant == 3) theta = 2.0 * PI_D - theta;
-}
-
 }  // namespace ptg

@@ -726,9 +726,61 @@ __device__ inline void dbessel_quadrature(int n, double x, double y, double j[5]
     j[0] = a0 * inv_k; j[1] = a1 * inv_k; j[2] = a2 * inv_k; j[3] = a3 * inv_k; j[4] = a4 * inv_k;
 }
 
+// The same quadrature shared by the 64 lanes of a wave, which all call it with
+// the same (n, x, y): lane l sums the contiguous nodes [l C, (l + 1) C), C =
+// ceil((K + 1) / 64), starting its rotations from a direct evaluation and
+// re-anchoring every 32 nodes, and a butterfly over the wave adds the partial
+// sums. Each butterfly step adds the same two values in every lane of a pair,
+// so every lane returns the same bits.
+__device__ inline void dbessel_quadrature_wave(int n, double x, double y, double j[5]) {
+    const int lane = int(__lane_id());
+    const int K = dbessel_quad_nodes(n, x, y);
+    const int C = (K + 64) / 64;
+    const int k_lo = lane * C, k_hi = min(k_lo + C, K + 1);
+    const long long two_k = 2LL * K;
+    const long long nm = ((long long)n % two_k + two_k) % two_k;
+    const double inv_k = 1.0 / double(K);
+    double sh, ch, snh, cnh;
+    sincos_phase(PI_D * inv_k, &sh, &ch);
+    sincos_phase(PI_D * double(nm) * inv_k, &snh, &cnh);
+    double cp = 1.0, sp = 0.0, cq = 1.0, sq = 0.0;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
+    for (int k = k_lo; k < k_hi; ++k) {
+        if (((k - k_lo) & 31) == 0) {
+            sincos_phase(PI_D * double(k) * inv_k, &sp, &cp);
+            sincos_phase(PI_D * double((nm * k) % two_k) * inv_k, &sq, &cq);
+        }
+        const double s2 = 2.0 * sp * cp, c2 = (cp - sp) * (cp + sp);
+        double sv, cv;
+        sincos_phase(y * s2 - x * sp, &sv, &cv);
+        const double w = (k == 0 || k == K) ? 0.5 : 1.0;
+        const double cr = w * (cq * cv - sq * sv), sr = w * (sq * cv + cq * sv);
+        a0 += cr * c2 + sr * s2;
+        a1 += cr * cp + sr * sp;
+        a2 += cr;
+        a3 += cr * cp - sr * sp;
+        a4 += cr * c2 - sr * s2;
+        const double cp1 = cp * ch - sp * sh;
+        sp = sp * ch + cp * sh;
+        cp = cp1;
+        const double cq1 = cq * cnh - sq * snh;
+        sq = sq * cnh + cq * snh;
+        cq = cq1;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a0 += __shfl_xor(a0, off, 64);
+        a1 += __shfl_xor(a1, off, 64);
+        a2 += __shfl_xor(a2, off, 64);
+        a3 += __shfl_xor(a3, off, 64);
+        a4 += __shfl_xor(a4, off, 64);
+    }
+    j[0] = a0 * inv_k; j[1] = a1 * inv_k; j[2] = a2 * inv_k; j[3] = a3 * inv_k; j[4] = a4 * inv_k;
+}
+
 // The reference's evaluation (double_bessel.rs:22-37): the saddle point where
 // dbessel_use_spa holds, else the recurrence (probe of the parity tests).
-__device__ inline void dbessel_eval(const DBesselRec& D, double x, double y, double j[5]) {
+PTG_HD inline void dbessel_eval(const DBesselRec& D, double x, double y, double j[5]) {
     if (dbessel_use_spa(D.n, x, y))
         for (int q = 0; q < 5; ++q) j[q] = dbessel_spa(D.n - 2 + q, x, y);
     else
@@ -743,6 +795,15 @@ __device__ inline void dbessel_eval(const DBessel& D, double x, double y, double
         for (int q = 0; q < 5; ++q) j[q] = dbessel_spa(D.n - 2 + q, x, y);
     else
         dbessel_quadrature(D.n, x, y, j);
+}
+// The same at the hand of a whole wave (all lanes, same arguments): the
+// quadrature is shared out over the lanes (dbessel_quadrature_wave).
+struct DBesselWave { int n; };
+__device__ inline void dbessel_eval(const DBesselWave& D, double x, double y, double j[5]) {
+    if (dbessel_use_spa(D.n, x, y))
+        for (int q = 0; q < 5; ++q) j[q] = dbessel_spa(D.n - 2 + q, x, y);
+    else
+        dbessel_quadrature_wave(D.n, x, y, j);
 }
 
 // ---------------------------------------------------------------- LP double-differential rate (lp/mod.rs:21-140)
@@ -812,19 +873,8 @@ __device__ inline void lp_ceiling_arg(int n, double a, double eta, int k, double
         }
     }
 }
-__device__ inline double lp_ceiling(const DBessel& D, double a, double eta) {
-    double mx = -INFINITY;
-    int m = lp_ceiling_count(D.n, a, eta);
-    for (int k = 0; k < m; ++k) {
-        double s, th;
-        lp_ceiling_arg(D.n, a, eta, k, s, th);
-        mx = fmax(mx, lp_ddpr(D, a, eta, s, th));
-    }
-    return 1.15 * mx;
-}
 
 // Largest theta contributing at each s (lp/mod.rs:143-301).
-struct ThetaBound { double s[16], f[16]; };
 PTG_HD inline double ln_dbessel_x_zero(int ni, double y) {
     double n = double(ni);
     double z = sqrt(n / (4.0 * y) - 0.5);
@@ -860,30 +910,13 @@ __device__ __forceinline__ void theta_bound_running_min(double* f) {
         else f[i] = f_min;
     }
 }
-__device__ inline ThetaBound theta_bound(int n, double a, double eta) {
-    ThetaBound tb;
-    for (int i = 0; i < 16; ++i) theta_bound_point(n, a, eta, i, tb.s[i], tb.f[i]);
-    theta_bound_running_min(tb.f);
-    return tb;
-}
-// theta_bound_at over a bound held elsewhere (e.g. one copy per wave in LDS)
+// the bound at s (or v), interpolated between its 16 points (held in LDS, one copy per wave)
 __device__ __forceinline__ double theta_bound_at(const double* ts, const double* tf, double arg) {
     double val = tf[15];
     for (int i = 1; i < 16; ++i) {
         if (arg > ts[i]) {
             double w = (arg - ts[i - 1]) / (ts[i] - ts[i - 1]);
             val = w * tf[i] + (1.0 - w) * tf[i - 1];
-            break;
-        }
-    }
-    return fmin(val, PI_D / 2.0);
-}
-__device__ __forceinline__ double theta_bound_at(const ThetaBound& tb, double arg) {
-    double val = tb.f[15];
-    for (int i = 1; i < 16; ++i) {
-        if (arg > tb.s[i]) {
-            double w = (arg - tb.s[i - 1]) / (tb.s[i] - tb.s[i - 1]);
-            val = w * tb.f[i] + (1.0 - w) * tb.f[i - 1];
             break;
         }
     }
@@ -904,12 +937,6 @@ __device__ inline void theta_bound_low_eta_point(int n, double a, int i, double&
     double cos_theta = fmax(1.0 - x_crit * log(0.01) / (x * (ln_j_bdy - ln_j_crit)), -1.0);
     double f = acos(cos_theta);
     f_i = isnan(f) ? PI_D : f;
-}
-__device__ inline ThetaBound theta_bound_low_eta(int n, double a) {
-    ThetaBound tb;
-    for (int i = 0; i < 16; ++i) theta_bound_low_eta_point(n, a, i, tb.s[i], tb.f[i]);
-    theta_bound_running_min(tb.f);
-    return tb;
 }
 // rate differential in v and theta, 0 < theta < pi/2 (lp/classical.rs:17-40)
 template <class DB>
@@ -1023,33 +1050,6 @@ __device__ inline sv4 lp_cl_stokes(const DB& D, double a, double v, double theta
     double xi2 = 2.0 * (u * A0a) * (u * A0a) * (2.0 * st_ * ct_) + 4.0 * u * A0 * A1 * st_ / a;
     return {1.0, xi1 / xi0, xi2 / xi0, 0.0};
 }
-// (s, theta, Stokes) at harmonic n (lp/classical.rs:254-333)
-__device__ inline void lp_cl_sample(const DBessel& D, double a, double eta, Rng& rng, double& s_out, double& theta_out, sv4& pol,
-                                    uint32_t& trials, bool& capped) {
-    int n = D.n;
-    ThetaBound tb = theta_bound_low_eta(n, a);
-    double smax = 2.0 * double(n) * eta / (1.0 + 0.5 * a * a);
-    double mx = lp_cl_ceiling(D, a);
-    double v = 0.5, theta = 0.0;
-    const uint32_t cap = 1u << 20;
-    for (uint32_t t = 0;; ++t) {
-        rng_uniform2(rng, v, theta);
-        theta *= PI_D / 2.0;
-        ++trials;
-        if (t >= cap) { capped = true; break; }
-        if (theta > theta_bound_at(tb, v)) continue;
-        double z = mx * rng_uniform(rng);
-        if (z < lp_cl_ddpr(D, a, v, theta)) break;
-    }
-    int quadrant = int(rng_uniform(rng) * 4.0);
-    if (quadrant == 1) theta = PI_D - theta;
-    else if (quadrant == 2) theta = PI_D + theta;
-    else if (quadrant == 3) theta = 2.0 * PI_D - theta;
-    s_out = v * smax;
-    theta_out = theta;
-    pol = lp_cl_stokes(D, a, v, theta);
-}
-
 // Stokes parameters of an LP photon at (s, theta) (lp/mod.rs:626-676)
 template <class DB>
 __device__ inline sv4 lp_stokes(const DB& D, double a, double eta, double s, double theta) {
@@ -1074,40 +1074,6 @@ __device__ inline sv4 lp_stokes(const DB& D, double a, double eta, double s, dou
     return {1.0, xi1 / xi0, xi2 / xi0, 0.0};
 }
 
-
-// (n, s, theta, Stokes) of one LP emission (lp/mod.rs:586-676). `trial_cap`
-// bounds the rejection loop; hitting it sets `capped` and keeps the last trial.
-__device__ inline void lp_sample(const double* cdf, const LmaScalars& sc, double a, double eta, Rng& rng, int& n_out,
-                                 double& s_out, double& theta_out, sv4& pol, uint32_t& trials, bool& capped, int fixed_n = 0) {
-    int n = fixed_n > 0 ? fixed_n : lp_harmonic(cdf, sc, a, eta, rng_uniform(rng));
-    const DBessel D = dbessel_of(n);
-    ThetaBound tb = theta_bound(n, a, eta);
-    double sn = 2.0 * double(n) * eta / (1.0 + 0.5 * a * a);
-    double smax = sn / (1.0 + sn);
-    double mx = lp_ceiling(D, a, eta);
-    double s = 0.5 * smax, theta = 0.0;
-    const uint32_t cap = 1u << 20;
-    for (uint32_t t = 0;; ++t) {
-        double u1, u2;
-        rng_uniform2(rng, u1, u2);
-        ++trials;
-        s = smax * u1;
-        theta = (PI_D / 2.0) * u2;
-        if (t >= cap) { capped = true; break; }
-        if (theta > theta_bound_at(tb, s)) continue;
-        double z = mx * rng_uniform(rng);
-        if (z < lp_ddpr(D, a, eta, s, theta)) break;
-    }
-    double uq = rng_uniform(rng);
-    int quadrant = int(uq * 4.0);
-    if (quadrant == 1) theta = PI_D - theta;
-    else if (quadrant == 2) theta = PI_D + theta;
-    else if (quadrant == 3) theta = 2.0 * PI_D - theta;
-    n_out = n;
-    s_out = s;
-    theta_out = theta;
-    pol = lp_stokes(D, a, eta, s, theta);
-}
 
 // ---------------------------------------------------------------- photon from the ZMF (nonlinear_compton/mod.rs:58-138)
 __device__ __forceinline__ d4 boost4(d4 s, d4 u) {  // four_vector.rs:68-80

@@ -900,8 +900,8 @@ static int lma_event_pass(ptg_ctx* c, LmaArgs m, bool pairs, size_t n_ev, size_t
     m.ev_in = in ? in->v : c->evq.v;
     m.ev_list = list;
     m.n_ev = n_ev;
-    // LP emission runs one event per wave (k_event_lma_wave), the others one per lane
-    const bool per_wave = !pairs && (m.mode == LMA_LP || m.mode == LMA_LP_CLASSICAL);
+    // LP emission and LP pair creation run one event per wave, the others one per lane
+    const bool per_wave = pairs ? !(m.mode == LMA_CP || m.mode == LMA_CP_CLASSICAL) : (m.mode == LMA_LP || m.mode == LMA_LP_CLASSICAL);
     size_t lanes = per_wave ? std::min<size_t>(max_lanes, n_ev * 64) : std::min<size_t>(max_lanes, ((n_ev + 63) / 64) * 64);
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     if (pairs) launch_pair_event_lma(c->d_args, m, int(lanes), c->stream);
@@ -1789,8 +1789,8 @@ int ptg_eval(ptg_ctx* c, int fn, const double* in, size_t n, size_t in_w, double
                         return fail(c, PTG_E_ARG, "J(n, x, y) probe needs an integer 3 <= n <= 2e5");
                     if (!(std::fabs(x[1]) <= nn * 1.4142135623730951 && x[2] >= 0.0 && x[2] <= 0.5 * nn))
                         return fail(c, PTG_E_ARG, "J(n, x, y) probe needs |x| <= n sqrt 2 and 0 <= y <= n / 2");
-                    if (!(x[3] == 0.0 || x[3] == 1.0 || x[3] == 2.0 || x[3] == 3.0))
-                        return fail(c, PTG_E_ARG, "J(n, x, y) probe mode is 0, 1, 2 or 3");
+                    if (!(x[3] == 0.0 || x[3] == 1.0 || x[3] == 2.0 || x[3] == 3.0 || x[3] == 4.0))
+                        return fail(c, PTG_E_ARG, "J(n, x, y) probe mode is 0, 1, 2, 3 or 4");
                     if (x[3] == 1.0 || x[3] == 3.0) len = std::max(len, size_t(dbessel_len(nn)));
                 } else if (fn == PTG_FN_LP_CL_CEILING) {
                     if (!(x[0] >= 1.0 && x[0] <= 2.0e5) || double(int(x[0])) != x[0] || !(x[1] > 0.0))

@@ -217,16 +217,13 @@ __device__ inline void lma_emit_finish(const TransportArgs* __restrict__ A, cons
     o.nev[j] = nev + 1;
 }
 
-// One emission per lane (nonlinear_compton::generate, mod.rs:58-138, with
-// lp::sample, lp/mod.rs:586-676, cp::sample, cp/mod.rs:181-234, or
-// cp::classical::sample, cp/classical.rs:128-178).
+// One CP emission per lane (nonlinear_compton::generate, mod.rs:58-138, with
+// cp::sample, cp/mod.rs:181-234, or cp::classical::sample, cp/classical.rs:128-178).
 template <int MODE>
 __global__ __launch_bounds__(64) void k_event_lma(const TransportArgs* __restrict__ A, LmaArgs M) {
-    constexpr bool classical = MODE == LMA_CP_CLASSICAL || MODE == LMA_LP_CLASSICAL;
-    constexpr bool linear = MODE == LMA_LP || MODE == LMA_LP_CLASSICAL;
+    constexpr bool classical = MODE == LMA_CP_CLASSICAL;
     const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n_lanes = gridDim.x * blockDim.x;
-    const double* cdf = M.tables + lma_pack::CDF;
     uint64_t n_emit = 0, trials = 0;
     uint32_t flags = 0;
     for (uint64_t e = lane; e < M.n_ev; e += n_lanes) {
@@ -239,34 +236,19 @@ __global__ __launch_bounds__(64) void k_event_lma(const TransportArgs* __restric
         rng.c1 = nev | EVENT_BLOCK;
         double a = sqrt(u.t * u.t - u.x * u.x - u.y * u.y - u.z * u.z - 1.0);  // rms a
         double eta = k0 * (u.t - u.z);
-        double alp = a * 1.4142135623730951;
         int n = 0;
         double s, cphi;
         sv4 sv;
         uint32_t tr = 0;
         bool capped = false;
         int n_cap_ok = 1;
-        if (!linear) {
-            int nh = classical ? cp_classical_harmonic(M.cp, M.csc, a, rng_uniform(rng))
-                               : cp_harmonic(M.cp, M.csc, a, eta, rng_uniform(rng));
-            if (nh < 1) {
-                flags |= PTG_FLAG_TABLE_RANGE;
-                n_cap_ok = 0;
-            } else {
-                n = nh;
-                cp_sample_at(M.lcfa, M.cp, nh, a, eta, classical, rng, s, cphi, sv, tr, capped);
-            }
+        int nh = classical ? cp_classical_harmonic(M.cp, M.csc, a, rng_uniform(rng)) : cp_harmonic(M.cp, M.csc, a, eta, rng_uniform(rng));
+        if (nh < 1) {
+            flags |= PTG_FLAG_TABLE_RANGE;
+            n_cap_ok = 0;
         } else {
-            int nh = classical ? lp_cl_harmonic(M.tables, M.sc, alp, rng_uniform(rng)) : lp_harmonic(cdf, M.sc, alp, eta, rng_uniform(rng));
-            if (nh < 1) {
-                flags |= PTG_FLAG_TABLE_RANGE;
-                n_cap_ok = 0;
-            } else if (classical) {
-                n = nh;
-                lp_cl_sample(dbessel_of(nh), alp, eta, rng, s, cphi, sv, tr, capped);
-            } else {
-                lp_sample(cdf, M.sc, alp, eta, rng, n, s, cphi, sv, tr, capped, nh);
-            }
+            n = nh;
+            cp_sample_at(M.lcfa, M.cp, nh, a, eta, classical, rng, s, cphi, sv, tr, capped);
         }
         trials += tr;
         if (capped) flags |= PTG_FLAG_REJECT_CAP;
@@ -291,20 +273,171 @@ __device__ __forceinline__ double wave_fmax(double v) {
 // t uses counters 16 + 2t, 17 + 2t of the event's RNG block), so the first
 // accepted one has the distribution of sequential rejection sampling; an
 // event's draws do not depend on how many waves run or on their order.
+// From harmonic PTG_LMA_SHARED_N up, where the quadrature of J(n, x, y) has
+// hundreds to tens of thousands of nodes, the wave instead shares every J
+// evaluation (dbessel_quadrature_wave) and takes the ceiling's points and the
+// candidates one after another: the same candidates and the same accepted one
+// (up to the rounding of the partial sums), without the work of the rounds'
+// candidates beyond it and at a fraction of the latency of one lane's J.
 #ifndef PTG_LMA_WAVE_WPE
 #define PTG_LMA_WAVE_WPE 2  // two waves per SIMD: 256 VGPRs, a little scratch
 #endif
+#ifndef PTG_LMA_SHARED_N
+#define PTG_LMA_SHARED_N 200
+#endif
+// (s, theta) and the Stokes parameters of an LP emission at harmonic nh, for
+// one emission per wave: all 64 lanes call this with the same arguments (a
+// block is one wave) and return the same values. Counter 1 of `base` gives the
+// quadrant of theta, candidate t draws (v, theta) from counter 16 + 2t and its
+// acceptance from 17 + 2t.
+template <bool classical>
+__device__ __noinline__ void lp_sample_wave(int nh, double alp, double eta, const Rng& base, double* tb_s, double* tb_f,
+                                            double& s, double& theta, sv4& sv, uint64_t& trials, uint32_t& flags) {
+    const int lane = threadIdx.x;
+    const uint32_t cap = 1u << 20;
+    __syncthreads();  // the previous event's bound is no longer read
+    const DBessel D = dbessel_of(nh);
+    const double sn = 2.0 * double(nh) * eta / (1.0 + 0.5 * alp * alp);
+    const double smax = classical ? sn : sn / (1.0 + sn);  // s = v * smax
+    if (lane < 16) {  // the theta bound's 16 points, one per lane
+        if (classical) theta_bound_low_eta_point(nh, alp, lane, tb_s[lane], tb_f[lane]);
+        else theta_bound_point(nh, alp, eta, lane, tb_s[lane], tb_f[lane]);
+    }
+    __syncthreads();
+    if (lane == 0) theta_bound_running_min(tb_f);
+    __syncthreads();
+    // A candidate t: (v, theta) from counter 16 + 2t, its acceptance draw from 17 + 2t.
+    // f is the rate at the candidate (not needed if theta is out of bounds).
+    auto candidate = [&](uint32_t t, double& v, double& th, double& uz, double& fc) -> bool {
+        Rng gc = base;
+        gc.c0 = 16u + 2u * t;
+        rng_uniform2(gc, v, th);
+        th *= PI_D / 2.0;
+        if (th > theta_bound_at(tb_s, tb_f, classical ? v : smax * v)) return false;
+        uz = rng_uniform(gc);
+        fc = classical ? lp_cl_ddpr(D, alp, v, th) : lp_ddpr(D, alp, eta, smax * v, th);
+        return true;
+    };
+    const int m = classical ? lp_cl_ceiling_count(nh, alp) : lp_ceiling_count(nh, alp, eta);
+    int first = -1;
+    double v_acc = 0.0, th_acc = 0.0;
+    if (nh >= PTG_LMA_SHARED_N) {  // every lane holds the same values from here
+        const DBesselWave W{nh};
+        double f = classical ? NAN : -INFINITY;
+        for (int k = 0; k < m; ++k) {
+            double sk, tk;
+            if (classical) {
+                lp_cl_ceiling_arg(nh, alp, k, sk, tk);
+                f = fmax(f, lp_cl_ddpr(W, alp, sk, tk));
+            } else {
+                lp_ceiling_arg(nh, alp, eta, k, sk, tk);
+                f = fmax(f, lp_ddpr(W, alp, eta, sk, tk));
+            }
+        }
+        const double mx = (classical ? 1.1 : 1.15) * f;
+        for (uint32_t t = 0; first < 0; ++t) {
+            Rng gc = base;
+            gc.c0 = 16u + 2u * t;
+            double v, th;
+            rng_uniform2(gc, v, th);
+            th *= PI_D / 2.0;
+            if (t >= cap) {  // as the sequential sampler: give up and keep candidate `cap`
+                flags |= PTG_FLAG_REJECT_CAP;
+                v_acc = v;
+                th_acc = th;
+                trials += cap;
+                first = 0;
+                break;
+            }
+            if (th > theta_bound_at(tb_s, tb_f, classical ? v : smax * v)) continue;
+            const double uz = rng_uniform(gc);
+            const double fc = classical ? lp_cl_ddpr(W, alp, v, th) : lp_ddpr(W, alp, eta, smax * v, th);
+            if (mx * uz < fc) {
+                v_acc = v;
+                th_acc = th;
+                trials += t + 1;
+                first = 0;
+            }
+        }
+    }
+    // round 0: the ceiling's sample points on lanes [0, m), the first 64 - m
+    // candidates on the other lanes (their rates do not depend on the ceiling)
+    double f = classical ? NAN : -INFINITY;
+    double v = 0.0, th = 0.0, uz = 0.0, fc = 0.0;
+    bool in_bound = false;
+    for (int k = lane; first < 0 && k < m; k += 64) {
+        double sk, tk;
+        if (classical) {
+            lp_cl_ceiling_arg(nh, alp, k, sk, tk);
+            f = fmax(f, lp_cl_ddpr(D, alp, sk, tk));
+        } else {
+            lp_ceiling_arg(nh, alp, eta, k, sk, tk);
+            f = fmax(f, lp_ddpr(D, alp, eta, sk, tk));
+        }
+    }
+    if (first < 0 && lane >= m) in_bound = candidate(uint32_t(lane - m), v, th, uz, fc);
+    const double mx = (classical ? 1.1 : 1.15) * wave_fmax(f);
+    uint32_t t0 = uint32_t(m < 64 ? 64 - m : 0);  // candidates consumed
+    if (first < 0) {
+        const uint64_t mask = __ballot(lane >= m && in_bound && mx * uz < fc);
+        if (mask) {
+            first = __ffsll((long long)mask) - 1;
+            v_acc = __shfl(v, first, 64);
+            th_acc = __shfl(th, first, 64);
+            trials += uint32_t(first - m) + 1;
+        }
+    }
+    uint32_t width = 8;
+    while (first < 0) {
+        bool acc = false;
+        const uint32_t t = t0 + uint32_t(lane);
+        if (uint32_t(lane) < width && t < cap && candidate(t, v, th, uz, fc)) acc = mx * uz < fc;
+        const uint64_t mask = __ballot(acc);
+        if (mask) {
+            first = __ffsll((long long)mask) - 1;
+            v_acc = __shfl(v, first, 64);
+            th_acc = __shfl(th, first, 64);
+            trials += t0 + uint32_t(first) + 1;
+        } else {
+            t0 += width;
+            width = width < 64 ? 4 * width : 64;
+            if (t0 >= cap) {  // as the sequential sampler: give up and keep candidate `cap`
+                flags |= PTG_FLAG_REJECT_CAP;
+                Rng gc = base;
+                gc.c0 = 16u + 2u * cap;
+                rng_uniform2(gc, v_acc, th_acc);
+                th_acc *= PI_D / 2.0;
+                trials += cap;
+                first = 0;
+            }
+        }
+    }
+    Rng gq = base;  // counter 1: the quadrant
+    gq.c0 = 1u;
+    const int quadrant = int(rng_uniform(gq) * 4.0);
+    theta = th_acc;
+    if (quadrant == 1) theta = PI_D - theta;
+    else if (quadrant == 2) theta = PI_D + theta;
+    else if (quadrant == 3) theta = 2.0 * PI_D - theta;
+    s = smax * v_acc;
+    if (nh >= PTG_LMA_SHARED_N) {
+        const DBesselWave W{nh};
+        sv = classical ? lp_cl_stokes(W, alp, v_acc, theta) : lp_stokes(W, alp, eta, s, theta);
+    } else {
+        if (lane == 0) sv = classical ? lp_cl_stokes(D, alp, v_acc, theta) : lp_stokes(D, alp, eta, s, theta);
+        sv = sv4{__shfl(sv.i, 0, 64), __shfl(sv.q, 0, 64), __shfl(sv.u, 0, 64), __shfl(sv.v, 0, 64)};
+    }
+}
+
 template <int MODE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PTG_LMA_WAVE_WPE))) void k_event_lma_wave(const TransportArgs* __restrict__ A, LmaArgs M) {
     constexpr bool classical = MODE == LMA_LP_CLASSICAL;
     const int lane = threadIdx.x;
     const double* cdf = M.tables + lma_pack::CDF;
-    const uint32_t cap = 1u << 20;
     __shared__ double tb_s[16], tb_f[16];  // the event's theta bound, one copy per wave
     uint64_t n_emit = 0, trials = 0;
     uint32_t flags = 0;
     for (uint64_t e = blockIdx.x; e < M.n_ev; e += gridDim.x) {
-        __syncthreads();  // the previous event's bound is no longer read
         const PBuf& q = M.ev_in;
         const uint32_t i = M.ev_list ? M.ev_list[e] : uint32_t(e);
         d4 u{q.d[P_UT][i], q.d[P_UX][i], q.d[P_UY][i], q.d[P_UZ][i]};
@@ -325,92 +458,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PTG_LMA_WAVE
         double s = 0.0, theta = 0.0;
         sv4 sv{1.0, 0.0, 0.0, 0.0};
         if (ok) {
-            const DBessel D = dbessel_of(nh);
-            const double sn = 2.0 * double(nh) * eta / (1.0 + 0.5 * alp * alp);
-            const double smax = classical ? sn : sn / (1.0 + sn);  // s = v * smax
-            if (lane < 16) {  // the theta bound's 16 points, one per lane
-                if (classical) theta_bound_low_eta_point(nh, alp, lane, tb_s[lane], tb_f[lane]);
-                else theta_bound_point(nh, alp, eta, lane, tb_s[lane], tb_f[lane]);
-            }
-            __syncthreads();
-            if (lane == 0) theta_bound_running_min(tb_f);
-            __syncthreads();
-            // A candidate t: (v, theta) from counter 16 + 2t, its acceptance draw from 17 + 2t.
-            // f is the rate at the candidate (not needed if theta is out of bounds).
-            auto candidate = [&](uint32_t t, double& v, double& th, double& uz, double& fc) -> bool {
-                Rng gc = base;
-                gc.c0 = 16u + 2u * t;
-                rng_uniform2(gc, v, th);
-                th *= PI_D / 2.0;
-                if (th > theta_bound_at(tb_s, tb_f, classical ? v : smax * v)) return false;
-                uz = rng_uniform(gc);
-                fc = classical ? lp_cl_ddpr(D, alp, v, th) : lp_ddpr(D, alp, eta, smax * v, th);
-                return true;
-            };
-            // round 0: the ceiling's sample points on lanes [0, m), the first 64 - m
-            // candidates on the other lanes (their rates do not depend on the ceiling)
-            const int m = classical ? lp_cl_ceiling_count(nh, alp) : lp_ceiling_count(nh, alp, eta);
-            double f = classical ? NAN : -INFINITY;
-            double v = 0.0, th = 0.0, uz = 0.0, fc = 0.0;
-            bool in_bound = false;
-            for (int k = lane; k < m; k += 64) {
-                double sk, tk;
-                if (classical) {
-                    lp_cl_ceiling_arg(nh, alp, k, sk, tk);
-                    f = fmax(f, lp_cl_ddpr(D, alp, sk, tk));
-                } else {
-                    lp_ceiling_arg(nh, alp, eta, k, sk, tk);
-                    f = fmax(f, lp_ddpr(D, alp, eta, sk, tk));
-                }
-            }
-            if (lane >= m) in_bound = candidate(uint32_t(lane - m), v, th, uz, fc);
-            const double mx = (classical ? 1.1 : 1.15) * wave_fmax(f);
-            uint32_t t0 = uint32_t(m < 64 ? 64 - m : 0);  // candidates consumed
-            int first = -1;
-            double v_acc = 0.0, th_acc = 0.0;
-            {
-                const uint64_t mask = __ballot(lane >= m && in_bound && mx * uz < fc);
-                if (mask) {
-                    first = __ffsll((long long)mask) - 1;
-                    v_acc = __shfl(v, first, 64);
-                    th_acc = __shfl(th, first, 64);
-                    trials += uint32_t(first - m) + 1;
-                }
-            }
-            uint32_t width = 8;
-            while (first < 0) {
-                bool acc = false;
-                const uint32_t t = t0 + uint32_t(lane);
-                if (uint32_t(lane) < width && t < cap && candidate(t, v, th, uz, fc)) acc = mx * uz < fc;
-                const uint64_t mask = __ballot(acc);
-                if (mask) {
-                    first = __ffsll((long long)mask) - 1;
-                    v_acc = __shfl(v, first, 64);
-                    th_acc = __shfl(th, first, 64);
-                    trials += t0 + uint32_t(first) + 1;
-                } else {
-                    t0 += width;
-                    width = width < 64 ? 4 * width : 64;
-                    if (t0 >= cap) {  // as the sequential sampler: give up and keep candidate `cap`
-                        flags |= PTG_FLAG_REJECT_CAP;
-                        Rng gc = base;
-                        gc.c0 = 16u + 2u * cap;
-                        rng_uniform2(gc, v_acc, th_acc);
-                        th_acc *= PI_D / 2.0;
-                        trials += cap;
-                        first = 0;
-                    }
-                }
-            }
-            Rng gq = base;  // counter 1: the quadrant
-            gq.c0 = 1u;
-            const int quadrant = int(rng_uniform(gq) * 4.0);
-            theta = th_acc;
-            if (quadrant == 1) theta = PI_D - theta;
-            else if (quadrant == 2) theta = PI_D + theta;
-            else if (quadrant == 3) theta = 2.0 * PI_D - theta;
-            s = smax * v_acc;
-            if (lane == 0) sv = classical ? lp_cl_stokes(D, alp, v_acc, theta) : lp_stokes(D, alp, eta, s, theta);
+            lp_sample_wave<classical>(nh, alp, eta, base, tb_s, tb_f, s, theta, sv, trials, flags);
         }
         if (lane == 0) lma_emit_finish<classical>(A, M, i, k0, a, eta, ok, nh, s, theta, sv, n_emit, flags);
     }
@@ -531,18 +579,155 @@ __global__ __launch_bounds__(256) void k_photon_step_lma(const TransportArgs* __
     if (flags) atomicOr(A->flags, flags);
 }
 
-// One pair per lane (pair_creation/mod.rs:47-107 with lp/mod.rs:424-442 and
+// (s, theta) of an LP pair at harmonic D.n by rejection on the double-differential
+// rate (lp/mod.rs:261-301), one decay per wave: all 64 lanes call this with the
+// same arguments (a block is one wave). Every trial costs a J(n, x, y) evaluation,
+// whose quadrature grows with n (thousands of nodes at the harmonics of a0 ~ 10-20),
+// so the trials run in parallel: the theta bound's 16 points on lanes 0-15, the
+// ceiling's 50 points along theta = 0 on lanes 0-49 while lanes 50-63 take the
+// first 14 candidates, then rounds of 8, 32, 64, ... candidates; the sample is the
+// first accepted candidate in candidate order. Candidate t draws (s, theta) from
+// counter 16 + 2t of `base` and its acceptance from 17 + 2t, so the sample does
+// not depend on the round sizes; counter 1 gives the side of s = 1/2 and the
+// quadrant of theta. From harmonic PTG_LMA_SHARED_N up the wave shares each J
+// evaluation instead and takes the ceiling's points and the candidates one after
+// another (the same candidates and accepted one, up to the rounding of the
+// partial sums). After 2^20 candidates the sampler gives up, keeps candidate
+// 2^20 and sets `capped`. `trials` counts the candidates consumed.
+__device__ __noinline__ void bw_sample_wave(const DBessel& D, double a, double eta, double sv1, double sv2, const Rng& base,
+                                      double* tb_s, double* tb_f, double& s_out, double& theta_out, uint32_t& trials,
+                                      bool& capped) {
+    const int lane = threadIdx.x;
+    const uint32_t cap = 1u << 20;
+    constexpr int m = BW_CEILING_POINTS;
+    double s_min, s_max;
+    bw_s_bounds(D.n, a, eta, s_min, s_max);
+    __syncthreads();  // the previous decay's bound is no longer read
+    if (lane < 16) bw_theta_bound_point(D.n, a, eta, lane, tb_s[lane], tb_f[lane]);
+    __syncthreads();
+    if (lane == 0) bw_theta_bound_running_min(tb_f);
+    __syncthreads();
+    auto candidate = [&](uint32_t t, double& sc, double& thc, double& z, double& fc) -> bool {
+        Rng g = base;
+        g.c0 = 16u + 2u * t;
+        double u1, u2;
+        rng_uniform2(g, u1, u2);
+        sc = s_min + (s_max - s_min) * u1;
+        thc = 0.5 * PI_D * u2;
+        if (thc > bw_theta_at(tb_s, tb_f, sc)) return false;
+        z = rng_uniform(g);
+        fc = bw_fully_resolved(D, a, eta, sc, thc, sv1, sv2);
+        return true;
+    };
+    double lower, upper;
+    bw_ceiling_range(D.n, a, eta, lower, upper);
+    int first = -1;
+    double s = 0.0, theta = 0.0;
+    if (D.n >= PTG_LMA_SHARED_N) {  // the wave shares each J evaluation; every lane holds the same values
+        const DBesselWave W{D.n};
+        double f = -1.0e300;
+        for (int k = 0; k < m; ++k) f = fmax(f, bw_fully_resolved(W, a, eta, lower + (upper - lower) * double(k) / double(m), 0.0, sv1, sv2));
+        const double mx = 1.2 * f;
+        for (uint32_t t = 0; first < 0; ++t) {
+            Rng g = base;
+            g.c0 = 16u + 2u * t;
+            double u1, u2;
+            rng_uniform2(g, u1, u2);
+            const double st = s_min + (s_max - s_min) * u1, tht = 0.5 * PI_D * u2;
+            if (t >= cap) {  // give up and keep candidate `cap`
+                capped = true;
+                s = st;
+                theta = tht;
+                trials += cap;
+                first = 0;
+                break;
+            }
+            if (tht > bw_theta_at(tb_s, tb_f, st)) continue;
+            const double zt = rng_uniform(g);
+            if (mx * zt < bw_fully_resolved(W, a, eta, st, tht, sv1, sv2)) {
+                s = st;
+                theta = tht;
+                trials += t + 1;
+                first = 0;
+            }
+        }
+    }
+    double f = -1.0e300, sc = 0.0, thc = 0.0, z = 0.0, fc = 0.0;
+    bool in_bound = false;
+    if (first < 0) {
+        if (lane < m) f = bw_fully_resolved(D, a, eta, lower + (upper - lower) * double(lane) / double(m), 0.0, sv1, sv2);
+        else in_bound = candidate(uint32_t(lane - m), sc, thc, z, fc);
+    }
+    const double mx = 1.2 * wave_fmax(f);
+    uint32_t t0 = 64u - uint32_t(m);  // candidates consumed
+    if (first < 0) {
+        const uint64_t mask = __ballot(lane >= m && in_bound && mx * z < fc);
+        if (mask) {
+            first = __ffsll((long long)mask) - 1;
+            s = __shfl(sc, first, 64);
+            theta = __shfl(thc, first, 64);
+            trials += uint32_t(first - m) + 1;
+        }
+    }
+    uint32_t width = 8;
+    while (first < 0) {
+        bool acc = false;
+        const uint32_t t = t0 + uint32_t(lane);
+        if (uint32_t(lane) < width && t < cap && candidate(t, sc, thc, z, fc)) acc = mx * z < fc;
+        const uint64_t mask = __ballot(acc);
+        if (mask) {
+            first = __ffsll((long long)mask) - 1;
+            s = __shfl(sc, first, 64);
+            theta = __shfl(thc, first, 64);
+            trials += t0 + uint32_t(first) + 1;
+        } else {
+            t0 += width;
+            width = width < 64 ? 4 * width : 64;
+            if (t0 >= cap) {  // give up and keep candidate `cap`
+                capped = true;
+                Rng g = base;
+                g.c0 = 16u + 2u * cap;
+                double u1, u2;
+                rng_uniform2(g, u1, u2);
+                s = s_min + (s_max - s_min) * u1;
+                theta = 0.5 * PI_D * u2;
+                trials += cap;
+                first = 0;
+            }
+        }
+    }
+    Rng g = base;
+    g.c0 = 1u;
+    double u1, u2;
+    rng_uniform2(g, u1, u2);
+    if (int(u1 * 2.0) == 0) s = 1.0 - s;
+    const int quadrant = int(u2 * 4.0);
+    if (quadrant == 1) theta = PI_D - theta;
+    else if (quadrant == 2) theta = PI_D + theta;
+    else if (quadrant == 3) theta = 2.0 * PI_D - theta;
+    s_out = s;
+    theta_out = theta;
+}
+
+// Pair events (pair_creation/mod.rs:47-107 with lp/mod.rs:424-442 and
 // plane_wave.rs:305-316): harmonic from the CDF tables, (s, theta) by rejection
 // on the double-differential rate, the positron built in the zero-momentum
 // frame; the electron and positron go to the pair queue, the photon is kept
-// (decayed) or re-queued with its reduced weight.
+// (decayed) or re-queued with its reduced weight. LP runs one event per wave
+// (every lane computes the event's scalars, the lanes share the rejection
+// sampling, lane 0 writes); CP, whose sampler reads tabulated Bessel functions,
+// one event per lane.
 template <bool CIRC>
-__global__ __launch_bounds__(64) void k_pair_event_lma(const TransportArgs* __restrict__ A, LmaArgs M) {
-    const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n_lanes = gridDim.x * blockDim.x;
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PTG_LMA_WAVE_WPE))) void k_pair_event_lma(
+    const TransportArgs* __restrict__ A, LmaArgs M) {
+    const uint32_t lane = threadIdx.x;
+    const uint64_t e0 = CIRC ? uint64_t(blockIdx.x) * 64u + lane : uint64_t(blockIdx.x);
+    const uint64_t stride = CIRC ? uint64_t(gridDim.x) * 64u : uint64_t(gridDim.x);
+    const bool writer = CIRC || lane == 0;
+    __shared__ double tb_s[16], tb_f[16];  // LP: the decay's theta bound, one copy per wave
     uint64_t n_pairs = 0, trials = 0;
     uint32_t flags = 0;
-    for (uint64_t e = lane; e < M.n_ev; e += n_lanes) {
+    for (uint64_t e = e0; e < M.n_ev; e += stride) {
         const PBuf& q = M.ev_in;
         uint32_t i = M.ev_list ? M.ev_list[e] : uint32_t(e);
         // a batch of collisions: the photon's own collision's laser
@@ -575,7 +760,7 @@ __global__ __launch_bounds__(64) void k_pair_event_lma(const TransportArgs* __re
             uint32_t tr = 0;
             bool capped = false;
             if (CIRC) cpbw_sample_at(M.lcfa, M.cp, n, a, eta, sl.q, sl.u, sl.v, rng, s, cphi, tr, capped);
-            else bw_sample_at(dbessel_of(n), alp, eta, sl.q, sl.u, rng, s, cphi, tr, capped);
+            else bw_sample_wave(dbessel_of(n), alp, eta, sl.q, sl.u, rng, tb_s, tb_f, s, cphi, tr, capped);
             trials += tr;
             if (capped) flags |= PTG_FLAG_REJECT_CAP;
             // positron in the zero-momentum frame (pair_creation/mod.rs:63-104)
@@ -599,9 +784,13 @@ __global__ __launch_bounds__(64) void k_pair_event_lma(const TransportArgs* __re
             if (!(fabs(s - s_new) / s < 1.0e-3)) flags |= PTG_FLAG_SANITY;
             d4 qe{k.t + jn * k0 - qp.t, k.x - qp.x, k.y - qp.y, k.z + jn * k0 - qp.z};
             // electron and positron (src/main.rs:220-243): weight frac * w, half the absorbed energy each
-            unsigned long long j = wave_reserve(M.pairs_ctr, 2u);
-            uint64_t id = A->id_base + wave_reserve(A->id_ctr, 2u);
-            if (j + 1 < M.pairs_cap) {
+            unsigned long long j = 0;
+            uint64_t id = 0;
+            if (writer) {
+                j = wave_reserve(M.pairs_ctr, 2u);
+                id = A->id_base + wave_reserve(A->id_ctr, 2u);
+            }
+            if (writer && j + 1 < M.pairs_cap) {
                 sv4 unpol{1.0, 0.0, 0.0, 0.0};
                 d4 u0{1.0, 0.0, 0.0, 0.0};
                 uint64_t base = rng_child(q.stream[i], nev);
@@ -619,6 +808,7 @@ __global__ __launch_bounds__(64) void k_pair_event_lma(const TransportArgs* __re
         if (made && !decayed) weight = weight * (1.0 - frac);
         if (A->opt.rotate_stokes_pars && A->opt.pol_resolved) pol = pnew;
         icount += prob;
+        if (!writer) continue;
         if (decayed) {
             finalize_photon_from(*A, q, i, r, k, pol, weight, icount, meta, true);
         } else {
@@ -634,6 +824,7 @@ __global__ __launch_bounds__(64) void k_pair_event_lma(const TransportArgs* __re
             o.nev[j] = nev + 1;
         }
     }
+    if (!writer) return;
     wave_add(A->stats + STAT_PAIRS, n_pairs);
     wave_add(A->stats + STAT_TRIALS, trials);
     if (flags) atomicOr(A->flags, flags);
@@ -1070,7 +1261,9 @@ __global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, ui
         // mode x[3]: 0 the kernels' evaluation (saddle point or quadrature), 1 the
         // reference's recurrence (the host checks 3 <= n, |x| <= n sqrt 2,
         // 0 <= y <= n / 2 and sizes m.scratch for it), 2 the quadrature alone,
-        // 3 the reference's evaluation (saddle point or recurrence)
+        // 3 the reference's evaluation (saddle point or recurrence), 4 the
+        // quadrature shared by a wave (k_eval_dbessel_wave)
+        if (x[3] == 4.0) break;
         const int nn = int(x[0]);
         const DBesselRec R = dbessel_rec_at(nn, m.scratch + i, n);
         double j[5];
@@ -1079,20 +1272,6 @@ __global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, ui
         else if (x[3] == 3.0) dbessel_eval(R, x[1], x[2], j);
         else dbessel_eval(dbessel_of(nn), x[1], x[2], j);
         for (int q = 0; q < 5; ++q) y[q] = j[q];
-        break;
-    }
-    case PTG_FN_LP_SAMPLE: {
-        // (a, eta, stream, fixed_n) -> (n, s, theta, S1, S2, trials)
-        Rng rng = rng_make(key, uint64_t(x[2]));
-        int nh = x[3] > 0.0 ? int(x[3]) : lp_harmonic(tables + lma_pack::CDF, sc, x[0], x[1], rng_uniform(rng));
-        if (nh < 1) { y[0] = 0.0; y[5] = -2.0; break; }  // outside the table
-        int nn;
-        double s, th;
-        sv4 pol;
-        uint32_t tr = 0;
-        bool capped = false;
-        lp_sample(tables + lma_pack::CDF, sc, x[0], x[1], rng, nn, s, th, pol, tr, capped, nh);
-        y[0] = nn; y[1] = s; y[2] = th; y[3] = pol.q; y[4] = pol.u; y[5] = tr;
         break;
     }
     case PTG_FN_BW_RATE: {
@@ -1106,19 +1285,6 @@ __global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, ui
         break;
     }
     case PTG_FN_BW_HARMONIC: y[0] = bw_harmonic(bw, bsc, x[0], x[1], x[2], x[3]); break;
-    case PTG_FN_BW_SAMPLE: {
-        // trials = -2: outside the tables
-        Rng rng = rng_make(key, uint64_t(x[4]));
-        int nh = x[5] > 0.0 ? int(x[5]) : bw_harmonic(bw, bsc, x[0], x[1], x[2], rng_uniform(rng));
-        if (nh < 1) { y[0] = 0.0; y[3] = -2.0; break; }
-        const DBessel D = dbessel_of(nh);
-        double s, th;
-        uint32_t tr = 0;
-        bool capped = false;
-        bw_sample_at(D, x[0], x[1], x[2], x[3], rng, s, th, tr, capped);
-        y[0] = nh; y[1] = s; y[2] = th; y[3] = tr;
-        break;
-    }
     case PTG_FN_LP_CL_RATE: {
         double f = 0.0;
         y[1] = lp_cl_rate(tables, sc, x[0], x[1], f) ? 1.0 : 0.0;
@@ -1129,19 +1295,6 @@ __global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, ui
     case PTG_FN_LP_CL_CEILING: {
         const int nn = int(x[0]);  // the host checks 1 <= n
         y[0] = lp_cl_ceiling(dbessel_of(nn), x[1]);
-        break;
-    }
-    case PTG_FN_LP_CL_SAMPLE: {
-        // (a, eta, stream, fixed_n) -> (n, s, theta, S1, S2, trials); trials -2: outside the table
-        Rng rng = rng_make(key, uint64_t(x[2]));
-        int nh = x[3] > 0.0 ? int(x[3]) : lp_cl_harmonic(tables, sc, x[0], rng_uniform(rng));
-        if (nh < 1) { y[0] = 0.0; y[5] = -2.0; break; }
-        double s, th;
-        sv4 pol;
-        uint32_t tr = 0;
-        bool capped = false;
-        lp_cl_sample(dbessel_of(nh), x[0], x[1], rng, s, th, pol, tr, capped);
-        y[0] = nh; y[1] = s; y[2] = th; y[3] = pol.q; y[4] = pol.u; y[5] = tr;
         break;
     }
     case PTG_FN_CPBW_RATE: {
@@ -1220,9 +1373,92 @@ __global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, ui
     default: break;
     }
 }
+// The Breit-Wheeler sampler probe, one row per wave through the pair kernel's
+// sampler: (a, eta, S1, S2, stream, fixed_n) -> (n, s, theta, trials), trials
+// -2 outside the tables.
+__global__ __launch_bounds__(64) void k_eval_bw_sample(const double* in, double* out, uint32_t n, uint32_t in_w, uint32_t out_w,
+                                                       LmaArgs m, uint64_t key) {
+    __shared__ double tb_s[16], tb_f[16];
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const double* x = in + uint64_t(i) * in_w;
+        double* y = out + uint64_t(i) * out_w;
+        Rng rng = rng_make(key, uint64_t(x[4]));
+        const int nh = x[5] > 0.0 ? int(x[5]) : bw_harmonic(m.bw, m.bsc, x[0], x[1], x[2], rng_uniform(rng));
+        if (nh < 1) {
+            if (threadIdx.x == 0) { y[0] = 0.0; y[3] = -2.0; }
+            continue;
+        }
+        double s, th;
+        uint32_t tr = 0;
+        bool capped = false;
+        bw_sample_wave(dbessel_of(nh), x[0], x[1], x[2], x[3], rng, tb_s, tb_f, s, th, tr, capped);
+        if (threadIdx.x == 0) { y[0] = nh; y[1] = s; y[2] = th; y[3] = tr; }
+    }
+}
+
+// The LP emission sampler probes (quantum and classical), one row per wave
+// through the emission kernel's sampler: (a, eta, stream, fixed_n) -> (n, s,
+// theta, S1, S2, trials), trials -2 outside the table.
+template <bool classical>
+__global__ __launch_bounds__(64) void k_eval_lp_sample(const double* in, double* out, uint32_t n, uint32_t in_w, uint32_t out_w,
+                                                       LmaArgs m, uint64_t key) {
+    __shared__ double tb_s[16], tb_f[16];
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const double* x = in + uint64_t(i) * in_w;
+        double* y = out + uint64_t(i) * out_w;
+        Rng rng = rng_make(key, uint64_t(x[2]));
+        int nh = int(x[3]);
+        if (!(x[3] > 0.0)) {
+            const double fr = rng_uniform(rng);
+            nh = classical ? lp_cl_harmonic(m.tables, m.sc, x[0], fr) : lp_harmonic(m.tables + lma_pack::CDF, m.sc, x[0], x[1], fr);
+        }
+        if (nh < 1) {
+            if (threadIdx.x == 0) { y[0] = 0.0; y[5] = -2.0; }
+            continue;
+        }
+        double s, th;
+        sv4 pol;
+        uint64_t tr = 0;
+        uint32_t flags = 0;
+        lp_sample_wave<classical>(nh, x[0], x[1], rng, tb_s, tb_f, s, th, pol, tr, flags);
+        if (threadIdx.x == 0) { y[0] = nh; y[1] = s; y[2] = th; y[3] = pol.q; y[4] = pol.u; y[5] = double(tr); }
+    }
+}
+
+// The J(n, x, y) probe in mode 4: the quadrature shared by a wave
+// (dbessel_quadrature_wave), one row per wave; every lane's result is checked
+// against lane 0's and a disagreement returns NaN.
+__global__ __launch_bounds__(64) void k_eval_dbessel_wave(const double* in, double* out, uint32_t n, uint32_t in_w,
+                                                          uint32_t out_w) {
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const double* x = in + uint64_t(i) * in_w;
+        double* y = out + uint64_t(i) * out_w;
+        if (x[3] != 4.0) continue;
+        double j[5];
+        dbessel_quadrature_wave(int(x[0]), x[1], x[2], j);
+        bool same = true;
+        for (int q = 0; q < 5; ++q) same = same && __shfl(j[q], 0, 64) == j[q];
+        same = __all(same);
+        if (threadIdx.x == 0)
+            for (int q = 0; q < 5; ++q) y[q] = same ? j[q] : NAN;
+    }
+}
+
 void launch_eval_lma(int fn, const double* in, double* out, uint32_t n, uint32_t in_w, uint32_t out_w, const LmaArgs& m, uint64_t key,
                      hipStream_t s) {
-    if (n) k_eval_lma<<<(n + 63) / 64, 64, 0, s>>>(fn, in, out, n, in_w, out_w, m, key);
+    if (!n) return;
+    const uint32_t rows = std::min<uint32_t>(n, 65536u);
+    if (fn == PTG_FN_BW_SAMPLE) {
+        k_eval_bw_sample<<<rows, 64, 0, s>>>(in, out, n, in_w, out_w, m, key);
+        return;
+    }
+    if (fn == PTG_FN_LP_SAMPLE || fn == PTG_FN_LP_CL_SAMPLE) {
+        if (fn == PTG_FN_LP_SAMPLE) k_eval_lp_sample<false><<<rows, 64, 0, s>>>(in, out, n, in_w, out_w, m, key);
+        else k_eval_lp_sample<true><<<rows, 64, 0, s>>>(in, out, n, in_w, out_w, m, key);
+        return;
+    }
+    k_eval_lma<<<(n + 63) / 64, 64, 0, s>>>(fn, in, out, n, in_w, out_w, m, key);
+    if (fn == PTG_FN_DOUBLE_BESSEL) k_eval_dbessel_wave<<<rows, 64, 0, s>>>(in, out, n, in_w, out_w);
 }
 
 }  // namespace ptg

@@ -37,12 +37,12 @@ double G16N[16], G16W[16], G32N[32], G32W[32];
 struct Partial {
     int n;
     double a, eta;
-    DBessel D;
+    DBesselRec D;
     BwThetaBound tb;
     std::vector<double> ws;
     Partial(int n_, double a_, double eta_) : n(n_), a(a_), eta(eta_) {
         ws.assign(size_t(dbessel_len(n)) * 5, 0.0);
-        D = dbessel_at(n, ws.data(), 1);
+        D = dbessel_rec_at(n, ws.data(), 1);
         tb = bw_theta_bound(n, a, eta);
     }
     Cx at(double s, double theta) {

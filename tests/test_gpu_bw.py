@@ -68,6 +68,8 @@ def test_bw_harmonic_matches_oracle(engine, orc):
 def test_bw_sampler_statistics(engine, orc, a, eta, sv1, sv2, fixed_n):
     from scipy.stats import ks_2samp
     from ptarmigan_amd import Fn
+    # at a = 6, eta = 0.3 the harmonics run from 127 to ~660, about half from 200
+    # up, where the wave shares each J(n, x, y) evaluation
     say("bw sample", a, eta, sv1, sv2, fixed_n)
     m = 20_000
     inp = np.stack([np.full(m, a), np.full(m, eta), np.full(m, sv1), np.full(m, sv2), np.arange(m, dtype=float),

@@ -205,9 +205,15 @@ def _against_fp64(ref, ref_primaries, got, got_primaries):
     """Photon yield per primary, spectra and Stokes means of a run against the
     plain FP64 run (independent seeds)."""
     import math
+    import numpy as np
     y_ref, y_got = ref[2] / ref_primaries, got[2] / got_primaries
+
+    def cdf_gap(hr, hg):  # the largest gap between the two runs' distribution functions (histogram edges)
+        return float(np.max(np.abs(np.cumsum(hr) / ref[2] - np.cumsum(hg) / got[2])))
     return {"photon_yield_err": y_got / y_ref - 1.0, "photon_yield_err_sigma": math.sqrt(1.0 / ref[2] + 1.0 / got[2]),
             "energy_chi2_p": _chi2_hist(ref[0], got[0]), "angle_chi2_p": _chi2_hist(ref[1], got[1]),
+            "energy_cdf_max_gap": cdf_gap(ref[0], got[0]), "angle_cdf_max_gap": cdf_gap(ref[1], got[1]),
+            "cdf_gap_1sigma": math.sqrt(0.25 / ref[2] + 0.25 / got[2]),
             "stokes_mean_diff": [b[0] - a[0] for a, b in zip(ref[3], got[3])],
             "stokes_mean_diff_in_sigma": [(b[0] - a[0]) / math.hypot(a[1], b[1]) if math.hypot(a[1], b[1]) > 0.0
                                           else None for a, b in zip(ref[3], got[3])]}
@@ -268,8 +274,7 @@ def config4_compare(eng, sync, n, fp64_fraction):
     """The learned sampler over its domain: for LP and CP, 64 plane-wave
     collisions with a0 uniform on [0.5, 20] and energy uniform on [2, 20] GeV
     (n / 2 primaries per polarization), surrogate + FP32 against plain FP64 on
-    a subsample of the same collisions (fp64_fraction of the primaries;
-    particle-steps/s is intensive)."""
+    the same collisions (fp64_fraction of the primaries, 1 by default)."""
     from ptarmigan_amd import surrogate, workloads
     from ptarmigan_amd.engine import Polarization
     out = {}
@@ -284,7 +289,7 @@ def config4_compare(eng, sync, n, fp64_fraction):
             opts.surrogate = mode != "fp64"
             opts.precision = 1 if mode != "fp64" else 0
             c, el, sps = _timed(eng, opts, seed, sync)
-            res[mode] = (m, c, el, sps, _final_photons(eng, 2.05e4, 0.05))
+            res[mode] = (m, c, el, sps, _final_photons(eng, 2.05e4, 0.01))
         (ms, cs, ts, ss, ps), (mf, cf, tf, sf, pf) = res["surrogate_fp32"], res["fp64"]
         out["lp" if pol == Polarization.LINEAR else "cp"] = {
             "surrogate_primaries": ms, "fp64_primaries": mf, "surrogate_fp32_steps_per_s": ss, "fp64_steps_per_s": sf,
@@ -292,7 +297,7 @@ def config4_compare(eng, sync, n, fp64_fraction):
             "surrogate_events": cs["surrogate_events"], "exact_fallbacks": cs["surrogate_fallbacks"],
             "model_valid": meta["valid"], **_against_fp64(pf, mf, ps, ms)}
     out["workload"] = ("64 PlaneWave collisions per polarization (LP, CP), a0 U[0.5, 20], 2-20 GeV, 0.8um 16-cycle "
-                       "cos^2, LMA Compton; FP64 timed on a subsample of the same collisions")
+                       "cos^2, LMA Compton; both legs on the same collisions")
     return out
 
 
@@ -314,7 +319,7 @@ def config5_compare(eng, sync, scale, fp64_scale):
         if tune is None:
             tune = eng.autotune(opts, seed=5, max_primaries=200_000)
         c, el, sps = _timed(eng, opts, seed, sync)
-        res[mode] = (sum(b.n for b in beams), c, el, sps, _final_photons(eng, 1.8e4, 0.05))
+        res[mode] = (sum(b.n for b in beams), c, el, sps, _final_photons(eng, 1.8e4, 0.005))
     (ms, cs, ts, ss, ps), (mf, cf, tf, sf, pf) = res["surrogate_fp32"], res["fp64"]
     return {"workload": "64 PlaneWave LP collisions in one batch, a0 0.5-20, 5-17.5 GeV, Compton + Breit-Wheeler",
             "surrogate_primaries": ms, "fp64_primaries": mf, "surrogate_fp32_steps_per_s": ss, "fp64_steps_per_s": sf,
@@ -334,10 +339,10 @@ def main():
                     help="skip the comparison workloads (plane-wave LMA vs CPU, mixed precision, learned sampler, scan)")
     ap.add_argument("--config4-primaries", type=int, default=10_000_000,
                     help="learned-sampler domain workload: primaries over both polarizations")
-    ap.add_argument("--config4-fp64-fraction", type=float, default=0.02,
+    ap.add_argument("--config4-fp64-fraction", type=float, default=1.0,
                     help="fraction of those primaries the plain FP64 leg is timed on")
     ap.add_argument("--config5-scale", type=float, default=1.0, help="scan: primaries per collision times this")
-    ap.add_argument("--config5-fp64-scale", type=float, default=0.02, help="scan: the plain FP64 leg's scale")
+    ap.add_argument("--config5-fp64-scale", type=float, default=1.0, help="scan: the plain FP64 leg's scale")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = the host CPUs one GPU is given (16), or fewer")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="lcfa",

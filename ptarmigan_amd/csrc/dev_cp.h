@@ -131,24 +131,22 @@ PTG_HD inline double cp_ddpr(TP tb, int n, double a, double eta, double v) {
                                       (2.0 * (j[1] * j[1]) - j[2] * j[2] - j[0] * j[0]));
 }
 PTG_HD __forceinline__ int cp_n_switch(double a) { return int(32.3 * (1.0 + 0.476 * pow(a, 1.56))); }
-// 1.2 x the largest of 32 samples of cp_ddpr (cp/mod.rs:42-75); g16 = 16 Gauss nodes
-PTG_HD inline double cp_ceiling(TP tb, const double* g16, int n, double a, double eta) {
+// 1.2 x the largest of 32 samples of cp_ddpr (cp/mod.rs:42-75); g16 = 16 Gauss nodes.
+// Sample k < 16 lies below v_opt, k >= 16 above (point k - 16).
+PTG_HD inline double cp_ceiling_arg(const double* g16, int n, double a, double eta, int k) {
     double sn = 2.0 * double(n) * eta / (1.0 + a * a);
-    double lower = 0.0, upper = 0.0;
     bool direct = sn < 2.0 || n < cp_n_switch(a);
     double v_opt = (1.0 + sn) / (2.0 + sn);
+    const int i = k & 15;
+    if (direct) return k < 16 ? double(i) * v_opt / 16.0 : v_opt + double(i) * (1.0 - v_opt) / 16.0;
+    double u = k < 16 ? 0.5 * (1.0 + sn) * (g16[i] + 1.0) : (1.0 + sn) + 0.5 * 2.0 * (1.0 + sn) * (g16[i] + 1.0);
+    return u / (1.0 + u);
+}
+PTG_HD inline double cp_ceiling(TP tb, const double* g16, int n, double a, double eta) {
+    double lower = 0.0, upper = 0.0;
     for (int i = 0; i < 16; ++i) {
-        double vl, vu;
-        if (direct) {
-            vl = double(i) * v_opt / 16.0;
-            vu = v_opt + double(i) * (1.0 - v_opt) / 16.0;
-        } else {
-            double ul = 0.5 * (1.0 + sn) * (g16[i] + 1.0);
-            double uu = (1.0 + sn) + 0.5 * 2.0 * (1.0 + sn) * (g16[i] + 1.0);
-            vl = ul / (1.0 + ul);
-            vu = uu / (1.0 + uu);
-        }
-        double fl = cp_ddpr(tb, n, a, eta, vl), fu = cp_ddpr(tb, n, a, eta, vu);
+        double fl = cp_ddpr(tb, n, a, eta, cp_ceiling_arg(g16, n, a, eta, i));
+        double fu = cp_ddpr(tb, n, a, eta, cp_ceiling_arg(g16, n, a, eta, 16 + i));
         lower = i == 0 ? fl : fmax(lower, fl);
         upper = i == 0 ? fu : fmax(upper, fu);
     }
@@ -193,21 +191,25 @@ PTG_HD inline double cp_integrated_spectrum_low_eta(TP tb, const double* g32n, c
     return lower + upper;
 }
 // 1.1 x the largest of at most 8 samples of the classical spectrum (cp/classical.rs:46-69)
-PTG_HD inline double cp_classical_ceiling(TP tb, int n, double a) {
+// the classical ceiling's sample points in v (cp/classical.rs:92-126): how many, and the k-th
+PTG_HD inline int cp_classical_ceiling_count(int n, double a) {
     double n_switch = round(3.0 * (1.0 + 0.54 * pow(a, 2.75)));
-    double mx;
-    if (n == 1) {
-        mx = cp_spectrum_low_eta(tb, n, a, 0.99);
-    } else if (a < 0.15 || n > int(n_switch)) {
-        mx = cp_spectrum_low_eta(tb, n, a, 0.5);
-    } else {
-        double b = a < 0.5 ? 2.3 * a : 0.8 * exp(-(a - 0.5) / 3.5) + 0.35;
-        double v = 1.0 - 0.5 * pow(log(double(n)), b) / pow(log(n_switch), b);
-        mx = 0.0;
-        for (int i = -4; i < 4; ++i) {
-            double f = cp_spectrum_low_eta(tb, n, a, fmin(v + 0.1 * double(i) / 4.0, 1.0));
-            mx = i == -4 ? f : fmax(mx, f);
-        }
+    return (n == 1 || a < 0.15 || n > int(n_switch)) ? 1 : 8;
+}
+PTG_HD inline double cp_classical_ceiling_arg(int n, double a, int k) {
+    double n_switch = round(3.0 * (1.0 + 0.54 * pow(a, 2.75)));
+    if (n == 1) return 0.99;
+    if (a < 0.15 || n > int(n_switch)) return 0.5;
+    double b = a < 0.5 ? 2.3 * a : 0.8 * exp(-(a - 0.5) / 3.5) + 0.35;
+    double v = 1.0 - 0.5 * pow(log(double(n)), b) / pow(log(n_switch), b);
+    return fmin(v + 0.1 * double(k - 4) / 4.0, 1.0);
+}
+PTG_HD inline double cp_classical_ceiling(TP tb, int n, double a) {
+    const int m = cp_classical_ceiling_count(n, a);
+    double mx = 0.0;
+    for (int k = 0; k < m; ++k) {
+        double f = cp_spectrum_low_eta(tb, n, a, cp_classical_ceiling_arg(n, a, k));
+        mx = k == 0 ? f : fmax(mx, f);
     }
     return 1.1 * mx;
 }

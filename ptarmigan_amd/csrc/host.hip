@@ -900,8 +900,8 @@ static int lma_event_pass(ptg_ctx* c, LmaArgs m, bool pairs, size_t n_ev, size_t
     m.ev_in = in ? in->v : c->evq.v;
     m.ev_list = list;
     m.n_ev = n_ev;
-    // LP emission and LP pair creation run one event per wave, the others one per lane
-    const bool per_wave = pairs ? !(m.mode == LMA_CP || m.mode == LMA_CP_CLASSICAL) : (m.mode == LMA_LP || m.mode == LMA_LP_CLASSICAL);
+    // emission and LP pair creation run one event per wave, CP pair creation one per lane
+    const bool per_wave = !pairs || !(m.mode == LMA_CP || m.mode == LMA_CP_CLASSICAL);
     size_t lanes = per_wave ? std::min<size_t>(max_lanes, n_ev * 64) : std::min<size_t>(max_lanes, ((n_ev + 63) / 64) * 64);
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     if (pairs) launch_pair_event_lma(c->d_args, m, int(lanes), c->stream);

@@ -217,58 +217,123 @@ __device__ inline void lma_emit_finish(const TransportArgs* __restrict__ A, cons
     o.nev[j] = nev + 1;
 }
 
-// One CP emission per lane (nonlinear_compton::generate, mod.rs:58-138, with
-// cp::sample, cp/mod.rs:181-234, or cp::classical::sample, cp/classical.rs:128-178).
-template <int MODE>
-__global__ __launch_bounds__(64) void k_event_lma(const TransportArgs* __restrict__ A, LmaArgs M) {
-    constexpr bool classical = MODE == LMA_CP_CLASSICAL;
-    const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n_lanes = gridDim.x * blockDim.x;
-    uint64_t n_emit = 0, trials = 0;
-    uint32_t flags = 0;
-    for (uint64_t e = lane; e < M.n_ev; e += n_lanes) {
-        const PBuf& q = M.ev_in;
-        uint32_t i = M.ev_list ? M.ev_list[e] : uint32_t(e);
-        d4 u{q.d[P_UT][i], q.d[P_UX][i], q.d[P_UY][i], q.d[P_UZ][i]};
-        double k0 = M.ev_k0[i];
-        uint32_t nev = q.nev[i];
-        Rng rng = rng_make(A->key, q.stream[i]);
-        rng.c1 = nev | EVENT_BLOCK;
-        double a = sqrt(u.t * u.t - u.x * u.x - u.y * u.y - u.z * u.z - 1.0);  // rms a
-        double eta = k0 * (u.t - u.z);
-        int n = 0;
-        double s, cphi;
-        sv4 sv;
-        uint32_t tr = 0;
-        bool capped = false;
-        int n_cap_ok = 1;
-        int nh = classical ? cp_classical_harmonic(M.cp, M.csc, a, rng_uniform(rng)) : cp_harmonic(M.cp, M.csc, a, eta, rng_uniform(rng));
-        if (nh < 1) {
-            flags |= PTG_FLAG_TABLE_RANGE;
-            n_cap_ok = 0;
-        } else {
-            n = nh;
-            cp_sample_at(M.lcfa, M.cp, nh, a, eta, classical, rng, s, cphi, sv, tr, capped);
-        }
-        trials += tr;
-        if (capped) flags |= PTG_FLAG_REJECT_CAP;
-        lma_emit_finish<classical>(A, M, i, k0, a, eta, n_cap_ok != 0, n, s, cphi, sv, n_emit, flags);
-    }
-    wave_add(A->stats + STAT_EMISSIONS, n_emit);
-    wave_add(A->stats + STAT_TRIALS, trials);
-    if (flags) atomicOr(A->flags, flags);
-}
-
 __device__ __forceinline__ double wave_fmax(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
     return v;
 }
 
+// (s, theta) and the Stokes parameters of a CP emission at harmonic n
+// (cp/mod.rs:181-234, cp/classical.rs:128-178), one emission per wave: all 64
+// lanes call this with the same arguments and return the same values. The
+// ceiling's points (32, or the classical 1 or 8) take lanes [0, m) while lanes
+// [m, 64) take the first candidates, then rounds of 64; the first accepted
+// candidate in candidate order is the sample. Candidate t draws (v, u) from
+// counter 16 + t of `base`, counter 1 gives theta. Every lane of a wave works on
+// the same harmonic, where one lane per emission would wait for the longest
+// Bessel recurrence among 64 different harmonics.
+template <bool classical>
+__device__ __noinline__ void cp_sample_wave(TP tb, const double* P, int n, double a, double eta, const Rng& base, double& s,
+                                            double& theta, sv4& sv, uint64_t& trials, uint32_t& flags) {
+    const int lane = threadIdx.x;
+    const uint32_t cap = 1u << 20;
+    const int m = classical ? cp_classical_ceiling_count(n, a) : 32;
+    auto rate = [&](double v) { return classical ? cp_spectrum_low_eta(tb, n, a, v) : cp_ddpr(tb, n, a, eta, v); };
+    auto candidate = [&](uint32_t t, double& v, double& u) {
+        Rng g = base;
+        g.c0 = 16u + t;
+        rng_uniform2(g, v, u);
+    };
+    double f = -INFINITY, v = 0.0, u = 0.0, fc = 0.0;
+    if (lane < m) {
+        f = rate(classical ? cp_classical_ceiling_arg(n, a, lane) : cp_ceiling_arg(P + cp_pack::G16N, n, a, eta, lane));
+    } else {
+        candidate(uint32_t(lane - m), v, u);
+        fc = rate(v);
+    }
+    const double mx = (classical ? 1.1 : 1.2) * wave_fmax(f);
+    int first = -1;
+    double v_acc = 0.0;
+    uint32_t t0 = uint32_t(64 - m);  // candidates consumed
+    {
+        const uint64_t mask = __ballot(lane >= m && u <= fc / mx);
+        if (mask) {
+            first = __ffsll((long long)mask) - 1;
+            v_acc = __shfl(v, first, 64);
+            trials += uint32_t(first - m) + 1;
+        }
+    }
+    while (first < 0) {
+        const uint32_t t = t0 + uint32_t(lane);
+        bool acc = false;
+        if (t < cap) {
+            candidate(t, v, u);
+            acc = u <= rate(v) / mx;
+        }
+        const uint64_t mask = __ballot(acc);
+        if (mask) {
+            first = __ffsll((long long)mask) - 1;
+            v_acc = __shfl(v, first, 64);
+            trials += t0 + uint32_t(first) + 1;
+        } else {
+            t0 += 64;
+            if (t0 >= cap) {  // as the sequential sampler: give up and keep candidate `cap`
+                flags |= PTG_FLAG_REJECT_CAP;
+                candidate(cap, v_acc, u);
+                trials += cap;
+                first = 0;
+            }
+        }
+    }
+    const double sn = 2.0 * double(n) * eta / (1.0 + a * a);
+    s = classical ? v_acc * sn : v_acc * (sn / (1.0 + sn));
+    Rng g = base;
+    g.c0 = 1u;
+    theta = 2.0 * PI_D * rng_uniform(g);
+    double st[4];
+    cp_stokes(tb, n, a, s, sn, v_acc, theta, classical, st);
+    sv = {st[0], st[1], st[2], st[3]};
+}
+
+// One CP emission per wave (nonlinear_compton::generate, mod.rs:58-138, with
+// cp_sample_wave); lane 0 builds the photon.
+template <int MODE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_event_lma(const TransportArgs* __restrict__ A,
+                                                                                           LmaArgs M) {
+    constexpr bool classical = MODE == LMA_CP_CLASSICAL;
+    const int lane = threadIdx.x;
+    uint64_t n_emit = 0, trials = 0;
+    uint32_t flags = 0;
+    for (uint64_t e = blockIdx.x; e < M.n_ev; e += gridDim.x) {
+        const PBuf& q = M.ev_in;
+        const uint32_t i = M.ev_list ? M.ev_list[e] : uint32_t(e);
+        d4 u{q.d[P_UT][i], q.d[P_UX][i], q.d[P_UY][i], q.d[P_UZ][i]};
+        const double k0 = M.ev_k0[i];
+        Rng base = rng_make(A->key, q.stream[i]);
+        base.c1 = q.nev[i] | EVENT_BLOCK;
+        const double a = sqrt(u.t * u.t - u.x * u.x - u.y * u.y - u.z * u.z - 1.0);  // rms a
+        const double eta = k0 * (u.t - u.z);
+        Rng g = base;  // counter 0: the harmonic
+        const double fr = rng_uniform(g);
+        const int nh = classical ? cp_classical_harmonic(M.cp, M.csc, a, fr) : cp_harmonic(M.cp, M.csc, a, eta, fr);
+        const bool ok = nh >= 1;
+        if (!ok) flags |= PTG_FLAG_TABLE_RANGE;
+        double s = 0.0, cphi = 0.0;
+        sv4 sv{1.0, 0.0, 0.0, 0.0};
+        if (ok) cp_sample_wave<classical>(M.lcfa, M.cp, nh, a, eta, base, s, cphi, sv, trials, flags);
+        if (lane == 0) lma_emit_finish<classical>(A, M, i, k0, a, eta, ok, ok ? nh : 0, s, cphi, sv, n_emit, flags);
+    }
+    if (lane == 0) {
+        wave_add(A->stats + STAT_EMISSIONS, n_emit);
+        wave_add(A->stats + STAT_TRIALS, trials);
+        if (flags) atomicOr(A->flags, flags);
+    }
+}
+
 // One emission per wave, for LP emission (quantum or classical), where every
 // trial costs a J(n, x, y) evaluation: the 64 lanes evaluate the ceiling's
 // sample points together (lp/mod.rs:53-140, lp/classical.rs:43-110), then
-// draw (s, theta) candidates in rounds of 8, 32, 64, ... and take the first
+// draw (s, theta) candidates in rounds of 64 (one per lane) and take the first
 // accepted one in candidate order. Candidates are independent draws (candidate
 // t uses counters 16 + 2t, 17 + 2t of the event's RNG block), so the first
 // accepted one has the distribution of sequential rejection sampling; an
@@ -285,6 +350,7 @@ __device__ __forceinline__ double wave_fmax(double v) {
 #ifndef PTG_LMA_SHARED_N
 #define PTG_LMA_SHARED_N 200
 #endif
+
 // (s, theta) and the Stokes parameters of an LP emission at harmonic nh, for
 // one emission per wave: all 64 lanes call this with the same arguments (a
 // block is one wave) and return the same values. Counter 1 of `base` gives the
@@ -387,11 +453,10 @@ __device__ __noinline__ void lp_sample_wave(int nh, double alp, double eta, cons
             trials += uint32_t(first - m) + 1;
         }
     }
-    uint32_t width = 8;
     while (first < 0) {
         bool acc = false;
         const uint32_t t = t0 + uint32_t(lane);
-        if (uint32_t(lane) < width && t < cap && candidate(t, v, th, uz, fc)) acc = mx * uz < fc;
+        if (t < cap && candidate(t, v, th, uz, fc)) acc = mx * uz < fc;
         const uint64_t mask = __ballot(acc);
         if (mask) {
             first = __ffsll((long long)mask) - 1;
@@ -399,8 +464,7 @@ __device__ __noinline__ void lp_sample_wave(int nh, double alp, double eta, cons
             th_acc = __shfl(th, first, 64);
             trials += t0 + uint32_t(first) + 1;
         } else {
-            t0 += width;
-            width = width < 64 ? 4 * width : 64;
+            t0 += 64;
             if (t0 >= cap) {  // as the sequential sampler: give up and keep candidate `cap`
                 flags |= PTG_FLAG_REJECT_CAP;
                 Rng gc = base;
@@ -585,7 +649,7 @@ __global__ __launch_bounds__(256) void k_photon_step_lma(const TransportArgs* __
 // whose quadrature grows with n (thousands of nodes at the harmonics of a0 ~ 10-20),
 // so the trials run in parallel: the theta bound's 16 points on lanes 0-15, the
 // ceiling's 50 points along theta = 0 on lanes 0-49 while lanes 50-63 take the
-// first 14 candidates, then rounds of 8, 32, 64, ... candidates; the sample is the
+// first 14 candidates, then rounds of 64 candidates; the sample is the
 // first accepted candidate in candidate order. Candidate t draws (s, theta) from
 // counter 16 + 2t of `base` and its acceptance from 17 + 2t, so the sample does
 // not depend on the round sizes; counter 1 gives the side of s = 1/2 and the
@@ -669,11 +733,10 @@ __device__ __noinline__ void bw_sample_wave(const DBessel& D, double a, double e
             trials += uint32_t(first - m) + 1;
         }
     }
-    uint32_t width = 8;
     while (first < 0) {
         bool acc = false;
         const uint32_t t = t0 + uint32_t(lane);
-        if (uint32_t(lane) < width && t < cap && candidate(t, sc, thc, z, fc)) acc = mx * z < fc;
+        if (t < cap && candidate(t, sc, thc, z, fc)) acc = mx * z < fc;
         const uint64_t mask = __ballot(acc);
         if (mask) {
             first = __ffsll((long long)mask) - 1;
@@ -681,8 +744,7 @@ __device__ __noinline__ void bw_sample_wave(const DBessel& D, double a, double e
             theta = __shfl(thc, first, 64);
             trials += t0 + uint32_t(first) + 1;
         } else {
-            t0 += width;
-            width = width < 64 ? 4 * width : 64;
+            t0 += 64;
             if (t0 >= cap) {  // give up and keep candidate `cap`
                 capped = true;
                 Rng g = base;
@@ -1355,21 +1417,6 @@ __global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, ui
         }
         break;
     }
-    case PTG_FN_CP_SAMPLE: {
-        // (a, eta, stream, fixed_n, classical) -> (n, s, theta, S1, S2, S3, trials); trials = -2: outside the table
-        Rng rng = rng_make(key, uint64_t(x[2]));
-        bool cl = x[4] != 0.0;
-        int nh = x[3] > 0.0 ? int(x[3]) : (cl ? cp_classical_harmonic(m.cp, m.csc, x[0], rng_uniform(rng))
-                                                 : cp_harmonic(m.cp, m.csc, x[0], x[1], rng_uniform(rng)));
-        if (nh < 1) { y[0] = 0.0; y[6] = -2.0; break; }
-        double s, th;
-        sv4 sv;
-        uint32_t tr = 0;
-        bool capped = false;
-        cp_sample_at(m.lcfa, m.cp, nh, x[0], x[1], cl, rng, s, th, sv, tr, capped);
-        y[0] = nh; y[1] = s; y[2] = th; y[3] = sv.q; y[4] = sv.u; y[5] = sv.v; y[6] = capped ? -1.0 : double(tr);
-        break;
-    }
     default: break;
     }
 }
@@ -1425,6 +1472,38 @@ __global__ __launch_bounds__(64) void k_eval_lp_sample(const double* in, double*
     }
 }
 
+// The CP emission sampler probe, one row per wave through the emission
+// kernel's sampler: (a, eta, stream, fixed_n, classical) -> (n, s, theta, S1,
+// S2, S3, trials), trials -2 outside the table, -1 at the trial cap.
+__global__ __launch_bounds__(64) void k_eval_cp_sample(const double* in, double* out, uint32_t n, uint32_t in_w, uint32_t out_w,
+                                                       LmaArgs m, uint64_t key) {
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const double* x = in + uint64_t(i) * in_w;
+        double* y = out + uint64_t(i) * out_w;
+        Rng rng = rng_make(key, uint64_t(x[2]));
+        const bool cl = x[4] != 0.0;
+        int nh = int(x[3]);
+        if (!(x[3] > 0.0)) {
+            const double fr = rng_uniform(rng);
+            nh = cl ? cp_classical_harmonic(m.cp, m.csc, x[0], fr) : cp_harmonic(m.cp, m.csc, x[0], x[1], fr);
+        }
+        if (nh < 1) {
+            if (threadIdx.x == 0) { y[0] = 0.0; y[6] = -2.0; }
+            continue;
+        }
+        double s, th;
+        sv4 sv;
+        uint64_t tr = 0;
+        uint32_t flags = 0;
+        if (cl) cp_sample_wave<true>(m.lcfa, m.cp, nh, x[0], x[1], rng, s, th, sv, tr, flags);
+        else cp_sample_wave<false>(m.lcfa, m.cp, nh, x[0], x[1], rng, s, th, sv, tr, flags);
+        if (threadIdx.x == 0) {
+            y[0] = nh; y[1] = s; y[2] = th; y[3] = sv.q; y[4] = sv.u; y[5] = sv.v;
+            y[6] = (flags & PTG_FLAG_REJECT_CAP) ? -1.0 : double(tr);
+        }
+    }
+}
+
 // The J(n, x, y) probe in mode 4: the quadrature shared by a wave
 // (dbessel_quadrature_wave), one row per wave; every lane's result is checked
 // against lane 0's and a disagreement returns NaN.
@@ -1450,6 +1529,10 @@ void launch_eval_lma(int fn, const double* in, double* out, uint32_t n, uint32_t
     const uint32_t rows = std::min<uint32_t>(n, 65536u);
     if (fn == PTG_FN_BW_SAMPLE) {
         k_eval_bw_sample<<<rows, 64, 0, s>>>(in, out, n, in_w, out_w, m, key);
+        return;
+    }
+    if (fn == PTG_FN_CP_SAMPLE) {
+        k_eval_cp_sample<<<rows, 64, 0, s>>>(in, out, n, in_w, out_w, m, key);
         return;
     }
     if (fn == PTG_FN_LP_SAMPLE || fn == PTG_FN_LP_CL_SAMPLE) {

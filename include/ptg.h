@@ -130,7 +130,7 @@ typedef struct {
     uint64_t generations;
     uint64_t rejection_trials;
     uint32_t error_flags;
-    uint32_t reserved;
+    uint32_t graph_stalls;      /* captured sequences whose events outgrew a queue and finished on the host's launches */
     double kernel_ms;           /* device time of all transport kernels (HIP events on the context stream) */
     double lepton_ms;           /* ... of the electron/positron kernels */
     double photon_ms;           /* ... of the photon kernels */
@@ -140,6 +140,7 @@ typedef struct {
                                           form (not included in photon_steps) */
     uint64_t surrogate_events;  /* emissions sampled by the surrogate */
     uint64_t surrogate_fallbacks; /* emissions outside its domain, sampled exactly */
+    uint64_t graph_launches;    /* captured sequences of generations launched (PTG_LAUNCH_GRAPH) */
 } ptg_counters;
 
 /* Beam description, mirroring BeamBuilder (src/particle/builder.rs:173-219) for
@@ -367,6 +368,20 @@ typedef struct {
     double cand_ms[16];              /* lepton step kernel time of each candidate */
 } ptg_tune_result;
 int ptg_autotune(ptg_ctx* ctx, const ptg_collide_options* opt, uint64_t seed, uint64_t max_primaries, ptg_tune_result* out);
+
+/* How ptg_run drives the generations of the LMA loop, which replaces the
+ * per-primary stack loop of collide() (src/main.rs:161-289) and has no
+ * counterpart there. PTG_LAUNCH_GRAPH (the default): after the first
+ * generation of each lepton phase, and from the first generation of each
+ * photon phase, generations run in sequences of 16 captured in a HIP graph and
+ * launched at once; each launch reads its counts from device memory and the
+ * host reads the counters once per sequence (ptg_counters.graph_launches).
+ * PTG_LAUNCH_EAGER: the host reads the counts after every launch, and
+ * lepton_ms / photon_ms time every kernel. The particles transported do not
+ * depend on the mode (the order of the final-state records and the ids given
+ * to secondaries follow the device's scheduling in both). */
+typedef enum { PTG_LAUNCH_EAGER = 0, PTG_LAUNCH_GRAPH = 1 } ptg_launch_mode;
+int ptg_set_launch_mode(ptg_ctx* ctx, int mode);
 
 /* ------------------------------------------------------------------ learned sampler
  * A surrogate for the sampling of LMA nonlinear Compton emission. No counterpart

@@ -45,10 +45,10 @@ class Counters(C.Structure):
     _fields_ = [("lepton_steps", C.c_uint64), ("photon_steps", C.c_uint64), ("emissions", C.c_uint64),
                 ("pair_events", C.c_uint64), ("n_secondaries", C.c_uint64), ("n_intermediates", C.c_uint64),
                 ("generations", C.c_uint64), ("rejection_trials", C.c_uint64), ("error_flags", C.c_uint32),
-                ("reserved", C.c_uint32), ("kernel_ms", C.c_double), ("lepton_ms", C.c_double),
+                ("graph_stalls", C.c_uint32), ("kernel_ms", C.c_double), ("lepton_ms", C.c_double),
                 ("photon_ms", C.c_double), ("lepton_launches", C.c_uint64), ("photon_launches", C.c_uint64),
                 ("photon_steps_closed_form", C.c_uint64), ("surrogate_events", C.c_uint64),
-                ("surrogate_fallbacks", C.c_uint64)]
+                ("surrogate_fallbacks", C.c_uint64), ("graph_launches", C.c_uint64)]
 
 
 MLP_NPARAM = 38304
@@ -102,12 +102,15 @@ class StatsSpec(C.Structure):
 
 STATS_N = 9
 
+# ptg_launch_mode
+LAUNCH_EAGER, LAUNCH_GRAPH = 0, 1
+
 # every extern "C" symbol include/ptg.h declares
 EXPORTS = ("ptg_create", "ptg_destroy", "ptg_last_error", "ptg_set_laser", "ptg_field_info", "ptg_build_beam",
            "ptg_load_primaries", "ptg_run", "ptg_collide", "ptg_fetch", "ptg_push", "ptg_histogram", "ptg_eval",
            "ptg_surrogate_train", "ptg_surrogate_load", "ptg_mlp_probe", "ptg_set_batch", "ptg_autotune",
            "ptg_fetch_showers", "ptg_histogram_2d", "ptg_stats", "ptg_finish",
-           "ptg_set_beam_spectrum", "ptg_histogram_noalloc", "ptg_laser_energy")
+           "ptg_set_beam_spectrum", "ptg_histogram_noalloc", "ptg_laser_energy", "ptg_set_launch_mode")
 
 _lib = None
 
@@ -152,6 +155,7 @@ def load():
     L.ptg_finish.argtypes = [vp, C.c_int, d, C.c_int, d, d, P(d)]
     L.ptg_set_beam_spectrum.argtypes = [vp, C.c_int, P(d), C.c_size_t, d, d]
     L.ptg_autotune.argtypes = [vp, P(CollideOptions), C.c_uint64, C.c_uint64, P(TuneResult)]
+    L.ptg_set_launch_mode.argtypes = [vp, C.c_int]
     for f in ("ptg_set_laser", "ptg_field_info", "ptg_build_beam", "ptg_load_primaries", "ptg_run", "ptg_collide",
               "ptg_fetch", "ptg_push", "ptg_histogram", "ptg_eval"):
         getattr(L, f).restype = C.c_int

@@ -507,27 +507,6 @@ PTG_HD inline double cpbw_probability(TP tb, const double* P, const CpScalars& c
           (sv.v * (1.0 - prob_avg) - delta) / (1.0 - prob)};
     return prob;
 }
-// harmonic by explicit summation of partial rates until the cumulative rate passes
-// frac x the total (:327-365); n_max - 1 with `fell_back` set if it never does,
-// -1 if the total is not available
-PTG_HD inline int cpbw_harmonic(TP tb, const double* P, const CpScalars& c, double a, double eta, double sv3, double frac,
-                                bool& fell_back) {
-    int lo, hi;
-    cpbw_sum_limits(a, eta, lo, hi);
-    double f[2];
-    if (!cpbw_total(tb, P, c, a, eta, f)) return -1;
-    double target = 0.5 * (f[0] * (1.0 + sv3) + f[1] * (1.0 - sv3)) * frac;
-    double cumsum = 0.0;
-    fell_back = false;
-    for (int k = lo; k <= hi; ++k) {
-        cplx pr = cpbw_partial_rate(tb, P + cp_pack::G32N, P + cp_pack::G32W, k, a, eta);
-        cumsum += 0.5 * ((1.0 + sv3) * pr.re + (1.0 - sv3) * pr.im);
-        if (cumsum > target) return k;
-    }
-    fell_back = true;
-    return hi - 1;
-}
-
 // ---------------------------------------------------------------- sampler
 // (s, theta, Stokes) of one emission at harmonic n (cp/mod.rs:181-234,
 // cp/classical.rs:128-178): v = s / s_max (quantum) or s / s_n (classical) by

@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <dlfcn.h>
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -78,10 +79,6 @@ void free_buf(DevBuf& b) {
     if (b.base) (void)hipFree(b.base);
     b = DevBuf{};
 }
-
-// Counter block layout (unsigned long long)
-enum { C_WORK = 0, C_NEXT, C_OUT, C_INTER, C_ID, C_FLAGS, C_EV, C_LNEXT, C_PH, C_PAIRS, C_SROWS, C_SFB, C_SNOW, C_STATS,
-       C_N = C_STATS + STAT_N };
 
 // Device scratch of ptg_histogram_noalloc: words [0, nbins) are the bins and
 // [nbins] the total weight (doubles), [PTG_NOALLOC_MAX_BINS + 1] and [+ 2] the
@@ -167,6 +164,14 @@ struct ptg_ctx {
     DevBuf defq;
     double* d_def_extra = nullptr;  // kappa^0, sqrt(<a^2>)
     size_t def_extra_cap = 0, n_def = 0;
+    // LMA generations: PTG_LAUNCH_GRAPH runs them in captured sequences (launch
+    // arguments of the two queue parities in d_gargs[2]); PTG_LAUNCH_EAGER reads
+    // the counts back after every launch
+    int launch_mode = PTG_LAUNCH_GRAPH;
+    TransportArgs* d_gargs = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    uint64_t graph_launches = 0;  // of the current ptg_run
+    uint32_t graph_stalls = 0;
 };
 
 namespace {
@@ -517,6 +522,7 @@ int ptg_create(ptg_ctx** out, const char* table_path, int device) {
     HIPCHK(c, scopy(c, c->d_lcfa, pack.data(), pack.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMalloc(&c->d_ctr, C_N * sizeof(unsigned long long)));
     HIPCHK(c, hipMalloc(&c->d_args, sizeof(TransportArgs)));
+    HIPCHK(c, hipMalloc(&c->d_gargs, 2 * sizeof(TransportArgs)));
     HIPCHK(c, hipEventCreate(&c->ev0));
     HIPCHK(c, hipEventCreate(&c->ev1));
     *out = c;
@@ -551,6 +557,8 @@ void ptg_destroy(ptg_ctx* c) {
     if (c->d_shower_count) (void)hipFree(c->d_shower_count);
     if (c->d_ctr) (void)hipFree(c->d_ctr);
     if (c->d_args) (void)hipFree(c->d_args);
+    if (c->d_gargs) (void)hipFree(c->d_gargs);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->d_hist) (void)hipFree(c->d_hist);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -900,9 +908,8 @@ static int lma_event_pass(ptg_ctx* c, LmaArgs m, bool pairs, size_t n_ev, size_t
     m.ev_in = in ? in->v : c->evq.v;
     m.ev_list = list;
     m.n_ev = n_ev;
-    // emission and LP pair creation run one event per wave, CP pair creation one per lane
-    const bool per_wave = !pairs || !(m.mode == LMA_CP || m.mode == LMA_CP_CLASSICAL);
-    size_t lanes = per_wave ? std::min<size_t>(max_lanes, n_ev * 64) : std::min<size_t>(max_lanes, ((n_ev + 63) / 64) * 64);
+    // one event per wave
+    const size_t lanes = std::min<size_t>(max_lanes, n_ev * 64);
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     if (pairs) launch_pair_event_lma(c->d_args, m, int(lanes), c->stream);
     else launch_event_lma(c->d_args, m, int(lanes), c->stream);
@@ -927,14 +934,9 @@ static void surr_box_args(const double* box, SurrArgs& S) {
     S.le_use = std::log(std::max(box[2], box[5]));
 }
 
-// Emission events of one generation through the learned sampler: inputs for
-// the events inside its domain, the network's forward pass, the photons; the
-// other events (outside the domain, or an LP harmonic beyond the workspace) go
-// to the exact kernels.
-static int lma_surrogate_pass(ptg_ctx* c, LmaArgs m, size_t n_ev, size_t max_lanes, double* ms, uint64_t* launches,
-                              uint64_t* n_fallback) {
-    const int p = m.mode == LMA_CP ? 1 : 0;
-    MlpNet& net = c->surr[p];
+// Index lists of the learned sampler's pass (rows, deferred, exact now, order)
+// for n_ev events, and its bucket counts.
+static int ensure_surr_lists(ptg_ctx* c, size_t n_ev) {
     if (c->surr_list_cap < n_ev) {
         if (c->d_surr_list) (void)hipFree(c->d_surr_list);
         if (c->d_surr_fb) (void)hipFree(c->d_surr_fb);
@@ -950,6 +952,39 @@ static int lma_surrogate_pass(ptg_ctx* c, LmaArgs m, size_t n_ev, size_t max_lan
         c->surr_list_cap = cap;
     }
     if (!c->d_surr_bucket) HIPCHK(c, hipMalloc(&c->d_surr_bucket, 64 * sizeof(unsigned int)));
+    return PTG_OK;
+}
+
+// Room for `need` events in the deferred queue (its records, kappa^0 and
+// sqrt(<a^2>)), keeping the n_def held.
+static int ensure_deferred(ptg_ctx* c, size_t need) {
+    if (c->defq.cap < need) HIPCHK(c, grow_buf(c->defq, need + need / 2 + 64, c->n_def, c->stream));
+    if (c->def_extra_cap < c->defq.cap) {
+        double* ne = nullptr;
+        HIPCHK(c, hipMalloc(&ne, 2 * c->defq.cap * sizeof(double)));
+        if (c->n_def) {
+            HIPCHK(c, hipMemcpyAsync(ne, c->d_def_extra, c->n_def * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(ne + c->defq.cap, c->d_def_extra + c->def_extra_cap, c->n_def * sizeof(double),
+                                     hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+        if (c->d_def_extra) (void)hipFree(c->d_def_extra);
+        c->d_def_extra = ne;
+        c->def_extra_cap = c->defq.cap;
+    }
+    return PTG_OK;
+}
+
+// Emission events of one generation through the learned sampler: inputs for
+// the events inside its domain, the network's forward pass, the photons; the
+// other events (outside the domain, or an LP harmonic beyond the workspace) go
+// to the exact kernels.
+static int lma_surrogate_pass(ptg_ctx* c, LmaArgs m, size_t n_ev, size_t max_lanes, double* ms, uint64_t* launches,
+                              uint64_t* n_fallback) {
+    const int p = m.mode == LMA_CP ? 1 : 0;
+    MlpNet& net = c->surr[p];
+    int rc = ensure_surr_lists(c, n_ev);
+    if (rc) return rc;
     const size_t rows_cap = (n_ev + 31) / 32 * 32;
     if (rows_cap > size_t(1) << 30) return fail(c, PTG_E_OOM, "too many emission events for the surrogate's buffers");
     HIPCHK(c, net.reserve(int(rows_cap)));
@@ -996,25 +1031,12 @@ static int lma_surrogate_pass(ptg_ctx* c, LmaArgs m, size_t n_ev, size_t max_lan
     const size_t n_fb = size_t(cnt[1]);
     *n_fallback += n_fb + n_now;
     if (n_now > 0) {  // below the domain's amplitude: the exact kernels, in this generation
-        int rc = lma_event_pass(c, m, false, n_now, max_lanes, ms, launches, c->d_surr_now, &c->evq);
+        rc = lma_event_pass(c, m, false, n_now, max_lanes, ms, launches, c->d_surr_now, &c->evq);
         if (rc) return rc;
     }
     if (n_fb > 0) {  // held for one exact pass when the lepton phase drains
-        const size_t need = c->n_def + n_fb;
-        if (c->defq.cap < need) HIPCHK(c, grow_buf(c->defq, need + need / 2 + 64, c->n_def, c->stream));
-        if (c->def_extra_cap < c->defq.cap) {
-            double* ne = nullptr;
-            HIPCHK(c, hipMalloc(&ne, 2 * c->defq.cap * sizeof(double)));
-            if (c->n_def) {
-                HIPCHK(c, hipMemcpyAsync(ne, c->d_def_extra, c->n_def * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-                HIPCHK(c, hipMemcpyAsync(ne + c->defq.cap, c->d_def_extra + c->def_extra_cap, c->n_def * sizeof(double),
-                                         hipMemcpyDeviceToDevice, c->stream));
-                HIPCHK(c, hipStreamSynchronize(c->stream));
-            }
-            if (c->d_def_extra) (void)hipFree(c->d_def_extra);
-            c->d_def_extra = ne;
-            c->def_extra_cap = c->defq.cap;
-        }
+        rc = ensure_deferred(c, c->n_def + n_fb);
+        if (rc) return rc;
         launch_gather_events(c->evq.v, m.ev_k0, m.ev_a, c->d_surr_fb, n_fb, c->defq.v, c->d_def_extra,
                              c->d_def_extra + c->def_extra_cap, c->n_def, c->stream);
         HIPCHK(c, hipGetLastError());
@@ -1028,6 +1050,353 @@ static unsigned long long read_ctr(ptg_ctx* c, int which) {
     unsigned long long v = 0;
     if (scopy(c, &v, c->d_ctr + which, sizeof(v), hipMemcpyDeviceToHost) != hipSuccess) return ~0ull;
     return v;
+}
+
+// ---------------------------------------------------------------- captured generations
+// PTG_LAUNCH_GRAPH: after its first generation, each phase of the LMA loop runs
+// as sequences of generations captured into a HIP graph and launched at once.
+// Every launch in a sequence reads its counts from device memory (the start of
+// each generation, k_gen_begin, moves the count the last one queued into the
+// launch arguments), so the host reads the counter block once per sequence
+// instead of after every launch. A generation that could overrun an output
+// queue stalls the rest of its sequence; the host grows the queues and resumes.
+
+// Generations per captured sequence (even: the queues alternate between two
+// buffers, so that every sequence ends on the buffer it started from).
+constexpr int GRAPH_GENS = 16;
+
+// Diagnostic override (tests of the stall path): PTG_GRAPH_QUEUE_ROOM = records
+// of room a sequence may use in each output queue beyond what it holds, however
+// large the queue; 0 or unset: the queues' capacities.
+static size_t graph_room_override() {
+    const char* e = std::getenv("PTG_GRAPH_QUEUE_ROOM");
+    if (!e) return 0;
+    const long long v = std::atoll(e);
+    return v > 0 ? size_t(v) : 0;
+}
+static unsigned long long room_cap(size_t cap, size_t held, size_t over) {
+    return over ? std::min<unsigned long long>(cap, held + over) : cap;
+}
+
+// Captures what `record` launches on the context stream, makes it executable
+// and launches it between ev0 and ev1.
+static int graph_launch(ptg_ctx* c, const std::function<void()>& record) {
+    hipGraph_t g = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    record();
+    const hipError_t le = hipGetLastError();
+    const hipError_t ee = hipStreamEndCapture(c->stream, &g);
+    if (le != hipSuccess || ee != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(c, PTG_E_HIP, std::string("graph capture: ") + hipGetErrorString(le != hipSuccess ? le : ee));
+    }
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+        c->gexec = nullptr;
+        return fail(c, PTG_E_HIP, std::string("graph instantiation: ") + hipGetErrorString(ie));
+    }
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    ++c->graph_launches;
+    return PTG_OK;
+}
+
+// Reads the counter block after a sequence; returns the device time.
+static int graph_finish(ptg_ctx* c, unsigned long long* now, double* ms) {
+    HIPCHK(c, hipMemcpyAsync(now, c->d_ctr, C_N * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float t = 0.0f;
+    (void)hipEventElapsedTime(&t, c->ev0, c->ev1);
+    *ms += t;
+    return PTG_OK;
+}
+
+// Sets the counters a sequence starts from: its input count (C_LNEXT), the
+// deferred events held (C_NDEF), no stall, no generation run yet (C_GENS).
+static int graph_prime(ptg_ctx* c, size_t n_in) {
+    unsigned long long v[2] = {n_in, 0ull};
+    HIPCHK(c, hipMemcpyAsync(c->d_ctr + C_LNEXT, &v[0], sizeof(v[0]), hipMemcpyHostToDevice, c->stream));
+    unsigned long long nd[1] = {c->n_def};
+    HIPCHK(c, hipMemcpyAsync(c->d_ctr + C_NDEF, nd, sizeof(nd), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_ctr + C_STALL, &v[1], sizeof(v[1]), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_ctr + C_GENS, &v[1], sizeof(v[1]), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PTG_OK;
+}
+
+// Lepton generations of the LMA loop from a captured sequence, the input in
+// lq[s] (n_cur electrons and positrons). On return *cur and *n_left hold what
+// is left and *li the buffer the next generation writes. A generation whose
+// events found too little room in the photon or final-state queues (the
+// sequence stalled there) has its events sampled here, eagerly, after the
+// queues have grown.
+static int lepton_sequence(ptg_ctx* c, TransportArgs a, LmaArgs m, bool track, bool use_surr, int s, size_t n_cur,
+                           size_t max_lanes, double* ms, uint64_t* launches, uint64_t* gens, uint64_t* surr_fb, DevBuf** cur,
+                           size_t* n_left, int* li) {
+    // every lepton of the phase leaves the field once (a final state); each event
+    // adds a photon (queued, or a final state) and re-queues its lepton. Room for
+    // the photons of four generations in which every lepton emits; the device
+    // checks the room before each generation's events
+    const size_t budget = 4 * n_cur + 4096;
+    HIPCHK(c, grow_buf(c->out, c->n_out + n_cur + (track ? 0 : budget) + 1024, c->n_out, c->stream));
+    if (c->evq.cap < n_cur) HIPCHK(c, alloc_buf(c->evq, n_cur + n_cur / 4));
+    int rc = ensure_ev_extra(c, c->evq.cap);
+    if (rc) return rc;
+    if (c->lq[s ^ 1].cap < n_cur) HIPCHK(c, alloc_buf(c->lq[s ^ 1], n_cur + n_cur / 4));
+    const unsigned long long n_ph = read_ctr(c, track ? C_PH : C_NEXT);
+    HIPCHK(c, grow_buf(c->phq, size_t(n_ph) + (track ? budget : 0) + 1024, size_t(n_ph), c->stream));
+
+    const int p = m.mode == LMA_CP ? 1 : 0;
+    MlpNet& net = c->surr[p];
+    SurrArgs S{};
+    const size_t rows_cap = (n_cur + 31) / 32 * 32;
+    if (use_surr) {
+        if (rows_cap > size_t(1) << 30) return fail(c, PTG_E_OOM, "too many emission events for the surrogate's buffers");
+        rc = ensure_surr_lists(c, n_cur);
+        if (rc) return rc;
+        HIPCHK(c, net.reserve(int(rows_cap)));
+        rc = ensure_deferred(c, c->n_def + n_cur);
+        if (rc) return rc;
+        surr_box_args(c->surr_box[p], S);
+        S.x = net.X;
+        S.out = net.O;
+        S.list = c->d_surr_list;
+        S.n_rows = c->d_ctr + C_SROWS;
+        S.fb = c->d_surr_fb;
+        S.n_fb = c->d_ctr + C_SFB;
+        S.now = c->d_surr_now;
+        S.n_now = c->d_ctr + C_SNOW;
+        S.n_ev = n_cur;
+        S.n_ev_dev = c->d_ctr + C_EVRUN;
+        S.perm = c->d_surr_perm;
+        S.bucket = c->d_surr_bucket;
+    }
+
+    // launch arguments of the two parities: slot k reads lq[k] and writes lq[k ^ 1]
+    TransportArgs ta[2];
+    LmaArgs mk[2];
+    for (int k = 0; k < 2; ++k) {
+        ta[k] = a;
+        ta[k].in = c->lq[k].v;
+        ta[k].n_in = 0;
+        ta[k].out = c->out.v;
+        ta[k].inter = c->inter.v;
+        ta[k].next = c->phq.v;
+        ta[k].next_ctr = c->d_ctr + (track ? C_PH : C_NEXT);
+        ta[k].next_cap = c->phq.cap;
+        mk[k] = m;
+        mk[k].ev = c->evq.v;
+        mk[k].ev_cap = c->evq.cap;
+        mk[k].ev_k0 = c->d_ev_extra;
+        mk[k].ev_a = c->d_ev_extra + c->ev_extra_cap;
+        mk[k].ph = c->phq.v;
+        mk[k].ph_cap = c->phq.cap;
+        mk[k].ev_in = c->evq.v;
+        mk[k].ev_list = nullptr;
+        mk[k].n_ev = 0;
+        mk[k].n_ev_dev = c->d_ctr + C_EVRUN;
+        mk[k].next = c->lq[k ^ 1].v;
+        mk[k].next_ctr = c->d_ctr + C_LNEXT;
+    }
+    HIPCHK(c, scopy(c, c->d_gargs, ta, sizeof(ta), hipMemcpyHostToDevice));
+    // per event: the photon when it is a final state, and room kept for the final
+    // state of the lepton it re-queues (the step kernel writes final states unchecked)
+    const size_t over = graph_room_override();
+    EventRoom room{};
+    room.out_cap = room_cap(c->out.cap, c->n_out + n_cur, over);
+    room.out_per = track ? 1u : 2u;
+    room.ph_cap = room_cap(c->phq.cap, size_t(n_ph), over);
+    room.ph_per = track ? 1u : 0u;
+    room.ph_slot = track ? C_PH : C_NEXT;
+    room.pairs_cap = ~0ull;
+    room.inter_cap = ~0ull;
+    rc = graph_prime(c, n_cur);
+    if (rc) return rc;
+
+    const size_t sb = size_t(c->step_block);
+    const int grid = int(std::min<size_t>(size_t(c->n_cu) * size_t(c->step_grid_per_cu), (n_cur + sb - 1) / sb));
+    const int ev_lanes = int(std::min<size_t>(max_lanes, n_cur * 64));
+    const int surr_lanes = int(std::min<size_t>(max_lanes, (n_cur + 63) / 64 * 64));
+    rc = graph_launch(c, [&]() {
+        for (int g = 0; g < GRAPH_GENS; ++g) {
+            const int k = s ^ (g & 1);
+            TransportArgs* A = c->d_gargs + k;
+            launch_gen_begin(c->d_ctr, A, use_surr ? c->d_surr_bucket : nullptr, c->stream);
+            launch_step_lma(A, mk[k], grid, c->stream, c->step_block);
+            launch_event_room(c->d_ctr, room, c->stream);
+            if (!use_surr) {
+                launch_event_lma(A, mk[k], ev_lanes, c->stream);
+                continue;
+            }
+            // the learned sampler: network rows, forward pass, rows ordered by
+            // harmonic, photons; the deferred events appended to the deferred
+            // queue (those without room, and the events below the network's
+            // range, to the exact kernels in this generation)
+            launch_surr_inputs(A, mk[k], S, c->stream);
+            launch_surr_pad(S, c->stream);
+            net.forward(int(rows_cap), false, c->stream, S.n_rows);
+            launch_surr_order_dev(S, uint32_t(n_cur), c->stream);
+            launch_surr_event(A, mk[k], S, surr_lanes, c->stream);
+            launch_gather_deferred(c->evq.v, c->d_ev_extra, c->d_ev_extra + c->ev_extra_cap, c->d_surr_fb, S.n_fb, n_cur,
+                                   c->defq.v, c->d_def_extra, c->d_def_extra + c->def_extra_cap, c->d_ctr + C_NDEF,
+                                   c->defq.cap, c->d_surr_now, S.n_now, c->stream);
+            launch_commit_deferred(S.n_fb, c->d_ctr + C_NDEF, c->defq.cap, S.n_now, c->d_ctr + C_STATS, c->stream);
+            LmaArgs mn = mk[k];
+            mn.ev_list = c->d_surr_now;
+            mn.n_ev_dev = S.n_now;
+            launch_event_lma(A, mn, ev_lanes, c->stream);
+        }
+    });
+    if (rc) return rc;
+    unsigned long long now[C_N];
+    rc = graph_finish(c, now, &ms[0]);
+    if (rc) return rc;
+    ++launches[0];
+    c->n_out = now[C_OUT];
+    c->n_def = size_t(now[C_NDEF]);
+    // generations that ran their step kernel; the last one's output is the input left
+    const uint64_t ran = now[C_GENS];
+    *gens += ran;
+    int last = int((uint64_t(s) + ran) & 1u);
+    if (now[C_STALL]) {
+        // the last generation's events are still queued: grow the queues and sample them
+        ++c->graph_stalls;
+        const size_t n_ev = size_t(now[C_EV]);
+        HIPCHK(c, grow_buf(c->out, c->n_out + (track ? 0 : n_ev) + 1024, c->n_out, c->stream));
+        HIPCHK(c, grow_buf(c->phq, size_t(now[track ? C_PH : C_NEXT]) + (track ? n_ev : 0) + 1024,
+                           size_t(now[track ? C_PH : C_NEXT]), c->stream));
+        LmaArgs me = mk[last ^ 1];
+        me.ph = c->phq.v;
+        me.ph_cap = c->phq.cap;
+        me.n_ev_dev = nullptr;
+        TransportArgs te = ta[last ^ 1];
+        te.out = c->out.v;
+        te.next = c->phq.v;
+        te.next_cap = c->phq.cap;
+        HIPCHK(c, scopy(c, c->d_args, &te, sizeof(te), hipMemcpyHostToDevice));
+        rc = use_surr ? lma_surrogate_pass(c, me, n_ev, max_lanes, &ms[1], &launches[1], surr_fb)
+                      : lma_event_pass(c, me, false, n_ev, max_lanes, &ms[1], &launches[1]);
+        if (rc) return rc;
+        now[C_LNEXT] = read_ctr(c, C_LNEXT);
+        c->n_out = read_ctr(c, C_OUT);
+    }
+    *cur = &c->lq[last];
+    *li = last ^ 1;
+    *n_left = size_t(now[C_LNEXT]);
+    return PTG_OK;
+}
+
+// Photon generations of the LMA loop from a captured sequence, the input in
+// phq (s = 0) or phq2 (s = 1), n_ph photons. On return *pcur and *n_left hold
+// what is left; as lepton_sequence, a stalled generation's pair events are
+// sampled here after the queues have grown.
+static int photon_sequence(ptg_ctx* c, TransportArgs a, LmaArgs m, int s, size_t n_ph, size_t max_lanes, double* ms,
+                           uint64_t* launches, uint64_t* gens, DevBuf** pcur, size_t* n_left) {
+    DevBuf* pb[2] = {&c->phq, &c->phq2};
+    const bool keep = a.opt.keep_decayed_photons != 0;
+    // every photon of the phase leaves the field once (a final state) or decays;
+    // each pair event writes two leptons and a decayed photon or re-queues the
+    // photon. Room for a pair event per eight photons; the device checks the room
+    // before each generation's events
+    const size_t budget = n_ph / 8 + 4096;
+    HIPCHK(c, grow_buf(c->out, c->n_out + n_ph + 1024, c->n_out, c->stream));
+    if (keep) HIPCHK(c, grow_buf(c->inter, c->n_inter + budget, c->n_inter, c->stream));
+    if (c->evq.cap < n_ph) HIPCHK(c, alloc_buf(c->evq, n_ph + n_ph / 4));
+    int rc = ensure_ev_extra(c, c->evq.cap);
+    if (rc) return rc;
+    if (pb[s ^ 1]->cap < n_ph) HIPCHK(c, alloc_buf(*pb[s ^ 1], n_ph + n_ph / 4));
+    const unsigned long long n_pairs = read_ctr(c, C_PAIRS);
+    HIPCHK(c, grow_buf(c->pairq, size_t(n_pairs) + 2 * budget, size_t(n_pairs), c->stream));
+
+    TransportArgs ta[2];
+    LmaArgs mk[2];
+    for (int k = 0; k < 2; ++k) {
+        ta[k] = a;
+        ta[k].in = pb[k]->v;
+        ta[k].n_in = 0;
+        ta[k].out = c->out.v;
+        ta[k].inter = c->inter.v;
+        ta[k].next = c->pairq.v;  // (no leptons are queued with photons)
+        ta[k].next_ctr = c->d_ctr + C_PAIRS;
+        ta[k].next_cap = c->pairq.cap;
+        mk[k] = m;
+        mk[k].ev = c->evq.v;
+        mk[k].ev_cap = c->evq.cap;
+        mk[k].ev_k0 = c->d_ev_extra;
+        mk[k].ev_a = c->d_ev_extra + c->ev_extra_cap;
+        mk[k].ev_in = c->evq.v;
+        mk[k].ev_list = nullptr;
+        mk[k].n_ev = 0;
+        mk[k].n_ev_dev = c->d_ctr + C_EVRUN;
+        mk[k].next = pb[k ^ 1]->v;
+        mk[k].next_ctr = c->d_ctr + C_LNEXT;
+        mk[k].pairs = c->pairq.v;
+        mk[k].pairs_cap = c->pairq.cap;
+    }
+    HIPCHK(c, scopy(c, c->d_gargs, ta, sizeof(ta), hipMemcpyHostToDevice));
+    // per event: two pair records, the decayed photon when kept, and room kept
+    // for the final state of the photon when it is re-queued
+    const size_t over = graph_room_override();
+    EventRoom room{};
+    room.out_cap = room_cap(c->out.cap, c->n_out + n_ph, over);
+    room.out_per = 1u;
+    room.ph_cap = ~0ull;
+    room.ph_slot = C_PH;
+    room.pairs_cap = room_cap(c->pairq.cap, size_t(n_pairs), over);
+    room.pairs_per = 2u;
+    room.inter_cap = keep ? room_cap(c->inter.cap, c->n_inter, over) : ~0ull;
+    room.inter_per = keep ? 1u : 0u;
+    rc = graph_prime(c, n_ph);
+    if (rc) return rc;
+    const int grid = int(std::min<size_t>(size_t(c->n_cu) * 4, (n_ph + 255) / 256));
+    const int ev_lanes = int(std::min<size_t>(max_lanes, n_ph * 64));
+    rc = graph_launch(c, [&]() {
+        for (int g = 0; g < GRAPH_GENS; ++g) {
+            const int k = s ^ (g & 1);
+            TransportArgs* A = c->d_gargs + k;
+            launch_gen_begin(c->d_ctr, A, nullptr, c->stream);
+            launch_photon_step_lma(A, mk[k], grid, c->stream);
+            launch_event_room(c->d_ctr, room, c->stream);
+            launch_pair_event_lma(A, mk[k], ev_lanes, c->stream);
+        }
+    });
+    if (rc) return rc;
+    unsigned long long now[C_N];
+    rc = graph_finish(c, now, &ms[1]);
+    if (rc) return rc;
+    ++launches[1];
+    c->n_out = now[C_OUT];
+    c->n_inter = now[C_INTER];
+    const uint64_t ran = now[C_GENS];
+    *gens += ran;
+    const int last = int((uint64_t(s) + ran) & 1u);
+    if (now[C_STALL]) {
+        ++c->graph_stalls;
+        const size_t n_ev = size_t(now[C_EV]);
+        if (keep) HIPCHK(c, grow_buf(c->inter, c->n_inter + n_ev + 1024, c->n_inter, c->stream));
+        HIPCHK(c, grow_buf(c->pairq, size_t(now[C_PAIRS]) + 2 * n_ev + 1024, size_t(now[C_PAIRS]), c->stream));
+        LmaArgs me = mk[last ^ 1];
+        me.pairs = c->pairq.v;
+        me.pairs_cap = c->pairq.cap;
+        me.n_ev_dev = nullptr;
+        TransportArgs te = ta[last ^ 1];
+        te.inter = c->inter.v;
+        te.next = c->pairq.v;
+        te.next_cap = c->pairq.cap;
+        HIPCHK(c, scopy(c, c->d_args, &te, sizeof(te), hipMemcpyHostToDevice));
+        rc = lma_event_pass(c, me, true, n_ev, max_lanes, &ms[1], &launches[1]);
+        if (rc) return rc;
+        now[C_LNEXT] = read_ctr(c, C_LNEXT);
+        c->n_out = read_ctr(c, C_OUT);
+        c->n_inter = read_ctr(c, C_INTER);
+    }
+    *pcur = pb[last];
+    *n_left = size_t(now[C_LNEXT]);
+    return PTG_OK;
 }
 
 // The LMA loop. Lepton phase: the step kernel over the live electrons and
@@ -1075,6 +1444,13 @@ static int run_lma(ptg_ctx* c, TransportArgs a, double* ms, uint64_t* launches, 
         // ---------------- lepton phase
         for (;;) {
         while (n_cur > 0) {
+            if (c->launch_mode == PTG_LAUNCH_GRAPH && (cur == &c->lq[0] || cur == &c->lq[1])) {
+                // after the phase's first generation: captured sequences
+                int rc = lepton_sequence(c, a, m, track, use_surr, cur == &c->lq[0] ? 0 : 1, n_cur, max_lanes, ms, launches,
+                                         gens, surr_fb, &cur, &n_cur, &li);
+                if (rc) return rc;
+                continue;
+            }
             // each live lepton yields at most one final state or one event; each event
             // at most one photon and one continuing lepton
             HIPCHK(c, grow_buf(c->out, c->n_out + 2 * n_cur + 1024, c->n_out, c->stream));
@@ -1173,6 +1549,11 @@ static int run_lma(ptg_ctx* c, TransportArgs a, double* ms, uint64_t* launches, 
         HIPCHK(c, zero_ctr(C_PAIRS));
         DevBuf* pcur = &c->phq;
         while (n_ph > 0) {
+            if (c->launch_mode == PTG_LAUNCH_GRAPH) {
+                int rc = photon_sequence(c, a, m, pcur == &c->phq ? 0 : 1, n_ph, max_lanes, ms, launches, gens, &pcur, &n_ph);
+                if (rc) return rc;
+                continue;
+            }
             HIPCHK(c, grow_buf(c->out, c->n_out + n_ph + 1024, c->n_out, c->stream));
             if (a.opt.keep_decayed_photons) HIPCHK(c, grow_buf(c->inter, c->n_inter + n_ph + 1024, c->n_inter, c->stream));
             if (c->evq.cap < n_ph) HIPCHK(c, alloc_buf(c->evq, n_ph + n_ph / 4));
@@ -1328,6 +1709,8 @@ int ptg_run(ptg_ctx* c, const ptg_collide_options* o, uint64_t seed, ptg_counter
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->n_out = 0;
     c->n_inter = 0;
+    c->graph_launches = 0;
+    c->graph_stalls = 0;
 
     double ms[2] = {0.0, 0.0};  // lepton, photon kernels
     uint64_t launches[2] = {0, 0};
@@ -1403,7 +1786,9 @@ int ptg_run(ptg_ctx* c, const ptg_collide_options* o, uint64_t seed, ptg_counter
         cnt->photon_launches = launches[1];
         cnt->photon_steps_closed_form = fin[C_STATS + STAT_PHOTON_STEPS_CF];
         cnt->surrogate_events = fin[C_STATS + STAT_SURROGATE];
-        cnt->surrogate_fallbacks = surr_fb;
+        cnt->surrogate_fallbacks = surr_fb + fin[C_STATS + STAT_SURR_FB];
+        cnt->graph_launches = c->graph_launches;
+        cnt->graph_stalls = c->graph_stalls;
     }
     return PTG_OK;
 }
@@ -1931,6 +2316,13 @@ int ptg_set_batch(ptg_ctx* c, const ptg_laser_desc* lasers, const ptg_beam_desc*
     c->n_prim = total;
     c->n_coll = n_coll;
     c->batch_a0_max = a0_max;
+    return PTG_OK;
+}
+
+int ptg_set_launch_mode(ptg_ctx* c, int mode) {
+    if (!c) return fail(c, PTG_E_ARG, "null argument");
+    if (mode != PTG_LAUNCH_EAGER && mode != PTG_LAUNCH_GRAPH) return fail(c, PTG_E_ARG, "unknown launch mode");
+    c->launch_mode = mode;
     return PTG_OK;
 }
 

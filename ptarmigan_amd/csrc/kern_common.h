@@ -10,7 +10,19 @@
 
 namespace ptg {
 
-enum { STAT_LEPTON_STEPS = 0, STAT_PHOTON_STEPS, STAT_EMISSIONS, STAT_PAIRS, STAT_TRIALS, STAT_PHOTON_STEPS_CF, STAT_SURROGATE, STAT_N };
+enum { STAT_LEPTON_STEPS = 0, STAT_PHOTON_STEPS, STAT_EMISSIONS, STAT_PAIRS, STAT_TRIALS, STAT_PHOTON_STEPS_CF, STAT_SURROGATE,
+       STAT_SURR_FB, STAT_N };
+
+// Counter block of a context (unsigned long long words): work and queue counters
+// of the transport, then the statistics. C_NCUR, C_GENS, C_NDEF, C_EVRUN and
+// C_STALL belong to generations run from a captured graph: the input count of
+// the generation running, the generations that ran their step kernel, the
+// events held for the learned sampler's deferred exact pass, the events the
+// event kernels of the generation take, and the stall flag (a generation found
+// too little room in the output queues for its events: its events stay queued
+// and the rest of the sequence does nothing).
+enum { C_WORK = 0, C_NEXT, C_OUT, C_INTER, C_ID, C_FLAGS, C_EV, C_LNEXT, C_PH, C_PAIRS, C_SROWS, C_SFB, C_SNOW,
+       C_NCUR, C_GENS, C_NDEF, C_EVRUN, C_STALL, C_STATS, C_N = C_STATS + STAT_N };
 
 // CollideOptions as the kernels need them (src/main.rs:72-105).
 struct DevOptions {

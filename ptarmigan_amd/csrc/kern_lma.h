@@ -31,10 +31,12 @@ struct LmaArgs {
     double* ev_a;                // sqrt(<a^2>) at the event position
     unsigned long long* ev_ctr;
     unsigned long long ev_cap;
-    // event kernel input: events ev_list[0..n_ev) of ev_in, or 0..n_ev when ev_list is null
+    // event kernel input: events ev_list[0..n_ev) of ev_in, or 0..n_ev when ev_list is null;
+    // n_ev is read from *n_ev_dev when that is set (generations run from a captured graph)
     PBuf ev_in;
     const uint32_t* ev_list;
     uint64_t n_ev;
+    const unsigned long long* n_ev_dev;
     // event kernel outputs
     PBuf next;                   // electrons continuing after their event
     unsigned long long* next_ctr;
@@ -67,6 +69,7 @@ struct SurrArgs {
     uint32_t* now;                   // events below the box's amplitude: the exact sampler in this generation
     unsigned long long* n_now;
     unsigned long long n_ev;         // events queued by the step kernel
+    const unsigned long long* n_ev_dev;  // or the count in device memory (captured generations)
     uint32_t* perm;                  // rows ordered by harmonic bucket (largest first), or null
     unsigned int* bucket;            // [2][32] bucket counts and cursors
     // training data
@@ -85,10 +88,50 @@ void launch_surr_inputs(const TransportArgs* a, const LmaArgs& m, const SurrArgs
 void launch_surr_event(const TransportArgs* a, const LmaArgs& m, const SurrArgs& s, int lanes, hipStream_t st);
 // orders the rows (n_rows on the host) by their harmonic's power-of-two bucket, largest first
 void launch_surr_order(const SurrArgs& s, uint32_t n_rows, hipStream_t st);
+// the same with the row count *s.n_rows in device memory, at most max_rows
+void launch_surr_order_dev(const SurrArgs& s, uint32_t max_rows, hipStream_t st);
+// zeroes the network input rows from *s.n_rows up to the next multiple of 32
+void launch_surr_pad(const SurrArgs& s, hipStream_t st);
 void launch_surr_groups(const LmaArgs& m, const SurrArgs& s, int lanes, hipStream_t st);
 void launch_surr_batch(const LmaArgs& m, const SurrArgs& s, hipStream_t st);
 void launch_gather_events(const PBuf& src, const double* k0s, const double* as_, const uint32_t* list, uint64_t n, const PBuf& dst,
                           double* k0d, double* ad, uint64_t off, hipStream_t s);
+// Generations run from a captured graph: the deferred events of one generation
+// (list[0..*n_list)) appended to dst at *n_dst while dst has room (capacity
+// cap); those beyond it go to the exact pass of the same generation
+// (now[*n_now ...]). The count is added to *n_dst by launch_commit_deferred,
+// after the gather. max_list bounds *n_list.
+void launch_gather_deferred(const PBuf& src, const double* k0s, const double* as_, const uint32_t* list,
+                            const unsigned long long* n_list, uint64_t max_list, const PBuf& dst, double* k0d, double* ad,
+                            const unsigned long long* n_dst, uint64_t cap, uint32_t* now, unsigned long long* n_now,
+                            hipStream_t s);
+// ... and the events sampled exactly in this generation (the deferred and the
+// spilled ones) are added to stats[STAT_SURR_FB].
+void launch_commit_deferred(const unsigned long long* n_list, unsigned long long* n_dst, uint64_t cap,
+                            const unsigned long long* n_now, unsigned long long* stats, hipStream_t s);
+
+// Start of a captured generation. Unless an earlier generation of the sequence
+// stalled: its input count (ctr[C_LNEXT], what the last generation queued) goes
+// to A->n_in and ctr[C_NCUR], the generation is counted in ctr[C_GENS] when it
+// has input, and C_WORK, C_EV and C_LNEXT are zeroed; after a stall A->n_in = 0
+// and the queue counters are left as they are. Either way C_EVRUN and the
+// learned sampler's C_SROWS, C_SFB, C_SNOW are zeroed, and bucket (64 words)
+// when given.
+void launch_gen_begin(unsigned long long* ctr, TransportArgs* A, unsigned int* bucket, hipStream_t s);
+
+// Room the events of a captured generation need: per event at most out_per
+// final-state records, ph_per photons queued at ctr[ph_slot], pairs_per pair
+// records and inter_per decayed photons; the capacities of those queues.
+struct EventRoom {
+    unsigned long long out_cap, ph_cap, pairs_cap, inter_cap;
+    uint32_t out_per, ph_per, pairs_per, inter_per;
+    int ph_slot;
+};
+// Between the step kernel and the event kernels of a captured generation:
+// ctr[C_EVRUN] = ctr[C_EV] when the queues have room for that many events;
+// otherwise C_EVRUN = 0 and C_STALL = 1, the events stay queued (C_EV) for the
+// host, and the rest of the sequence does nothing.
+void launch_event_room(unsigned long long* ctr, const EventRoom& k, hipStream_t s);
 
 // block: 64, 128 or 256 threads (the kernel's bound); grid: persistent blocks
 void launch_step_lma(const TransportArgs* a, const LmaArgs& m, int grid, hipStream_t s, int block = 256);

@@ -39,6 +39,7 @@ __global__ __launch_bounds__(256) void k_step_lma(const TransportArgs* __restric
     const double dt = A->dt;
     const double kc = C_LIGHT * T_COMPTON;
     __shared__ TT rt[StepTable<MODE>::N];
+    if (A->n_in == 0u) return;  // a captured generation without input
     if (MODE == LMA_LP || MODE == LMA_CP) {
         const double* src = StepTable<MODE>::src(M);
         for (int i = threadIdx.x; i < StepTable<MODE>::N; i += blockDim.x) rt[i] = TT(src[i]);
@@ -304,7 +305,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const int lane = threadIdx.x;
     uint64_t n_emit = 0, trials = 0;
     uint32_t flags = 0;
-    for (uint64_t e = blockIdx.x; e < M.n_ev; e += gridDim.x) {
+    const uint64_t n_ev = M.n_ev_dev ? *M.n_ev_dev : M.n_ev;
+    for (uint64_t e = blockIdx.x; e < n_ev; e += gridDim.x) {
         const PBuf& q = M.ev_in;
         const uint32_t i = M.ev_list ? M.ev_list[e] : uint32_t(e);
         d4 u{q.d[P_UT][i], q.d[P_UX][i], q.d[P_UY][i], q.d[P_UZ][i]};
@@ -501,7 +503,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PTG_LMA_WAVE
     __shared__ double tb_s[16], tb_f[16];  // the event's theta bound, one copy per wave
     uint64_t n_emit = 0, trials = 0;
     uint32_t flags = 0;
-    for (uint64_t e = blockIdx.x; e < M.n_ev; e += gridDim.x) {
+    const uint64_t n_ev = M.n_ev_dev ? *M.n_ev_dev : M.n_ev;
+    for (uint64_t e = blockIdx.x; e < n_ev; e += gridDim.x) {
         const PBuf& q = M.ev_in;
         const uint32_t i = M.ev_list ? M.ev_list[e] : uint32_t(e);
         d4 u{q.d[P_UT][i], q.d[P_UX][i], q.d[P_UY][i], q.d[P_UZ][i]};
@@ -771,25 +774,71 @@ __device__ __noinline__ void bw_sample_wave(const DBessel& D, double a, double e
     theta_out = theta;
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave (a block is one wave).
+__device__ __forceinline__ double wave_inclusive_sum(double v) {
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const double t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+// Harmonic of a CP pair by explicit summation of the partial rates until the
+// cumulative rate passes frac x the total (pair_creation/cp/mod.rs:327-365),
+// one decay per wave: all 64 lanes call this with the same arguments and return
+// the same value. The reference sums one harmonic after another; at the
+// amplitudes of a0 ~ 10-20 the sum runs over thousands of harmonics, each a
+// 64-96 point quadrature of Bessel functions, so the lanes take 64 consecutive
+// harmonics at a time and a prefix sum over the wave finds the first one whose
+// cumulative rate passes the target (the same harmonic, up to the rounding of
+// the partial sums). n_max - 1 with `fell_back` set if the sum never passes it,
+// -1 if the total is not available.
+__device__ __noinline__ int cpbw_harmonic_wave(TP tb, const double* P, const CpScalars& c, double a, double eta, double sv3,
+                                               double frac, bool& fell_back) {
+    const int lane = threadIdx.x;
+    int lo, hi;
+    cpbw_sum_limits(a, eta, lo, hi);
+    double f[2];
+    fell_back = false;
+    if (!cpbw_total(tb, P, c, a, eta, f)) return -1;
+    const double target = 0.5 * (f[0] * (1.0 + sv3) + f[1] * (1.0 - sv3)) * frac;
+    double base = 0.0;
+    for (int k0 = lo; k0 <= hi; k0 += 64) {
+        const int k = k0 + lane;
+        double v = 0.0;
+        if (k <= hi) {
+            const cplx pr = cpbw_partial_rate(tb, P + cp_pack::G32N, P + cp_pack::G32W, k, a, eta);
+            v = 0.5 * ((1.0 + sv3) * pr.re + (1.0 - sv3) * pr.im);
+        }
+        const double cum = base + wave_inclusive_sum(v);
+        const uint64_t mask = __ballot(k <= hi && cum > target);
+        if (mask) return k0 + (__ffsll((long long)mask) - 1);
+        base = __shfl(cum, 63, 64);
+    }
+    fell_back = true;
+    return hi - 1;
+}
+
 // Pair events (pair_creation/mod.rs:47-107 with lp/mod.rs:424-442 and
 // plane_wave.rs:305-316): harmonic from the CDF tables, (s, theta) by rejection
 // on the double-differential rate, the positron built in the zero-momentum
 // frame; the electron and positron go to the pair queue, the photon is kept
-// (decayed) or re-queued with its reduced weight. LP runs one event per wave
-// (every lane computes the event's scalars, the lanes share the rejection
-// sampling, lane 0 writes); CP, whose sampler reads tabulated Bessel functions,
-// one event per lane.
+// (decayed) or re-queued with its reduced weight. One event per wave: every lane
+// computes the event's scalars, the lanes share the harmonic sum (CP) or the
+// rejection sampling (LP), and lane 0 writes.
 template <bool CIRC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PTG_LMA_WAVE_WPE))) void k_pair_event_lma(
     const TransportArgs* __restrict__ A, LmaArgs M) {
     const uint32_t lane = threadIdx.x;
-    const uint64_t e0 = CIRC ? uint64_t(blockIdx.x) * 64u + lane : uint64_t(blockIdx.x);
-    const uint64_t stride = CIRC ? uint64_t(gridDim.x) * 64u : uint64_t(gridDim.x);
-    const bool writer = CIRC || lane == 0;
+    const uint64_t e0 = blockIdx.x, stride = gridDim.x;
+    const bool writer = lane == 0;
     __shared__ double tb_s[16], tb_f[16];  // LP: the decay's theta bound, one copy per wave
     uint64_t n_pairs = 0, trials = 0;
     uint32_t flags = 0;
-    for (uint64_t e = e0; e < M.n_ev; e += stride) {
+    const uint64_t n_ev = M.n_ev_dev ? *M.n_ev_dev : M.n_ev;
+    for (uint64_t e = e0; e < n_ev; e += stride) {
         const PBuf& q = M.ev_in;
         uint32_t i = M.ev_list ? M.ev_list[e] : uint32_t(e);
         // a batch of collisions: the photon's own collision's laser
@@ -812,7 +861,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PTG_LMA_WAVE
         double alp = a * 1.4142135623730951;
         sv4 sl = stokes_rotate(stokes_to_lma(pin, k), M.pol_angle);
         bool fell_back = false;
-        int n = CIRC ? cpbw_harmonic(M.lcfa, M.cp, M.csc, a, eta, sl.v, rng_uniform(rng), fell_back)
+        int n = CIRC ? cpbw_harmonic_wave(M.lcfa, M.cp, M.csc, a, eta, sl.v, rng_uniform(rng), fell_back)
                      : bw_harmonic(M.bw, M.bsc, alp, eta, sl.q, rng_uniform(rng));
         bool made = false;
         if (n < 1) {
@@ -992,7 +1041,7 @@ __device__ __forceinline__ double surr_nu(const LmaArgs& M, double a, double eta
 template <bool CIRC>
 __global__ void k_surr_inputs(const TransportArgs* __restrict__ A, LmaArgs M, SurrArgs S) {
     const uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (e >= S.n_ev) return;
+    if (e >= (S.n_ev_dev ? *S.n_ev_dev : S.n_ev)) return;
     const PBuf& q = M.ev;
     const uint32_t i = uint32_t(e);
     const double ut = q.d[P_UT][i], ux = q.d[P_UX][i], uy = q.d[P_UY][i], uz = q.d[P_UZ][i];
@@ -1300,6 +1349,116 @@ void launch_gather_events(const PBuf& src, const double* k0s, const double* as_,
     if (n) k_gather_events<<<unsigned((n + 255) / 256), 256, 0, s>>>(src, k0s, as_, list, n, dst, k0d, ad, off);
 }
 
+// ---------------------------------------------------------------- captured generations
+// Kernels whose counts live in device memory, so that a sequence of
+// generations can be captured once and replayed without the host reading a
+// count between launches.
+
+__global__ void k_gen_begin(unsigned long long* ctr, TransportArgs* A, unsigned int* bucket) {
+    if (threadIdx.x == 0) {
+        ctr[C_EVRUN] = 0ull;
+        ctr[C_SROWS] = 0ull;
+        ctr[C_SFB] = 0ull;
+        ctr[C_SNOW] = 0ull;
+        if (ctr[C_STALL]) {
+            A->n_in = 0u;
+            ctr[C_NCUR] = 0ull;
+        } else {
+            const unsigned long long n = ctr[C_LNEXT];
+            A->n_in = uint32_t(n);
+            ctr[C_NCUR] = n;
+            if (n) ctr[C_GENS] += 1ull;
+            ctr[C_WORK] = 0ull;
+            ctr[C_EV] = 0ull;
+            ctr[C_LNEXT] = 0ull;
+        }
+    }
+    if (bucket) bucket[threadIdx.x] = 0u;
+}
+void launch_gen_begin(unsigned long long* ctr, TransportArgs* A, unsigned int* bucket, hipStream_t s) {
+    k_gen_begin<<<1, 64, 0, s>>>(ctr, A, bucket);
+}
+
+__global__ void k_event_room(unsigned long long* ctr, EventRoom k) {
+    if (threadIdx.x != 0 || ctr[C_STALL]) return;
+    const unsigned long long n = ctr[C_EV];
+    const bool room = ctr[C_OUT] + k.out_per * n <= k.out_cap && ctr[k.ph_slot] + k.ph_per * n <= k.ph_cap &&
+                      ctr[C_PAIRS] + k.pairs_per * n <= k.pairs_cap && ctr[C_INTER] + k.inter_per * n <= k.inter_cap;
+    if (room) ctr[C_EVRUN] = n;
+    else if (n) ctr[C_STALL] = 1ull;
+}
+void launch_event_room(unsigned long long* ctr, const EventRoom& k, hipStream_t s) { k_event_room<<<1, 64, 0, s>>>(ctr, k); }
+
+__global__ void k_surr_count_dev(SurrArgs S) {
+    const uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r < *S.n_rows) atomicAdd(S.bucket + surr_bucket(S.out + 32 * size_t(r)), 1u);
+}
+__global__ void k_surr_scatter_dev(SurrArgs S) {
+    const uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r < *S.n_rows) S.perm[atomicAdd(S.bucket + 32 + surr_bucket(S.out + 32 * size_t(r)), 1u)] = uint32_t(r);
+}
+void launch_surr_order_dev(const SurrArgs& s, uint32_t max_rows, hipStream_t st) {
+    if (!max_rows) return;
+    const unsigned g = (max_rows + 255) / 256;
+    k_surr_count_dev<<<g, 256, 0, st>>>(s);
+    k_surr_scan<<<1, 64, 0, st>>>(s);
+    k_surr_scatter_dev<<<g, 256, 0, st>>>(s);
+}
+
+// rows [n_rows, n_rows rounded up to 32) of the network's inputs, 8 floats each
+__global__ void k_surr_pad(SurrArgs S) {
+    const uint64_t n = *S.n_rows, m = (n + 31) / 32 * 32;
+    const uint64_t k = n * 8 + threadIdx.x;
+    if (k < m * 8) S.x[k] = 0.0f;
+}
+void launch_surr_pad(const SurrArgs& s, hipStream_t st) { k_surr_pad<<<1, 256, 0, st>>>(s); }
+
+__global__ void k_gather_deferred(PBuf src, const double* __restrict__ k0s, const double* __restrict__ as_,
+                                  const uint32_t* __restrict__ list, const unsigned long long* n_list, PBuf dst, double* k0d,
+                                  double* ad, const unsigned long long* n_dst, uint64_t cap, uint32_t* now,
+                                  unsigned long long* n_now) {
+    const uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool have = j < *n_list;
+    const uint64_t o = *n_dst + j;
+    const uint32_t i = have ? list[j] : 0u;
+    // the lanes without room take their places in the exact pass of this generation
+    const bool spill = have && o >= cap;
+    if (__ballot(spill)) {
+        if (spill) now[wave_reserve(n_now, 1u)] = i;
+    }
+    if (!have || spill) return;
+#pragma unroll
+    for (int c = 0; c < P_NDOUBLE; ++c) dst.d[c][o] = src.d[c][i];
+    dst.id[o] = src.id[i];
+    dst.parent_id[o] = src.parent_id[i];
+    dst.stream[o] = src.stream[i];
+    dst.shower[o] = src.shower[i];
+    dst.meta[o] = src.meta[i];
+    dst.nev[o] = src.nev[i];
+    k0d[o] = k0s[i];
+    ad[o] = as_[i];
+}
+void launch_gather_deferred(const PBuf& src, const double* k0s, const double* as_, const uint32_t* list,
+                            const unsigned long long* n_list, uint64_t max_list, const PBuf& dst, double* k0d, double* ad,
+                            const unsigned long long* n_dst, uint64_t cap, uint32_t* now, unsigned long long* n_now,
+                            hipStream_t s) {
+    if (max_list)
+        k_gather_deferred<<<unsigned((max_list + 255) / 256), 256, 0, s>>>(src, k0s, as_, list, n_list, dst, k0d, ad, n_dst, cap,
+                                                                          now, n_now);
+}
+__global__ void k_commit_deferred(const unsigned long long* n_list, unsigned long long* n_dst, uint64_t cap,
+                                  const unsigned long long* n_now, unsigned long long* stats) {
+    if (threadIdx.x != 0) return;
+    const unsigned long long n = *n_dst + *n_list;
+    const unsigned long long spilled = n > cap ? n - cap : 0ull;
+    *n_dst = n - spilled;
+    stats[STAT_SURR_FB] += *n_list + *n_now - spilled;
+}
+void launch_commit_deferred(const unsigned long long* n_list, unsigned long long* n_dst, uint64_t cap,
+                            const unsigned long long* n_now, unsigned long long* stats, hipStream_t s) {
+    k_commit_deferred<<<1, 64, 0, s>>>(n_list, n_dst, cap, n_now, stats);
+}
+
 // Probes of the LMA device functions (parity tests).
 __global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, uint32_t in_w, uint32_t out_w, LmaArgs m,
                            uint64_t key) {
@@ -1370,24 +1529,6 @@ __global__ void k_eval_lma(int fn, const double* in, double* out, uint32_t n, ui
     case PTG_FN_CPBW_PARTIAL_RATE: {
         cplx r = cpbw_partial_rate(m.lcfa, m.cp + cp_pack::G32N, m.cp + cp_pack::G32W, int(x[0]), x[1], x[2]);
         y[0] = r.re; y[1] = r.im;
-        break;
-    }
-    case PTG_FN_CPBW_HARMONIC: {
-        bool fb = false;
-        y[0] = cpbw_harmonic(m.lcfa, m.cp, m.csc, x[0], x[1], x[2], x[3], fb);
-        y[1] = fb ? 1.0 : 0.0;
-        break;
-    }
-    case PTG_FN_CPBW_SAMPLE: {
-        Rng rng = rng_make(key, uint64_t(x[5]));
-        bool fb = false;
-        int nh = x[6] > 0.0 ? int(x[6]) : cpbw_harmonic(m.lcfa, m.cp, m.csc, x[0], x[1], x[4], rng_uniform(rng), fb);
-        if (nh < 1) { y[0] = 0.0; y[3] = -2.0; break; }
-        double s, th;
-        uint32_t tr = 0;
-        bool capped = false;
-        cpbw_sample_at(m.lcfa, m.cp, nh, x[0], x[1], x[2], x[3], x[4], rng, s, th, tr, capped);
-        y[0] = nh; y[1] = s; y[2] = th; y[3] = capped ? -1.0 : double(tr);
         break;
     }
     case PTG_FN_BESSEL_J: {
@@ -1504,6 +1645,34 @@ __global__ __launch_bounds__(64) void k_eval_cp_sample(const double* in, double*
     }
 }
 
+// The CP Breit-Wheeler harmonic and (n, s, theta) probes, one row per wave as the
+// pair-event kernel: the harmonic by the wave's shared sum (cpbw_harmonic_wave),
+// (s, theta) by cpbw_sample_at on every lane.
+__global__ __launch_bounds__(64) void k_eval_cpbw_wave(int fn, const double* in, double* out, uint32_t n, uint32_t in_w,
+                                                       uint32_t out_w, LmaArgs m, uint64_t key) {
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const double* x = in + uint64_t(i) * in_w;
+        double* y = out + uint64_t(i) * out_w;
+        bool fb = false;
+        if (fn == PTG_FN_CPBW_HARMONIC) {
+            const int nh = cpbw_harmonic_wave(m.lcfa, m.cp, m.csc, x[0], x[1], x[2], x[3], fb);
+            if (threadIdx.x == 0) { y[0] = nh; y[1] = fb ? 1.0 : 0.0; }
+            continue;
+        }
+        Rng rng = rng_make(key, uint64_t(x[5]));
+        const int nh = x[6] > 0.0 ? int(x[6]) : cpbw_harmonic_wave(m.lcfa, m.cp, m.csc, x[0], x[1], x[4], rng_uniform(rng), fb);
+        if (nh < 1) {
+            if (threadIdx.x == 0) { y[0] = 0.0; y[3] = -2.0; }
+            continue;
+        }
+        double sv, th;
+        uint32_t tr = 0;
+        bool capped = false;
+        cpbw_sample_at(m.lcfa, m.cp, nh, x[0], x[1], x[2], x[3], x[4], rng, sv, th, tr, capped);
+        if (threadIdx.x == 0) { y[0] = nh; y[1] = sv; y[2] = th; y[3] = capped ? -1.0 : double(tr); }
+    }
+}
+
 // The J(n, x, y) probe in mode 4: the quadrature shared by a wave
 // (dbessel_quadrature_wave), one row per wave; every lane's result is checked
 // against lane 0's and a disagreement returns NaN.
@@ -1533,6 +1702,10 @@ void launch_eval_lma(int fn, const double* in, double* out, uint32_t n, uint32_t
     }
     if (fn == PTG_FN_CP_SAMPLE) {
         k_eval_cp_sample<<<rows, 64, 0, s>>>(in, out, n, in_w, out_w, m, key);
+        return;
+    }
+    if (fn == PTG_FN_CPBW_HARMONIC || fn == PTG_FN_CPBW_SAMPLE) {
+        k_eval_cpbw_wave<<<rows, 64, 0, s>>>(fn, in, out, n, in_w, out_w, m, key);
         return;
     }
     if (fn == PTG_FN_LP_SAMPLE || fn == PTG_FN_LP_CL_SAMPLE) {

@@ -42,12 +42,15 @@ struct GemmArgs {
     uint16_t* C; int ldc;        // BF16 result (after activation), or null
     uint16_t* Ct; int ldct;      // BF16 transposed result, or null
     int n_store;                 // columns stored (<= N)
+    const unsigned long long* m_dev;  // when set, rows from *m_dev rounded up to 32 on are skipped
 };
 void launch_gemm(const GemmArgs& g, hipStream_t s);
 
 // Layer 1 forward in FP32: z1 = x W1^T + b1, a1 = SiLU(z1) (x: [M][IN] FP32, W1 from
 // the FP32 parameters); writes a1 BF16 [M][H] (and transposed [H][M] if a1t), z1 FP32.
-void launch_layer1_forward(const float* x, const float* P, int M, uint16_t* a1, uint16_t* a1t, float* z1, hipStream_t s);
+// With m_dev set, rows from *m_dev rounded up to 32 on are skipped.
+void launch_layer1_forward(const float* x, const float* P, int M, uint16_t* a1, uint16_t* a1t, float* z1, hipStream_t s,
+                           const unsigned long long* m_dev = nullptr);
 // dW1 = d1^T x over the M rows (FP32), db for every layer's delta (column sums).
 void launch_layer1_wgrad(const float* d1, const float* x, int M, float* G, hipStream_t s);
 void launch_colsum(const float* d, int M, int N, int ld, float* out, hipStream_t s);

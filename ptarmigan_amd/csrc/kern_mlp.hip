@@ -25,6 +25,7 @@ __device__ __forceinline__ float silu_grad(float z) {
 __global__ __launch_bounds__(64) void k_gemm_bf16(GemmArgs g) {
     const int lane = threadIdx.x;
     const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+    if (g.m_dev && uint64_t(i0) >= (*g.m_dev + 31) / 32 * 32) return;
     const int r = lane & 31, kq = (lane >> 5) * 4;
     const uint16_t* pa = g.A + size_t(i0 + r) * g.lda + kq;
     const uint16_t* pb = g.Bt + size_t(j0 + r) * g.ldb + kq;
@@ -58,10 +59,11 @@ void launch_gemm(const GemmArgs& g, hipStream_t s) {
 
 // Layer 1 in FP32: one thread per (row, hidden unit).
 __global__ void k_layer1_forward(const float* __restrict__ x, const float* __restrict__ P, int M, uint16_t* a1, uint16_t* a1t,
-                                 float* z1) {
+                                 float* z1, const unsigned long long* m_dev) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= M * mlp::H) return;
     const int i = idx / mlp::H, h = idx % mlp::H;
+    if (m_dev && uint64_t(i) >= (*m_dev + 31) / 32 * 32) return;
     const float* w = P + mlp::W1 + h * mlp::IN;
     const float* xi = x + size_t(i) * mlp::IN;
     float zv = P[mlp::B1 + h];
@@ -72,9 +74,10 @@ __global__ void k_layer1_forward(const float* __restrict__ x, const float* __res
     a1[idx] = to_bf16(y);
     if (a1t) a1t[size_t(h) * M + i] = to_bf16(y);
 }
-void launch_layer1_forward(const float* x, const float* P, int M, uint16_t* a1, uint16_t* a1t, float* z1, hipStream_t s) {
+void launch_layer1_forward(const float* x, const float* P, int M, uint16_t* a1, uint16_t* a1t, float* z1, hipStream_t s,
+                           const unsigned long long* m_dev) {
     const int n = M * mlp::H;
-    if (n > 0) k_layer1_forward<<<(n + 255) / 256, 256, 0, s>>>(x, P, M, a1, a1t, z1);
+    if (n > 0) k_layer1_forward<<<(n + 255) / 256, 256, 0, s>>>(x, P, M, a1, a1t, z1, m_dev);
 }
 
 // dW1[h][k] = sum_i d1[i][h] x[i][k]: one block of 256 per (h, k), a tree sum over rows.

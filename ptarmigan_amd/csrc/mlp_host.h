@@ -27,8 +27,9 @@ struct MlpNet {
     hipError_t init_params(const float* host_params, hipStream_t s);  // upload + publish; allocates P..Wb
     hipError_t reserve(int rows);                                     // activation buffers for `rows` (multiple of 32)
     // Forward over M rows of X (M a multiple of 32): O [M][OUT]. `train` keeps
-    // what the backward pass needs.
-    void forward(int M, bool train, hipStream_t s);
+    // what the backward pass needs. With m_dev set, M bounds the rows and only
+    // the first *m_dev (rounded up to 32) are computed.
+    void forward(int M, bool train, hipStream_t s, const unsigned long long* m_dev = nullptr);
     // Loss of T against O, gradients into G (all M rows), then no update.
     void backward(int M, float w_t, hipStream_t s);
     void adam(float lr, hipStream_t s);

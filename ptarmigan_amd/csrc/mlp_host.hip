@@ -71,18 +71,20 @@ hipError_t MlpNet::reserve(int rows) {
     return hipSuccess;
 }
 
-void MlpNet::forward(int M, bool train, hipStream_t s) {
+void MlpNet::forward(int M, bool train, hipStream_t s, const unsigned long long* m_dev) {
     using namespace mlp;
-    launch_layer1_forward(X, P, M, A1, train ? A1t : nullptr, train ? Z1 : nullptr, s);
+    launch_layer1_forward(X, P, M, A1, train ? A1t : nullptr, train ? Z1 : nullptr, s, m_dev);
     GemmArgs g{};
     // layer 2: A2 = SiLU(A1 W2^T + b2)
-    g = GemmArgs{A1, H, Wb + BW2, H, M, H, H, P + B2, 1, nullptr, 0, train ? Z2 : nullptr, H, A2, H, train ? A2t : nullptr, M, H};
+    g = GemmArgs{A1, H, Wb + BW2, H, M, H, H, P + B2, 1, nullptr, 0, train ? Z2 : nullptr, H, A2, H, train ? A2t : nullptr, M, H,
+                 m_dev};
     launch_gemm(g, s);
     // layer 3
-    g = GemmArgs{A2, H, Wb + BW3, H, M, H, H, P + B3, 1, nullptr, 0, train ? Z3 : nullptr, H, A3, H, train ? A3t : nullptr, M, H};
+    g = GemmArgs{A2, H, Wb + BW3, H, M, H, H, P + B3, 1, nullptr, 0, train ? Z3 : nullptr, H, A3, H, train ? A3t : nullptr, M, H,
+                 m_dev};
     launch_gemm(g, s);
     // layer 4 (linear): O = A3 W4^T + b4
-    g = GemmArgs{A3, H, Wb + BW4, H, M, OUT, H, P + B4, 0, nullptr, 0, O, OUT, nullptr, 0, nullptr, 0, OUT};
+    g = GemmArgs{A3, H, Wb + BW4, H, M, OUT, H, P + B4, 0, nullptr, 0, O, OUT, nullptr, 0, nullptr, 0, OUT, m_dev};
     launch_gemm(g, s);
 }
 

@@ -256,7 +256,7 @@ class Engine:
     def run(self, options: CollideOptions, seed=0):
         cnt = _abi.Counters()
         self._check(self._lib.ptg_run(self._ctx, C.byref(options.desc()), seed, C.byref(cnt)))
-        return {k: getattr(cnt, k) for k, _ in _abi.Counters._fields_ if k != "reserved"}
+        return {k: getattr(cnt, k) for k, _ in _abi.Counters._fields_}
 
     def fetch(self, counters, intermediates=False):
         sec = empty_particles(int(counters["n_secondaries"]))
@@ -352,6 +352,11 @@ class Engine:
         be = (_abi.BeamDesc * n)(*[b.desc() for b in beams])
         self._check(self._lib.ptg_set_batch(self._ctx, la, be, n, seed))
         self.laser = lasers[0]
+
+    def set_launch_mode(self, graph: bool):
+        """LMA generations in captured sequences (graph=True, the default) or with
+        the counts read back after every launch (graph=False, per-kernel timings)."""
+        self._check(self._lib.ptg_set_launch_mode(self._ctx, _abi.LAUNCH_GRAPH if graph else _abi.LAUNCH_EAGER))
 
     def autotune(self, options: CollideOptions, seed=0, max_primaries=0):
         r = _abi.TuneResult()

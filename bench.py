@@ -48,7 +48,12 @@ WORKLOADS = {
              "16.5 GeV e- beam 5um x 20um, emission + Breit-Wheeler, secondaries tracked, FP64"),
     "lma": ("LMA plane wave: PlaneWave a0=2 0.8um LP 16-cycle cos^2, 10 GeV e- beam 1% spread 1urad, "
             "nonlinear Compton only, FP64"),
+    "domain": ("learned-sampler domain: 64 PlaneWave collisions, a0 U[0.5, 20], 2-20 GeV, 0.8um 16-cycle cos^2, "
+               "LMA Compton"),
+    "scan": ("batched scan: 64 PlaneWave LP collisions in one batch, a0 0.5-20, 5-17.5 GeV, uneven counts, "
+             "Compton + Breit-Wheeler"),
 }
+BATCHES = ("domain", "scan")
 
 
 def make_workload(name, n):
@@ -324,7 +329,9 @@ def config5_compare(eng, sync, scale, fp64_scale):
     return {"workload": "64 PlaneWave LP collisions in one batch, a0 0.5-20, 5-17.5 GeV, Compton + Breit-Wheeler",
             "surrogate_primaries": ms, "fp64_primaries": mf, "surrogate_fp32_steps_per_s": ss, "fp64_steps_per_s": sf,
             "speedup": ss / sf, "surrogate_ms": 1e3 * ts, "fp64_ms": 1e3 * tf,
-            "pair_events": [cs["pair_events"], cf["pair_events"]], **_against_fp64(pf, mf, ps, ms),
+            "pair_events": [cs["pair_events"], cf["pair_events"]],
+            "generations": [cs["generations"], cf["generations"]],
+            "graph_launches": [cs["graph_launches"], cf["graph_launches"]], **_against_fp64(pf, mf, ps, ms),
             "tuned_launch": {"block": tune["block"], "grid_per_cu": tune["grid_per_cu"], "consistent": tune["consistent"]}}
 
 
@@ -346,7 +353,13 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = the host CPUs one GPU is given (16), or fewer")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="lcfa",
-                    help="lcfa: the focused-laser cascade (default, the headline workload); lma: the plane-wave LMA case")
+                    help="lcfa: the focused-laser cascade (default, the headline workload); lma: the plane-wave LMA "
+                         "case; domain: the learned sampler's domain (--primaries over 64 collisions); scan: the "
+                         "64-collision scan (--scale)")
+    ap.add_argument("--scale", type=float, default=1.0, help="scan workload: primaries per collision times this")
+    ap.add_argument("--eager", action="store_true",
+                    help="LMA workloads: read the counts back after every launch (per-kernel timings) instead of "
+                         "running the generations in captured sequences")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
                     help="mixed: FP32 rate tables and pusher algebra, FP64 phase, momentum update and "
                          "survival product (LMA workloads only)")
@@ -368,23 +381,34 @@ def main():
               "(the transport shards by primaries, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
-    from ptarmigan_amd import Engine
+    from ptarmigan_amd import Engine, Polarization, workloads
     if args.workload == "lma" and args.primaries == 10_000_000:
         args.primaries = 100_000  # the plane-wave case is quoted on 1e5 primaries
-    laser, beam, opts = make_workload(args.workload, args.primaries)
-    opts.precision = 1 if args.precision == "mixed" else 0
-    if args.a0 > 0.0:
-        laser.a0 = args.a0
-    if args.polarization:
-        from ptarmigan_amd import Polarization
-        laser.polarization = Polarization.CIRCULAR if args.polarization == "circular" else Polarization.LINEAR
+    pol = Polarization.CIRCULAR if args.polarization == "circular" else Polarization.LINEAR
     eng = Engine(0)
+    if args.eager:
+        eng.set_launch_mode(False)
+    if args.workload in BATCHES:
+        if args.workload == "domain":
+            lasers, beams, opts = workloads.surrogate_domain(args.primaries, pol)
+        else:
+            lasers, beams, opts = workloads.batched_scan(64, scale=args.scale, polarization=pol)
+        args.primaries = sum(b.n for b in beams)
+        eng.set_batch(lasers, beams, seed=0)
+    else:
+        laser, beam, opts = make_workload(args.workload, args.primaries)
+        if args.a0 > 0.0:
+            laser.a0 = args.a0
+        if args.polarization:
+            laser.polarization = pol
+        pol = laser.polarization
+        eng.set_laser(laser)
+        eng.build_beam(beam, seed=0)
+    opts.precision = 1 if args.precision == "mixed" else 0
     if args.surrogate:
         from ptarmigan_amd import surrogate
-        eng.surrogate_load(surrogate.load(surrogate.default_path(int(laser.polarization)))[0])
+        eng.surrogate_load(surrogate.load(surrogate.default_path(int(pol)))[0])
         opts.surrogate = True
-    eng.set_laser(laser)
-    eng.build_beam(beam, seed=0)
     try:
         import torch
         sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
@@ -434,7 +458,7 @@ def main():
     if not args.cpu_primaries:
         args.cpu_primaries = 80_000 if lcfa else 4_000_000
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload not in BATCHES:
         log(f"CPU baseline on {threads} threads")
         v, sample, flags = cpu_baseline(args.workload, args.cpu_primaries, threads)
         cpu = {"value": v, "unit": "particle-steps/s", "cores": threads, "kind": "port", "sample": sample,
@@ -462,7 +486,8 @@ def main():
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic",
-        "config": {"workload": WORKLOADS[args.workload] + (f", a0={args.a0:g}" if args.a0 > 0.0 else ""),
+        "config": {"workload": WORKLOADS[args.workload] + (f", a0={args.a0:g}" if args.a0 > 0.0 else "")
+                   + (f", {pol.name.lower()} polarization" if args.workload != "lcfa" else ""),
                    "primaries": args.primaries, "precision": args.precision,
                    "sampler": "learned" if args.surrogate else "exact"},
         "roofline": roofline,
@@ -472,6 +497,8 @@ def main():
                    "emissions": totals["emissions"], "pair_events": totals["pair_events"],
                    "final_state_per_step": totals["n_secondaries"] / args.steps,
                    "generations_per_step": totals["generations"] / args.steps,
+                   "launch_mode": "eager" if args.eager else "graph",
+                   "graph_launches_per_step": totals["graph_launches"] / args.steps,
                    "kernel_ms_per_step": totals["kernel_ms"] / args.steps,
                    "lepton_kernel_ms_per_step": totals["lepton_ms"] / args.steps,
                    "second_kernel_ms_per_step": totals["photon_ms"] / args.steps,

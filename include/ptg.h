@@ -141,6 +141,8 @@ typedef struct {
     uint64_t surrogate_events;  /* emissions sampled by the surrogate */
     uint64_t surrogate_fallbacks; /* emissions outside its domain, sampled exactly */
     uint64_t graph_launches;    /* captured sequences of generations launched (PTG_LAUNCH_GRAPH) */
+    uint64_t ceiling_restarts;  /* LP emissions whose tabulated ceiling a candidate passed: sampled again under the
+                                   reference's per-event ceiling */
 } ptg_counters;
 
 /* Beam description, mirroring BeamBuilder (src/particle/builder.rs:173-219) for
@@ -340,7 +342,9 @@ typedef enum {
     PTG_FN_CPBW_PARTIAL_RATE = 25, /* (n, a, eta) -> (S3 = +1, S3 = -1)  pair_creation/cp/mod.rs:98-184 */
     PTG_FN_CPBW_HARMONIC = 26,   /* (a, eta, S3, frac) -> (n, fell_back)  pair_creation/cp/mod.rs:327-365 */
     PTG_FN_CPBW_SAMPLE = 27,     /* (a, eta, S1, S2, S3, stream, fixed_n) -> (n, s, theta, trials)  pair_creation/cp/mod.rs:188-245 */
-    PTG_FN_COUNT = 28
+    PTG_FN_LP_CEILING = 28,      /* (n, a, eta) -> (largest sample of the reference's ceiling points, its tabulated value or -1)
+                                    lp/mod.rs:53-140 */
+    PTG_FN_COUNT = 29
 } ptg_eval_fn;
 int ptg_eval(ptg_ctx* ctx, int fn, const double* in, size_t n, size_t in_w, double* out, size_t out_w);
 

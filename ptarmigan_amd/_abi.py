@@ -48,7 +48,7 @@ class Counters(C.Structure):
                 ("graph_stalls", C.c_uint32), ("kernel_ms", C.c_double), ("lepton_ms", C.c_double),
                 ("photon_ms", C.c_double), ("lepton_launches", C.c_uint64), ("photon_launches", C.c_uint64),
                 ("photon_steps_closed_form", C.c_uint64), ("surrogate_events", C.c_uint64),
-                ("surrogate_fallbacks", C.c_uint64), ("graph_launches", C.c_uint64)]
+                ("surrogate_fallbacks", C.c_uint64), ("graph_launches", C.c_uint64), ("ceiling_restarts", C.c_uint64)]
 
 
 MLP_NPARAM = 38304

@@ -825,6 +825,59 @@ __device__ inline int lp_sum_limit(double a, double eta) {
     double m = 3.0 - 0.4 * pow(eta, 0.25);
     return int(ceil(6.0 * (1.0 + a * a) + 2.0 * pow(a, m)));
 }
+// Tabulated rejection ceiling of LP emission at high harmonics. The reference
+// computes, for every emission, the largest of 32-64 samples of the
+// double-differential rate (lp/mod.rs:53-140); from harmonic 200 up each sample
+// is a J(n, x, y) quadrature of ~n nodes, and the samples are ~2/3 of an
+// event's work. That largest sample varies smoothly with (a, eta, n) within the
+// reference's regimes, so a table of it over ln a, ln eta and ln n (generated
+// by the library's own sampling code, tables/gen_lp_ceiling.py) gives the
+// ceiling by trilinear interpolation of its logarithm, times a margin kept per
+// cell: the largest ratio of the sample to the interpolated value at eight
+// interior points of the cell (x 1.02, at least 1). Cells that needed more than
+// 1.25 (where the reference changes its sample points, and near the harmonic
+// cutoff) hold 0 and leave their events to the direct ceiling. Nodes: a = 3 ...
+// 20, eta = 1e-3 ... 2, n = 200 ... 40000 (rounded to integers), all
+// log-spaced; nodes <= 0 are not tabulated (harmonics beyond twice the sum
+// limit). Layout: the nodes [NA][NE][NN], then the cell margins
+// [NA - 1][NE - 1][NN - 1].
+namespace lp_ceil {
+constexpr int NA = 33, NE = 44, NN = 110;
+constexpr double LN_A0 = 1.0986122886681098;                        // ln 3
+constexpr double LN_A_STEP = 1.8971199848858813 / double(NA - 1);   // ln (20 / 3)
+constexpr double LN_E0 = -6.907755278982137;                        // ln 1e-3
+constexpr double LN_E_STEP = 7.600902459542082 / double(NE - 1);    // ln 2000
+constexpr double LN_N0 = 5.298317366548036;                         // ln 200
+constexpr double LN_N_STEP = 5.298317366548036 / double(NN - 1);    // ln 200
+constexpr int NODES = NA * NE * NN;
+constexpr int SIZE = NODES + (NA - 1) * (NE - 1) * (NN - 1);
+}  // namespace lp_ceil
+
+// The tabulated ceiling sample (interpolated largest sample x the cell's
+// margin) at (n, a, eta); false outside the table or where the cell is not
+// tabulated.
+__device__ inline bool lp_ceiling_lookup(const double* T, int n, double a, double eta, double& m) {
+    using namespace lp_ceil;
+    const double fa = (log(a) - LN_A0) / LN_A_STEP, fe = (log(eta) - LN_E0) / LN_E_STEP;
+    const double fn = (log(double(n)) - LN_N0) / LN_N_STEP;
+    if (!(fa >= 0.0 && fa < double(NA - 1) && fe >= 0.0 && fe < double(NE - 1) && fn >= 0.0 && fn < double(NN - 1)))
+        return false;
+    const int ia = int(fa), ie = int(fe), in = int(fn);
+    const double mu = T[NODES + (ia * (NE - 1) + ie) * (NN - 1) + in];
+    if (!(mu > 0.0)) return false;
+    const double da = fa - double(ia), de = fe - double(ie), dn = fn - double(in);
+    double acc = 0.0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int xa = c & 1, xe = (c >> 1) & 1, xn = c >> 2;
+        const double v = T[((ia + xa) * NE + ie + xe) * NN + in + xn];
+        if (!(v > 0.0)) return false;
+        acc += (xa ? da : 1.0 - da) * (xe ? de : 1.0 - de) * (xn ? dn : 1.0 - dn) * log(v);
+    }
+    m = mu * exp(acc);
+    return true;
+}
+
 // The points at which lp_ceiling samples the double-differential rate
 // (lp/mod.rs:53-140): how many, and the k-th (s, theta). Split out so that a
 // wave can evaluate them in parallel.

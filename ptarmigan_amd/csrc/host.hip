@@ -111,6 +111,7 @@ struct ptg_ctx {
     double* d_lma = nullptr;
     float* d_lma_f = nullptr;  // FP32 copy of the LMA pack (mixed precision)
     LmaScalars lsc{};
+    double* d_lp_ceil = nullptr;    // tabulated LP emission ceiling (lp_ceil::); null if absent
     double* d_bw = nullptr;         // LP Breit-Wheeler pack (dev_bw.h); null if its tables are absent
     BwScalars bsc{};
     double* d_cp = nullptr;         // CP nonlinear Compton pack (dev_cp.h); null if its tables are absent
@@ -518,6 +519,14 @@ int ptg_create(ptg_ctx** out, const char* table_path, int device) {
         HIPCHK(c, hipMalloc(&c->d_lma_f, lf.size() * sizeof(float)));
         HIPCHK(c, scopy(c, c->d_lma_f, lf.data(), lf.size() * sizeof(float), hipMemcpyHostToDevice));
     }
+    {
+        // the tabulated LP ceiling, when the table blob holds it (tables/gen_lp_ceiling.py)
+        auto it = c->tables.find("nonlinear_compton/lp:CEILING_MAX");
+        if (it != c->tables.end() && it->second.size() == size_t(lp_ceil::SIZE)) {
+            HIPCHK(c, hipMalloc(&c->d_lp_ceil, it->second.size() * sizeof(double)));
+            HIPCHK(c, scopy(c, c->d_lp_ceil, it->second.data(), it->second.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+    }
     HIPCHK(c, hipMalloc(&c->d_lcfa, pack.size() * sizeof(double)));
     HIPCHK(c, scopy(c, c->d_lcfa, pack.data(), pack.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMalloc(&c->d_ctr, C_N * sizeof(unsigned long long)));
@@ -537,6 +546,7 @@ void ptg_destroy(ptg_ctx* c) {
     if (c->d_lcfa) (void)hipFree(c->d_lcfa);
     if (c->d_lma) (void)hipFree(c->d_lma);
     if (c->d_lma_f) (void)hipFree(c->d_lma_f);
+    if (c->d_lp_ceil) (void)hipFree(c->d_lp_ceil);
     c->surr[0].release();
     c->surr[1].release();
     if (c->d_surr_list) (void)hipFree(c->d_surr_list);
@@ -1413,6 +1423,7 @@ static int run_lma(ptg_ctx* c, TransportArgs a, double* ms, uint64_t* launches, 
     LmaArgs m{};
     m.tables = c->d_lma;
     m.sc = c->lsc;
+    m.lp_ceil = c->d_lp_ceil;
     m.bw = c->d_bw;
     m.bsc = c->bsc;
     m.cp = c->d_cp;
@@ -1789,6 +1800,7 @@ int ptg_run(ptg_ctx* c, const ptg_collide_options* o, uint64_t seed, ptg_counter
         cnt->surrogate_fallbacks = surr_fb + fin[C_STATS + STAT_SURR_FB];
         cnt->graph_launches = c->graph_launches;
         cnt->graph_stalls = c->graph_stalls;
+        cnt->ceiling_restarts = fin[C_STATS + STAT_CEIL_RESTART];
     }
     return PTG_OK;
 }
@@ -2122,16 +2134,16 @@ int ptg_histogram_2d(ptg_ctx* c, ptg_hist_spec* sx, ptg_hist_spec* sy, double* b
 int ptg_eval(ptg_ctx* c, int fn, const double* in, size_t n, size_t in_w, double* out, size_t out_w) {
     if (!c || !in || !out) return fail(c, PTG_E_ARG, "null argument");
     if (fn < 0 || fn >= PTG_FN_COUNT) return fail(c, PTG_E_ARG, "unknown function id");
-    static const int need_in[] = {2, 5, 1, 1, 4, 1, 3, 12, 2, 3, 4, 4, 2, 4, 6, 2, 3, 4, 4, 5, 2, 2, 2, 4, 2, 3, 4, 7};
-    static const int need_out[] = {1, 4, 2, 4, 7, 1, 1, 4, 2, 1, 5, 6, 3, 1, 4, 3, 2, 1, 2, 7, 2, 1, 1, 6, 3, 2, 2, 4};
+    static const int need_in[] = {2, 5, 1, 1, 4, 1, 3, 12, 2, 3, 4, 4, 2, 4, 6, 2, 3, 4, 4, 5, 2, 2, 2, 4, 2, 3, 4, 7, 3};
+    static const int need_out[] = {1, 4, 2, 4, 7, 1, 1, 4, 2, 1, 5, 6, 3, 1, 4, 3, 2, 1, 2, 7, 2, 1, 1, 6, 3, 2, 2, 4, 2};
     static_assert(sizeof(need_in) / sizeof(int) == PTG_FN_COUNT && sizeof(need_out) / sizeof(int) == PTG_FN_COUNT, "probe widths");
     if (int(in_w) < need_in[fn] || int(out_w) < need_out[fn]) return fail(c, PTG_E_ARG, "row width too small for this function");
     if (fn == PTG_FN_FIELDS && !c->have_laser) return fail(c, PTG_E_STATE, "no laser set");
     if (n == 0) return PTG_OK;
     if (fn >= PTG_FN_BW_RATE && fn <= PTG_FN_BW_SAMPLE && !c->d_bw) return fail(c, PTG_E_UNSUPPORTED, "Breit-Wheeler tables not loaded");
-    if (((fn >= PTG_FN_CP_RATE && fn <= PTG_FN_CP_SAMPLE) || fn >= PTG_FN_CPBW_RATE) && !c->d_cp)
+    if (((fn >= PTG_FN_CP_RATE && fn <= PTG_FN_CP_SAMPLE) || (fn >= PTG_FN_CPBW_RATE && fn <= PTG_FN_CPBW_SAMPLE)) && !c->d_cp)
         return fail(c, PTG_E_UNSUPPORTED, "CP tables not loaded");
-    if (fn >= PTG_FN_CPBW_RATE) {
+    if (fn >= PTG_FN_CPBW_RATE && fn <= PTG_FN_CPBW_SAMPLE) {
         for (size_t i = 0; i < n; ++i) {
             const double* x = in + i * in_w;
             double a = fn == PTG_FN_CPBW_PARTIAL_RATE ? x[1] : x[0], eta = fn == PTG_FN_CPBW_PARTIAL_RATE ? x[2] : x[1];
@@ -2143,6 +2155,14 @@ int ptg_eval(ptg_ctx* c, int fn, const double* in, size_t n, size_t in_w, double
                 return fail(c, PTG_E_ARG, "CP Breit-Wheeler sample probe needs fixed n = 0 or 1 <= n <= 1e6");
             if (fn != PTG_FN_CPBW_PARTIAL_RATE && a < 0.3 && 2.0 * (0.25 + a * a) / eta > 1.0e5)
                 return fail(c, PTG_E_ARG, "CP Breit-Wheeler probe: more than 1e5 harmonics to sum");
+        }
+    }
+    if (fn == PTG_FN_LP_CEILING) {
+        for (size_t i = 0; i < n; ++i) {
+            const double* x = in + i * in_w;
+            if (!(x[0] >= 2.0 && x[0] <= 2.0e5 && double(int(x[0])) == x[0] && x[1] > 0.0 && x[1] <= 100.0 && x[2] > 0.0 &&
+                  x[2] <= 100.0))
+                return fail(c, PTG_E_ARG, "LP ceiling probe needs an integer 2 <= n <= 2e5, 0 < a <= 100, 0 < eta <= 100");
         }
     }
     if (fn == PTG_FN_BESSEL_J || fn == PTG_FN_CP_PARTIAL_RATE || fn == PTG_FN_CP_SAMPLE) {
@@ -2195,6 +2215,7 @@ int ptg_eval(ptg_ctx* c, int fn, const double* in, size_t n, size_t in_w, double
         LmaArgs la{};
         la.tables = c->d_lma;
         la.sc = c->lsc;
+        la.lp_ceil = c->d_lp_ceil;
         la.bw = c->d_bw;
         la.bsc = c->bsc;
         la.cp = c->d_cp;
@@ -2443,6 +2464,7 @@ int ptg_surrogate_train(ptg_ctx* c, const ptg_surrogate_train_desc* d, ptg_surro
     LmaArgs m{};
     m.tables = c->d_lma;
     m.sc = c->lsc;
+    m.lp_ceil = c->d_lp_ceil;
     m.cp = c->d_cp;
     m.csc = c->csc;
     m.lcfa = c->d_lcfa;

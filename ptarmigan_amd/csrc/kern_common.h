@@ -11,7 +11,7 @@
 namespace ptg {
 
 enum { STAT_LEPTON_STEPS = 0, STAT_PHOTON_STEPS, STAT_EMISSIONS, STAT_PAIRS, STAT_TRIALS, STAT_PHOTON_STEPS_CF, STAT_SURROGATE,
-       STAT_SURR_FB, STAT_N };
+       STAT_SURR_FB, STAT_CEIL_RESTART, STAT_N };
 
 // Counter block of a context (unsigned long long words): work and queue counters
 // of the transport, then the statistics. C_NCUR, C_GENS, C_NDEF, C_EVRUN and

@@ -14,6 +14,7 @@ enum LmaMode : int { LMA_LP = 0, LMA_CP = 1, LMA_CP_CLASSICAL = 2, LMA_LP_CLASSI
 struct LmaArgs {
     const double* tables;        // LMA pack (dev_lma.h)
     LmaScalars sc;
+    const double* lp_ceil;       // tabulated LP emission ceiling (lp_ceil:: in dev_lma.h), or null
     const double* cp;            // CP pack (dev_cp.h)
     CpScalars csc;
     const double* lcfa;          // LCFA pack (Airy functions for J_n at n >= 1000)

@@ -360,7 +360,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
 // acceptance from 17 + 2t.
 template <bool classical>
 __device__ __noinline__ void lp_sample_wave(int nh, double alp, double eta, const Rng& base, double* tb_s, double* tb_f,
-                                            double& s, double& theta, sv4& sv, uint64_t& trials, uint32_t& flags) {
+                                            const double* ceil_tab, double& s, double& theta, sv4& sv, uint64_t& trials,
+                                            uint32_t& flags, uint64_t& restarts) {
     const int lane = threadIdx.x;
     const uint32_t cap = 1u << 20;
     __syncthreads();  // the previous event's bound is no longer read
@@ -391,18 +392,26 @@ __device__ __noinline__ void lp_sample_wave(int nh, double alp, double eta, cons
     double v_acc = 0.0, th_acc = 0.0;
     if (nh >= PTG_LMA_SHARED_N) {  // every lane holds the same values from here
         const DBesselWave W{nh};
-        double f = classical ? NAN : -INFINITY;
-        for (int k = 0; k < m; ++k) {
-            double sk, tk;
-            if (classical) {
-                lp_cl_ceiling_arg(nh, alp, k, sk, tk);
-                f = fmax(f, lp_cl_ddpr(W, alp, sk, tk));
-            } else {
-                lp_ceiling_arg(nh, alp, eta, k, sk, tk);
-                f = fmax(f, lp_ddpr(W, alp, eta, sk, tk));
+        // the ceiling from the table where it covers (n, a, eta); a candidate whose
+        // rate passes it (the tabulated envelope short there) restarts the event's
+        // rejection from candidate 0 under the reference's own ceiling
+        double tab = 0.0;
+        bool from_table = !classical && ceil_tab && lp_ceiling_lookup(ceil_tab, nh, alp, eta, tab);
+        auto direct = [&]() {
+            double f = classical ? NAN : -INFINITY;
+            for (int k = 0; k < m; ++k) {
+                double sk, tk;
+                if (classical) {
+                    lp_cl_ceiling_arg(nh, alp, k, sk, tk);
+                    f = fmax(f, lp_cl_ddpr(W, alp, sk, tk));
+                } else {
+                    lp_ceiling_arg(nh, alp, eta, k, sk, tk);
+                    f = fmax(f, lp_ddpr(W, alp, eta, sk, tk));
+                }
             }
-        }
-        const double mx = (classical ? 1.1 : 1.15) * f;
+            return (classical ? 1.1 : 1.15) * f;
+        };
+        double mx = from_table ? 1.15 * tab : direct();
         for (uint32_t t = 0; first < 0; ++t) {
             Rng gc = base;
             gc.c0 = 16u + 2u * t;
@@ -420,6 +429,14 @@ __device__ __noinline__ void lp_sample_wave(int nh, double alp, double eta, cons
             if (th > theta_bound_at(tb_s, tb_f, classical ? v : smax * v)) continue;
             const double uz = rng_uniform(gc);
             const double fc = classical ? lp_cl_ddpr(W, alp, v, th) : lp_ddpr(W, alp, eta, smax * v, th);
+            if (from_table && fc > mx) {
+                ++restarts;
+                trials += t + 1;
+                mx = direct();
+                from_table = false;
+                t = ~0u;  // the loop's increment starts again at candidate 0
+                continue;
+            }
             if (mx * uz < fc) {
                 v_acc = v;
                 th_acc = th;
@@ -501,7 +518,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PTG_LMA_WAVE
     const int lane = threadIdx.x;
     const double* cdf = M.tables + lma_pack::CDF;
     __shared__ double tb_s[16], tb_f[16];  // the event's theta bound, one copy per wave
-    uint64_t n_emit = 0, trials = 0;
+    uint64_t n_emit = 0, trials = 0, restarts = 0;
     uint32_t flags = 0;
     const uint64_t n_ev = M.n_ev_dev ? *M.n_ev_dev : M.n_ev;
     for (uint64_t e = blockIdx.x; e < n_ev; e += gridDim.x) {
@@ -525,13 +542,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PTG_LMA_WAVE
         double s = 0.0, theta = 0.0;
         sv4 sv{1.0, 0.0, 0.0, 0.0};
         if (ok) {
-            lp_sample_wave<classical>(nh, alp, eta, base, tb_s, tb_f, s, theta, sv, trials, flags);
+            lp_sample_wave<classical>(nh, alp, eta, base, tb_s, tb_f, M.lp_ceil, s, theta, sv, trials, flags, restarts);
         }
         if (lane == 0) lma_emit_finish<classical>(A, M, i, k0, a, eta, ok, nh, s, theta, sv, n_emit, flags);
     }
     if (lane == 0) {
         wave_add(A->stats + STAT_EMISSIONS, n_emit);
         wave_add(A->stats + STAT_TRIALS, trials);
+        wave_add(A->stats + STAT_CEIL_RESTART, restarts);
         if (flags) atomicOr(A->flags, flags);
     }
 }
@@ -1608,8 +1626,37 @@ __global__ __launch_bounds__(64) void k_eval_lp_sample(const double* in, double*
         sv4 pol;
         uint64_t tr = 0;
         uint32_t flags = 0;
-        lp_sample_wave<classical>(nh, x[0], x[1], rng, tb_s, tb_f, s, th, pol, tr, flags);
-        if (threadIdx.x == 0) { y[0] = nh; y[1] = s; y[2] = th; y[3] = pol.q; y[4] = pol.u; y[5] = double(tr); }
+        uint64_t restarts = 0;
+        lp_sample_wave<classical>(nh, x[0], x[1], rng, tb_s, tb_f, m.lp_ceil, s, th, pol, tr, flags, restarts);
+        if (threadIdx.x == 0) {
+            y[0] = nh; y[1] = s; y[2] = th; y[3] = pol.q; y[4] = pol.u; y[5] = double(tr);
+            if (out_w > 6) y[6] = double(restarts);
+        }
+    }
+}
+
+// The LP ceiling probe, one row per wave: (n, a, eta) -> the largest of the
+// reference's ceiling samples of the double-differential rate (lp/mod.rs:53-140,
+// without its factor 1.15; J(n, x, y) shared by the wave as in the sampler), and
+// the tabulated value at (n, a, eta), -1 where the table does not cover it.
+__global__ __launch_bounds__(64) void k_eval_lp_ceiling(const double* in, double* out, uint32_t n, uint32_t in_w, uint32_t out_w,
+                                                        LmaArgs m) {
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const double* x = in + uint64_t(i) * in_w;
+        double* y = out + uint64_t(i) * out_w;
+        const int nh = int(x[0]);
+        const double a = x[1], eta = x[2];
+        const DBesselWave W{nh};
+        const int mm = lp_ceiling_count(nh, a, eta);
+        double f = -INFINITY;
+        for (int k = 0; k < mm; ++k) {
+            double sk, tk;
+            lp_ceiling_arg(nh, a, eta, k, sk, tk);
+            f = fmax(f, lp_ddpr(W, a, eta, sk, tk));
+        }
+        double t = -1.0;
+        if (!(m.lp_ceil && lp_ceiling_lookup(m.lp_ceil, nh, a, eta, t))) t = -1.0;
+        if (threadIdx.x == 0) { y[0] = f; y[1] = t; }
     }
 }
 
@@ -1706,6 +1753,10 @@ void launch_eval_lma(int fn, const double* in, double* out, uint32_t n, uint32_t
     }
     if (fn == PTG_FN_CPBW_HARMONIC || fn == PTG_FN_CPBW_SAMPLE) {
         k_eval_cpbw_wave<<<rows, 64, 0, s>>>(fn, in, out, n, in_w, out_w, m, key);
+        return;
+    }
+    if (fn == PTG_FN_LP_CEILING) {
+        k_eval_lp_ceiling<<<rows, 64, 0, s>>>(in, out, n, in_w, out_w, m);
         return;
     }
     if (fn == PTG_FN_LP_SAMPLE || fn == PTG_FN_LP_CL_SAMPLE) {

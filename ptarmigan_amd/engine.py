@@ -427,3 +427,4 @@ class Fn(enum.IntEnum):
     CPBW_PARTIAL_RATE = 25
     CPBW_HARMONIC = 26
     CPBW_SAMPLE = 27
+    LP_CEILING = 28

@@ -140,6 +140,11 @@ GENERATED = [
      "regenerated by tables/gen_bw_tables.py (reference generator: src/pair_creation/lp/mod.rs:1065-1163)"),
     ("nonlinear_compton/cp/tables/cdf:TABLE", "nlc_cp_cdf.f64", [4270, 32, 2],
      "regenerated by tables/gen_cp_tables.py (reference generator: src/nonlinear_compton/cp/mod.rs:474-612)"),
+    # not a reference table: the largest of the reference's per-emission ceiling samples
+    # (src/nonlinear_compton/lp/mod.rs:53-140) over a grid, for the tabulated ceiling
+    ("nonlinear_compton/lp:CEILING_MAX", "nlc_lp_ceiling.f64", [33 * 44 * 110 + 32 * 43 * 109],
+     "generated by tables/gen_lp_ceiling.py with the library's sampling code (ceiling samples of "
+     "src/nonlinear_compton/lp/mod.rs:53-140)"),
 ]
 
 

@@ -19,6 +19,16 @@ namespace ptg {
 // (c0, c1 = k); sampling event k uses (c0, c1 = k | EVENT_BLOCK).
 constexpr uint32_t EVENT_BLOCK = 0x80000000u;
 
+// Lanes of the persistent step kernels: stepping, holding a particle that left the
+// field or reached its event (record not yet written), without a particle, or
+// with no input left.
+enum : uint32_t { LANE_ACTIVE = 0, LANE_EXIT = 1, LANE_EVENT = 2, LANE_EMPTY = 3, LANE_DONE = 4 };
+// waiting lanes that trigger the wave's write-out and refill (8, 16, 32 measured on the
+// a0 = 2 plane wave, the learned-sampler domain and the scan: 32 steps fastest)
+#ifndef PTG_LMA_REFILL
+#define PTG_LMA_REFILL 32
+#endif
+
 // rate table staged in LDS per emission model: LP [68][61], CP [70][61]; the
 // classical tables (61 values) are read through the cache
 template <int MODE>
@@ -35,7 +45,12 @@ struct StepTable {
 template <int MODE, bool MIX>
 __global__ __launch_bounds__(256) void k_step_lma(const TransportArgs* __restrict__ A, LmaArgs M) {
     using TT = typename std::conditional<MIX, float, double>::type;
-    const LaserDev* Lp = &A->laser;  // a batch of collisions: the particle's own laser
+    // The launch's laser, read once through the uniform argument pointer (its fields
+    // stay in scalar registers); the lasers of a batch differ only in a0 (ptg_set_batch),
+    // which each particle takes from its own collision when it is fetched. Reading the
+    // whole laser through a per-particle pointer instead cost ~25 vector loads per
+    // step, and the waves spent most of their time waiting on them.
+    LaserDev L = A->laser;
     const double dt = A->dt;
     const double kc = C_LIGHT * T_COMPTON;
     __shared__ TT rt[StepTable<MODE>::N];
@@ -62,7 +77,7 @@ __global__ __launch_bounds__(256) void k_step_lma(const TransportArgs* __restric
             meta = in.meta[i];
             if ((meta & 3u) == 2u) { forward_record(*A, i); continue; }
             idx = i;
-            if (M.coll) Lp = M.lasers + M.coll[in.shower[i]];
+            if (M.coll) L.a0 = M.lasers[M.coll[in.shower[i]]].a0;
             r = {in.d[P_RT][i], in.d[P_RX][i], in.d[P_RY][i], in.d[P_RZ][i]};
             u = {in.d[P_UT][i], in.d[P_UX][i], in.d[P_UY][i], in.d[P_UZ][i]};
             work = in.d[P_WORK][i];
@@ -77,55 +92,76 @@ __global__ __launch_bounds__(256) void k_step_lma(const TransportArgs* __restric
             return true;
         }
     };
-    bool live = fetch();
-    while (live) {
-        const LaserDev& L = *Lp;
-        bool inside = !skip && field_contains(L, r) && r.t / C_LIGHT < A->opt.t_stop;
-        if (inside && psteps >= A->step_cap) { flags |= PTG_FLAG_STEP_CAP; inside = false; }
-        if (!inside) {
-            double icount = A->in.d[P_ICOUNT][idx];
-            finalize_lepton(*A, idx, r, u.t, sp(u), work, icount, meta);
-            live = fetch();
+    // A lane whose particle leaves the field or reaches its event holds it (its final
+    // state or event record unwritten) while the other lanes step on; once a quarter
+    // of the wave waits, or no lane steps, the waiting lanes write their records and
+    // take new particles together, so the atomics and record loads of a refill are
+    // paid once per refill rather than in nearly every step (a particle's stretch
+    // between events is ~10^2 steps, so some lane of 64 ends one in most steps).
+    uint32_t st = LANE_EMPTY;
+    double k0_ev = 0.0;
+    for (;;) {
+        if (st == LANE_ACTIVE) {
+            bool inside = !skip && field_contains(L, r) && r.t / C_LIGHT < A->opt.t_stop;
+            if (inside && psteps >= A->step_cap) { flags |= PTG_FLAG_STEP_CAP; inside = false; }
+            if (!inside) {
+                st = LANE_EXIT;
+            } else {
+                ++n_steps;
+                ++psteps;
+                if (MIX && L.kind == K_PLANE_WAVE) work += pw_push_mixed(L, r, u, rqm, dt, A->rr_mode == 1);
+                else work += lma_push(L, r, u, rqm, dt, A->rr_mode == 1);
+                // PlaneWave::radiate (plane_wave.rs:251-282)
+                double phase = L.k0 * (r.t - r.z);
+                double chirp = 1.0 + 2.0 * L.chirp_b * phase;
+                double width = 1.0;
+                if (M.bandwidth != 0.0) {
+                    double u1, u2;
+                    rng_uniform2(rng, u1, u2);
+                    double sn, cs;
+                    sincos(2.0 * PI_D * u2, &sn, &cs);
+                    width = 1.0 + M.bandwidth * sqrt(-2.0 * log(1.0 - u1)) * cs;
+                }
+                double k0 = kc * L.k0 * chirp * width;  // kappa = k0 (1, 0, 0, 1)
+                double a_sqd_q = u.t * u.t - u.x * u.x - u.y * u.y - u.z * u.z - 1.0;
+                double a_rms = a_sqd_q >= 0.0 ? sqrt(a_sqd_q) : 0.0;
+                double eta = k0 * (u.t - u.z);
+                double dphi = dt * eta / (T_COMPTON * u.t);
+                double f;
+                double prob = 0.0;
+                bool ok;
+                if constexpr (MIX && (MODE == LMA_LP || MODE == LMA_CP)) {
+                    float ff = 0.0f;
+                    if (MODE == LMA_LP) ok = lp_rate_f(rt, M.sc, a_rms * 1.4142135623730951, eta, ff);
+                    else ok = cp_rate_f(rt, M.cp, M.csc, a_rms, eta, ff);
+                    if (ok) prob = double(float(ALPHA) * ff * (float(dphi) / float(eta)));
+                } else {
+                    if (MODE == LMA_LP) ok = lp_rate(rt, M.sc, a_rms * 1.4142135623730951, eta, f);
+                    else if (MODE == LMA_CP) ok = cp_rate(rt, M.cp, M.csc, a_rms, eta, f);
+                    else if (MODE == LMA_CP_CLASSICAL) ok = cp_classical_rate(M.cp, M.csc, a_rms, eta, f);
+                    else ok = lp_cl_rate(M.tables, M.sc, a_rms * 1.4142135623730951, eta, f);
+                    if (ok) prob = ALPHA * f * dphi / eta;
+                }
+                if (!ok) flags |= PTG_FLAG_TABLE_RANGE;
+                S *= (1.0 - prob);
+                if (S < U) {
+                    st = LANE_EVENT;
+                    k0_ev = k0;
+                }
+            }
+        }
+        const uint64_t waiting = __ballot(st == LANE_EXIT || st == LANE_EVENT || st == LANE_EMPTY);
+        const uint64_t active = __ballot(st == LANE_ACTIVE);
+        if (!waiting) {
+            if (!active) break;
             continue;
         }
-        ++n_steps;
-        ++psteps;
-        if (MIX && L.kind == K_PLANE_WAVE) work += pw_push_mixed(L, r, u, rqm, dt, A->rr_mode == 1);
-        else work += lma_push(L, r, u, rqm, dt, A->rr_mode == 1);
-        // PlaneWave::radiate (plane_wave.rs:251-282)
-        double phase = L.k0 * (r.t - r.z);
-        double chirp = 1.0 + 2.0 * L.chirp_b * phase;
-        double width = 1.0;
-        if (M.bandwidth != 0.0) {
-            double u1, u2;
-            rng_uniform2(rng, u1, u2);
-            double sn, cs;
-            sincos(2.0 * PI_D * u2, &sn, &cs);
-            width = 1.0 + M.bandwidth * sqrt(-2.0 * log(1.0 - u1)) * cs;
-        }
-        double k0 = kc * L.k0 * chirp * width;  // kappa = k0 (1, 0, 0, 1)
-        double a_sqd_q = u.t * u.t - u.x * u.x - u.y * u.y - u.z * u.z - 1.0;
-        double a_rms = a_sqd_q >= 0.0 ? sqrt(a_sqd_q) : 0.0;
-        double eta = k0 * (u.t - u.z);
-        double dphi = dt * eta / (T_COMPTON * u.t);
-        double f;
-        double prob = 0.0;
-        bool ok;
-        if constexpr (MIX && (MODE == LMA_LP || MODE == LMA_CP)) {
-            float ff = 0.0f;
-            if (MODE == LMA_LP) ok = lp_rate_f(rt, M.sc, a_rms * 1.4142135623730951, eta, ff);
-            else ok = cp_rate_f(rt, M.cp, M.csc, a_rms, eta, ff);
-            if (ok) prob = double(float(ALPHA) * ff * (float(dphi) / float(eta)));
-        } else {
-            if (MODE == LMA_LP) ok = lp_rate(rt, M.sc, a_rms * 1.4142135623730951, eta, f);
-            else if (MODE == LMA_CP) ok = cp_rate(rt, M.cp, M.csc, a_rms, eta, f);
-            else if (MODE == LMA_CP_CLASSICAL) ok = cp_classical_rate(M.cp, M.csc, a_rms, eta, f);
-            else ok = lp_cl_rate(M.tables, M.sc, a_rms * 1.4142135623730951, eta, f);
-            if (ok) prob = ALPHA * f * dphi / eta;
-        }
-        if (!ok) flags |= PTG_FLAG_TABLE_RANGE;
-        S *= (1.0 - prob);
-        if (S < U) {
+        if (active && __popcll(waiting) < PTG_LMA_REFILL) continue;
+        if (st == LANE_EXIT) {
+            double icount = A->in.d[P_ICOUNT][idx];
+            finalize_lepton(*A, idx, r, u.t, sp(u), work, icount, meta);
+            st = LANE_EMPTY;
+        } else if (st == LANE_EVENT) {
             // suspend: queue the particle with the event parameters
             unsigned long long j = wave_reserve(M.ev_ctr, 1u);
             if (j < M.ev_cap) {
@@ -136,12 +172,12 @@ __global__ __launch_bounds__(256) void k_step_lma(const TransportArgs* __restric
                 q.d[P_WORK][j] = work;
                 q.d[P_ICOUNT][j] = A->in.d[P_ICOUNT][idx];
                 q.nev[j] = nev;
-                M.ev_k0[j] = k0;
+                M.ev_k0[j] = k0_ev;
                 M.ev_a[j] = sqrt(lma_a_sqd(L, r));
             }
-            live = fetch();
-            continue;
+            st = LANE_EMPTY;
         }
+        if (st == LANE_EMPTY) st = fetch() ? LANE_ACTIVE : LANE_DONE;
     }
     wave_add(A->stats + STAT_LEPTON_STEPS, n_steps);
     if (flags) atomicOr(A->flags, flags);
@@ -581,7 +617,7 @@ __device__ __forceinline__ double lma_photon_prob(const LmaArgs& M, const LaserD
 
 template <bool CIRC>
 __global__ __launch_bounds__(256) void k_photon_step_lma(const TransportArgs* __restrict__ A, LmaArgs M) {
-    const LaserDev* Lp = &A->laser;
+    LaserDev L = A->laser;  // uniform; a batch's per-collision a0 is taken at fetch (as k_step_lma)
     const double dt = A->dt, cdt = C_LIGHT * dt;
     uint64_t n_steps = 0;
     uint32_t flags = 0, psteps = 0;
@@ -600,7 +636,7 @@ __global__ __launch_bounds__(256) void k_photon_step_lma(const TransportArgs* __
             meta = in.meta[i];
             if ((meta & 3u) != 2u) { forward_record(*A, i); continue; }
             idx = i;
-            if (M.coll) Lp = M.lasers + M.coll[in.shower[i]];
+            if (M.coll) L.a0 = M.lasers[M.coll[in.shower[i]]].a0;
             r = {in.d[P_RT][i], in.d[P_RX][i], in.d[P_RY][i], in.d[P_RZ][i]};
             k = {in.d[P_UT][i], in.d[P_UX][i], in.d[P_UY][i], in.d[P_UZ][i]};
             pol = {in.d[P_SI][i], in.d[P_SQ][i], in.d[P_SU][i], in.d[P_SV][i]};
@@ -616,30 +652,50 @@ __global__ __launch_bounds__(256) void k_photon_step_lma(const TransportArgs* __
             return true;
         }
     };
-    bool live = fetch();
-    while (live) {
-        const LaserDev& L = *Lp;
-        bool inside = !skip && field_contains(L, r) && r.t / C_LIGHT < A->opt.t_stop;
-        if (inside && psteps >= A->step_cap) { flags |= PTG_FLAG_STEP_CAP; inside = false; }
-        if (!inside) {
-            finalize_photon(*A, idx, r, k, pol, weight, icount, meta, false);
-            live = fetch();
+    // finished lanes wait and are written out and refilled together (as k_step_lma)
+    uint32_t st = LANE_EMPTY;
+    double k0_ev = 0.0, a_ev = 0.0;
+    for (;;) {
+        if (st == LANE_ACTIVE) {
+            bool inside = !skip && field_contains(L, r) && r.t / C_LIGHT < A->opt.t_stop;
+            if (inside && psteps >= A->step_cap) { flags |= PTG_FLAG_STEP_CAP; inside = false; }
+            if (!inside) {
+                st = LANE_EXIT;
+            } else {
+                ++n_steps;
+                ++psteps;
+                double ik = cdt / k.t;
+                r = {r.t + cdt, r.x + ik * k.x, r.y + ik * k.y, r.z + ik * k.z};
+                sv4 pin = A->opt.pol_resolved ? pol : sv4{1.0, 0.0, 0.0, 0.0};
+                double a, k0, eta;
+                sv4 pnew;
+                bool in_table;
+                // outside both rate tables the probability is 0 without a diagnostic (lp/mod.rs:388-397)
+                double prob = lma_photon_prob<CIRC>(M, L, dt, r, k, pin, a, k0, eta, pnew, in_table);
+                double ri = A->opt.rate_increase;
+                if (prob * ri > 0.1) ri = 0.1 / prob;
+                S *= (1.0 - prob * ri);
+                if (S < U) {
+                    st = LANE_EVENT;
+                    k0_ev = k0;
+                    a_ev = a;
+                } else {
+                    if (A->opt.rotate_stokes_pars && A->opt.pol_resolved) pol = pnew;
+                    icount += prob;
+                }
+            }
+        }
+        const uint64_t waiting = __ballot(st == LANE_EXIT || st == LANE_EVENT || st == LANE_EMPTY);
+        const uint64_t active = __ballot(st == LANE_ACTIVE);
+        if (!waiting) {
+            if (!active) break;
             continue;
         }
-        ++n_steps;
-        ++psteps;
-        double ik = cdt / k.t;
-        r = {r.t + cdt, r.x + ik * k.x, r.y + ik * k.y, r.z + ik * k.z};
-        sv4 pin = A->opt.pol_resolved ? pol : sv4{1.0, 0.0, 0.0, 0.0};
-        double a, k0, eta;
-        sv4 pnew;
-        bool in_table;
-        // outside both rate tables the probability is 0 without a diagnostic (lp/mod.rs:388-397)
-        double prob = lma_photon_prob<CIRC>(M, L, dt, r, k, pin, a, k0, eta, pnew, in_table);
-        double ri = A->opt.rate_increase;
-        if (prob * ri > 0.1) ri = 0.1 / prob;
-        S *= (1.0 - prob * ri);
-        if (S < U) {
+        if (active && __popcll(waiting) < PTG_LMA_REFILL) continue;
+        if (st == LANE_EXIT) {
+            finalize_photon(*A, idx, r, k, pol, weight, icount, meta, false);
+            st = LANE_EMPTY;
+        } else if (st == LANE_EVENT) {
             unsigned long long j = wave_reserve(M.ev_ctr, 1u);
             if (j < M.ev_cap) {
                 const PBuf& q = M.ev;
@@ -651,14 +707,12 @@ __global__ __launch_bounds__(256) void k_photon_step_lma(const TransportArgs* __
                 q.d[P_ICOUNT][j] = icount;
                 q.d[P_WEIGHT][j] = weight;
                 q.nev[j] = nev;
-                M.ev_k0[j] = k0;
-                M.ev_a[j] = a;
+                M.ev_k0[j] = k0_ev;
+                M.ev_a[j] = a_ev;
             }
-            live = fetch();
-            continue;
+            st = LANE_EMPTY;
         }
-        if (A->opt.rotate_stokes_pars && A->opt.pol_resolved) pol = pnew;
-        icount += prob;
+        if (st == LANE_EMPTY) st = fetch() ? LANE_ACTIVE : LANE_DONE;
     }
     wave_add(A->stats + STAT_PHOTON_STEPS, n_steps);
     if (flags) atomicOr(A->flags, flags);

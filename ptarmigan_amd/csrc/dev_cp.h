@@ -110,8 +110,8 @@ PTG_HD inline double bessel_j_asymptotic(TP tb, int n, double x) {
     double zeta = bessel_zeta(z);
     double m13 = cbrt(m), m23 = m13 * m13;
     double arg = m23 * zeta;
-    double ai = arg < 0.0 ? 0.0 : airy_ai(tb, arg);
-    double aip = arg < 0.0 ? 0.0 : airy_ai_prime(tb, arg);
+    double ai = 0.0, aip = 0.0;
+    if (!(arg < 0.0)) airy_pair(tb, arg, ai, aip);
     double w = 1.0 - z * z;
     double prefactor = 4.0 * zeta / w;
     double bk = 5.0 / (48.0 * zeta * zeta) - (2.0 + 3.0 * z * z) / (24.0 * sqrt(zeta) * (w * sqrt(w)));

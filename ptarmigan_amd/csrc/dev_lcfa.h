@@ -19,45 +19,91 @@ using TP = const double*;
 // quadrature above (Gil, Segura & Temme 2002), node count chosen by x
 // (special_functions/airy.rs:41-116).
 PTG_HD __forceinline__ double airy_series(TP a, double x) {
-    // sum of a[k] x^n[k] over the 20 tabulated terms with n < 20
-    double acc = 0.0;
+    // sum of a[k] x^n[k] over the 20 tabulated terms with n < 20; the powers
+    // ascend, and x^n[k] is carried over from x^n[k-1] by the same products
+    // 1 * x * x ... that a fresh power would take
+    double acc = 0.0, xp = 1.0;
+    int have = 0;
     for (int k = 0; k < 20; ++k) {
         int p = int(a[20 + k]);
         if (p >= 20) break;
-        double xp = 1.0;
-        for (int j = 0; j < p; ++j) xp *= x;
+        for (; have < p; ++have) xp *= x;
         acc += a[k] * xp;
     }
     return acc;
 }
-PTG_HD __forceinline__ double gl_pow_sum(TP nodes, TP weights, int n, double s, double expo) {
-    double acc = 0.0;
-    double inv_s = 1.0 / s;
-    for (int k = 0; k < n; ++k) acc += weights[k] * exp(expo * log(2.0 + nodes[k] * inv_s));
-    return acc;
+// Above x = 1 the reference writes Ai = c s^(-1/6) e^(-s) F(1/s) and
+// Ai' = -c' s^(1/6) e^(-s) G(1/s), s = 2 x^(3/2) / 3, with F and G sums of
+// w_k (2 + x_k / s)^(-/+1/6) over 40, 16 or 4 nodes (Ai; x < 2, < 10, >= 10) and
+// 32, 16, 8 or 4 (Ai'; x < 2, < 4, < 11, >= 11). Each sum is a smooth function
+// of t = 1/s on its interval; AIRY_CHEB holds its Chebyshev expansion of degree
+// 15 on one or two segments (tables/gen_airy_cheb.py: truncation error below
+// 1e-17 relative), so a sum costs one Clenshaw recurrence of 16 terms instead
+// of a logarithm and an exponential per node. Segment record: t_lo, c, 1/h,
+// a_0 .. a_15, in u = (t - c) / h.
+namespace airy_cheb {
+constexpr int REC = 19;
+constexpr int AI_2 = 0, AI_10 = 2, AI_INF = 4;                 // Ai: first segment of x < 2, < 10, >= 10
+constexpr int AP_2 = 5, AP_4 = 7, AP_11 = 8, AP_INF = 9;       // Ai': x < 2, < 4, < 11, >= 11
+}  // namespace airy_cheb
+PTG_HD __forceinline__ double airy_cheb_sum(TP seg, double t) {
+    const double u = (t - seg[1]) * seg[2];
+    const double u2 = 2.0 * u;
+    double b1 = 0.0, b2 = 0.0;
+#pragma unroll
+    for (int k = 15; k >= 1; --k) {
+        const double b0 = u2 * b1 - b2 + seg[3 + k];
+        b2 = b1;
+        b1 = b0;
+    }
+    return u * b1 - b2 + seg[3];
+}
+// 0.262183997088323 s^(-1/6) and -0.2898380090915846 s^(1/6) (airy.rs:53,110) with
+// s^(1/6) = (2/3)^(1/6) x^(1/4), x^(1/4) by two square roots
+constexpr double AIRY_C_AI = 0.28051411772306184;     // 0.262183997088323 (2/3)^(-1/6)
+constexpr double AIRY_C_AIP = -0.2708986212479171;   // -0.2898380090915846 (2/3)^(1/6)
+PTG_HD __forceinline__ void airy_large(TP tb, double x, bool want_ai, bool want_aip, double& ai, double& aip) {
+    using namespace lcfa_pack;
+    const double rx = sqrt(x);
+    const double s = 2.0 * x * rx / 3.0;
+    const double t = 1.0 / s;
+    const double q = sqrt(rx);
+    const double e = exp(-s);
+    TP C = tb + AIRY_CHEB;
+    if (want_ai) {
+        int seg = x < 2.0 ? airy_cheb::AI_2 : (x < 10.0 ? airy_cheb::AI_10 : airy_cheb::AI_INF);
+        if (seg != airy_cheb::AI_INF && t >= C[(seg + 1) * airy_cheb::REC]) ++seg;
+        ai = AIRY_C_AI / q * e * airy_cheb_sum(C + seg * airy_cheb::REC, t);
+    }
+    if (want_aip) {
+        int seg = x < 2.0 ? airy_cheb::AP_2 : (x < 4.0 ? airy_cheb::AP_4 : (x < 11.0 ? airy_cheb::AP_11 : airy_cheb::AP_INF));
+        if (seg == airy_cheb::AP_2 && t >= C[(seg + 1) * airy_cheb::REC]) ++seg;
+        aip = AIRY_C_AIP * q * e * airy_cheb_sum(C + seg * airy_cheb::REC, t);
+    }
 }
 PTG_HD inline double airy_ai(TP tb, double x) {
     using namespace lcfa_pack;
     if (x < 1.0) return airy_series(tb + AIRY_SERIES, x);
-    double s = 2.0 * x * sqrt(x) / 3.0;
-    double a = 0.262183997088323 * pow(s, -1.0 / 6.0) * exp(-s);
-    double integral;
-    if (x < 2.0) integral = gl_pow_sum(tb + AIRY_N40, tb + AIRY_W40, 40, s, -1.0 / 6.0);
-    else if (x < 10.0) integral = gl_pow_sum(tb + AIRY_N16, tb + AIRY_W16, 16, s, -1.0 / 6.0);
-    else integral = gl_pow_sum(tb + AIRY_N4, tb + AIRY_W4, 4, s, -1.0 / 6.0);
-    return a * integral;
+    double ai, aip;
+    airy_large(tb, x, true, false, ai, aip);
+    return ai;
 }
 PTG_HD inline double airy_ai_prime(TP tb, double x) {
     using namespace lcfa_pack;
     if (x < 1.0) return airy_series(tb + AIRYP_SERIES, x);
-    double s = 2.0 * x * sqrt(x) / 3.0;
-    double b = -0.2898380090915846 * pow(s, 1.0 / 6.0) * exp(-s);
-    double integral;
-    if (x < 2.0) integral = gl_pow_sum(tb + AIRYP_N32, tb + AIRYP_W32, 32, s, 1.0 / 6.0);
-    else if (x < 4.0) integral = gl_pow_sum(tb + AIRYP_N16, tb + AIRYP_W16, 16, s, 1.0 / 6.0);
-    else if (x < 11.0) integral = gl_pow_sum(tb + AIRYP_N8, tb + AIRYP_W8, 8, s, 1.0 / 6.0);
-    else integral = gl_pow_sum(tb + AIRYP_N4, tb + AIRYP_W4, 4, s, 1.0 / 6.0);
-    return b * integral;
+    double ai, aip;
+    airy_large(tb, x, false, true, ai, aip);
+    return aip;
+}
+// both at one argument (the uniform expansion of J_n needs the pair)
+PTG_HD inline void airy_pair(TP tb, double x, double& ai, double& aip) {
+    using namespace lcfa_pack;
+    if (x < 1.0) {
+        ai = airy_series(tb + AIRY_SERIES, x);
+        aip = airy_series(tb + AIRYP_SERIES, x);
+        return;
+    }
+    airy_large(tb, x, true, true, ai, aip);
 }
 // K_{1/3}(x) = pi (2 sqrt3 / x)^{1/3} Ai((1.5x)^{2/3});  K_{2/3}(x) = -pi 3^{-1/6} (2/x)^{2/3} Ai'(...)
 // (airy.rs:118-136). Non-positive arguments give 0, as the reference's None -> unwrap_or(0.0).

@@ -43,7 +43,9 @@ constexpr int AIRYP_W8 = AIRYP_N8 + 8;
 constexpr int AIRYP_N4 = AIRYP_W8 + 8;
 constexpr int AIRYP_W4 = AIRYP_N4 + 4;
 constexpr int PAIR_QN = AIRYP_W4 + 4;                // [32][t, w, cosh t, cosh 2t/3, cosh(t/3)/cosh t] derived from GL_NODES
-constexpr int SIZE = PAIR_QN + 160;                  // 5763 doubles = 46.1 KB
+constexpr int AIRY_CHEB = PAIR_QN + 160;             // [10][t_lo, c, 1/h, 16 Chebyshev coefficients] of the
+                                                     //   Gauss-Laguerre sums above (tables/gen_airy_cheb.py)
+constexpr int SIZE = AIRY_CHEB + 190;                // 5953 doubles = 47.6 KB
 }  // namespace lcfa_pack
 
 // Scalars used with the LCFA pack (lcfa/photon_emission/tables.rs:4,99-103;

@@ -258,7 +258,8 @@ bool build_lcfa_pack(ptg_ctx* c, std::vector<double>& pack) {
               put(AIRYP_N32, "special_functions/airy:GL_p1_6_NODES_32", 32) && put(AIRYP_W32, "special_functions/airy:GL_p1_6_WEIGHTS_32", 32) &&
               put(AIRYP_N16, "special_functions/airy:GL_p1_6_NODES_16", 16) && put(AIRYP_W16, "special_functions/airy:GL_p1_6_WEIGHTS_16", 16) &&
               put(AIRYP_N8, "special_functions/airy:GL_p1_6_NODES_8", 8) && put(AIRYP_W8, "special_functions/airy:GL_p1_6_WEIGHTS_8", 8) &&
-              put(AIRYP_N4, "special_functions/airy:GL_p1_6_NODES_4", 4) && put(AIRYP_W4, "special_functions/airy:GL_p1_6_WEIGHTS_4", 4);
+              put(AIRYP_N4, "special_functions/airy:GL_p1_6_NODES_4", 4) && put(AIRYP_W4, "special_functions/airy:GL_p1_6_WEIGHTS_4", 4) &&
+              put(AIRY_CHEB, "special_functions/airy:CHEB_GL_SUMS", 190);
     if (!ok) return false;
     // node-only factors of the pair spectrum quadrature (lcfa/pair_creation/mod.rs:80-89)
     for (int k = 0; k < 32; ++k) {

@@ -145,6 +145,11 @@ GENERATED = [
     ("nonlinear_compton/lp:CEILING_MAX", "nlc_lp_ceiling.f64", [33 * 44 * 110 + 32 * 43 * 109],
      "generated by tables/gen_lp_ceiling.py with the library's sampling code (ceiling samples of "
      "src/nonlinear_compton/lp/mod.rs:53-140)"),
+    # not a reference table: Chebyshev expansions of the reference's Gauss-Laguerre sums
+    # for Ai and Ai' at x >= 1 (src/special_functions/airy.rs:41-116)
+    ("special_functions/airy:CHEB_GL_SUMS", "airy_gl_cheb.f64", [10, 19],
+     "generated by tables/gen_airy_cheb.py from the GL_m1_6_*/GL_p1_6_* nodes and weights of "
+     "src/special_functions/airy.rs:285-421,451-593"),
 ]
 
 

@@ -110,7 +110,10 @@ def test_lcfa_rate_and_aux_t(engine, orc):
 
 def test_airy_and_bessel_k(engine, orc):
     from ptarmigan_amd import Fn
-    x = np.concatenate([np.linspace(0.0, 0.999, 200), np.exp(np.linspace(0, np.log(120.0), 400))])
+    # both sides of the series / quadrature switch at 1 and of the node-count
+    # switches at 2, 4, 10 and 11, where the device changes Chebyshev segment
+    edges = [v for e in (1.0, 2.0, 4.0, 10.0, 11.0) for v in (np.nextafter(e, 0.0), e, np.nextafter(e, 20.0))]
+    x = np.concatenate([np.linspace(0.0, 0.999, 200), np.exp(np.linspace(0, np.log(120.0), 4000)), edges])
     got = engine.eval(Fn.AIRY, x[:, None], 4)
     ref = np.array([orc.airy(v)[0] for v in x])
     for col in (0, 1):

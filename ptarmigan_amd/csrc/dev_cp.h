@@ -36,7 +36,12 @@ constexpr int G32N = G16N + 16;
 constexpr int G32W = G32N + 32;
 constexpr int BW_COLS = 74, BW_ROWS = 121;     // CP Breit-Wheeler total rate [121][74][2]
 constexpr int BW_TOTAL = G32W + 32;            //   ln (S_3 = +1, S_3 = -1) (pair_creation/cp/total.rs:2-8)
-constexpr int SIZE = BW_TOTAL + BW_COLS * BW_ROWS * 2;
+// the first 31 points of every CDF row above as (ln n, ln(-ln(1 - F))), the
+// coordinates its inversion interpolates in; filled on the device from the rows
+// (launch_cp_cdf_logs) with the same functions the inversion used to apply per call
+constexpr int CDF_LN = BW_TOTAL + BW_COLS * BW_ROWS * 2;   // [CDF_N][31][2]
+constexpr int CL_CDF_LN = CDF_LN + CDF_N * 62;             // [CL_N][31][2]
+constexpr int SIZE = CL_CDF_LN + CL_N * 62;
 }  // namespace cp_pack
 
 struct CpScalars {
@@ -70,19 +75,23 @@ PTG_HD inline void bessel_triple_j(int n, double x, double out[3]) {
     const double inv_x = 1.0 / x;
     double q0 = 1.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
     double total = 0.0;
-    for (int k = nmax; k >= k_stop; k -= 2) {
+    // k runs down by 2 from nmax; the factors 2 (k + 2) and 2 (k + 1) are carried
+    // as doubles, and the loops end on f2 = 2 (k + 2) (integers below 2^53:
+    // stepping by 4 is exact and gives the converted values)
+    double f2 = double(2 * (nmax + 2)), f1 = double(2 * (nmax + 1));
+    for (const double end = double(2 * (k_stop + 2)); f2 >= end; f2 -= 4.0, f1 -= 4.0) {
         q3 = q1;
         q2 = q0;
-        q1 = double(2 * (k + 2)) * q2 * inv_x - q3;
-        q0 = double(2 * (k + 1)) * q1 * inv_x - q2;
+        q1 = f2 * q2 * inv_x - q3;
+        q0 = f1 * q1 * inv_x - q2;
         total += q0;
     }
     double t0 = n_even ? q1 : q0, t1 = n_even ? q2 : q1, t2 = n_even ? q3 : q2;
-    for (int k = k_stop - 2; k >= 0; k -= 2) {
+    for (; f2 >= 4.0; f2 -= 4.0, f1 -= 4.0) {  // k >= 0
         q3 = q1;
         q2 = q0;
-        q1 = double(2 * (k + 2)) * q2 * inv_x - q3;
-        q0 = double(2 * (k + 1)) * q1 * inv_x - q2;
+        q1 = f2 * q2 * inv_x - q3;
+        q0 = f1 * q1 * inv_x - q2;
         total += q0;
     }
     double norm = 2.0 * total - q0;
@@ -286,18 +295,21 @@ __device__ inline bool cp_rate_f(const float* rt, const double* P, const CpScala
 }
 
 // ---------------------------------------------------------------- harmonic index
+// The k-th point of a CDF row [32][2] in the inversion's coordinates (ln n,
+// ln(-ln(1 - F))) (cp/tables/mod.rs:64-76).
+PTG_HD __forceinline__ void cp_cdf_log_point(const double* t32, int k, double* out2) {
+    out2[0] = log(t32[2 * k]);
+    out2[1] = log(-1.0 * log(1.0 - t32[2 * k + 1]));
+}
 // One tabulated CDF [32][2] inverted at frac: 0.9 below its first value, else the
 // first 31 points in (ln n, ln(-ln(1 - F))) with linear extrapolation
-// (cp/tables/mod.rs:36-44,64-76; cp/classical.rs:71-79,100-112)
-PTG_HD inline double cp_invert_one(const double* t32, double frac) {
+// (cp/tables/mod.rs:36-44,64-76; cp/classical.rs:71-79,100-112). `l62` is the row
+// in those coordinates (CDF_LN / CL_CDF_LN): 93 logarithms per row, four rows
+// per event, were a quarter of a CP emission's work when formed per call.
+PTG_HD inline double cp_invert_one(const double* t32, const double* l62, double frac) {
     if (frac <= t32[1]) return 0.9;
-    double rs[62];
-    for (int i = 0; i < 31; ++i) {
-        rs[2 * i] = log(t32[2 * i]);
-        rs[2 * i + 1] = log(-1.0 * log(1.0 - t32[2 * i + 1]));
-    }
     double x = 0.0;
-    pwmci_invert(rs, 31, log(-1.0 * log(1.0 - frac)), true, x);
+    pwmci_invert(l62, 31, log(-1.0 * log(1.0 - frac)), true, x);
     return exp(x);
 }
 // quantum harmonic (cp/tables/mod.rs:46-121); 0 if the lookup leaves the table
@@ -310,18 +322,18 @@ PTG_HD inline double cp_harmonic_cont(const double* P, const CpScalars& c, doubl
     int ix = int(fx);
     double dx = fx - double(ix);
     const double* T = P + CDF;
+    const double* L = P + CDF_LN;
+    auto inv = [&](int row) { return cp_invert_one(T + 64 * row, L + 62 * row, frac); };
     double n;
     if (le <= c.min[1]) {
-        n = cp_invert_one(T + 64 * ix, frac) * (1.0 - dx) + cp_invert_one(T + 64 * (ix + 1), frac) * dx;
+        n = inv(ix) * (1.0 - dx) + inv(ix + 1) * dx;
     } else {
         double fy = (le - c.min[1]) / c.step[1];
         if (!(fy < double(ROWS - 1))) return 0;
         int iy = int(fy);
         double dy = fy - double(iy);
-        n = cp_invert_one(T + 64 * (COLS * iy + ix), frac) * ((1.0 - dx) * (1.0 - dy)) +
-            cp_invert_one(T + 64 * (COLS * iy + ix + 1), frac) * (dx * (1.0 - dy)) +
-            cp_invert_one(T + 64 * (COLS * (iy + 1) + ix), frac) * ((1.0 - dx) * dy) +
-            cp_invert_one(T + 64 * (COLS * (iy + 1) + ix + 1), frac) * (dx * dy);
+        n = inv(COLS * iy + ix) * ((1.0 - dx) * (1.0 - dy)) + inv(COLS * iy + ix + 1) * (dx * (1.0 - dy)) +
+            inv(COLS * (iy + 1) + ix) * ((1.0 - dx) * dy) + inv(COLS * (iy + 1) + ix + 1) * (dx * dy);
     }
     return n;
 }
@@ -339,7 +351,9 @@ PTG_HD inline int cp_classical_harmonic(const double* P, const CpScalars& c, dou
     int ix = int(fx);
     double dx = fx - double(ix);
     const double* T = P + CL_CDF;
-    double n = cp_invert_one(T + 64 * ix, frac) * (1.0 - dx) + cp_invert_one(T + 64 * (ix + 1), frac) * dx;
+    const double* L = P + CL_CDF_LN;
+    double n = cp_invert_one(T + 64 * ix, L + 62 * ix, frac) * (1.0 - dx) +
+               cp_invert_one(T + 64 * (ix + 1), L + 62 * (ix + 1), frac) * dx;
     n = ceil(n);
     return (n >= 1.0 && n <= 1.0e7) ? int(n) : 0;
 }

@@ -512,6 +512,9 @@ int ptg_create(ptg_ctx** out, const char* table_path, int device) {
     if (build_cp_pack(c, cpack)) {
         HIPCHK(c, hipMalloc(&c->d_cp, cpack.size() * sizeof(double)));
         HIPCHK(c, scopy(c, c->d_cp, cpack.data(), cpack.size() * sizeof(double), hipMemcpyHostToDevice));
+        launch_cp_cdf_logs(c->d_cp, c->stream);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     HIPCHK(c, hipMalloc(&c->d_lma, lpack.size() * sizeof(double)));
     HIPCHK(c, scopy(c, c->d_lma, lpack.data(), lpack.size() * sizeof(double), hipMemcpyHostToDevice));

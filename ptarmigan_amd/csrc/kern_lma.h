@@ -134,6 +134,9 @@ struct EventRoom {
 // host, and the rest of the sequence does nothing.
 void launch_event_room(unsigned long long* ctr, const EventRoom& k, hipStream_t s);
 
+// Fills the CP pack's CDF_LN and CL_CDF_LN from its CDF rows (once, after upload).
+void launch_cp_cdf_logs(double* cp, hipStream_t s);
+
 // block: 64, 128 or 256 threads (the kernel's bound); grid: persistent blocks
 void launch_step_lma(const TransportArgs* a, const LmaArgs& m, int grid, hipStream_t s, int block = 256);
 void launch_event_lma(const TransportArgs* a, const LmaArgs& m, int lanes, hipStream_t s);

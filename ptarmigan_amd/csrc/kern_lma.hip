@@ -332,6 +332,20 @@ __device__ __noinline__ void cp_sample_wave(TP tb, const double* P, int n, doubl
     sv = {st[0], st[1], st[2], st[3]};
 }
 
+// (ln n, ln(-ln(1 - F))) of the first 31 points of every CP CDF row: one thread per point.
+__global__ void k_cp_cdf_logs(double* P) {
+    using namespace cp_pack;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = t / 31, k = t % 31;
+    if (row < CDF_N) cp_cdf_log_point(P + CDF + 64 * row, k, P + CDF_LN + 62 * row + 2 * k);
+    else if (row < CDF_N + CL_N)
+        cp_cdf_log_point(P + CL_CDF + 64 * (row - CDF_N), k, P + CL_CDF_LN + 62 * (row - CDF_N) + 2 * k);
+}
+void launch_cp_cdf_logs(double* cp, hipStream_t s) {
+    const int n = (cp_pack::CDF_N + cp_pack::CL_N) * 31;
+    k_cp_cdf_logs<<<(n + 255) / 256, 256, 0, s>>>(cp);
+}
+
 // One CP emission per wave (nonlinear_compton::generate, mod.rs:58-138, with
 // cp_sample_wave); lane 0 builds the photon.
 template <int MODE>

@@ -27,7 +27,11 @@ constexpr int CDF_COLS = 61, CDF_N = 4148;
 constexpr int CL_N = 61;                          // classical tables in ln a (lp/classical/rate_table.rs:1)
 constexpr int CL_RATE = CDF + CDF_N * 32;         // [61]
 constexpr int CL_CDF = CL_RATE + CL_N;            // [61][16][2] (lp/classical/cdf_table.rs:4)
-constexpr int SIZE = CL_CDF + CL_N * 32;
+// the first 15 points of every CDF row as (ln n, ln(-ln(1 - F))), the coordinates
+// the inversion interpolates in from a = 5 up; filled on the device from the rows
+// (launch_lp_cdf_logs) with the same functions the inversion used to apply per call
+constexpr int CDF_LN = CL_CDF + CL_N * 32;         // [CDF_N][15][2]
+constexpr int SIZE = CDF_LN + CDF_N * 30;
 }  // namespace lma_pack
 
 struct LmaScalars {
@@ -432,8 +436,16 @@ __device__ __forceinline__ bool lp_rate_f(const float* rt, const LmaScalars& s, 
 }
 
 // ---------------------------------------------------------------- harmonic index (lp/mod.rs:465-553)
+// The k-th point of a CDF row [16][2] in the coordinates of the inversion from
+// a = 5 up (ln n, ln(-ln(1 - F))).
+__device__ __forceinline__ void lp_cdf_log_point(const double* t16, int k, double* out2) {
+    out2[0] = log(t16[2 * k]);
+    out2[1] = log(-log(1.0 - t16[2 * k + 1]));
+}
+// `l30` is the row's first 15 points in those coordinates (lma_pack::CDF_LN):
+// formed per call they were 45 logarithms a row, four rows per emission.
 template <class P>
-__device__ inline double harmonic_from_table(P t16, double a, double frac) {
+__device__ inline double harmonic_from_table(P t16, P l30, double a, double frac) {
     if (frac <= t16[1]) return 0.9;
     double n = 0.0;
     if (a < 5.0) {
@@ -441,12 +453,7 @@ __device__ inline double harmonic_from_table(P t16, double a, double frac) {
         return n;
     }
     // rescaled table: ln n against ln(-ln(1 - F)), first 15 points
-    double rs[30];
-    for (int i = 0; i < 15; ++i) {
-        rs[2 * i] = log(t16[2 * i]);
-        rs[2 * i + 1] = log(-log(1.0 - t16[2 * i + 1]));
-    }
-    pwmci_invert(rs, 15, log(-log(1.0 - frac)), true, n);
+    pwmci_invert(l30, 15, log(-log(1.0 - frac)), true, n);
     return exp(n);
 }
 // Returns 0 when (a, eta) is outside the table, where the reference's lookup
@@ -463,18 +470,18 @@ __device__ inline double lp_harmonic_cont(P cdf, const LmaScalars& sc, double a,
         return 0;
     int ix = int(fx);
     double dx = fx - double(ix);
+    const P ln = cdf + (lma_pack::CDF_LN - lma_pack::CDF);  // cdf is the pack's CDF region
+    auto inv = [&](int row) { return harmonic_from_table(cdf + 32 * row, ln + 30 * row, a, frac); };
     double n;
     if (le <= sc.cdf_min[1]) {
-        n = harmonic_from_table(cdf + 32 * ix, a, frac) * (1.0 - dx) + harmonic_from_table(cdf + 32 * (ix + 1), a, frac) * dx;
+        n = inv(ix) * (1.0 - dx) + inv(ix + 1) * dx;
     } else {
         double fy = (le - sc.cdf_min[1]) / sc.cdf_step[1];
         int iy = int(fy);
         double dy = fy - double(iy);
         const int C = lma_pack::CDF_COLS;
-        n = harmonic_from_table(cdf + 32 * (C * iy + ix), a, frac) * ((1.0 - dx) * (1.0 - dy)) +
-            harmonic_from_table(cdf + 32 * (C * iy + ix + 1), a, frac) * (dx * (1.0 - dy)) +
-            harmonic_from_table(cdf + 32 * (C * (iy + 1) + ix), a, frac) * ((1.0 - dx) * dy) +
-            harmonic_from_table(cdf + 32 * (C * (iy + 1) + ix + 1), a, frac) * (dx * dy);
+        n = inv(C * iy + ix) * ((1.0 - dx) * (1.0 - dy)) + inv(C * iy + ix + 1) * (dx * (1.0 - dy)) +
+            inv(C * (iy + 1) + ix) * ((1.0 - dx) * dy) + inv(C * (iy + 1) + ix + 1) * (dx * dy);
     }
     return n;
 }

@@ -518,6 +518,9 @@ int ptg_create(ptg_ctx** out, const char* table_path, int device) {
     }
     HIPCHK(c, hipMalloc(&c->d_lma, lpack.size() * sizeof(double)));
     HIPCHK(c, scopy(c, c->d_lma, lpack.data(), lpack.size() * sizeof(double), hipMemcpyHostToDevice));
+    launch_lp_cdf_logs(c->d_lma, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     {
         std::vector<float> lf(lpack.begin(), lpack.end());
         HIPCHK(c, hipMalloc(&c->d_lma_f, lf.size() * sizeof(float)));

@@ -136,6 +136,8 @@ void launch_event_room(unsigned long long* ctr, const EventRoom& k, hipStream_t 
 
 // Fills the CP pack's CDF_LN and CL_CDF_LN from its CDF rows (once, after upload).
 void launch_cp_cdf_logs(double* cp, hipStream_t s);
+// Fills the LMA pack's CDF_LN from its CDF rows (once, after upload).
+void launch_lp_cdf_logs(double* lma, hipStream_t s);
 
 // block: 64, 128 or 256 threads (the kernel's bound); grid: persistent blocks
 void launch_step_lma(const TransportArgs* a, const LmaArgs& m, int grid, hipStream_t s, int block = 256);

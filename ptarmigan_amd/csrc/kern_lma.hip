@@ -346,6 +346,18 @@ void launch_cp_cdf_logs(double* cp, hipStream_t s) {
     k_cp_cdf_logs<<<(n + 255) / 256, 256, 0, s>>>(cp);
 }
 
+// (ln n, ln(-ln(1 - F))) of the first 15 points of every LP CDF row: one thread per point.
+__global__ void k_lp_cdf_logs(double* P) {
+    using namespace lma_pack;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = t / 15, k = t % 15;
+    if (row < CDF_N) lp_cdf_log_point(P + CDF + 32 * row, k, P + CDF_LN + 30 * row + 2 * k);
+}
+void launch_lp_cdf_logs(double* lma, hipStream_t s) {
+    const int n = lma_pack::CDF_N * 15;
+    k_lp_cdf_logs<<<(n + 255) / 256, 256, 0, s>>>(lma);
+}
+
 // One CP emission per wave (nonlinear_compton::generate, mod.rs:58-138, with
 // cp_sample_wave); lane 0 builds the photon.
 template <int MODE>

@@ -341,6 +341,40 @@ PTG_HD inline int cp_harmonic(const double* P, const CpScalars& c, double a, dou
     double n = ceil(cp_harmonic_cont(P, c, a, eta, frac));
     return (n >= 1.0 && n <= 1.0e7) ? int(n) : 0;
 }
+// cp_harmonic at the hand of a whole wave (all lanes call it with the same
+// arguments and return the same value): the two or four row inversions are
+// independent, so lane q of every quad inverts corner q (lanes 0-3 supply the
+// values) and the wave waits for one table search and root iteration instead
+// of four in a row. The corners are combined as cp_harmonic_cont combines them.
+__device__ inline int cp_harmonic_wave(const double* P, const CpScalars& c, double a, double eta, double frac) {
+    using namespace cp_pack;
+    const int q = int(__lane_id()) & 3;
+    double la = log(a), le = log(eta);
+    if (la <= c.min[0]) return 1;
+    double fx = (la - c.min[0]) / c.step[0];
+    if (!(fx < double(COLS - 1))) return 0;
+    int ix = int(fx);
+    double dx = fx - double(ix);
+    const double* T = P + CDF;
+    const double* L = P + CDF_LN;
+    auto inv = [&](int row) { return cp_invert_one(T + 64 * row, L + 62 * row, frac); };
+    double n;
+    if (le <= c.min[1]) {
+        const double v = inv(ix + (q & 1));
+        const double v0 = __shfl(v, 0, 64), v1 = __shfl(v, 1, 64);
+        n = v0 * (1.0 - dx) + v1 * dx;
+    } else {
+        double fy = (le - c.min[1]) / c.step[1];
+        if (!(fy < double(ROWS - 1))) return 0;
+        int iy = int(fy);
+        double dy = fy - double(iy);
+        const double v = inv(COLS * (iy + (q >> 1)) + ix + (q & 1));
+        const double v0 = __shfl(v, 0, 64), v1 = __shfl(v, 1, 64), v2 = __shfl(v, 2, 64), v3 = __shfl(v, 3, 64);
+        n = v0 * ((1.0 - dx) * (1.0 - dy)) + v1 * (dx * (1.0 - dy)) + v2 * ((1.0 - dx) * dy) + v3 * (dx * dy);
+    }
+    n = ceil(n);
+    return (n >= 1.0 && n <= 1.0e7) ? int(n) : 0;
+}
 // classical harmonic (cp/classical.rs:83-123)
 PTG_HD inline int cp_classical_harmonic(const double* P, const CpScalars& c, double a, double frac) {
     using namespace cp_pack;

@@ -490,6 +490,40 @@ __device__ inline int lp_harmonic(P cdf, const LmaScalars& sc, double a, double 
     double n = ceil(lp_harmonic_cont(cdf, sc, a, eta, frac));
     return (n >= 1.0 && n <= 1.0e7) ? int(n) : 0;
 }
+// lp_harmonic at the hand of a whole wave (all lanes call it with the same
+// arguments and return the same value): lane q of every quad inverts corner q
+// of the interpolation (lanes 0-3 supply the values), so the wave waits for one
+// table search and root iteration instead of four in a row; the corners are
+// combined as lp_harmonic_cont combines them.
+__device__ inline int lp_harmonic_wave(const double* cdf, const LmaScalars& sc, double a, double eta, double frac) {
+    const int q = int(__lane_id()) & 3;
+    double la = log(a), le = log(eta);
+    if (la <= sc.cdf_min[0]) return 1;
+    double fx = (la - sc.cdf_min[0]) / sc.cdf_step[0];
+    if (!(fx >= 0.0 && fx < double(lma_pack::CDF_COLS - 1))) return 0;
+    if (!(le <= sc.cdf_min[1]) && !((le - sc.cdf_min[1]) / sc.cdf_step[1] < double(lma_pack::CDF_N / lma_pack::CDF_COLS - 1)))
+        return 0;
+    int ix = int(fx);
+    double dx = fx - double(ix);
+    const double* ln = cdf + (lma_pack::CDF_LN - lma_pack::CDF);
+    auto inv = [&](int row) { return harmonic_from_table(cdf + 32 * row, ln + 30 * row, a, frac); };
+    double n;
+    if (le <= sc.cdf_min[1]) {
+        const double v = inv(ix + (q & 1));
+        const double v0 = __shfl(v, 0, 64), v1 = __shfl(v, 1, 64);
+        n = v0 * (1.0 - dx) + v1 * dx;
+    } else {
+        double fy = (le - sc.cdf_min[1]) / sc.cdf_step[1];
+        int iy = int(fy);
+        double dy = fy - double(iy);
+        const int C = lma_pack::CDF_COLS;
+        const double v = inv(C * (iy + (q >> 1)) + ix + (q & 1));
+        const double v0 = __shfl(v, 0, 64), v1 = __shfl(v, 1, 64), v2 = __shfl(v, 2, 64), v3 = __shfl(v, 3, 64);
+        n = v0 * ((1.0 - dx) * (1.0 - dy)) + v1 * (dx * (1.0 - dy)) + v2 * ((1.0 - dx) * dy) + v3 * (dx * dy);
+    }
+    n = ceil(n);
+    return (n >= 1.0 && n <= 1.0e7) ? int(n) : 0;
+}
 
 // ---------------------------------------------------------------- generalised Bessel function J(n, x, y)
 // J(n, x, y) = sum_k J_{n+2k}(x) J_k(y) for the five orders n-2..n+2.

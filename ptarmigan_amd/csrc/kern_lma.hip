@@ -379,7 +379,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
         const double eta = k0 * (u.t - u.z);
         Rng g = base;  // counter 0: the harmonic
         const double fr = rng_uniform(g);
-        const int nh = classical ? cp_classical_harmonic(M.cp, M.csc, a, fr) : cp_harmonic(M.cp, M.csc, a, eta, fr);
+        const int nh = classical ? cp_classical_harmonic(M.cp, M.csc, a, fr) : cp_harmonic_wave(M.cp, M.csc, a, eta, fr);
         const bool ok = nh >= 1;
         if (!ok) flags |= PTG_FLAG_TABLE_RANGE;
         double s = 0.0, cphi = 0.0;
@@ -598,7 +598,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PTG_LMA_WAVE
         // the harmonic is inverted from the FP64 CDF in mixed precision too: near F = 1
         // neighbouring FP32 entries round to the same value (and 1 - F to 0), where the
         // monotone inversion divides by zero; it is one lookup per event, not per step
-        const int nh = classical ? lp_cl_harmonic(M.tables, M.sc, alp, fr) : lp_harmonic(cdf, M.sc, alp, eta, fr);
+        const int nh = classical ? lp_cl_harmonic(M.tables, M.sc, alp, fr) : lp_harmonic_wave(cdf, M.sc, alp, eta, fr);
         const bool ok = nh >= 1;
         if (!ok) flags |= PTG_FLAG_TABLE_RANGE;
         double s = 0.0, theta = 0.0;
@@ -1696,7 +1696,7 @@ __global__ __launch_bounds__(64) void k_eval_lp_sample(const double* in, double*
         int nh = int(x[3]);
         if (!(x[3] > 0.0)) {
             const double fr = rng_uniform(rng);
-            nh = classical ? lp_cl_harmonic(m.tables, m.sc, x[0], fr) : lp_harmonic(m.tables + lma_pack::CDF, m.sc, x[0], x[1], fr);
+            nh = classical ? lp_cl_harmonic(m.tables, m.sc, x[0], fr) : lp_harmonic_wave(m.tables + lma_pack::CDF, m.sc, x[0], x[1], fr);
         }
         if (nh < 1) {
             if (threadIdx.x == 0) { y[0] = 0.0; y[5] = -2.0; }
@@ -1753,7 +1753,7 @@ __global__ __launch_bounds__(64) void k_eval_cp_sample(const double* in, double*
         int nh = int(x[3]);
         if (!(x[3] > 0.0)) {
             const double fr = rng_uniform(rng);
-            nh = cl ? cp_classical_harmonic(m.cp, m.csc, x[0], fr) : cp_harmonic(m.cp, m.csc, x[0], x[1], fr);
+            nh = cl ? cp_classical_harmonic(m.cp, m.csc, x[0], fr) : cp_harmonic_wave(m.cp, m.csc, x[0], x[1], fr);
         }
         if (nh < 1) {
             if (threadIdx.x == 0) { y[0] = 0.0; y[6] = -2.0; }
